@@ -1,0 +1,140 @@
+// gs_kernels.h -- internal launchers of the gfx950 kernels (C++ linkage; the
+// exported C ABI lives in gsparc_abi.cpp and include/gsparc.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gs {
+
+struct Dims {
+    int P, R, W, H;
+    int gx, gy, T;   // tile grid
+    int NBW;         // wave-blocks of 64 Gaussians
+};
+
+inline Dims make_dims(int P, int R, int W, int H)
+{
+    Dims d;
+    d.P = P; d.R = R; d.W = W; d.H = H;
+    d.gx = (W + 15) / 16; d.gy = (H + 15) / 16; d.T = d.gx * d.gy;
+    d.NBW = (P + 63) / 64;
+    return d;
+}
+
+struct PreFwdArgs {
+    const float* means;      // [P,3]
+    const float* scales;     // [P,3]
+    const float* rots;       // [P,4]
+    const float* opac;       // [P]
+    const float* mlp;        // [P*194] or nullptr (emission supplied elsewhere)
+    const float* views;      // [R,16] column-major
+    const float* tx;         // [3]
+    const float* rx;         // [R,3]
+    float mod;
+    float4* splat;           // [R*P*2]
+    float2* sig;             // [R*P] (written only when mlp != nullptr)
+    float* depth;            // [R*P]
+    int* radii;              // [R*P]
+    float* cov3;             // [P*6]
+    uint32_t* wcnt;          // [R*NBW*T]
+};
+hipError_t launch_preprocess_fwd(const Dims& d, const PreFwdArgs& a, hipStream_t s);
+
+struct BinArgs {
+    const float4* splat;     // for px, py
+    const int* radii;
+    const float* depth;
+    uint32_t* wcnt;          // [R*NBW*T] counts in, exclusive prefix out
+    uint32_t* segcnt;        // [R*T]
+    uint32_t* segbase;       // [R*T]
+    uint2* ranges;           // [R*T]
+    uint32_t* total;         // [2]: K, max segment length
+    uint32_t* total_host;    // optional pinned mirror of total
+    uint32_t* keys;          // [Kcap] depth bits, tile-major
+    uint32_t* vals;          // [Kcap]
+    uint32_t* keys_tmp;      // [Kcap] scratch for long segments
+    uint32_t* vals_tmp;
+    uint32_t* plist;         // [Kcap] sorted Gaussian indices
+    uint32_t* keys_sorted;   // optional [Kcap] sorted depth bits
+    uint64_t Kcap;
+};
+hipError_t launch_bin_scan(const Dims& d, const BinArgs& a, hipStream_t s);
+hipError_t launch_bin_scatter_sort(const Dims& d, const BinArgs& a, hipStream_t s);
+
+struct RenderFwdArgs {
+    const uint2* ranges;     // [R*T]
+    const uint32_t* plist;
+    const float4* splat;     // [R*P*2]
+    const float2* sig;       // [R*P]
+    float* out_re;           // [R*N]
+    float* out_im;           // [R*N]
+    float* finalT;           // [R*N]
+    uint32_t* ncontrib;      // [R*N]
+};
+hipError_t launch_render_fwd(const Dims& d, const RenderFwdArgs& a, hipStream_t s);
+
+struct RenderBwdArgs {
+    const uint2* ranges;
+    const uint32_t* plist;
+    const float4* splat;
+    const float2* sig;
+    const float* finalT;
+    const uint32_t* ncontrib;
+    const float* dout_re;    // [R*N]
+    const float* dout_im;
+    float* g8;               // [R*P*8] accumulated: dpx dpy dqa dqb dqc dop dre dim
+    int mode;
+};
+hipError_t launch_render_bwd(const Dims& d, const RenderBwdArgs& a, hipStream_t s);
+
+struct PreBwdArgs {
+    const float* means;
+    const float* scales;
+    const float* rots;
+    const float* mlp;        // or nullptr
+    const float* views;
+    const float* tx;
+    const float* rx;
+    float mod;
+    const int* radii;        // [R*P]
+    const float* g8;         // [R*P*8]
+    float* dmeans;           // [P,3]
+    float* dscales;          // [P,3]
+    float* drots;            // [P,4]
+    float* dopac;            // [P]
+    float* dmlp;             // [P*194] or nullptr
+    float* dcov3;            // [P,6] or nullptr
+    float* dmeans2d;         // [R*P*3] or nullptr (reference screen-space layout)
+    float* dsig;             // [R*P*2] or nullptr (emission gradient)
+    int mode;
+};
+hipError_t launch_preprocess_bwd(const Dims& d, const PreBwdArgs& a, hipStream_t s);
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+
+struct SsimArgs {
+    int planes, H, W;
+    float C1, C2;
+    const float* img1;
+    const float* img2;
+    float* map;
+    float* dmu1;             // nullptr when not training
+    float* ds1;
+    float* ds12;
+};
+hipError_t launch_ssim_fwd(const SsimArgs& a, hipStream_t s);
+
+struct SsimBwdArgs {
+    int planes, H, W;
+    const float* img1;
+    const float* img2;
+    const float* dmap;
+    const float* dmu1;
+    const float* ds1;
+    const float* ds12;
+    float* dimg1;
+};
+hipError_t launch_ssim_bwd(const SsimBwdArgs& a, hipStream_t s);
+
+}  // namespace gs

@@ -1,0 +1,178 @@
+// gs_math.h -- float32 transcendental functions shared by the gfx950 kernels
+// and the host code that checks them.
+//
+// Every function here is built only from IEEE-754 operations that are
+// correctly rounded on both gfx950 and x86-64 (+, -, *, /, sqrt, fma,
+// round-to-nearest-even) plus integer bit manipulation.  Compiled with
+// -ffp-contract=off on both sides, the same input therefore gives the same
+// output bits on the GPU and on the CPU.  That is what lets the per-pixel
+// blend decisions (alpha < 1/255, T < 1e-4) of the GPU path agree exactly with
+// a CPU restatement of the rasterizer.
+//
+// Methods (textbook; J.-M. Muller, "Elementary Functions", chapters 3 and 11):
+//   gs_expf   Cody-Waite reduction x = n*ln2 + r, e^r = 1 + r + r^2 q(r)
+//   gs_logf   x = 2^e * m, m in [sqrt(1/2), sqrt(2)), log1p(f) = f + f^2 Q(f)
+//   gs_atan2f octant reduction to a in [0,1], atan(a) = a + a^3 P(a^2)
+//   gs_asinf  asin(x) = x + x^3 S(x^2) on [0, 1/2];
+//             asin(x) = pi/2 - 2 asin(sqrt((1 - x) / 2)) above
+// The polynomial coefficients come from gen_gs_math_coeffs.py (near-minimax
+// relative-error fits); accuracy is measured against double precision by the
+// test suite (all within 2 ulp over the tested ranges).
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define GS_HD __host__ __device__ inline
+#else
+#define GS_HD static inline
+#endif
+
+// ---- polynomial coefficients (generated by gen_gs_math_coeffs.py) ----
+#define GS_EXP_Q_0 0x1.0000000000000p-1f  /* 0.499999994195 */
+#define GS_EXP_Q_1 0x1.5554de0000000p-3f  /* 0.166665774514 */
+#define GS_EXP_Q_2 0x1.5555ea0000000p-5f  /* 0.0416669426924 */
+#define GS_EXP_Q_3 0x1.120afc0000000p-7f  /* 0.00836312576496 */
+#define GS_EXP_Q_4 0x1.6c37ec0000000p-10f  /* 0.00138938315321 */
+#define GS_LOG_Q_0 -0x1.fffffe0000000p-2f  /* -0.499999976593 */
+#define GS_LOG_Q_1 0x1.5555560000000p-2f  /* 0.333333339083 */
+#define GS_LOG_Q_2 -0x1.00020e0000000p-2f  /* -0.250007823186 */
+#define GS_LOG_Q_3 0x1.999e520000000p-3f  /* 0.20000900083 */
+#define GS_LOG_Q_4 -0x1.5481b80000000p-3f  /* -0.166263049023 */
+#define GS_LOG_Q_5 0x1.2310760000000p-3f  /* 0.142121237295 */
+#define GS_LOG_Q_6 -0x1.0d13ba0000000p-3f  /* -0.131385281011 */
+#define GS_LOG_Q_7 0x1.039fb60000000p-3f  /* 0.126769462151 */
+#define GS_LOG_Q_8 -0x1.36c5780000000p-4f  /* -0.075871916613 */
+#define GS_ATAN_P_0 -0x1.5555540000000p-2f  /* -0.333333316425 */
+#define GS_ATAN_P_1 0x1.99982e0000000p-3f  /* 0.199997284768 */
+#define GS_ATAN_P_2 -0x1.246bb00000000p-3f  /* -0.142783516781 */
+#define GS_ATAN_P_3 0x1.c3dfde0000000p-4f  /* 0.110320917165 */
+#define GS_ATAN_P_4 -0x1.62531e0000000p-4f  /* -0.0865050477669 */
+#define GS_ATAN_P_5 0x1.feed3c0000000p-5f  /* 0.062368983369 */
+#define GS_ATAN_P_6 -0x1.2499d40000000p-5f  /* -0.0357178821944 */
+#define GS_ATAN_P_7 0x1.b793dc0000000p-7f  /* 0.0134148426016 */
+#define GS_ATAN_P_8 -0x1.35de580000000p-9f  /* -0.00236410926598 */
+#define GS_ASIN_S_0 0x1.5555540000000p-3f  /* 0.166666663386 */
+#define GS_ASIN_S_1 0x1.3334300000000p-4f  /* 0.0750009431412 */
+#define GS_ASIN_S_2 0x1.6d5be00000000p-5f  /* 0.0445994749825 */
+#define GS_ASIN_S_3 0x1.fd8a2e0000000p-6f  /* 0.0310998398876 */
+#define GS_ASIN_S_4 0x1.1908e60000000p-6f  /* 0.0171529998826 */
+#define GS_ASIN_S_5 0x1.13f2360000000p-5f  /* 0.0336848294275 */
+
+// ln2 = GS_LN2_HI + GS_LN2_LO (hi is float(ln2))
+#define GS_LN2_HI 0x1.62e430p-1f
+#define GS_LN2_LO -0x1.05c610p-29f
+#define GS_LOG2E 0x1.715476p+0f
+// pi/2 and pi split the same way
+#define GS_PIO2_HI 0x1.921fb6p+0f
+#define GS_PIO2_LO -0x1.777a5cp-25f
+#define GS_PI_HI 0x1.921fb6p+1f
+#define GS_PI_LO -0x1.777a5cp-24f
+
+typedef union { float f; uint32_t u; } gs_f32_bits;
+GS_HD float gs_as_float(uint32_t u) { gs_f32_bits b; b.u = u; return b.f; }
+GS_HD uint32_t gs_as_uint(float f) { gs_f32_bits b; b.f = f; return b.u; }
+
+// 2^k for k in [-126, 127], built from the exponent field.
+GS_HD float gs_pow2i(int k) { return gs_as_float((uint32_t)(k + 127) << 23); }
+
+GS_HD float gs_expf(float x)
+{
+    if (x != x) return x;
+    if (x >= 88.72283935546875f) return __builtin_huge_valf();
+    if (x < -104.0f) return 0.0f;
+    const float n = __builtin_rintf(x * GS_LOG2E);
+    float r = __builtin_fmaf(-n, GS_LN2_HI, x);
+    r = __builtin_fmaf(-n, GS_LN2_LO, r);
+    float q = GS_EXP_Q_4;
+    q = __builtin_fmaf(q, r, GS_EXP_Q_3);
+    q = __builtin_fmaf(q, r, GS_EXP_Q_2);
+    q = __builtin_fmaf(q, r, GS_EXP_Q_1);
+    q = __builtin_fmaf(q, r, GS_EXP_Q_0);
+    const float p = __builtin_fmaf(r * r, q, r) + 1.0f;
+    const int ni = (int)n;          // in [-150, 128]
+    const int n1 = ni >> 1;         // both halves in [-75, 64]
+    const int n2 = ni - n1;
+    return (p * gs_pow2i(n1)) * gs_pow2i(n2);
+}
+
+GS_HD float gs_logf(float x)
+{
+    if (x != x) return x;
+    if (x <= 0.0f) return (x == 0.0f) ? -__builtin_huge_valf() : __builtin_nanf("");
+    if (x == __builtin_huge_valf()) return x;
+    int eadj = 0;
+    if (x < 0x1p-126f) { x *= 0x1p23f; eadj = -23; }
+    // Split x = 2^e * m with m in [sqrt(1/2), sqrt(2)).
+    const int32_t ix = (int32_t)gs_as_uint(x) - 0x3f3504f3;
+    const int e = (ix >> 23) + eadj;
+    const float m = gs_as_float((uint32_t)((ix & 0x007fffff) + 0x3f3504f3));
+    const float f = m - 1.0f;       // exact (Sterbenz)
+    float q = GS_LOG_Q_8;
+    q = __builtin_fmaf(q, f, GS_LOG_Q_7);
+    q = __builtin_fmaf(q, f, GS_LOG_Q_6);
+    q = __builtin_fmaf(q, f, GS_LOG_Q_5);
+    q = __builtin_fmaf(q, f, GS_LOG_Q_4);
+    q = __builtin_fmaf(q, f, GS_LOG_Q_3);
+    q = __builtin_fmaf(q, f, GS_LOG_Q_2);
+    q = __builtin_fmaf(q, f, GS_LOG_Q_1);
+    q = __builtin_fmaf(q, f, GS_LOG_Q_0);
+    const float lp = __builtin_fmaf(f * f, q, f);
+    const float ef = (float)e;
+    return __builtin_fmaf(ef, GS_LN2_HI, __builtin_fmaf(ef, GS_LN2_LO, lp));
+}
+
+GS_HD float gs_atanf_unit(float a)   // a in [0, 1]
+{
+    const float u = a * a;
+    float p = GS_ATAN_P_8;
+    p = __builtin_fmaf(p, u, GS_ATAN_P_7);
+    p = __builtin_fmaf(p, u, GS_ATAN_P_6);
+    p = __builtin_fmaf(p, u, GS_ATAN_P_5);
+    p = __builtin_fmaf(p, u, GS_ATAN_P_4);
+    p = __builtin_fmaf(p, u, GS_ATAN_P_3);
+    p = __builtin_fmaf(p, u, GS_ATAN_P_2);
+    p = __builtin_fmaf(p, u, GS_ATAN_P_1);
+    p = __builtin_fmaf(p, u, GS_ATAN_P_0);
+    return __builtin_fmaf(a * u, p, a);
+}
+
+GS_HD float gs_atan2f(float y, float x)
+{
+    if (x != x || y != y) return x + y;
+    const float ax = __builtin_fabsf(x), ay = __builtin_fabsf(y);
+    const float mx = ax > ay ? ax : ay;
+    const float mn = ax > ay ? ay : ax;
+    const float a = (mx == 0.0f) ? 0.0f : mn / mx;
+    float t = gs_atanf_unit(a);
+    if (ay > ax) t = (GS_PIO2_HI - t) + GS_PIO2_LO;
+    if (__builtin_signbit(x)) t = (GS_PI_HI - t) + GS_PI_LO;
+    return __builtin_copysignf(t, y);
+}
+
+GS_HD float gs_asin_poly(float s, float u)   // asin(s), u = s*s, s in [0, 1/2]
+{
+    float p = GS_ASIN_S_5;
+    p = __builtin_fmaf(p, u, GS_ASIN_S_4);
+    p = __builtin_fmaf(p, u, GS_ASIN_S_3);
+    p = __builtin_fmaf(p, u, GS_ASIN_S_2);
+    p = __builtin_fmaf(p, u, GS_ASIN_S_1);
+    p = __builtin_fmaf(p, u, GS_ASIN_S_0);
+    return __builtin_fmaf(s * u, p, s);
+}
+
+GS_HD float gs_asinf(float x)
+{
+    const float ax = __builtin_fabsf(x);
+    if (!(ax <= 1.0f)) return __builtin_nanf("");
+    float r;
+    if (ax <= 0.5f) {
+        r = gs_asin_poly(ax, ax * ax);
+    } else {
+        const float z = (1.0f - ax) * 0.5f;   // exact
+        const float s = __builtin_sqrtf(z);
+        const float as = gs_asin_poly(s, z);
+        r = (GS_PIO2_HI - 2.0f * as) + GS_PIO2_LO;
+    }
+    return __builtin_copysignf(r, x);
+}

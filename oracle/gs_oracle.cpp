@@ -1,0 +1,819 @@
+// gs_oracle.cpp -- CPU restatement of GSpaRC's equirectangular RF rasterizer
+// and fused-SSIM loss.  TEST INFRASTRUCTURE ONLY: it is loaded by tests/,
+// __graft_entry__.smoke() and the cpu_baseline leg of bench.py as the checker.
+// The product path never links, imports or calls it.
+//
+// Two instantiations of every stage:
+//   R = float   "mirror": FP32 with the operation order the gfx950 kernels use
+//               (built with -ffp-contract=off; every fused multiply-add is an
+//               explicit fmaf), transcendental functions from gs_math.h.  The
+//               GPU path must agree with it bit for bit on discrete outputs and
+//               to rounding on the rest.
+//   R = double  "twin": the same algorithm in FP64 with libm functions, used
+//               for central finite differences of the exact-mode gradients.
+//
+// Each function cites the reference file:line it restates (paths relative to
+// the GSpaRC repository root).
+//
+// Gradient modes of the backward:
+//   GRAD_EXACT (0)      the true adjoint of the forward below.
+//   GRAD_REFERENCE (1)  reproduces the reference backward literally, including
+//                       its deviations from the adjoint: the halved x-row of
+//                       the projection Jacobian (cuda_wireless_rasterizer/
+//                       backward.cu:172-173), the imaginary channel reading the
+//                       real signal (backward.cu:577,587), 1/(det^2+1e-7)
+//                       (backward.cu:214), 1/(d+eps)^2 in the MLP mean
+//                       gradient (backward.cu:42,91), the opacity clamp being
+//                       ignored (backward.cu:561) and the scale modifier being
+//                       dropped from dL/dscale (backward.cu:353-355).
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "../gsparc-wirelessgaussiansplatting_amd/csrc/gs_math.h"
+
+namespace {
+
+enum { GRAD_EXACT = 0, GRAD_REFERENCE = 1 };
+constexpr int BLOCK = 16;   // binning tile, config.h:16-17
+
+template <typename R> struct Fn;
+template <> struct Fn<float> {
+    static float exp(float x) { return gs_expf(x); }
+    static float log(float x) { return gs_logf(x); }
+    static float atan2(float y, float x) { return gs_atan2f(y, x); }
+    static float asin(float x) { return gs_asinf(x); }
+    static float sqrt(float x) { return sqrtf(x); }
+    static float fma(float a, float b, float c) { return fmaf(a, b, c); }
+    static float ceil(float x) { return ceilf(x); }
+};
+template <> struct Fn<double> {
+    static double exp(double x) { return std::exp(x); }
+    static double log(double x) { return std::log(x); }
+    static double atan2(double y, double x) { return std::atan2(y, x); }
+    static double asin(double x) { return std::asin(x); }
+    static double sqrt(double x) { return std::sqrt(x); }
+    static double fma(double a, double b, double c) { return std::fma(a, b, c); }
+    static double ceil(double x) { return std::ceil(x); }
+};
+
+// float constants as written in the reference, promoted to R
+template <typename R> constexpr R K(float v) { return R(v); }
+constexpr float ONE_OVER_PI_F = 0.318309886183790671538f;   // M_1_PIf32
+constexpr float TWO_OVER_PI_F = 0.636619772367581343076f;   // M_2_PIf32
+
+// ---------------------------------------------------------------------------
+// Forward preprocess: forward.cu:252-327, auxiliary.h:10-24,38-46,101-113,
+// 133-155, computeCov3D forward.cu:124-149, computeCov2D forward.cu:78-122,
+// computeSignalFromMLP forward.cu:17-75.
+// ---------------------------------------------------------------------------
+template <typename R>
+struct Geo {
+    R t[3];          // view-space position
+    R r;             // distance to receiver (the depth, auxiliary.h:148)
+    R J[2][3];       // d(px,py)/dt as used for the covariance
+    R M2[2][3];      // J * Rv
+    R cov3[6];       // xx xy xz yy yz zz
+    R A, B, C;       // 2D covariance incl. the 0.3 low-pass
+    R det;
+    R px, py;
+};
+
+template <typename R>
+static void view_xform(const R* v, const R* m, R* t)   // auxiliary.h:38-46
+{
+    t[0] = v[0] * m[0] + v[1] * m[4] + v[2] * m[8] + m[12];
+    t[1] = v[0] * m[1] + v[1] * m[5] + v[2] * m[9] + m[13];
+    t[2] = v[0] * m[2] + v[1] * m[6] + v[2] * m[10] + m[14];
+}
+
+template <typename R>
+static void cov3d(const R* s, R mod, const R* q, R* c)   // forward.cu:124-149
+{
+    const R sx = mod * s[0], sy = mod * s[1], sz = mod * s[2];
+    const R r = q[0], x = q[1], y = q[2], z = q[3];
+    // standard rotation matrix of q = (r, x, y, z), row-major
+    R Rq[3][3] = {
+        {R(1) - R(2) * (y * y + z * z), R(2) * (x * y - r * z), R(2) * (x * z + r * y)},
+        {R(2) * (x * y + r * z), R(1) - R(2) * (x * x + z * z), R(2) * (y * z - r * x)},
+        {R(2) * (x * z - r * y), R(2) * (y * z + r * x), R(1) - R(2) * (x * x + y * y)}};
+    // Sigma = Rq diag(s^2) Rq^T; entry (i,j) = sum_k Rq[i][k] s_k^2 Rq[j][k]
+    const R s2[3] = {sx * sx, sy * sy, sz * sz};
+    const int ij[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+    for (int e = 0; e < 6; ++e) {
+        const int i = ij[e][0], j = ij[e][1];
+        c[e] = Rq[i][0] * s2[0] * Rq[j][0] + Rq[i][1] * s2[1] * Rq[j][1] + Rq[i][2] * s2[2] * Rq[j][2];
+    }
+}
+
+// Projection of t onto the equirect map and its Jacobian (forward.cu:82-97,
+// auxiliary.h:10-13, 101-113).  px is ((lon/pi + 1) W - 1)/2 evaluated in
+// double as the reference's ndc2Pix does.
+template <typename R>
+static void project(const R* t, R r, int W, int H, Geo<R>& g)
+{
+    const R inv_r = R(1) / (r + K<R>(0.0000001f));
+    const R lon = Fn<R>::atan2(t[0], t[2]);
+    const R lat = Fn<R>::asin(t[1] * inv_r);
+    const R sx = lon * K<R>(ONE_OVER_PI_F);
+    const R sy = lat * K<R>(TWO_OVER_PI_F);
+    g.px = (R)((((double)sx + 1.0) * W - 1.0) * 0.5);
+    g.py = (R)((((double)sy + 1.0) * H - 1.0) * 0.5);
+
+    const R rxz2 = t[0] * t[0] + t[2] * t[2];
+    const R rxz2_inv = R(1) / (rxz2 + K<R>(1e-7f));
+    const R rxz = Fn<R>::sqrt(rxz2);
+    const R rxz_inv = R(1) / (rxz + K<R>(1e-7f));
+    const R r2 = rxz2 + t[1] * t[1];
+    const R r2_inv = R(1) / (r2 + K<R>(1e-7f));
+    const R Wd = (R)W * R(0.5f) * K<R>(ONE_OVER_PI_F);
+    const R Hd = (R)H * K<R>(ONE_OVER_PI_F);
+    g.J[0][0] = Wd * t[2] * rxz2_inv;
+    g.J[0][1] = R(0);
+    g.J[0][2] = -Wd * t[0] * rxz2_inv;
+    g.J[1][0] = -Hd * t[0] * t[1] * rxz_inv * r2_inv;
+    g.J[1][1] = Hd * rxz * r2_inv;
+    g.J[1][2] = -Hd * t[2] * t[1] * rxz_inv * r2_inv;
+}
+
+// 2D covariance (J Rv) Sigma (J Rv)^T + 0.3 I (forward.cu:99-121).
+template <typename R>
+static void cov2d(const R J[2][3], const R* view, const R* c, Geo<R>& g)
+{
+    // Rv[i][k] = view[k*4 + i] (column-major 4x4, auxiliary.h:38-46)
+    for (int a = 0; a < 2; ++a)
+        for (int k = 0; k < 3; ++k)
+            g.M2[a][k] = J[a][0] * view[k * 4 + 0] + J[a][1] * view[k * 4 + 1] + J[a][2] * view[k * 4 + 2];
+    const R S[3][3] = {{c[0], c[1], c[2]}, {c[1], c[3], c[4]}, {c[2], c[4], c[5]}};
+    R v0[3], v1[3];
+    for (int i = 0; i < 3; ++i) {
+        v0[i] = S[i][0] * g.M2[0][0] + S[i][1] * g.M2[0][1] + S[i][2] * g.M2[0][2];
+        v1[i] = S[i][0] * g.M2[1][0] + S[i][1] * g.M2[1][1] + S[i][2] * g.M2[1][2];
+    }
+    g.A = g.M2[0][0] * v0[0] + g.M2[0][1] * v0[1] + g.M2[0][2] * v0[2] + K<R>(0.3f);
+    g.B = g.M2[1][0] * v0[0] + g.M2[1][1] * v0[1] + g.M2[1][2] * v0[2];
+    g.C = g.M2[1][0] * v1[0] + g.M2[1][1] * v1[1] + g.M2[1][2] * v1[2] + K<R>(0.3f);
+}
+
+// getRect, auxiliary.h:14-24 (the radius is an int there).
+template <typename R>
+static void get_rect(R px, R py, int radius, int gx, int gy, int* rect)
+{
+    const R rr = (R)radius;
+    auto clampi = [](int v, int hi) { return std::min(hi, std::max(0, v)); };
+    rect[0] = clampi((int)((px - rr) / R(BLOCK)), gx);
+    rect[1] = clampi((int)((py - rr) / R(BLOCK)), gy);
+    rect[2] = clampi((int)(((px + rr) + R(BLOCK - 1)) / R(BLOCK)), gx);
+    rect[3] = clampi((int)(((py + rr) + R(BLOCK - 1)) / R(BLOCK)), gy);
+}
+
+// Per-Gaussian emission MLP 3->32->2 (forward.cu:17-75, packing
+// gaussian.py:554-572): params = [W1 P*32*3 | b1 P*32 | W2 P*2*32 | b2 P*2].
+template <typename R>
+static void mlp_forward(int idx, int P, const R* mlp, const R* mean, const R* rx, const R* tx,
+                        R* hidden_pre, R* out_pre, R* sig, R* logd, R* dist)
+{
+    const R* W1 = mlp + (size_t)idx * 96;
+    const R* b1 = mlp + (size_t)P * 96 + (size_t)idx * 32;
+    const R* W2 = mlp + (size_t)P * 128 + (size_t)idx * 64;
+    const R* b2 = mlp + (size_t)P * 192 + (size_t)idx * 2;
+    const R diff[3] = {tx[0] - mean[0], tx[1] - mean[1], tx[2] - mean[2]};
+    const R d = Fn<R>::sqrt(diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2]);
+    const R ld = Fn<R>::log(d + K<R>(1e-6f));
+    R h[32];
+    for (int i = 0; i < 32; ++i) {
+        R s = b1[i];
+        for (int j = 0; j < 3; ++j) s += W1[i * 3 + j] * rx[j];
+        hidden_pre[i] = s;
+        h[i] = (s > R(0)) ? s : K<R>(0.01f) * s;
+    }
+    for (int k = 0; k < 2; ++k) {
+        R s = b2[k];
+        for (int j = 0; j < 32; ++j) s += W2[k * 32 + j] * h[j];
+        out_pre[k] = s;
+        sig[k] = R(1) / (R(1) + Fn<R>::exp(-(s - ld)));
+    }
+    *logd = ld;
+    *dist = d;
+}
+
+template <typename R>
+struct PreOut {
+    int* radii; R* xy; R* depth; R* cov3; R* conic_o; R* sig_re; R* sig_im; int* tiles; int* rect;
+};
+
+// Returns 1 if the Gaussian is visible (radius > 0).
+template <typename R>
+static int preprocess_one(int idx, int P, int W, int H, const R* means, const R* scales, const R* rots,
+                          const R* opac, const R* mlp, const R* view, const R* tx, const R* rx, R mod,
+                          const PreOut<R>& o, Geo<R>* gout)
+{
+    const int gx = (W + BLOCK - 1) / BLOCK, gy = (H + BLOCK - 1) / BLOCK;
+    o.radii[idx] = 0;
+    o.tiles[idx] = 0;
+    for (int k = 0; k < 4; ++k) o.rect[idx * 4 + k] = 0;
+    Geo<R> g;
+    view_xform(means + 3 * idx, view, g.t);
+    const R rr = g.t[0] * g.t[0] + g.t[1] * g.t[1] + g.t[2] * g.t[2];
+    if (rr <= K<R>(0.04f)) return 0;                                  // auxiliary.h:143
+    g.r = Fn<R>::sqrt(rr);
+    project(g.t, g.r, W, H, g);
+    cov3d(scales + 3 * idx, mod, rots + 4 * idx, g.cov3);
+    cov2d(g.J, view, g.cov3, g);
+    g.det = g.A * g.C - g.B * g.B;
+    if (g.det == R(0)) return 0;                                       // forward.cu:296
+    const R det_inv = R(1) / g.det;
+    const R mid = R(0.5f) * (g.A + g.C);
+    const R disc = Fn<R>::sqrt(std::max(K<R>(0.1f), mid * mid - g.det));
+    const R l1 = mid + disc;                                           // >= lambda2
+    const R radf = Fn<R>::ceil(R(3) * Fn<R>::sqrt(l1));
+    const int radius = (int)radf;
+    int rect[4];
+    get_rect(g.px, g.py, radius, gx, gy, rect);
+    if ((rect[2] - rect[0]) * (rect[3] - rect[1]) == 0) return 0;     // forward.cu:311
+    R hp[32], op[2], sig[2], ld, dist;
+    mlp_forward(idx, P, mlp, means + 3 * idx, rx, tx, hp, op, sig, &ld, &dist);
+    o.sig_re[idx] = sig[0];
+    o.sig_im[idx] = sig[1];
+    o.depth[idx] = g.r;
+    o.radii[idx] = radius;
+    o.xy[2 * idx] = g.px;
+    o.xy[2 * idx + 1] = g.py;
+    for (int k = 0; k < 6; ++k) o.cov3[6 * idx + k] = g.cov3[k];
+    o.conic_o[4 * idx + 0] = g.C * det_inv;
+    o.conic_o[4 * idx + 1] = -g.B * det_inv;
+    o.conic_o[4 * idx + 2] = g.A * det_inv;
+    o.conic_o[4 * idx + 3] = opac[idx];
+    o.tiles[idx] = (rect[3] - rect[1]) * (rect[2] - rect[0]);
+    for (int k = 0; k < 4; ++k) o.rect[idx * 4 + k] = rect[k];
+    if (gout) *gout = g;
+    return 1;
+}
+
+// ---------------------------------------------------------------------------
+// Blend: renderCUDA forward.cu:151-249 and backward.cu:442-616.
+// Per pair the power is evaluated as
+//   power = dx * (nb*dy + ha*dx) + hc*dy*dy,  ha = -qa/2, nb = -qb, hc = -qc/2
+// which equals -0.5(qa dx^2 + qc dy^2) - qb dx dy (forward.cu:215).
+// ---------------------------------------------------------------------------
+template <typename R>
+static R pair_power(R qa, R qb, R qc, R dx, R dy)
+{
+    const R ha = R(-0.5f) * qa, nb = -qb, hc = R(-0.5f) * qc;
+    const R e = Fn<R>::fma(nb, dy, ha * dx);
+    return Fn<R>::fma(dx, e, (hc * dy) * dy);
+}
+
+}  // namespace
+
+// ============================ exported C ABI ===============================
+
+#define PRE_ARGS(R)                                                                         \
+    int P, int W, int H, const R *means, const R *scales, const R *rots, const R *opac,     \
+        const R *mlp, const R *view, const R *tx, const R *rx, R mod, int *radii, R *xy,    \
+        R *depth, R *cov3, R *conic_o, R *sig_re, R *sig_im, int *tiles, int *rect
+
+template <typename R>
+static int preprocess_all(PRE_ARGS(R))
+{
+    PreOut<R> o{radii, xy, depth, cov3, conic_o, sig_re, sig_im, tiles, rect};
+    int vis = 0;
+    for (int i = 0; i < P; ++i)
+        vis += preprocess_one<R>(i, P, W, H, means, scales, rots, opac, mlp, view, tx, rx, mod, o, nullptr);
+    return vis;
+}
+extern "C" int orc_preprocess_f32(PRE_ARGS(float)) { return preprocess_all<float>(P, W, H, means, scales, rots, opac, mlp, view, tx, rx, mod, radii, xy, depth, cov3, conic_o, sig_re, sig_im, tiles, rect); }
+extern "C" int orc_preprocess_f64(PRE_ARGS(double)) { return preprocess_all<double>(P, W, H, means, scales, rots, opac, mlp, view, tx, rx, mod, radii, xy, depth, cov3, conic_o, sig_re, sig_im, tiles, rect); }
+
+// Binning: duplicateWithKeys (rasterizer_impl.cu:102-143), stable radix sort
+// on 32+msb bits (rasterizer_impl.cu:293-300) and identifyTileRanges
+// (rasterizer_impl.cu:149-171).  Keys are (tile << 32) | bits(float depth);
+// values are Gaussian indices in ascending order before the stable sort.
+// Call with keys == nullptr to get K.
+extern "C" int64_t orc_bin(int P, int W, int H, const int* rect, const float* depth, uint64_t* keys, uint32_t* vals,
+                uint32_t* ranges)
+{
+    const int gx = (W + BLOCK - 1) / BLOCK, gy = (H + BLOCK - 1) / BLOCK;
+    int64_t K = 0;
+    for (int i = 0; i < P; ++i) K += (int64_t)(rect[4 * i + 2] - rect[4 * i]) * (rect[4 * i + 3] - rect[4 * i + 1]);
+    if (!keys) return K;
+    std::vector<std::pair<uint64_t, uint32_t>> kv;
+    kv.reserve(K);
+    for (int i = 0; i < P; ++i) {
+        const int* rc = rect + 4 * i;
+        uint32_t db;
+        std::memcpy(&db, depth + i, 4);
+        for (int y = rc[1]; y < rc[3]; ++y)
+            for (int x = rc[0]; x < rc[2]; ++x)
+                kv.push_back({((uint64_t)(uint32_t)(y * gx + x) << 32) | db, (uint32_t)i});
+    }
+    std::stable_sort(kv.begin(), kv.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    for (int t = 0; t < gx * gy; ++t) ranges[2 * t] = ranges[2 * t + 1] = 0;
+    for (int64_t k = 0; k < K; ++k) {
+        keys[k] = kv[k].first;
+        vals[k] = kv[k].second;
+        const uint32_t tile = (uint32_t)(kv[k].first >> 32);
+        if (k == 0 || (uint32_t)(kv[k - 1].first >> 32) != tile) ranges[2 * tile] = (uint32_t)k;
+        if (k == K - 1 || (uint32_t)(kv[k + 1].first >> 32) != tile) ranges[2 * tile + 1] = (uint32_t)(k + 1);
+    }
+    return K;
+}
+
+// Blend forward.  stats[0] = pairs visited, [1] = skipped (power > 0),
+// [2] = skipped (alpha < 1/255), [3] = blended.  hash (optional) gets a
+// per-pixel fingerprint of the blended set, for finite-difference guards.
+template <typename R>
+static void render_fwd(int W, int H, const uint32_t* ranges, const uint32_t* plist, const R* xy, const R* conic_o,
+                       const R* sre, const R* sim, R* out_re, R* out_im, R* finalT, uint32_t* ncontrib,
+                       int64_t* stats, uint64_t* hash)
+{
+    const int gx = (W + BLOCK - 1) / BLOCK;
+    int64_t st[4] = {0, 0, 0, 0};
+    for (int py = 0; py < H; ++py)
+        for (int px = 0; px < W; ++px) {
+            const int tile = (py / BLOCK) * gx + (px / BLOCK);
+            const uint32_t b = ranges[2 * tile], e = ranges[2 * tile + 1];
+            R T = R(1), are = R(0), aim = R(0);
+            uint32_t contributor = 0, last = 0;
+            uint64_t h = 1469598103934665603ull;
+            const R pxf = (R)px, pyf = (R)py;
+            for (uint32_t k = b; k < e; ++k) {
+                const uint32_t id = plist[k];
+                contributor++;
+                st[0]++;
+                const R dx = xy[2 * id] - pxf, dy = xy[2 * id + 1] - pyf;
+                const R* co = conic_o + 4 * id;
+                const R power = pair_power(co[0], co[1], co[2], dx, dy);
+                if (power > R(0)) { st[1]++; continue; }
+                const R alpha = std::min(K<R>(0.99f), co[3] * Fn<R>::exp(power));
+                if (alpha < R(1.0f / 255.0f)) { st[2]++; continue; }
+                const R test_T = T * (R(1) - alpha);
+                if (test_T < K<R>(0.0001f)) break;
+                const R w = alpha * T;
+                are = Fn<R>::fma(sre[id], w, are);
+                aim = Fn<R>::fma(sim[id], w, aim);
+                T = test_T;
+                last = contributor;
+                st[3]++;
+                h = (h ^ id) * 1099511628211ull;
+            }
+            const int p = py * W + px;
+            out_re[p] = are;
+            out_im[p] = aim;
+            finalT[p] = T;
+            ncontrib[p] = last;
+            if (hash) hash[p] = h ^ last;
+        }
+    if (stats)
+        for (int i = 0; i < 4; ++i) stats[i] = st[i];
+}
+
+extern "C" void orc_render_fwd_f32(int W, int H, const uint32_t* ranges, const uint32_t* plist, const float* xy,
+                        const float* conic_o, const float* sre, const float* sim, float* out_re, float* out_im,
+                        float* finalT, uint32_t* nc, int64_t* stats, uint64_t* hash)
+{ render_fwd<float>(W, H, ranges, plist, xy, conic_o, sre, sim, out_re, out_im, finalT, nc, stats, hash); }
+extern "C" void orc_render_fwd_f64(int W, int H, const uint32_t* ranges, const uint32_t* plist, const double* xy,
+                        const double* conic_o, const double* sre, const double* sim, double* out_re,
+                        double* out_im, double* finalT, uint32_t* nc, int64_t* stats, uint64_t* hash)
+{ render_fwd<double>(W, H, ranges, plist, xy, conic_o, sre, sim, out_re, out_im, finalT, nc, stats, hash); }
+
+// Blend backward (backward.cu:442-616).  Outputs are accumulated in double,
+// 8 values per Gaussian: dL/dpx, dL/dpy (pixel units), dL/dqa, dL/dqb,
+// dL/dqc (true conic gradients, qb counted once), dL/dopacity, dL/dsig_re,
+// dL/dsig_im.  Gradients are not cleared here.
+template <typename R>
+static void render_bwd(int mode, int W, int H, const uint32_t* ranges, const uint32_t* plist, const R* xy,
+                       const R* conic_o, const R* sre, const R* sim, const R* finalT, const uint32_t* ncontrib,
+                       const R* dre, const R* dim, double* g8)
+{
+    const int gx = (W + BLOCK - 1) / BLOCK;
+    for (int py = 0; py < H; ++py)
+        for (int px = 0; px < W; ++px) {
+            const int tile = (py / BLOCK) * gx + (px / BLOCK);
+            const uint32_t b = ranges[2 * tile];
+            const int p = py * W + px;
+            R T = finalT[p];
+            const uint32_t last = ncontrib[p];
+            const R gre = dre[p], gim = dim[p];
+            R acc_re = R(0), acc_im = R(0), last_alpha = R(0), last_re = R(0), last_im = R(0);
+            const R pxf = (R)px, pyf = (R)py;
+            // contributor numbering: Gaussian k (0-based from b) has number k-b+1
+            for (uint32_t kk = (last > 0 ? b + last : b); kk-- > b;) {
+                const uint32_t id = plist[kk];
+                const R dx = xy[2 * id] - pxf, dy = xy[2 * id + 1] - pyf;
+                const R* co = conic_o + 4 * id;
+                const R power = pair_power(co[0], co[1], co[2], dx, dy);
+                if (power > R(0)) continue;
+                const R G = Fn<R>::exp(power);
+                const R oG = co[3] * G;
+                const R alpha = std::min(K<R>(0.99f), oG);
+                if (alpha < R(1.0f / 255.0f)) continue;
+                T = T / (R(1) - alpha);
+                const R w = alpha * T;
+                const R sr = sre[id];
+                const R si = (mode == GRAD_REFERENCE) ? sre[id] : sim[id];
+                acc_re = last_alpha * last_re + (R(1) - last_alpha) * acc_re;
+                acc_im = last_alpha * last_im + (R(1) - last_alpha) * acc_im;
+                last_re = sr;
+                last_im = si;
+                R dalpha = (sr - acc_re) * gre;
+                if (mode == GRAD_EXACT) dalpha += (si - acc_im) * gim;
+                dalpha *= T;
+                last_alpha = alpha;
+                double* o = g8 + 8 * (size_t)id;
+                o[6] += (double)(w * gre);
+                o[7] += (double)(w * gim);
+                if (mode == GRAD_EXACT && oG > K<R>(0.99f)) continue;   // clamped: no dependence on G, o
+                const R dG = co[3] * dalpha;
+                const R gdx = G * dx, gdy = G * dy;
+                const R dG_dx = -gdx * co[0] - gdy * co[1];
+                const R dG_dy = -gdy * co[2] - gdx * co[1];
+                o[0] += (double)(dG * dG_dx);
+                o[1] += (double)(dG * dG_dy);
+                o[2] += (double)(R(-0.5f) * gdx * dx * dG);
+                o[3] += (double)(-gdx * dy * dG);
+                o[4] += (double)(R(-0.5f) * gdy * dy * dG);
+                o[5] += (double)(G * dalpha);
+            }
+        }
+}
+
+extern "C" void orc_render_bwd_f32(int mode, int W, int H, const uint32_t* ranges, const uint32_t* plist, const float* xy,
+                        const float* conic_o, const float* sre, const float* sim, const float* finalT,
+                        const uint32_t* nc, const float* dre, const float* dim, double* g8)
+{ render_bwd<float>(mode, W, H, ranges, plist, xy, conic_o, sre, sim, finalT, nc, dre, dim, g8); }
+extern "C" void orc_render_bwd_f64(int mode, int W, int H, const uint32_t* ranges, const uint32_t* plist, const double* xy,
+                        const double* conic_o, const double* sre, const double* sim, const double* finalT,
+                        const uint32_t* nc, const double* dre, const double* dim, double* g8)
+{ render_bwd<double>(mode, W, H, ranges, plist, xy, conic_o, sre, sim, finalT, nc, dre, dim, g8); }
+
+
+// ---------------------------------------------------------------------------
+// Preprocess backward: computeCov2DCUDA backward.cu:124-303, preprocessCUDA
+// backward.cu:378-437, computeCov3D backward.cu:308-371, MLP backward
+// backward.cu:9-118.  Exact mode derives every term from the forward above.
+// ---------------------------------------------------------------------------
+namespace {
+
+template <typename R>
+static void pre_bwd_one(int mode, int idx, int P, int W, int H, const R* means, const R* scales, const R* rots,
+                        const R* mlp, const R* view, const R* tx, const R* rx, R mod, const double* g8,
+                        double* dmean, double* dscale, double* drot, double* dopac, double* dmlp, double* dcov)
+{
+    Geo<R> g;
+    view_xform(means + 3 * idx, view, g.t);
+    const R rr = g.t[0] * g.t[0] + g.t[1] * g.t[1] + g.t[2] * g.t[2];
+    g.r = Fn<R>::sqrt(rr);
+    project(g.t, g.r, W, H, g);
+    cov3d(scales + 3 * idx, mod, rots + 4 * idx, g.cov3);
+    const double* gi = g8 + 8 * (size_t)idx;
+    const R dpx = (R)gi[0], dpy = (R)gi[1];
+    const R gqa = (R)gi[2], gqb = (R)gi[3], gqc = (R)gi[4];
+    dopac[idx] += gi[5];
+    const R tx_ = g.t[0], ty_ = g.t[1], tz_ = g.t[2];
+
+    R Jc[2][3];
+    for (int a = 0; a < 2; ++a)
+        for (int k = 0; k < 3; ++k) Jc[a][k] = g.J[a][k];
+    if (mode == GRAD_REFERENCE) {   // backward.cu:172-173
+        Jc[0][0] *= R(0.5f);
+        Jc[0][2] *= R(0.5f);
+    }
+    cov2d(Jc, view, g.cov3, g);
+    const R A = g.A, B = g.B, C = g.C;
+    const R det = A * C - B * B;
+    R dA, dB, dC;
+    if (mode == GRAD_REFERENCE) {   // backward.cu:212-224
+        const R d2i = R(1) / (det * det + K<R>(1e-7f));
+        const R gy = R(0.5f) * gqb;
+        dA = d2i * (-C * C * gqa + R(2) * B * C * gy + (det - A * C) * gqc);
+        dC = d2i * (-A * A * gqc + R(2) * A * B * gy + (det - A * C) * gqa);
+        dB = d2i * R(2) * (B * C * gqa - (det + R(2) * B * B) * gy + A * B * gqc);
+    } else {
+        const R d2i = R(1) / (det * det);
+        dA = d2i * (-C * C * gqa + B * C * gqb - B * B * gqc);
+        dB = d2i * (R(2) * B * C * gqa - (det + R(2) * B * B) * gqb + R(2) * A * B * gqc);
+        dC = d2i * (-B * B * gqa + A * B * gqb - A * A * gqc);
+    }
+    // dL/dcov3 (6 unique entries), backward.cu:229-239
+    const R (*m)[3] = g.M2;
+    R dc[6];
+    const int ij[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+    for (int e = 0; e < 6; ++e) {
+        const int i = ij[e][0], j = ij[e][1];
+        if (i == j)
+            dc[e] = m[0][i] * m[0][i] * dA + m[0][i] * m[1][i] * dB + m[1][i] * m[1][i] * dC;
+        else
+            dc[e] = R(2) * m[0][i] * m[0][j] * dA + (m[0][i] * m[1][j] + m[0][j] * m[1][i]) * dB +
+                    R(2) * m[1][i] * m[1][j] * dC;
+    }
+    for (int e = 0; e < 6; ++e) dcov[6 * idx + e] += (double)dc[e];
+
+    // dL/dM2 rows, then dL/dJ = dL/dM2 Rv^T
+    const R* c = g.cov3;
+    const R S[3][3] = {{c[0], c[1], c[2]}, {c[1], c[3], c[4]}, {c[2], c[4], c[5]}};
+    R Sm0[3], Sm1[3];
+    for (int i = 0; i < 3; ++i) {
+        Sm0[i] = S[i][0] * m[0][0] + S[i][1] * m[0][1] + S[i][2] * m[0][2];
+        Sm1[i] = S[i][0] * m[1][0] + S[i][1] * m[1][1] + S[i][2] * m[1][2];
+    }
+    R dm0[3], dm1[3];
+    for (int i = 0; i < 3; ++i) {
+        dm0[i] = R(2) * dA * Sm0[i] + dB * Sm1[i];
+        dm1[i] = R(2) * dC * Sm1[i] + dB * Sm0[i];
+    }
+    // M2[a][k] = sum_r J[a][r] Rv[r][k], Rv[r][k] = view[k*4+r]
+    R dJ[2][3];
+    for (int r = 0; r < 3; ++r) {
+        dJ[0][r] = dm0[0] * view[0 * 4 + r] + dm0[1] * view[1 * 4 + r] + dm0[2] * view[2 * 4 + r];
+        dJ[1][r] = dm1[0] * view[0 * 4 + r] + dm1[1] * view[1 * 4 + r] + dm1[2] * view[2 * 4 + r];
+    }
+
+    const R Wd = (R)W * R(0.5f) * K<R>(ONE_OVER_PI_F);
+    const R Hd = (R)H * K<R>(ONE_OVER_PI_F);
+    R dt[3] = {R(0), R(0), R(0)};
+    if (mode == GRAD_REFERENCE) {
+        // backward.cu:152-177, 249-292 (second derivatives of the projection)
+        const R txtx = tx_ * tx_, tyty = ty_ * ty_, tztz = tz_ * tz_, txtytz = tx_ * ty_ * tz_;
+        const R rxz2 = txtx + tztz;
+        const R rxz2_inv = R(1) / (rxz2 + K<R>(1e-7f));
+        const R rxz4_inv = rxz2_inv * rxz2_inv;
+        const R rxz = Fn<R>::sqrt(rxz2);
+        const R rxz_inv = R(1) / (rxz + K<R>(1e-7f));
+        const R r2 = rxz2 + tyty;
+        const R r2_inv = R(1) / (r2 + K<R>(1e-7f));
+        const R r4_inv = r2_inv * r2_inv;
+        const R rxz_r4_inv = rxz_inv * r4_inv;
+        const R rxz3_r4_inv = rxz2_inv * rxz_r4_inv;
+        const R tyty_m_rxz2 = tyty - rxz2;
+        const R temp1 = Hd * tyty_m_rxz2 * rxz_r4_inv;
+        const R temp2 = Hd * txtytz * (r2 + R(2) * rxz2) * rxz3_r4_inv;
+        const R temp3 = Wd * (txtx - tztz) * rxz4_inv;
+        const R temp4 = Wd * R(2) * tx_ * tz_ * rxz4_inv;
+        const R temp5 = Hd * ty_ * rxz3_r4_inv;
+        dt[0] = -dJ[0][0] * temp4 + dJ[0][2] * temp3 + dJ[1][0] * temp5 * (R(2) * txtx * rxz2 - tztz * r2) +
+                dJ[1][1] * tx_ * temp1 + dJ[1][2] * temp2;
+        dt[1] = dJ[1][0] * tx_ * temp1 - dJ[1][1] * Hd * R(2) * rxz * ty_ * r4_inv + dJ[1][2] * tz_ * temp1;
+        dt[2] = dJ[0][0] * temp3 + dJ[0][2] * temp4 + dJ[1][0] * temp2 + dJ[1][1] * tz_ * temp1 +
+                dJ[1][2] * temp5 * (R(2) * tztz * rxz2 - txtx * r2);
+        // direct term through (px, py) with the stored dpx_dt / dpy_dt
+        dt[0] += dpx * Jc[0][0] + dpy * Jc[1][0];
+        dt[1] += dpx * Jc[0][1] + dpy * Jc[1][1];
+        dt[2] += dpx * Jc[0][2] + dpy * Jc[1][2];
+    } else {
+        // d J / d t of the forward formulas (forward.cu:82-97), epsilons included
+        const R eps = K<R>(1e-7f);
+        const R rxz2 = tx_ * tx_ + tz_ * tz_;
+        const R ia = R(1) / (rxz2 + eps);
+        const R rho = Fn<R>::sqrt(rxz2);
+        const R ib = R(1) / (rho + eps);
+        const R r2 = rxz2 + ty_ * ty_;
+        const R ic = R(1) / (r2 + eps);
+        const R g0x = dJ[0][0], g0z = dJ[0][2], h0 = dJ[1][0], h1 = dJ[1][1], h2 = dJ[1][2];
+        // row 0: (Wd tz ia, 0, -Wd tx ia)
+        dt[0] += g0x * Wd * tz_ * (R(-2) * tx_ * ia * ia) + g0z * (-Wd) * (ia - R(2) * tx_ * tx_ * ia * ia);
+        dt[2] += g0x * Wd * (ia - R(2) * tz_ * tz_ * ia * ia) + g0z * (-Wd) * tx_ * (R(-2) * tz_ * ia * ia);
+        // row 1 with K = ib*ic
+        const R Kc = ib * ic;
+        const R drho_x = rho > R(0) ? tx_ / rho : R(0), drho_z = rho > R(0) ? tz_ / rho : R(0);
+        const R dK[3] = {-ic * ib * ib * drho_x - R(2) * ib * ic * ic * tx_, -R(2) * ib * ic * ic * ty_,
+                         -ic * ib * ib * drho_z - R(2) * ib * ic * ic * tz_};
+        // j1x = -Hd tx ty K
+        dt[0] += h0 * (-Hd) * (ty_ * Kc + tx_ * ty_ * dK[0]);
+        dt[1] += h0 * (-Hd) * (tx_ * Kc + tx_ * ty_ * dK[1]);
+        dt[2] += h0 * (-Hd) * (tx_ * ty_ * dK[2]);
+        // j1z = -Hd tz ty K
+        dt[0] += h2 * (-Hd) * (tz_ * ty_ * dK[0]);
+        dt[1] += h2 * (-Hd) * (tz_ * Kc + tz_ * ty_ * dK[1]);
+        dt[2] += h2 * (-Hd) * (ty_ * Kc + tz_ * ty_ * dK[2]);
+        // j1y = Hd rho ic
+        dt[0] += h1 * Hd * (drho_x * ic - R(2) * rho * tx_ * ic * ic);
+        dt[1] += h1 * Hd * (-R(2) * rho * ty_ * ic * ic);
+        dt[2] += h1 * Hd * (drho_z * ic - R(2) * rho * tz_ * ic * ic);
+        // direct term: px = Wd' lon + c, lon = atan2(tx, tz); py = Hd' lat + c,
+        // lat = asin(ty / (r + eps)); dpx/dlon = W/2 fl(1/pi), dpy/dlat = H/2 fl(2/pi)
+        const R Wdp = (R)W * R(0.5) * K<R>(ONE_OVER_PI_F);
+        const R Hdp = (R)H * R(0.5) * K<R>(TWO_OVER_PI_F);
+        if (rxz2 > R(0)) {
+            dt[0] += dpx * Wdp * tz_ / rxz2;
+            dt[2] += dpx * Wdp * (-tx_) / rxz2;
+        }
+        const R re = g.r + eps;
+        const R u = ty_ / re;
+        const R dlat_du = R(1) / Fn<R>::sqrt(R(1) - u * u);
+        // du/dt = e_y / re - ty t / (r re^2)
+        const R k2 = ty_ / (g.r * re * re);
+        dt[0] += dpy * Hdp * dlat_du * (-k2 * tx_);
+        dt[1] += dpy * Hdp * dlat_du * (R(1) / re - k2 * ty_);
+        dt[2] += dpy * Hdp * dlat_du * (-k2 * tz_);
+    }
+    // dL/dmean = Rv^T dt (transformVec4x3Transpose, auxiliary.h:117-125)
+    for (int k = 0; k < 3; ++k)
+        dmean[3 * idx + k] += (double)(view[k * 4 + 0] * dt[0] + view[k * 4 + 1] * dt[1] + view[k * 4 + 2] * dt[2]);
+
+    // MLP backward (backward.cu:9-118)
+    {
+        R hp[32], op[2], sig[2], ld, d;
+        mlp_forward(idx, P, mlp, means + 3 * idx, rx, tx, hp, op, sig, &ld, &d);
+        const R* W1 = mlp + (size_t)idx * 96;
+        const R* W2 = mlp + (size_t)P * 128 + (size_t)idx * 64;
+        const R dsig[2] = {(R)gi[6], (R)gi[7]};
+        R dz[2];
+        for (int k = 0; k < 2; ++k) dz[k] = dsig[k] * sig[k] * (R(1) - sig[k]);
+        const R dld = -(dz[0] + dz[1]);
+        const R diff[3] = {tx[0] - means[3 * idx], tx[1] - means[3 * idx + 1], tx[2] - means[3 * idx + 2]};
+        const R de = d + K<R>(1e-6f);
+        // exact: dlogd/dmean = -diff / (d (d+eps)); reference: -diff / (d+eps)^2
+        const R den = (mode == GRAD_REFERENCE) ? de * de : d * de;
+        for (int k = 0; k < 3; ++k) dmean[3 * idx + k] += (double)(-dld * diff[k] / den);
+        R h[32], dh[32];
+        for (int j = 0; j < 32; ++j) { h[j] = hp[j] > R(0) ? hp[j] : K<R>(0.01f) * hp[j]; dh[j] = R(0); }
+        for (int k = 0; k < 2; ++k) {
+            for (int j = 0; j < 32; ++j) {
+                dmlp[(size_t)P * 128 + (size_t)idx * 64 + k * 32 + j] += (double)(dz[k] * h[j]);
+                dh[j] += dz[k] * W2[k * 32 + j];
+            }
+            dmlp[(size_t)P * 192 + (size_t)idx * 2 + k] += (double)dz[k];
+        }
+        for (int j = 0; j < 32; ++j) {
+            const R dp = hp[j] > R(0) ? dh[j] : K<R>(0.01f) * dh[j];
+            for (int i = 0; i < 3; ++i) dmlp[(size_t)idx * 96 + j * 3 + i] += (double)(dp * rx[i]);
+            dmlp[(size_t)P * 96 + (size_t)idx * 32 + j] += (double)dp;
+        }
+        (void)W1;
+    }
+
+    // cov3D backward -> scale, rotation (backward.cu:308-371)
+    {
+        const R* s = scales + 3 * idx;
+        const R* q = rots + 4 * idx;
+        const R r = q[0], x = q[1], y = q[2], z = q[3];
+        const R Rq[3][3] = {
+            {R(1) - R(2) * (y * y + z * z), R(2) * (x * y - r * z), R(2) * (x * z + r * y)},
+            {R(2) * (x * y + r * z), R(1) - R(2) * (x * x + z * z), R(2) * (y * z - r * x)},
+            {R(2) * (x * z - r * y), R(2) * (y * z + r * x), R(1) - R(2) * (x * x + y * y)}};
+        const R se[3] = {mod * s[0], mod * s[1], mod * s[2]};
+        const R Dg[3] = {se[0] * se[0], se[1] * se[1], se[2] * se[2]};
+        // symmetric matrix gradient from unique-entry gradients
+        const R G[3][3] = {{dc[0], R(0.5f) * dc[1], R(0.5f) * dc[2]},
+                           {R(0.5f) * dc[1], dc[3], R(0.5f) * dc[4]},
+                           {R(0.5f) * dc[2], R(0.5f) * dc[4], dc[5]}};
+        // dL/dRq = 2 G Rq D ; dL/dD_kk = (Rq^T G Rq)_kk
+        R GR[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int k = 0; k < 3; ++k)
+                GR[i][k] = R(2) * (G[i][0] * Rq[0][k] + G[i][1] * Rq[1][k] + G[i][2] * Rq[2][k]) * Dg[k];
+        for (int k = 0; k < 3; ++k) {
+            R dD = R(0);
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) dD += Rq[i][k] * G[i][j] * Rq[j][k];
+            // dD/ds = 2 mod^2 s ; the reference drops one factor mod (backward.cu:353-355)
+            const R f = (mode == GRAD_REFERENCE) ? R(2) * se[k] : R(2) * se[k] * mod;
+            dscale[3 * idx + k] += (double)(dD * f);
+        }
+        const R dRr[3][3] = {{0, -2 * z, 2 * y}, {2 * z, 0, -2 * x}, {-2 * y, 2 * x, 0}};
+        const R dRx[3][3] = {{0, 2 * y, 2 * z}, {2 * y, -4 * x, -2 * r}, {2 * z, 2 * r, -4 * x}};
+        const R dRy[3][3] = {{-4 * y, 2 * x, 2 * r}, {2 * x, 0, 2 * z}, {-2 * r, 2 * z, -4 * y}};
+        const R dRz[3][3] = {{-4 * z, -2 * r, 2 * x}, {2 * r, -4 * z, 2 * y}, {2 * x, 2 * y, 0}};
+        R dq[4] = {0, 0, 0, 0};
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                dq[0] += GR[i][j] * dRr[i][j];
+                dq[1] += GR[i][j] * dRx[i][j];
+                dq[2] += GR[i][j] * dRy[i][j];
+                dq[3] += GR[i][j] * dRz[i][j];
+            }
+        for (int k = 0; k < 4; ++k) drot[4 * idx + k] += (double)dq[k];
+    }
+}
+
+}  // namespace
+
+
+#define BWD_ARGS(R)                                                                                \
+    int mode, int P, int W, int H, const R *means, const R *scales, const R *rots, const R *mlp,   \
+        const R *view, const R *tx, const R *rx, R mod, const int *radii, const double *g8,        \
+        double *dmean, double *dscale, double *drot, double *dopac, double *dmlp, double *dcov
+
+template <typename R>
+static void pre_bwd_all(BWD_ARGS(R))
+{
+    for (int i = 0; i < P; ++i)
+        if (radii[i] > 0)
+            pre_bwd_one<R>(mode, i, P, W, H, means, scales, rots, mlp, view, tx, rx, mod, g8, dmean, dscale, drot,
+                           dopac, dmlp, dcov);
+}
+extern "C" void orc_preprocess_bwd_f32(BWD_ARGS(float)) { pre_bwd_all<float>(mode, P, W, H, means, scales, rots, mlp, view, tx, rx, mod, radii, g8, dmean, dscale, drot, dopac, dmlp, dcov); }
+extern "C" void orc_preprocess_bwd_f64(BWD_ARGS(double)) { pre_bwd_all<double>(mode, P, W, H, means, scales, rots, mlp, view, tx, rx, mod, radii, g8, dmean, dscale, drot, dopac, dmlp, dcov); }
+
+// ---------------------------------------------------------------------------
+// Fused SSIM (fused-ssim/ssim.cu:187-366, fused_ssim/__init__.py:8-41):
+// 11x11 separable Gaussian window (sigma 1.5) with zero padding.  Forward
+// returns the map and the three partial-derivative maps; backward returns
+// dL/dimg1.  Images are [B*CH] planes of H x W.
+// ---------------------------------------------------------------------------
+template <typename R>
+static void gauss11(R* w)
+{
+    double g[11], s = 0.0;
+    for (int i = 0; i < 11; ++i) { g[i] = std::exp(-(double)((i - 5) * (i - 5)) / (2.0 * 1.5 * 1.5)); s += g[i]; }
+    for (int i = 0; i < 11; ++i) w[i] = (R)(float)(g[i] / s);
+}
+
+template <typename R>
+static void conv_same(int H, int W, const R* in, R* out)   // separable, x then y
+{
+    R w[11];
+    gauss11(w);
+    std::vector<R> tmp((size_t)H * W);
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            R v = R(0);
+            for (int k = 0; k < 11; ++k) {
+                const int xx = x + k - 5;
+                v += w[k] * ((xx >= 0 && xx < W) ? in[(size_t)y * W + xx] : R(0));
+            }
+            tmp[(size_t)y * W + x] = v;
+        }
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            R v = R(0);
+            for (int k = 0; k < 11; ++k) {
+                const int yy = y + k - 5;
+                v += w[k] * ((yy >= 0 && yy < H) ? tmp[(size_t)yy * W + x] : R(0));
+            }
+            out[(size_t)y * W + x] = v;
+        }
+}
+
+template <typename R>
+static void ssim_fwd(int planes, int H, int W, R C1, R C2, const R* a, const R* b, R* map, R* dmu1, R* ds1, R* ds12)
+{
+    const size_t N = (size_t)H * W;
+    std::vector<R> mu1(N), mu2(N), e11(N), e22(N), e12(N), aa(N), bb(N), ab(N);
+    for (int p = 0; p < planes; ++p) {
+        const R* x = a + p * N;
+        const R* y = b + p * N;
+        for (size_t i = 0; i < N; ++i) { aa[i] = x[i] * x[i]; bb[i] = y[i] * y[i]; ab[i] = x[i] * y[i]; }
+        conv_same(H, W, x, mu1.data());
+        conv_same(H, W, y, mu2.data());
+        conv_same(H, W, aa.data(), e11.data());
+        conv_same(H, W, bb.data(), e22.data());
+        conv_same(H, W, ab.data(), e12.data());
+        for (size_t i = 0; i < N; ++i) {
+            const R m1 = mu1[i], m2 = mu2[i];
+            const R s1 = e11[i] - m1 * m1, s2 = e22[i] - m2 * m2, s12 = e12[i] - m1 * m2;
+            const R Cc = R(2) * m1 * m2 + C1, D = R(2) * s12 + C2;
+            const R A = m1 * m1 + m2 * m2 + C1, B = s1 + s2 + C2;
+            map[p * N + i] = (Cc * D) / (A * B);
+            if (dmu1) {
+                dmu1[p * N + i] = (m2 * R(2) * D) / (A * B) - (m2 * R(2) * Cc) / (A * B) -
+                                  (m1 * R(2) * Cc * D) / (A * A * B) + (m1 * R(2) * Cc * D) / (A * B * B);
+                ds1[p * N + i] = (-Cc * D) / (A * B * B);
+                ds12[p * N + i] = (R(2) * Cc) / (A * B);
+            }
+        }
+    }
+}
+
+template <typename R>
+static void ssim_bwd(int planes, int H, int W, const R* a, const R* b, const R* g, const R* dmu1, const R* ds1,
+                     const R* ds12, R* out)
+{
+    const size_t N = (size_t)H * W;
+    std::vector<R> t0(N), t1(N), t2(N), c0(N), c1(N), c2(N);
+    for (int p = 0; p < planes; ++p) {
+        for (size_t i = 0; i < N; ++i) {
+            t0[i] = g[p * N + i] * dmu1[p * N + i];
+            t1[i] = g[p * N + i] * ds1[p * N + i];
+            t2[i] = g[p * N + i] * ds12[p * N + i];
+        }
+        conv_same(H, W, t0.data(), c0.data());
+        conv_same(H, W, t1.data(), c1.data());
+        conv_same(H, W, t2.data(), c2.data());
+        for (size_t i = 0; i < N; ++i)
+            out[p * N + i] = c0[i] + a[p * N + i] * R(2) * c1[i] + b[p * N + i] * c2[i];
+    }
+}
+
+extern "C" void orc_ssim_fwd_f32(int planes, int H, int W, float C1, float C2, const float* a, const float* b, float* map,
+                      float* dmu1, float* ds1, float* ds12)
+{ ssim_fwd<float>(planes, H, W, C1, C2, a, b, map, dmu1, ds1, ds12); }
+extern "C" void orc_ssim_fwd_f64(int planes, int H, int W, double C1, double C2, const double* a, const double* b,
+                      double* map, double* dmu1, double* ds1, double* ds12)
+{ ssim_fwd<double>(planes, H, W, C1, C2, a, b, map, dmu1, ds1, ds12); }
+extern "C" void orc_ssim_bwd_f32(int planes, int H, int W, const float* a, const float* b, const float* g, const float* dmu1,
+                      const float* ds1, const float* ds12, float* out)
+{ ssim_bwd<float>(planes, H, W, a, b, g, dmu1, ds1, ds12, out); }
+extern "C" void orc_ssim_bwd_f64(int planes, int H, int W, const double* a, const double* b, const double* g,
+                      const double* dmu1, const double* ds1, const double* ds12, double* out)
+{ ssim_bwd<double>(planes, H, W, a, b, g, dmu1, ds1, ds12, out); }
+
+// math functions exported for the accuracy tests
+extern "C" float orc_gs_expf(float x) { return gs_expf(x); }
+extern "C" float orc_gs_logf(float x) { return gs_logf(x); }
+extern "C" float orc_gs_atan2f(float y, float x) { return gs_atan2f(y, x); }
+extern "C" float orc_gs_asinf(float x) { return gs_asinf(x); }
+

@@ -1,0 +1,199 @@
+"""ctypes front end of the CPU oracle (oracle/gs_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  Nothing in the product imports
+it.  Build with `make -C oracle` (done by __graft_entry__.build()).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libgs_oracle.so")
+_lib = None
+
+GRAD_EXACT = 0
+GRAD_REFERENCE = 1
+BLOCK = 16
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-C", _HERE, "-s"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _lib.orc_bin.restype = C.c_int64
+        for n in ("orc_gs_expf", "orc_gs_logf", "orc_gs_asinf"):
+            getattr(_lib, n).restype = C.c_float
+            getattr(_lib, n).argtypes = [C.c_float]
+        _lib.orc_gs_atan2f.restype = C.c_float
+        _lib.orc_gs_atan2f.argtypes = [C.c_float, C.c_float]
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _ft(dtype):
+    return C.c_float if dtype == np.float32 else C.c_double
+
+
+class Pre:
+    """Per-Gaussian preprocess outputs for one receiver."""
+
+    def __init__(self, P, dtype):
+        self.radii = np.zeros(P, np.int32)
+        self.xy = np.zeros((P, 2), dtype)
+        self.depth = np.zeros(P, dtype)
+        self.cov3 = np.zeros((P, 6), dtype)
+        self.conic_o = np.zeros((P, 4), dtype)
+        self.sig_re = np.zeros(P, dtype)
+        self.sig_im = np.zeros(P, dtype)
+        self.tiles = np.zeros(P, np.int32)
+        self.rect = np.zeros((P, 4), np.int32)
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def preprocess(scene, r=0, dtype=np.float32, mod=1.0):
+    """forward.cu:252-327 for receiver r."""
+    P, W, H = scene.P, scene.W, scene.H
+    args = [_c(scene.means, dtype), _c(scene.scales, dtype), _c(scene.rotations, dtype),
+            _c(scene.opacities.reshape(-1), dtype), _c(scene.mlp, dtype), _c(scene.views[r].reshape(-1), dtype),
+            _c(scene.tx_pos, dtype), _c(scene.rx_pos[r], dtype)]
+    o = Pre(P, dtype)
+    f = lib().orc_preprocess_f32 if dtype == np.float32 else lib().orc_preprocess_f64
+    f(C.c_int(P), C.c_int(W), C.c_int(H), *[_p(a) for a in args], _ft(dtype)(mod),
+      _p(o.radii), _p(o.xy), _p(o.depth), _p(o.cov3), _p(o.conic_o), _p(o.sig_re), _p(o.sig_im),
+      _p(o.tiles), _p(o.rect))
+    return o
+
+
+def n_tiles(W, H):
+    return ((W + BLOCK - 1) // BLOCK) * ((H + BLOCK - 1) // BLOCK)
+
+
+def bin_gaussians(pre, W, H):
+    """Keys, values (point list) and tile ranges (rasterizer_impl.cu:102-171)."""
+    P = pre.radii.shape[0]
+    depth32 = np.ascontiguousarray(pre.depth, dtype=np.float32)
+    rect = np.ascontiguousarray(pre.rect)
+    K = lib().orc_bin(C.c_int(P), C.c_int(W), C.c_int(H), _p(rect), _p(depth32), None, None, None)
+    keys = np.zeros(max(K, 1), np.uint64)
+    vals = np.zeros(max(K, 1), np.uint32)
+    ranges = np.zeros((n_tiles(W, H), 2), np.uint32)
+    lib().orc_bin(C.c_int(P), C.c_int(W), C.c_int(H), _p(rect), _p(depth32), _p(keys), _p(vals), _p(ranges))
+    return keys[:K], vals[:K], ranges
+
+
+def render_fwd(pre, vals, ranges, W, H, dtype=None, want_hash=False):
+    dtype = dtype or pre.xy.dtype
+    N = W * H
+    out_re = np.zeros(N, dtype)
+    out_im = np.zeros(N, dtype)
+    T = np.zeros(N, dtype)
+    nc = np.zeros(N, np.uint32)
+    stats = np.zeros(4, np.int64)
+    h = np.zeros(N, np.uint64) if want_hash else None
+    vals = np.ascontiguousarray(vals if len(vals) else np.zeros(1, np.uint32), dtype=np.uint32)
+    f = lib().orc_render_fwd_f32 if dtype == np.float32 else lib().orc_render_fwd_f64
+    f(C.c_int(W), C.c_int(H), _p(np.ascontiguousarray(ranges)), _p(vals), _p(_c(pre.xy, dtype)),
+      _p(_c(pre.conic_o, dtype)), _p(_c(pre.sig_re, dtype)), _p(_c(pre.sig_im, dtype)),
+      _p(out_re), _p(out_im), _p(T), _p(nc), _p(stats), _p(h) if h is not None else None)
+    res = dict(out_re=out_re.reshape(H, W), out_im=out_im.reshape(H, W), final_T=T.reshape(H, W),
+               n_contrib=nc.reshape(H, W), stats=stats)
+    if want_hash:
+        res["hash"] = h
+    return res
+
+
+def render_bwd(pre, vals, ranges, fwd, dre, dim, W, H, mode, dtype=None):
+    """Returns [P,8] float64: dpx, dpy, dqa, dqb, dqc, dopacity, dsig_re, dsig_im."""
+    dtype = dtype or pre.xy.dtype
+    P = pre.radii.shape[0]
+    g8 = np.zeros((P, 8), np.float64)
+    vals = np.ascontiguousarray(vals if len(vals) else np.zeros(1, np.uint32), dtype=np.uint32)
+    f = lib().orc_render_bwd_f32 if dtype == np.float32 else lib().orc_render_bwd_f64
+    f(C.c_int(mode), C.c_int(W), C.c_int(H), _p(np.ascontiguousarray(ranges)), _p(vals),
+      _p(_c(pre.xy, dtype)), _p(_c(pre.conic_o, dtype)), _p(_c(pre.sig_re, dtype)), _p(_c(pre.sig_im, dtype)),
+      _p(_c(fwd["final_T"].reshape(-1), dtype)), _p(np.ascontiguousarray(fwd["n_contrib"].reshape(-1))),
+      _p(_c(np.asarray(dre).reshape(-1), dtype)), _p(_c(np.asarray(dim).reshape(-1), dtype)), _p(g8))
+    return g8
+
+
+class Grads:
+    def __init__(self, P):
+        self.means = np.zeros((P, 3))
+        self.scales = np.zeros((P, 3))
+        self.rotations = np.zeros((P, 4))
+        self.opacities = np.zeros((P, 1))
+        self.mlp = np.zeros(P * 194)
+        self.cov3 = np.zeros((P, 6))
+
+
+def preprocess_bwd(scene, pre, g8, mode, r=0, dtype=np.float32, mod=1.0, grads=None):
+    """backward.cu:124-437 for receiver r; accumulates into `grads`."""
+    P, W, H = scene.P, scene.W, scene.H
+    g = grads or Grads(P)
+    args = [_c(scene.means, dtype), _c(scene.scales, dtype), _c(scene.rotations, dtype), _c(scene.mlp, dtype),
+            _c(scene.views[r].reshape(-1), dtype), _c(scene.tx_pos, dtype), _c(scene.rx_pos[r], dtype)]
+    f = lib().orc_preprocess_bwd_f32 if dtype == np.float32 else lib().orc_preprocess_bwd_f64
+    f(C.c_int(mode), C.c_int(P), C.c_int(W), C.c_int(H), *[_p(a) for a in args], _ft(dtype)(mod),
+      _p(np.ascontiguousarray(pre.radii)), _p(np.ascontiguousarray(g8, dtype=np.float64)),
+      _p(g.means), _p(g.scales), _p(g.rotations), _p(g.opacities), _p(g.mlp), _p(g.cov3))
+    return g
+
+
+def render(scene, r=0, dtype=np.float32):
+    """Full forward for one receiver: preprocess, binning, blend."""
+    pre = preprocess(scene, r, dtype)
+    keys, vals, ranges = bin_gaussians(pre, scene.W, scene.H)
+    fwd = render_fwd(pre, vals, ranges, scene.W, scene.H, dtype)
+    return pre, keys, vals, ranges, fwd
+
+
+def backward(scene, pre, vals, ranges, fwd, dre, dim, mode, r=0, dtype=np.float32, grads=None):
+    g8 = render_bwd(pre, vals, ranges, fwd, dre, dim, scene.W, scene.H, mode, dtype)
+    return preprocess_bwd(scene, pre, g8, mode, r, dtype, grads=grads), g8
+
+
+# ---------------------------------------------------------------- SSIM
+C1 = 0.01 ** 2
+C2 = 0.03 ** 2
+
+
+def ssim_fwd(img1, img2, dtype=np.float64, train=True):
+    """fused-ssim/ssim.cu:187-286.  img: [B,CH,H,W]."""
+    B, CH, H, W = img1.shape
+    a = _c(img1, dtype)
+    b = _c(img2, dtype)
+    m = np.zeros_like(a)
+    d1 = np.zeros_like(a) if train else None
+    d2 = np.zeros_like(a) if train else None
+    d3 = np.zeros_like(a) if train else None
+    f = lib().orc_ssim_fwd_f32 if dtype == np.float32 else lib().orc_ssim_fwd_f64
+    f(C.c_int(B * CH), C.c_int(H), C.c_int(W), _ft(dtype)(C1), _ft(dtype)(C2), _p(a), _p(b), _p(m),
+      _p(d1) if train else None, _p(d2) if train else None, _p(d3) if train else None)
+    return m, d1, d2, d3
+
+
+def ssim_bwd(img1, img2, dmap, d1, d2, d3, dtype=np.float64):
+    """fused-ssim/ssim.cu:288-366."""
+    B, CH, H, W = img1.shape
+    out = np.zeros(img1.shape, dtype)
+    f = lib().orc_ssim_bwd_f32 if dtype == np.float32 else lib().orc_ssim_bwd_f64
+    f(C.c_int(B * CH), C.c_int(H), C.c_int(W), _p(_c(img1, dtype)), _p(_c(img2, dtype)), _p(_c(dmap, dtype)),
+      _p(_c(d1, dtype)), _p(_c(d2, dtype)), _p(_c(d3, dtype)), _p(out))
+    return out
