@@ -10,7 +10,7 @@
 //   sig[r*P + g]    = float2 {real, imag} emission
 //   depth[r*P + g]  = float distance to the receiver (sort key)
 //   radii[r*P + g]  = int, 0 = culled
-//   blockcnt[(r*T + t)*NB + b] = Gaussians of block b (256 Gaussians) that
+//   wcnt[(r*NB + b)*T + t] = Gaussians of block b (256 Gaussians) that
 //                     touch tile t of receiver r; after the column scan, the
 //                     exclusive prefix over b
 //   segbase[r*T + t], segcnt[r*T + t] = tile range (global offsets)

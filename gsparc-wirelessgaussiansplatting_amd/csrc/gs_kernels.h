@@ -10,7 +10,7 @@ namespace gs {
 struct Dims {
     int P, R, W, H;
     int gx, gy, T;   // tile grid
-    int NBW;         // wave-blocks of 64 Gaussians
+    int NB;          // preprocess blocks of 256 Gaussians
 };
 
 inline Dims make_dims(int P, int R, int W, int H)
@@ -18,7 +18,7 @@ inline Dims make_dims(int P, int R, int W, int H)
     Dims d;
     d.P = P; d.R = R; d.W = W; d.H = H;
     d.gx = (W + 15) / 16; d.gy = (H + 15) / 16; d.T = d.gx * d.gy;
-    d.NBW = (P + 63) / 64;
+    d.NB = (P + 255) / 256;
     return d;
 }
 
@@ -37,7 +37,7 @@ struct PreFwdArgs {
     float* depth;            // [R*P]
     int* radii;              // [R*P]
     float* cov3;             // [P*6]
-    uint32_t* wcnt;          // [R*NBW*T]
+    uint32_t* wcnt;          // [R*NB*T]
 };
 hipError_t launch_preprocess_fwd(const Dims& d, const PreFwdArgs& a, hipStream_t s);
 
@@ -45,7 +45,7 @@ struct BinArgs {
     const float4* splat;     // for px, py
     const int* radii;
     const float* depth;
-    uint32_t* wcnt;          // [R*NBW*T] counts in, exclusive prefix out
+    uint32_t* wcnt;          // [R*NB*T] counts in, exclusive prefix out
     uint32_t* segcnt;        // [R*T]
     uint32_t* segbase;       // [R*T]
     uint2* ranges;           // [R*T]
@@ -71,6 +71,8 @@ struct RenderFwdArgs {
     float* out_im;           // [R*N]
     float* finalT;           // [R*N]
     uint32_t* ncontrib;      // [R*N]
+    const uint32_t* total;   // K as computed on the device
+    uint64_t Kcap;           // capacity of plist (nothing is read when K > Kcap)
 };
 hipError_t launch_render_fwd(const Dims& d, const RenderFwdArgs& a, hipStream_t s);
 
@@ -85,6 +87,8 @@ struct RenderBwdArgs {
     const float* dout_im;
     float* g8;               // [R*P*8] accumulated: dpx dpy dqa dqb dqc dop dre dim
     int mode;
+    const uint32_t* total;
+    uint64_t Kcap;
 };
 hipError_t launch_render_bwd(const Dims& d, const RenderBwdArgs& a, hipStream_t s);
 

@@ -1,0 +1,294 @@
+// gsparc_abi.hip -- the exported C ABI (include/gsparc.h): buffer layout,
+// argument checks and the launch sequence of the forward and backward.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/gsparc.h"
+#include "gs_device.h"
+#include "gs_kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define GS_HIP(call)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) return fail(GSPARC_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr size_t ALIGN = 256;
+inline size_t al(size_t x) { return (x + ALIGN - 1) / ALIGN * ALIGN; }
+
+void make_layout(int P, int R, int W, int H, uint64_t K, gsparc_layout& L)
+{
+    const gs::Dims d = gs::make_dims(P, R, W, H);
+    const size_t RP = (size_t)R * P, N = (size_t)W * H, RT = (size_t)R * d.T;
+    size_t o = 0;
+    L.splat = o; o = al(o + RP * 2 * sizeof(float4));
+    L.sig = o; o = al(o + RP * sizeof(float2));
+    L.depth = o; o = al(o + RP * sizeof(float));
+    L.cov3 = o; o = al(o + (size_t)P * 6 * sizeof(float));
+    L.wcnt = o; o = al(o + (size_t)R * d.NB * d.T * sizeof(uint32_t));
+    L.geom_bytes = o + ALIGN;
+    o = 0;
+    L.final_t = o; o = al(o + (size_t)R * N * sizeof(float));
+    L.n_contrib = o; o = al(o + (size_t)R * N * sizeof(uint32_t));
+    L.ranges = o; o = al(o + RT * sizeof(uint2));
+    L.segcnt = o; o = al(o + RT * sizeof(uint32_t));
+    L.segbase = o; o = al(o + RT * sizeof(uint32_t));
+    L.total = o; o = al(o + 4 * sizeof(uint32_t));
+    L.image_bytes = o + ALIGN;
+    o = 0;
+    const size_t Kn = (size_t)(K > 0 ? K : 1);
+    L.keys = o; o = al(o + Kn * 4);
+    L.vals = o; o = al(o + Kn * 4);
+    L.keys_tmp = o; o = al(o + Kn * 4);
+    L.vals_tmp = o; o = al(o + Kn * 4);
+    L.plist = o; o = al(o + Kn * 4);
+    L.keys_sorted = o; o = al(o + Kn * 4);
+    L.bin_bytes = o + ALIGN;
+}
+
+// Buffers from torch are 256-byte aligned already; be safe for others.
+inline char* base_of(void* p) { return (char*)(((uintptr_t)p + ALIGN - 1) / ALIGN * ALIGN); }
+
+int check_dims(int P, int R, int W, int H)
+{
+    if (P < 0 || R < 0 || W <= 0 || H <= 0) return fail(GSPARC_ERR_ARG, "negative or zero size");
+    if ((int64_t)R * W * H > (int64_t)1 << 31) return fail(GSPARC_ERR_ARG, "R*H*W too large");
+    if ((int64_t)P * R >= (int64_t)1 << 31) return fail(GSPARC_ERR_ARG, "P*R too large");
+    return GSPARC_OK;
+}
+
+__global__ void k_mark_visible(int P, const float* means, const float* view, uint8_t* present)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    float t[3];
+    gs::view_xform(means[3 * i], means[3 * i + 1], means[3 * i + 2], view, t);
+    const float rr = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+    present[i] = rr <= 0.04f ? 0 : 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gsparc_version(void) { return "gsparc-mi355x 0.1 (gfx950)"; }
+const char* gsparc_last_error(void) { return g_err.c_str(); }
+
+size_t gsparc_geom_buffer_size(int P, int R, int W, int H)
+{
+    gsparc_layout L;
+    make_layout(P, R, W, H, 0, L);
+    return L.geom_bytes;
+}
+
+size_t gsparc_image_buffer_size(int P, int R, int W, int H)
+{
+    gsparc_layout L;
+    make_layout(P, R, W, H, 0, L);
+    return L.image_bytes;
+}
+
+size_t gsparc_binning_buffer_size(uint64_t K)
+{
+    gsparc_layout L;
+    make_layout(0, 0, 1, 1, K, L);
+    return L.bin_bytes;
+}
+
+size_t gsparc_grad_buffer_size(int P, int R) { return (size_t)P * R * 8 * sizeof(float) + ALIGN; }
+
+int gsparc_layout_query(int P, int R, int W, int H, uint64_t K, gsparc_layout* out)
+{
+    if (!out) return fail(GSPARC_ERR_ARG, "null layout");
+    make_layout(P, R, W, H, K, *out);
+    return GSPARC_OK;
+}
+
+int gsparc_rasterize_forward_geom(int P, int R, int W, int H, const float* means, const float* scales,
+                                  const float* rotations, const float* opacities, const float* emission_mlps,
+                                  const float* viewmatrices, const float* tx_pos, const float* rx_pos,
+                                  float scale_modifier, void* geom, void* image, int* radii_out,
+                                  uint64_t* K_out, void* stream)
+{
+    if (int e = check_dims(P, R, W, H)) return e;
+    if (!geom || !image || !K_out || (P > 0 && R > 0 && (!means || !scales || !rotations || !opacities ||
+                                                        !viewmatrices || !tx_pos || !rx_pos || !radii_out)))
+        return fail(GSPARC_ERR_ARG, "null pointer");
+    hipStream_t s = (hipStream_t)stream;
+    const gs::Dims d = gs::make_dims(P, R, W, H);
+    gsparc_layout L;
+    make_layout(P, R, W, H, 0, L);
+    char* gb = base_of(geom);
+    char* ib = base_of(image);
+    gs::PreFwdArgs pa;
+    pa.means = means; pa.scales = scales; pa.rots = rotations; pa.opac = opacities; pa.mlp = emission_mlps;
+    pa.views = viewmatrices; pa.tx = tx_pos; pa.rx = rx_pos; pa.mod = scale_modifier;
+    pa.splat = (float4*)(gb + L.splat); pa.sig = (float2*)(gb + L.sig); pa.depth = (float*)(gb + L.depth);
+    pa.radii = radii_out; pa.cov3 = (float*)(gb + L.cov3); pa.wcnt = (uint32_t*)(gb + L.wcnt);
+    if (P > 0 && R > 0) GS_HIP(gs::launch_preprocess_fwd(d, pa, s));
+    gs::BinArgs ba{};
+    ba.wcnt = pa.wcnt;
+    ba.segcnt = (uint32_t*)(ib + L.segcnt);
+    ba.segbase = (uint32_t*)(ib + L.segbase);
+    ba.ranges = (uint2*)(ib + L.ranges);
+    ba.total = (uint32_t*)(ib + L.total);
+    if (R > 0) {
+        GS_HIP(gs::launch_bin_scan(d, ba, s));
+        uint32_t tot[2] = {0, 0};
+        GS_HIP(hipMemcpyAsync(tot, ba.total, sizeof(tot), hipMemcpyDeviceToHost, s));
+        GS_HIP(hipStreamSynchronize(s));
+        *K_out = tot[0];
+    } else {
+        *K_out = 0;
+    }
+    return GSPARC_OK;
+}
+
+int gsparc_rasterize_forward_render(int P, int R, int W, int H, const float* emission, void* geom, void* image,
+                                    void* bin, uint64_t K_capacity, const int* radii, float* out_real,
+                                    float* out_imag, int flags, void* stream)
+{
+    if (int e = check_dims(P, R, W, H)) return e;
+    if (!geom || !image || !bin || (R > 0 && (!out_real || !out_imag || (P > 0 && !radii))))
+        return fail(GSPARC_ERR_ARG, "null pointer");
+    if (R == 0) return GSPARC_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const gs::Dims d = gs::make_dims(P, R, W, H);
+    gsparc_layout L;
+    make_layout(P, R, W, H, K_capacity, L);
+    char* gb = base_of(geom);
+    char* ib = base_of(image);
+    char* bb = base_of(bin);
+    gs::BinArgs ba{};
+    ba.splat = (const float4*)(gb + L.splat);
+    ba.radii = radii;
+    ba.depth = (const float*)(gb + L.depth);
+    ba.wcnt = (uint32_t*)(gb + L.wcnt);
+    ba.segcnt = (uint32_t*)(ib + L.segcnt);
+    ba.segbase = (uint32_t*)(ib + L.segbase);
+    ba.ranges = (uint2*)(ib + L.ranges);
+    ba.total = (uint32_t*)(ib + L.total);
+    ba.keys = (uint32_t*)(bb + L.keys);
+    ba.vals = (uint32_t*)(bb + L.vals);
+    ba.keys_tmp = (uint32_t*)(bb + L.keys_tmp);
+    ba.vals_tmp = (uint32_t*)(bb + L.vals_tmp);
+    ba.plist = (uint32_t*)(bb + L.plist);
+    ba.keys_sorted = (flags & GSPARC_FLAG_KEEP_KEYS) ? (uint32_t*)(bb + L.keys_sorted) : nullptr;
+    ba.Kcap = K_capacity;
+    if (P > 0) GS_HIP(gs::launch_bin_scatter_sort(d, ba, s));
+    gs::RenderFwdArgs ra;
+    ra.ranges = ba.ranges;
+    ra.plist = ba.plist;
+    ra.splat = ba.splat;
+    ra.sig = emission ? (const float2*)emission : (const float2*)(gb + L.sig);
+    ra.out_re = out_real;
+    ra.out_im = out_imag;
+    ra.finalT = (float*)(ib + L.final_t);
+    ra.ncontrib = (uint32_t*)(ib + L.n_contrib);
+    ra.total = ba.total;
+    ra.Kcap = K_capacity;
+    GS_HIP(gs::launch_render_fwd(d, ra, s));
+    return GSPARC_OK;
+}
+
+int gsparc_rasterize_backward(int grad_mode, int P, int R, int W, int H, const float* means, const float* scales,
+                              const float* rotations, const float* emission_mlps, const float* emission,
+                              const float* viewmatrices, const float* tx_pos, const float* rx_pos,
+                              float scale_modifier, const int* radii, void* geom, void* image, void* bin,
+                              uint64_t K_capacity, void* grad, const float* dout_real, const float* dout_imag, float* dmeans2D,
+                              float* dmeans3D, float* demission_mlps, float* dopacity, float* dcov3D,
+                              float* dscales, float* drotations, float* demission, void* stream)
+{
+    if (int e = check_dims(P, R, W, H)) return e;
+    if (grad_mode != GSPARC_GRAD_EXACT && grad_mode != GSPARC_GRAD_REFERENCE)
+        return fail(GSPARC_ERR_ARG, "unknown grad_mode");
+    if (P == 0) return GSPARC_OK;
+    if (!geom || !image || !bin || !grad || !means || !scales || !rotations || !viewmatrices || !tx_pos ||
+        !rx_pos || !radii || !dout_real || !dout_imag || !dmeans3D || !dopacity || !dscales || !drotations)
+        return fail(GSPARC_ERR_ARG, "null pointer");
+    if (emission_mlps && R > 256) return fail(GSPARC_ERR_ARG, "at most 256 receivers per call with emission MLPs");
+    hipStream_t s = (hipStream_t)stream;
+    const gs::Dims d = gs::make_dims(P, R, W, H);
+    gsparc_layout L;
+    make_layout(P, R, W, H, K_capacity, L);
+    char* gb = base_of(geom);
+    char* ib = base_of(image);
+    char* bb = base_of(bin);
+    float* g8 = (float*)base_of(grad);
+    GS_HIP(gs::launch_fill_u32((uint32_t*)g8, 0u, (size_t)R * P * 8, s));
+    gs::RenderBwdArgs ra;
+    ra.ranges = (const uint2*)(ib + L.ranges);
+    ra.plist = (const uint32_t*)(bb + L.plist);
+    ra.splat = (const float4*)(gb + L.splat);
+    ra.sig = emission ? (const float2*)emission : (const float2*)(gb + L.sig);
+    ra.finalT = (const float*)(ib + L.final_t);
+    ra.ncontrib = (const uint32_t*)(ib + L.n_contrib);
+    ra.dout_re = dout_real;
+    ra.dout_im = dout_imag;
+    ra.g8 = g8;
+    ra.mode = grad_mode;
+    ra.total = (const uint32_t*)(ib + L.total);
+    ra.Kcap = K_capacity;
+    GS_HIP(gs::launch_render_bwd(d, ra, s));
+    gs::PreBwdArgs pa;
+    pa.means = means; pa.scales = scales; pa.rots = rotations; pa.mlp = emission_mlps; pa.views = viewmatrices;
+    pa.tx = tx_pos; pa.rx = rx_pos; pa.mod = scale_modifier; pa.radii = radii; pa.g8 = g8;
+    pa.dmeans = dmeans3D; pa.dscales = dscales; pa.drots = drotations; pa.dopac = dopacity;
+    pa.dmlp = demission_mlps; pa.dcov3 = dcov3D; pa.dmeans2d = dmeans2D; pa.dsig = demission; pa.mode = grad_mode;
+    GS_HIP(gs::launch_preprocess_bwd(d, pa, s));
+    return GSPARC_OK;
+}
+
+int gsparc_mark_visible(int P, const float* means, const float* viewmatrix, uint8_t* present, void* stream)
+{
+    if (P < 0) return fail(GSPARC_ERR_ARG, "negative size");
+    if (P == 0) return GSPARC_OK;
+    if (!means || !viewmatrix || !present) return fail(GSPARC_ERR_ARG, "null pointer");
+    hipLaunchKernelGGL(k_mark_visible, dim3((P + 255) / 256), dim3(256), 0, (hipStream_t)stream, P, means,
+                       viewmatrix, present);
+    GS_HIP(hipGetLastError());
+    return GSPARC_OK;
+}
+
+int gsparc_ssim_forward(int planes, int H, int W, float C1, float C2, const float* img1, const float* img2,
+                        float* ssim_map, float* dm_dmu1, float* dm_dsigma1_sq, float* dm_dsigma12, void* stream)
+{
+    if (planes < 0 || H < 0 || W < 0) return fail(GSPARC_ERR_ARG, "negative size");
+    if (planes > 65535) return fail(GSPARC_ERR_ARG, "at most 65535 planes per call");
+    if ((size_t)planes * H * W == 0) return GSPARC_OK;
+    if (!img1 || !img2 || !ssim_map) return fail(GSPARC_ERR_ARG, "null pointer");
+    if (dm_dmu1 && (!dm_dsigma1_sq || !dm_dsigma12)) return fail(GSPARC_ERR_ARG, "partial derivative maps");
+    gs::SsimArgs a{planes, H, W, C1, C2, img1, img2, ssim_map, dm_dmu1, dm_dsigma1_sq, dm_dsigma12};
+    GS_HIP(gs::launch_ssim_fwd(a, (hipStream_t)stream));
+    return GSPARC_OK;
+}
+
+int gsparc_ssim_backward(int planes, int H, int W, float C1, float C2, const float* img1, const float* img2,
+                         const float* dL_dmap, const float* dm_dmu1, const float* dm_dsigma1_sq,
+                         const float* dm_dsigma12, float* dL_dimg1, void* stream)
+{
+    (void)C1; (void)C2;
+    if (planes < 0 || H < 0 || W < 0) return fail(GSPARC_ERR_ARG, "negative size");
+    if (planes > 65535) return fail(GSPARC_ERR_ARG, "at most 65535 planes per call");
+    if ((size_t)planes * H * W == 0) return GSPARC_OK;
+    if (!img1 || !img2 || !dL_dmap || !dm_dmu1 || !dm_dsigma1_sq || !dm_dsigma12 || !dL_dimg1)
+        return fail(GSPARC_ERR_ARG, "null pointer");
+    gs::SsimBwdArgs a{planes, H, W, img1, img2, dL_dmap, dm_dmu1, dm_dsigma1_sq, dm_dsigma12, dL_dimg1};
+    GS_HIP(gs::launch_ssim_bwd(a, (hipStream_t)stream));
+    return GSPARC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,404 @@
+// Binning for R receivers: per-tile instance counts -> tile ranges, then the
+// (tile, depth) ordering of every tile list.  Together these replace
+// duplicateWithKeys, the 40-bit radix sort and identifyTileRanges
+// (rasterizer_impl.cu:102-171, 270-312) and produce the same ordering:
+// within a tile, Gaussians by ascending float depth bits, ties by index.
+//
+//   k_colscan   exclusive prefix over preprocess blocks of the per-tile counts
+//   k_segscan   exclusive prefix over (receiver, tile) -> ranges, K
+//   k_scatter   writes (depth bits, index) of every instance into its tile
+//               segment, in Gaussian-index order (wave ballots, no atomics)
+//   k_sort      one workgroup per tile segment: stable LSD radix sort on the
+//               depth bits (4-bit digits, ballot ranking), in LDS when the
+//               segment fits and chunk by chunk in global memory otherwise.
+#include "gs_device.h"
+#include "gs_kernels.h"
+
+namespace gs {
+
+// ---------------------------------------------------------------- scans
+__global__ void __launch_bounds__(256) k_colscan(Dims d, uint32_t* wcnt, uint32_t* segcnt)
+{
+    __shared__ uint32_t s_part[4][GS_WAVE];
+    const int wave = threadIdx.x / GS_WAVE, lane = threadIdx.x % GS_WAVE;
+    const int r = blockIdx.y;
+    const int t = blockIdx.x * GS_WAVE + lane;
+    const bool ok = t < d.T;
+    const int per = (d.NB + 3) / 4;
+    const int b0 = min(d.NB, wave * per), b1 = min(d.NB, b0 + per);
+    uint32_t* col = wcnt + (size_t)r * d.NB * d.T + t;
+    uint32_t sum = 0;
+    if (ok)
+        for (int b = b0; b < b1; ++b) sum += col[(size_t)b * d.T];
+    s_part[wave][lane] = sum;
+    __syncthreads();
+    uint32_t run = 0;
+    for (int w = 0; w < wave; ++w) run += s_part[w][lane];
+    if (ok) {
+        for (int b = b0; b < b1; ++b) {
+            const uint32_t v = col[(size_t)b * d.T];
+            col[(size_t)b * d.T] = run;
+            run += v;
+        }
+        if (wave == 3) segcnt[(size_t)r * d.T + t] = run;
+    }
+}
+
+// Block-wide exclusive scan of one value per thread (1024 threads max).
+template <int NT>
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, uint32_t& total)
+{
+    const int wave = threadIdx.x / GS_WAVE, lane = threadIdx.x % GS_WAVE;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < GS_WAVE; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == GS_WAVE - 1) s_wave[wave] = x;
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NT / GS_WAVE; ++w) {
+        const uint32_t s = s_wave[w];
+        if (w < wave) before += s;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return before + x - v;
+}
+
+__global__ void __launch_bounds__(1024) k_segscan(int n, const uint32_t* segcnt, uint32_t* segbase, uint2* ranges,
+                                                   uint32_t* total)
+{
+    __shared__ uint32_t s_wave[1024 / GS_WAVE];
+    const int per = (n + 1023) / 1024;
+    const int i0 = min(n, (int)threadIdx.x * per), i1 = min(n, i0 + per);
+    uint32_t sum = 0, mx = 0;
+    for (int i = i0; i < i1; ++i) { sum += segcnt[i]; mx = max(mx, segcnt[i]); }
+    uint32_t tot;
+    uint32_t run = block_exclusive_scan<1024>(sum, s_wave, tot);
+    for (int i = i0; i < i1; ++i) {
+        const uint32_t c = segcnt[i];
+        segbase[i] = run;
+        ranges[i] = make_uint2(run, run + c);
+        run += c;
+    }
+    // max segment length
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, off));
+    if ((threadIdx.x % GS_WAVE) == 0) s_wave[threadIdx.x / GS_WAVE] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t m = 0;
+        for (int w = 0; w < 1024 / GS_WAVE; ++w) m = max(m, s_wave[w]);
+        total[0] = tot;
+        total[1] = m;
+    }
+}
+
+// ---------------------------------------------------------------- scatter
+__global__ void __launch_bounds__(GS_PRE_THREADS) k_scatter(Dims d, BinArgs a)
+{
+    __shared__ uint32_t s_cnt[GS_PRE_THREADS / GS_WAVE][GS_WAVE];
+    __shared__ uint32_t s_base[GS_PRE_THREADS / GS_WAVE][GS_WAVE];
+    if ((uint64_t)a.total[0] > a.Kcap) return;    // capacity overflow: reported by the host
+    const int wave = threadIdx.x / GS_WAVE, lane = threadIdx.x % GS_WAVE;
+    const int b = blockIdx.x, r = blockIdx.y;
+    const int g = b * GS_PRE_THREADS + threadIdx.x;
+    const size_t ri = (size_t)r * d.P + g;
+    int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
+    uint32_t dbits = 0;
+    if (g < d.P) {
+        const int radius = a.radii[ri];
+        if (radius > 0) {
+            const float4 s0 = a.splat[2 * ri];
+            get_rect(s0.x, s0.y, radius, d.gx, d.gy, x0, y0, x1, y1);
+            dbits = gs_as_uint(a.depth[ri]);
+        }
+    }
+    const bool has = x1 > x0 && y1 > y0;
+    const uint64_t any = ballot(has);
+    int bx0 = has ? x0 : 0x7fffffff, by0 = has ? y0 : 0x7fffffff, bx1 = has ? x1 : 0, by1 = has ? y1 : 0;
+    for (int off = 32; off >= 1; off >>= 1) {
+        bx0 = min(bx0, __shfl_xor(bx0, off));
+        by0 = min(by0, __shfl_xor(by0, off));
+        bx1 = max(bx1, __shfl_xor(bx1, off));
+        by1 = max(by1, __shfl_xor(by1, off));
+    }
+    const uint64_t lt = lanemask_lt();
+    for (int c0 = 0; c0 < d.T; c0 += GS_WAVE) {
+        const int cend = min(d.T, c0 + GS_WAVE);
+        uint32_t mine = 0;
+        if (any) {
+            for (int t = c0; t < cend; ++t) {
+                const int tx = t % d.gx, ty = t / d.gx;
+                if (tx < bx0 || tx >= bx1 || ty < by0 || ty >= by1) continue;
+                const uint64_t m = ballot(tx >= x0 && tx < x1 && ty >= y0 && ty < y1);
+                if (lane == t - c0) mine = (uint32_t)__popcll(m);
+            }
+        }
+        s_cnt[wave][lane] = mine;
+        __syncthreads();
+        {
+            const int t = c0 + lane;
+            if (t < d.T) {
+                uint32_t base = a.segbase[(size_t)r * d.T + t] + a.wcnt[((size_t)r * d.NB + b) * d.T + t];
+                for (int w = 0; w < wave; ++w) base += s_cnt[w][lane];
+                s_base[wave][lane] = base;
+            }
+        }
+        __syncthreads();
+        if (any) {
+            for (int t = c0; t < cend; ++t) {
+                const int tx = t % d.gx, ty = t / d.gx;
+                if (tx < bx0 || tx >= bx1 || ty < by0 || ty >= by1) continue;
+                const bool in = tx >= x0 && tx < x1 && ty >= y0 && ty < y1;
+                const uint64_t m = ballot(in);
+                if (in) {
+                    const uint32_t pos = s_base[wave][t - c0] + (uint32_t)__popcll(m & lt);
+                    a.keys[pos] = dbits;
+                    a.vals[pos] = (uint32_t)g;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- sort
+#define GS_SORT_THREADS 512
+#define GS_SORT_WAVES (GS_SORT_THREADS / GS_WAVE)
+#define GS_SORT_ROWS 16
+#define GS_SORT_CHUNK (GS_SORT_THREADS * GS_SORT_ROWS)   // 8192 instances
+#define GS_SORT_TABLE (16 * GS_SORT_ROWS * GS_SORT_WAVES)
+
+struct SortShared {
+    uint32_t k[2][GS_SORT_CHUNK];
+    uint32_t v[2][GS_SORT_CHUNK];
+    uint32_t table[GS_SORT_TABLE];
+    uint32_t hist[16];
+    uint32_t base[16];
+    uint32_t wscan[GS_SORT_WAVES];
+    uint32_t red[2][GS_SORT_WAVES];
+};
+
+// Ballot-based stable ranking of 4-bit digits for one row of 512 elements.
+__device__ __forceinline__ uint64_t digit_match(bool valid, uint32_t dgt)
+{
+    uint64_t m = ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 4; ++bit) {
+        const bool set = (dgt >> bit) & 1u;
+        const uint64_t B = ballot(valid && set);
+        m &= set ? B : ~B;
+    }
+    return m;
+}
+
+// One stable LSD pass over n elements (any n), chunk by chunk.
+template <typename KP, typename VP, typename DKP, typename DVP>
+__device__ void radix_pass(SortShared& sm, int n, KP sk, VP sv, DKP dk, DVP dv, uint32_t kmin, int shift)
+{
+    const int tid = threadIdx.x, wave = tid / GS_WAVE;
+    const uint64_t lt = lanemask_lt();
+    const int nchunks = (n + GS_SORT_CHUNK - 1) / GS_SORT_CHUNK;
+    if (nchunks > 1) {
+        // digit histogram of the whole segment
+        if (tid < 16) sm.hist[tid] = 0;
+        __syncthreads();
+        uint32_t pk[4] = {0, 0, 0, 0};   // 16 packed 8-bit counters... flushed every 255
+        int cnt = 0;
+        for (int e = tid; e < n; e += GS_SORT_THREADS) {
+            const uint32_t dg = ((sk[e] - kmin) >> shift) & 15u;
+            pk[dg >> 2] += 1u << (8 * (dg & 3));
+            if (++cnt == 255) {
+                for (int q = 0; q < 16; ++q) {
+                    const uint32_t c = (pk[q >> 2] >> (8 * (q & 3))) & 255u;
+                    if (c) atomicAdd(&sm.hist[q], c);
+                }
+                pk[0] = pk[1] = pk[2] = pk[3] = 0;
+                cnt = 0;
+            }
+        }
+        for (int q = 0; q < 16; ++q) {
+            const uint32_t c = (pk[q >> 2] >> (8 * (q & 3))) & 255u;
+            if (c) atomicAdd(&sm.hist[q], c);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t run = 0;
+            for (int q = 0; q < 16; ++q) { sm.base[q] = run; run += sm.hist[q]; }
+        }
+        __syncthreads();
+    }
+    for (int ch = 0; ch < nchunks; ++ch) {
+        const int e0 = ch * GS_SORT_CHUNK;
+        const int cn = min(GS_SORT_CHUNK, n - e0);
+        const int rows = (cn + GS_SORT_THREADS - 1) / GS_SORT_THREADS;
+        const int S = 16 * rows * GS_SORT_WAVES;
+        for (int i = tid; i < S; i += GS_SORT_THREADS) sm.table[i] = 0;
+        __syncthreads();
+        for (int i = 0; i < rows; ++i) {
+            const int e = i * GS_SORT_THREADS + tid;
+            const bool valid = e < cn;
+            const uint32_t dg = valid ? ((sk[e0 + e] - kmin) >> shift) & 15u : 0u;
+            const uint64_t m = digit_match(valid, dg);
+            if (valid && (m & lt) == 0)
+                sm.table[(dg * rows + i) * GS_SORT_WAVES + wave] = (uint32_t)__popcll(m);
+        }
+        __syncthreads();
+        // exclusive scan of the table in (digit, row, wave) order
+        {
+            const int per = (S + GS_SORT_THREADS - 1) / GS_SORT_THREADS;
+            const int i0 = min(S, tid * per), i1 = min(S, i0 + per);
+            uint32_t s = 0;
+            for (int i = i0; i < i1; ++i) s += sm.table[i];
+            uint32_t tot;
+            uint32_t run = block_exclusive_scan<GS_SORT_THREADS>(s, sm.wscan, tot);
+            for (int i = i0; i < i1; ++i) {
+                const uint32_t c = sm.table[i];
+                sm.table[i] = run;
+                run += c;
+            }
+        }
+        __syncthreads();
+        // per-digit start inside this chunk: P(d,0,0)
+        for (int i = 0; i < rows; ++i) {
+            const int e = i * GS_SORT_THREADS + tid;
+            const bool valid = e < cn;
+            uint32_t key = 0;
+            uint32_t dg = 0;
+            if (valid) {
+                key = sk[e0 + e];
+                dg = ((key - kmin) >> shift) & 15u;
+            }
+            const uint64_t m = digit_match(valid, dg);
+            if (valid) {
+                uint32_t pos = sm.table[(dg * rows + i) * GS_SORT_WAVES + wave] + (uint32_t)__popcll(m & lt);
+                if (nchunks > 1) pos = pos - sm.table[dg * rows * GS_SORT_WAVES] + sm.base[dg];
+                dk[pos] = key;
+                dv[pos] = sv[e0 + e];
+            }
+        }
+        __syncthreads();
+        if (nchunks > 1) {
+            if (tid < 16) {
+                const uint32_t start = sm.table[tid * rows * GS_SORT_WAVES];
+                const uint32_t end = (tid < 15) ? sm.table[(tid + 1) * rows * GS_SORT_WAVES] : (uint32_t)cn;
+                sm.base[tid] += end - start;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__global__ void __launch_bounds__(GS_SORT_THREADS) k_sort(BinArgs a, int nseg)
+{
+    extern __shared__ unsigned char s_raw[];
+    SortShared& sm = *reinterpret_cast<SortShared*>(s_raw);
+    if ((uint64_t)a.total[0] > a.Kcap) return;
+    const int seg = blockIdx.x;
+    if (seg >= nseg) return;
+    const int tid = threadIdx.x;
+    const int n = (int)a.ranges[seg].y - (int)a.ranges[seg].x;
+    const uint32_t base = a.ranges[seg].x;
+    if (n <= 0) return;
+    uint32_t* gk = a.keys + base;
+    uint32_t* gv = a.vals + base;
+    // key range of the segment
+    uint32_t kmin = 0xffffffffu, kmax = 0;
+    for (int e = tid; e < n; e += GS_SORT_THREADS) {
+        const uint32_t k = gk[e];
+        kmin = min(kmin, k);
+        kmax = max(kmax, k);
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        kmin = min(kmin, (uint32_t)__shfl_xor(kmin, off));
+        kmax = max(kmax, (uint32_t)__shfl_xor(kmax, off));
+    }
+    if ((tid % GS_WAVE) == 0) { sm.red[0][tid / GS_WAVE] = kmin; sm.red[1][tid / GS_WAVE] = kmax; }
+    __syncthreads();
+    kmin = sm.red[0][0];
+    kmax = sm.red[1][0];
+    for (int w = 1; w < GS_SORT_WAVES; ++w) { kmin = min(kmin, sm.red[0][w]); kmax = max(kmax, sm.red[1][w]); }
+    const uint32_t span = kmax - kmin;
+    const int nbits = span ? 32 - __clz((int)span) : 0;
+    const int passes = (nbits + 3) / 4;
+    __syncthreads();
+
+    if (n <= GS_SORT_CHUNK) {
+        for (int e = tid; e < n; e += GS_SORT_THREADS) { sm.k[0][e] = gk[e]; sm.v[0][e] = gv[e]; }
+        __syncthreads();
+        int cur = 0;
+        for (int p = 0; p < passes; ++p) {
+            radix_pass(sm, n, sm.k[cur], sm.v[cur], sm.k[cur ^ 1], sm.v[cur ^ 1], kmin, 4 * p);
+            cur ^= 1;
+        }
+        for (int e = tid; e < n; e += GS_SORT_THREADS) {
+            a.plist[base + e] = sm.v[cur][e];
+            if (a.keys_sorted) a.keys_sorted[base + e] = sm.k[cur][e];
+        }
+    } else {
+        uint32_t* tk = a.keys_tmp + base;
+        uint32_t* tv = a.vals_tmp + base;
+        int cur = 0;
+        for (int p = 0; p < passes; ++p) {
+            if (cur == 0) radix_pass(sm, n, gk, gv, tk, tv, kmin, 4 * p);
+            else radix_pass(sm, n, tk, tv, gk, gv, kmin, 4 * p);
+            cur ^= 1;
+        }
+        const uint32_t* fk = cur ? tk : gk;
+        const uint32_t* fv = cur ? tv : gv;
+        for (int e = tid; e < n; e += GS_SORT_THREADS) {
+            a.plist[base + e] = fv[e];
+            if (a.keys_sorted) a.keys_sorted[base + e] = fk[e];
+        }
+    }
+}
+
+hipError_t launch_bin_scan(const Dims& d, const BinArgs& a, hipStream_t s)
+{
+    const int nseg = d.R * d.T;
+    if (d.NB > 0) {
+        hipLaunchKernelGGL(k_colscan, dim3((d.T + GS_WAVE - 1) / GS_WAVE, d.R), dim3(256), 0, s, d, a.wcnt, a.segcnt);
+    } else {
+        hipError_t e = launch_fill_u32(a.segcnt, 0u, (size_t)nseg, s);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_segscan, dim3(1), dim3(1024), 0, s, nseg, a.segcnt, a.segbase, a.ranges, a.total);
+    return hipGetLastError();
+}
+
+hipError_t launch_bin_scatter_sort(const Dims& d, const BinArgs& a, hipStream_t s)
+{
+    if (d.NB > 0 && d.R > 0)
+        hipLaunchKernelGGL(k_scatter, dim3(d.NB, d.R), dim3(GS_PRE_THREADS), 0, s, d, a);
+    const int nseg = d.R * d.T;
+    if (nseg > 0) {
+        // the sort stages up to GS_SORT_CHUNK instances in LDS (about 136 KiB of
+        // the 160 KiB a gfx950 workgroup can hold)
+        static hipError_t attr = hipFuncSetAttribute((const void*)k_sort, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                     (int)sizeof(SortShared));
+        if (attr != hipSuccess) return attr;
+        hipLaunchKernelGGL(k_sort, dim3(nseg), dim3(GS_SORT_THREADS), sizeof(SortShared), s, a, nseg);
+    }
+    return hipGetLastError();
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+}  // namespace gs

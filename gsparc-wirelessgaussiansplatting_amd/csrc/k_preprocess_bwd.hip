@@ -1,0 +1,356 @@
+// Preprocess backward for R receivers (computeCov2DCUDA backward.cu:124-303,
+// preprocessCUDA backward.cu:378-437, computeCov3D backward.cu:308-371,
+// computeSignalFromMLP backward backward.cu:9-118), fused in one kernel.
+//
+// One thread per Gaussian walks all receivers: the geometry adjoint of every
+// receiver is summed in registers (no atomics, deterministic order), the MLP
+// output adjoints dz[r] are parked in LDS, and the 194 MLP weight gradients
+// are then formed hidden unit by hidden unit over all receivers.  The cov3D
+// adjoint is linear in dL/dcov3D, so it runs once on the receiver sum.
+//
+// Modes: GRAD_EXACT is the true adjoint of the forward; GRAD_REFERENCE
+// reproduces the reference's backward literally (halved x-row of the
+// Jacobian, 1/(det^2 + 1e-7), 1/(d+eps)^2 in the MLP distance gradient,
+// scale modifier dropped from dL/dscale).
+#include "gs_device.h"
+#include "gs_kernels.h"
+
+namespace gs {
+
+#define GS_BWD_THREADS 64
+#define GS_BRC 8
+
+template <int MODE>
+__device__ __forceinline__ void geo_bwd(const float* mean, const float* c3, const float* view, int W, int H,
+                                        const float* g, float* dmean, float* dc)
+{
+    float t[3];
+    view_xform(mean[0], mean[1], mean[2], view, t);
+    const float rr = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+    GeoOut go;
+    const float rd = sqrtf(rr);
+    project(t, rd, W, H, go);
+    const float dpx = g[0], dpy = g[1], gqa = g[2], gqb = g[3], gqc = g[4];
+    float Jc[2][3];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) Jc[a][k] = go.J[a][k];
+    if (MODE == GRAD_REFERENCE) {   // backward.cu:172-173
+        Jc[0][0] *= 0.5f;
+        Jc[0][2] *= 0.5f;
+    }
+    float m[2][3], A, B, C;
+    cov2d(Jc, view, c3, m, A, B, C);
+    const float det = A * C - B * B;
+    float dA, dB, dC;
+    if (MODE == GRAD_REFERENCE) {   // backward.cu:212-224
+        const float d2i = 1.0f / (det * det + 1e-7f);
+        const float gy = 0.5f * gqb;
+        dA = d2i * (-C * C * gqa + 2.0f * B * C * gy + (det - A * C) * gqc);
+        dC = d2i * (-A * A * gqc + 2.0f * A * B * gy + (det - A * C) * gqa);
+        dB = d2i * 2.0f * (B * C * gqa - (det + 2.0f * B * B) * gy + A * B * gqc);
+    } else {
+        const float d2i = 1.0f / (det * det);
+        dA = d2i * (-C * C * gqa + B * C * gqb - B * B * gqc);
+        dB = d2i * (2.0f * B * C * gqa - (det + 2.0f * B * B) * gqb + 2.0f * A * B * gqc);
+        dC = d2i * (-B * B * gqa + A * B * gqb - A * A * gqc);
+    }
+    const int ij[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+#pragma unroll
+    for (int e = 0; e < 6; ++e) {
+        const int i = ij[e][0], j = ij[e][1];
+        if (i == j)
+            dc[e] += m[0][i] * m[0][i] * dA + m[0][i] * m[1][i] * dB + m[1][i] * m[1][i] * dC;
+        else
+            dc[e] += 2.0f * m[0][i] * m[0][j] * dA + (m[0][i] * m[1][j] + m[0][j] * m[1][i]) * dB +
+                     2.0f * m[1][i] * m[1][j] * dC;
+    }
+    const float S[3][3] = {{c3[0], c3[1], c3[2]}, {c3[1], c3[3], c3[4]}, {c3[2], c3[4], c3[5]}};
+    float Sm0[3], Sm1[3], dm0[3], dm1[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        Sm0[i] = S[i][0] * m[0][0] + S[i][1] * m[0][1] + S[i][2] * m[0][2];
+        Sm1[i] = S[i][0] * m[1][0] + S[i][1] * m[1][1] + S[i][2] * m[1][2];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        dm0[i] = 2.0f * dA * Sm0[i] + dB * Sm1[i];
+        dm1[i] = 2.0f * dC * Sm1[i] + dB * Sm0[i];
+    }
+    float dJ[2][3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        dJ[0][q] = dm0[0] * view[0 * 4 + q] + dm0[1] * view[1 * 4 + q] + dm0[2] * view[2 * 4 + q];
+        dJ[1][q] = dm1[0] * view[0 * 4 + q] + dm1[1] * view[1 * 4 + q] + dm1[2] * view[2 * 4 + q];
+    }
+    const float Wd = (float)W * 0.5f * GS_ONE_OVER_PI_F;
+    const float Hd = (float)H * GS_ONE_OVER_PI_F;
+    const float tx = t[0], ty = t[1], tz = t[2];
+    float dt[3] = {0.0f, 0.0f, 0.0f};
+    if (MODE == GRAD_REFERENCE) {
+        // backward.cu:152-177, 249-292
+        const float txtx = tx * tx, tyty = ty * ty, tztz = tz * tz, txtytz = tx * ty * tz;
+        const float rxz2 = txtx + tztz;
+        const float rxz2_inv = 1.0f / (rxz2 + 1e-7f);
+        const float rxz4_inv = rxz2_inv * rxz2_inv;
+        const float rxz = sqrtf(rxz2);
+        const float rxz_inv = 1.0f / (rxz + 1e-7f);
+        const float r2 = rxz2 + tyty;
+        const float r2_inv = 1.0f / (r2 + 1e-7f);
+        const float r4_inv = r2_inv * r2_inv;
+        const float rxz_r4_inv = rxz_inv * r4_inv;
+        const float rxz3_r4_inv = rxz2_inv * rxz_r4_inv;
+        const float temp1 = Hd * (tyty - rxz2) * rxz_r4_inv;
+        const float temp2 = Hd * txtytz * (r2 + 2.0f * rxz2) * rxz3_r4_inv;
+        const float temp3 = Wd * (txtx - tztz) * rxz4_inv;
+        const float temp4 = Wd * 2.0f * tx * tz * rxz4_inv;
+        const float temp5 = Hd * ty * rxz3_r4_inv;
+        dt[0] = -dJ[0][0] * temp4 + dJ[0][2] * temp3 + dJ[1][0] * temp5 * (2.0f * txtx * rxz2 - tztz * r2) +
+                dJ[1][1] * tx * temp1 + dJ[1][2] * temp2;
+        dt[1] = dJ[1][0] * tx * temp1 - dJ[1][1] * Hd * 2.0f * rxz * ty * r4_inv + dJ[1][2] * tz * temp1;
+        dt[2] = dJ[0][0] * temp3 + dJ[0][2] * temp4 + dJ[1][0] * temp2 + dJ[1][1] * tz * temp1 +
+                dJ[1][2] * temp5 * (2.0f * tztz * rxz2 - txtx * r2);
+        dt[0] += dpx * Jc[0][0] + dpy * Jc[1][0];
+        dt[1] += dpx * Jc[0][1] + dpy * Jc[1][1];
+        dt[2] += dpx * Jc[0][2] + dpy * Jc[1][2];
+    } else {
+        const float eps = 1e-7f;
+        const float rxz2 = tx * tx + tz * tz;
+        const float ia = 1.0f / (rxz2 + eps);
+        const float rho = sqrtf(rxz2);
+        const float ib = 1.0f / (rho + eps);
+        const float r2 = rxz2 + ty * ty;
+        const float ic = 1.0f / (r2 + eps);
+        const float g0x = dJ[0][0], g0z = dJ[0][2], h0 = dJ[1][0], h1 = dJ[1][1], h2 = dJ[1][2];
+        dt[0] += g0x * Wd * tz * (-2.0f * tx * ia * ia) + g0z * (-Wd) * (ia - 2.0f * tx * tx * ia * ia);
+        dt[2] += g0x * Wd * (ia - 2.0f * tz * tz * ia * ia) + g0z * (-Wd) * tx * (-2.0f * tz * ia * ia);
+        const float Kc = ib * ic;
+        const float drho_x = rho > 0.0f ? tx / rho : 0.0f, drho_z = rho > 0.0f ? tz / rho : 0.0f;
+        const float dK0 = -ic * ib * ib * drho_x - 2.0f * ib * ic * ic * tx;
+        const float dK1 = -2.0f * ib * ic * ic * ty;
+        const float dK2 = -ic * ib * ib * drho_z - 2.0f * ib * ic * ic * tz;
+        dt[0] += h0 * (-Hd) * (ty * Kc + tx * ty * dK0);
+        dt[1] += h0 * (-Hd) * (tx * Kc + tx * ty * dK1);
+        dt[2] += h0 * (-Hd) * (tx * ty * dK2);
+        dt[0] += h2 * (-Hd) * (tz * ty * dK0);
+        dt[1] += h2 * (-Hd) * (tz * Kc + tz * ty * dK1);
+        dt[2] += h2 * (-Hd) * (ty * Kc + tz * ty * dK2);
+        dt[0] += h1 * Hd * (drho_x * ic - 2.0f * rho * tx * ic * ic);
+        dt[1] += h1 * Hd * (-2.0f * rho * ty * ic * ic);
+        dt[2] += h1 * Hd * (drho_z * ic - 2.0f * rho * tz * ic * ic);
+        const float Wdp = (float)W * 0.5f * GS_ONE_OVER_PI_F;
+        const float Hdp = (float)H * 0.5f * GS_TWO_OVER_PI_F;
+        if (rxz2 > 0.0f) {
+            dt[0] += dpx * Wdp * tz / rxz2;
+            dt[2] += dpx * Wdp * (-tx) / rxz2;
+        }
+        const float re = rd + eps;
+        const float u = ty / re;
+        const float dlat_du = 1.0f / sqrtf(1.0f - u * u);
+        const float k2 = ty / (rd * re * re);
+        dt[0] += dpy * Hdp * dlat_du * (-k2 * tx);
+        dt[1] += dpy * Hdp * dlat_du * (1.0f / re - k2 * ty);
+        dt[2] += dpy * Hdp * dlat_du * (-k2 * tz);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) dmean[k] += view[k * 4 + 0] * dt[0] + view[k * 4 + 1] * dt[1] + view[k * 4 + 2] * dt[2];
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(GS_BWD_THREADS) k_preprocess_bwd(Dims d, PreBwdArgs a)
+{
+    extern __shared__ float2 s_dz[];   // [R][GS_BWD_THREADS]
+    const int g = blockIdx.x * GS_BWD_THREADS + threadIdx.x;
+    if (g >= d.P) return;   // no block-level synchronisation below
+    const int lane = threadIdx.x;
+    const int P = d.P;
+    const float mean[3] = {a.means[3 * g], a.means[3 * g + 1], a.means[3 * g + 2]};
+    const float s[3] = {a.scales[3 * g], a.scales[3 * g + 1], a.scales[3 * g + 2]};
+    const float q[4] = {a.rots[4 * g], a.rots[4 * g + 1], a.rots[4 * g + 2], a.rots[4 * g + 3]};
+    float c3[6];
+    cov3d(s, a.mod, q, c3);
+    const float diff[3] = {a.tx[0] - mean[0], a.tx[1] - mean[1], a.tx[2] - mean[2]};
+    const float dist = sqrtf(diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2]);
+    const float ld = gs_logf(dist + 1e-6f);
+    const float de = dist + 1e-6f;
+    const float den = (MODE == GRAD_REFERENCE) ? de * de : dist * de;
+
+    float dmean[3] = {0, 0, 0}, dc[6] = {0, 0, 0, 0, 0, 0}, dop = 0.0f, db2_0 = 0.0f, db2_1 = 0.0f;
+    for (int r0 = 0; r0 < d.R; r0 += GS_BRC) {
+        const int nr = min(GS_BRC, d.R - r0);
+        float z0[GS_BRC], z1[GS_BRC];
+        if (a.mlp) {
+            const float* W1 = a.mlp + (size_t)g * 96;
+            const float* b1 = a.mlp + (size_t)P * 96 + (size_t)g * 32;
+            const float* W2 = a.mlp + (size_t)P * 128 + (size_t)g * 64;
+            const float* b2 = a.mlp + (size_t)P * 192 + (size_t)g * 2;
+#pragma unroll
+            for (int c = 0; c < GS_BRC; ++c) { z0[c] = b2[0]; z1[c] = b2[1]; }
+            for (int j = 0; j < 32; ++j) {
+                const float w0 = W1[j * 3], w1 = W1[j * 3 + 1], w2 = W1[j * 3 + 2], bb = b1[j];
+                const float v0 = W2[j], v1 = W2[32 + j];
+#pragma unroll
+                for (int c = 0; c < GS_BRC; ++c) {
+                    if (c < nr) {
+                        const float* rx = a.rx + 3 * (r0 + c);
+                        float hs = bb;
+                        hs = hs + w0 * rx[0];
+                        hs = hs + w1 * rx[1];
+                        hs = hs + w2 * rx[2];
+                        const float h = (hs > 0.0f) ? hs : 0.01f * hs;
+                        z0[c] = z0[c] + v0 * h;
+                        z1[c] = z1[c] + v1 * h;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < GS_BRC; ++c) {
+            if (c >= nr) break;
+            const int r = r0 + c;
+            const size_t ri = (size_t)r * P + g;
+            float2 dz = make_float2(0.0f, 0.0f);
+            if (a.radii[ri] > 0) {
+                const float4 ga = reinterpret_cast<const float4*>(a.g8)[2 * ri];
+                const float4 gb = reinterpret_cast<const float4*>(a.g8)[2 * ri + 1];
+                const float gv[8] = {ga.x, ga.y, ga.z, ga.w, gb.x, gb.y, gb.z, gb.w};
+                geo_bwd<MODE>(mean, c3, a.views + 16 * r, d.W, d.H, gv, dmean, dc);
+                dop += gv[5];
+                if (a.dmeans2d) {
+                    a.dmeans2d[3 * ri + 0] = gv[0] * (0.5f * (float)d.W);
+                    a.dmeans2d[3 * ri + 1] = gv[1] * (0.5f * (float)d.H);
+                    a.dmeans2d[3 * ri + 2] = 0.0f;
+                }
+                if (a.mlp) {
+                    const float s0 = 1.0f / (1.0f + gs_expf(-(z0[c] - ld)));
+                    const float s1 = 1.0f / (1.0f + gs_expf(-(z1[c] - ld)));
+                    dz.x = gv[6] * s0 * (1.0f - s0);
+                    dz.y = gv[7] * s1 * (1.0f - s1);
+                    const float dld = -(dz.x + dz.y);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) dmean[k] += -dld * diff[k] / den;
+                    db2_0 += dz.x;
+                    db2_1 += dz.y;
+                } else if (a.dsig) {
+                    a.dsig[2 * ri] = gv[6];
+                    a.dsig[2 * ri + 1] = gv[7];
+                }
+            } else {
+                if (a.dmeans2d) {
+                    a.dmeans2d[3 * ri + 0] = 0.0f;
+                    a.dmeans2d[3 * ri + 1] = 0.0f;
+                    a.dmeans2d[3 * ri + 2] = 0.0f;
+                }
+                if (!a.mlp && a.dsig) {
+                    a.dsig[2 * ri] = 0.0f;
+                    a.dsig[2 * ri + 1] = 0.0f;
+                }
+            }
+            if (a.mlp) s_dz[(size_t)r * GS_BWD_THREADS + lane] = dz;
+        }
+    }
+
+    // MLP weight gradients, hidden unit by hidden unit (backward.cu:95-115)
+    if (a.mlp && a.dmlp) {
+        const float* W1 = a.mlp + (size_t)g * 96;
+        const float* b1 = a.mlp + (size_t)P * 96 + (size_t)g * 32;
+        const float* W2 = a.mlp + (size_t)P * 128 + (size_t)g * 64;
+        for (int j = 0; j < 32; ++j) {
+            const float w0 = W1[j * 3], w1 = W1[j * 3 + 1], w2 = W1[j * 3 + 2], bb = b1[j];
+            const float v0 = W2[j], v1 = W2[32 + j];
+            float gw0 = 0.0f, gw1 = 0.0f, gw2 = 0.0f, gb = 0.0f, gv0 = 0.0f, gv1 = 0.0f;
+            for (int r = 0; r < d.R; ++r) {
+                const float2 dz = s_dz[(size_t)r * GS_BWD_THREADS + lane];
+                const float* rx = a.rx + 3 * r;
+                float hs = bb;
+                hs = hs + w0 * rx[0];
+                hs = hs + w1 * rx[1];
+                hs = hs + w2 * rx[2];
+                const float h = (hs > 0.0f) ? hs : 0.01f * hs;
+                gv0 += dz.x * h;
+                gv1 += dz.y * h;
+                const float dh = dz.x * v0 + dz.y * v1;
+                const float dp = (hs > 0.0f) ? dh : 0.01f * dh;
+                gw0 += dp * rx[0];
+                gw1 += dp * rx[1];
+                gw2 += dp * rx[2];
+                gb += dp;
+            }
+            a.dmlp[(size_t)g * 96 + j * 3 + 0] = gw0;
+            a.dmlp[(size_t)g * 96 + j * 3 + 1] = gw1;
+            a.dmlp[(size_t)g * 96 + j * 3 + 2] = gw2;
+            a.dmlp[(size_t)P * 96 + (size_t)g * 32 + j] = gb;
+            a.dmlp[(size_t)P * 128 + (size_t)g * 64 + j] = gv0;
+            a.dmlp[(size_t)P * 128 + (size_t)g * 64 + 32 + j] = gv1;
+        }
+        a.dmlp[(size_t)P * 192 + (size_t)g * 2] = db2_0;
+        a.dmlp[(size_t)P * 192 + (size_t)g * 2 + 1] = db2_1;
+    }
+
+    // cov3D adjoint -> scale, rotation (backward.cu:308-371)
+    {
+        const float r = q[0], x = q[1], y = q[2], z = q[3];
+        const float Rq[3][3] = {
+            {1.0f - 2.0f * (y * y + z * z), 2.0f * (x * y - r * z), 2.0f * (x * z + r * y)},
+            {2.0f * (x * y + r * z), 1.0f - 2.0f * (x * x + z * z), 2.0f * (y * z - r * x)},
+            {2.0f * (x * z - r * y), 2.0f * (y * z + r * x), 1.0f - 2.0f * (x * x + y * y)}};
+        const float se[3] = {a.mod * s[0], a.mod * s[1], a.mod * s[2]};
+        const float Dg[3] = {se[0] * se[0], se[1] * se[1], se[2] * se[2]};
+        const float G[3][3] = {{dc[0], 0.5f * dc[1], 0.5f * dc[2]},
+                               {0.5f * dc[1], dc[3], 0.5f * dc[4]},
+                               {0.5f * dc[2], 0.5f * dc[4], dc[5]}};
+        float GR[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                GR[i][k] = 2.0f * (G[i][0] * Rq[0][k] + G[i][1] * Rq[1][k] + G[i][2] * Rq[2][k]) * Dg[k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            float dD = 0.0f;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) dD += Rq[i][k] * G[i][j] * Rq[j][k];
+            const float f = (MODE == GRAD_REFERENCE) ? 2.0f * se[k] : 2.0f * se[k] * a.mod;
+            a.dscales[3 * g + k] = dD * f;
+        }
+        const float dRr[3][3] = {{0, -2 * z, 2 * y}, {2 * z, 0, -2 * x}, {-2 * y, 2 * x, 0}};
+        const float dRx[3][3] = {{0, 2 * y, 2 * z}, {2 * y, -4 * x, -2 * r}, {2 * z, 2 * r, -4 * x}};
+        const float dRy[3][3] = {{-4 * y, 2 * x, 2 * r}, {2 * x, 0, 2 * z}, {-2 * r, 2 * z, -4 * y}};
+        const float dRz[3][3] = {{-4 * z, -2 * r, 2 * x}, {2 * r, -4 * z, 2 * y}, {2 * x, 2 * y, 0}};
+        float dq[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                dq[0] += GR[i][j] * dRr[i][j];
+                dq[1] += GR[i][j] * dRx[i][j];
+                dq[2] += GR[i][j] * dRy[i][j];
+                dq[3] += GR[i][j] * dRz[i][j];
+            }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) a.drots[4 * g + k] = dq[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) a.dmeans[3 * g + k] = dmean[k];
+    a.dopac[g] = dop;
+    if (a.dcov3)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) a.dcov3[6 * g + k] = dc[k];
+}
+
+hipError_t launch_preprocess_bwd(const Dims& d, const PreBwdArgs& a, hipStream_t s)
+{
+    if (d.P == 0) return hipSuccess;
+    const int blocks = (d.P + GS_BWD_THREADS - 1) / GS_BWD_THREADS;
+    const size_t lds = a.mlp ? (size_t)d.R * GS_BWD_THREADS * sizeof(float2) : 0;
+    if (a.mode == GRAD_REFERENCE)
+        hipLaunchKernelGGL(k_preprocess_bwd<GRAD_REFERENCE>, dim3(blocks), dim3(GS_BWD_THREADS), lds, s, d, a);
+    else
+        hipLaunchKernelGGL(k_preprocess_bwd<GRAD_EXACT>, dim3(blocks), dim3(GS_BWD_THREADS), lds, s, d, a);
+    return hipGetLastError();
+}
+
+}  // namespace gs

@@ -1,0 +1,162 @@
+// Forward preprocess for R receivers at once (forward.cu:252-327 per receiver).
+//
+// One thread per Gaussian.  Receiver-independent work (cov3D, the log of the
+// transmitter distance, opacity threshold) is done once; the emission MLP is
+// evaluated for chunks of GS_RC receivers so that each Gaussian's 194 weights
+// are streamed once per chunk instead of once per receiver.  For every
+// receiver the block then counts, per tile, how many of its 256 Gaussians
+// touch the tile (wave ballots, so the later scatter can place instances in
+// Gaussian-index order without atomics).
+#include "gs_device.h"
+#include "gs_kernels.h"
+
+namespace gs {
+
+#define GS_RC 8
+
+__global__ void __launch_bounds__(GS_PRE_THREADS)
+k_preprocess_fwd(Dims d, PreFwdArgs a, int NB)
+{
+    __shared__ uint32_t s_cnt[GS_PRE_THREADS / GS_WAVE][GS_WAVE];
+    const int g = blockIdx.x * GS_PRE_THREADS + threadIdx.x;
+    const bool valid = g < d.P;
+    const int wave = threadIdx.x / GS_WAVE, lane = threadIdx.x % GS_WAVE;
+    const int P = d.P;
+
+    float mean[3] = {0, 0, 0}, c3[6] = {0, 0, 0, 0, 0, 0}, o = 0.0f, ld = 0.0f;
+    if (valid) {
+        mean[0] = a.means[3 * g]; mean[1] = a.means[3 * g + 1]; mean[2] = a.means[3 * g + 2];
+        const float s[3] = {a.scales[3 * g], a.scales[3 * g + 1], a.scales[3 * g + 2]};
+        const float q[4] = {a.rots[4 * g], a.rots[4 * g + 1], a.rots[4 * g + 2], a.rots[4 * g + 3]};
+        cov3d(s, a.mod, q, c3);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) a.cov3[6 * g + k] = c3[k];
+        o = a.opac[g];
+        const float df[3] = {a.tx[0] - mean[0], a.tx[1] - mean[1], a.tx[2] - mean[2]};
+        const float dist = sqrtf(df[0] * df[0] + df[1] * df[1] + df[2] * df[2]);
+        ld = gs_logf(dist + 1e-6f);
+    }
+    // alpha = o*G < 1/255 for every power < thr (see gs_device.h)
+    const float thr = -gs_logf(255.0f * o) - 1e-3f;
+
+    for (int r0 = 0; r0 < d.R; r0 += GS_RC) {
+        const int nr = min(GS_RC, d.R - r0);
+        // ---- emission MLP 3->32->2 for receivers r0..r0+nr (forward.cu:17-75)
+        float z0[GS_RC], z1[GS_RC];
+        if (a.mlp && valid) {
+            const float* W1 = a.mlp + (size_t)g * 96;
+            const float* b1 = a.mlp + (size_t)P * 96 + (size_t)g * 32;
+            const float* W2 = a.mlp + (size_t)P * 128 + (size_t)g * 64;
+            const float* b2 = a.mlp + (size_t)P * 192 + (size_t)g * 2;
+            const float b20 = b2[0], b21 = b2[1];
+#pragma unroll
+            for (int c = 0; c < GS_RC; ++c) { z0[c] = b20; z1[c] = b21; }
+            for (int j = 0; j < 32; ++j) {
+                const float w0 = W1[j * 3], w1 = W1[j * 3 + 1], w2 = W1[j * 3 + 2], bb = b1[j];
+                const float v0 = W2[j], v1 = W2[32 + j];
+#pragma unroll
+                for (int c = 0; c < GS_RC; ++c) {
+                    if (c < nr) {
+                        const float* rx = a.rx + 3 * (r0 + c);
+                        float s = bb;
+                        s = s + w0 * rx[0];
+                        s = s + w1 * rx[1];
+                        s = s + w2 * rx[2];
+                        const float h = (s > 0.0f) ? s : 0.01f * s;
+                        z0[c] = z0[c] + v0 * h;
+                        z1[c] = z1[c] + v1 * h;
+                    }
+                }
+            }
+        }
+        for (int c = 0; c < nr; ++c) {
+            const int r = r0 + c;
+            const float* view = a.views + 16 * r;
+            const size_t ri = (size_t)r * P + g;
+            int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
+            if (valid) {
+                int radius = 0;
+                float t[3];
+                view_xform(mean[0], mean[1], mean[2], view, t);
+                const float rr = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+                if (!(rr <= 0.04f)) {                              // auxiliary.h:143
+                    GeoOut go;
+                    const float rd = sqrtf(rr);
+                    project(t, rd, d.W, d.H, go);
+                    float M2[2][3], A, B, C;
+                    cov2d(go.J, view, c3, M2, A, B, C);
+                    const float det = A * C - B * B;
+                    if (det != 0.0f) {                             // forward.cu:296
+                        const float det_inv = 1.0f / det;
+                        const float mid = 0.5f * (A + C);
+                        const float disc = sqrtf(fmaxf(0.1f, mid * mid - det));
+                        const float l1 = mid + disc;
+                        radius = (int)ceilf(3.0f * sqrtf(l1));
+                        get_rect(go.px, go.py, radius, d.gx, d.gy, x0, y0, x1, y1);
+                        if ((x1 - x0) * (y1 - y0) == 0) {          // forward.cu:311
+                            radius = 0;
+                            x0 = y0 = x1 = y1 = 0;
+                        } else {
+                            const float qa = C * det_inv, qb = -B * det_inv, qc = A * det_inv;
+                            a.splat[2 * ri] = make_float4(go.px, go.py, -0.5f * qa, -qb);
+                            a.splat[2 * ri + 1] = make_float4(-0.5f * qc, o, thr, 0.0f);
+                            a.depth[ri] = rd;
+                            if (a.mlp) {
+                                const float s0 = 1.0f / (1.0f + gs_expf(-(z0[c] - ld)));
+                                const float s1 = 1.0f / (1.0f + gs_expf(-(z1[c] - ld)));
+                                a.sig[ri] = make_float2(s0, s1);
+                            }
+                        }
+                    }
+                }
+                a.radii[ri] = radius;
+            }
+            // ---- per-tile counts of this block's Gaussians for receiver r
+            const uint64_t any = ballot(x1 > x0);
+            int bx0 = 0x7fffffff, by0 = 0x7fffffff, bx1 = 0, by1 = 0;   // wave union box
+            if (any) {
+                const bool has = x1 > x0;
+                int lx0 = has ? x0 : 0x7fffffff, ly0 = has ? y0 : 0x7fffffff;
+                int lx1 = has ? x1 : 0, ly1 = has ? y1 : 0;
+                for (int off = 32; off >= 1; off >>= 1) {
+                    lx0 = min(lx0, __shfl_xor(lx0, off));
+                    ly0 = min(ly0, __shfl_xor(ly0, off));
+                    lx1 = max(lx1, __shfl_xor(lx1, off));
+                    ly1 = max(ly1, __shfl_xor(ly1, off));
+                }
+                bx0 = lx0; by0 = ly0; bx1 = lx1; by1 = ly1;
+            }
+            for (int c0 = 0; c0 < d.T; c0 += GS_WAVE) {
+                uint32_t mine = 0;
+                const int cend = min(d.T, c0 + GS_WAVE);
+                if (any) {
+                    for (int t = c0; t < cend; ++t) {
+                        const int tx = t % d.gx, ty = t / d.gx;
+                        if (tx < bx0 || tx >= bx1 || ty < by0 || ty >= by1) continue;   // uniform
+                        const uint64_t m = ballot(tx >= x0 && tx < x1 && ty >= y0 && ty < y1);
+                        if (lane == t - c0) mine = (uint32_t)__popcll(m);
+                    }
+                }
+                s_cnt[wave][lane] = mine;
+                __syncthreads();
+                if (threadIdx.x < GS_WAVE && c0 + (int)threadIdx.x < d.T) {
+                    uint32_t sum = 0;
+#pragma unroll
+                    for (int w = 0; w < GS_PRE_THREADS / GS_WAVE; ++w) sum += s_cnt[w][threadIdx.x];
+                    a.wcnt[((size_t)r * NB + blockIdx.x) * d.T + c0 + threadIdx.x] = sum;
+                }
+                __syncthreads();
+            }
+        }
+    }
+}
+
+hipError_t launch_preprocess_fwd(const Dims& d, const PreFwdArgs& a, hipStream_t s)
+{
+    const int NB = (d.P + GS_PRE_THREADS - 1) / GS_PRE_THREADS;
+    if (NB == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_preprocess_fwd, dim3(NB), dim3(GS_PRE_THREADS), 0, s, d, a, NB);
+    return hipGetLastError();
+}
+
+}  // namespace gs

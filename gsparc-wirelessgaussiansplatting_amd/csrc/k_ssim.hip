@@ -1,0 +1,173 @@
+// Fused SSIM forward and backward (fused-ssim/ssim.cu:187-366) over a batch
+// of planes (B * CH planes of H x W; for the rasterizer, one plane per
+// receiver spectrum).
+//
+// A 256-thread workgroup owns a 32 x 16 output tile.  The zero-padded halo
+// (42 x 26) of the inputs is staged in LDS once; the horizontal 11-tap pass
+// produces all five (forward) or three (backward) filtered quantities in one
+// sweep, then the vertical pass finishes them per output pixel.  Summation
+// order matches the reference (taps 0..10, horizontal first), so the result
+// agrees with the CPU restatement to the last bit for the same taps.
+#include <cmath>
+
+#include "gs_device.h"
+#include "gs_kernels.h"
+
+namespace gs {
+
+#define SS_TW 32
+#define SS_TH 16
+#define SS_HW (SS_TW + 10)
+#define SS_HH (SS_TH + 10)
+#define SS_THREADS 256
+
+struct Taps { float w[11]; };
+
+__global__ void __launch_bounds__(SS_THREADS) k_ssim_fwd(SsimArgs a, Taps tp)
+{
+    __shared__ float s1[SS_HH][SS_HW + 1];
+    __shared__ float s2[SS_HH][SS_HW + 1];
+    __shared__ float h[5][SS_HH][SS_TW + 1];
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * SS_TW, y0 = blockIdx.y * SS_TH;
+    const size_t plane = (size_t)blockIdx.z * a.H * a.W;
+    const float* im1 = a.img1 + plane;
+    const float* im2 = a.img2 + plane;
+    for (int i = tid; i < SS_HH * SS_HW; i += SS_THREADS) {
+        const int ly = i / SS_HW, lx = i % SS_HW;
+        const int gy = y0 + ly - 5, gx = x0 + lx - 5;
+        const bool in = gx >= 0 && gx < a.W && gy >= 0 && gy < a.H;
+        s1[ly][lx] = in ? im1[(size_t)gy * a.W + gx] : 0.0f;
+        s2[ly][lx] = in ? im2[(size_t)gy * a.W + gx] : 0.0f;
+    }
+    __syncthreads();
+    for (int i = tid; i < SS_HH * SS_TW; i += SS_THREADS) {
+        const int ly = i / SS_TW, lx = i % SS_TW;
+        float m1 = 0.0f, m2 = 0.0f, e11 = 0.0f, e22 = 0.0f, e12 = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 11; ++k) {
+            const float p = s1[ly][lx + k], q = s2[ly][lx + k];
+            m1 = m1 + tp.w[k] * p;
+            m2 = m2 + tp.w[k] * q;
+            e11 = e11 + tp.w[k] * (p * p);
+            e22 = e22 + tp.w[k] * (q * q);
+            e12 = e12 + tp.w[k] * (p * q);
+        }
+        h[0][ly][lx] = m1; h[1][ly][lx] = m2; h[2][ly][lx] = e11; h[3][ly][lx] = e22; h[4][ly][lx] = e12;
+    }
+    __syncthreads();
+    for (int i = tid; i < SS_TH * SS_TW; i += SS_THREADS) {
+        const int ly = i / SS_TW, lx = i % SS_TW;
+        const int gy = y0 + ly, gx = x0 + lx;
+        if (gy >= a.H || gx >= a.W) continue;
+        float v[5];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 11; ++k) acc = acc + tp.w[k] * h[c][ly + k][lx];
+            v[c] = acc;
+        }
+        const float mu1 = v[0], mu2 = v[1];
+        const float sigma1_sq = v[2] - mu1 * mu1;
+        const float sigma2_sq = v[3] - mu2 * mu2;
+        const float sigma12 = v[4] - mu1 * mu2;
+        const float C = 2.0f * mu1 * mu2 + a.C1;
+        const float D = 2.0f * sigma12 + a.C2;
+        const float A = mu1 * mu1 + mu2 * mu2 + a.C1;
+        const float B = sigma1_sq + sigma2_sq + a.C2;
+        const size_t o = plane + (size_t)gy * a.W + gx;
+        a.map[o] = (C * D) / (A * B);
+        if (a.dmu1) {
+            a.dmu1[o] = (mu2 * 2.0f * D) / (A * B) - (mu2 * 2.0f * C) / (A * B) - (mu1 * 2.0f * C * D) / (A * A * B) +
+                        (mu1 * 2.0f * C * D) / (A * B * B);
+            a.ds1[o] = (-C * D) / (A * B * B);
+            a.ds12[o] = (2.0f * C) / (A * B);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(SS_THREADS) k_ssim_bwd(SsimBwdArgs a, Taps tp)
+{
+    __shared__ float s[3][SS_HH][SS_HW + 1];
+    __shared__ float h[3][SS_HH][SS_TW + 1];
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * SS_TW, y0 = blockIdx.y * SS_TH;
+    const size_t plane = (size_t)blockIdx.z * a.H * a.W;
+    for (int i = tid; i < SS_HH * SS_HW; i += SS_THREADS) {
+        const int ly = i / SS_HW, lx = i % SS_HW;
+        const int gy = y0 + ly - 5, gx = x0 + lx - 5;
+        float p0 = 0.0f, p1 = 0.0f, p2 = 0.0f;
+        if (gx >= 0 && gx < a.W && gy >= 0 && gy < a.H) {
+            const size_t o = plane + (size_t)gy * a.W + gx;
+            const float g = a.dmap[o];
+            p0 = g * a.dmu1[o];
+            p1 = g * a.ds1[o];
+            p2 = g * a.ds12[o];
+        }
+        s[0][ly][lx] = p0; s[1][ly][lx] = p1; s[2][ly][lx] = p2;
+    }
+    __syncthreads();
+    for (int i = tid; i < SS_HH * SS_TW; i += SS_THREADS) {
+        const int ly = i / SS_TW, lx = i % SS_TW;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 11; ++k) acc = acc + tp.w[k] * s[c][ly][lx + k];
+            h[c][ly][lx] = acc;
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < SS_TH * SS_TW; i += SS_THREADS) {
+        const int ly = i / SS_TW, lx = i % SS_TW;
+        const int gy = y0 + ly, gx = x0 + lx;
+        if (gy >= a.H || gx >= a.W) continue;
+        float v[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 11; ++k) acc = acc + tp.w[k] * h[c][ly + k][lx];
+            v[c] = acc;
+        }
+        const size_t o = plane + (size_t)gy * a.W + gx;
+        a.dimg1[o] = v[0] + a.img1[o] * 2.0f * v[1] + a.img2[o] * v[2];
+    }
+}
+
+// 11-tap Gaussian window, sigma 1.5 (utils.py:116-124): float32 samples
+// normalised by their sum, which reproduces the float literals of
+// ssim.cu:9-19 bit for bit.
+static Taps make_taps()
+{
+    Taps t;
+    double s = 0.0;
+    for (int i = 0; i < 11; ++i) {
+        t.w[i] = (float)std::exp(-(double)((i - 5) * (i - 5)) / (2.0 * 1.5 * 1.5));
+        s += t.w[i];
+    }
+    const float sf = (float)s;
+    for (int i = 0; i < 11; ++i) t.w[i] = t.w[i] / sf;
+    return t;
+}
+
+hipError_t launch_ssim_fwd(const SsimArgs& a, hipStream_t s)
+{
+    if (a.planes == 0 || a.H == 0 || a.W == 0) return hipSuccess;
+    static const Taps tp = make_taps();
+    dim3 grid((a.W + SS_TW - 1) / SS_TW, (a.H + SS_TH - 1) / SS_TH, a.planes);
+    hipLaunchKernelGGL(k_ssim_fwd, grid, dim3(SS_THREADS), 0, s, a, tp);
+    return hipGetLastError();
+}
+
+hipError_t launch_ssim_bwd(const SsimBwdArgs& a, hipStream_t s)
+{
+    if (a.planes == 0 || a.H == 0 || a.W == 0) return hipSuccess;
+    static const Taps tp = make_taps();
+    dim3 grid((a.W + SS_TW - 1) / SS_TW, (a.H + SS_TH - 1) / SS_TH, a.planes);
+    hipLaunchKernelGGL(k_ssim_bwd, grid, dim3(SS_THREADS), 0, s, a, tp);
+    return hipGetLastError();
+}
+
+}  // namespace gs

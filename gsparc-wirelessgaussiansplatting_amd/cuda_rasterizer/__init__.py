@@ -1,0 +1,135 @@
+"""Python rasterizer API: drop-in for the reference's
+cuda_kernel_rasterize/cuda_rasterizer/__init__.py (lines 12-131).
+
+Same names, argument order, outputs and gradient order:
+  rasterize_gaussians(means3D, means2D, emission_mlps, opacities, scales,
+                      rotations, raster_settings) -> (out_real, out_imag, radii)
+  _RasterizeGaussians.backward returns grads for (means3D, means2D,
+  emission_mlps, opacities, scales, rotations, raster_settings)
+  GaussianRasterizerSetting(image_height, image_width, scale_modifier,
+                            viewmatrix, projmatrix, tx_pos, rx_pos)
+  GaussianRasterizer(raster_settings)(means3D, means2D, emission_mlps,
+                                      opacities, scales=None, rotations=None)
+
+Additions: GaussianRasterizerSetting.grad_mode ("reference", the default,
+reproduces the reference backward; "exact" is the true adjoint), the
+`rasterize_gaussians_batched` function for many receivers per launch, and
+`emission=` for precomputed per-Gaussian emission.
+"""
+from __future__ import annotations
+
+from typing import NamedTuple, Optional
+
+import torch
+import torch.nn as nn
+
+import wireless_rasterizer as wr
+
+
+def rasterize_gaussians(means3D, means2D, emission_mlps, opacities, scales, rotations, raster_settings):
+    return _RasterizeGaussians.apply(means3D, means2D, emission_mlps, opacities, scales, rotations,
+                                     raster_settings)
+
+
+class _RasterizeGaussians(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, means3D, means2D, emission_mlps, opacities, scales, rotations, raster_settings):
+        s = raster_settings
+        num_rendered, out_real, out_imag, radii, geom, binbuf, img = wr.rasterize_gaussians(
+            means3D, emission_mlps, opacities, scales, rotations, s.image_height, s.image_width, s.scale_modifier,
+            s.viewmatrix, s.projmatrix, s.tx_pos, s.rx_pos)
+        ctx.raster_settings = s
+        ctx.num_rendered = num_rendered
+        ctx.save_for_backward(emission_mlps, out_real, out_imag, means3D, scales, rotations, opacities, radii, geom,
+                              binbuf, img)
+        ctx.mark_non_differentiable(radii)
+        return out_real, out_imag, radii
+
+    @staticmethod
+    def backward(ctx, grad_out_real, grad_out_imag, grad_radii):
+        s = ctx.raster_settings
+        (emission_mlps, out_real, out_imag, means3D, scales, rotations, opacities, radii, geom, binbuf,
+         img) = ctx.saved_tensors
+        g2, g3, gmlp, gop, _gcov, gsc, grot = wr.rasterize_gaussians_backward(
+            means3D, emission_mlps, radii, out_real, out_imag, opacities, scales, rotations, s.scale_modifier,
+            s.viewmatrix, s.projmatrix, grad_out_real, grad_out_imag, s.tx_pos, s.rx_pos, geom, ctx.num_rendered,
+            binbuf, img, grad_mode=getattr(s, "grad_mode", "reference"))
+        return g3, g2, gmlp.reshape(emission_mlps.shape), gop.reshape(opacities.shape), gsc, grot, None
+
+
+class GaussianRasterizerSetting(NamedTuple):
+    image_height: int
+    image_width: int
+    scale_modifier: float
+    viewmatrix: torch.Tensor
+    projmatrix: torch.Tensor
+    tx_pos: torch.Tensor
+    rx_pos: torch.Tensor
+    grad_mode: str = "reference"
+
+
+class GaussianRasterizer(nn.Module):
+    def __init__(self, raster_settings):
+        super().__init__()
+        self.raster_settings = raster_settings
+
+    def markVisible(self, positions):
+        with torch.no_grad():
+            s = self.raster_settings
+            return wr.mark_visible(positions, s.viewmatrix, s.projmatrix)
+
+    def forward(self, means3D, means2D, emission_mlps, opacities, scales=None, rotations=None):
+        s = self.raster_settings
+        if scales is None:
+            scales = torch.Tensor([])
+        if rotations is None:
+            rotations = torch.Tensor([])
+        return rasterize_gaussians(means3D, means2D, emission_mlps, opacities, scales, rotations, s)
+
+
+# ------------------------------------------------------------------ batched
+class BatchedSetting(NamedTuple):
+    """R receivers rendered in one launch sequence (no reference counterpart:
+    the reference renders one receiver per call, renderer.py:39-88)."""
+    image_height: int
+    image_width: int
+    scale_modifier: float
+    viewmatrices: torch.Tensor      # [R,4,4]
+    tx_pos: torch.Tensor            # [3]
+    rx_pos: torch.Tensor            # [R,3]
+    grad_mode: str = "reference"
+
+
+class _RasterizeBatched(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, means3D, emission_mlps, emission, opacities, scales, rotations, settings):
+        s = settings
+        K, ore, oim, radii, geom, binbuf, img = wr.forward_batched(
+            means3D, emission_mlps, opacities, scales, rotations, s.image_height, s.image_width, s.scale_modifier,
+            s.viewmatrices, s.tx_pos, s.rx_pos, emission=emission)
+        ctx.s = s
+        ctx.K = K
+        ctx.has_emission = emission is not None
+        ctx.save_for_backward(means3D, emission_mlps if emission is None else emission, opacities, scales, rotations,
+                              radii, geom, binbuf, img)
+        ctx.mark_non_differentiable(radii)
+        return ore, oim, radii
+
+    @staticmethod
+    def backward(ctx, gre, gim, _gr):
+        s = ctx.s
+        means3D, em_or_mlp, opacities, scales, rotations, radii, geom, binbuf, img = ctx.saved_tensors
+        mlp = None if ctx.has_emission else em_or_mlp
+        em = em_or_mlp if ctx.has_emission else None
+        d2, d3, dmlp, dop, _dcov, dsc, drot, dem = wr.backward_batched(
+            means3D, mlp, radii, opacities, scales, rotations, s.scale_modifier, s.viewmatrices, gre, gim, s.tx_pos,
+            s.rx_pos, geom, ctx.K, binbuf, img, grad_mode=s.grad_mode, emission=em)
+        return (d3, None if dmlp is None else dmlp.reshape(mlp.shape), None if dem is None else dem.reshape(em.shape),
+                dop.reshape(opacities.shape), dsc, drot, None)
+
+
+def rasterize_gaussians_batched(means3D, emission_mlps, opacities, scales, rotations, settings: BatchedSetting,
+                                emission: Optional[torch.Tensor] = None):
+    """Render R receivers: returns (out_real [R,H,W], out_imag [R,H,W], radii [R,P]).
+    Pass emission_mlps (per-Gaussian 3->32->2 MLPs) or emission [R,P,2]."""
+    return _RasterizeBatched.apply(means3D, emission_mlps, emission, opacities, scales, rotations, settings)

@@ -1,0 +1,108 @@
+"""ctypes binding of libgsparc.so (the C ABI declared in include/gsparc.h).
+
+Loads the in-tree gfx950 library and fails loudly when it is missing: there is
+no CPU or eager fallback anywhere in the product path.  PyTorch provides device
+memory and the current HIP stream; the kernels are the library's.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgsparc.so")
+
+GRAD_EXACT = 0
+GRAD_REFERENCE = 1
+GRAD_MODES = {"exact": GRAD_EXACT, "reference": GRAD_REFERENCE}
+FLAG_KEEP_KEYS = 1
+
+_vp = C.c_void_p
+_i = C.c_int
+_f = C.c_float
+_u64 = C.c_uint64
+_sz = C.c_size_t
+
+
+class Layout(C.Structure):
+    _fields_ = [(n, C.c_size_t) for n in (
+        "splat", "sig", "depth", "cov3", "wcnt", "geom_bytes",
+        "final_t", "n_contrib", "ranges", "segcnt", "segbase", "total", "image_bytes",
+        "keys", "vals", "keys_tmp", "vals_tmp", "plist", "keys_sorted", "bin_bytes")]
+
+
+_SIGS = {
+    "gsparc_version": (C.c_char_p, []),
+    "gsparc_last_error": (C.c_char_p, []),
+    "gsparc_geom_buffer_size": (_sz, [_i, _i, _i, _i]),
+    "gsparc_image_buffer_size": (_sz, [_i, _i, _i, _i]),
+    "gsparc_binning_buffer_size": (_sz, [_u64]),
+    "gsparc_grad_buffer_size": (_sz, [_i, _i]),
+    "gsparc_layout_query": (_i, [_i, _i, _i, _i, _u64, C.POINTER(Layout)]),
+    "gsparc_rasterize_forward_geom": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f, _vp, _vp,
+                                           _vp, C.POINTER(_u64), _vp]),
+    "gsparc_rasterize_forward_render": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _u64, _vp, _vp, _vp, _i, _vp]),
+    "gsparc_rasterize_backward": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f, _vp, _vp,
+                                       _vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                       _vp]),
+    "gsparc_mark_visible": (_i, [_i, _vp, _vp, _vp, _vp]),
+    "gsparc_ssim_forward": (_i, [_i, _i, _i, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "gsparc_ssim_backward": (_i, [_i, _i, _i, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded library (raises ImportError if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with "
+                              "`python gsparc-wirelessgaussiansplatting_amd/csrc/build.py`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def check(status: int, what: str):
+    if status != 0:
+        msg = lib().gsparc_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed (status {status}): {msg}")
+
+
+def ptr(t: torch.Tensor | None):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def stream_handle(device=None):
+    return C.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def layout(P, R, W, H, K=0) -> Layout:
+    L = Layout()
+    check(lib().gsparc_layout_query(P, R, W, H, K, C.byref(L)), "gsparc_layout_query")
+    return L
+
+
+def require_cuda(*tensors):
+    for t in tensors:
+        if t is not None and t.numel() > 0 and not t.is_cuda:
+            raise RuntimeError("gsparc kernels need tensors on a HIP device (got a CPU tensor)")
+
+
+def f32(t: torch.Tensor) -> torch.Tensor:
+    """Contiguous float32 view/copy, as the reference's .contiguous().data_ptr<float>()."""
+    if t.dtype != torch.float32:
+        t = t.float()
+    return t.contiguous()

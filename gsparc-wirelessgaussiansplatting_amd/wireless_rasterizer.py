@@ -1,0 +1,200 @@
+"""Native-level rasterizer API: drop-in for the reference's `wireless_rasterizer`
+torch extension (cuda_kernel_rasterize/ext.cpp:7-11, rasterize_points.h:8-46).
+
+    rasterize_gaussians(means3D, emission_mlps, opacities, scales, rotations,
+                        image_height, image_width, scale_modifier, viewmatrix,
+                        projmatrix, tx_pos, rx_pos)
+        -> (num_rendered, out_real [1,H,W], out_imag [1,H,W], radii [P] int32,
+            geomBuffer, binningBuffer, imgBuffer)            (rasterize_points.cu:32-102)
+
+    rasterize_gaussians_backward(means3D, emission_mlps, radii, out_real,
+                                 out_imag, opacities, scales, rotations,
+                                 scale_modifier, viewmatrix, projmatrix,
+                                 dL_dout_real, dL_dout_imag, tx_pos, rx_pos,
+                                 geomBuffer, num_rendered, binningBuffer,
+                                 imageBuffer)
+        -> (dL_dmeans2D [P,3], dL_dmeans3D [P,3], dL_demission_mlps [M],
+            dL_dopacity [P,1], dL_dcov3D [P,6], dL_dscales [P,3],
+            dL_drotations [P,4])                                (rasterize_points.cu:106-199)
+
+    mark_visible(means3D, viewmatrix, projmatrix) -> bool [P]   (rasterize_points.cu:203-221)
+
+Extensions beyond the reference (keyword-only, defaults keep the reference
+behaviour): `grad_mode` ("reference" reproduces the reference backward,
+"exact" is the true adjoint of the forward) and `emission` (per-Gaussian
+(real, imag) emission supplied directly instead of the per-Gaussian MLP).
+The batched functions render R receivers in one launch sequence.
+
+The byte buffers hold private state of this implementation (their layout is
+described in csrc/gs_device.h); like the reference's they are kept by the
+autograd function between forward and backward.  projmatrix is accepted and
+ignored, as in the reference (it is never read by its kernels).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+import gsparc_hip as gh
+
+U8 = torch.uint8
+
+
+def _empty_bytes(device, n=0):
+    return torch.empty(n, dtype=U8, device=device)
+
+
+def _check_means(means3D):
+    if means3D.ndimension() != 2 or means3D.size(1) != 3:
+        raise RuntimeError("means3D must have dimensions (num_points, 3)")
+
+
+def forward_batched(means3D, emission_mlps, opacities, scales, rotations, image_height, image_width,
+                    scale_modifier, viewmatrices, tx_pos, rx_pos, emission=None, keep_keys=False):
+    """Forward for R receivers.  viewmatrices [R,4,4] (column-major storage as
+    the reference's viewmatrix), rx_pos [R,3], emission None or [R,P,2].
+    Returns (K, out_real [R,H,W], out_imag [R,H,W], radii [R,P], geom, bin, img)."""
+    _check_means(means3D)
+    P = int(means3D.size(0))
+    H, W = int(image_height), int(image_width)
+    dev = means3D.device
+    views = gh.f32(viewmatrices).reshape(-1, 16)
+    rxs = gh.f32(rx_pos).reshape(-1, 3)
+    R = int(rxs.size(0))
+    if views.size(0) != R:
+        raise RuntimeError("one view matrix per receiver position is required")
+    if P == 0 or R == 0:
+        z = torch.zeros((R, H, W), dtype=torch.float32, device=dev)
+        return (0, z, z.clone(), torch.zeros((R, P), dtype=torch.int32, device=dev), _empty_bytes(dev),
+                _empty_bytes(dev), _empty_bytes(dev))
+    if scales is None or rotations is None or scales.numel() == 0 or rotations.numel() == 0:
+        raise RuntimeError("scales and rotations are required")
+    means = gh.f32(means3D)
+    sc = gh.f32(scales)
+    rot = gh.f32(rotations)
+    op = gh.f32(opacities).reshape(-1)
+    tx = gh.f32(tx_pos).reshape(-1)
+    mlp = gh.f32(emission_mlps).reshape(-1) if emission is None else None
+    em = None
+    if emission is not None:
+        em = gh.f32(emission).reshape(R, P, 2)
+    elif mlp.numel() != P * 194:
+        raise RuntimeError("emission_mlps must hold 194 floats per Gaussian")
+    gh.require_cuda(means, sc, rot, op, tx, views, rxs, mlp, em)
+    L = gh.lib()
+    geom = _empty_bytes(dev, L.gsparc_geom_buffer_size(P, R, W, H))
+    img = _empty_bytes(dev, L.gsparc_image_buffer_size(P, R, W, H))
+    radii = torch.empty((R, P), dtype=torch.int32, device=dev)
+    stream = gh.stream_handle(dev)
+    K = C.c_uint64(0)
+    gh.check(L.gsparc_rasterize_forward_geom(P, R, W, H, gh.ptr(means), gh.ptr(sc), gh.ptr(rot), gh.ptr(op),
+                                            gh.ptr(mlp), gh.ptr(views), gh.ptr(tx), gh.ptr(rxs),
+                                            float(scale_modifier), gh.ptr(geom), gh.ptr(img), gh.ptr(radii),
+                                            C.byref(K), stream), "rasterize forward (preprocess)")
+    K = int(K.value)
+    binbuf = _empty_bytes(dev, L.gsparc_binning_buffer_size(K))
+    out_re = torch.empty((R, H, W), dtype=torch.float32, device=dev)
+    out_im = torch.empty((R, H, W), dtype=torch.float32, device=dev)
+    gh.check(L.gsparc_rasterize_forward_render(P, R, W, H, gh.ptr(em), gh.ptr(geom), gh.ptr(img), gh.ptr(binbuf),
+                                              K, gh.ptr(radii), gh.ptr(out_re), gh.ptr(out_im),
+                                              gh.FLAG_KEEP_KEYS if keep_keys else 0, stream),
+             "rasterize forward (render)")
+    return K, out_re, out_im, radii, geom, binbuf, img
+
+
+def backward_batched(means3D, emission_mlps, radii, opacities, scales, rotations, scale_modifier,
+                     viewmatrices, dL_dout_real, dL_dout_imag, tx_pos, rx_pos, geomBuffer, num_rendered,
+                     binningBuffer, imageBuffer, grad_mode="reference", emission=None):
+    """Backward for R receivers; gradients are summed over receivers.
+    Returns (dmeans2D [R,P,3], dmeans3D [P,3], demission_mlps [M] or None,
+    dopacity [P,1], dcov3D [P,6], dscales [P,3], drotations [P,4],
+    demission [R,P,2] or None)."""
+    P = int(means3D.size(0))
+    R = int(dL_dout_real.size(0))
+    H, W = int(dL_dout_real.size(1)), int(dL_dout_real.size(2))
+    dev = means3D.device
+    opts = dict(dtype=torch.float32, device=dev)
+    mode = gh.GRAD_MODES[grad_mode] if isinstance(grad_mode, str) else int(grad_mode)
+    M = int(emission_mlps.numel()) if (emission_mlps is not None and emission is None) else 0
+    dmeans2D = torch.zeros((R, P, 3), **opts)
+    dmeans3D = torch.zeros((P, 3), **opts)
+    dmlp = torch.zeros(M, **opts) if emission is None else None
+    dop = torch.zeros((P, 1), **opts)
+    dcov = torch.zeros((P, 6), **opts)
+    dsc = torch.zeros((P, 3), **opts)
+    drot = torch.zeros((P, 4), **opts)
+    dem = torch.zeros((R, P, 2), **opts) if emission is not None else None
+    if P == 0 or R == 0:
+        return dmeans2D, dmeans3D, dmlp, dop, dcov, dsc, drot, dem
+    means = gh.f32(means3D)
+    sc = gh.f32(scales)
+    rot = gh.f32(rotations)
+    views = gh.f32(viewmatrices).reshape(-1, 16)
+    rxs = gh.f32(rx_pos).reshape(-1, 3)
+    tx = gh.f32(tx_pos).reshape(-1)
+    mlp = gh.f32(emission_mlps).reshape(-1) if emission is None else None
+    em = gh.f32(emission).reshape(R, P, 2) if emission is not None else None
+    rad = radii.reshape(R, P).to(torch.int32).contiguous()
+    gre = gh.f32(dL_dout_real)
+    gim = gh.f32(dL_dout_imag)
+    gh.require_cuda(means, sc, rot, views, rxs, tx, mlp, em, rad, gre, gim)
+    L = gh.lib()
+    grad = _empty_bytes(dev, L.gsparc_grad_buffer_size(P, R))
+    gh.check(L.gsparc_rasterize_backward(mode, P, R, W, H, gh.ptr(means), gh.ptr(sc), gh.ptr(rot), gh.ptr(mlp),
+                                        gh.ptr(em), gh.ptr(views), gh.ptr(tx), gh.ptr(rxs), float(scale_modifier),
+                                        gh.ptr(rad), gh.ptr(geomBuffer), gh.ptr(imageBuffer), gh.ptr(binningBuffer),
+                                        int(num_rendered), gh.ptr(grad), gh.ptr(gre), gh.ptr(gim),
+                                        gh.ptr(dmeans2D), gh.ptr(dmeans3D), gh.ptr(dmlp), gh.ptr(dop), gh.ptr(dcov),
+                                        gh.ptr(dsc), gh.ptr(drot), gh.ptr(dem), gh.stream_handle(dev)),
+             "rasterize backward")
+    return dmeans2D, dmeans3D, dmlp, dop, dcov, dsc, drot, dem
+
+
+def rasterize_gaussians(means3D, emission_mlps, opacities, scales, rotations, image_height, image_width,
+                        scale_modifier, viewmatrix, projmatrix, tx_pos, rx_pos, *, emission=None):
+    """One receiver, reference tuple layout (rasterize_points.cu:32-102)."""
+    _check_means(means3D)
+    em = None if emission is None else emission.reshape(1, -1, 2)
+    K, ore, oim, radii, geom, binbuf, img = forward_batched(
+        means3D, emission_mlps, opacities, scales, rotations, image_height, image_width, scale_modifier,
+        viewmatrix.reshape(1, 4, 4), tx_pos, rx_pos.reshape(1, 3), emission=em)
+    return K, ore, oim, radii.reshape(-1), geom, binbuf, img
+
+
+def rasterize_gaussians_backward(means3D, emission_mlps, radii, out_signal_real, out_signal_imag, opacities, scales,
+                                 rotations, scale_modifier, viewmatrix, projmatrix, dL_dout_signal_real,
+                                 dL_dout_signal_imag, tx_pos, rx_pos, geomBuffer, R, binningBuffer, imageBuffer,
+                                 *, grad_mode="reference", emission=None):
+    """One receiver, reference tuple layout (rasterize_points.cu:106-199); R is
+    num_rendered from the forward."""
+    P = int(means3D.size(0))
+    em = None if emission is None else emission.reshape(1, -1, 2)
+    d2, d3, dmlp, dop, dcov, dsc, drot, dem = backward_batched(
+        means3D, emission_mlps, radii.reshape(1, -1) if P else radii, opacities, scales, rotations, scale_modifier,
+        viewmatrix.reshape(1, 4, 4), dL_dout_signal_real.reshape(1, *dL_dout_signal_real.shape[-2:]),
+        dL_dout_signal_imag.reshape(1, *dL_dout_signal_imag.shape[-2:]), tx_pos, rx_pos.reshape(1, 3), geomBuffer, R,
+        binningBuffer, imageBuffer, grad_mode=grad_mode, emission=em)
+    if dmlp is None:
+        dmlp = dem.reshape(P, 2)
+    return d2.reshape(P, 3), d3, dmlp, dop, dcov, dsc, drot
+
+
+def mark_visible(means3D, viewmatrix, projmatrix):
+    """bool [P]: r^2 > 0.04 in receiver space (rasterize_points.cu:203-221)."""
+    P = int(means3D.size(0))
+    present = torch.zeros(P, dtype=torch.bool, device=means3D.device)
+    if P == 0:
+        return present
+    means = gh.f32(means3D)
+    view = gh.f32(viewmatrix).reshape(-1)
+    gh.require_cuda(means, view)
+    out = torch.empty(P, dtype=torch.uint8, device=means3D.device)
+    gh.check(gh.lib().gsparc_mark_visible(P, gh.ptr(means), gh.ptr(view), gh.ptr(out), gh.stream_handle(means.device)),
+             "mark_visible")
+    return out.bool()
+
+
+def mark_visisble(means3D, viewmatrix, projmatrix):
+    """The name the reference's Python wrapper calls (cuda_rasterizer/__init__.py:115)."""
+    return mark_visible(means3D, viewmatrix, projmatrix)

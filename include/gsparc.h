@@ -1,0 +1,131 @@
+/* gsparc.h -- C ABI of the MI355X (gfx950) equirectangular RF rasterizer and
+ * fused-SSIM loss.
+ *
+ * Plain device pointers, sizes and a hipStream_t (passed as void*); every call
+ * returns a status code (GSPARC_OK == 0) and never throws.  All tensors are
+ * float32, C-contiguous, on the current HIP device.  Receivers are batched:
+ * R = 1 reproduces the reference's one-receiver-per-call interface.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the GSpaRC repository root).  The reference binds those through
+ * pybind11/torch (cuda_kernel_rasterize/ext.cpp:7-11, fused-ssim/ext.cpp:4-7);
+ * the ctypes binding in wireless_rasterizer.py / fused_ssim_cuda.py of this
+ * package binds these.
+ *
+ * Buffers.  The caller owns four byte buffers, sized by the queries below:
+ *   geom  per (receiver, Gaussian) records             gsparc_geom_buffer_size
+ *   image per pixel / per tile state                   gsparc_image_buffer_size
+ *   bin   per duplicated (tile, Gaussian) instance     gsparc_binning_buffer_size
+ *   grad  backward accumulation                        gsparc_grad_buffer_size
+ * geom, image and bin must survive from the forward to the backward, as the
+ * reference's geomBuffer / imgBuffer / binningBuffer do
+ * (cuda_rasterizer/__init__.py:48).
+ */
+#ifndef GSPARC_H
+#define GSPARC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GSPARC_OK 0
+#define GSPARC_ERR_ARG 1        /* bad size / null pointer */
+#define GSPARC_ERR_HIP 2        /* a HIP runtime call failed */
+#define GSPARC_ERR_CAPACITY 3   /* binning buffer smaller than K */
+
+#define GSPARC_GRAD_EXACT 0     /* true adjoint of the forward */
+#define GSPARC_GRAD_REFERENCE 1 /* the reference backward, quirks included */
+
+#define GSPARC_FLAG_KEEP_KEYS 1 /* also store the sorted depth keys */
+
+/* Byte offsets of the arrays inside the caller's buffers (for inspection). */
+typedef struct gsparc_layout {
+    /* geom */
+    size_t splat, sig, depth, cov3, wcnt, geom_bytes;
+    /* image */
+    size_t final_t, n_contrib, ranges, segcnt, segbase, total, image_bytes;
+    /* bin */
+    size_t keys, vals, keys_tmp, vals_tmp, plist, keys_sorted, bin_bytes;
+} gsparc_layout;
+
+const char* gsparc_version(void);
+const char* gsparc_last_error(void);
+
+size_t gsparc_geom_buffer_size(int P, int R, int W, int H);
+size_t gsparc_image_buffer_size(int P, int R, int W, int H);
+size_t gsparc_binning_buffer_size(uint64_t K);
+size_t gsparc_grad_buffer_size(int P, int R);
+int gsparc_layout_query(int P, int R, int W, int H, uint64_t K, gsparc_layout* out);
+
+/* Forward, part 1: per-Gaussian preprocess for R receivers (projection,
+ * covariance, conic, radius, tile rect, emission MLP), per-tile instance
+ * counts and tile ranges.  Copies K (the number of duplicated instances,
+ * "num_rendered") to *K_out and synchronises the stream, as the reference
+ * does before sizing its binning buffer.
+ * Replaces: CudaRasterizer::Rasterizer::forward up to the K readback
+ * (cuda_wireless_rasterizer/rasterizer_impl.cu:206-278), FORWARD::preprocess
+ * (forward.cu:252-327, 359-406) and the inclusive scan (rasterizer_impl.cu:270).
+ *   means [P,3], scales [P,3], rotations [P,4] (w,x,y,z), opacities [P],
+ *   emission_mlps [P*194] (gaussian.py:554-572) or NULL when emission is
+ *   supplied to part 2, viewmatrices [R,16] column-major, tx_pos [3],
+ *   rx_pos [R,3]; radii_out [R,P] int32. */
+int gsparc_rasterize_forward_geom(int P, int R, int W, int H, const float* means, const float* scales,
+                                  const float* rotations, const float* opacities, const float* emission_mlps,
+                                  const float* viewmatrices, const float* tx_pos, const float* rx_pos,
+                                  float scale_modifier, void* geom, void* image, int* radii_out,
+                                  uint64_t* K_out, void* stream);
+
+/* Forward, part 2: duplicate instances into tile segments, sort every segment
+ * by depth and blend front to back.  emission: NULL (use the MLP output of
+ * part 1) or [R,P,2] (real, imag) per receiver and Gaussian.  K_capacity is
+ * the instance capacity of `bin`.  Outputs out_real / out_imag [R,H,W].
+ * Replaces: duplicateWithKeys, SortPairs, identifyTileRanges and
+ * FORWARD::render (rasterizer_impl.cu:102-171, 280-333; forward.cu:151-249). */
+int gsparc_rasterize_forward_render(int P, int R, int W, int H, const float* emission, void* geom, void* image,
+                                    void* bin, uint64_t K_capacity, const int* radii, float* out_real,
+                                    float* out_imag, int flags, void* stream);
+
+/* Backward.  grad_mode: GSPARC_GRAD_EXACT or GSPARC_GRAD_REFERENCE.
+ * Inputs as in the forward plus dL/dout_real, dL/dout_imag [R,H,W];
+ * K_capacity is the instance capacity `bin` was laid out with.
+ * Outputs (all written, none accumulated):
+ *   dmeans2D [R,P,3] (screen-space, reference layout; may be NULL),
+ *   dmeans3D [P,3], demission_mlps [P*194] (NULL if no MLP),
+ *   dopacity [P], dcov3D [P,6] (may be NULL), dscales [P,3], drotations [P,4],
+ *   demission [R,P,2] (emission supplied to the forward; may be NULL).
+ * Gradients are summed over the R receivers.
+ * Replaces: CudaRasterizer::Rasterizer::backward (rasterizer_impl.cu:339-442),
+ * BACKWARD::render (backward.cu:442-616, 701-740) and BACKWARD::preprocess
+ * (backward.cu:124-437, 618-699) behind RasterizeGaussiansBackwardCUDA
+ * (rasterize_points.cu:106-199). */
+int gsparc_rasterize_backward(int grad_mode, int P, int R, int W, int H, const float* means, const float* scales,
+                              const float* rotations, const float* emission_mlps, const float* emission,
+                              const float* viewmatrices, const float* tx_pos, const float* rx_pos,
+                              float scale_modifier, const int* radii, void* geom, void* image, void* bin,
+                              uint64_t K_capacity, void* grad, const float* dout_real, const float* dout_imag, float* dmeans2D,
+                              float* dmeans3D, float* demission_mlps, float* dopacity, float* dcov3D,
+                              float* dscales, float* drotations, float* demission, void* stream);
+
+/* Visibility test r^2 > 0.04 (checkFrustum, rasterizer_impl.cu:173-202;
+ * markVisible, rasterize_points.cu:203-221).  present [P] uint8. */
+int gsparc_mark_visible(int P, const float* means, const float* viewmatrix, uint8_t* present, void* stream);
+
+/* Fused SSIM over `planes` planes of H x W (B*CH in the reference).
+ * Forward writes the SSIM map and, when dm_dmu1 != NULL, the three partial
+ * derivative maps.  Replaces fusedssim (fused-ssim/ssim.cu:368-404). */
+int gsparc_ssim_forward(int planes, int H, int W, float C1, float C2, const float* img1, const float* img2,
+                        float* ssim_map, float* dm_dmu1, float* dm_dsigma1_sq, float* dm_dsigma12, void* stream);
+
+/* dL/dimg1.  Replaces fusedssim_backward (fused-ssim/ssim.cu:406-444). */
+int gsparc_ssim_backward(int planes, int H, int W, float C1, float C2, const float* img1, const float* img2,
+                         const float* dL_dmap, const float* dm_dmu1, const float* dm_dsigma1_sq,
+                         const float* dm_dsigma12, float* dL_dimg1, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSPARC_H */

@@ -717,9 +717,13 @@ extern "C" void orc_preprocess_bwd_f64(BWD_ARGS(double)) { pre_bwd_all<double>(m
 template <typename R>
 static void gauss11(R* w)
 {
-    double g[11], s = 0.0;
-    for (int i = 0; i < 11; ++i) { g[i] = std::exp(-(double)((i - 5) * (i - 5)) / (2.0 * 1.5 * 1.5)); s += g[i]; }
-    for (int i = 0; i < 11; ++i) w[i] = (R)(float)(g[i] / s);
+    // float32 samples of exp(-x^2 / (2 sigma^2)) normalised by their sum: the
+    // float literals of fused-ssim/ssim.cu:9-19 are exactly these values
+    float g[11];
+    double s = 0.0;
+    for (int i = 0; i < 11; ++i) { g[i] = (float)std::exp(-(double)((i - 5) * (i - 5)) / (2.0 * 1.5 * 1.5)); s += g[i]; }
+    const float sf = (float)s;
+    for (int i = 0; i < 11; ++i) w[i] = (R)(g[i] / sf);
 }
 
 template <typename R>
@@ -817,3 +821,16 @@ extern "C" float orc_gs_logf(float x) { return gs_logf(x); }
 extern "C" float orc_gs_atan2f(float y, float x) { return gs_atan2f(y, x); }
 extern "C" float orc_gs_asinf(float x) { return gs_asinf(x); }
 
+
+// vectorised math for the accuracy tests: which = 0 exp, 1 log, 2 asin, 3 atan2(y, x)
+extern "C" void orc_gs_math_vec(int which, const float* x, const float* y, float* out, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        switch (which) {
+        case 0: out[i] = gs_expf(x[i]); break;
+        case 1: out[i] = gs_logf(x[i]); break;
+        case 2: out[i] = gs_asinf(x[i]); break;
+        default: out[i] = gs_atan2f(y[i], x[i]); break;
+        }
+    }
+}

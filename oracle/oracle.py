@@ -40,6 +40,16 @@ def lib():
     return _lib
 
 
+def math_vec(which, x, y=None):
+    """gs_math.h functions over arrays: which in {"exp", "log", "asin", "atan2"}."""
+    code = {"exp": 0, "log": 1, "asin": 2, "atan2": 3}[which]
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(x if y is None else y, np.float32)
+    out = np.empty_like(x)
+    lib().orc_gs_math_vec(C.c_int(code), _p(x), _p(y), _p(out), C.c_int64(x.size))
+    return out
+
+
 def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 

@@ -1,0 +1,73 @@
+"""CPU tests: accuracy of the shared float32 math functions (gs_math.h) and
+the C ABI library's exported symbols (no GPU calls)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _ulps(got, ref):
+    got = got.astype(np.float64)
+    rf = ref.astype(np.float32)
+    spacing = np.spacing(np.abs(rf)).astype(np.float64)
+    spacing = np.maximum(spacing, 2.0 ** -149)
+    return np.abs(got - ref) / spacing
+
+
+def test_expf_accuracy():
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.uniform(-103.0, 88.7, 2_000_000), rng.uniform(-12, 0, 2_000_000)]).astype(np.float32)
+    got = O.math_vec("exp", x)
+    assert _ulps(got, np.exp(x.astype(np.float64))).max() <= 2.0
+    assert O.math_vec("exp", np.array([0.0, -200.0, 100.0, np.nan], np.float32)).tolist()[:3] == [1.0, 0.0, np.inf]
+
+
+def test_logf_accuracy():
+    rng = np.random.default_rng(1)
+    x = np.exp(rng.uniform(-80, 80, 3_000_000)).astype(np.float32)
+    x = np.concatenate([x, rng.uniform(0.2, 12.0, 1_000_000).astype(np.float32),
+                        np.array([1e-40, 1.0, 2.0], np.float32)])
+    got = O.math_vec("log", x)
+    assert _ulps(got, np.log(x.astype(np.float64))).max() <= 2.0
+    assert O.math_vec("log", np.array([1.0], np.float32))[0] == 0.0
+
+
+def test_asinf_accuracy():
+    x = np.linspace(-1.0, 1.0, 4_000_001, dtype=np.float32)
+    got = O.math_vec("asin", x)
+    assert _ulps(got, np.arcsin(x.astype(np.float64))).max() <= 2.0
+
+
+def test_atan2f_accuracy():
+    rng = np.random.default_rng(2)
+    y = (rng.normal(size=3_000_000) * np.exp(rng.uniform(-5, 5, 3_000_000))).astype(np.float32)
+    x = (rng.normal(size=3_000_000) * np.exp(rng.uniform(-5, 5, 3_000_000))).astype(np.float32)
+    got = O.math_vec("atan2", x, y)
+    assert _ulps(got, np.arctan2(y.astype(np.float64), x.astype(np.float64))).max() <= 2.0
+    special = O.math_vec("atan2", np.array([1, -1, 0, 0], np.float32), np.array([0, 0, 1, -1], np.float32))
+    np.testing.assert_allclose(special, [0.0, np.pi, np.pi / 2, -np.pi / 2], rtol=1e-7)
+
+
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "gsparc.h")).read()
+    declared = set(re.findall(r"\b(gsparc_[a-z_]+)\s*\(", header))
+    import gsparc_hip as gh
+    lib = ctypes.CDLL(gh.LIB_PATH)
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert declared == set(gh.exported_symbols())
+    assert gh.lib().gsparc_version().startswith(b"gsparc")
+
+
+def test_buffer_layout_sizes():
+    import gsparc_hip as gh
+    L = gh.layout(50000, 64, 360, 90, 1 << 20)
+    assert L.splat == 0 and L.sig >= 64 * 50000 * 32 and L.geom_bytes > L.wcnt
+    for off in (L.sig, L.depth, L.cov3, L.wcnt, L.n_contrib, L.ranges, L.plist):
+        assert off % 256 == 0
+    assert gh.lib().gsparc_binning_buffer_size(1 << 20) >= 6 * 4 * (1 << 20)
