@@ -59,8 +59,10 @@ struct BinArgs {
     uint32_t* keys_sorted;   // optional [Kcap] sorted depth bits
     uint64_t Kcap;
 };
-hipError_t launch_bin_scan(const Dims& d, const BinArgs& a, hipStream_t s);
-hipError_t launch_bin_scatter_sort(const Dims& d, const BinArgs& a, hipStream_t s);
+hipError_t launch_colscan(const Dims& d, const BinArgs& a, hipStream_t s);
+hipError_t launch_segscan(const Dims& d, const BinArgs& a, hipStream_t s);
+hipError_t launch_scatter(const Dims& d, const BinArgs& a, hipStream_t s);
+hipError_t launch_sort(const Dims& d, const BinArgs& a, hipStream_t s);
 
 struct RenderFwdArgs {
     const uint2* ranges;     // [R*T]
@@ -73,6 +75,7 @@ struct RenderFwdArgs {
     uint32_t* ncontrib;      // [R*N]
     const uint32_t* total;   // K as computed on the device
     uint64_t Kcap;           // capacity of plist (nothing is read when K > Kcap)
+    unsigned long long* pairs;   // optional: += (pixel, Gaussian) pairs visited
 };
 hipError_t launch_render_fwd(const Dims& d, const RenderFwdArgs& a, hipStream_t s);
 

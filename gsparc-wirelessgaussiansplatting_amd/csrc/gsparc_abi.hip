@@ -4,7 +4,10 @@
 
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/gsparc.h"
 #include "gs_device.h"
@@ -69,6 +72,47 @@ int check_dims(int P, int R, int W, int H)
     if ((int64_t)P * R >= (int64_t)1 << 31) return fail(GSPARC_ERR_ARG, "P*R too large");
     return GSPARC_OK;
 }
+
+// ---- per-kernel timing (HIP events on the launch stream), off by default
+struct ProfPending {
+    const char* name;
+    hipEvent_t a, b;
+};
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+std::vector<ProfPending> g_prof_pending;
+std::vector<hipEvent_t> g_prof_free;
+std::map<std::string, std::pair<double, long long>> g_prof_acc;
+unsigned long long* g_pairs_dev = nullptr;
+
+hipEvent_t prof_event()
+{
+    if (!g_prof_free.empty()) {
+        hipEvent_t e = g_prof_free.back();
+        g_prof_free.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+#define GS_LAUNCH(name, call, stream)                                                   \
+    do {                                                                                \
+        hipEvent_t a_ = nullptr, b_ = nullptr;                                          \
+        if (g_prof_on) {                                                                \
+            std::lock_guard<std::mutex> lk_(g_prof_mu);                                 \
+            a_ = prof_event();                                                          \
+            b_ = prof_event();                                                          \
+            if (a_) hipEventRecord(a_, stream);                                         \
+        }                                                                               \
+        GS_HIP(call);                                                                   \
+        if (g_prof_on && a_ && b_) {                                                    \
+            std::lock_guard<std::mutex> lk_(g_prof_mu);                                 \
+            hipEventRecord(b_, stream);                                                 \
+            g_prof_pending.push_back({name, a_, b_});                                   \
+        }                                                                               \
+    } while (0)
 
 __global__ void k_mark_visible(int P, const float* means, const float* view, uint8_t* present)
 {
@@ -138,7 +182,7 @@ int gsparc_rasterize_forward_geom(int P, int R, int W, int H, const float* means
     pa.views = viewmatrices; pa.tx = tx_pos; pa.rx = rx_pos; pa.mod = scale_modifier;
     pa.splat = (float4*)(gb + L.splat); pa.sig = (float2*)(gb + L.sig); pa.depth = (float*)(gb + L.depth);
     pa.radii = radii_out; pa.cov3 = (float*)(gb + L.cov3); pa.wcnt = (uint32_t*)(gb + L.wcnt);
-    if (P > 0 && R > 0) GS_HIP(gs::launch_preprocess_fwd(d, pa, s));
+    if (P > 0 && R > 0) GS_LAUNCH("preprocess_fwd", gs::launch_preprocess_fwd(d, pa, s), s);
     gs::BinArgs ba{};
     ba.wcnt = pa.wcnt;
     ba.segcnt = (uint32_t*)(ib + L.segcnt);
@@ -146,7 +190,8 @@ int gsparc_rasterize_forward_geom(int P, int R, int W, int H, const float* means
     ba.ranges = (uint2*)(ib + L.ranges);
     ba.total = (uint32_t*)(ib + L.total);
     if (R > 0) {
-        GS_HIP(gs::launch_bin_scan(d, ba, s));
+        GS_LAUNCH("bin_colscan", gs::launch_colscan(d, ba, s), s);
+        GS_LAUNCH("bin_segscan", gs::launch_segscan(d, ba, s), s);
         uint32_t tot[2] = {0, 0};
         GS_HIP(hipMemcpyAsync(tot, ba.total, sizeof(tot), hipMemcpyDeviceToHost, s));
         GS_HIP(hipStreamSynchronize(s));
@@ -188,7 +233,10 @@ int gsparc_rasterize_forward_render(int P, int R, int W, int H, const float* emi
     ba.plist = (uint32_t*)(bb + L.plist);
     ba.keys_sorted = (flags & GSPARC_FLAG_KEEP_KEYS) ? (uint32_t*)(bb + L.keys_sorted) : nullptr;
     ba.Kcap = K_capacity;
-    if (P > 0) GS_HIP(gs::launch_bin_scatter_sort(d, ba, s));
+    if (P > 0) {
+        GS_LAUNCH("bin_scatter", gs::launch_scatter(d, ba, s), s);
+        GS_LAUNCH("bin_sort", gs::launch_sort(d, ba, s), s);
+    }
     gs::RenderFwdArgs ra;
     ra.ranges = ba.ranges;
     ra.plist = ba.plist;
@@ -200,7 +248,8 @@ int gsparc_rasterize_forward_render(int P, int R, int W, int H, const float* emi
     ra.ncontrib = (uint32_t*)(ib + L.n_contrib);
     ra.total = ba.total;
     ra.Kcap = K_capacity;
-    GS_HIP(gs::launch_render_fwd(d, ra, s));
+    ra.pairs = g_prof_on ? g_pairs_dev : nullptr;
+    GS_LAUNCH("render_fwd", gs::launch_render_fwd(d, ra, s), s);
     return GSPARC_OK;
 }
 
@@ -228,7 +277,7 @@ int gsparc_rasterize_backward(int grad_mode, int P, int R, int W, int H, const f
     char* ib = base_of(image);
     char* bb = base_of(bin);
     float* g8 = (float*)base_of(grad);
-    GS_HIP(gs::launch_fill_u32((uint32_t*)g8, 0u, (size_t)R * P * 8, s));
+    GS_LAUNCH("grad_clear", gs::launch_fill_u32((uint32_t*)g8, 0u, (size_t)R * P * 8, s), s);
     gs::RenderBwdArgs ra;
     ra.ranges = (const uint2*)(ib + L.ranges);
     ra.plist = (const uint32_t*)(bb + L.plist);
@@ -242,13 +291,13 @@ int gsparc_rasterize_backward(int grad_mode, int P, int R, int W, int H, const f
     ra.mode = grad_mode;
     ra.total = (const uint32_t*)(ib + L.total);
     ra.Kcap = K_capacity;
-    GS_HIP(gs::launch_render_bwd(d, ra, s));
+    GS_LAUNCH("render_bwd", gs::launch_render_bwd(d, ra, s), s);
     gs::PreBwdArgs pa;
     pa.means = means; pa.scales = scales; pa.rots = rotations; pa.mlp = emission_mlps; pa.views = viewmatrices;
     pa.tx = tx_pos; pa.rx = rx_pos; pa.mod = scale_modifier; pa.radii = radii; pa.g8 = g8;
     pa.dmeans = dmeans3D; pa.dscales = dscales; pa.drots = drotations; pa.dopac = dopacity;
     pa.dmlp = demission_mlps; pa.dcov3 = dcov3D; pa.dmeans2d = dmeans2D; pa.dsig = demission; pa.mode = grad_mode;
-    GS_HIP(gs::launch_preprocess_bwd(d, pa, s));
+    GS_LAUNCH("preprocess_bwd", gs::launch_preprocess_bwd(d, pa, s), s);
     return GSPARC_OK;
 }
 
@@ -272,7 +321,7 @@ int gsparc_ssim_forward(int planes, int H, int W, float C1, float C2, const floa
     if (!img1 || !img2 || !ssim_map) return fail(GSPARC_ERR_ARG, "null pointer");
     if (dm_dmu1 && (!dm_dsigma1_sq || !dm_dsigma12)) return fail(GSPARC_ERR_ARG, "partial derivative maps");
     gs::SsimArgs a{planes, H, W, C1, C2, img1, img2, ssim_map, dm_dmu1, dm_dsigma1_sq, dm_dsigma12};
-    GS_HIP(gs::launch_ssim_fwd(a, (hipStream_t)stream));
+    GS_LAUNCH("ssim_fwd", gs::launch_ssim_fwd(a, (hipStream_t)stream), (hipStream_t)stream);
     return GSPARC_OK;
 }
 
@@ -287,7 +336,66 @@ int gsparc_ssim_backward(int planes, int H, int W, float C1, float C2, const flo
     if (!img1 || !img2 || !dL_dmap || !dm_dmu1 || !dm_dsigma1_sq || !dm_dsigma12 || !dL_dimg1)
         return fail(GSPARC_ERR_ARG, "null pointer");
     gs::SsimBwdArgs a{planes, H, W, img1, img2, dL_dmap, dm_dmu1, dm_dsigma1_sq, dm_dsigma12, dL_dimg1};
-    GS_HIP(gs::launch_ssim_bwd(a, (hipStream_t)stream));
+    GS_LAUNCH("ssim_bwd", gs::launch_ssim_bwd(a, (hipStream_t)stream), (hipStream_t)stream);
+    return GSPARC_OK;
+}
+
+int gsparc_profile_enable(int on)
+{
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    for (auto& p : g_prof_pending) {
+        g_prof_free.push_back(p.a);
+        g_prof_free.push_back(p.b);
+    }
+    g_prof_pending.clear();
+    g_prof_acc.clear();
+    if (on && !g_pairs_dev) GS_HIP(hipMalloc((void**)&g_pairs_dev, sizeof(unsigned long long)));
+    if (on) GS_HIP(hipMemset(g_pairs_dev, 0, sizeof(unsigned long long)));
+    g_prof_on = on != 0;
+    return GSPARC_OK;
+}
+
+int gsparc_profile_collect(int* n_kernels)
+{
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    for (auto& p : g_prof_pending) {
+        GS_HIP(hipEventSynchronize(p.b));
+        float ms = 0.0f;
+        GS_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+        auto& acc = g_prof_acc[p.name];
+        acc.first += ms;
+        acc.second += 1;
+        g_prof_free.push_back(p.a);
+        g_prof_free.push_back(p.b);
+    }
+    g_prof_pending.clear();
+    if (n_kernels) *n_kernels = (int)g_prof_acc.size();
+    return GSPARC_OK;
+}
+
+int gsparc_profile_query(int i, char* name, int name_len, double* total_ms, long long* launches)
+{
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    if (i < 0 || i >= (int)g_prof_acc.size()) return fail(GSPARC_ERR_ARG, "profile index out of range");
+    auto it = g_prof_acc.begin();
+    std::advance(it, i);
+    if (name && name_len > 0) {
+        std::strncpy(name, it->first.c_str(), (size_t)name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    if (total_ms) *total_ms = it->second.first;
+    if (launches) *launches = it->second.second;
+    return GSPARC_OK;
+}
+
+int gsparc_profile_pairs(uint64_t* pairs)
+{
+    if (!pairs) return fail(GSPARC_ERR_ARG, "null pointer");
+    *pairs = 0;
+    if (!g_pairs_dev) return GSPARC_OK;
+    unsigned long long v = 0;
+    GS_HIP(hipMemcpy(&v, g_pairs_dev, sizeof(v), hipMemcpyDeviceToHost));
+    *pairs = v;
     return GSPARC_OK;
 }
 

@@ -82,7 +82,9 @@ __global__ void __launch_bounds__(1024) k_segscan(int n, const uint32_t* segcnt,
     for (int i = i0; i < i1; ++i) {
         const uint32_t c = segcnt[i];
         segbase[i] = run;
-        ranges[i] = make_uint2(run, run + c);
+        // empty tiles keep the range (0, 0) as after the reference's memset
+        // (rasterizer_impl.cu:303)
+        ranges[i] = c ? make_uint2(run, run + c) : make_uint2(0u, 0u);
         run += c;
     }
     // max segment length
@@ -359,23 +361,31 @@ __global__ void __launch_bounds__(GS_SORT_THREADS) k_sort(BinArgs a, int nseg)
     }
 }
 
-hipError_t launch_bin_scan(const Dims& d, const BinArgs& a, hipStream_t s)
+hipError_t launch_colscan(const Dims& d, const BinArgs& a, hipStream_t s)
 {
-    const int nseg = d.R * d.T;
-    if (d.NB > 0) {
+    if (d.R == 0) return hipSuccess;
+    if (d.NB > 0)
         hipLaunchKernelGGL(k_colscan, dim3((d.T + GS_WAVE - 1) / GS_WAVE, d.R), dim3(256), 0, s, d, a.wcnt, a.segcnt);
-    } else {
-        hipError_t e = launch_fill_u32(a.segcnt, 0u, (size_t)nseg, s);
-        if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL(k_segscan, dim3(1), dim3(1024), 0, s, nseg, a.segcnt, a.segbase, a.ranges, a.total);
+    else
+        return launch_fill_u32(a.segcnt, 0u, (size_t)d.R * d.T, s);
     return hipGetLastError();
 }
 
-hipError_t launch_bin_scatter_sort(const Dims& d, const BinArgs& a, hipStream_t s)
+hipError_t launch_segscan(const Dims& d, const BinArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_segscan, dim3(1), dim3(1024), 0, s, d.R * d.T, a.segcnt, a.segbase, a.ranges, a.total);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter(const Dims& d, const BinArgs& a, hipStream_t s)
 {
     if (d.NB > 0 && d.R > 0)
         hipLaunchKernelGGL(k_scatter, dim3(d.NB, d.R), dim3(GS_PRE_THREADS), 0, s, d, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_sort(const Dims& d, const BinArgs& a, hipStream_t s)
+{
     const int nseg = d.R * d.T;
     if (nseg > 0) {
         // the sort stages up to GS_SORT_CHUNK instances in LDS (about 136 KiB of

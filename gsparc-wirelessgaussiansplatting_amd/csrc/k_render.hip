@@ -88,6 +88,12 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
         }
         __syncthreads();
     }
+    if (a.pairs) {
+        unsigned long long c = contributor;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off);
+        if (lane == 0) atomicAdd(a.pairs, c);
+    }
     if (inside) {
         const size_t p = (size_t)r * d.W * d.H + (size_t)pm.py * d.W + pm.px;
         a.out_re[p] = are;

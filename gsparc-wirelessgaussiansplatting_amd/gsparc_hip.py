@@ -50,6 +50,10 @@ _SIGS = {
     "gsparc_mark_visible": (_i, [_i, _vp, _vp, _vp, _vp]),
     "gsparc_ssim_forward": (_i, [_i, _i, _i, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gsparc_ssim_backward": (_i, [_i, _i, _i, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "gsparc_profile_enable": (_i, [_i]),
+    "gsparc_profile_collect": (_i, [C.POINTER(_i)]),
+    "gsparc_profile_query": (_i, [_i, C.c_char_p, _i, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
+    "gsparc_profile_pairs": (_i, [C.POINTER(_u64)]),
 }
 
 _lib = None
@@ -87,6 +91,31 @@ def ptr(t: torch.Tensor | None):
 
 def stream_handle(device=None):
     return C.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def profile_enable(on: bool = True):
+    check(lib().gsparc_profile_enable(1 if on else 0), "gsparc_profile_enable")
+
+
+def profile_collect() -> dict:
+    """{kernel name: (total ms, launches)} since profile_enable(True)."""
+    nk = C.c_int(0)
+    check(lib().gsparc_profile_collect(C.byref(nk)), "gsparc_profile_collect")
+    n = nk.value
+    out = {}
+    buf = C.create_string_buffer(64)
+    for i in range(n):
+        ms = C.c_double(0.0)
+        cnt = C.c_longlong(0)
+        check(lib().gsparc_profile_query(i, buf, 64, C.byref(ms), C.byref(cnt)), "gsparc_profile_query")
+        out[buf.value.decode()] = (ms.value, cnt.value)
+    return out
+
+
+def profile_pairs() -> int:
+    v = C.c_uint64(0)
+    check(lib().gsparc_profile_pairs(C.byref(v)), "gsparc_profile_pairs")
+    return int(v.value)
 
 
 def layout(P, R, W, H, K=0) -> Layout:

@@ -124,6 +124,17 @@ int gsparc_ssim_backward(int planes, int H, int W, float C1, float C2, const flo
                          const float* dL_dmap, const float* dm_dmu1, const float* dm_dsigma1_sq,
                          const float* dm_dsigma12, float* dL_dimg1, void* stream);
 
+/* Per-kernel timing.  While enabled, every kernel launched by the calls above
+ * is bracketed by HIP events on its own stream; collect() waits for them,
+ * adds the elapsed times to per-kernel totals and returns how many kernels
+ * have totals; query() reads entry i (sorted by kernel name).  The forward blend also counts the (pixel, Gaussian) pairs it
+ * visits (profile_pairs).  No counterpart in the reference (which only times
+ * render() on the host, train.py:381-384). */
+int gsparc_profile_enable(int on);
+int gsparc_profile_collect(int* n_kernels);
+int gsparc_profile_query(int i, char* name, int name_len, double* total_ms, long long* launches);
+int gsparc_profile_pairs(uint64_t* pairs);
+
 #ifdef __cplusplus
 }
 #endif
