@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--latency-iters", type=int, default=100)
     ap.add_argument("--cpu-receivers", type=int, default=48, help="receivers in the CPU baseline sample (0: skip)")
     ap.add_argument("--grad-mode", default="reference", choices=["reference", "exact"])
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run a plain device-to-device copy of this size (counter calibration)")
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
 
@@ -125,11 +127,13 @@ def main():
     value = R * args.steps / el
     log(args, f"batched: {ms_per_step:.2f} ms/step, {value:.1f} spectra/s")
 
-    step_plain()
-    log(args, f"timing {args.plain_steps} plain steps (one receiver per call)")
-    el_plain = timed(step_plain, args.plain_steps)
-    plain_value = R * args.plain_steps / el_plain
-    log(args, f"plain: {1e3 * el_plain / args.plain_steps:.2f} ms/step, {plain_value:.1f} spectra/s")
+    plain_value = None
+    if args.plain_steps > 0:
+        step_plain()
+        log(args, f"timing {args.plain_steps} plain steps (one receiver per call)")
+        el_plain = timed(step_plain, args.plain_steps)
+        plain_value = R * args.plain_steps / el_plain
+        log(args, f"plain: {1e3 * el_plain / args.plain_steps:.2f} ms/step, {plain_value:.1f} spectra/s")
 
     # single-spectrum forward latency (one receiver, reference API)
     lat = []
@@ -142,7 +146,13 @@ def main():
             torch.cuda.synchronize()
             if i >= 5:
                 lat.append(1e3 * (time.perf_counter() - t0))
-    lat = np.asarray(lat)
+    lat = np.asarray(lat if lat else [float("nan")])
+    if args.copy_mib > 0:
+        src = torch.empty(args.copy_mib << 18, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(args, f"calibration copy of {args.copy_mib} MiB done")
 
     # per-kernel times with HIP events on the launch stream
     gh.profile_enable(True)
@@ -233,8 +243,8 @@ def main():
         "config": {"workload": "batched receivers, fwd+bwd, 0.8*L1 + 0.2*(1-SSIM)", "gaussians": P,
                    "receivers": R, "map": f"{sc.H}x{sc.W}", "emission": "per-Gaussian MLP 3->32->2",
                    "grad_mode": args.grad_mode},
-        "plain_spectra_per_s": round(plain_value, 2),
-        "batched_over_plain": round(value / plain_value, 2),
+        "plain_spectra_per_s": None if plain_value is None else round(plain_value, 2),
+        "batched_over_plain": None if plain_value is None else round(value / plain_value, 2),
         "single_spectrum_fwd_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                                    "p99": round(float(np.percentile(lat, 99)), 3)},
         "K_instances": int(K),

@@ -10,11 +10,9 @@
 //   sig[r*P + g]    = float2 {real, imag} emission
 //   depth[r*P + g]  = float distance to the receiver (sort key)
 //   radii[r*P + g]  = int, 0 = culled
-//   wcnt[(r*NB + b)*T + t] = Gaussians of block b (256 Gaussians) that
-//                     touch tile t of receiver r; after the column scan, the
-//                     exclusive prefix over b
-//   segbase[r*T + t], segcnt[r*T + t] = tile range (global offsets)
-//   keys/vals[K]    = depth bits / Gaussian index, tile-major then idx order
+//   segcnt[r*T + t] = instances of tile t of receiver r; segbase = its
+//                     exclusive prefix over (r, t); cursor = scatter fill count
+//   keys/vals[K]    = depth bits / Gaussian index, grouped by (receiver, tile)
 //   plist[K]        = Gaussian index, sorted by (receiver, tile, depth, idx)
 //   finalT[r*N + p], ncontrib[r*N + p] = per-pixel blend state
 #pragma once

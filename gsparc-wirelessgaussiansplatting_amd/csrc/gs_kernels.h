@@ -10,8 +10,12 @@ namespace gs {
 struct Dims {
     int P, R, W, H;
     int gx, gy, T;   // tile grid
-    int NB;          // preprocess blocks of 256 Gaussians
+    int NB;          // blocks of 256 Gaussians
 };
+
+// Tile counts are aggregated in LDS when the map has at most this many tiles
+// (90x360 has 138); larger maps use global atomics directly.
+#define GS_LDS_TILES 1024
 
 inline Dims make_dims(int P, int R, int W, int H)
 {
@@ -37,7 +41,7 @@ struct PreFwdArgs {
     float* depth;            // [R*P]
     int* radii;              // [R*P]
     float* cov3;             // [P*6]
-    uint32_t* wcnt;          // [R*NB*T]
+    uint32_t* segcnt;        // [R*T] instance counts per tile (zeroed before)
 };
 hipError_t launch_preprocess_fwd(const Dims& d, const PreFwdArgs& a, hipStream_t s);
 
@@ -45,8 +49,8 @@ struct BinArgs {
     const float4* splat;     // for px, py
     const int* radii;
     const float* depth;
-    uint32_t* wcnt;          // [R*NB*T] counts in, exclusive prefix out
     uint32_t* segcnt;        // [R*T]
+    uint32_t* cursor;        // [R*T] fill cursors of the scatter (zeroed before)
     uint32_t* segbase;       // [R*T]
     uint2* ranges;           // [R*T]
     uint32_t* total;         // [2]: K, max segment length
@@ -59,7 +63,6 @@ struct BinArgs {
     uint32_t* keys_sorted;   // optional [Kcap] sorted depth bits
     uint64_t Kcap;
 };
-hipError_t launch_colscan(const Dims& d, const BinArgs& a, hipStream_t s);
 hipError_t launch_segscan(const Dims& d, const BinArgs& a, hipStream_t s);
 hipError_t launch_scatter(const Dims& d, const BinArgs& a, hipStream_t s);
 hipError_t launch_sort(const Dims& d, const BinArgs& a, hipStream_t s);
@@ -77,7 +80,10 @@ struct RenderFwdArgs {
     uint64_t Kcap;           // capacity of plist (nothing is read when K > Kcap)
     unsigned long long* pairs;   // optional: += (pixel, Gaussian) pairs visited
 };
-hipError_t launch_render_fwd(const Dims& d, const RenderFwdArgs& a, hipStream_t s);
+// pixels per lane of the blend kernels: 1 (8x8 quarter-tile per wave) or 4
+// (16x16 tile per wave); requested = 0 picks by the number of tiles
+int choose_ppl(const Dims& d, int requested);
+hipError_t launch_render_fwd(const Dims& d, const RenderFwdArgs& a, int ppl, hipStream_t s);
 
 struct RenderBwdArgs {
     const uint2* ranges;
@@ -93,7 +99,7 @@ struct RenderBwdArgs {
     const uint32_t* total;
     uint64_t Kcap;
 };
-hipError_t launch_render_bwd(const Dims& d, const RenderBwdArgs& a, hipStream_t s);
+hipError_t launch_render_bwd(const Dims& d, const RenderBwdArgs& a, int ppl, hipStream_t s);
 
 struct PreBwdArgs {
     const float* means;

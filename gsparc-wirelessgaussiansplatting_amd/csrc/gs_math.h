@@ -96,6 +96,27 @@ GS_HD float gs_expf(float x)
     return (p * gs_pow2i(n1)) * gs_pow2i(n2);
 }
 
+// gs_expf for x <= 0 without the special-case branches: the argument is
+// clamped to -104, where the core path already rounds to +0, so the result is
+// bit-identical to gs_expf for every x <= 0 (checked by the test suite).
+GS_HD float gs_expf_nonpos(float x)
+{
+    x = x > -104.0f ? x : -104.0f;
+    const float n = __builtin_rintf(x * GS_LOG2E);
+    float r = __builtin_fmaf(-n, GS_LN2_HI, x);
+    r = __builtin_fmaf(-n, GS_LN2_LO, r);
+    float q = GS_EXP_Q_4;
+    q = __builtin_fmaf(q, r, GS_EXP_Q_3);
+    q = __builtin_fmaf(q, r, GS_EXP_Q_2);
+    q = __builtin_fmaf(q, r, GS_EXP_Q_1);
+    q = __builtin_fmaf(q, r, GS_EXP_Q_0);
+    const float p = __builtin_fmaf(r * r, q, r) + 1.0f;
+    const int ni = (int)n;
+    const int n1 = ni >> 1;
+    const int n2 = ni - n1;
+    return (p * gs_pow2i(n1)) * gs_pow2i(n2);
+}
+
 GS_HD float gs_logf(float x)
 {
     if (x != x) return x;

@@ -17,6 +17,9 @@ namespace {
 
 thread_local std::string g_err;
 
+// tuning options (gsparc_set_option)
+int g_opt_render_ppl = 0;   // 0 = automatic, 1 or 4
+
 int fail(int code, const std::string& msg)
 {
     g_err = msg;
@@ -41,13 +44,13 @@ void make_layout(int P, int R, int W, int H, uint64_t K, gsparc_layout& L)
     L.sig = o; o = al(o + RP * sizeof(float2));
     L.depth = o; o = al(o + RP * sizeof(float));
     L.cov3 = o; o = al(o + (size_t)P * 6 * sizeof(float));
-    L.wcnt = o; o = al(o + (size_t)R * d.NB * d.T * sizeof(uint32_t));
     L.geom_bytes = o + ALIGN;
     o = 0;
     L.final_t = o; o = al(o + (size_t)R * N * sizeof(float));
     L.n_contrib = o; o = al(o + (size_t)R * N * sizeof(uint32_t));
     L.ranges = o; o = al(o + RT * sizeof(uint2));
-    L.segcnt = o; o = al(o + RT * sizeof(uint32_t));
+    L.segcnt = o; o = o + RT * sizeof(uint32_t);   // segcnt and cursor are adjacent:
+    L.cursor = o; o = al(o + RT * sizeof(uint32_t)); // one fill clears both
     L.segbase = o; o = al(o + RT * sizeof(uint32_t));
     L.total = o; o = al(o + 4 * sizeof(uint32_t));
     L.image_bytes = o + ALIGN;
@@ -131,6 +134,27 @@ extern "C" {
 const char* gsparc_version(void) { return "gsparc-mi355x 0.1 (gfx950)"; }
 const char* gsparc_last_error(void) { return g_err.c_str(); }
 
+int gsparc_set_option(const char* name, int value)
+{
+    if (!name) return fail(GSPARC_ERR_ARG, "null option name");
+    if (std::strcmp(name, "render_ppl") == 0) {
+        if (value != 0 && value != 1 && value != 4) return fail(GSPARC_ERR_ARG, "render_ppl must be 0, 1 or 4");
+        g_opt_render_ppl = value;
+        return GSPARC_OK;
+    }
+    return fail(GSPARC_ERR_ARG, std::string("unknown option ") + name);
+}
+
+int gsparc_get_option(const char* name, int* value)
+{
+    if (!name || !value) return fail(GSPARC_ERR_ARG, "null pointer");
+    if (std::strcmp(name, "render_ppl") == 0) {
+        *value = g_opt_render_ppl;
+        return GSPARC_OK;
+    }
+    return fail(GSPARC_ERR_ARG, std::string("unknown option ") + name);
+}
+
 size_t gsparc_geom_buffer_size(int P, int R, int W, int H)
 {
     gsparc_layout L;
@@ -181,16 +205,16 @@ int gsparc_rasterize_forward_geom(int P, int R, int W, int H, const float* means
     pa.means = means; pa.scales = scales; pa.rots = rotations; pa.opac = opacities; pa.mlp = emission_mlps;
     pa.views = viewmatrices; pa.tx = tx_pos; pa.rx = rx_pos; pa.mod = scale_modifier;
     pa.splat = (float4*)(gb + L.splat); pa.sig = (float2*)(gb + L.sig); pa.depth = (float*)(gb + L.depth);
-    pa.radii = radii_out; pa.cov3 = (float*)(gb + L.cov3); pa.wcnt = (uint32_t*)(gb + L.wcnt);
+    pa.radii = radii_out; pa.cov3 = (float*)(gb + L.cov3); pa.segcnt = (uint32_t*)(ib + L.segcnt);
+    if (R > 0) GS_LAUNCH("clear_counts", gs::launch_fill_u32(pa.segcnt, 0u, (size_t)2 * R * d.T, s), s);
     if (P > 0 && R > 0) GS_LAUNCH("preprocess_fwd", gs::launch_preprocess_fwd(d, pa, s), s);
     gs::BinArgs ba{};
-    ba.wcnt = pa.wcnt;
     ba.segcnt = (uint32_t*)(ib + L.segcnt);
+    ba.cursor = (uint32_t*)(ib + L.cursor);
     ba.segbase = (uint32_t*)(ib + L.segbase);
     ba.ranges = (uint2*)(ib + L.ranges);
     ba.total = (uint32_t*)(ib + L.total);
     if (R > 0) {
-        GS_LAUNCH("bin_colscan", gs::launch_colscan(d, ba, s), s);
         GS_LAUNCH("bin_segscan", gs::launch_segscan(d, ba, s), s);
         uint32_t tot[2] = {0, 0};
         GS_HIP(hipMemcpyAsync(tot, ba.total, sizeof(tot), hipMemcpyDeviceToHost, s));
@@ -221,8 +245,8 @@ int gsparc_rasterize_forward_render(int P, int R, int W, int H, const float* emi
     ba.splat = (const float4*)(gb + L.splat);
     ba.radii = radii;
     ba.depth = (const float*)(gb + L.depth);
-    ba.wcnt = (uint32_t*)(gb + L.wcnt);
     ba.segcnt = (uint32_t*)(ib + L.segcnt);
+    ba.cursor = (uint32_t*)(ib + L.cursor);
     ba.segbase = (uint32_t*)(ib + L.segbase);
     ba.ranges = (uint2*)(ib + L.ranges);
     ba.total = (uint32_t*)(ib + L.total);
@@ -249,7 +273,7 @@ int gsparc_rasterize_forward_render(int P, int R, int W, int H, const float* emi
     ra.total = ba.total;
     ra.Kcap = K_capacity;
     ra.pairs = g_prof_on ? g_pairs_dev : nullptr;
-    GS_LAUNCH("render_fwd", gs::launch_render_fwd(d, ra, s), s);
+    GS_LAUNCH("render_fwd", gs::launch_render_fwd(d, ra, gs::choose_ppl(d, g_opt_render_ppl), s), s);
     return GSPARC_OK;
 }
 
@@ -291,7 +315,7 @@ int gsparc_rasterize_backward(int grad_mode, int P, int R, int W, int H, const f
     ra.mode = grad_mode;
     ra.total = (const uint32_t*)(ib + L.total);
     ra.Kcap = K_capacity;
-    GS_LAUNCH("render_bwd", gs::launch_render_bwd(d, ra, s), s);
+    GS_LAUNCH("render_bwd", gs::launch_render_bwd(d, ra, gs::choose_ppl(d, g_opt_render_ppl), s), s);
     gs::PreBwdArgs pa;
     pa.means = means; pa.scales = scales; pa.rots = rotations; pa.mlp = emission_mlps; pa.views = viewmatrices;
     pa.tx = tx_pos; pa.rx = rx_pos; pa.mod = scale_modifier; pa.radii = radii; pa.g8 = g8;

@@ -4,46 +4,23 @@
 // (rasterizer_impl.cu:102-171, 270-312) and produce the same ordering:
 // within a tile, Gaussians by ascending float depth bits, ties by index.
 //
-//   k_colscan   exclusive prefix over preprocess blocks of the per-tile counts
-//   k_segscan   exclusive prefix over (receiver, tile) -> ranges, K
+//   k_segscan   exclusive prefix over (receiver, tile) of the per-tile counts
+//               (accumulated by the preprocess) -> ranges, K
 //   k_scatter   writes (depth bits, index) of every instance into its tile
-//               segment, in Gaussian-index order (wave ballots, no atomics)
-//   k_sort      one workgroup per tile segment: stable LSD radix sort on the
-//               depth bits (4-bit digits, ballot ranking), in LDS when the
-//               segment fits and chunk by chunk in global memory otherwise.
+//               segment: per-block LDS counts, one global atomic per touched
+//               tile to reserve space, LDS atomics for the slots
+//   k_sort      one workgroup per tile segment: LSD radix sort on the depth
+//               bits (4-bit digits, ballot ranking), in LDS when the segment
+//               fits and chunk by chunk in global memory otherwise, then runs
+//               of equal depth are put in index order (the order the
+//               reference's stable sort gives, since its input is in index
+//               order).
 #include "gs_device.h"
 #include "gs_kernels.h"
 
 namespace gs {
 
 // ---------------------------------------------------------------- scans
-__global__ void __launch_bounds__(256) k_colscan(Dims d, uint32_t* wcnt, uint32_t* segcnt)
-{
-    __shared__ uint32_t s_part[4][GS_WAVE];
-    const int wave = threadIdx.x / GS_WAVE, lane = threadIdx.x % GS_WAVE;
-    const int r = blockIdx.y;
-    const int t = blockIdx.x * GS_WAVE + lane;
-    const bool ok = t < d.T;
-    const int per = (d.NB + 3) / 4;
-    const int b0 = min(d.NB, wave * per), b1 = min(d.NB, b0 + per);
-    uint32_t* col = wcnt + (size_t)r * d.NB * d.T + t;
-    uint32_t sum = 0;
-    if (ok)
-        for (int b = b0; b < b1; ++b) sum += col[(size_t)b * d.T];
-    s_part[wave][lane] = sum;
-    __syncthreads();
-    uint32_t run = 0;
-    for (int w = 0; w < wave; ++w) run += s_part[w][lane];
-    if (ok) {
-        for (int b = b0; b < b1; ++b) {
-            const uint32_t v = col[(size_t)b * d.T];
-            col[(size_t)b * d.T] = run;
-            run += v;
-        }
-        if (wave == 3) segcnt[(size_t)r * d.T + t] = run;
-    }
-}
-
 // Block-wide exclusive scan of one value per thread (1024 threads max).
 template <int NT>
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, uint32_t& total)
@@ -103,13 +80,14 @@ __global__ void __launch_bounds__(1024) k_segscan(int n, const uint32_t* segcnt,
 // ---------------------------------------------------------------- scatter
 __global__ void __launch_bounds__(GS_PRE_THREADS) k_scatter(Dims d, BinArgs a)
 {
-    __shared__ uint32_t s_cnt[GS_PRE_THREADS / GS_WAVE][GS_WAVE];
-    __shared__ uint32_t s_base[GS_PRE_THREADS / GS_WAVE][GS_WAVE];
+    __shared__ uint32_t s_cnt[GS_LDS_TILES];
+    __shared__ uint32_t s_base[GS_LDS_TILES];
     if ((uint64_t)a.total[0] > a.Kcap) return;    // capacity overflow: reported by the host
-    const int wave = threadIdx.x / GS_WAVE, lane = threadIdx.x % GS_WAVE;
+    const bool lds_tiles = d.T <= GS_LDS_TILES;
     const int b = blockIdx.x, r = blockIdx.y;
     const int g = b * GS_PRE_THREADS + threadIdx.x;
     const size_t ri = (size_t)r * d.P + g;
+    const size_t rT = (size_t)r * d.T;
     int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
     uint32_t dbits = 0;
     if (g < d.P) {
@@ -120,53 +98,34 @@ __global__ void __launch_bounds__(GS_PRE_THREADS) k_scatter(Dims d, BinArgs a)
             dbits = gs_as_uint(a.depth[ri]);
         }
     }
-    const bool has = x1 > x0 && y1 > y0;
-    const uint64_t any = ballot(has);
-    int bx0 = has ? x0 : 0x7fffffff, by0 = has ? y0 : 0x7fffffff, bx1 = has ? x1 : 0, by1 = has ? y1 : 0;
-    for (int off = 32; off >= 1; off >>= 1) {
-        bx0 = min(bx0, __shfl_xor(bx0, off));
-        by0 = min(by0, __shfl_xor(by0, off));
-        bx1 = max(bx1, __shfl_xor(bx1, off));
-        by1 = max(by1, __shfl_xor(by1, off));
+    if (!lds_tiles) {
+        for (int ty = y0; ty < y1; ++ty)
+            for (int tx = x0; tx < x1; ++tx) {
+                const int t = ty * d.gx + tx;
+                const uint32_t pos = a.segbase[rT + t] + atomicAdd(&a.cursor[rT + t], 1u);
+                a.keys[pos] = dbits;
+                a.vals[pos] = (uint32_t)g;
+            }
+        return;
     }
-    const uint64_t lt = lanemask_lt();
-    for (int c0 = 0; c0 < d.T; c0 += GS_WAVE) {
-        const int cend = min(d.T, c0 + GS_WAVE);
-        uint32_t mine = 0;
-        if (any) {
-            for (int t = c0; t < cend; ++t) {
-                const int tx = t % d.gx, ty = t / d.gx;
-                if (tx < bx0 || tx >= bx1 || ty < by0 || ty >= by1) continue;
-                const uint64_t m = ballot(tx >= x0 && tx < x1 && ty >= y0 && ty < y1);
-                if (lane == t - c0) mine = (uint32_t)__popcll(m);
-            }
-        }
-        s_cnt[wave][lane] = mine;
-        __syncthreads();
-        {
-            const int t = c0 + lane;
-            if (t < d.T) {
-                uint32_t base = a.segbase[(size_t)r * d.T + t] + a.wcnt[((size_t)r * d.NB + b) * d.T + t];
-                for (int w = 0; w < wave; ++w) base += s_cnt[w][lane];
-                s_base[wave][lane] = base;
-            }
-        }
-        __syncthreads();
-        if (any) {
-            for (int t = c0; t < cend; ++t) {
-                const int tx = t % d.gx, ty = t / d.gx;
-                if (tx < bx0 || tx >= bx1 || ty < by0 || ty >= by1) continue;
-                const bool in = tx >= x0 && tx < x1 && ty >= y0 && ty < y1;
-                const uint64_t m = ballot(in);
-                if (in) {
-                    const uint32_t pos = s_base[wave][t - c0] + (uint32_t)__popcll(m & lt);
-                    a.keys[pos] = dbits;
-                    a.vals[pos] = (uint32_t)g;
-                }
-            }
-        }
-        __syncthreads();
+    for (int i = threadIdx.x; i < d.T; i += GS_PRE_THREADS) s_cnt[i] = 0;
+    __syncthreads();
+    for (int ty = y0; ty < y1; ++ty)
+        for (int tx = x0; tx < x1; ++tx) atomicAdd(&s_cnt[ty * d.gx + tx], 1u);
+    __syncthreads();
+    for (int t = threadIdx.x; t < d.T; t += GS_PRE_THREADS) {
+        const uint32_t c = s_cnt[t];
+        if (c) s_base[t] = a.segbase[rT + t] + atomicAdd(&a.cursor[rT + t], c);
+        s_cnt[t] = 0;
     }
+    __syncthreads();
+    for (int ty = y0; ty < y1; ++ty)
+        for (int tx = x0; tx < x1; ++tx) {
+            const int t = ty * d.gx + tx;
+            const uint32_t pos = s_base[t] + atomicAdd(&s_cnt[t], 1u);
+            a.keys[pos] = dbits;
+            a.vals[pos] = (uint32_t)g;
+        }
 }
 
 // ---------------------------------------------------------------- sort
@@ -174,7 +133,7 @@ __global__ void __launch_bounds__(GS_PRE_THREADS) k_scatter(Dims d, BinArgs a)
 #define GS_SORT_WAVES (GS_SORT_THREADS / GS_WAVE)
 #define GS_SORT_ROWS 16
 #define GS_SORT_CHUNK (GS_SORT_THREADS * GS_SORT_ROWS)   // 8192 instances
-#define GS_SORT_TABLE (16 * GS_SORT_ROWS * GS_SORT_WAVES)
+#define GS_SORT_TABLE (16 * (GS_SORT_ROWS * GS_SORT_WAVES + 1))
 
 struct SortShared {
     uint32_t k[2][GS_SORT_CHUNK];
@@ -184,6 +143,7 @@ struct SortShared {
     uint32_t base[16];
     uint32_t wscan[GS_SORT_WAVES];
     uint32_t red[2][GS_SORT_WAVES];
+    uint32_t flag;
 };
 
 // Ballot-based stable ranking of 4-bit digits for one row of 512 elements.
@@ -239,7 +199,8 @@ __device__ void radix_pass(SortShared& sm, int n, KP sk, VP sv, DKP dk, DVP dv, 
         const int e0 = ch * GS_SORT_CHUNK;
         const int cn = min(GS_SORT_CHUNK, n - e0);
         const int rows = (cn + GS_SORT_THREADS - 1) / GS_SORT_THREADS;
-        const int S = 16 * rows * GS_SORT_WAVES;
+        const int DS = rows * GS_SORT_WAVES + 1;   // digit stride, padded against bank conflicts
+        const int S = 16 * DS;
         for (int i = tid; i < S; i += GS_SORT_THREADS) sm.table[i] = 0;
         __syncthreads();
         for (int i = 0; i < rows; ++i) {
@@ -248,7 +209,7 @@ __device__ void radix_pass(SortShared& sm, int n, KP sk, VP sv, DKP dk, DVP dv, 
             const uint32_t dg = valid ? ((sk[e0 + e] - kmin) >> shift) & 15u : 0u;
             const uint64_t m = digit_match(valid, dg);
             if (valid && (m & lt) == 0)
-                sm.table[(dg * rows + i) * GS_SORT_WAVES + wave] = (uint32_t)__popcll(m);
+                sm.table[dg * DS + i * GS_SORT_WAVES + wave] = (uint32_t)__popcll(m);
         }
         __syncthreads();
         // exclusive scan of the table in (digit, row, wave) order
@@ -278,8 +239,8 @@ __device__ void radix_pass(SortShared& sm, int n, KP sk, VP sv, DKP dk, DVP dv, 
             }
             const uint64_t m = digit_match(valid, dg);
             if (valid) {
-                uint32_t pos = sm.table[(dg * rows + i) * GS_SORT_WAVES + wave] + (uint32_t)__popcll(m & lt);
-                if (nchunks > 1) pos = pos - sm.table[dg * rows * GS_SORT_WAVES] + sm.base[dg];
+                uint32_t pos = sm.table[dg * DS + i * GS_SORT_WAVES + wave] + (uint32_t)__popcll(m & lt);
+                if (nchunks > 1) pos = pos - sm.table[dg * DS] + sm.base[dg];
                 dk[pos] = key;
                 dv[pos] = sv[e0 + e];
             }
@@ -287,13 +248,43 @@ __device__ void radix_pass(SortShared& sm, int n, KP sk, VP sv, DKP dk, DVP dv, 
         __syncthreads();
         if (nchunks > 1) {
             if (tid < 16) {
-                const uint32_t start = sm.table[tid * rows * GS_SORT_WAVES];
-                const uint32_t end = (tid < 15) ? sm.table[(tid + 1) * rows * GS_SORT_WAVES] : (uint32_t)cn;
+                const uint32_t start = sm.table[tid * DS];
+                const uint32_t end = (tid < 15) ? sm.table[(tid + 1) * DS] : (uint32_t)cn;
                 sm.base[tid] += end - start;
             }
             __syncthreads();
         }
     }
+}
+
+// Runs of equal depth bits come out of the radix passes in scatter order;
+// put each run in ascending Gaussian index, the order the reference's stable
+// sort produces from its index-ordered input.  Runs are rare and short.
+template <typename KP, typename VP>
+__device__ void order_ties(int n, KP k, VP v, uint32_t* s_flag)
+{
+    const int tid = threadIdx.x;
+    if (tid == 0) *s_flag = 0;
+    __syncthreads();
+    bool any = false;
+    for (int e = tid + 1; e < n; e += GS_SORT_THREADS) any |= (k[e] == k[e - 1]);
+    if (any) *s_flag = 1;
+    __syncthreads();
+    if (*s_flag) {
+        for (int e = tid; e < n; e += GS_SORT_THREADS) {
+            const bool starts = (e == 0 || k[e - 1] != k[e]) && e + 1 < n && k[e + 1] == k[e];
+            if (!starts) continue;
+            int end = e + 1;
+            while (end < n && k[end] == k[e]) ++end;
+            for (int i = e + 1; i < end; ++i) {        // insertion sort by index
+                const uint32_t x = v[i];
+                int j = i - 1;
+                while (j >= e && v[j] > x) { v[j + 1] = v[j]; --j; }
+                v[j + 1] = x;
+            }
+        }
+    }
+    __syncthreads();
 }
 
 __global__ void __launch_bounds__(GS_SORT_THREADS) k_sort(BinArgs a, int nseg)
@@ -339,6 +330,7 @@ __global__ void __launch_bounds__(GS_SORT_THREADS) k_sort(BinArgs a, int nseg)
             radix_pass(sm, n, sm.k[cur], sm.v[cur], sm.k[cur ^ 1], sm.v[cur ^ 1], kmin, 4 * p);
             cur ^= 1;
         }
+        order_ties(n, sm.k[cur], sm.v[cur], &sm.flag);
         for (int e = tid; e < n; e += GS_SORT_THREADS) {
             a.plist[base + e] = sm.v[cur][e];
             if (a.keys_sorted) a.keys_sorted[base + e] = sm.k[cur][e];
@@ -352,23 +344,14 @@ __global__ void __launch_bounds__(GS_SORT_THREADS) k_sort(BinArgs a, int nseg)
             else radix_pass(sm, n, tk, tv, gk, gv, kmin, 4 * p);
             cur ^= 1;
         }
-        const uint32_t* fk = cur ? tk : gk;
-        const uint32_t* fv = cur ? tv : gv;
+        uint32_t* fk = cur ? tk : gk;
+        uint32_t* fv = cur ? tv : gv;
+        order_ties(n, fk, fv, &sm.flag);
         for (int e = tid; e < n; e += GS_SORT_THREADS) {
             a.plist[base + e] = fv[e];
             if (a.keys_sorted) a.keys_sorted[base + e] = fk[e];
         }
     }
-}
-
-hipError_t launch_colscan(const Dims& d, const BinArgs& a, hipStream_t s)
-{
-    if (d.R == 0) return hipSuccess;
-    if (d.NB > 0)
-        hipLaunchKernelGGL(k_colscan, dim3((d.T + GS_WAVE - 1) / GS_WAVE, d.R), dim3(256), 0, s, d, a.wcnt, a.segcnt);
-    else
-        return launch_fill_u32(a.segcnt, 0u, (size_t)d.R * d.T, s);
-    return hipGetLastError();
 }
 
 hipError_t launch_segscan(const Dims& d, const BinArgs& a, hipStream_t s)

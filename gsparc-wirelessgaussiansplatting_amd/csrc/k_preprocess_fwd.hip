@@ -2,11 +2,10 @@
 //
 // One thread per Gaussian.  Receiver-independent work (cov3D, the log of the
 // transmitter distance, opacity threshold) is done once; the emission MLP is
-// evaluated for chunks of GS_RC receivers so that each Gaussian's 194 weights
-// are streamed once per chunk instead of once per receiver.  For every
-// receiver the block then counts, per tile, how many of its 256 Gaussians
-// touch the tile (wave ballots, so the later scatter can place instances in
-// Gaussian-index order without atomics).
+// evaluated for chunks of GS_RC receivers (one chunk per blockIdx.y) so that
+// each Gaussian's 194 weights are streamed once per chunk, not per receiver.
+// Per-tile instance counts are accumulated in an LDS histogram and flushed
+// with one global atomic per (block, receiver, touched tile).
 #include "gs_device.h"
 #include "gs_kernels.h"
 
@@ -15,12 +14,16 @@ namespace gs {
 #define GS_RC 8
 
 __global__ void __launch_bounds__(GS_PRE_THREADS)
-k_preprocess_fwd(Dims d, PreFwdArgs a, int NB)
+k_preprocess_fwd(Dims d, PreFwdArgs a)
 {
-    __shared__ uint32_t s_cnt[GS_PRE_THREADS / GS_WAVE][GS_WAVE];
+    __shared__ uint32_t s_hist[GS_RC][GS_LDS_TILES];
+    const bool lds_tiles = d.T <= GS_LDS_TILES;
+    if (lds_tiles) {
+        for (int i = threadIdx.x; i < GS_RC * d.T; i += GS_PRE_THREADS) s_hist[i / d.T][i % d.T] = 0;
+        __syncthreads();
+    }
     const int g = blockIdx.x * GS_PRE_THREADS + threadIdx.x;
     const bool valid = g < d.P;
-    const int wave = threadIdx.x / GS_WAVE, lane = threadIdx.x % GS_WAVE;
     const int P = d.P;
 
     float mean[3] = {0, 0, 0}, c3[6] = {0, 0, 0, 0, 0, 0}, o = 0.0f, ld = 0.0f;
@@ -29,8 +32,9 @@ k_preprocess_fwd(Dims d, PreFwdArgs a, int NB)
         const float s[3] = {a.scales[3 * g], a.scales[3 * g + 1], a.scales[3 * g + 2]};
         const float q[4] = {a.rots[4 * g], a.rots[4 * g + 1], a.rots[4 * g + 2], a.rots[4 * g + 3]};
         cov3d(s, a.mod, q, c3);
+        if (blockIdx.y == 0)
 #pragma unroll
-        for (int k = 0; k < 6; ++k) a.cov3[6 * g + k] = c3[k];
+            for (int k = 0; k < 6; ++k) a.cov3[6 * g + k] = c3[k];
         o = a.opac[g];
         const float df[3] = {a.tx[0] - mean[0], a.tx[1] - mean[1], a.tx[2] - mean[2]};
         const float dist = sqrtf(df[0] * df[0] + df[1] * df[1] + df[2] * df[2]);
@@ -39,7 +43,9 @@ k_preprocess_fwd(Dims d, PreFwdArgs a, int NB)
     // alpha = o*G < 1/255 for every power < thr (see gs_device.h)
     const float thr = -gs_logf(255.0f * o) - 1e-3f;
 
-    for (int r0 = 0; r0 < d.R; r0 += GS_RC) {
+    // blockIdx.y selects a chunk of GS_RC receivers
+    {
+        const int r0 = blockIdx.y * GS_RC;
         const int nr = min(GS_RC, d.R - r0);
         // ---- emission MLP 3->32->2 for receivers r0..r0+nr (forward.cu:17-75)
         float z0[GS_RC], z1[GS_RC];
@@ -111,42 +117,24 @@ k_preprocess_fwd(Dims d, PreFwdArgs a, int NB)
                 }
                 a.radii[ri] = radius;
             }
-            // ---- per-tile counts of this block's Gaussians for receiver r
-            const uint64_t any = ballot(x1 > x0);
-            int bx0 = 0x7fffffff, by0 = 0x7fffffff, bx1 = 0, by1 = 0;   // wave union box
-            if (any) {
-                const bool has = x1 > x0;
-                int lx0 = has ? x0 : 0x7fffffff, ly0 = has ? y0 : 0x7fffffff;
-                int lx1 = has ? x1 : 0, ly1 = has ? y1 : 0;
-                for (int off = 32; off >= 1; off >>= 1) {
-                    lx0 = min(lx0, __shfl_xor(lx0, off));
-                    ly0 = min(ly0, __shfl_xor(ly0, off));
-                    lx1 = max(lx1, __shfl_xor(lx1, off));
-                    ly1 = max(ly1, __shfl_xor(ly1, off));
-                }
-                bx0 = lx0; by0 = ly0; bx1 = lx1; by1 = ly1;
+            // ---- per-tile instance counts of this block for receiver r:
+            // LDS histogram, then one global atomic per touched tile
+            if (lds_tiles) {
+                for (int ty = y0; ty < y1; ++ty)
+                    for (int tx = x0; tx < x1; ++tx) atomicAdd(&s_hist[c][ty * d.gx + tx], 1u);
+            } else {
+                for (int ty = y0; ty < y1; ++ty)
+                    for (int tx = x0; tx < x1; ++tx) atomicAdd(&a.segcnt[(size_t)r * d.T + ty * d.gx + tx], 1u);
             }
-            for (int c0 = 0; c0 < d.T; c0 += GS_WAVE) {
-                uint32_t mine = 0;
-                const int cend = min(d.T, c0 + GS_WAVE);
-                if (any) {
-                    for (int t = c0; t < cend; ++t) {
-                        const int tx = t % d.gx, ty = t / d.gx;
-                        if (tx < bx0 || tx >= bx1 || ty < by0 || ty >= by1) continue;   // uniform
-                        const uint64_t m = ballot(tx >= x0 && tx < x1 && ty >= y0 && ty < y1);
-                        if (lane == t - c0) mine = (uint32_t)__popcll(m);
-                    }
-                }
-                s_cnt[wave][lane] = mine;
-                __syncthreads();
-                if (threadIdx.x < GS_WAVE && c0 + (int)threadIdx.x < d.T) {
-                    uint32_t sum = 0;
-#pragma unroll
-                    for (int w = 0; w < GS_PRE_THREADS / GS_WAVE; ++w) sum += s_cnt[w][threadIdx.x];
-                    a.wcnt[((size_t)r * NB + blockIdx.x) * d.T + c0 + threadIdx.x] = sum;
-                }
-                __syncthreads();
-            }
+        }
+    }
+    if (lds_tiles) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < GS_RC * d.T; i += GS_PRE_THREADS) {
+            const int c = i / d.T, t = i % d.T;
+            const int r = blockIdx.y * GS_RC + c;
+            const uint32_t v = s_hist[c][t];
+            if (v && r < d.R) atomicAdd(&a.segcnt[(size_t)r * d.T + t], v);
         }
     }
 }
@@ -155,7 +143,9 @@ hipError_t launch_preprocess_fwd(const Dims& d, const PreFwdArgs& a, hipStream_t
 {
     const int NB = (d.P + GS_PRE_THREADS - 1) / GS_PRE_THREADS;
     if (NB == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_preprocess_fwd, dim3(NB), dim3(GS_PRE_THREADS), 0, s, d, a, NB);
+    const int nchunks = (d.R + GS_RC - 1) / GS_RC;
+    if (nchunks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_preprocess_fwd, dim3(NB, nchunks), dim3(GS_PRE_THREADS), 0, s, d, a);
     return hipGetLastError();
 }
 

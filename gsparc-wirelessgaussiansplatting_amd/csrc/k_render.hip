@@ -1,12 +1,15 @@
 // Front-to-back alpha blend (renderCUDA, forward.cu:151-249) and its adjoint
 // (renderCUDA, backward.cu:442-616) for R receivers.
 //
-// Work unit = one wave64 workgroup per 8x8 quarter of a 16x16 binning tile
-// (the 16x16 binning grid is part of the numerics and is kept).  A wave
-// stages 64 list entries at a time in LDS and every lane walks them for its
-// pixel; waves exit as soon as all 64 of their pixels are saturated, without
-// waiting for the other quarters of the tile.  Work-group ids are mapped so
-// that the four quarters of a tile run on the same XCD and share its L2.
+// Work unit = one wave64 workgroup.  With PPL = 1 a wave owns one 8x8 quarter
+// of a 16x16 binning tile (four waves per tile: many small work items, best
+// for one receiver); with PPL = 4 a wave owns the whole tile and every lane
+// carries four independent pixel chains, one per quarter (best for batches:
+// the LDS staging of each list entry and, in the backward, the 8-value wave
+// reduction are shared by 256 pixels).  The 16x16 binning grid itself is part
+// of the numerics and never changes.  A wave stages 64 list entries at a time
+// in LDS and exits as soon as all of its pixels are saturated.  Work-group ids
+// are mapped so that all waves of a tile run on the same XCD (shared L2).
 //
 // Per pair the exponential is skipped when power < thr: then
 // opacity * exp(power) < 1/255 is guaranteed (thr carries a 1e-3 margin), so
@@ -16,47 +19,69 @@
 
 namespace gs {
 
-struct PixelMap {
-    int tile, px, py;
+template <int PPL>
+struct Pixels {
+    int tile;
     bool valid;
+    int px[PPL], py[PPL];
+    bool inside[PPL];
 };
 
-__device__ __forceinline__ PixelMap map_pixel(const Dims& d)
+template <int PPL>
+__device__ __forceinline__ Pixels<PPL> map_pixels(const Dims& d)
 {
-    PixelMap m;
+    Pixels<PPL> m;
     const int b = blockIdx.x;
     const int xcd = b & 7, s = b >> 3;
-    const int sub = s & 3;
-    m.tile = (s >> 2) * 8 + xcd;
+    const int sub = (PPL == 1) ? (s & 3) : 0;
+    m.tile = ((PPL == 1) ? (s >> 2) : s) * 8 + xcd;
     m.valid = m.tile < d.T;
     const int lane = threadIdx.x;
     const int tx = m.valid ? m.tile % d.gx : 0, ty = m.valid ? m.tile / d.gx : 0;
-    m.px = tx * GS_BLOCK + (sub & 1) * 8 + (lane & 7);
-    m.py = ty * GS_BLOCK + (sub >> 1) * 8 + (lane >> 3);
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+        const int q = (PPL == 1) ? sub : k;
+        m.px[k] = tx * GS_BLOCK + (q & 1) * 8 + (lane & 7);
+        m.py[k] = ty * GS_BLOCK + (q >> 1) * 8 + (lane >> 3);
+        m.inside[k] = m.px[k] < d.W && m.py[k] < d.H;
+    }
     return m;
 }
 
+template <int PPL>
 __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
 {
     __shared__ float4 s0[GS_WAVE];
     __shared__ float4 s1[GS_WAVE];
     __shared__ float2 ss[GS_WAVE];
-    const PixelMap pm = map_pixel(d);
+    const Pixels<PPL> pm = map_pixels<PPL>(d);
     if (!pm.valid) return;
     if ((uint64_t)a.total[0] > a.Kcap) return;   // binning overflow, reported by the host
-    const bool inside = pm.px < d.W && pm.py < d.H;
-    if (ballot(inside) == 0) return;
+    bool any_inside = false;
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) any_inside |= pm.inside[k];
+    if (ballot(any_inside) == 0) return;
     const int r = blockIdx.y;
     const int lane = threadIdx.x;
     const uint2 range = a.ranges[(size_t)r * d.T + pm.tile];
     const size_t rP = (size_t)r * d.P;
-    const float pxf = (float)pm.px, pyf = (float)pm.py;
 
-    float T = 1.0f, are = 0.0f, aim = 0.0f;
-    uint32_t contributor = 0, last = 0;
-    bool done = !inside;
+    float T[PPL], are[PPL], aim[PPL], pxf[PPL], pyf[PPL];
+    uint32_t contributor[PPL], last[PPL];
+    bool done[PPL];
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+        T[k] = 1.0f; are[k] = 0.0f; aim[k] = 0.0f;
+        contributor[k] = 0; last[k] = 0;
+        done[k] = !pm.inside[k];
+        pxf[k] = (float)pm.px[k];
+        pyf[k] = (float)pm.py[k];
+    }
     for (uint32_t start = range.x; start < range.y; start += GS_WAVE) {
-        if (ballot(!done) == 0) break;
+        bool live = false;
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) live |= !done[k];
+        if (ballot(live) == 0) break;
         const uint32_t e = start + lane;
         if (e < range.y) {
             const size_t gi = rP + a.plist[e];
@@ -67,76 +92,108 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
         __syncthreads();
         const int n = (int)min((uint32_t)GS_WAVE, range.y - start);
         for (int j = 0; j < n; ++j) {
-            if (done) continue;
-            contributor++;
             const float4 A = s0[j];
             const float4 B = s1[j];
-            const float dx = A.x - pxf, dy = A.y - pyf;
-            const float power = pair_power(A.z, A.w, B.x, dx, dy);
-            if (power > 0.0f) continue;
-            if (power < B.z) continue;
-            const float alpha = fminf(0.99f, B.y * gs_expf(power));
-            if (alpha < 1.0f / 255.0f) continue;
-            const float test_T = T * (1.0f - alpha);
-            if (test_T < 0.0001f) { done = true; continue; }
-            const float2 S = ss[j];
-            const float w = alpha * T;
-            are = fmaf(S.x, w, are);
-            aim = fmaf(S.y, w, aim);
-            T = test_T;
-            last = contributor;
+#pragma unroll
+            for (int k = 0; k < PPL; ++k) {
+                if (done[k]) continue;
+                contributor[k]++;
+                const float dx = A.x - pxf[k], dy = A.y - pyf[k];
+                const float power = pair_power(A.z, A.w, B.x, dx, dy);
+                if (power > 0.0f) continue;
+                if (power < B.z) continue;
+                const float alpha = fminf(0.99f, B.y * gs_expf_nonpos(power));
+                if (alpha < 1.0f / 255.0f) continue;
+                const float test_T = T[k] * (1.0f - alpha);
+                if (test_T < 0.0001f) { done[k] = true; continue; }
+                const float2 S = ss[j];
+                const float w = alpha * T[k];
+                are[k] = fmaf(S.x, w, are[k]);
+                aim[k] = fmaf(S.y, w, aim[k]);
+                T[k] = test_T;
+                last[k] = contributor[k];
+            }
         }
         __syncthreads();
     }
     if (a.pairs) {
-        unsigned long long c = contributor;
+        unsigned long long c = 0;
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) c += contributor[k];
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off);
         if (lane == 0) atomicAdd(a.pairs, c);
     }
-    if (inside) {
-        const size_t p = (size_t)r * d.W * d.H + (size_t)pm.py * d.W + pm.px;
-        a.out_re[p] = are;
-        a.out_im[p] = aim;
-        a.finalT[p] = T;
-        a.ncontrib[p] = last;
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+        if (!pm.inside[k]) continue;
+        const size_t p = (size_t)r * d.W * d.H + (size_t)pm.py[k] * d.W + pm.px[k];
+        a.out_re[p] = are[k];
+        a.out_im[p] = aim[k];
+        a.finalT[p] = T[k];
+        a.ncontrib[p] = last[k];
     }
 }
 
-__device__ __forceinline__ float wave_sum(float v)
+// Sum over the 64 lanes with DPP row operations; the total lands in lane 63.
+//   quad_perm [1,0,3,2], quad_perm [2,3,0,1]   -> sums of quads
+//   row_shr:4, row_shr:8                        -> lane 15 of each row holds the row sum
+//   row_bcast:15 (rows 1,3), row_bcast:31 (rows 2,3) -> lane 63 holds the wave sum
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_step(float v)
 {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    const int moved = __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false);
+    return v + __int_as_float(moved);
+}
+
+__device__ __forceinline__ float wave_sum_to_lane63(float v)
+{
+    v = dpp_step<0xb1, 0xf>(v);
+    v = dpp_step<0x4e, 0xf>(v);
+    v = dpp_step<0x114, 0xf>(v);
+    v = dpp_step<0x118, 0xf>(v);
+    v = dpp_step<0x142, 0xa>(v);
+    v = dpp_step<0x143, 0xc>(v);
     return v;
 }
 
-template <int MODE>
+template <int MODE, int PPL>
 __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
 {
     __shared__ float4 s0[GS_WAVE];
     __shared__ float4 s1[GS_WAVE];
     __shared__ float2 ss[GS_WAVE];
     __shared__ uint32_t sid[GS_WAVE];
-    const PixelMap pm = map_pixel(d);
+    const Pixels<PPL> pm = map_pixels<PPL>(d);
     if (!pm.valid) return;
     if ((uint64_t)a.total[0] > a.Kcap) return;
-    const bool inside = pm.px < d.W && pm.py < d.H;
     const int r = blockIdx.y;
     const int lane = threadIdx.x;
-    const size_t p = (size_t)r * d.W * d.H + (size_t)pm.py * d.W + pm.px;
-    float T = inside ? a.finalT[p] : 0.0f;
-    const int last = inside ? (int)a.ncontrib[p] : 0;
-    int maxlast = last;
+    float T[PPL], gre[PPL], gim[PPL], pxf[PPL], pyf[PPL];
+    int last[PPL];
+    int maxlast = 0;
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+        const size_t p = (size_t)r * d.W * d.H + (size_t)pm.py[k] * d.W + pm.px[k];
+        T[k] = pm.inside[k] ? a.finalT[p] : 0.0f;
+        last[k] = pm.inside[k] ? (int)a.ncontrib[p] : 0;
+        gre[k] = pm.inside[k] ? a.dout_re[p] : 0.0f;
+        gim[k] = pm.inside[k] ? a.dout_im[p] : 0.0f;
+        maxlast = max(maxlast, last[k]);
+        pxf[k] = (float)pm.px[k];
+        pyf[k] = (float)pm.py[k];
+    }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) maxlast = max(maxlast, __shfl_xor(maxlast, off));
     if (maxlast == 0) return;
-    const float gre = inside ? a.dout_re[p] : 0.0f;
-    const float gim = inside ? a.dout_im[p] : 0.0f;
     const uint2 range = a.ranges[(size_t)r * d.T + pm.tile];
     const size_t rP = (size_t)r * d.P;
-    const float pxf = (float)pm.px, pyf = (float)pm.py;
 
-    float acc_re = 0.0f, acc_im = 0.0f, last_alpha = 0.0f, last_re = 0.0f, last_im = 0.0f;
+    float acc_re[PPL], acc_im[PPL], last_alpha[PPL], last_re[PPL], last_im[PPL];
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+        acc_re[k] = 0.0f; acc_im[k] = 0.0f; last_alpha[k] = 0.0f; last_re[k] = 0.0f; last_im[k] = 0.0f;
+    }
     for (int top = maxlast - 1; top >= 0; top -= GS_WAVE) {
         const int rel = top - lane;
         if (rel >= 0) {
@@ -150,57 +207,63 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
         __syncthreads();
         const int n = min(GS_WAVE, top + 1);
         for (int j = 0; j < n; ++j) {
-            const bool active = (top - j) < last;
             const float4 A = s0[j];
             const float4 B = s1[j];
-            const float dx = A.x - pxf, dy = A.y - pyf;
-            const float power = pair_power(A.z, A.w, B.x, dx, dy);
-            bool c = active && !(power > 0.0f) && !(power < B.z);
-            float G = 0.0f, oG = 0.0f, alpha = 0.0f;
-            if (c) {
-                G = gs_expf(power);
-                oG = B.y * G;
-                alpha = fminf(0.99f, oG);
-                c = !(alpha < 1.0f / 255.0f);
-            }
+            const float2 S = ss[j];
+            const float qa = -2.0f * A.z, qb = -A.w, qc = -2.0f * B.x;
             float v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-            if (c) {
-                T = T / (1.0f - alpha);
-                const float w = alpha * T;
-                const float2 S = ss[j];
-                const float sr = S.x;
-                const float si = (MODE == GRAD_REFERENCE) ? S.x : S.y;
-                acc_re = last_alpha * last_re + (1.0f - last_alpha) * acc_re;
-                acc_im = last_alpha * last_im + (1.0f - last_alpha) * acc_im;
-                last_re = sr;
-                last_im = si;
-                float dalpha = (sr - acc_re) * gre;
-                if (MODE == GRAD_EXACT) dalpha += (si - acc_im) * gim;
-                dalpha *= T;
-                last_alpha = alpha;
-                v[6] = w * gre;
-                v[7] = w * gim;
-                if (!(MODE == GRAD_EXACT && oG > 0.99f)) {
-                    const float qa = -2.0f * A.z, qb = -A.w, qc = -2.0f * B.x;
-                    const float dG = B.y * dalpha;
-                    const float gdx = G * dx, gdy = G * dy;
-                    const float dG_dx = -gdx * qa - gdy * qb;
-                    const float dG_dy = -gdy * qc - gdx * qb;
-                    v[0] = dG * dG_dx;
-                    v[1] = dG * dG_dy;
-                    v[2] = -0.5f * gdx * dx * dG;
-                    v[3] = -gdx * dy * dG;
-                    v[4] = -0.5f * gdy * dy * dG;
-                    v[5] = G * dalpha;
+            bool anyc = false;
+#pragma unroll
+            for (int k = 0; k < PPL; ++k) {
+                const bool active = (top - j) < last[k];
+                const float dx = A.x - pxf[k], dy = A.y - pyf[k];
+                const float power = pair_power(A.z, A.w, B.x, dx, dy);
+                bool c = active && !(power > 0.0f) && !(power < B.z);
+                float G = 0.0f, oG = 0.0f, alpha = 0.0f;
+                if (c) {
+                    G = gs_expf_nonpos(power);
+                    oG = B.y * G;
+                    alpha = fminf(0.99f, oG);
+                    c = !(alpha < 1.0f / 255.0f);
+                }
+                if (c) {
+                    // T before this Gaussian: T_after / (1 - alpha) (backward.cu:565)
+                    T[k] = T[k] * __builtin_amdgcn_rcpf(1.0f - alpha);
+                    const float w = alpha * T[k];
+                    const float sr = S.x;
+                    const float si = (MODE == GRAD_REFERENCE) ? S.x : S.y;
+                    acc_re[k] = last_alpha[k] * last_re[k] + (1.0f - last_alpha[k]) * acc_re[k];
+                    acc_im[k] = last_alpha[k] * last_im[k] + (1.0f - last_alpha[k]) * acc_im[k];
+                    last_re[k] = sr;
+                    last_im[k] = si;
+                    float dalpha = (sr - acc_re[k]) * gre[k];
+                    if (MODE == GRAD_EXACT) dalpha += (si - acc_im[k]) * gim[k];
+                    dalpha *= T[k];
+                    last_alpha[k] = alpha;
+                    v[6] += w * gre[k];
+                    v[7] += w * gim[k];
+                    if (!(MODE == GRAD_EXACT && oG > 0.99f)) {
+                        const float dG = B.y * dalpha;
+                        const float gdx = G * dx, gdy = G * dy;
+                        v[0] += dG * (-gdx * qa - gdy * qb);
+                        v[1] += dG * (-gdy * qc - gdx * qb);
+                        v[2] += -0.5f * gdx * dx * dG;
+                        v[3] += -gdx * dy * dG;
+                        v[4] += -0.5f * gdy * dy * dG;
+                        v[5] += G * dalpha;
+                    }
+                    anyc = true;
                 }
             }
-            if (ballot(c) == 0) continue;
+            if (ballot(anyc) == 0) continue;
+            float tot[8];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) v[k] = wave_sum(v[k]);
+            for (int q = 0; q < 8; ++q)
+                tot[q] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wave_sum_to_lane63(v[q])), 63));
             if (lane < 8) {
-                float x = v[0];
+                float x = tot[0];
 #pragma unroll
-                for (int k = 1; k < 8; ++k) x = (lane == k) ? v[k] : x;
+                for (int q = 1; q < 8; ++q) x = (lane == q) ? tot[q] : x;
                 atomicAdd(a.g8 + (rP + sid[j]) * 8 + lane, x);
             }
         }
@@ -208,26 +271,40 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
     }
 }
 
-static inline dim3 render_grid(const Dims& d)
+static inline dim3 render_grid(const Dims& d, int ppl)
 {
     const int tiles8 = (d.T + 7) / 8 * 8;
-    return dim3(tiles8 * 4, d.R);
+    return dim3(tiles8 * (ppl == 1 ? 4 : 1), d.R);
 }
 
-hipError_t launch_render_fwd(const Dims& d, const RenderFwdArgs& a, hipStream_t s)
+int choose_ppl(const Dims& d, int requested)
+{
+    if (requested == 1 || requested == 4) return requested;
+    // enough tiles to keep every SIMD busy with one wave per tile
+    return ((long long)d.R * d.T >= 2048) ? 4 : 1;
+}
+
+hipError_t launch_render_fwd(const Dims& d, const RenderFwdArgs& a, int ppl, hipStream_t s)
 {
     if (d.R == 0 || d.T == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_render_fwd, render_grid(d), dim3(GS_WAVE), 0, s, d, a);
+    if (ppl == 4)
+        hipLaunchKernelGGL(k_render_fwd<4>, render_grid(d, 4), dim3(GS_WAVE), 0, s, d, a);
+    else
+        hipLaunchKernelGGL(k_render_fwd<1>, render_grid(d, 1), dim3(GS_WAVE), 0, s, d, a);
     return hipGetLastError();
 }
 
-hipError_t launch_render_bwd(const Dims& d, const RenderBwdArgs& a, hipStream_t s)
+hipError_t launch_render_bwd(const Dims& d, const RenderBwdArgs& a, int ppl, hipStream_t s)
 {
     if (d.R == 0 || d.T == 0) return hipSuccess;
-    if (a.mode == GRAD_REFERENCE)
-        hipLaunchKernelGGL(k_render_bwd<GRAD_REFERENCE>, render_grid(d), dim3(GS_WAVE), 0, s, d, a);
-    else
-        hipLaunchKernelGGL(k_render_bwd<GRAD_EXACT>, render_grid(d), dim3(GS_WAVE), 0, s, d, a);
+    const dim3 grid = render_grid(d, ppl);
+    if (a.mode == GRAD_REFERENCE) {
+        if (ppl == 4) hipLaunchKernelGGL((k_render_bwd<GRAD_REFERENCE, 4>), grid, dim3(GS_WAVE), 0, s, d, a);
+        else hipLaunchKernelGGL((k_render_bwd<GRAD_REFERENCE, 1>), grid, dim3(GS_WAVE), 0, s, d, a);
+    } else {
+        if (ppl == 4) hipLaunchKernelGGL((k_render_bwd<GRAD_EXACT, 4>), grid, dim3(GS_WAVE), 0, s, d, a);
+        else hipLaunchKernelGGL((k_render_bwd<GRAD_EXACT, 1>), grid, dim3(GS_WAVE), 0, s, d, a);
+    }
     return hipGetLastError();
 }
 

@@ -28,14 +28,16 @@ _sz = C.c_size_t
 
 class Layout(C.Structure):
     _fields_ = [(n, C.c_size_t) for n in (
-        "splat", "sig", "depth", "cov3", "wcnt", "geom_bytes",
-        "final_t", "n_contrib", "ranges", "segcnt", "segbase", "total", "image_bytes",
+        "splat", "sig", "depth", "cov3", "geom_bytes",
+        "final_t", "n_contrib", "ranges", "segcnt", "cursor", "segbase", "total", "image_bytes",
         "keys", "vals", "keys_tmp", "vals_tmp", "plist", "keys_sorted", "bin_bytes")]
 
 
 _SIGS = {
     "gsparc_version": (C.c_char_p, []),
     "gsparc_last_error": (C.c_char_p, []),
+    "gsparc_set_option": (_i, [C.c_char_p, _i]),
+    "gsparc_get_option": (_i, [C.c_char_p, C.POINTER(_i)]),
     "gsparc_geom_buffer_size": (_sz, [_i, _i, _i, _i]),
     "gsparc_image_buffer_size": (_sz, [_i, _i, _i, _i]),
     "gsparc_binning_buffer_size": (_sz, [_u64]),
@@ -91,6 +93,16 @@ def ptr(t: torch.Tensor | None):
 
 def stream_handle(device=None):
     return C.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def set_option(name: str, value: int):
+    check(lib().gsparc_set_option(name.encode(), int(value)), f"gsparc_set_option({name})")
+
+
+def get_option(name: str) -> int:
+    v = C.c_int(0)
+    check(lib().gsparc_get_option(name.encode(), C.byref(v)), f"gsparc_get_option({name})")
+    return v.value
 
 
 def profile_enable(on: bool = True):

@@ -44,15 +44,22 @@ extern "C" {
 /* Byte offsets of the arrays inside the caller's buffers (for inspection). */
 typedef struct gsparc_layout {
     /* geom */
-    size_t splat, sig, depth, cov3, wcnt, geom_bytes;
+    size_t splat, sig, depth, cov3, geom_bytes;
     /* image */
-    size_t final_t, n_contrib, ranges, segcnt, segbase, total, image_bytes;
+    size_t final_t, n_contrib, ranges, segcnt, cursor, segbase, total, image_bytes;
     /* bin */
     size_t keys, vals, keys_tmp, vals_tmp, plist, keys_sorted, bin_bytes;
 } gsparc_layout;
 
 const char* gsparc_version(void);
 const char* gsparc_last_error(void);
+
+/* Tuning knobs (process-wide; no counterpart in the reference):
+ *   "render_ppl"  pixels per lane of the blend kernels: 0 = automatic,
+ *                 1 = one 8x8 quarter-tile per wave, 4 = one 16x16 tile per
+ *                 wave.  Changes speed only, never results. */
+int gsparc_set_option(const char* name, int value);
+int gsparc_get_option(const char* name, int* value);
 
 size_t gsparc_geom_buffer_size(int P, int R, int W, int H);
 size_t gsparc_image_buffer_size(int P, int R, int W, int H);

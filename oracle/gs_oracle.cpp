@@ -822,7 +822,7 @@ extern "C" float orc_gs_atan2f(float y, float x) { return gs_atan2f(y, x); }
 extern "C" float orc_gs_asinf(float x) { return gs_asinf(x); }
 
 
-// vectorised math for the accuracy tests: which = 0 exp, 1 log, 2 asin, 3 atan2(y, x)
+// vectorised math for the accuracy tests: which = 0 exp, 1 log, 2 asin, 3 atan2(y, x), 4 exp (x <= 0)
 extern "C" void orc_gs_math_vec(int which, const float* x, const float* y, float* out, int64_t n)
 {
     for (int64_t i = 0; i < n; ++i) {
@@ -830,6 +830,7 @@ extern "C" void orc_gs_math_vec(int which, const float* x, const float* y, float
         case 0: out[i] = gs_expf(x[i]); break;
         case 1: out[i] = gs_logf(x[i]); break;
         case 2: out[i] = gs_asinf(x[i]); break;
+        case 4: out[i] = gs_expf_nonpos(x[i]); break;
         default: out[i] = gs_atan2f(y[i], x[i]); break;
         }
     }

@@ -17,6 +17,15 @@ import wireless_rasterizer as wr
 
 pytestmark = pytest.mark.gpu
 
+
+@pytest.fixture(params=[1, 4], ids=["ppl1", "ppl4"])
+def ppl(request):
+    """Run under both blend work mappings (8x8 quarter-tile or 16x16 tile per wave)."""
+    import gsparc_hip as gh
+    gh.set_option("render_ppl", request.param)
+    yield request.param
+    gh.set_option("render_ppl", 0)
+
 SPEC_TOL = 1e-5     # relative to the peak of the spectrum
 GRAD_TOL = 1e-4     # relative to the largest |gradient| of the tensor
 
@@ -64,7 +73,7 @@ def _check_forward(scene, st, r):
 
 
 @pytest.mark.parametrize("seed,P", [(10, 64), (11, 500), (12, 3000)])
-def test_forward_single_receiver(seed, P):
+def test_forward_single_receiver(seed, P, ppl):
     sc = S.small_scene(seed, P)
     _, out = gpu_forward(sc)
     st = read_state(sc, out)
@@ -78,7 +87,7 @@ def test_forward_config0():
     _check_forward(sc, st, 0)
 
 
-def test_forward_batched_equals_single():
+def test_forward_batched_equals_single(ppl):
     sc = S.small_scene(21, 2000, R=5, walls=True)
     _, out = gpu_forward(sc)
     st = read_state(sc, out)
@@ -137,7 +146,7 @@ def _gpu_backward(sc, d, out, dre, dim, mode):
 
 @pytest.mark.parametrize("mode", ["exact", "reference"])
 @pytest.mark.parametrize("seed,P,R", [(31, 400, 1), (32, 3000, 3)])
-def test_backward_parity(mode, seed, P, R):
+def test_backward_parity(mode, seed, P, R, ppl):
     sc = S.small_scene(seed, P, R=R)
     d, out = gpu_forward(sc)
     rng = np.random.default_rng(seed)

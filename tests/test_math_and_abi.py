@@ -27,6 +27,16 @@ def test_expf_accuracy():
     assert O.math_vec("exp", np.array([0.0, -200.0, 100.0, np.nan], np.float32)).tolist()[:3] == [1.0, 0.0, np.inf]
 
 
+def test_expf_nonpos_is_bit_identical():
+    """The blend kernels use the branch-free variant for power <= 0."""
+    rng = np.random.default_rng(3)
+    x = np.concatenate([-rng.uniform(0, 120, 4_000_000), -np.exp(rng.uniform(-30, 5, 1_000_000)),
+                        np.array([0.0, -0.0, -104.0, -103.99999, -104.00001, -1e30, -np.inf])]).astype(np.float32)
+    a = O.math_vec("exp", x)
+    b = O.math_vec("exp_nonpos", x)
+    np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
 def test_logf_accuracy():
     rng = np.random.default_rng(1)
     x = np.exp(rng.uniform(-80, 80, 3_000_000)).astype(np.float32)
@@ -67,7 +77,21 @@ def test_library_exports_every_declared_symbol():
 def test_buffer_layout_sizes():
     import gsparc_hip as gh
     L = gh.layout(50000, 64, 360, 90, 1 << 20)
-    assert L.splat == 0 and L.sig >= 64 * 50000 * 32 and L.geom_bytes > L.wcnt
-    for off in (L.sig, L.depth, L.cov3, L.wcnt, L.n_contrib, L.ranges, L.plist):
+    assert L.splat == 0 and L.sig >= 64 * 50000 * 32 and L.geom_bytes > L.cov3
+    assert L.cursor == L.segcnt + 4 * 64 * 138
+    for off in (L.sig, L.depth, L.cov3, L.n_contrib, L.ranges, L.segcnt, L.plist):
         assert off % 256 == 0
     assert gh.lib().gsparc_binning_buffer_size(1 << 20) >= 6 * 4 * (1 << 20)
+
+
+def test_tuning_options():
+    import gsparc_hip as gh
+    assert gh.get_option("render_ppl") == 0
+    gh.set_option("render_ppl", 4)
+    assert gh.get_option("render_ppl") == 4
+    gh.set_option("render_ppl", 0)
+    import pytest
+    with pytest.raises(RuntimeError, match="render_ppl"):
+        gh.set_option("render_ppl", 3)
+    with pytest.raises(RuntimeError, match="unknown option"):
+        gh.set_option("no_such_option", 1)
