@@ -53,7 +53,8 @@ struct BinArgs {
     uint32_t* cursor;        // [R*T] fill cursors of the scatter (zeroed before)
     uint32_t* segbase;       // [R*T]
     uint2* ranges;           // [R*T]
-    uint32_t* total;         // [2]: K, max segment length
+    uint32_t* total;         // [3]: K, max segment length, number of long segments
+    uint32_t* long_list;     // [R*T] segments longer than the one-wave sort handles
     uint32_t* total_host;    // optional pinned mirror of total
     uint32_t* keys;          // [Kcap] depth bits, tile-major
     uint32_t* vals;          // [Kcap]
@@ -65,7 +66,8 @@ struct BinArgs {
 };
 hipError_t launch_segscan(const Dims& d, const BinArgs& a, hipStream_t s);
 hipError_t launch_scatter(const Dims& d, const BinArgs& a, hipStream_t s);
-hipError_t launch_sort(const Dims& d, const BinArgs& a, hipStream_t s);
+hipError_t launch_sort_small(const Dims& d, const BinArgs& a, hipStream_t s);
+hipError_t launch_sort_long(const Dims& d, const BinArgs& a, hipStream_t s);
 
 struct RenderFwdArgs {
     const uint2* ranges;     // [R*T]
@@ -121,6 +123,8 @@ struct PreBwdArgs {
     float* dmeans2d;         // [R*P*3] or nullptr (reference screen-space layout)
     float* dsig;             // [R*P*2] or nullptr (emission gradient)
     int mode;
+    float2* dz;              // workspace [R*P]: dL/d(MLP output) per receiver
+    float* part;             // workspace [ceil(R/8)*P*12]: per-chunk partial sums
 };
 hipError_t launch_preprocess_bwd(const Dims& d, const PreBwdArgs& a, hipStream_t s);
 

@@ -17,8 +17,11 @@ namespace {
 
 thread_local std::string g_err;
 
-// tuning options (gsparc_set_option)
-int g_opt_render_ppl = 0;   // 0 = automatic, 1 or 4
+// tuning options (gsparc_set_option): pixels per lane of the forward and
+// backward blend; 0 = automatic (forward 1; backward 4 when the batch has
+// enough tiles to fill the GPU with one wave per tile)
+int g_opt_fwd_ppl = 0;
+int g_opt_bwd_ppl = 0;
 
 int fail(int code, const std::string& msg)
 {
@@ -52,6 +55,7 @@ void make_layout(int P, int R, int W, int H, uint64_t K, gsparc_layout& L)
     L.segcnt = o; o = o + RT * sizeof(uint32_t);   // segcnt and cursor are adjacent:
     L.cursor = o; o = al(o + RT * sizeof(uint32_t)); // one fill clears both
     L.segbase = o; o = al(o + RT * sizeof(uint32_t));
+    L.long_list = o; o = al(o + RT * sizeof(uint32_t));
     L.total = o; o = al(o + 4 * sizeof(uint32_t));
     L.image_bytes = o + ALIGN;
     o = 0;
@@ -134,25 +138,31 @@ extern "C" {
 const char* gsparc_version(void) { return "gsparc-mi355x 0.1 (gfx950)"; }
 const char* gsparc_last_error(void) { return g_err.c_str(); }
 
+static int* option_slot(const char* name)
+{
+    if (std::strcmp(name, "render_fwd_ppl") == 0) return &g_opt_fwd_ppl;
+    if (std::strcmp(name, "render_bwd_ppl") == 0) return &g_opt_bwd_ppl;
+    return nullptr;
+}
+
 int gsparc_set_option(const char* name, int value)
 {
     if (!name) return fail(GSPARC_ERR_ARG, "null option name");
-    if (std::strcmp(name, "render_ppl") == 0) {
-        if (value != 0 && value != 1 && value != 4) return fail(GSPARC_ERR_ARG, "render_ppl must be 0, 1 or 4");
-        g_opt_render_ppl = value;
-        return GSPARC_OK;
-    }
-    return fail(GSPARC_ERR_ARG, std::string("unknown option ") + name);
+    int* slot = option_slot(name);
+    if (!slot) return fail(GSPARC_ERR_ARG, std::string("unknown option ") + name);
+    if (value != 0 && value != 1 && value != 2 && value != 4)
+        return fail(GSPARC_ERR_ARG, std::string(name) + " must be 0, 1, 2 or 4");
+    *slot = value;
+    return GSPARC_OK;
 }
 
 int gsparc_get_option(const char* name, int* value)
 {
     if (!name || !value) return fail(GSPARC_ERR_ARG, "null pointer");
-    if (std::strcmp(name, "render_ppl") == 0) {
-        *value = g_opt_render_ppl;
-        return GSPARC_OK;
-    }
-    return fail(GSPARC_ERR_ARG, std::string("unknown option ") + name);
+    int* slot = option_slot(name);
+    if (!slot) return fail(GSPARC_ERR_ARG, std::string("unknown option ") + name);
+    *value = *slot;
+    return GSPARC_OK;
 }
 
 size_t gsparc_geom_buffer_size(int P, int R, int W, int H)
@@ -176,7 +186,20 @@ size_t gsparc_binning_buffer_size(uint64_t K)
     return L.bin_bytes;
 }
 
-size_t gsparc_grad_buffer_size(int P, int R) { return (size_t)P * R * 8 * sizeof(float) + ALIGN; }
+// grad workspace: g8 [R*P*8] | dz [R*P*2] | partials [ceil(R/8)*P*12]
+static void grad_layout(int P, int R, size_t& dz_off, size_t& part_off, size_t& bytes)
+{
+    dz_off = al((size_t)P * R * 8 * sizeof(float));
+    part_off = al(dz_off + (size_t)P * R * 2 * sizeof(float));
+    bytes = part_off + (size_t)((R + 7) / 8) * P * 12 * sizeof(float) + ALIGN;
+}
+
+size_t gsparc_grad_buffer_size(int P, int R)
+{
+    size_t a, b, c;
+    grad_layout(P, R, a, b, c);
+    return c;
+}
 
 int gsparc_layout_query(int P, int R, int W, int H, uint64_t K, gsparc_layout* out)
 {
@@ -214,6 +237,7 @@ int gsparc_rasterize_forward_geom(int P, int R, int W, int H, const float* means
     ba.segbase = (uint32_t*)(ib + L.segbase);
     ba.ranges = (uint2*)(ib + L.ranges);
     ba.total = (uint32_t*)(ib + L.total);
+    ba.long_list = (uint32_t*)(ib + L.long_list);
     if (R > 0) {
         GS_LAUNCH("bin_segscan", gs::launch_segscan(d, ba, s), s);
         uint32_t tot[2] = {0, 0};
@@ -250,6 +274,7 @@ int gsparc_rasterize_forward_render(int P, int R, int W, int H, const float* emi
     ba.segbase = (uint32_t*)(ib + L.segbase);
     ba.ranges = (uint2*)(ib + L.ranges);
     ba.total = (uint32_t*)(ib + L.total);
+    ba.long_list = (uint32_t*)(ib + L.long_list);
     ba.keys = (uint32_t*)(bb + L.keys);
     ba.vals = (uint32_t*)(bb + L.vals);
     ba.keys_tmp = (uint32_t*)(bb + L.keys_tmp);
@@ -259,7 +284,8 @@ int gsparc_rasterize_forward_render(int P, int R, int W, int H, const float* emi
     ba.Kcap = K_capacity;
     if (P > 0) {
         GS_LAUNCH("bin_scatter", gs::launch_scatter(d, ba, s), s);
-        GS_LAUNCH("bin_sort", gs::launch_sort(d, ba, s), s);
+        GS_LAUNCH("bin_sort_small", gs::launch_sort_small(d, ba, s), s);
+        GS_LAUNCH("bin_sort_long", gs::launch_sort_long(d, ba, s), s);
     }
     gs::RenderFwdArgs ra;
     ra.ranges = ba.ranges;
@@ -273,7 +299,7 @@ int gsparc_rasterize_forward_render(int P, int R, int W, int H, const float* emi
     ra.total = ba.total;
     ra.Kcap = K_capacity;
     ra.pairs = g_prof_on ? g_pairs_dev : nullptr;
-    GS_LAUNCH("render_fwd", gs::launch_render_fwd(d, ra, gs::choose_ppl(d, g_opt_render_ppl), s), s);
+    GS_LAUNCH("render_fwd", gs::launch_render_fwd(d, ra, g_opt_fwd_ppl ? g_opt_fwd_ppl : 1, s), s);
     return GSPARC_OK;
 }
 
@@ -292,7 +318,6 @@ int gsparc_rasterize_backward(int grad_mode, int P, int R, int W, int H, const f
     if (!geom || !image || !bin || !grad || !means || !scales || !rotations || !viewmatrices || !tx_pos ||
         !rx_pos || !radii || !dout_real || !dout_imag || !dmeans3D || !dopacity || !dscales || !drotations)
         return fail(GSPARC_ERR_ARG, "null pointer");
-    if (emission_mlps && R > 256) return fail(GSPARC_ERR_ARG, "at most 256 receivers per call with emission MLPs");
     hipStream_t s = (hipStream_t)stream;
     const gs::Dims d = gs::make_dims(P, R, W, H);
     gsparc_layout L;
@@ -315,12 +340,18 @@ int gsparc_rasterize_backward(int grad_mode, int P, int R, int W, int H, const f
     ra.mode = grad_mode;
     ra.total = (const uint32_t*)(ib + L.total);
     ra.Kcap = K_capacity;
-    GS_LAUNCH("render_bwd", gs::launch_render_bwd(d, ra, gs::choose_ppl(d, g_opt_render_ppl), s), s);
+    GS_LAUNCH("render_bwd", gs::launch_render_bwd(d, ra, gs::choose_ppl(d, g_opt_bwd_ppl), s), s);
     gs::PreBwdArgs pa;
     pa.means = means; pa.scales = scales; pa.rots = rotations; pa.mlp = emission_mlps; pa.views = viewmatrices;
     pa.tx = tx_pos; pa.rx = rx_pos; pa.mod = scale_modifier; pa.radii = radii; pa.g8 = g8;
     pa.dmeans = dmeans3D; pa.dscales = dscales; pa.drots = drotations; pa.dopac = dopacity;
     pa.dmlp = demission_mlps; pa.dcov3 = dcov3D; pa.dmeans2d = dmeans2D; pa.dsig = demission; pa.mode = grad_mode;
+    {
+        size_t dz_off, part_off, bytes;
+        grad_layout(P, R, dz_off, part_off, bytes);
+        pa.dz = (float2*)((char*)g8 + dz_off);
+        pa.part = (float*)((char*)g8 + part_off);
+    }
     GS_LAUNCH("preprocess_bwd", gs::launch_preprocess_bwd(d, pa, s), s);
     return GSPARC_OK;
 }

@@ -9,16 +9,22 @@
 //   k_scatter   writes (depth bits, index) of every instance into its tile
 //               segment: per-block LDS counts, one global atomic per touched
 //               tile to reserve space, LDS atomics for the slots
-//   k_sort      one workgroup per tile segment: LSD radix sort on the depth
-//               bits (4-bit digits, ballot ranking), in LDS when the segment
-//               fits and chunk by chunk in global memory otherwise, then runs
-//               of equal depth are put in index order (the order the
-//               reference's stable sort gives, since its input is in index
-//               order).
+//   k_sort_small one wave per tile segment of up to 2048 instances: LSD radix
+//               sort on the depth bits (4-bit digits, ballot ranking) with the
+//               segment in registers, no workgroup barriers
+//   k_sort_long  one workgroup per longer segment (persistent over the list of
+//               long segments): the same sort in LDS when the segment fits,
+//               chunk by chunk through global memory otherwise
+//   Both then put runs of equal depth in index order (the order the
+//   reference's stable sort gives, since its input is in index order).
 #include "gs_device.h"
 #include "gs_kernels.h"
 
 namespace gs {
+
+// segments up to this length are sorted by one wave (k_sort_small), longer
+// ones by a workgroup (k_sort_long)
+#define GS_SORT_SMALL 2048
 
 // ---------------------------------------------------------------- scans
 // Block-wide exclusive scan of one value per thread (1024 threads max).
@@ -47,23 +53,30 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
 }
 
 __global__ void __launch_bounds__(1024) k_segscan(int n, const uint32_t* segcnt, uint32_t* segbase, uint2* ranges,
-                                                   uint32_t* total)
+                                                   uint32_t* total, uint32_t* long_list)
 {
     __shared__ uint32_t s_wave[1024 / GS_WAVE];
     const int per = (n + 1023) / 1024;
     const int i0 = min(n, (int)threadIdx.x * per), i1 = min(n, i0 + per);
-    uint32_t sum = 0, mx = 0;
-    for (int i = i0; i < i1; ++i) { sum += segcnt[i]; mx = max(mx, segcnt[i]); }
-    uint32_t tot;
+    uint32_t sum = 0, mx = 0, nlong = 0;
+    for (int i = i0; i < i1; ++i) {
+        sum += segcnt[i];
+        mx = max(mx, segcnt[i]);
+        nlong += segcnt[i] > GS_SORT_SMALL ? 1u : 0u;
+    }
+    uint32_t tot, totlong;
     uint32_t run = block_exclusive_scan<1024>(sum, s_wave, tot);
+    uint32_t lpos = block_exclusive_scan<1024>(nlong, s_wave, totlong);
     for (int i = i0; i < i1; ++i) {
         const uint32_t c = segcnt[i];
         segbase[i] = run;
         // empty tiles keep the range (0, 0) as after the reference's memset
         // (rasterizer_impl.cu:303)
         ranges[i] = c ? make_uint2(run, run + c) : make_uint2(0u, 0u);
+        if (c > GS_SORT_SMALL) long_list[lpos++] = (uint32_t)i;
         run += c;
     }
+    if (threadIdx.x == 0) total[2] = totlong;
     // max segment length
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, off));
@@ -287,76 +300,189 @@ __device__ void order_ties(int n, KP k, VP v, uint32_t* s_flag)
     __syncthreads();
 }
 
-__global__ void __launch_bounds__(GS_SORT_THREADS) k_sort(BinArgs a, int nseg)
+// One workgroup per long segment (> GS_SORT_SMALL instances).  Persistent:
+// the grid walks the compacted list of long segments written by k_segscan.
+__global__ void __launch_bounds__(GS_SORT_THREADS) k_sort_long(BinArgs a)
 {
     extern __shared__ unsigned char s_raw[];
     SortShared& sm = *reinterpret_cast<SortShared*>(s_raw);
     if ((uint64_t)a.total[0] > a.Kcap) return;
+    const int tid = threadIdx.x;
+    const uint32_t nlong = a.total[2];
+    for (uint32_t li = blockIdx.x; li < nlong; li += gridDim.x) {
+        const uint32_t seg = a.long_list[li];
+        const int n = (int)a.ranges[seg].y - (int)a.ranges[seg].x;
+        const uint32_t base = a.ranges[seg].x;
+        uint32_t* gk = a.keys + base;
+        uint32_t* gv = a.vals + base;
+        // key range of the segment
+        uint32_t kmin = 0xffffffffu, kmax = 0;
+        for (int e = tid; e < n; e += GS_SORT_THREADS) {
+            const uint32_t k = gk[e];
+            kmin = min(kmin, k);
+            kmax = max(kmax, k);
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            kmin = min(kmin, (uint32_t)__shfl_xor(kmin, off));
+            kmax = max(kmax, (uint32_t)__shfl_xor(kmax, off));
+        }
+        if ((tid % GS_WAVE) == 0) { sm.red[0][tid / GS_WAVE] = kmin; sm.red[1][tid / GS_WAVE] = kmax; }
+        __syncthreads();
+        kmin = sm.red[0][0];
+        kmax = sm.red[1][0];
+        for (int w = 1; w < GS_SORT_WAVES; ++w) { kmin = min(kmin, sm.red[0][w]); kmax = max(kmax, sm.red[1][w]); }
+        const uint32_t span = kmax - kmin;
+        const int nbits = span ? 32 - __clz((int)span) : 0;
+        const int passes = (nbits + 3) / 4;
+        __syncthreads();
+
+        if (n <= GS_SORT_CHUNK) {
+            for (int e = tid; e < n; e += GS_SORT_THREADS) { sm.k[0][e] = gk[e]; sm.v[0][e] = gv[e]; }
+            __syncthreads();
+            int cur = 0;
+            for (int p = 0; p < passes; ++p) {
+                radix_pass(sm, n, sm.k[cur], sm.v[cur], sm.k[cur ^ 1], sm.v[cur ^ 1], kmin, 4 * p);
+                cur ^= 1;
+            }
+            order_ties(n, sm.k[cur], sm.v[cur], &sm.flag);
+            for (int e = tid; e < n; e += GS_SORT_THREADS) {
+                a.plist[base + e] = sm.v[cur][e];
+                if (a.keys_sorted) a.keys_sorted[base + e] = sm.k[cur][e];
+            }
+        } else {
+            uint32_t* tk = a.keys_tmp + base;
+            uint32_t* tv = a.vals_tmp + base;
+            int cur = 0;
+            for (int p = 0; p < passes; ++p) {
+                if (cur == 0) radix_pass(sm, n, gk, gv, tk, tv, kmin, 4 * p);
+                else radix_pass(sm, n, tk, tv, gk, gv, kmin, 4 * p);
+                cur ^= 1;
+            }
+            uint32_t* fk = cur ? tk : gk;
+            uint32_t* fv = cur ? tv : gv;
+            order_ties(n, fk, fv, &sm.flag);
+            for (int e = tid; e < n; e += GS_SORT_THREADS) {
+                a.plist[base + e] = fv[e];
+                if (a.keys_sorted) a.keys_sorted[base + e] = fk[e];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// One wave per short segment (<= GS_SORT_SMALL instances), no workgroup
+// barriers.  The segment is double-buffered in LDS (2 x 16 KiB).  Each 4-bit
+// LSD pass takes four ballots per row of 64: lane d < 16 derives from them how
+// many elements of the row carry digit d (so the digit histogram and the
+// running digit offsets stay in lanes 0..15, with no LDS read-modify-write
+// chain) and every element derives its stable rank inside the row.  Runs of
+// equal depth are finally put in index order.
+__device__ __forceinline__ uint64_t digit_mask_of(uint32_t dg, uint64_t V, const uint64_t* B)
+{
+    uint64_t m = V;
+#pragma unroll
+    for (int bit = 0; bit < 4; ++bit) m &= ((dg >> bit) & 1u) ? B[bit] : ~B[bit];
+    return m;
+}
+
+__global__ void __launch_bounds__(GS_WAVE) k_sort_small(BinArgs a, int nseg)
+{
+    __shared__ uint2 buf[2][GS_SORT_SMALL];
+    if ((uint64_t)a.total[0] > a.Kcap) return;
     const int seg = blockIdx.x;
     if (seg >= nseg) return;
-    const int tid = threadIdx.x;
-    const int n = (int)a.ranges[seg].y - (int)a.ranges[seg].x;
-    const uint32_t base = a.ranges[seg].x;
-    if (n <= 0) return;
-    uint32_t* gk = a.keys + base;
-    uint32_t* gv = a.vals + base;
-    // key range of the segment
+    const uint2 rg = a.ranges[seg];
+    const int n = (int)(rg.y - rg.x);
+    if (n <= 0 || n > GS_SORT_SMALL) return;
+    const int lane = threadIdx.x;
+    const uint64_t lt = lanemask_lt();
+    const int rows = (n + GS_WAVE - 1) / GS_WAVE;
     uint32_t kmin = 0xffffffffu, kmax = 0;
-    for (int e = tid; e < n; e += GS_SORT_THREADS) {
-        const uint32_t k = gk[e];
-        kmin = min(kmin, k);
-        kmax = max(kmax, k);
+    for (int e = lane; e < n; e += GS_WAVE) {
+        const uint2 t = make_uint2(a.keys[rg.x + e], a.vals[rg.x + e]);
+        buf[0][e] = t;
+        kmin = min(kmin, t.x);
+        kmax = max(kmax, t.x);
     }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
         kmin = min(kmin, (uint32_t)__shfl_xor(kmin, off));
         kmax = max(kmax, (uint32_t)__shfl_xor(kmax, off));
     }
-    if ((tid % GS_WAVE) == 0) { sm.red[0][tid / GS_WAVE] = kmin; sm.red[1][tid / GS_WAVE] = kmax; }
-    __syncthreads();
-    kmin = sm.red[0][0];
-    kmax = sm.red[1][0];
-    for (int w = 1; w < GS_SORT_WAVES; ++w) { kmin = min(kmin, sm.red[0][w]); kmax = max(kmax, sm.red[1][w]); }
     const uint32_t span = kmax - kmin;
     const int nbits = span ? 32 - __clz((int)span) : 0;
     const int passes = (nbits + 3) / 4;
+    const uint32_t my_digit = (uint32_t)(lane & 15);
     __syncthreads();
-
-    if (n <= GS_SORT_CHUNK) {
-        for (int e = tid; e < n; e += GS_SORT_THREADS) { sm.k[0][e] = gk[e]; sm.v[0][e] = gv[e]; }
+    int cur = 0;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = 4 * p;
+        const uint2* src = buf[cur];
+        uint2* dst = buf[cur ^ 1];
+        // digit histogram: lane d < 16 counts digit d
+        uint32_t cnt = 0;
+        for (int i = 0; i < rows; ++i) {
+            const int e = i * GS_WAVE + lane;
+            const bool valid = e < n;
+            const uint32_t dg = valid ? ((src[e].x - kmin) >> shift) & 15u : 0u;
+            uint64_t B[4];
+#pragma unroll
+            for (int bit = 0; bit < 4; ++bit) B[bit] = ballot(valid && ((dg >> bit) & 1u));
+            cnt += (uint32_t)__popcll(digit_mask_of(my_digit, ballot(valid), B));
+        }
+        // exclusive scan over the 16 digits
+        uint32_t x = lane < 16 ? cnt : 0u;
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        uint32_t base = x - (lane < 16 ? cnt : 0u);
+        // stable scatter
+        for (int i = 0; i < rows; ++i) {
+            const int e = i * GS_WAVE + lane;
+            const bool valid = e < n;
+            const uint2 t = valid ? src[e] : make_uint2(0u, 0u);
+            const uint32_t dg = ((t.x - kmin) >> shift) & 15u;
+            uint64_t B[4];
+#pragma unroll
+            for (int bit = 0; bit < 4; ++bit) B[bit] = ballot(valid && ((dg >> bit) & 1u));
+            const uint64_t V = ballot(valid);
+            const uint32_t rank = (uint32_t)__popcll(digit_mask_of(dg, V, B) & lt);
+            const uint32_t b = (uint32_t)__shfl((int)base, (int)dg);
+            if (valid) dst[b + rank] = t;
+            base += (uint32_t)__popcll(digit_mask_of(my_digit, V, B));
+        }
         __syncthreads();
-        int cur = 0;
-        for (int p = 0; p < passes; ++p) {
-            radix_pass(sm, n, sm.k[cur], sm.v[cur], sm.k[cur ^ 1], sm.v[cur ^ 1], kmin, 4 * p);
-            cur ^= 1;
+        cur ^= 1;
+    }
+    // runs of equal depth bits -> index order
+    uint2* fin = buf[cur];
+    for (int e = lane; e < n; e += GS_WAVE) {
+        const bool starts = (e == 0 || fin[e - 1].x != fin[e].x) && e + 1 < n && fin[e + 1].x == fin[e].x;
+        if (!starts) continue;
+        int end = e + 1;
+        while (end < n && fin[end].x == fin[e].x) ++end;
+        for (int i = e + 1; i < end; ++i) {
+            const uint2 t = fin[i];
+            int j = i - 1;
+            while (j >= e && fin[j].y > t.y) { fin[j + 1] = fin[j]; --j; }
+            fin[j + 1] = t;
         }
-        order_ties(n, sm.k[cur], sm.v[cur], &sm.flag);
-        for (int e = tid; e < n; e += GS_SORT_THREADS) {
-            a.plist[base + e] = sm.v[cur][e];
-            if (a.keys_sorted) a.keys_sorted[base + e] = sm.k[cur][e];
-        }
-    } else {
-        uint32_t* tk = a.keys_tmp + base;
-        uint32_t* tv = a.vals_tmp + base;
-        int cur = 0;
-        for (int p = 0; p < passes; ++p) {
-            if (cur == 0) radix_pass(sm, n, gk, gv, tk, tv, kmin, 4 * p);
-            else radix_pass(sm, n, tk, tv, gk, gv, kmin, 4 * p);
-            cur ^= 1;
-        }
-        uint32_t* fk = cur ? tk : gk;
-        uint32_t* fv = cur ? tv : gv;
-        order_ties(n, fk, fv, &sm.flag);
-        for (int e = tid; e < n; e += GS_SORT_THREADS) {
-            a.plist[base + e] = fv[e];
-            if (a.keys_sorted) a.keys_sorted[base + e] = fk[e];
-        }
+    }
+    __syncthreads();
+    for (int e = lane; e < n; e += GS_WAVE) {
+        const uint2 t = fin[e];
+        a.plist[rg.x + e] = t.y;
+        if (a.keys_sorted) a.keys_sorted[rg.x + e] = t.x;
     }
 }
 
 hipError_t launch_segscan(const Dims& d, const BinArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_segscan, dim3(1), dim3(1024), 0, s, d.R * d.T, a.segcnt, a.segbase, a.ranges, a.total);
+    hipLaunchKernelGGL(k_segscan, dim3(1), dim3(1024), 0, s, d.R * d.T, a.segcnt, a.segbase, a.ranges, a.total,
+                       a.long_list);
     return hipGetLastError();
 }
 
@@ -367,17 +493,24 @@ hipError_t launch_scatter(const Dims& d, const BinArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_sort(const Dims& d, const BinArgs& a, hipStream_t s)
+hipError_t launch_sort_small(const Dims& d, const BinArgs& a, hipStream_t s)
 {
     const int nseg = d.R * d.T;
-    if (nseg > 0) {
-        // the sort stages up to GS_SORT_CHUNK instances in LDS (about 136 KiB of
-        // the 160 KiB a gfx950 workgroup can hold)
-        static hipError_t attr = hipFuncSetAttribute((const void*)k_sort, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                     (int)sizeof(SortShared));
-        if (attr != hipSuccess) return attr;
-        hipLaunchKernelGGL(k_sort, dim3(nseg), dim3(GS_SORT_THREADS), sizeof(SortShared), s, a, nseg);
-    }
+    if (nseg == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sort_small, dim3(nseg), dim3(GS_WAVE), 0, s, a, nseg);
+    return hipGetLastError();
+}
+
+hipError_t launch_sort_long(const Dims& d, const BinArgs& a, hipStream_t s)
+{
+    const int nseg = d.R * d.T;
+    if (nseg == 0) return hipSuccess;
+    // stages up to GS_SORT_CHUNK instances in LDS (about 136 KiB of the
+    // 160 KiB a gfx950 workgroup can hold); one block per CU
+    static hipError_t attr = hipFuncSetAttribute((const void*)k_sort_long, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 (int)sizeof(SortShared));
+    if (attr != hipSuccess) return attr;
+    hipLaunchKernelGGL(k_sort_long, dim3(std::min(nseg, 256)), dim3(GS_SORT_THREADS), sizeof(SortShared), s, a);
     return hipGetLastError();
 }
 

@@ -1,12 +1,12 @@
 // Preprocess backward for R receivers (computeCov2DCUDA backward.cu:124-303,
 // preprocessCUDA backward.cu:378-437, computeCov3D backward.cu:308-371,
-// computeSignalFromMLP backward backward.cu:9-118), fused in one kernel.
+// computeSignalFromMLP backward backward.cu:9-118).
 //
-// One thread per Gaussian walks all receivers: the geometry adjoint of every
-// receiver is summed in registers (no atomics, deterministic order), the MLP
-// output adjoints dz[r] are parked in LDS, and the 194 MLP weight gradients
-// are then formed hidden unit by hidden unit over all receivers.  The cov3D
-// adjoint is linear in dL/dcov3D, so it runs once on the receiver sum.
+// Two kernels, no atomics (summation order is fixed): k_pre_bwd_geo runs the
+// geometry adjoint per (Gaussian, chunk of receivers) and parks partial sums
+// and the MLP output adjoints in the grad workspace; k_pre_bwd_param forms the
+// 194 MLP weight gradients with one thread per (Gaussian, hidden unit), sums
+// the partials and runs the cov3D adjoint once on the receiver sum.
 //
 // Modes: GRAD_EXACT is the true adjoint of the forward; GRAD_REFERENCE
 // reproduces the reference's backward literally (halved x-row of the
@@ -19,6 +19,7 @@ namespace gs {
 
 #define GS_BWD_THREADS 64
 #define GS_BRC 8
+#define GS_PART 12   // per (receiver chunk, Gaussian): dmean[3], dcov3[6], dopacity, db2[2]
 
 template <int MODE>
 __device__ __forceinline__ void geo_bwd(const float* mean, const float* c3, const float* view, int W, int H,
@@ -157,14 +158,20 @@ __device__ __forceinline__ void geo_bwd(const float* mean, const float* c3, cons
     for (int k = 0; k < 3; ++k) dmean[k] += view[k * 4 + 0] * dt[0] + view[k * 4 + 1] * dt[1] + view[k * 4 + 2] * dt[2];
 }
 
+// Kernel 1: one thread per (Gaussian, chunk of GS_BRC receivers).  Recomputes
+// the MLP outputs of the chunk (weights streamed once per chunk), runs the
+// geometry adjoint per receiver and writes the chunk's partial sums
+// (dmean[3], dcov3[6], dopacity, db2[2]) plus dz[r] = dL/d(MLP output) for the
+// weight-gradient kernel.
 template <int MODE>
-__global__ void __launch_bounds__(GS_BWD_THREADS) k_preprocess_bwd(Dims d, PreBwdArgs a)
+__global__ void __launch_bounds__(GS_BWD_THREADS) k_pre_bwd_geo(Dims d, PreBwdArgs a)
 {
-    extern __shared__ float2 s_dz[];   // [R][GS_BWD_THREADS]
     const int g = blockIdx.x * GS_BWD_THREADS + threadIdx.x;
     if (g >= d.P) return;   // no block-level synchronisation below
-    const int lane = threadIdx.x;
     const int P = d.P;
+    const int chunk = blockIdx.y;
+    const int r0 = chunk * GS_BRC;
+    const int nr = min(GS_BRC, d.R - r0);
     const float mean[3] = {a.means[3 * g], a.means[3 * g + 1], a.means[3 * g + 2]};
     const float s[3] = {a.scales[3 * g], a.scales[3 * g + 1], a.scales[3 * g + 2]};
     const float q[4] = {a.rots[4 * g], a.rots[4 * g + 1], a.rots[4 * g + 2], a.rots[4 * g + 3]};
@@ -177,118 +184,140 @@ __global__ void __launch_bounds__(GS_BWD_THREADS) k_preprocess_bwd(Dims d, PreBw
     const float den = (MODE == GRAD_REFERENCE) ? de * de : dist * de;
 
     float dmean[3] = {0, 0, 0}, dc[6] = {0, 0, 0, 0, 0, 0}, dop = 0.0f, db2_0 = 0.0f, db2_1 = 0.0f;
-    for (int r0 = 0; r0 < d.R; r0 += GS_BRC) {
-        const int nr = min(GS_BRC, d.R - r0);
-        float z0[GS_BRC], z1[GS_BRC];
-        if (a.mlp) {
-            const float* W1 = a.mlp + (size_t)g * 96;
-            const float* b1 = a.mlp + (size_t)P * 96 + (size_t)g * 32;
-            const float* W2 = a.mlp + (size_t)P * 128 + (size_t)g * 64;
-            const float* b2 = a.mlp + (size_t)P * 192 + (size_t)g * 2;
+    float z0[GS_BRC], z1[GS_BRC];
+    if (a.mlp) {
+        const float* W1 = a.mlp + (size_t)g * 96;
+        const float* b1 = a.mlp + (size_t)P * 96 + (size_t)g * 32;
+        const float* W2 = a.mlp + (size_t)P * 128 + (size_t)g * 64;
+        const float* b2 = a.mlp + (size_t)P * 192 + (size_t)g * 2;
 #pragma unroll
-            for (int c = 0; c < GS_BRC; ++c) { z0[c] = b2[0]; z1[c] = b2[1]; }
-            for (int j = 0; j < 32; ++j) {
-                const float w0 = W1[j * 3], w1 = W1[j * 3 + 1], w2 = W1[j * 3 + 2], bb = b1[j];
-                const float v0 = W2[j], v1 = W2[32 + j];
+        for (int c = 0; c < GS_BRC; ++c) { z0[c] = b2[0]; z1[c] = b2[1]; }
+        for (int j = 0; j < 32; ++j) {
+            const float w0 = W1[j * 3], w1 = W1[j * 3 + 1], w2 = W1[j * 3 + 2], bb = b1[j];
+            const float v0 = W2[j], v1 = W2[32 + j];
 #pragma unroll
-                for (int c = 0; c < GS_BRC; ++c) {
-                    if (c < nr) {
-                        const float* rx = a.rx + 3 * (r0 + c);
-                        float hs = bb;
-                        hs = hs + w0 * rx[0];
-                        hs = hs + w1 * rx[1];
-                        hs = hs + w2 * rx[2];
-                        const float h = (hs > 0.0f) ? hs : 0.01f * hs;
-                        z0[c] = z0[c] + v0 * h;
-                        z1[c] = z1[c] + v1 * h;
-                    }
+            for (int c = 0; c < GS_BRC; ++c) {
+                if (c < nr) {
+                    const float* rx = a.rx + 3 * (r0 + c);
+                    float hs = bb;
+                    hs = hs + w0 * rx[0];
+                    hs = hs + w1 * rx[1];
+                    hs = hs + w2 * rx[2];
+                    const float h = (hs > 0.0f) ? hs : 0.01f * hs;
+                    z0[c] = z0[c] + v0 * h;
+                    z1[c] = z1[c] + v1 * h;
                 }
             }
-        }
-#pragma unroll
-        for (int c = 0; c < GS_BRC; ++c) {
-            if (c >= nr) break;
-            const int r = r0 + c;
-            const size_t ri = (size_t)r * P + g;
-            float2 dz = make_float2(0.0f, 0.0f);
-            if (a.radii[ri] > 0) {
-                const float4 ga = reinterpret_cast<const float4*>(a.g8)[2 * ri];
-                const float4 gb = reinterpret_cast<const float4*>(a.g8)[2 * ri + 1];
-                const float gv[8] = {ga.x, ga.y, ga.z, ga.w, gb.x, gb.y, gb.z, gb.w};
-                geo_bwd<MODE>(mean, c3, a.views + 16 * r, d.W, d.H, gv, dmean, dc);
-                dop += gv[5];
-                if (a.dmeans2d) {
-                    a.dmeans2d[3 * ri + 0] = gv[0] * (0.5f * (float)d.W);
-                    a.dmeans2d[3 * ri + 1] = gv[1] * (0.5f * (float)d.H);
-                    a.dmeans2d[3 * ri + 2] = 0.0f;
-                }
-                if (a.mlp) {
-                    const float s0 = 1.0f / (1.0f + gs_expf(-(z0[c] - ld)));
-                    const float s1 = 1.0f / (1.0f + gs_expf(-(z1[c] - ld)));
-                    dz.x = gv[6] * s0 * (1.0f - s0);
-                    dz.y = gv[7] * s1 * (1.0f - s1);
-                    const float dld = -(dz.x + dz.y);
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) dmean[k] += -dld * diff[k] / den;
-                    db2_0 += dz.x;
-                    db2_1 += dz.y;
-                } else if (a.dsig) {
-                    a.dsig[2 * ri] = gv[6];
-                    a.dsig[2 * ri + 1] = gv[7];
-                }
-            } else {
-                if (a.dmeans2d) {
-                    a.dmeans2d[3 * ri + 0] = 0.0f;
-                    a.dmeans2d[3 * ri + 1] = 0.0f;
-                    a.dmeans2d[3 * ri + 2] = 0.0f;
-                }
-                if (!a.mlp && a.dsig) {
-                    a.dsig[2 * ri] = 0.0f;
-                    a.dsig[2 * ri + 1] = 0.0f;
-                }
-            }
-            if (a.mlp) s_dz[(size_t)r * GS_BWD_THREADS + lane] = dz;
         }
     }
+#pragma unroll
+    for (int c = 0; c < GS_BRC; ++c) {
+        if (c >= nr) break;
+        const int r = r0 + c;
+        const size_t ri = (size_t)r * P + g;
+        float2 dz = make_float2(0.0f, 0.0f);
+        if (a.radii[ri] > 0) {
+            const float4 ga = reinterpret_cast<const float4*>(a.g8)[2 * ri];
+            const float4 gb = reinterpret_cast<const float4*>(a.g8)[2 * ri + 1];
+            const float gv[8] = {ga.x, ga.y, ga.z, ga.w, gb.x, gb.y, gb.z, gb.w};
+            geo_bwd<MODE>(mean, c3, a.views + 16 * r, d.W, d.H, gv, dmean, dc);
+            dop += gv[5];
+            if (a.dmeans2d) {
+                a.dmeans2d[3 * ri + 0] = gv[0] * (0.5f * (float)d.W);
+                a.dmeans2d[3 * ri + 1] = gv[1] * (0.5f * (float)d.H);
+                a.dmeans2d[3 * ri + 2] = 0.0f;
+            }
+            if (a.mlp) {
+                const float s0 = 1.0f / (1.0f + gs_expf(-(z0[c] - ld)));
+                const float s1 = 1.0f / (1.0f + gs_expf(-(z1[c] - ld)));
+                dz.x = gv[6] * s0 * (1.0f - s0);
+                dz.y = gv[7] * s1 * (1.0f - s1);
+                const float dld = -(dz.x + dz.y);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) dmean[k] += -dld * diff[k] / den;
+                db2_0 += dz.x;
+                db2_1 += dz.y;
+            } else if (a.dsig) {
+                a.dsig[2 * ri] = gv[6];
+                a.dsig[2 * ri + 1] = gv[7];
+            }
+        } else {
+            if (a.dmeans2d) {
+                a.dmeans2d[3 * ri + 0] = 0.0f;
+                a.dmeans2d[3 * ri + 1] = 0.0f;
+                a.dmeans2d[3 * ri + 2] = 0.0f;
+            }
+            if (!a.mlp && a.dsig) {
+                a.dsig[2 * ri] = 0.0f;
+                a.dsig[2 * ri + 1] = 0.0f;
+            }
+        }
+        if (a.mlp) a.dz[ri] = dz;
+    }
+    float* pt = a.part + ((size_t)chunk * P + g) * GS_PART;
+    pt[0] = dmean[0]; pt[1] = dmean[1]; pt[2] = dmean[2];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) pt[3 + k] = dc[k];
+    pt[9] = dop; pt[10] = db2_0; pt[11] = db2_1;
+}
 
-    // MLP weight gradients, hidden unit by hidden unit (backward.cu:95-115)
+// Kernel 2: 32 threads per Gaussian, one per hidden unit j of its MLP: the six
+// weight gradients of unit j summed over all receivers (backward.cu:95-115).
+// Thread j == 0 also sums the chunk partials and runs the cov3D adjoint
+// (backward.cu:308-371), which is linear in dL/dcov3D and so done once.
+__global__ void __launch_bounds__(256) k_pre_bwd_param(Dims d, PreBwdArgs a, int nchunks)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int g = t >> 5, j = t & 31;
+    if (g >= d.P) return;
+    const int P = d.P;
     if (a.mlp && a.dmlp) {
         const float* W1 = a.mlp + (size_t)g * 96;
         const float* b1 = a.mlp + (size_t)P * 96 + (size_t)g * 32;
         const float* W2 = a.mlp + (size_t)P * 128 + (size_t)g * 64;
-        for (int j = 0; j < 32; ++j) {
-            const float w0 = W1[j * 3], w1 = W1[j * 3 + 1], w2 = W1[j * 3 + 2], bb = b1[j];
-            const float v0 = W2[j], v1 = W2[32 + j];
-            float gw0 = 0.0f, gw1 = 0.0f, gw2 = 0.0f, gb = 0.0f, gv0 = 0.0f, gv1 = 0.0f;
-            for (int r = 0; r < d.R; ++r) {
-                const float2 dz = s_dz[(size_t)r * GS_BWD_THREADS + lane];
-                const float* rx = a.rx + 3 * r;
-                float hs = bb;
-                hs = hs + w0 * rx[0];
-                hs = hs + w1 * rx[1];
-                hs = hs + w2 * rx[2];
-                const float h = (hs > 0.0f) ? hs : 0.01f * hs;
-                gv0 += dz.x * h;
-                gv1 += dz.y * h;
-                const float dh = dz.x * v0 + dz.y * v1;
-                const float dp = (hs > 0.0f) ? dh : 0.01f * dh;
-                gw0 += dp * rx[0];
-                gw1 += dp * rx[1];
-                gw2 += dp * rx[2];
-                gb += dp;
-            }
-            a.dmlp[(size_t)g * 96 + j * 3 + 0] = gw0;
-            a.dmlp[(size_t)g * 96 + j * 3 + 1] = gw1;
-            a.dmlp[(size_t)g * 96 + j * 3 + 2] = gw2;
-            a.dmlp[(size_t)P * 96 + (size_t)g * 32 + j] = gb;
-            a.dmlp[(size_t)P * 128 + (size_t)g * 64 + j] = gv0;
-            a.dmlp[(size_t)P * 128 + (size_t)g * 64 + 32 + j] = gv1;
+        const float w0 = W1[j * 3], w1 = W1[j * 3 + 1], w2 = W1[j * 3 + 2], bb = b1[j];
+        const float v0 = W2[j], v1 = W2[32 + j];
+        float gw0 = 0.0f, gw1 = 0.0f, gw2 = 0.0f, gb = 0.0f, gv0 = 0.0f, gv1 = 0.0f;
+        for (int r = 0; r < d.R; ++r) {
+            const float2 dz = a.dz[(size_t)r * P + g];
+            const float* rx = a.rx + 3 * r;
+            float hs = bb;
+            hs = hs + w0 * rx[0];
+            hs = hs + w1 * rx[1];
+            hs = hs + w2 * rx[2];
+            const float h = (hs > 0.0f) ? hs : 0.01f * hs;
+            gv0 += dz.x * h;
+            gv1 += dz.y * h;
+            const float dh = dz.x * v0 + dz.y * v1;
+            const float dp = (hs > 0.0f) ? dh : 0.01f * dh;
+            gw0 += dp * rx[0];
+            gw1 += dp * rx[1];
+            gw2 += dp * rx[2];
+            gb += dp;
         }
-        a.dmlp[(size_t)P * 192 + (size_t)g * 2] = db2_0;
-        a.dmlp[(size_t)P * 192 + (size_t)g * 2 + 1] = db2_1;
+        a.dmlp[(size_t)g * 96 + j * 3 + 0] = gw0;
+        a.dmlp[(size_t)g * 96 + j * 3 + 1] = gw1;
+        a.dmlp[(size_t)g * 96 + j * 3 + 2] = gw2;
+        a.dmlp[(size_t)P * 96 + (size_t)g * 32 + j] = gb;
+        a.dmlp[(size_t)P * 128 + (size_t)g * 64 + j] = gv0;
+        a.dmlp[(size_t)P * 128 + (size_t)g * 64 + 32 + j] = gv1;
     }
-
-    // cov3D adjoint -> scale, rotation (backward.cu:308-371)
+    if (j != 0) return;
+    float acc[GS_PART];
+#pragma unroll
+    for (int k = 0; k < GS_PART; ++k) acc[k] = 0.0f;
+    for (int c = 0; c < nchunks; ++c) {
+        const float* pt = a.part + ((size_t)c * P + g) * GS_PART;
+#pragma unroll
+        for (int k = 0; k < GS_PART; ++k) acc[k] += pt[k];
+    }
+    if (a.mlp && a.dmlp) {
+        a.dmlp[(size_t)P * 192 + (size_t)g * 2] = acc[10];
+        a.dmlp[(size_t)P * 192 + (size_t)g * 2 + 1] = acc[11];
+    }
+    const float* dc = acc + 3;
+    const float s[3] = {a.scales[3 * g], a.scales[3 * g + 1], a.scales[3 * g + 2]};
+    const float q[4] = {a.rots[4 * g], a.rots[4 * g + 1], a.rots[4 * g + 2], a.rots[4 * g + 3]};
     {
         const float r = q[0], x = q[1], y = q[2], z = q[3];
         const float Rq[3][3] = {
@@ -312,8 +341,9 @@ __global__ void __launch_bounds__(GS_BWD_THREADS) k_preprocess_bwd(Dims d, PreBw
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
-                for (int j = 0; j < 3; ++j) dD += Rq[i][k] * G[i][j] * Rq[j][k];
-            const float f = (MODE == GRAD_REFERENCE) ? 2.0f * se[k] : 2.0f * se[k] * a.mod;
+                for (int jj = 0; jj < 3; ++jj) dD += Rq[i][k] * G[i][jj] * Rq[jj][k];
+            // dD/ds = 2 mod^2 s; the reference drops one factor mod (backward.cu:353-355)
+            const float f = (a.mode == GRAD_REFERENCE) ? 2.0f * se[k] : 2.0f * se[k] * a.mod;
             a.dscales[3 * g + k] = dD * f;
         }
         const float dRr[3][3] = {{0, -2 * z, 2 * y}, {2 * z, 0, -2 * x}, {-2 * y, 2 * x, 0}};
@@ -324,18 +354,18 @@ __global__ void __launch_bounds__(GS_BWD_THREADS) k_preprocess_bwd(Dims d, PreBw
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                dq[0] += GR[i][j] * dRr[i][j];
-                dq[1] += GR[i][j] * dRx[i][j];
-                dq[2] += GR[i][j] * dRy[i][j];
-                dq[3] += GR[i][j] * dRz[i][j];
+            for (int jj = 0; jj < 3; ++jj) {
+                dq[0] += GR[i][jj] * dRr[i][jj];
+                dq[1] += GR[i][jj] * dRx[i][jj];
+                dq[2] += GR[i][jj] * dRy[i][jj];
+                dq[3] += GR[i][jj] * dRz[i][jj];
             }
 #pragma unroll
         for (int k = 0; k < 4; ++k) a.drots[4 * g + k] = dq[k];
     }
 #pragma unroll
-    for (int k = 0; k < 3; ++k) a.dmeans[3 * g + k] = dmean[k];
-    a.dopac[g] = dop;
+    for (int k = 0; k < 3; ++k) a.dmeans[3 * g + k] = acc[k];
+    a.dopac[g] = acc[9];
     if (a.dcov3)
 #pragma unroll
         for (int k = 0; k < 6; ++k) a.dcov3[6 * g + k] = dc[k];
@@ -343,13 +373,17 @@ __global__ void __launch_bounds__(GS_BWD_THREADS) k_preprocess_bwd(Dims d, PreBw
 
 hipError_t launch_preprocess_bwd(const Dims& d, const PreBwdArgs& a, hipStream_t s)
 {
-    if (d.P == 0) return hipSuccess;
-    const int blocks = (d.P + GS_BWD_THREADS - 1) / GS_BWD_THREADS;
-    const size_t lds = a.mlp ? (size_t)d.R * GS_BWD_THREADS * sizeof(float2) : 0;
+    if (d.P == 0 || d.R == 0) return hipSuccess;
+    const int nchunks = (d.R + GS_BRC - 1) / GS_BRC;
+    const dim3 g1((d.P + GS_BWD_THREADS - 1) / GS_BWD_THREADS, nchunks);
     if (a.mode == GRAD_REFERENCE)
-        hipLaunchKernelGGL(k_preprocess_bwd<GRAD_REFERENCE>, dim3(blocks), dim3(GS_BWD_THREADS), lds, s, d, a);
+        hipLaunchKernelGGL(k_pre_bwd_geo<GRAD_REFERENCE>, g1, dim3(GS_BWD_THREADS), 0, s, d, a);
     else
-        hipLaunchKernelGGL(k_preprocess_bwd<GRAD_EXACT>, dim3(blocks), dim3(GS_BWD_THREADS), lds, s, d, a);
+        hipLaunchKernelGGL(k_pre_bwd_geo<GRAD_EXACT>, g1, dim3(GS_BWD_THREADS), 0, s, d, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int threads = d.P * 32;
+    hipLaunchKernelGGL(k_pre_bwd_param, dim3((threads + 255) / 256), dim3(256), 0, s, d, a, nchunks);
     return hipGetLastError();
 }
 

@@ -1,12 +1,12 @@
 // Front-to-back alpha blend (renderCUDA, forward.cu:151-249) and its adjoint
 // (renderCUDA, backward.cu:442-616) for R receivers.
 //
-// Work unit = one wave64 workgroup.  With PPL = 1 a wave owns one 8x8 quarter
-// of a 16x16 binning tile (four waves per tile: many small work items, best
-// for one receiver); with PPL = 4 a wave owns the whole tile and every lane
-// carries four independent pixel chains, one per quarter (best for batches:
-// the LDS staging of each list entry and, in the backward, the 8-value wave
-// reduction are shared by 256 pixels).  The 16x16 binning grid itself is part
+// Work unit = one wave64 workgroup owning PPL of the four 8x8 quarters of a
+// 16x16 binning tile; every lane carries PPL independent pixel chains, one per
+// quarter.  PPL = 1 gives many small work items (best for one receiver and
+// for the forward); PPL = 4 shares the LDS staging of each list entry and, in
+// the backward, the 8-value wave reduction among 256 pixels (best for the
+// batched backward).  The 16x16 binning grid itself is part
 // of the numerics and never changes.  A wave stages 64 list entries at a time
 // in LDS and exits as soon as all of its pixels are saturated.  Work-group ids
 // are mapped so that all waves of a tile run on the same XCD (shared L2).
@@ -30,17 +30,18 @@ struct Pixels {
 template <int PPL>
 __device__ __forceinline__ Pixels<PPL> map_pixels(const Dims& d)
 {
+    constexpr int WPT = 4 / PPL;   // waves per tile
     Pixels<PPL> m;
     const int b = blockIdx.x;
     const int xcd = b & 7, s = b >> 3;
-    const int sub = (PPL == 1) ? (s & 3) : 0;
-    m.tile = ((PPL == 1) ? (s >> 2) : s) * 8 + xcd;
+    const int sub = s % WPT;
+    m.tile = (s / WPT) * 8 + xcd;
     m.valid = m.tile < d.T;
     const int lane = threadIdx.x;
     const int tx = m.valid ? m.tile % d.gx : 0, ty = m.valid ? m.tile / d.gx : 0;
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
-        const int q = (PPL == 1) ? sub : k;
+        const int q = sub * PPL + k;   // quarter of the tile
         m.px[k] = tx * GS_BLOCK + (q & 1) * 8 + (lane & 7);
         m.py[k] = ty * GS_BLOCK + (q >> 1) * 8 + (lane >> 3);
         m.inside[k] = m.px[k] < d.W && m.py[k] < d.H;
@@ -91,28 +92,36 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
         }
         __syncthreads();
         const int n = (int)min((uint32_t)GS_WAVE, range.y - start);
+        // one entry of look-ahead on the LDS reads
+        float4 A = s0[0], B = s1[0];
+        float2 S = ss[0];
         for (int j = 0; j < n; ++j) {
-            const float4 A = s0[j];
-            const float4 B = s1[j];
+            const int jn = (j + 1 < n) ? j + 1 : j;
+            const float4 An = s0[jn], Bn = s1[jn];
+            const float2 Sn = ss[jn];
 #pragma unroll
             for (int k = 0; k < PPL; ++k) {
-                if (done[k]) continue;
-                contributor[k]++;
+                const bool act = !done[k];
+                contributor[k] += act ? 1u : 0u;
                 const float dx = A.x - pxf[k], dy = A.y - pyf[k];
                 const float power = pair_power(A.z, A.w, B.x, dx, dy);
-                if (power > 0.0f) continue;
-                if (power < B.z) continue;
-                const float alpha = fminf(0.99f, B.y * gs_expf_nonpos(power));
-                if (alpha < 1.0f / 255.0f) continue;
-                const float test_T = T[k] * (1.0f - alpha);
-                if (test_T < 0.0001f) { done[k] = true; continue; }
-                const float2 S = ss[j];
-                const float w = alpha * T[k];
-                are[k] = fmaf(S.x, w, are[k]);
-                aim[k] = fmaf(S.y, w, aim[k]);
-                T[k] = test_T;
-                last[k] = contributor[k];
+                if (act && !(power > 0.0f) && !(power < B.z)) {
+                    const float alpha = fminf(0.99f, B.y * gs_expf_nonpos(power));
+                    const float test_T = T[k] * (1.0f - alpha);
+                    const bool pass = !(alpha < 1.0f / 255.0f);
+                    const bool term = pass && test_T < 0.0001f;
+                    const bool blend = pass && !term;
+                    const float w = alpha * T[k];
+                    are[k] = blend ? fmaf(S.x, w, are[k]) : are[k];
+                    aim[k] = blend ? fmaf(S.y, w, aim[k]) : aim[k];
+                    T[k] = blend ? test_T : T[k];
+                    last[k] = blend ? contributor[k] : last[k];
+                    done[k] = done[k] || term;
+                }
             }
+            A = An;
+            B = Bn;
+            S = Sn;
         }
         __syncthreads();
     }
@@ -206,10 +215,9 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
         }
         __syncthreads();
         const int n = min(GS_WAVE, top + 1);
-        for (int j = 0; j < n; ++j) {
-            const float4 A = s0[j];
-            const float4 B = s1[j];
-            const float2 S = ss[j];
+        float4 A = s0[0], B = s1[0];
+        float2 S = ss[0];
+        for (int j = 0; j < n; ++j, A = s0[j < n ? j : 0], B = s1[j < n ? j : 0], S = ss[j < n ? j : 0]) {
             const float qa = -2.0f * A.z, qb = -A.w, qc = -2.0f * B.x;
             float v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             bool anyc = false;
@@ -274,12 +282,12 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
 static inline dim3 render_grid(const Dims& d, int ppl)
 {
     const int tiles8 = (d.T + 7) / 8 * 8;
-    return dim3(tiles8 * (ppl == 1 ? 4 : 1), d.R);
+    return dim3(tiles8 * (4 / ppl), d.R);
 }
 
 int choose_ppl(const Dims& d, int requested)
 {
-    if (requested == 1 || requested == 4) return requested;
+    if (requested == 1 || requested == 2 || requested == 4) return requested;
     // enough tiles to keep every SIMD busy with one wave per tile
     return ((long long)d.R * d.T >= 2048) ? 4 : 1;
 }
@@ -287,24 +295,27 @@ int choose_ppl(const Dims& d, int requested)
 hipError_t launch_render_fwd(const Dims& d, const RenderFwdArgs& a, int ppl, hipStream_t s)
 {
     if (d.R == 0 || d.T == 0) return hipSuccess;
-    if (ppl == 4)
-        hipLaunchKernelGGL(k_render_fwd<4>, render_grid(d, 4), dim3(GS_WAVE), 0, s, d, a);
-    else
-        hipLaunchKernelGGL(k_render_fwd<1>, render_grid(d, 1), dim3(GS_WAVE), 0, s, d, a);
+    const dim3 grid = render_grid(d, ppl);
+    if (ppl == 4) hipLaunchKernelGGL(k_render_fwd<4>, grid, dim3(GS_WAVE), 0, s, d, a);
+    else if (ppl == 2) hipLaunchKernelGGL(k_render_fwd<2>, grid, dim3(GS_WAVE), 0, s, d, a);
+    else hipLaunchKernelGGL(k_render_fwd<1>, grid, dim3(GS_WAVE), 0, s, d, a);
     return hipGetLastError();
+}
+
+template <int MODE>
+static void launch_bwd_mode(const Dims& d, const RenderBwdArgs& a, int ppl, hipStream_t s)
+{
+    const dim3 grid = render_grid(d, ppl);
+    if (ppl == 4) hipLaunchKernelGGL((k_render_bwd<MODE, 4>), grid, dim3(GS_WAVE), 0, s, d, a);
+    else if (ppl == 2) hipLaunchKernelGGL((k_render_bwd<MODE, 2>), grid, dim3(GS_WAVE), 0, s, d, a);
+    else hipLaunchKernelGGL((k_render_bwd<MODE, 1>), grid, dim3(GS_WAVE), 0, s, d, a);
 }
 
 hipError_t launch_render_bwd(const Dims& d, const RenderBwdArgs& a, int ppl, hipStream_t s)
 {
     if (d.R == 0 || d.T == 0) return hipSuccess;
-    const dim3 grid = render_grid(d, ppl);
-    if (a.mode == GRAD_REFERENCE) {
-        if (ppl == 4) hipLaunchKernelGGL((k_render_bwd<GRAD_REFERENCE, 4>), grid, dim3(GS_WAVE), 0, s, d, a);
-        else hipLaunchKernelGGL((k_render_bwd<GRAD_REFERENCE, 1>), grid, dim3(GS_WAVE), 0, s, d, a);
-    } else {
-        if (ppl == 4) hipLaunchKernelGGL((k_render_bwd<GRAD_EXACT, 4>), grid, dim3(GS_WAVE), 0, s, d, a);
-        else hipLaunchKernelGGL((k_render_bwd<GRAD_EXACT, 1>), grid, dim3(GS_WAVE), 0, s, d, a);
-    }
+    if (a.mode == GRAD_REFERENCE) launch_bwd_mode<GRAD_REFERENCE>(d, a, ppl, s);
+    else launch_bwd_mode<GRAD_EXACT>(d, a, ppl, s);
     return hipGetLastError();
 }
 

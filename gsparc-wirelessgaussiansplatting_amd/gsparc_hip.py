@@ -29,7 +29,7 @@ _sz = C.c_size_t
 class Layout(C.Structure):
     _fields_ = [(n, C.c_size_t) for n in (
         "splat", "sig", "depth", "cov3", "geom_bytes",
-        "final_t", "n_contrib", "ranges", "segcnt", "cursor", "segbase", "total", "image_bytes",
+        "final_t", "n_contrib", "ranges", "segcnt", "cursor", "segbase", "long_list", "total", "image_bytes",
         "keys", "vals", "keys_tmp", "vals_tmp", "plist", "keys_sorted", "bin_bytes")]
 
 

@@ -46,7 +46,7 @@ typedef struct gsparc_layout {
     /* geom */
     size_t splat, sig, depth, cov3, geom_bytes;
     /* image */
-    size_t final_t, n_contrib, ranges, segcnt, cursor, segbase, total, image_bytes;
+    size_t final_t, n_contrib, ranges, segcnt, cursor, segbase, long_list, total, image_bytes;
     /* bin */
     size_t keys, vals, keys_tmp, vals_tmp, plist, keys_sorted, bin_bytes;
 } gsparc_layout;
@@ -55,9 +55,10 @@ const char* gsparc_version(void);
 const char* gsparc_last_error(void);
 
 /* Tuning knobs (process-wide; no counterpart in the reference):
- *   "render_ppl"  pixels per lane of the blend kernels: 0 = automatic,
- *                 1 = one 8x8 quarter-tile per wave, 4 = one 16x16 tile per
- *                 wave.  Changes speed only, never results. */
+ *   "render_fwd_ppl", "render_bwd_ppl"  pixels per lane of the forward and
+ *       backward blend kernels: 0 = automatic, 1 = one 8x8 quarter-tile per
+ *       wave, 2 = half a tile per wave, 4 = one 16x16 tile per wave.  Speed only: the forward result is
+ *       bit-identical either way, the backward differs by summation order. */
 int gsparc_set_option(const char* name, int value);
 int gsparc_get_option(const char* name, int* value);
 

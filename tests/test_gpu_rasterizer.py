@@ -18,13 +18,15 @@ import wireless_rasterizer as wr
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(params=[1, 4], ids=["ppl1", "ppl4"])
+@pytest.fixture(params=[1, 2, 4], ids=["ppl1", "ppl2", "ppl4"])
 def ppl(request):
     """Run under both blend work mappings (8x8 quarter-tile or 16x16 tile per wave)."""
     import gsparc_hip as gh
-    gh.set_option("render_ppl", request.param)
+    gh.set_option("render_fwd_ppl", request.param)
+    gh.set_option("render_bwd_ppl", request.param)
     yield request.param
-    gh.set_option("render_ppl", 0)
+    gh.set_option("render_fwd_ppl", 0)
+    gh.set_option("render_bwd_ppl", 0)
 
 SPEC_TOL = 1e-5     # relative to the peak of the spectrum
 GRAD_TOL = 1e-4     # relative to the largest |gradient| of the tensor
@@ -111,6 +113,19 @@ def test_forward_long_segments():
     assert lens.max() > 8192
     for r in range(sc.R):
         _check_forward(sc, st, r)
+
+
+def test_equal_depth_ties_in_index_order():
+    """Gaussians sharing a centre have identical depth keys; the reference's
+    stable sort keeps them in index order (rasterizer_impl.cu:295-300)."""
+    sc = S.small_scene(23, 3000, walls=True)
+    sc.means[1::2] = sc.means[0::2]
+    sc.means[2::3] = sc.means[0::3]
+    _, out = gpu_forward(sc)
+    st = read_state(sc, out)
+    keys = st["keys"]
+    assert (keys[1:] == keys[:-1]).sum() > 1000
+    _check_forward(sc, st, 0)
 
 
 def test_empty_and_culled():

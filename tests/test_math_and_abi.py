@@ -86,12 +86,12 @@ def test_buffer_layout_sizes():
 
 def test_tuning_options():
     import gsparc_hip as gh
-    assert gh.get_option("render_ppl") == 0
-    gh.set_option("render_ppl", 4)
-    assert gh.get_option("render_ppl") == 4
-    gh.set_option("render_ppl", 0)
+    assert gh.get_option("render_bwd_ppl") == 0
+    gh.set_option("render_bwd_ppl", 4)
+    assert gh.get_option("render_bwd_ppl") == 4
+    gh.set_option("render_bwd_ppl", 0)
     import pytest
-    with pytest.raises(RuntimeError, match="render_ppl"):
-        gh.set_option("render_ppl", 3)
+    with pytest.raises(RuntimeError, match="render_fwd_ppl"):
+        gh.set_option("render_fwd_ppl", 3)
     with pytest.raises(RuntimeError, match="unknown option"):
         gh.set_option("no_such_option", 1)
