@@ -27,6 +27,7 @@ PEAK_HBM_GBS = 8000.0
 PEAK_FP32_TFLOPS = 157.3
 FWD_FLOP_PER_PAIR = 15      # per visited (pixel, Gaussian) pair
 BWD_FLOP_PER_PAIR = 40      # per back-propagated pair
+CULL_FLOP = 40              # per list entry tested against a wave's pixel block
 
 
 def parse():
@@ -42,6 +43,8 @@ def parse():
     ap.add_argument("--grad-mode", default="reference", choices=["reference", "exact"])
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run a plain device-to-device copy of this size (counter calibration)")
+    ap.add_argument("--fwd-ppl", type=int, default=0, help="forward blend pixels per lane (0: library default)")
+    ap.add_argument("--bwd-ppl", type=int, default=0, help="backward blend pixels per lane (0: automatic)")
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
 
@@ -69,6 +72,8 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     gh.lib()
+    gh.set_option("render_fwd_ppl", args.fwd_ppl)
+    gh.set_option("render_bwd_ppl", args.bwd_ppl)
     P, R = args.gaussians, args.receivers
     sc = S.config2(P=P, R=R)
     tgt_sc = S.config2(seed=3, P=P, R=R)
@@ -161,19 +166,14 @@ def main():
         step_batched()
     torch.cuda.synchronize()
     kt = gh.profile_collect()
-    pairs_fwd = gh.profile_pairs() / nprof
+    cnt = {k: v / nprof for k, v in gh.profile_counts().items()}
     gh.profile_enable(False)
     with torch.no_grad():
-        K, ore, oim, radii, geom, binbuf, img = wr.forward_batched(means, mlp, opac, scales, rots, sc.H, sc.W, 1.0,
-                                                                   views, txp, rxp)
-        L = gh.layout(P, R, sc.W, sc.H, K)
-        off = (img.data_ptr() + 255) // 256 * 256 - img.data_ptr()
-        nc = img[off + L.n_contrib: off + L.n_contrib + 4 * R * sc.W * sc.H].view(torch.int32)
-        pairs_bwd = float(nc.to(torch.int64).sum())
+        K = wr.forward_batched(means, mlp, opac, scales, rots, sc.H, sc.W, 1.0, views, txp, rxp)[0]
     N = sc.W * sc.H
     work = {
-        "render_fwd": ("valu", pairs_fwd * FWD_FLOP_PER_PAIR, "TFLOP/s"),
-        "render_bwd": ("valu", pairs_bwd * BWD_FLOP_PER_PAIR, "TFLOP/s"),
+        "render_fwd": ("valu", cnt["fwd_pairs"] * FWD_FLOP_PER_PAIR + cnt["fwd_entries"] * CULL_FLOP, "TFLOP/s"),
+        "render_bwd": ("valu", cnt["bwd_pairs"] * BWD_FLOP_PER_PAIR + cnt["bwd_entries"] * CULL_FLOP, "TFLOP/s"),
         "preprocess_fwd": ("hbm", P * 820 + 44 * P * R, "GB/s"),
         "preprocess_bwd": ("hbm", P * (820 + 32 * 2 + 776) + 32 * P * R, "GB/s"),
         "bin_sort_small": ("hbm", 12.0 * K, "GB/s"),
@@ -200,10 +200,10 @@ def main():
             "peak": PEAK_HBM_GBS if d.get("unit") == "GB/s" else PEAK_FP32_TFLOPS, "unit": d.get("unit"),
             "frac": d.get("frac"), "traffic": None,
             "ms_per_launch": d.get("ms_per_launch")}
-    if dom == "render_fwd":
-        roof["pairs_per_launch"] = pairs_fwd
-    if dom == "render_bwd":
-        roof["pairs_per_launch"] = pairs_bwd
+    if dom in ("render_fwd", "render_bwd"):
+        side = dom[-3:]
+        roof["pairs_per_launch"] = cnt[f"{side}_pairs"]
+        roof["entry_tests_per_launch"] = cnt[f"{side}_entries"]
 
     cpu = None
     if args.cpu_receivers > 0:

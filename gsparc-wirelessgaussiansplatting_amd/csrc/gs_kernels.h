@@ -80,10 +80,14 @@ struct RenderFwdArgs {
     uint32_t* ncontrib;      // [R*N]
     const uint32_t* total;   // K as computed on the device
     uint64_t Kcap;           // capacity of plist (nothing is read when K > Kcap)
-    unsigned long long* pairs;   // optional: += (pixel, Gaussian) pairs visited
+    unsigned long long* counts;  // optional profiling counters (see below)
 };
-// pixels per lane of the blend kernels: 1 (8x8 quarter-tile per wave) or 4
-// (16x16 tile per wave); requested = 0 picks by the number of tiles
+// Profiling counters (u64 x 4, optional): [0] forward (pixel, Gaussian) pairs
+// evaluated, [1] backward pairs evaluated, [2] forward wave-level entry
+// tests, [3] backward wave-level entry tests.
+// pixels per lane of the blend kernels: 1 (8x8 quarter-tile per wave), 2
+// (16x8 half tile) or 4 (16x16 tile per wave); requested = 0 picks by the
+// number of tiles
 int choose_ppl(const Dims& d, int requested);
 hipError_t launch_render_fwd(const Dims& d, const RenderFwdArgs& a, int ppl, hipStream_t s);
 
@@ -100,6 +104,7 @@ struct RenderBwdArgs {
     int mode;
     const uint32_t* total;
     uint64_t Kcap;
+    unsigned long long* counts;  // optional profiling counters
 };
 hipError_t launch_render_bwd(const Dims& d, const RenderBwdArgs& a, int ppl, hipStream_t s);
 

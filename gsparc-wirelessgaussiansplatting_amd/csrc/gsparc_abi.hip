@@ -90,7 +90,7 @@ bool g_prof_on = false;
 std::vector<ProfPending> g_prof_pending;
 std::vector<hipEvent_t> g_prof_free;
 std::map<std::string, std::pair<double, long long>> g_prof_acc;
-unsigned long long* g_pairs_dev = nullptr;
+unsigned long long* g_counts_dev = nullptr;   // 4 profiling counters of the blend kernels
 
 hipEvent_t prof_event()
 {
@@ -298,8 +298,8 @@ int gsparc_rasterize_forward_render(int P, int R, int W, int H, const float* emi
     ra.ncontrib = (uint32_t*)(ib + L.n_contrib);
     ra.total = ba.total;
     ra.Kcap = K_capacity;
-    ra.pairs = g_prof_on ? g_pairs_dev : nullptr;
-    GS_LAUNCH("render_fwd", gs::launch_render_fwd(d, ra, g_opt_fwd_ppl ? g_opt_fwd_ppl : 1, s), s);
+    ra.counts = g_prof_on ? g_counts_dev : nullptr;
+    GS_LAUNCH("render_fwd", gs::launch_render_fwd(d, ra, gs::choose_ppl(d, g_opt_fwd_ppl), s), s);
     return GSPARC_OK;
 }
 
@@ -340,6 +340,7 @@ int gsparc_rasterize_backward(int grad_mode, int P, int R, int W, int H, const f
     ra.mode = grad_mode;
     ra.total = (const uint32_t*)(ib + L.total);
     ra.Kcap = K_capacity;
+    ra.counts = g_prof_on ? g_counts_dev : nullptr;
     GS_LAUNCH("render_bwd", gs::launch_render_bwd(d, ra, gs::choose_ppl(d, g_opt_bwd_ppl), s), s);
     gs::PreBwdArgs pa;
     pa.means = means; pa.scales = scales; pa.rots = rotations; pa.mlp = emission_mlps; pa.views = viewmatrices;
@@ -404,8 +405,8 @@ int gsparc_profile_enable(int on)
     }
     g_prof_pending.clear();
     g_prof_acc.clear();
-    if (on && !g_pairs_dev) GS_HIP(hipMalloc((void**)&g_pairs_dev, sizeof(unsigned long long)));
-    if (on) GS_HIP(hipMemset(g_pairs_dev, 0, sizeof(unsigned long long)));
+    if (on && !g_counts_dev) GS_HIP(hipMalloc((void**)&g_counts_dev, 4 * sizeof(unsigned long long)));
+    if (on) GS_HIP(hipMemset(g_counts_dev, 0, 4 * sizeof(unsigned long long)));
     g_prof_on = on != 0;
     return GSPARC_OK;
 }
@@ -443,14 +444,14 @@ int gsparc_profile_query(int i, char* name, int name_len, double* total_ms, long
     return GSPARC_OK;
 }
 
-int gsparc_profile_pairs(uint64_t* pairs)
+int gsparc_profile_counts(uint64_t* counts)
 {
-    if (!pairs) return fail(GSPARC_ERR_ARG, "null pointer");
-    *pairs = 0;
-    if (!g_pairs_dev) return GSPARC_OK;
-    unsigned long long v = 0;
-    GS_HIP(hipMemcpy(&v, g_pairs_dev, sizeof(v), hipMemcpyDeviceToHost));
-    *pairs = v;
+    if (!counts) return fail(GSPARC_ERR_ARG, "null pointer");
+    for (int i = 0; i < 4; ++i) counts[i] = 0;
+    if (!g_counts_dev) return GSPARC_OK;
+    unsigned long long v[4] = {0, 0, 0, 0};
+    GS_HIP(hipMemcpy(v, g_counts_dev, sizeof(v), hipMemcpyDeviceToHost));
+    for (int i = 0; i < 4; ++i) counts[i] = v[i];
     return GSPARC_OK;
 }
 

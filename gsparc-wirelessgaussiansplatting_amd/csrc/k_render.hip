@@ -3,12 +3,15 @@
 //
 // Work unit = one wave64 workgroup owning PPL of the four 8x8 quarters of a
 // 16x16 binning tile; every lane carries PPL independent pixel chains, one per
-// quarter.  PPL = 1 gives many small work items (best for one receiver and
-// for the forward); PPL = 4 shares the LDS staging of each list entry and, in
-// the backward, the 8-value wave reduction among 256 pixels (best for the
-// batched backward).  The 16x16 binning grid itself is part
-// of the numerics and never changes.  A wave stages 64 list entries at a time
-// in LDS and exits as soon as all of its pixels are saturated.  Work-group ids
+// quarter.  PPL = 1 gives many small work items and the finest culling block
+// (best for one receiver); PPL = 2 and 4 share the staging and culling test of
+// each list entry and, in the backward, the 8-value wave reduction among 128
+// or 256 pixels (PPL = 2 is best for receiver batches).  The 16x16 binning grid itself is part of the numerics
+// and never changes.  A wave stages 64 list entries at a time in LDS, testing
+// each against its own pixel block on the way in (block_misses): only entries
+// that can reach alpha >= 1/255 somewhere in the block are walked, so the
+// per-pixel loop sees a fraction of the tile list.  A wave exits as soon as
+// all of its pixels are saturated.  Work-group ids
 // are mapped so that all waves of a tile run on the same XCD (shared L2).
 //
 // Per pair the exponential is skipped when power < thr: then
@@ -25,6 +28,7 @@ struct Pixels {
     bool valid;
     int px[PPL], py[PPL];
     bool inside[PPL];
+    float x0, x1, y0, y1;   // pixel-centre bounds of the wave's block (inclusive)
 };
 
 template <int PPL>
@@ -39,17 +43,61 @@ __device__ __forceinline__ Pixels<PPL> map_pixels(const Dims& d)
     m.valid = m.tile < d.T;
     const int lane = threadIdx.x;
     const int tx = m.valid ? m.tile % d.gx : 0, ty = m.valid ? m.tile / d.gx : 0;
+    int qx0 = 1, qx1 = 0, qy0 = 1, qy1 = 0;
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
         const int q = sub * PPL + k;   // quarter of the tile
         m.px[k] = tx * GS_BLOCK + (q & 1) * 8 + (lane & 7);
         m.py[k] = ty * GS_BLOCK + (q >> 1) * 8 + (lane >> 3);
         m.inside[k] = m.px[k] < d.W && m.py[k] < d.H;
+        qx0 = min(qx0, q & 1); qx1 = max(qx1, q & 1);
+        qy0 = min(qy0, q >> 1); qy1 = max(qy1, q >> 1);
     }
+    m.x0 = (float)(tx * GS_BLOCK + qx0 * 8);
+    m.x1 = (float)(tx * GS_BLOCK + qx1 * 8 + 7);
+    m.y0 = (float)(ty * GS_BLOCK + qy0 * 8);
+    m.y1 = (float)(ty * GS_BLOCK + qy1 * 8 + 7);
     return m;
 }
 
-template <int PPL>
+// Wave-level culling of one list entry.  Returns true only when every pixel
+// centre of the block [x0,x1] x [y0,y1] is certain to evaluate power < thr in
+// the per-pixel loop, so that skipping the entry for the whole block changes
+// no result bit.
+//
+// power(d) = ha dx^2 + nb dx dy + hc dy^2 (d = mean - pixel) is concave for a
+// valid conic (ha, hc < 0), so its maximum over the rectangle of offsets is 0
+// when the rectangle holds the origin and otherwise lies on an edge, where it
+// is a 1-D concave quadratic maximised in closed form and clamped.  Rounding:
+// the per-pixel float evaluation (with the rounded offsets) exceeds the exact
+// value by at most ~5u * S, and the float maximum here undershoots the exact
+// one by at most ~4u * S, where u = 2^-24 and S bounds |ha| dx^2 + |nb||dx dy|
+// + |hc| dy^2 over the block; the test keeps a margin of 4e-6 * S (~67u * S).
+// Anything non-finite or non-concave is never culled.
+__device__ __forceinline__ bool block_misses(float mx, float my, float ha, float nb, float hc, float thr,
+                                             float x0, float x1, float y0, float y1)
+{
+    if (!(ha < 0.0f && hc < 0.0f)) return false;
+    const float dxl = mx - x1, dxh = mx - x0, dyl = my - y1, dyh = my - y0;
+    const float DX = fmaxf(fabsf(dxl), fabsf(dxh)), DY = fmaxf(fabsf(dyl), fabsf(dyh));
+    const float smax = fmaf(-ha * DX, DX, fmaf(fabsf(nb) * DX, DY, -hc * DY * DY));
+    float pmax = 0.0f;
+    if (!(dxl <= 0.0f && dxh >= 0.0f && dyl <= 0.0f && dyh >= 0.0f)) {
+        const float ky = -0.5f * nb * __builtin_amdgcn_rcpf(hc);   // argmax y on an edge x = X is ky X
+        const float kx = -0.5f * nb * __builtin_amdgcn_rcpf(ha);
+        const float ya = fminf(fmaxf(ky * dxl, dyl), dyh), yb = fminf(fmaxf(ky * dxh, dyl), dyh);
+        const float xa = fminf(fmaxf(kx * dyl, dxl), dxh), xb = fminf(fmaxf(kx * dyh, dxl), dxh);
+        const float e0 = fmaf(ha * dxl, dxl, fmaf(nb * dxl, ya, hc * ya * ya));
+        const float e1 = fmaf(ha * dxh, dxh, fmaf(nb * dxh, yb, hc * yb * yb));
+        const float e2 = fmaf(ha * xa, xa, fmaf(nb * xa, dyl, hc * dyl * dyl));
+        const float e3 = fmaf(ha * xb, xb, fmaf(nb * xb, dyh, hc * dyh * dyh));
+        pmax = fmaxf(fmaxf(e0, e1), fmaxf(e2, e3));
+        if (!(e0 == e0 && e1 == e1 && e2 == e2 && e3 == e3)) return false;
+    }
+    return fmaf(4.0e-6f, smax, pmax) < thr;
+}
+
+template <int PPL, bool COUNT>
 __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
 {
     __shared__ float4 s0[GS_WAVE];
@@ -65,73 +113,78 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
     const int r = blockIdx.y;
     const int lane = threadIdx.x;
     const uint2 range = a.ranges[(size_t)r * d.T + pm.tile];
+    const uint32_t rb = __builtin_amdgcn_readfirstlane(range.x);
+    const uint32_t re = __builtin_amdgcn_readfirstlane(range.y);
     const size_t rP = (size_t)r * d.P;
 
     float T[PPL], are[PPL], aim[PPL], pxf[PPL], pyf[PPL];
-    uint32_t contributor[PPL], last[PPL];
+    uint32_t last[PPL], cnt[PPL];
     bool done[PPL];
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
         T[k] = 1.0f; are[k] = 0.0f; aim[k] = 0.0f;
-        contributor[k] = 0; last[k] = 0;
+        last[k] = 0; cnt[k] = 0;
         done[k] = !pm.inside[k];
         pxf[k] = (float)pm.px[k];
         pyf[k] = (float)pm.py[k];
     }
-    for (uint32_t start = range.x; start < range.y; start += GS_WAVE) {
+    uint32_t tested = 0;
+    for (uint32_t start = rb; start < re; start += GS_WAVE) {
         bool live = false;
 #pragma unroll
         for (int k = 0; k < PPL; ++k) live |= !done[k];
         if (ballot(live) == 0) break;
         const uint32_t e = start + lane;
-        if (e < range.y) {
+        bool keep = false;
+        if (e < re) {
             const size_t gi = rP + a.plist[e];
-            s0[lane] = a.splat[2 * gi];
-            s1[lane] = a.splat[2 * gi + 1];
+            const float4 A = a.splat[2 * gi];
+            const float4 B = a.splat[2 * gi + 1];
+            s0[lane] = A;
+            s1[lane] = B;
             ss[lane] = a.sig[gi];
+            keep = !block_misses(A.x, A.y, A.z, A.w, B.x, B.z, pm.x0, pm.x1, pm.y0, pm.y1);
         }
+        uint64_t mask = ballot(keep);
+        if (COUNT) tested += min(re - start, (uint32_t)GS_WAVE);
         __syncthreads();
-        const int n = (int)min((uint32_t)GS_WAVE, range.y - start);
-        // one entry of look-ahead on the LDS reads
-        float4 A = s0[0], B = s1[0];
-        float2 S = ss[0];
-        for (int j = 0; j < n; ++j) {
-            const int jn = (j + 1 < n) ? j + 1 : j;
-            const float4 An = s0[jn], Bn = s1[jn];
-            const float2 Sn = ss[jn];
+        // list position (1-based) of entry j of this batch is base + j
+        const uint32_t base = start - rb + 1;
+        while (mask) {
+            const int j = __builtin_ctzll(mask);
+            mask &= mask - 1;
+            const float4 A = s0[j];
+            const float4 B = s1[j];
 #pragma unroll
             for (int k = 0; k < PPL; ++k) {
-                const bool act = !done[k];
-                contributor[k] += act ? 1u : 0u;
+                if (COUNT) cnt[k] += done[k] ? 0u : 1u;
                 const float dx = A.x - pxf[k], dy = A.y - pyf[k];
                 const float power = pair_power(A.z, A.w, B.x, dx, dy);
-                if (act && !(power > 0.0f) && !(power < B.z)) {
-                    const float alpha = fminf(0.99f, B.y * gs_expf_nonpos(power));
-                    const float test_T = T[k] * (1.0f - alpha);
-                    const bool pass = !(alpha < 1.0f / 255.0f);
-                    const bool term = pass && test_T < 0.0001f;
-                    const bool blend = pass && !term;
-                    const float w = alpha * T[k];
-                    are[k] = blend ? fmaf(S.x, w, are[k]) : are[k];
-                    aim[k] = blend ? fmaf(S.y, w, aim[k]) : aim[k];
-                    T[k] = blend ? test_T : T[k];
-                    last[k] = blend ? contributor[k] : last[k];
-                    done[k] = done[k] || term;
-                }
+                if (done[k] || power > 0.0f || power < B.z) continue;
+                const float alpha = fminf(0.99f, B.y * gs_expf_nonpos(power));
+                if (alpha < 1.0f / 255.0f) continue;
+                const float test_T = T[k] * (1.0f - alpha);
+                if (test_T < 0.0001f) { done[k] = true; continue; }
+                const float2 S = ss[j];
+                const float w = alpha * T[k];
+                are[k] = fmaf(S.x, w, are[k]);
+                aim[k] = fmaf(S.y, w, aim[k]);
+                T[k] = test_T;
+                last[k] = base + j;
             }
-            A = An;
-            B = Bn;
-            S = Sn;
         }
         __syncthreads();
     }
-    if (a.pairs) {
+    if (COUNT) {
         unsigned long long c = 0;
 #pragma unroll
-        for (int k = 0; k < PPL; ++k) c += contributor[k];
+        for (int k = 0; k < PPL; ++k) c += cnt[k];
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off);
-        if (lane == 0) atomicAdd(a.pairs, c);
+        if (lane == 0) {
+            atomicAdd(a.counts + 0, c);
+            atomicAdd(a.counts + 2, (unsigned long long)tested);
+        }
     }
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
@@ -166,7 +219,7 @@ __device__ __forceinline__ float wave_sum_to_lane63(float v)
     return v;
 }
 
-template <int MODE, int PPL>
+template <int MODE, int PPL, bool COUNT>
 __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
 {
     __shared__ float4 s0[GS_WAVE];
@@ -180,6 +233,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
     const int lane = threadIdx.x;
     float T[PPL], gre[PPL], gim[PPL], pxf[PPL], pyf[PPL];
     int last[PPL];
+    uint32_t cnt = 0;
     int maxlast = 0;
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
@@ -194,8 +248,10 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
     }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) maxlast = max(maxlast, __shfl_xor(maxlast, off));
+    maxlast = __builtin_amdgcn_readfirstlane(maxlast);
     if (maxlast == 0) return;
     const uint2 range = a.ranges[(size_t)r * d.T + pm.tile];
+    const uint32_t rb = __builtin_amdgcn_readfirstlane(range.x);
     const size_t rP = (size_t)r * d.P;
 
     float acc_re[PPL], acc_im[PPL], last_alpha[PPL], last_re[PPL], last_im[PPL];
@@ -203,27 +259,38 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
     for (int k = 0; k < PPL; ++k) {
         acc_re[k] = 0.0f; acc_im[k] = 0.0f; last_alpha[k] = 0.0f; last_re[k] = 0.0f; last_im[k] = 0.0f;
     }
+    uint32_t tested = 0;
     for (int top = maxlast - 1; top >= 0; top -= GS_WAVE) {
+        // entry j of this batch sits at list position top - j (walked back to front)
         const int rel = top - lane;
+        bool keep = false;
         if (rel >= 0) {
-            const uint32_t id = a.plist[range.x + rel];
+            const uint32_t id = a.plist[rb + rel];
             const size_t gi = rP + id;
+            const float4 A = a.splat[2 * gi];
+            const float4 B = a.splat[2 * gi + 1];
             sid[lane] = id;
-            s0[lane] = a.splat[2 * gi];
-            s1[lane] = a.splat[2 * gi + 1];
+            s0[lane] = A;
+            s1[lane] = B;
             ss[lane] = a.sig[gi];
+            keep = !block_misses(A.x, A.y, A.z, A.w, B.x, B.z, pm.x0, pm.x1, pm.y0, pm.y1);
         }
+        uint64_t mask = ballot(keep);
+        if (COUNT) tested += min(top + 1, GS_WAVE);
         __syncthreads();
-        const int n = min(GS_WAVE, top + 1);
-        float4 A = s0[0], B = s1[0];
-        float2 S = ss[0];
-        for (int j = 0; j < n; ++j, A = s0[j < n ? j : 0], B = s1[j < n ? j : 0], S = ss[j < n ? j : 0]) {
+        while (mask) {
+            const int j = __builtin_ctzll(mask);
+            mask &= mask - 1;
+            const float4 A = s0[j];
+            const float4 B = s1[j];
+            const float2 S = ss[j];
             const float qa = -2.0f * A.z, qb = -A.w, qc = -2.0f * B.x;
             float v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             bool anyc = false;
 #pragma unroll
             for (int k = 0; k < PPL; ++k) {
                 const bool active = (top - j) < last[k];
+                if (COUNT) cnt += active ? 1u : 0u;
                 const float dx = A.x - pxf[k], dy = A.y - pyf[k];
                 const float power = pair_power(A.z, A.w, B.x, dx, dy);
                 bool c = active && !(power > 0.0f) && !(power < B.z);
@@ -277,6 +344,15 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
         }
         __syncthreads();
     }
+    if (COUNT) {
+        unsigned long long c = cnt;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off);
+        if (lane == 0) {
+            atomicAdd(a.counts + 1, c);
+            atomicAdd(a.counts + 3, (unsigned long long)tested);
+        }
+    }
 }
 
 static inline dim3 render_grid(const Dims& d, int ppl)
@@ -288,34 +364,48 @@ static inline dim3 render_grid(const Dims& d, int ppl)
 int choose_ppl(const Dims& d, int requested)
 {
     if (requested == 1 || requested == 2 || requested == 4) return requested;
-    // enough tiles to keep every SIMD busy with one wave per tile
-    return ((long long)d.R * d.T >= 2048) ? 4 : 1;
+    // half a tile per wave once there are enough tiles to fill the chip with
+    // two waves per tile; quarter tiles (finest culling, most waves) below
+    return ((long long)d.R * d.T >= 2048) ? 2 : 1;
+}
+
+template <bool COUNT>
+static void launch_fwd_count(const Dims& d, const RenderFwdArgs& a, int ppl, hipStream_t s)
+{
+    const dim3 grid = render_grid(d, ppl);
+    if (ppl == 4) hipLaunchKernelGGL((k_render_fwd<4, COUNT>), grid, dim3(GS_WAVE), 0, s, d, a);
+    else if (ppl == 2) hipLaunchKernelGGL((k_render_fwd<2, COUNT>), grid, dim3(GS_WAVE), 0, s, d, a);
+    else hipLaunchKernelGGL((k_render_fwd<1, COUNT>), grid, dim3(GS_WAVE), 0, s, d, a);
 }
 
 hipError_t launch_render_fwd(const Dims& d, const RenderFwdArgs& a, int ppl, hipStream_t s)
 {
     if (d.R == 0 || d.T == 0) return hipSuccess;
-    const dim3 grid = render_grid(d, ppl);
-    if (ppl == 4) hipLaunchKernelGGL(k_render_fwd<4>, grid, dim3(GS_WAVE), 0, s, d, a);
-    else if (ppl == 2) hipLaunchKernelGGL(k_render_fwd<2>, grid, dim3(GS_WAVE), 0, s, d, a);
-    else hipLaunchKernelGGL(k_render_fwd<1>, grid, dim3(GS_WAVE), 0, s, d, a);
+    if (a.counts) launch_fwd_count<true>(d, a, ppl, s);
+    else launch_fwd_count<false>(d, a, ppl, s);
     return hipGetLastError();
 }
 
-template <int MODE>
+template <int MODE, bool COUNT>
 static void launch_bwd_mode(const Dims& d, const RenderBwdArgs& a, int ppl, hipStream_t s)
 {
     const dim3 grid = render_grid(d, ppl);
-    if (ppl == 4) hipLaunchKernelGGL((k_render_bwd<MODE, 4>), grid, dim3(GS_WAVE), 0, s, d, a);
-    else if (ppl == 2) hipLaunchKernelGGL((k_render_bwd<MODE, 2>), grid, dim3(GS_WAVE), 0, s, d, a);
-    else hipLaunchKernelGGL((k_render_bwd<MODE, 1>), grid, dim3(GS_WAVE), 0, s, d, a);
+    if (ppl == 4) hipLaunchKernelGGL((k_render_bwd<MODE, 4, COUNT>), grid, dim3(GS_WAVE), 0, s, d, a);
+    else if (ppl == 2) hipLaunchKernelGGL((k_render_bwd<MODE, 2, COUNT>), grid, dim3(GS_WAVE), 0, s, d, a);
+    else hipLaunchKernelGGL((k_render_bwd<MODE, 1, COUNT>), grid, dim3(GS_WAVE), 0, s, d, a);
 }
 
 hipError_t launch_render_bwd(const Dims& d, const RenderBwdArgs& a, int ppl, hipStream_t s)
 {
     if (d.R == 0 || d.T == 0) return hipSuccess;
-    if (a.mode == GRAD_REFERENCE) launch_bwd_mode<GRAD_REFERENCE>(d, a, ppl, s);
-    else launch_bwd_mode<GRAD_EXACT>(d, a, ppl, s);
+    const bool count = a.counts != nullptr;
+    if (a.mode == GRAD_REFERENCE) {
+        if (count) launch_bwd_mode<GRAD_REFERENCE, true>(d, a, ppl, s);
+        else launch_bwd_mode<GRAD_REFERENCE, false>(d, a, ppl, s);
+    } else {
+        if (count) launch_bwd_mode<GRAD_EXACT, true>(d, a, ppl, s);
+        else launch_bwd_mode<GRAD_EXACT, false>(d, a, ppl, s);
+    }
     return hipGetLastError();
 }
 

@@ -55,7 +55,7 @@ _SIGS = {
     "gsparc_profile_enable": (_i, [_i]),
     "gsparc_profile_collect": (_i, [C.POINTER(_i)]),
     "gsparc_profile_query": (_i, [_i, C.c_char_p, _i, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
-    "gsparc_profile_pairs": (_i, [C.POINTER(_u64)]),
+    "gsparc_profile_counts": (_i, [C.POINTER(_u64)]),
 }
 
 _lib = None
@@ -124,10 +124,11 @@ def profile_collect() -> dict:
     return out
 
 
-def profile_pairs() -> int:
-    v = C.c_uint64(0)
-    check(lib().gsparc_profile_pairs(C.byref(v)), "gsparc_profile_pairs")
-    return int(v.value)
+def profile_counts() -> dict:
+    """Blend-kernel counters accumulated since profile_enable(True)."""
+    v = (C.c_uint64 * 4)()
+    check(lib().gsparc_profile_counts(v), "gsparc_profile_counts")
+    return {"fwd_pairs": int(v[0]), "bwd_pairs": int(v[1]), "fwd_entries": int(v[2]), "bwd_entries": int(v[3])}
 
 
 def layout(P, R, W, H, K=0) -> Layout:

@@ -56,9 +56,10 @@ const char* gsparc_last_error(void);
 
 /* Tuning knobs (process-wide; no counterpart in the reference):
  *   "render_fwd_ppl", "render_bwd_ppl"  pixels per lane of the forward and
- *       backward blend kernels: 0 = automatic, 1 = one 8x8 quarter-tile per
- *       wave, 2 = half a tile per wave, 4 = one 16x16 tile per wave.  Speed only: the forward result is
- *       bit-identical either way, the backward differs by summation order. */
+ *       backward blend kernels: 0 = automatic (1 for few tiles, else 2),
+ *       1 = one 8x8 quarter-tile per wave, 2 = half a tile per wave, 4 = one
+ *       16x16 tile per wave.  Speed only: the forward result is bit-identical
+ *       either way, the backward differs by summation order. */
 int gsparc_set_option(const char* name, int value);
 int gsparc_get_option(const char* name, int* value);
 
@@ -135,13 +136,16 @@ int gsparc_ssim_backward(int planes, int H, int W, float C1, float C2, const flo
 /* Per-kernel timing.  While enabled, every kernel launched by the calls above
  * is bracketed by HIP events on its own stream; collect() waits for them,
  * adds the elapsed times to per-kernel totals and returns how many kernels
- * have totals; query() reads entry i (sorted by kernel name).  The forward blend also counts the (pixel, Gaussian) pairs it
- * visits (profile_pairs).  No counterpart in the reference (which only times
+ * have totals; query() reads entry i (sorted by kernel name).  The blend
+ * kernels also count, since enable: counts[0] forward (pixel, Gaussian) pairs
+ * evaluated, counts[1] backward pairs evaluated, counts[2] / counts[3] list
+ * entries tested against a wave's pixel block in the forward / backward
+ * (profile_counts).  No counterpart in the reference (which only times
  * render() on the host, train.py:381-384). */
 int gsparc_profile_enable(int on);
 int gsparc_profile_collect(int* n_kernels);
 int gsparc_profile_query(int i, char* name, int name_len, double* total_ms, long long* launches);
-int gsparc_profile_pairs(uint64_t* pairs);
+int gsparc_profile_counts(uint64_t counts[4]);
 
 #ifdef __cplusplus
 }
