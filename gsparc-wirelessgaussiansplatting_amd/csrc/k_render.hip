@@ -17,6 +17,7 @@
 // Per pair the exponential is skipped when power < thr: then
 // opacity * exp(power) < 1/255 is guaranteed (thr carries a 1e-3 margin), so
 // the decision is the same as evaluating it.
+#include "gs_cull.h"
 #include "gs_device.h"
 #include "gs_kernels.h"
 
@@ -58,43 +59,6 @@ __device__ __forceinline__ Pixels<PPL> map_pixels(const Dims& d)
     m.y0 = (float)(ty * GS_BLOCK + qy0 * 8);
     m.y1 = (float)(ty * GS_BLOCK + qy1 * 8 + 7);
     return m;
-}
-
-// Wave-level culling of one list entry.  Returns true only when every pixel
-// centre of the block [x0,x1] x [y0,y1] is certain to evaluate power < thr in
-// the per-pixel loop, so that skipping the entry for the whole block changes
-// no result bit.
-//
-// power(d) = ha dx^2 + nb dx dy + hc dy^2 (d = mean - pixel) is concave for a
-// valid conic (ha, hc < 0), so its maximum over the rectangle of offsets is 0
-// when the rectangle holds the origin and otherwise lies on an edge, where it
-// is a 1-D concave quadratic maximised in closed form and clamped.  Rounding:
-// the per-pixel float evaluation (with the rounded offsets) exceeds the exact
-// value by at most ~5u * S, and the float maximum here undershoots the exact
-// one by at most ~4u * S, where u = 2^-24 and S bounds |ha| dx^2 + |nb||dx dy|
-// + |hc| dy^2 over the block; the test keeps a margin of 4e-6 * S (~67u * S).
-// Anything non-finite or non-concave is never culled.
-__device__ __forceinline__ bool block_misses(float mx, float my, float ha, float nb, float hc, float thr,
-                                             float x0, float x1, float y0, float y1)
-{
-    if (!(ha < 0.0f && hc < 0.0f)) return false;
-    const float dxl = mx - x1, dxh = mx - x0, dyl = my - y1, dyh = my - y0;
-    const float DX = fmaxf(fabsf(dxl), fabsf(dxh)), DY = fmaxf(fabsf(dyl), fabsf(dyh));
-    const float smax = fmaf(-ha * DX, DX, fmaf(fabsf(nb) * DX, DY, -hc * DY * DY));
-    float pmax = 0.0f;
-    if (!(dxl <= 0.0f && dxh >= 0.0f && dyl <= 0.0f && dyh >= 0.0f)) {
-        const float ky = -0.5f * nb * __builtin_amdgcn_rcpf(hc);   // argmax y on an edge x = X is ky X
-        const float kx = -0.5f * nb * __builtin_amdgcn_rcpf(ha);
-        const float ya = fminf(fmaxf(ky * dxl, dyl), dyh), yb = fminf(fmaxf(ky * dxh, dyl), dyh);
-        const float xa = fminf(fmaxf(kx * dyl, dxl), dxh), xb = fminf(fmaxf(kx * dyh, dxl), dxh);
-        const float e0 = fmaf(ha * dxl, dxl, fmaf(nb * dxl, ya, hc * ya * ya));
-        const float e1 = fmaf(ha * dxh, dxh, fmaf(nb * dxh, yb, hc * yb * yb));
-        const float e2 = fmaf(ha * xa, xa, fmaf(nb * xa, dyl, hc * dyl * dyl));
-        const float e3 = fmaf(ha * xb, xb, fmaf(nb * xb, dyh, hc * dyh * dyh));
-        pmax = fmaxf(fmaxf(e0, e1), fmaxf(e2, e3));
-        if (!(e0 == e0 && e1 == e1 && e2 == e2 && e3 == e3)) return false;
-    }
-    return fmaf(4.0e-6f, smax, pmax) < thr;
 }
 
 template <int PPL, bool COUNT>

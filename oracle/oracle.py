@@ -50,6 +50,14 @@ def math_vec(which, x, y=None):
     return out
 
 
+def cull_check(n, seed=0, margin=4.0e-6):
+    """Brute-force check of the blend kernels' wave-level culling test:
+    [cases, culled, violations, cases where every pixel skips]."""
+    stats = np.zeros(4, np.int64)
+    lib().orc_cull_check(C.c_int64(n), C.c_uint64(seed), C.c_float(margin), _p(stats))
+    return stats
+
+
 def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
