@@ -176,8 +176,9 @@ def main():
         "render_bwd": ("valu", cnt["bwd_pairs"] * BWD_FLOP_PER_PAIR + cnt["bwd_entries"] * CULL_FLOP, "TFLOP/s"),
         "preprocess_fwd": ("hbm", P * 820 + 44 * P * R, "GB/s"),
         "preprocess_bwd": ("hbm", P * (820 + 32 * 2 + 776) + 32 * P * R, "GB/s"),
-        "bin_sort_small": ("hbm", 12.0 * K, "GB/s"),
-        "bin_scatter": ("hbm", 8.0 * K + 20.0 * P * R, "GB/s"),
+        "bin_depth_sort": ("hbm", 84.0 * P * R, "GB/s"),
+        "bin_chunk_count": ("hbm", 16.0 * P * R, "GB/s"),
+        "bin_scatter": ("hbm", 4.0 * K + 20.0 * P * R, "GB/s"),
         "ssim_fwd": ("hbm", 8 * N * R + 16 * N * R, "GB/s"),
         "ssim_bwd": ("hbm", 28 * N * R, "GB/s"),
     }

@@ -13,9 +13,15 @@ struct Dims {
     int NB;          // blocks of 256 Gaussians
 };
 
-// Tile counts are aggregated in LDS when the map has at most this many tiles
-// (90x360 has 138); larger maps use global atomics directly.
-#define GS_LDS_TILES 1024
+// depth sort: 256-thread blocks of 8 elements per thread
+#define GS_RS_THREADS 256
+#define GS_RS_ITEMS 8
+#define GS_RS_TILE (GS_RS_THREADS * GS_RS_ITEMS)
+// binning chunks: at least this many sorted Gaussians per chunk; per-tile
+// counters of a chunk live in LDS up to this many tiles (90x360 has 138,
+// 1080x1920 has 8160), in the global count matrix beyond
+#define GS_CHUNK_MIN 1024
+#define GS_CHUNK_LDS_TILES 8192
 
 inline Dims make_dims(int P, int R, int W, int H)
 {
@@ -41,7 +47,6 @@ struct PreFwdArgs {
     float* depth;            // [R*P]
     int* radii;              // [R*P]
     float* cov3;             // [P*6]
-    uint32_t* segcnt;        // [R*T] instance counts per tile (zeroed before)
 };
 hipError_t launch_preprocess_fwd(const Dims& d, const PreFwdArgs& a, hipStream_t s);
 
@@ -49,25 +54,28 @@ struct BinArgs {
     const float4* splat;     // for px, py
     const int* radii;
     const float* depth;
-    uint32_t* segcnt;        // [R*T]
-    uint32_t* cursor;        // [R*T] fill cursors of the scatter (zeroed before)
-    uint32_t* segbase;       // [R*T]
+    uint32_t* segcnt;        // [R*T] instances per tile
+    uint32_t* segbase;       // [R*T] exclusive prefix of segcnt
     uint2* ranges;           // [R*T]
-    uint32_t* total;         // [3]: K, max segment length, number of long segments
-    uint32_t* long_list;     // [R*T] segments longer than the one-wave sort handles
-    uint32_t* total_host;    // optional pinned mirror of total
-    uint32_t* keys;          // [Kcap] depth bits, tile-major
-    uint32_t* vals;          // [Kcap]
-    uint32_t* keys_tmp;      // [Kcap] scratch for long segments
-    uint32_t* vals_tmp;
-    uint32_t* plist;         // [Kcap] sorted Gaussian indices
-    uint32_t* keys_sorted;   // optional [Kcap] sorted depth bits
+    uint32_t* total;         // [1]: K
+    uint32_t* nvis;          // [R] visible Gaussians per receiver
+    uint32_t* skey[2];       // [R*P] depth-sort ping-pong: depth bits
+    uint32_t* sval[2];       // [R*P]                       Gaussian index
+    uint32_t* shist;         // [R*257*rs_blocks(P)] digit histograms
+    uint32_t* ccnt;          // [R*T*nchunks] per-chunk tile counts / bases
+    int nchunks, chunk_len;
+    uint32_t* plist;         // [Kcap] Gaussian indices in tile-list order
+    uint32_t* keys_sorted;   // optional [Kcap] their depth bits
     uint64_t Kcap;
 };
+int rs_blocks(int P);
+int chunk_len_for(const Dims& d);
+int chunk_count_for(const Dims& d);
+hipError_t launch_depth_sort(const Dims& d, const BinArgs& a, hipStream_t s);
+hipError_t launch_chunk_count(const Dims& d, const BinArgs& a, hipStream_t s);
+hipError_t launch_chunk_scan(const Dims& d, const BinArgs& a, hipStream_t s);
 hipError_t launch_segscan(const Dims& d, const BinArgs& a, hipStream_t s);
-hipError_t launch_scatter(const Dims& d, const BinArgs& a, hipStream_t s);
-hipError_t launch_sort_small(const Dims& d, const BinArgs& a, hipStream_t s);
-hipError_t launch_sort_long(const Dims& d, const BinArgs& a, hipStream_t s);
+hipError_t launch_chunk_scatter(const Dims& d, const BinArgs& a, hipStream_t s);
 
 struct RenderFwdArgs {
     const uint2* ranges;     // [R*T]

@@ -47,26 +47,50 @@ void make_layout(int P, int R, int W, int H, uint64_t K, gsparc_layout& L)
     L.sig = o; o = al(o + RP * sizeof(float2));
     L.depth = o; o = al(o + RP * sizeof(float));
     L.cov3 = o; o = al(o + (size_t)P * 6 * sizeof(float));
+    L.sort_key0 = o; o = al(o + RP * sizeof(uint32_t));
+    L.sort_val0 = o; o = al(o + RP * sizeof(uint32_t));
+    L.sort_key1 = o; o = al(o + RP * sizeof(uint32_t));
+    L.sort_val1 = o; o = al(o + RP * sizeof(uint32_t));
+    L.sort_hist = o; o = al(o + (size_t)R * 257 * gs::rs_blocks(P) * sizeof(uint32_t));
+    L.nvis = o; o = al(o + (size_t)R * sizeof(uint32_t));
     L.geom_bytes = o + ALIGN;
     o = 0;
     L.final_t = o; o = al(o + (size_t)R * N * sizeof(float));
     L.n_contrib = o; o = al(o + (size_t)R * N * sizeof(uint32_t));
     L.ranges = o; o = al(o + RT * sizeof(uint2));
-    L.segcnt = o; o = o + RT * sizeof(uint32_t);   // segcnt and cursor are adjacent:
-    L.cursor = o; o = al(o + RT * sizeof(uint32_t)); // one fill clears both
+    L.segcnt = o; o = al(o + RT * sizeof(uint32_t));
     L.segbase = o; o = al(o + RT * sizeof(uint32_t));
-    L.long_list = o; o = al(o + RT * sizeof(uint32_t));
+    L.chunk_cnt = o; o = al(o + RT * gs::chunk_count_for(d) * sizeof(uint32_t));
     L.total = o; o = al(o + 4 * sizeof(uint32_t));
     L.image_bytes = o + ALIGN;
     o = 0;
     const size_t Kn = (size_t)(K > 0 ? K : 1);
-    L.keys = o; o = al(o + Kn * 4);
-    L.vals = o; o = al(o + Kn * 4);
-    L.keys_tmp = o; o = al(o + Kn * 4);
-    L.vals_tmp = o; o = al(o + Kn * 4);
     L.plist = o; o = al(o + Kn * 4);
     L.keys_sorted = o; o = al(o + Kn * 4);
     L.bin_bytes = o + ALIGN;
+}
+
+// binning state shared by the two forward calls
+static gs::BinArgs bin_args(const gs::Dims& d, const gsparc_layout& L, char* gb, char* ib, const int* radii)
+{
+    gs::BinArgs ba{};
+    ba.splat = (const float4*)(gb + L.splat);
+    ba.radii = radii;
+    ba.depth = (const float*)(gb + L.depth);
+    ba.segcnt = (uint32_t*)(ib + L.segcnt);
+    ba.segbase = (uint32_t*)(ib + L.segbase);
+    ba.ranges = (uint2*)(ib + L.ranges);
+    ba.total = (uint32_t*)(ib + L.total);
+    ba.nvis = (uint32_t*)(gb + L.nvis);
+    ba.skey[0] = (uint32_t*)(gb + L.sort_key0);
+    ba.sval[0] = (uint32_t*)(gb + L.sort_val0);
+    ba.skey[1] = (uint32_t*)(gb + L.sort_key1);
+    ba.sval[1] = (uint32_t*)(gb + L.sort_val1);
+    ba.shist = (uint32_t*)(gb + L.sort_hist);
+    ba.ccnt = (uint32_t*)(ib + L.chunk_cnt);
+    ba.chunk_len = gs::chunk_len_for(d);
+    ba.nchunks = gs::chunk_count_for(d);
+    return ba;
 }
 
 // Buffers from torch are 256-byte aligned already; be safe for others.
@@ -77,6 +101,8 @@ int check_dims(int P, int R, int W, int H)
     if (P < 0 || R < 0 || W <= 0 || H <= 0) return fail(GSPARC_ERR_ARG, "negative or zero size");
     if ((int64_t)R * W * H > (int64_t)1 << 31) return fail(GSPARC_ERR_ARG, "R*H*W too large");
     if ((int64_t)P * R >= (int64_t)1 << 31) return fail(GSPARC_ERR_ARG, "P*R too large");
+    // the binning keeps one 64-bit mask per tile column and row in LDS
+    if ((W + 15) / 16 + (H + 15) / 16 > 8192) return fail(GSPARC_ERR_ARG, "W + H too large (at most 131072 pixels)");
     return GSPARC_OK;
 }
 
@@ -228,22 +254,22 @@ int gsparc_rasterize_forward_geom(int P, int R, int W, int H, const float* means
     pa.means = means; pa.scales = scales; pa.rots = rotations; pa.opac = opacities; pa.mlp = emission_mlps;
     pa.views = viewmatrices; pa.tx = tx_pos; pa.rx = rx_pos; pa.mod = scale_modifier;
     pa.splat = (float4*)(gb + L.splat); pa.sig = (float2*)(gb + L.sig); pa.depth = (float*)(gb + L.depth);
-    pa.radii = radii_out; pa.cov3 = (float*)(gb + L.cov3); pa.segcnt = (uint32_t*)(ib + L.segcnt);
-    if (R > 0) GS_LAUNCH("clear_counts", gs::launch_fill_u32(pa.segcnt, 0u, (size_t)2 * R * d.T, s), s);
+    pa.radii = radii_out; pa.cov3 = (float*)(gb + L.cov3);
     if (P > 0 && R > 0) GS_LAUNCH("preprocess_fwd", gs::launch_preprocess_fwd(d, pa, s), s);
-    gs::BinArgs ba{};
-    ba.segcnt = (uint32_t*)(ib + L.segcnt);
-    ba.cursor = (uint32_t*)(ib + L.cursor);
-    ba.segbase = (uint32_t*)(ib + L.segbase);
-    ba.ranges = (uint2*)(ib + L.ranges);
-    ba.total = (uint32_t*)(ib + L.total);
-    ba.long_list = (uint32_t*)(ib + L.long_list);
+    gs::BinArgs ba = bin_args(d, L, gb, ib, radii_out);
     if (R > 0) {
+        if (P > 0) {
+            GS_LAUNCH("bin_depth_sort", gs::launch_depth_sort(d, ba, s), s);
+            GS_LAUNCH("bin_chunk_count", gs::launch_chunk_count(d, ba, s), s);
+            GS_LAUNCH("bin_chunk_scan", gs::launch_chunk_scan(d, ba, s), s);
+        } else {
+            GS_LAUNCH("clear_counts", gs::launch_fill_u32(ba.segcnt, 0u, (size_t)R * d.T, s), s);
+        }
         GS_LAUNCH("bin_segscan", gs::launch_segscan(d, ba, s), s);
-        uint32_t tot[2] = {0, 0};
-        GS_HIP(hipMemcpyAsync(tot, ba.total, sizeof(tot), hipMemcpyDeviceToHost, s));
+        uint32_t tot = 0;
+        GS_HIP(hipMemcpyAsync(&tot, ba.total, sizeof(tot), hipMemcpyDeviceToHost, s));
         GS_HIP(hipStreamSynchronize(s));
-        *K_out = tot[0];
+        *K_out = tot;
     } else {
         *K_out = 0;
     }
@@ -265,28 +291,11 @@ int gsparc_rasterize_forward_render(int P, int R, int W, int H, const float* emi
     char* gb = base_of(geom);
     char* ib = base_of(image);
     char* bb = base_of(bin);
-    gs::BinArgs ba{};
-    ba.splat = (const float4*)(gb + L.splat);
-    ba.radii = radii;
-    ba.depth = (const float*)(gb + L.depth);
-    ba.segcnt = (uint32_t*)(ib + L.segcnt);
-    ba.cursor = (uint32_t*)(ib + L.cursor);
-    ba.segbase = (uint32_t*)(ib + L.segbase);
-    ba.ranges = (uint2*)(ib + L.ranges);
-    ba.total = (uint32_t*)(ib + L.total);
-    ba.long_list = (uint32_t*)(ib + L.long_list);
-    ba.keys = (uint32_t*)(bb + L.keys);
-    ba.vals = (uint32_t*)(bb + L.vals);
-    ba.keys_tmp = (uint32_t*)(bb + L.keys_tmp);
-    ba.vals_tmp = (uint32_t*)(bb + L.vals_tmp);
+    gs::BinArgs ba = bin_args(d, L, gb, ib, radii);
     ba.plist = (uint32_t*)(bb + L.plist);
     ba.keys_sorted = (flags & GSPARC_FLAG_KEEP_KEYS) ? (uint32_t*)(bb + L.keys_sorted) : nullptr;
     ba.Kcap = K_capacity;
-    if (P > 0) {
-        GS_LAUNCH("bin_scatter", gs::launch_scatter(d, ba, s), s);
-        GS_LAUNCH("bin_sort_small", gs::launch_sort_small(d, ba, s), s);
-        GS_LAUNCH("bin_sort_long", gs::launch_sort_long(d, ba, s), s);
-    }
+    if (P > 0) GS_LAUNCH("bin_scatter", gs::launch_chunk_scatter(d, ba, s), s);
     gs::RenderFwdArgs ra;
     ra.ranges = ba.ranges;
     ra.plist = ba.plist;

@@ -1,30 +1,32 @@
-// Binning for R receivers: per-tile instance counts -> tile ranges, then the
-// (tile, depth) ordering of every tile list.  Together these replace
-// duplicateWithKeys, the 40-bit radix sort and identifyTileRanges
-// (rasterizer_impl.cu:102-171, 270-312) and produce the same ordering:
-// within a tile, Gaussians by ascending float depth bits, ties by index.
+// Binning for R receivers: the per-tile lists of Gaussians in (depth, index)
+// order.  Together these replace duplicateWithKeys, the 40-bit radix sort of
+// (tile, depth) keys and identifyTileRanges (rasterizer_impl.cu:102-171,
+// 270-312) and produce the same order: within a tile, Gaussians by ascending
+// float depth bits, ties by index (the reference's stable sort sees its
+// instances in index order).
 //
-//   k_segscan   exclusive prefix over (receiver, tile) of the per-tile counts
-//               (accumulated by the preprocess) -> ranges, K
-//   k_scatter   writes (depth bits, index) of every instance into its tile
-//               segment: per-block LDS counts, one global atomic per touched
-//               tile to reserve space, LDS atomics for the slots
-//   k_sort_small one wave per tile segment of up to 2048 instances: LSD radix
-//               sort on the depth bits (4-bit digits, ballot ranking) with the
-//               segment in registers, no workgroup barriers
-//   k_sort_long  one workgroup per longer segment (persistent over the list of
-//               long segments): the same sort in LDS when the segment fits,
-//               chunk by chunk through global memory otherwise
-//   Both then put runs of equal depth in index order (the order the
-//   reference's stable sort gives, since its input is in index order).
+// Sort first, then bin, so that no per-tile sort is needed:
+//   k_rs_hist / k_rs_scan / k_rs_scatter
+//               stable LSD radix sort of each receiver's visible Gaussians by
+//               the 32 depth bits (4 passes of 8-bit digits; input in index
+//               order, so equal depths stay in index order).  The first pass
+//               reads depth and radii directly and drops culled Gaussians.
+//   k_chunk_count  the sorted sequence of a receiver is cut into chunks; per
+//               chunk and tile, the number of instances
+//   k_chunk_scan   per (receiver, tile): exclusive prefix over the chunks,
+//               total -> segcnt
+//   k_segscan   exclusive prefix over (receiver, tile) -> tile ranges, K
+//   k_chunk_scatter  one wave per chunk walks it in sorted order, 64
+//               Gaussians at a time, and writes each instance at
+//               tile base + (instances of the tile before it); the rank
+//               among the 64 comes from per-column and per-row ballot masks.
+//               The tile lists come out in depth order without a sort.
 #include "gs_device.h"
 #include "gs_kernels.h"
 
-namespace gs {
+#include <climits>
 
-// segments up to this length are sorted by one wave (k_sort_small), longer
-// ones by a workgroup (k_sort_long)
-#define GS_SORT_SMALL 2048
+namespace gs {
 
 // ---------------------------------------------------------------- scans
 // Block-wide exclusive scan of one value per thread (1024 threads max).
@@ -53,477 +55,398 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
 }
 
 __global__ void __launch_bounds__(1024) k_segscan(int n, const uint32_t* segcnt, uint32_t* segbase, uint2* ranges,
-                                                   uint32_t* total, uint32_t* long_list)
+                                                   uint32_t* total)
 {
     __shared__ uint32_t s_wave[1024 / GS_WAVE];
     const int per = (n + 1023) / 1024;
     const int i0 = min(n, (int)threadIdx.x * per), i1 = min(n, i0 + per);
-    uint32_t sum = 0, mx = 0, nlong = 0;
-    for (int i = i0; i < i1; ++i) {
-        sum += segcnt[i];
-        mx = max(mx, segcnt[i]);
-        nlong += segcnt[i] > GS_SORT_SMALL ? 1u : 0u;
-    }
-    uint32_t tot, totlong;
+    uint32_t sum = 0;
+    for (int i = i0; i < i1; ++i) sum += segcnt[i];
+    uint32_t tot;
     uint32_t run = block_exclusive_scan<1024>(sum, s_wave, tot);
-    uint32_t lpos = block_exclusive_scan<1024>(nlong, s_wave, totlong);
     for (int i = i0; i < i1; ++i) {
         const uint32_t c = segcnt[i];
         segbase[i] = run;
         // empty tiles keep the range (0, 0) as after the reference's memset
         // (rasterizer_impl.cu:303)
         ranges[i] = c ? make_uint2(run, run + c) : make_uint2(0u, 0u);
-        if (c > GS_SORT_SMALL) long_list[lpos++] = (uint32_t)i;
         run += c;
     }
-    if (threadIdx.x == 0) total[2] = totlong;
-    // max segment length
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, off));
-    if ((threadIdx.x % GS_WAVE) == 0) s_wave[threadIdx.x / GS_WAVE] = mx;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t m = 0;
-        for (int w = 0; w < 1024 / GS_WAVE; ++w) m = max(m, s_wave[w]);
-        total[0] = tot;
-        total[1] = m;
-    }
+    if (threadIdx.x == 0) total[0] = tot;
 }
 
-// ---------------------------------------------------------------- scatter
-__global__ void __launch_bounds__(GS_PRE_THREADS) k_scatter(Dims d, BinArgs a)
-{
-    __shared__ uint32_t s_cnt[GS_LDS_TILES];
-    __shared__ uint32_t s_base[GS_LDS_TILES];
-    if ((uint64_t)a.total[0] > a.Kcap) return;    // capacity overflow: reported by the host
-    const bool lds_tiles = d.T <= GS_LDS_TILES;
-    const int b = blockIdx.x, r = blockIdx.y;
-    const int g = b * GS_PRE_THREADS + threadIdx.x;
-    const size_t ri = (size_t)r * d.P + g;
-    const size_t rT = (size_t)r * d.T;
-    int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
-    uint32_t dbits = 0;
-    if (g < d.P) {
-        const int radius = a.radii[ri];
-        if (radius > 0) {
-            const float4 s0 = a.splat[2 * ri];
-            get_rect(s0.x, s0.y, radius, d.gx, d.gy, x0, y0, x1, y1);
-            dbits = gs_as_uint(a.depth[ri]);
-        }
-    }
-    if (!lds_tiles) {
-        for (int ty = y0; ty < y1; ++ty)
-            for (int tx = x0; tx < x1; ++tx) {
-                const int t = ty * d.gx + tx;
-                const uint32_t pos = a.segbase[rT + t] + atomicAdd(&a.cursor[rT + t], 1u);
-                a.keys[pos] = dbits;
-                a.vals[pos] = (uint32_t)g;
-            }
-        return;
-    }
-    for (int i = threadIdx.x; i < d.T; i += GS_PRE_THREADS) s_cnt[i] = 0;
-    __syncthreads();
-    for (int ty = y0; ty < y1; ++ty)
-        for (int tx = x0; tx < x1; ++tx) atomicAdd(&s_cnt[ty * d.gx + tx], 1u);
-    __syncthreads();
-    for (int t = threadIdx.x; t < d.T; t += GS_PRE_THREADS) {
-        const uint32_t c = s_cnt[t];
-        if (c) s_base[t] = a.segbase[rT + t] + atomicAdd(&a.cursor[rT + t], c);
-        s_cnt[t] = 0;
-    }
-    __syncthreads();
-    for (int ty = y0; ty < y1; ++ty)
-        for (int tx = x0; tx < x1; ++tx) {
-            const int t = ty * d.gx + tx;
-            const uint32_t pos = s_base[t] + atomicAdd(&s_cnt[t], 1u);
-            a.keys[pos] = dbits;
-            a.vals[pos] = (uint32_t)g;
-        }
-}
-
-// ---------------------------------------------------------------- sort
-#define GS_SORT_THREADS 512
-#define GS_SORT_WAVES (GS_SORT_THREADS / GS_WAVE)
-#define GS_SORT_ROWS 16
-#define GS_SORT_CHUNK (GS_SORT_THREADS * GS_SORT_ROWS)   // 8192 instances
-#define GS_SORT_TABLE (16 * (GS_SORT_ROWS * GS_SORT_WAVES + 1))
-
-struct SortShared {
-    uint32_t k[2][GS_SORT_CHUNK];
-    uint32_t v[2][GS_SORT_CHUNK];
-    uint32_t table[GS_SORT_TABLE];
-    uint32_t hist[16];
-    uint32_t base[16];
-    uint32_t wscan[GS_SORT_WAVES];
-    uint32_t red[2][GS_SORT_WAVES];
-    uint32_t flag;
-};
-
-// Ballot-based stable ranking of 4-bit digits for one row of 512 elements.
-__device__ __forceinline__ uint64_t digit_match(bool valid, uint32_t dgt)
+// ---------------------------------------------------------------- depth sort
+// Lanes of the wave holding the same NBITS-bit digit (among valid lanes).
+template <int NBITS>
+__device__ __forceinline__ uint64_t match_digit(uint32_t dg, bool valid)
 {
     uint64_t m = ballot(valid);
 #pragma unroll
-    for (int bit = 0; bit < 4; ++bit) {
-        const bool set = (dgt >> bit) & 1u;
-        const uint64_t B = ballot(valid && set);
-        m &= set ? B : ~B;
+    for (int b = 0; b < NBITS; ++b) {
+        const bool bit = (dg >> b) & 1u;
+        const uint64_t v = ballot(bit);
+        m &= bit ? v : ~v;
     }
     return m;
 }
 
-// One stable LSD pass over n elements (any n), chunk by chunk.
-template <typename KP, typename VP, typename DKP, typename DVP>
-__device__ void radix_pass(SortShared& sm, int n, KP sk, VP sv, DKP dk, DVP dv, uint32_t kmin, int shift)
+// Digit of element i of receiver r in pass p.  The first pass reads the
+// depth and radius directly: culled Gaussians get the extra digit 256 and
+// sort behind every visible one (their positions are never read).
+template <bool FIRST>
+__device__ __forceinline__ void rs_load(const Dims& d, const BinArgs& a, size_t rP, int i, int pass, uint32_t& key,
+                                        uint32_t& val, uint32_t& dg)
 {
-    const int tid = threadIdx.x, wave = tid / GS_WAVE;
-    const uint64_t lt = lanemask_lt();
-    const int nchunks = (n + GS_SORT_CHUNK - 1) / GS_SORT_CHUNK;
-    if (nchunks > 1) {
-        // digit histogram of the whole segment
-        if (tid < 16) sm.hist[tid] = 0;
-        __syncthreads();
-        uint32_t pk[4] = {0, 0, 0, 0};   // 16 packed 8-bit counters... flushed every 255
-        int cnt = 0;
-        for (int e = tid; e < n; e += GS_SORT_THREADS) {
-            const uint32_t dg = ((sk[e] - kmin) >> shift) & 15u;
-            pk[dg >> 2] += 1u << (8 * (dg & 3));
-            if (++cnt == 255) {
-                for (int q = 0; q < 16; ++q) {
-                    const uint32_t c = (pk[q >> 2] >> (8 * (q & 3))) & 255u;
-                    if (c) atomicAdd(&sm.hist[q], c);
-                }
-                pk[0] = pk[1] = pk[2] = pk[3] = 0;
-                cnt = 0;
-            }
-        }
-        for (int q = 0; q < 16; ++q) {
-            const uint32_t c = (pk[q >> 2] >> (8 * (q & 3))) & 255u;
-            if (c) atomicAdd(&sm.hist[q], c);
-        }
-        __syncthreads();
-        if (tid == 0) {
-            uint32_t run = 0;
-            for (int q = 0; q < 16; ++q) { sm.base[q] = run; run += sm.hist[q]; }
-        }
-        __syncthreads();
+    if (FIRST) {
+        const bool vis = a.radii[rP + i] > 0;
+        key = vis ? gs_as_uint(a.depth[rP + i]) : 0xffffffffu;
+        val = (uint32_t)i;
+        dg = vis ? (key & 0xffu) : 256u;
+    } else {
+        key = a.skey[(pass + 1) & 1][rP + i];
+        val = a.sval[(pass + 1) & 1][rP + i];
+        dg = (key >> (8 * pass)) & 0xffu;
     }
-    for (int ch = 0; ch < nchunks; ++ch) {
-        const int e0 = ch * GS_SORT_CHUNK;
-        const int cn = min(GS_SORT_CHUNK, n - e0);
-        const int rows = (cn + GS_SORT_THREADS - 1) / GS_SORT_THREADS;
-        const int DS = rows * GS_SORT_WAVES + 1;   // digit stride, padded against bank conflicts
-        const int S = 16 * DS;
-        for (int i = tid; i < S; i += GS_SORT_THREADS) sm.table[i] = 0;
+}
+
+template <bool FIRST>
+__global__ void __launch_bounds__(GS_RS_THREADS) k_rs_hist(Dims d, BinArgs a, int pass)
+{
+    constexpr int NB = FIRST ? 257 : 256;
+    __shared__ uint32_t h[NB];
+    const int r = blockIdx.y, blk = blockIdx.x, nblk = gridDim.x;
+    const int n = FIRST ? d.P : (int)a.nvis[r];
+    const size_t rP = (size_t)r * d.P;
+    for (int i = threadIdx.x; i < NB; i += GS_RS_THREADS) h[i] = 0;
+    __syncthreads();
+    if (blk * GS_RS_TILE < n) {
+        for (int k = 0; k < GS_RS_ITEMS; ++k) {
+            const int i = blk * GS_RS_TILE + k * GS_RS_THREADS + threadIdx.x;
+            const bool valid = i < n;
+            uint32_t key = 0, val = 0, dg = 0;
+            if (valid) rs_load<FIRST>(d, a, rP, i, pass, key, val, dg);
+            const uint64_t m = match_digit<FIRST ? 9 : 8>(dg, valid);
+            // one LDS atomic per distinct digit of the wave
+            if (valid && (m & lanemask_lt()) == 0) atomicAdd(&h[dg], (uint32_t)__popcll(m));
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < NB; i += GS_RS_THREADS) a.shist[((size_t)r * 257 + i) * nblk + blk] = h[i];
+}
+
+// Per receiver: exclusive scan over (digit, block) of the block histograms.
+__global__ void __launch_bounds__(1024) k_rs_scan(Dims d, BinArgs a, int nblk, int first)
+{
+    __shared__ uint32_t s_wave[1024 / GS_WAVE];
+    const int r = blockIdx.x;
+    const int n = (first ? 257 : 256) * nblk;
+    uint32_t* h = a.shist + (size_t)r * 257 * nblk;
+    uint32_t carry = 0;
+    for (int base = 0; base < n; base += 1024 * 4) {
+        uint32_t v[4], s = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int idx = base + threadIdx.x * 4 + j;
+            v[j] = idx < n ? h[idx] : 0u;
+            s += v[j];
+        }
+        uint32_t tot;
+        uint32_t run = carry + block_exclusive_scan<1024>(s, s_wave, tot);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int idx = base + threadIdx.x * 4 + j;
+            if (idx < n) h[idx] = run;
+            run += v[j];
+        }
+        carry += tot;
+    }
+    __syncthreads();
+    // visible Gaussians = everything before the culled digit
+    if (first && threadIdx.x == 0) a.nvis[r] = h[(size_t)256 * nblk];
+}
+
+// Stable scatter of one pass: element order inside the block is
+// (round, thread), the order of the input positions.
+template <bool FIRST>
+__global__ void __launch_bounds__(GS_RS_THREADS) k_rs_scatter(Dims d, BinArgs a, int pass)
+{
+    constexpr int NB = FIRST ? 257 : 256;
+    constexpr int NW = GS_RS_THREADS / GS_WAVE;
+    __shared__ uint32_t run[NB];
+    __shared__ uint32_t wcnt[NW][NB];
+    const int r = blockIdx.y, blk = blockIdx.x, nblk = gridDim.x;
+    const int n = FIRST ? d.P : (int)a.nvis[r];
+    if (blk * GS_RS_TILE >= n) return;
+    const size_t rP = (size_t)r * d.P;
+    const int wave = threadIdx.x / GS_WAVE;
+    for (int i = threadIdx.x; i < NB; i += GS_RS_THREADS) {
+        run[i] = a.shist[((size_t)r * 257 + i) * nblk + blk];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) wcnt[w][i] = 0;
+    }
+    __syncthreads();
+    uint32_t* kout = a.skey[pass & 1] + rP;
+    uint32_t* vout = a.sval[pass & 1] + rP;
+    for (int k = 0; k < GS_RS_ITEMS; ++k) {
+        const int i = blk * GS_RS_TILE + k * GS_RS_THREADS + threadIdx.x;
+        const bool valid = i < n;
+        uint32_t key = 0, val = 0, dg = 0;
+        if (valid) rs_load<FIRST>(d, a, rP, i, pass, key, val, dg);
+        const uint64_t m = match_digit<FIRST ? 9 : 8>(dg, valid);
+        const uint32_t rank = (uint32_t)__popcll(m & lanemask_lt());
+        if (valid && rank == 0) wcnt[wave][dg] = (uint32_t)__popcll(m);
         __syncthreads();
-        for (int i = 0; i < rows; ++i) {
-            const int e = i * GS_SORT_THREADS + tid;
-            const bool valid = e < cn;
-            const uint32_t dg = valid ? ((sk[e0 + e] - kmin) >> shift) & 15u : 0u;
-            const uint64_t m = digit_match(valid, dg);
-            if (valid && (m & lt) == 0)
-                sm.table[dg * DS + i * GS_SORT_WAVES + wave] = (uint32_t)__popcll(m);
+        if (valid) {
+            uint32_t pos = run[dg] + rank;
+            for (int w = 0; w < wave; ++w) pos += wcnt[w][dg];
+            kout[pos] = key;
+            vout[pos] = val;
         }
         __syncthreads();
-        // exclusive scan of the table in (digit, row, wave) order
-        {
-            const int per = (S + GS_SORT_THREADS - 1) / GS_SORT_THREADS;
-            const int i0 = min(S, tid * per), i1 = min(S, i0 + per);
+        for (int j = threadIdx.x; j < NB; j += GS_RS_THREADS) {
             uint32_t s = 0;
-            for (int i = i0; i < i1; ++i) s += sm.table[i];
-            uint32_t tot;
-            uint32_t run = block_exclusive_scan<GS_SORT_THREADS>(s, sm.wscan, tot);
-            for (int i = i0; i < i1; ++i) {
-                const uint32_t c = sm.table[i];
-                sm.table[i] = run;
-                run += c;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                s += wcnt[w][j];
+                wcnt[w][j] = 0;
             }
+            run[j] += s;
         }
         __syncthreads();
-        // per-digit start inside this chunk: P(d,0,0)
-        for (int i = 0; i < rows; ++i) {
-            const int e = i * GS_SORT_THREADS + tid;
-            const bool valid = e < cn;
-            uint32_t key = 0;
-            uint32_t dg = 0;
-            if (valid) {
-                key = sk[e0 + e];
-                dg = ((key - kmin) >> shift) & 15u;
-            }
-            const uint64_t m = digit_match(valid, dg);
-            if (valid) {
-                uint32_t pos = sm.table[dg * DS + i * GS_SORT_WAVES + wave] + (uint32_t)__popcll(m & lt);
-                if (nchunks > 1) pos = pos - sm.table[dg * DS] + sm.base[dg];
-                dk[pos] = key;
-                dv[pos] = sv[e0 + e];
-            }
-        }
-        __syncthreads();
-        if (nchunks > 1) {
-            if (tid < 16) {
-                const uint32_t start = sm.table[tid * DS];
-                const uint32_t end = (tid < 15) ? sm.table[(tid + 1) * DS] : (uint32_t)cn;
-                sm.base[tid] += end - start;
-            }
-            __syncthreads();
-        }
     }
 }
 
-// Runs of equal depth bits come out of the radix passes in scatter order;
-// put each run in ascending Gaussian index, the order the reference's stable
-// sort produces from its index-ordered input.  Runs are rare and short.
-template <typename KP, typename VP>
-__device__ void order_ties(int n, KP k, VP v, uint32_t* s_flag)
+// ---------------------------------------------------------------- chunks
+__device__ __forceinline__ void chunk_rect(const Dims& d, const BinArgs& a, size_t rP, uint32_t g, int& x0, int& y0,
+                                           int& x1, int& y1)
 {
-    const int tid = threadIdx.x;
-    if (tid == 0) *s_flag = 0;
-    __syncthreads();
-    bool any = false;
-    for (int e = tid + 1; e < n; e += GS_SORT_THREADS) any |= (k[e] == k[e - 1]);
-    if (any) *s_flag = 1;
-    __syncthreads();
-    if (*s_flag) {
-        for (int e = tid; e < n; e += GS_SORT_THREADS) {
-            const bool starts = (e == 0 || k[e - 1] != k[e]) && e + 1 < n && k[e + 1] == k[e];
-            if (!starts) continue;
-            int end = e + 1;
-            while (end < n && k[end] == k[e]) ++end;
-            for (int i = e + 1; i < end; ++i) {        // insertion sort by index
-                const uint32_t x = v[i];
-                int j = i - 1;
-                while (j >= e && v[j] > x) { v[j + 1] = v[j]; --j; }
-                v[j + 1] = x;
-            }
-        }
+    const int radius = a.radii[rP + g];
+    x0 = y0 = x1 = y1 = 0;
+    if (radius > 0) {
+        const float4 s0 = a.splat[2 * (rP + g)];
+        get_rect(s0.x, s0.y, radius, d.gx, d.gy, x0, y0, x1, y1);
     }
-    __syncthreads();
 }
 
-// One workgroup per long segment (> GS_SORT_SMALL instances).  Persistent:
-// the grid walks the compacted list of long segments written by k_segscan.
-__global__ void __launch_bounds__(GS_SORT_THREADS) k_sort_long(BinArgs a)
+// per-tile counters of a chunk in LDS (else in the global count matrix)
+__host__ __device__ inline bool chunk_lds(const Dims& d)
 {
-    extern __shared__ unsigned char s_raw[];
-    SortShared& sm = *reinterpret_cast<SortShared*>(s_raw);
-    if ((uint64_t)a.total[0] > a.Kcap) return;
-    const int tid = threadIdx.x;
-    const uint32_t nlong = a.total[2];
-    for (uint32_t li = blockIdx.x; li < nlong; li += gridDim.x) {
-        const uint32_t seg = a.long_list[li];
-        const int n = (int)a.ranges[seg].y - (int)a.ranges[seg].x;
-        const uint32_t base = a.ranges[seg].x;
-        uint32_t* gk = a.keys + base;
-        uint32_t* gv = a.vals + base;
-        // key range of the segment
-        uint32_t kmin = 0xffffffffu, kmax = 0;
-        for (int e = tid; e < n; e += GS_SORT_THREADS) {
-            const uint32_t k = gk[e];
-            kmin = min(kmin, k);
-            kmax = max(kmax, k);
+    return d.T <= GS_CHUNK_LDS_TILES && (size_t)(d.gx + d.gy) * 8 + (size_t)d.T * 4 <= 65536;
+}
+
+// instances per (chunk, tile) -> ccnt[(r*T + t) * nchunks + c]
+__global__ void __launch_bounds__(GS_RS_THREADS) k_chunk_count(Dims d, BinArgs a)
+{
+    extern __shared__ uint32_t s_cnt[];
+    const int c = blockIdx.x, r = blockIdx.y;
+    const bool lds = chunk_lds(d);
+    const int n = (int)a.nvis[r];
+    const int i0 = c * a.chunk_len, i1 = min(n, i0 + a.chunk_len);
+    const size_t rP = (size_t)r * d.P;
+    uint32_t* slot = a.ccnt + (size_t)r * d.T * a.nchunks + c;   // tile t at slot[t * nchunks]
+    for (int t = threadIdx.x; t < d.T; t += GS_RS_THREADS) {
+        if (lds) s_cnt[t] = 0;
+        else slot[(size_t)t * a.nchunks] = 0;
+    }
+    __syncthreads();
+    for (int i = i0 + threadIdx.x; i < i1; i += GS_RS_THREADS) {
+        int x0, y0, x1, y1;
+        chunk_rect(d, a, rP, a.sval[1][rP + i], x0, y0, x1, y1);
+        for (int ty = y0; ty < y1; ++ty)
+            for (int tx = x0; tx < x1; ++tx) {
+                const int t = ty * d.gx + tx;
+                if (lds) atomicAdd(&s_cnt[t], 1u);
+                else atomicAdd(&slot[(size_t)t * a.nchunks], 1u);
+            }
+    }
+    if (lds) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < d.T; t += GS_RS_THREADS) slot[(size_t)t * a.nchunks] = s_cnt[t];
+    }
+}
+
+// per (receiver, tile): exclusive prefix over chunks; the total is the tile's
+// instance count
+__global__ void __launch_bounds__(256) k_chunk_scan(Dims d, BinArgs a)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= d.R * d.T) return;
+    uint32_t* row = a.ccnt + (size_t)i * a.nchunks;
+    uint32_t run = 0;
+    for (int c = 0; c < a.nchunks; ++c) {
+        const uint32_t v = row[c];
+        row[c] = run;
+        run += v;
+    }
+    a.segcnt[i] = run;
+}
+
+// One wave per chunk: order-preserving placement of the chunk's instances.
+// Dynamic LDS: column masks [gx], row masks [gy] (u64), then (when the tiles
+// fit) the running base of every tile [T] (u32).
+__global__ void __launch_bounds__(GS_WAVE) k_chunk_scatter(Dims d, BinArgs a)
+{
+    extern __shared__ uint64_t s_mask[];
+    uint64_t* s_mx = s_mask;
+    uint64_t* s_my = s_mask + d.gx;
+    uint32_t* s_base = (uint32_t*)(s_mask + d.gx + d.gy);
+    if ((uint64_t)a.total[0] > a.Kcap) return;   // capacity overflow: reported by the host
+    const int c = blockIdx.x, r = blockIdx.y;
+    const int n = (int)a.nvis[r];
+    const int i0 = c * a.chunk_len;
+    if (i0 >= n) return;
+    const int i1 = min(n, i0 + a.chunk_len);
+    const bool lds = chunk_lds(d);
+    const int lane = threadIdx.x;
+    const size_t rP = (size_t)r * d.P;
+    const size_t rT = (size_t)r * d.T;
+    uint32_t* slot = a.ccnt + rT * a.nchunks + c;
+    for (int t = lane; t < d.T; t += GS_WAVE) {
+        const uint32_t b = a.segbase[rT + t] + slot[(size_t)t * a.nchunks];
+        if (lds) s_base[t] = b;
+        else slot[(size_t)t * a.nchunks] = b;
+    }
+    __syncthreads();
+    const uint64_t lt = lanemask_lt();
+    for (int b0 = i0; b0 < i1; b0 += GS_WAVE) {
+        const int i = b0 + lane;
+        int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
+        uint32_t g = 0, dbits = 0;
+        if (i < i1) {
+            g = a.sval[1][rP + i];
+            dbits = a.skey[1][rP + i];
+            chunk_rect(d, a, rP, g, x0, y0, x1, y1);
         }
+        const bool ne = x0 < x1 && y0 < y1;
+        int ux0 = ne ? x0 : INT_MAX, uy0 = ne ? y0 : INT_MAX, ux1 = ne ? x1 : 0, uy1 = ne ? y1 : 0;
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
-            kmin = min(kmin, (uint32_t)__shfl_xor(kmin, off));
-            kmax = max(kmax, (uint32_t)__shfl_xor(kmax, off));
+            ux0 = min(ux0, __shfl_xor(ux0, off));
+            uy0 = min(uy0, __shfl_xor(uy0, off));
+            ux1 = max(ux1, __shfl_xor(ux1, off));
+            uy1 = max(uy1, __shfl_xor(uy1, off));
         }
-        if ((tid % GS_WAVE) == 0) { sm.red[0][tid / GS_WAVE] = kmin; sm.red[1][tid / GS_WAVE] = kmax; }
+        ux0 = __builtin_amdgcn_readfirstlane(ux0);
+        uy0 = __builtin_amdgcn_readfirstlane(uy0);
+        ux1 = __builtin_amdgcn_readfirstlane(ux1);
+        uy1 = __builtin_amdgcn_readfirstlane(uy1);
+        // lanes (Gaussians of this batch) covering each column / row
+        for (int tx = ux0; tx < ux1; ++tx) {
+            const uint64_t m = ballot(ne && x0 <= tx && tx < x1);
+            if (lane == 0) s_mx[tx] = m;
+        }
+        for (int ty = uy0; ty < uy1; ++ty) {
+            const uint64_t m = ballot(ne && y0 <= ty && ty < y1);
+            if (lane == 0) s_my[ty] = m;
+        }
         __syncthreads();
-        kmin = sm.red[0][0];
-        kmax = sm.red[1][0];
-        for (int w = 1; w < GS_SORT_WAVES; ++w) { kmin = min(kmin, sm.red[0][w]); kmax = max(kmax, sm.red[1][w]); }
-        const uint32_t span = kmax - kmin;
-        const int nbits = span ? 32 - __clz((int)span) : 0;
-        const int passes = (nbits + 3) / 4;
-        __syncthreads();
-
-        if (n <= GS_SORT_CHUNK) {
-            for (int e = tid; e < n; e += GS_SORT_THREADS) { sm.k[0][e] = gk[e]; sm.v[0][e] = gv[e]; }
-            __syncthreads();
-            int cur = 0;
-            for (int p = 0; p < passes; ++p) {
-                radix_pass(sm, n, sm.k[cur], sm.v[cur], sm.k[cur ^ 1], sm.v[cur ^ 1], kmin, 4 * p);
-                cur ^= 1;
-            }
-            order_ties(n, sm.k[cur], sm.v[cur], &sm.flag);
-            for (int e = tid; e < n; e += GS_SORT_THREADS) {
-                a.plist[base + e] = sm.v[cur][e];
-                if (a.keys_sorted) a.keys_sorted[base + e] = sm.k[cur][e];
-            }
-        } else {
-            uint32_t* tk = a.keys_tmp + base;
-            uint32_t* tv = a.vals_tmp + base;
-            int cur = 0;
-            for (int p = 0; p < passes; ++p) {
-                if (cur == 0) radix_pass(sm, n, gk, gv, tk, tv, kmin, 4 * p);
-                else radix_pass(sm, n, tk, tv, gk, gv, kmin, 4 * p);
-                cur ^= 1;
-            }
-            uint32_t* fk = cur ? tk : gk;
-            uint32_t* fv = cur ? tv : gv;
-            order_ties(n, fk, fv, &sm.flag);
-            for (int e = tid; e < n; e += GS_SORT_THREADS) {
-                a.plist[base + e] = fv[e];
-                if (a.keys_sorted) a.keys_sorted[base + e] = fk[e];
+        if (ne) {
+            for (int ty = y0; ty < y1; ++ty) {
+                const uint64_t myv = s_my[ty];
+                for (int tx = x0; tx < x1; ++tx) {
+                    const int t = ty * d.gx + tx;
+                    const uint64_t m = s_mx[tx] & myv;
+                    const uint32_t b = lds ? s_base[t] : slot[(size_t)t * a.nchunks];
+                    const uint32_t pos = b + (uint32_t)__popcll(m & lt);
+                    a.plist[pos] = g;
+                    if (a.keys_sorted) a.keys_sorted[pos] = dbits;
+                }
             }
         }
         __syncthreads();
-    }
-}
-
-// One wave per short segment (<= GS_SORT_SMALL instances), no workgroup
-// barriers.  The segment is double-buffered in LDS (2 x 16 KiB).  Each 4-bit
-// LSD pass takes four ballots per row of 64: lane d < 16 derives from them how
-// many elements of the row carry digit d (so the digit histogram and the
-// running digit offsets stay in lanes 0..15, with no LDS read-modify-write
-// chain) and every element derives its stable rank inside the row.  Runs of
-// equal depth are finally put in index order.
-__device__ __forceinline__ uint64_t digit_mask_of(uint32_t dg, uint64_t V, const uint64_t* B)
-{
-    uint64_t m = V;
-#pragma unroll
-    for (int bit = 0; bit < 4; ++bit) m &= ((dg >> bit) & 1u) ? B[bit] : ~B[bit];
-    return m;
-}
-
-__global__ void __launch_bounds__(GS_WAVE) k_sort_small(BinArgs a, int nseg)
-{
-    __shared__ uint2 buf[2][GS_SORT_SMALL];
-    if ((uint64_t)a.total[0] > a.Kcap) return;
-    const int seg = blockIdx.x;
-    if (seg >= nseg) return;
-    const uint2 rg = a.ranges[seg];
-    const int n = (int)(rg.y - rg.x);
-    if (n <= 0 || n > GS_SORT_SMALL) return;
-    const int lane = threadIdx.x;
-    const uint64_t lt = lanemask_lt();
-    const int rows = (n + GS_WAVE - 1) / GS_WAVE;
-    uint32_t kmin = 0xffffffffu, kmax = 0;
-    for (int e = lane; e < n; e += GS_WAVE) {
-        const uint2 t = make_uint2(a.keys[rg.x + e], a.vals[rg.x + e]);
-        buf[0][e] = t;
-        kmin = min(kmin, t.x);
-        kmax = max(kmax, t.x);
-    }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        kmin = min(kmin, (uint32_t)__shfl_xor(kmin, off));
-        kmax = max(kmax, (uint32_t)__shfl_xor(kmax, off));
-    }
-    const uint32_t span = kmax - kmin;
-    const int nbits = span ? 32 - __clz((int)span) : 0;
-    const int passes = (nbits + 3) / 4;
-    const uint32_t my_digit = (uint32_t)(lane & 15);
-    __syncthreads();
-    int cur = 0;
-    for (int p = 0; p < passes; ++p) {
-        const int shift = 4 * p;
-        const uint2* src = buf[cur];
-        uint2* dst = buf[cur ^ 1];
-        // digit histogram: lane d < 16 counts digit d
-        uint32_t cnt = 0;
-        for (int i = 0; i < rows; ++i) {
-            const int e = i * GS_WAVE + lane;
-            const bool valid = e < n;
-            const uint32_t dg = valid ? ((src[e].x - kmin) >> shift) & 15u : 0u;
-            uint64_t B[4];
-#pragma unroll
-            for (int bit = 0; bit < 4; ++bit) B[bit] = ballot(valid && ((dg >> bit) & 1u));
-            cnt += (uint32_t)__popcll(digit_mask_of(my_digit, ballot(valid), B));
-        }
-        // exclusive scan over the 16 digits
-        uint32_t x = lane < 16 ? cnt : 0u;
-#pragma unroll
-        for (int off = 1; off < 16; off <<= 1) {
-            const uint32_t y = __shfl_up(x, off);
-            if (lane >= off) x += y;
-        }
-        uint32_t base = x - (lane < 16 ? cnt : 0u);
-        // stable scatter
-        for (int i = 0; i < rows; ++i) {
-            const int e = i * GS_WAVE + lane;
-            const bool valid = e < n;
-            const uint2 t = valid ? src[e] : make_uint2(0u, 0u);
-            const uint32_t dg = ((t.x - kmin) >> shift) & 15u;
-            uint64_t B[4];
-#pragma unroll
-            for (int bit = 0; bit < 4; ++bit) B[bit] = ballot(valid && ((dg >> bit) & 1u));
-            const uint64_t V = ballot(valid);
-            const uint32_t rank = (uint32_t)__popcll(digit_mask_of(dg, V, B) & lt);
-            const uint32_t b = (uint32_t)__shfl((int)base, (int)dg);
-            if (valid) dst[b + rank] = t;
-            base += (uint32_t)__popcll(digit_mask_of(my_digit, V, B));
+        // advance the tile bases: the highest lane covering a tile does it
+        if (ne) {
+            for (int ty = y0; ty < y1; ++ty) {
+                const uint64_t myv = s_my[ty];
+                for (int tx = x0; tx < x1; ++tx) {
+                    const uint64_t m = s_mx[tx] & myv;
+                    if ((m >> lane) == 1ull) {
+                        const int t = ty * d.gx + tx;
+                        const uint32_t cnt = (uint32_t)__popcll(m);
+                        if (lds) s_base[t] += cnt;
+                        else slot[(size_t)t * a.nchunks] += cnt;
+                    }
+                }
+            }
         }
         __syncthreads();
-        cur ^= 1;
     }
-    // runs of equal depth bits -> index order
-    uint2* fin = buf[cur];
-    for (int e = lane; e < n; e += GS_WAVE) {
-        const bool starts = (e == 0 || fin[e - 1].x != fin[e].x) && e + 1 < n && fin[e + 1].x == fin[e].x;
-        if (!starts) continue;
-        int end = e + 1;
-        while (end < n && fin[end].x == fin[e].x) ++end;
-        for (int i = e + 1; i < end; ++i) {
-            const uint2 t = fin[i];
-            int j = i - 1;
-            while (j >= e && fin[j].y > t.y) { fin[j + 1] = fin[j]; --j; }
-            fin[j + 1] = t;
-        }
-    }
-    __syncthreads();
-    for (int e = lane; e < n; e += GS_WAVE) {
-        const uint2 t = fin[e];
-        a.plist[rg.x + e] = t.y;
-        if (a.keys_sorted) a.keys_sorted[rg.x + e] = t.x;
-    }
-}
-
-hipError_t launch_segscan(const Dims& d, const BinArgs& a, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_segscan, dim3(1), dim3(1024), 0, s, d.R * d.T, a.segcnt, a.segbase, a.ranges, a.total,
-                       a.long_list);
-    return hipGetLastError();
-}
-
-hipError_t launch_scatter(const Dims& d, const BinArgs& a, hipStream_t s)
-{
-    if (d.NB > 0 && d.R > 0)
-        hipLaunchKernelGGL(k_scatter, dim3(d.NB, d.R), dim3(GS_PRE_THREADS), 0, s, d, a);
-    return hipGetLastError();
-}
-
-hipError_t launch_sort_small(const Dims& d, const BinArgs& a, hipStream_t s)
-{
-    const int nseg = d.R * d.T;
-    if (nseg == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sort_small, dim3(nseg), dim3(GS_WAVE), 0, s, a, nseg);
-    return hipGetLastError();
-}
-
-hipError_t launch_sort_long(const Dims& d, const BinArgs& a, hipStream_t s)
-{
-    const int nseg = d.R * d.T;
-    if (nseg == 0) return hipSuccess;
-    // stages up to GS_SORT_CHUNK instances in LDS (about 136 KiB of the
-    // 160 KiB a gfx950 workgroup can hold); one block per CU
-    static hipError_t attr = hipFuncSetAttribute((const void*)k_sort_long, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                 (int)sizeof(SortShared));
-    if (attr != hipSuccess) return attr;
-    hipLaunchKernelGGL(k_sort_long, dim3(std::min(nseg, 256)), dim3(GS_SORT_THREADS), sizeof(SortShared), s, a);
-    return hipGetLastError();
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n)
 {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------- launchers
+int rs_blocks(int P) { return (P + GS_RS_TILE - 1) / GS_RS_TILE; }
+
+int chunk_len_for(const Dims& d)
+{
+    // long enough chunks that the count matrix stays within ~R*P entries
+    return GS_CHUNK_MIN * max(1, (d.T + GS_CHUNK_MIN - 1) / GS_CHUNK_MIN);
+}
+
+int chunk_count_for(const Dims& d) { return max(1, (d.P + chunk_len_for(d) - 1) / chunk_len_for(d)); }
+
+size_t chunk_scatter_lds(const Dims& d)
+{
+    return (size_t)(d.gx + d.gy) * sizeof(uint64_t) + (chunk_lds(d) ? (size_t)d.T * 4 : 0);
+}
+
+hipError_t launch_depth_sort(const Dims& d, const BinArgs& a, hipStream_t s)
+{
+    if (d.R == 0 || d.P == 0) return hipSuccess;
+    const int nblk = rs_blocks(d.P);
+    const dim3 grid(nblk, d.R);
+    for (int pass = 0; pass < 4; ++pass) {
+        if (pass == 0) {
+            hipLaunchKernelGGL(k_rs_hist<true>, grid, dim3(GS_RS_THREADS), 0, s, d, a, pass);
+            hipLaunchKernelGGL(k_rs_scan, dim3(d.R), dim3(1024), 0, s, d, a, nblk, 1);
+            hipLaunchKernelGGL(k_rs_scatter<true>, grid, dim3(GS_RS_THREADS), 0, s, d, a, pass);
+        } else {
+            hipLaunchKernelGGL(k_rs_hist<false>, grid, dim3(GS_RS_THREADS), 0, s, d, a, pass);
+            hipLaunchKernelGGL(k_rs_scan, dim3(d.R), dim3(1024), 0, s, d, a, nblk, 0);
+            hipLaunchKernelGGL(k_rs_scatter<false>, grid, dim3(GS_RS_THREADS), 0, s, d, a, pass);
+        }
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_chunk_count(const Dims& d, const BinArgs& a, hipStream_t s)
+{
+    if (d.R == 0) return hipSuccess;
+    const size_t lds = chunk_lds(d) ? (size_t)d.T * 4 : 0;
+    hipLaunchKernelGGL(k_chunk_count, dim3(a.nchunks, d.R), dim3(GS_RS_THREADS), lds, s, d, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_chunk_scan(const Dims& d, const BinArgs& a, hipStream_t s)
+{
+    const int n = d.R * d.T;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_chunk_scan, dim3((n + 255) / 256), dim3(256), 0, s, d, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_segscan(const Dims& d, const BinArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_segscan, dim3(1), dim3(1024), 0, s, d.R * d.T, a.segcnt, a.segbase, a.ranges, a.total);
+    return hipGetLastError();
+}
+
+hipError_t launch_chunk_scatter(const Dims& d, const BinArgs& a, hipStream_t s)
+{
+    if (d.R == 0 || d.P == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_chunk_scatter, dim3(a.nchunks, d.R), dim3(GS_WAVE), chunk_scatter_lds(d), s, d, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
-    const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, s, p, v, n);
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, v, n);
     return hipGetLastError();
 }
 

@@ -4,8 +4,6 @@
 // transmitter distance, opacity threshold) is done once; the emission MLP is
 // evaluated for chunks of GS_RC receivers (one chunk per blockIdx.y) so that
 // each Gaussian's 194 weights are streamed once per chunk, not per receiver.
-// Per-tile instance counts are accumulated in an LDS histogram and flushed
-// with one global atomic per (block, receiver, touched tile).
 #include "gs_device.h"
 #include "gs_kernels.h"
 
@@ -16,12 +14,6 @@ namespace gs {
 __global__ void __launch_bounds__(GS_PRE_THREADS)
 k_preprocess_fwd(Dims d, PreFwdArgs a)
 {
-    __shared__ uint32_t s_hist[GS_RC][GS_LDS_TILES];
-    const bool lds_tiles = d.T <= GS_LDS_TILES;
-    if (lds_tiles) {
-        for (int i = threadIdx.x; i < GS_RC * d.T; i += GS_PRE_THREADS) s_hist[i / d.T][i % d.T] = 0;
-        __syncthreads();
-    }
     const int g = blockIdx.x * GS_PRE_THREADS + threadIdx.x;
     const bool valid = g < d.P;
     const int P = d.P;
@@ -117,24 +109,6 @@ k_preprocess_fwd(Dims d, PreFwdArgs a)
                 }
                 a.radii[ri] = radius;
             }
-            // ---- per-tile instance counts of this block for receiver r:
-            // LDS histogram, then one global atomic per touched tile
-            if (lds_tiles) {
-                for (int ty = y0; ty < y1; ++ty)
-                    for (int tx = x0; tx < x1; ++tx) atomicAdd(&s_hist[c][ty * d.gx + tx], 1u);
-            } else {
-                for (int ty = y0; ty < y1; ++ty)
-                    for (int tx = x0; tx < x1; ++tx) atomicAdd(&a.segcnt[(size_t)r * d.T + ty * d.gx + tx], 1u);
-            }
-        }
-    }
-    if (lds_tiles) {
-        __syncthreads();
-        for (int i = threadIdx.x; i < GS_RC * d.T; i += GS_PRE_THREADS) {
-            const int c = i / d.T, t = i % d.T;
-            const int r = blockIdx.y * GS_RC + c;
-            const uint32_t v = s_hist[c][t];
-            if (v && r < d.R) atomicAdd(&a.segcnt[(size_t)r * d.T + t], v);
         }
     }
 }

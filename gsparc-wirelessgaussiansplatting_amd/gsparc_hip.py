@@ -28,9 +28,10 @@ _sz = C.c_size_t
 
 class Layout(C.Structure):
     _fields_ = [(n, C.c_size_t) for n in (
-        "splat", "sig", "depth", "cov3", "geom_bytes",
-        "final_t", "n_contrib", "ranges", "segcnt", "cursor", "segbase", "long_list", "total", "image_bytes",
-        "keys", "vals", "keys_tmp", "vals_tmp", "plist", "keys_sorted", "bin_bytes")]
+        "splat", "sig", "depth", "cov3", "sort_key0", "sort_val0", "sort_key1", "sort_val1", "sort_hist", "nvis",
+        "geom_bytes",
+        "final_t", "n_contrib", "ranges", "segcnt", "segbase", "chunk_cnt", "total", "image_bytes",
+        "plist", "keys_sorted", "bin_bytes")]
 
 
 _SIGS = {

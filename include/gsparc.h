@@ -43,12 +43,12 @@ extern "C" {
 
 /* Byte offsets of the arrays inside the caller's buffers (for inspection). */
 typedef struct gsparc_layout {
-    /* geom */
-    size_t splat, sig, depth, cov3, geom_bytes;
-    /* image */
-    size_t final_t, n_contrib, ranges, segcnt, cursor, segbase, long_list, total, image_bytes;
-    /* bin */
-    size_t keys, vals, keys_tmp, vals_tmp, plist, keys_sorted, bin_bytes;
+    /* geom: per (receiver, Gaussian) records and the per-receiver depth sort */
+    size_t splat, sig, depth, cov3, sort_key0, sort_val0, sort_key1, sort_val1, sort_hist, nvis, geom_bytes;
+    /* image: per pixel and per tile state, per (chunk, tile) counts */
+    size_t final_t, n_contrib, ranges, segcnt, segbase, chunk_cnt, total, image_bytes;
+    /* bin: the tile lists */
+    size_t plist, keys_sorted, bin_bytes;
 } gsparc_layout;
 
 const char* gsparc_version(void);

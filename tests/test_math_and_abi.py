@@ -77,11 +77,14 @@ def test_library_exports_every_declared_symbol():
 def test_buffer_layout_sizes():
     import gsparc_hip as gh
     L = gh.layout(50000, 64, 360, 90, 1 << 20)
-    assert L.splat == 0 and L.sig >= 64 * 50000 * 32 and L.geom_bytes > L.cov3
-    assert L.cursor == L.segcnt + 4 * 64 * 138
-    for off in (L.sig, L.depth, L.cov3, L.n_contrib, L.ranges, L.segcnt, L.plist):
+    assert L.splat == 0 and L.sig >= 64 * 50000 * 32 and L.geom_bytes > L.nvis > L.sort_hist
+    assert L.sort_val1 - L.sort_key1 >= 4 * 64 * 50000
+    assert L.chunk_cnt - L.segbase >= 4 * 64 * 138 and L.total - L.chunk_cnt >= 4 * 64 * 138
+    for off in (L.sig, L.depth, L.cov3, L.sort_key0, L.sort_val0, L.sort_key1, L.sort_val1, L.sort_hist,
+                L.n_contrib, L.ranges, L.segcnt, L.segbase, L.chunk_cnt, L.plist, L.keys_sorted):
         assert off % 256 == 0
-    assert gh.lib().gsparc_binning_buffer_size(1 << 20) >= 6 * 4 * (1 << 20)
+    # the tile lists need 4 bytes per instance, 8 with the sorted keys kept
+    assert 8 * (1 << 20) <= gh.lib().gsparc_binning_buffer_size(1 << 20) <= 8 * (1 << 20) + 1024
 
 
 def test_tuning_options():
