@@ -20,7 +20,7 @@ struct Dims {
 // binning chunks: at least this many sorted Gaussians per chunk; per-tile
 // counters of a chunk live in LDS up to this many tiles (90x360 has 138,
 // 1080x1920 has 8160), in the global count matrix beyond
-#define GS_CHUNK_MIN 1024
+#define GS_CHUNK_MIN 256
 #define GS_CHUNK_LDS_TILES 8192
 
 inline Dims make_dims(int P, int R, int W, int H)
@@ -127,6 +127,8 @@ struct PreBwdArgs {
     float mod;
     const int* radii;        // [R*P]
     const float* g8;         // [R*P*8]
+    const float2* sig;       // [R*P] MLP outputs of the forward (used when mlp != nullptr)
+    const float* cov3;       // [P*6] cov3D of the forward
     float* dmeans;           // [P,3]
     float* dscales;          // [P,3]
     float* drots;            // [P,4]

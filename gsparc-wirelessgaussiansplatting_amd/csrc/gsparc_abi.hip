@@ -244,6 +244,7 @@ int gsparc_rasterize_forward_geom(int P, int R, int W, int H, const float* means
     if (!geom || !image || !K_out || (P > 0 && R > 0 && (!means || !scales || !rotations || !opacities ||
                                                         !viewmatrices || !tx_pos || !rx_pos || !radii_out)))
         return fail(GSPARC_ERR_ARG, "null pointer");
+    if (((uintptr_t)emission_mlps & 15) != 0) return fail(GSPARC_ERR_ARG, "emission_mlps must be 16-byte aligned");
     hipStream_t s = (hipStream_t)stream;
     const gs::Dims d = gs::make_dims(P, R, W, H);
     gsparc_layout L;
@@ -353,6 +354,7 @@ int gsparc_rasterize_backward(int grad_mode, int P, int R, int W, int H, const f
     GS_LAUNCH("render_bwd", gs::launch_render_bwd(d, ra, gs::choose_ppl(d, g_opt_bwd_ppl), s), s);
     gs::PreBwdArgs pa;
     pa.means = means; pa.scales = scales; pa.rots = rotations; pa.mlp = emission_mlps; pa.views = viewmatrices;
+    pa.sig = (const float2*)(gb + L.sig); pa.cov3 = (const float*)(gb + L.cov3);
     pa.tx = tx_pos; pa.rx = rx_pos; pa.mod = scale_modifier; pa.radii = radii; pa.g8 = g8;
     pa.dmeans = dmeans3D; pa.dscales = dscales; pa.drots = drotations; pa.dopac = dopacity;
     pa.dmlp = demission_mlps; pa.dcov3 = dcov3D; pa.dmeans2d = dmeans2D; pa.dsig = demission; pa.mode = grad_mode;

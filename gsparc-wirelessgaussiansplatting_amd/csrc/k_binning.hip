@@ -265,28 +265,44 @@ __global__ void __launch_bounds__(GS_RS_THREADS) k_chunk_count(Dims d, BinArgs a
     }
 }
 
-// per (receiver, tile): exclusive prefix over chunks; the total is the tile's
-// instance count
+// per (receiver, tile): exclusive prefix over chunks, one wave per row; the
+// total is the tile's instance count
 __global__ void __launch_bounds__(256) k_chunk_scan(Dims d, BinArgs a)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= d.R * d.T) return;
-    uint32_t* row = a.ccnt + (size_t)i * a.nchunks;
-    uint32_t run = 0;
-    for (int c = 0; c < a.nchunks; ++c) {
-        const uint32_t v = row[c];
-        row[c] = run;
-        run += v;
+    const int row = blockIdx.x * (256 / GS_WAVE) + threadIdx.x / GS_WAVE;
+    const int lane = threadIdx.x % GS_WAVE;
+    if (row >= d.R * d.T) return;
+    uint32_t* cnt = a.ccnt + (size_t)row * a.nchunks;
+    uint32_t carry = 0;
+    for (int c0 = 0; c0 < a.nchunks; c0 += GS_WAVE) {
+        const int c = c0 + lane;
+        const uint32_t v = c < a.nchunks ? cnt[c] : 0u;
+        uint32_t x = v;
+#pragma unroll
+        for (int off = 1; off < GS_WAVE; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (c < a.nchunks) cnt[c] = carry + x - v;
+        carry += __shfl(x, GS_WAVE - 1);
     }
-    a.segcnt[i] = run;
+    if (lane == 0) a.segcnt[row] = carry;
 }
 
 // One wave per chunk: order-preserving placement of the chunk's instances.
+// The chunk is walked 64 sorted Gaussians (lanes) at a time; the batch's
+// (Gaussian, tile) pairs are spread evenly over the lanes, so one large
+// Gaussian does not serialise its batch.  An instance of lane i in tile t goes
+// to base(t) + (lanes below i covering t); the lanes covering t are the AND of
+// the column and row ballot masks.
 // Dynamic LDS: column masks [gx], row masks [gy] (u64), then (when the tiles
 // fit) the running base of every tile [T] (u32).
 __global__ void __launch_bounds__(GS_WAVE) k_chunk_scatter(Dims d, BinArgs a)
 {
     extern __shared__ uint64_t s_mask[];
+    __shared__ uint32_t s_end[GS_WAVE];    // inclusive prefix of the pair counts
+    __shared__ int4 s_own[GS_WAVE];        // x0, y0, width, Gaussian index
+    __shared__ uint32_t s_key[GS_WAVE];
     uint64_t* s_mx = s_mask;
     uint64_t* s_my = s_mask + d.gx;
     uint32_t* s_base = (uint32_t*)(s_mask + d.gx + d.gy);
@@ -306,8 +322,6 @@ __global__ void __launch_bounds__(GS_WAVE) k_chunk_scatter(Dims d, BinArgs a)
         if (lds) s_base[t] = b;
         else slot[(size_t)t * a.nchunks] = b;
     }
-    __syncthreads();
-    const uint64_t lt = lanemask_lt();
     for (int b0 = i0; b0 < i1; b0 += GS_WAVE) {
         const int i = b0 + lane;
         int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
@@ -318,6 +332,14 @@ __global__ void __launch_bounds__(GS_WAVE) k_chunk_scatter(Dims d, BinArgs a)
             chunk_rect(d, a, rP, g, x0, y0, x1, y1);
         }
         const bool ne = x0 < x1 && y0 < y1;
+        const uint32_t area = ne ? (uint32_t)((x1 - x0) * (y1 - y0)) : 0u;
+        uint32_t incl = area;
+#pragma unroll
+        for (int off = 1; off < GS_WAVE; off <<= 1) {
+            const uint32_t y = __shfl_up(incl, off);
+            if (lane >= off) incl += y;
+        }
+        const uint32_t npairs = __builtin_amdgcn_readlane(incl, GS_WAVE - 1);
         int ux0 = ne ? x0 : INT_MAX, uy0 = ne ? y0 : INT_MAX, ux1 = ne ? x1 : 0, uy1 = ne ? y1 : 0;
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
@@ -330,6 +352,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_chunk_scatter(Dims d, BinArgs a)
         uy0 = __builtin_amdgcn_readfirstlane(uy0);
         ux1 = __builtin_amdgcn_readfirstlane(ux1);
         uy1 = __builtin_amdgcn_readfirstlane(uy1);
+        __syncthreads();   // the previous batch is done with the LDS tables
         // lanes (Gaussians of this batch) covering each column / row
         for (int tx = ux0; tx < ux1; ++tx) {
             const uint64_t m = ballot(ne && x0 <= tx && tx < x1);
@@ -339,37 +362,38 @@ __global__ void __launch_bounds__(GS_WAVE) k_chunk_scatter(Dims d, BinArgs a)
             const uint64_t m = ballot(ne && y0 <= ty && ty < y1);
             if (lane == 0) s_my[ty] = m;
         }
+        s_end[lane] = incl;
+        s_own[lane] = make_int4(x0, y0, x1 - x0, (int)g);
+        s_key[lane] = dbits;
         __syncthreads();
-        if (ne) {
-            for (int ty = y0; ty < y1; ++ty) {
-                const uint64_t myv = s_my[ty];
-                for (int tx = x0; tx < x1; ++tx) {
-                    const int t = ty * d.gx + tx;
-                    const uint64_t m = s_mx[tx] & myv;
+        // pass 0 places the instances (reading the bases), pass 1 advances
+        // the bases (the highest lane covering a tile does it)
+        for (int pass = 0; pass < 2; ++pass) {
+            for (uint32_t p = lane; p < npairs; p += GS_WAVE) {
+                // owner: the first lane whose inclusive prefix exceeds p
+                int lo = 0;
+#pragma unroll
+                for (int step = GS_WAVE / 2; step >= 1; step >>= 1)
+                    if (s_end[lo + step - 1] <= p) lo += step;
+                const int4 ow = s_own[lo];
+                const uint32_t q = p - (lo ? s_end[lo - 1] : 0u);   // pair index within the owner's rect
+                const int ty = ow.y + (int)(q / (uint32_t)ow.z);
+                const int tx = ow.x + (int)(q % (uint32_t)ow.z);
+                const int t = ty * d.gx + tx;
+                const uint64_t m = s_mx[tx] & s_my[ty];
+                if (pass == 0) {
                     const uint32_t b = lds ? s_base[t] : slot[(size_t)t * a.nchunks];
-                    const uint32_t pos = b + (uint32_t)__popcll(m & lt);
-                    a.plist[pos] = g;
-                    if (a.keys_sorted) a.keys_sorted[pos] = dbits;
+                    const uint32_t pos = b + (uint32_t)__popcll(m & ((1ull << lo) - 1ull));
+                    a.plist[pos] = (uint32_t)ow.w;
+                    if (a.keys_sorted) a.keys_sorted[pos] = s_key[lo];
+                } else if ((m >> lo) == 1ull) {
+                    const uint32_t cnt = (uint32_t)__popcll(m);
+                    if (lds) s_base[t] += cnt;
+                    else slot[(size_t)t * a.nchunks] += cnt;
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
-        // advance the tile bases: the highest lane covering a tile does it
-        if (ne) {
-            for (int ty = y0; ty < y1; ++ty) {
-                const uint64_t myv = s_my[ty];
-                for (int tx = x0; tx < x1; ++tx) {
-                    const uint64_t m = s_mx[tx] & myv;
-                    if ((m >> lane) == 1ull) {
-                        const int t = ty * d.gx + tx;
-                        const uint32_t cnt = (uint32_t)__popcll(m);
-                        if (lds) s_base[t] += cnt;
-                        else slot[(size_t)t * a.nchunks] += cnt;
-                    }
-                }
-            }
-        }
-        __syncthreads();
     }
 }
 
@@ -426,7 +450,7 @@ hipError_t launch_chunk_scan(const Dims& d, const BinArgs& a, hipStream_t s)
 {
     const int n = d.R * d.T;
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_chunk_scan, dim3((n + 255) / 256), dim3(256), 0, s, d, a);
+    hipLaunchKernelGGL(k_chunk_scan, dim3((n + 3) / 4), dim3(256), 0, s, d, a);
     return hipGetLastError();
 }
 

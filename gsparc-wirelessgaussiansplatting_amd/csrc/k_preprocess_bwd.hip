@@ -173,43 +173,16 @@ __global__ void __launch_bounds__(GS_BWD_THREADS) k_pre_bwd_geo(Dims d, PreBwdAr
     const int r0 = chunk * GS_BRC;
     const int nr = min(GS_BRC, d.R - r0);
     const float mean[3] = {a.means[3 * g], a.means[3 * g + 1], a.means[3 * g + 2]};
-    const float s[3] = {a.scales[3 * g], a.scales[3 * g + 1], a.scales[3 * g + 2]};
-    const float q[4] = {a.rots[4 * g], a.rots[4 * g + 1], a.rots[4 * g + 2], a.rots[4 * g + 3]};
+    // cov3D as the forward stored it
     float c3[6];
-    cov3d(s, a.mod, q, c3);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) c3[k] = a.cov3[6 * g + k];
     const float diff[3] = {a.tx[0] - mean[0], a.tx[1] - mean[1], a.tx[2] - mean[2]};
     const float dist = sqrtf(diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2]);
-    const float ld = gs_logf(dist + 1e-6f);
     const float de = dist + 1e-6f;
     const float den = (MODE == GRAD_REFERENCE) ? de * de : dist * de;
 
     float dmean[3] = {0, 0, 0}, dc[6] = {0, 0, 0, 0, 0, 0}, dop = 0.0f, db2_0 = 0.0f, db2_1 = 0.0f;
-    float z0[GS_BRC], z1[GS_BRC];
-    if (a.mlp) {
-        const float* W1 = a.mlp + (size_t)g * 96;
-        const float* b1 = a.mlp + (size_t)P * 96 + (size_t)g * 32;
-        const float* W2 = a.mlp + (size_t)P * 128 + (size_t)g * 64;
-        const float* b2 = a.mlp + (size_t)P * 192 + (size_t)g * 2;
-#pragma unroll
-        for (int c = 0; c < GS_BRC; ++c) { z0[c] = b2[0]; z1[c] = b2[1]; }
-        for (int j = 0; j < 32; ++j) {
-            const float w0 = W1[j * 3], w1 = W1[j * 3 + 1], w2 = W1[j * 3 + 2], bb = b1[j];
-            const float v0 = W2[j], v1 = W2[32 + j];
-#pragma unroll
-            for (int c = 0; c < GS_BRC; ++c) {
-                if (c < nr) {
-                    const float* rx = a.rx + 3 * (r0 + c);
-                    float hs = bb;
-                    hs = hs + w0 * rx[0];
-                    hs = hs + w1 * rx[1];
-                    hs = hs + w2 * rx[2];
-                    const float h = (hs > 0.0f) ? hs : 0.01f * hs;
-                    z0[c] = z0[c] + v0 * h;
-                    z1[c] = z1[c] + v1 * h;
-                }
-            }
-        }
-    }
 #pragma unroll
     for (int c = 0; c < GS_BRC; ++c) {
         if (c >= nr) break;
@@ -228,8 +201,9 @@ __global__ void __launch_bounds__(GS_BWD_THREADS) k_pre_bwd_geo(Dims d, PreBwdAr
                 a.dmeans2d[3 * ri + 2] = 0.0f;
             }
             if (a.mlp) {
-                const float s0 = 1.0f / (1.0f + gs_expf(-(z0[c] - ld)));
-                const float s1 = 1.0f / (1.0f + gs_expf(-(z1[c] - ld)));
+                // MLP outputs as the forward computed them
+                const float2 sg = a.sig[ri];
+                const float s0 = sg.x, s1 = sg.y;
                 dz.x = gv[6] * s0 * (1.0f - s0);
                 dz.y = gv[7] * s1 * (1.0f - s1);
                 const float dld = -(dz.x + dz.y);

@@ -42,27 +42,36 @@ k_preprocess_fwd(Dims d, PreFwdArgs a)
         // ---- emission MLP 3->32->2 for receivers r0..r0+nr (forward.cu:17-75)
         float z0[GS_RC], z1[GS_RC];
         if (a.mlp && valid) {
-            const float* W1 = a.mlp + (size_t)g * 96;
-            const float* b1 = a.mlp + (size_t)P * 96 + (size_t)g * 32;
-            const float* W2 = a.mlp + (size_t)P * 128 + (size_t)g * 64;
-            const float* b2 = a.mlp + (size_t)P * 192 + (size_t)g * 2;
-            const float b20 = b2[0], b21 = b2[1];
+            // the Gaussian's weights as 16-byte loads (the ABI checks the
+            // alignment), four hidden units per step, summed in unit order
+            const float4* W1v = reinterpret_cast<const float4*>(a.mlp + (size_t)g * 96);
+            const float4* b1v = reinterpret_cast<const float4*>(a.mlp + (size_t)P * 96 + (size_t)g * 32);
+            const float4* W2v = reinterpret_cast<const float4*>(a.mlp + (size_t)P * 128 + (size_t)g * 64);
+            const float2 b2 = *reinterpret_cast<const float2*>(a.mlp + (size_t)P * 192 + (size_t)g * 2);
 #pragma unroll
-            for (int c = 0; c < GS_RC; ++c) { z0[c] = b20; z1[c] = b21; }
-            for (int j = 0; j < 32; ++j) {
-                const float w0 = W1[j * 3], w1 = W1[j * 3 + 1], w2 = W1[j * 3 + 2], bb = b1[j];
-                const float v0 = W2[j], v1 = W2[32 + j];
+            for (int c = 0; c < GS_RC; ++c) { z0[c] = b2.x; z1[c] = b2.y; }
+            for (int j4 = 0; j4 < 8; ++j4) {
+                const float4 wa = W1v[3 * j4], wb = W1v[3 * j4 + 1], wc = W1v[3 * j4 + 2];
+                const float4 bq = b1v[j4], va = W2v[j4], vb = W2v[8 + j4];
+                const float w[12] = {wa.x, wa.y, wa.z, wa.w, wb.x, wb.y, wb.z, wb.w, wc.x, wc.y, wc.z, wc.w};
+                const float bu[4] = {bq.x, bq.y, bq.z, bq.w};
+                const float v0u[4] = {va.x, va.y, va.z, va.w}, v1u[4] = {vb.x, vb.y, vb.z, vb.w};
 #pragma unroll
-                for (int c = 0; c < GS_RC; ++c) {
-                    if (c < nr) {
-                        const float* rx = a.rx + 3 * (r0 + c);
-                        float s = bb;
-                        s = s + w0 * rx[0];
-                        s = s + w1 * rx[1];
-                        s = s + w2 * rx[2];
-                        const float h = (s > 0.0f) ? s : 0.01f * s;
-                        z0[c] = z0[c] + v0 * h;
-                        z1[c] = z1[c] + v1 * h;
+                for (int u = 0; u < 4; ++u) {
+                    const float w0 = w[3 * u], w1 = w[3 * u + 1], w2 = w[3 * u + 2], bb = bu[u];
+                    const float v0 = v0u[u], v1 = v1u[u];
+#pragma unroll
+                    for (int c = 0; c < GS_RC; ++c) {
+                        if (c < nr) {
+                            const float* rx = a.rx + 3 * (r0 + c);
+                            float s = bb;
+                            s = s + w0 * rx[0];
+                            s = s + w1 * rx[1];
+                            s = s + w2 * rx[2];
+                            const float h = (s > 0.0f) ? s : 0.01f * s;
+                            z0[c] = z0[c] + v0 * h;
+                            z1[c] = z1[c] + v1 * h;
+                        }
                     }
                 }
             }

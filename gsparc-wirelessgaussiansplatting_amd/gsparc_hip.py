@@ -149,3 +149,9 @@ def f32(t: torch.Tensor) -> torch.Tensor:
     if t.dtype != torch.float32:
         t = t.float()
     return t.contiguous()
+
+
+def f32_aligned(t: torch.Tensor) -> torch.Tensor:
+    """As f32, and 16-byte aligned (a copy when a view starts mid-vector)."""
+    t = f32(t)
+    return t if t.data_ptr() % 16 == 0 else t.clone()

@@ -75,7 +75,7 @@ def forward_batched(means3D, emission_mlps, opacities, scales, rotations, image_
     rot = gh.f32(rotations)
     op = gh.f32(opacities).reshape(-1)
     tx = gh.f32(tx_pos).reshape(-1)
-    mlp = gh.f32(emission_mlps).reshape(-1) if emission is None else None
+    mlp = gh.f32_aligned(emission_mlps).reshape(-1) if emission is None else None
     em = None
     if emission is not None:
         em = gh.f32(emission).reshape(R, P, 2)
@@ -133,7 +133,7 @@ def backward_batched(means3D, emission_mlps, radii, opacities, scales, rotations
     views = gh.f32(viewmatrices).reshape(-1, 16)
     rxs = gh.f32(rx_pos).reshape(-1, 3)
     tx = gh.f32(tx_pos).reshape(-1)
-    mlp = gh.f32(emission_mlps).reshape(-1) if emission is None else None
+    mlp = gh.f32_aligned(emission_mlps).reshape(-1) if emission is None else None
     em = gh.f32(emission).reshape(R, P, 2) if emission is not None else None
     rad = radii.reshape(R, P).to(torch.int32).contiguous()
     gre = gh.f32(dL_dout_real)

@@ -71,15 +71,16 @@ int gsparc_layout_query(int P, int R, int W, int H, uint64_t K, gsparc_layout* o
 
 /* Forward, part 1: per-Gaussian preprocess for R receivers (projection,
  * covariance, conic, radius, tile rect, emission MLP), per-tile instance
- * counts and tile ranges.  Copies K (the number of duplicated instances,
+ * depth order per receiver and per-tile instance counts and tile ranges.
+ * Copies K (the number of duplicated instances,
  * "num_rendered") to *K_out and synchronises the stream, as the reference
  * does before sizing its binning buffer.
  * Replaces: CudaRasterizer::Rasterizer::forward up to the K readback
  * (cuda_wireless_rasterizer/rasterizer_impl.cu:206-278), FORWARD::preprocess
  * (forward.cu:252-327, 359-406) and the inclusive scan (rasterizer_impl.cu:270).
  *   means [P,3], scales [P,3], rotations [P,4] (w,x,y,z), opacities [P],
- *   emission_mlps [P*194] (gaussian.py:554-572) or NULL when emission is
- *   supplied to part 2, viewmatrices [R,16] column-major, tx_pos [3],
+ *   emission_mlps [P*194] (gaussian.py:554-572; 16-byte aligned) or NULL when
+ *   emission is supplied to part 2, viewmatrices [R,16] column-major, tx_pos [3],
  *   rx_pos [R,3]; radii_out [R,P] int32. */
 int gsparc_rasterize_forward_geom(int P, int R, int W, int H, const float* means, const float* scales,
                                   const float* rotations, const float* opacities, const float* emission_mlps,
