@@ -161,26 +161,39 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
     }
 }
 
-// Sum over the 64 lanes with DPP row operations; the total lands in lane 63.
-//   quad_perm [1,0,3,2], quad_perm [2,3,0,1]   -> sums of quads
-//   row_shr:4, row_shr:8                        -> lane 15 of each row holds the row sum
-//   row_bcast:15 (rows 1,3), row_bcast:31 (rows 2,3) -> lane 63 holds the wave sum
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ float dpp_step(float v)
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v)
 {
-    const int moved = __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false);
-    return v + __int_as_float(moved);
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
 }
 
-__device__ __forceinline__ float wave_sum_to_lane63(float v)
+// Sums each of v[0..7] over the 64 lanes; lane 8*i returns the total of v[i]
+// (every lane of the 8-lane group i holds it).  Transpose reduction: the
+// exchanges across 32, 16 and 8 lanes each halve the number of values a lane
+// carries (4, 2, 1), then an 8-lane sum: quad_perm [1,0,3,2], quad_perm
+// [2,3,0,1], row_half_mirror.  ~30 instructions instead of 8 full reductions.
+__device__ __forceinline__ float wave_sum8(const float (&v)[8], int lane)
 {
-    v = dpp_step<0xb1, 0xf>(v);
-    v = dpp_step<0x4e, 0xf>(v);
-    v = dpp_step<0x114, 0xf>(v);
-    v = dpp_step<0x118, 0xf>(v);
-    v = dpp_step<0x142, 0xa>(v);
-    v = dpp_step<0x143, 0xc>(v);
-    return v;
+    const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8;
+    float w[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float keep = b5 ? v[i + 4] : v[i];
+        const float send = b5 ? v[i] : v[i + 4];
+        w[i] = keep + __shfl_xor(send, 32);
+    }
+    float u[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const float keep = b4 ? w[i + 2] : w[i];
+        const float send = b4 ? w[i] : w[i + 2];
+        u[i] = keep + __shfl_xor(send, 16);
+    }
+    float x = (b3 ? u[1] : u[0]) + dpp_move<0x128>(b3 ? u[0] : u[1]);   // row_ror:8 = lane ^ 8
+    x = x + dpp_move<0xb1>(x);
+    x = x + dpp_move<0x4e>(x);
+    x = x + dpp_move<0x141>(x);
+    return x;
 }
 
 template <int MODE, int PPL, bool COUNT>
@@ -249,7 +262,8 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
             const float4 B = s1[j];
             const float2 S = ss[j];
             const float qa = -2.0f * A.z, qb = -A.w, qc = -2.0f * B.x;
-            float v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            // -0 is the identity of float addition, so the first += folds away
+            float v[8] = {-0.0f, -0.0f, -0.0f, -0.0f, -0.0f, -0.0f, -0.0f, -0.0f};
             bool anyc = false;
 #pragma unroll
             for (int k = 0; k < PPL; ++k) {
@@ -295,16 +309,8 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
                 }
             }
             if (ballot(anyc) == 0) continue;
-            float tot[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q)
-                tot[q] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wave_sum_to_lane63(v[q])), 63));
-            if (lane < 8) {
-                float x = tot[0];
-#pragma unroll
-                for (int q = 1; q < 8; ++q) x = (lane == q) ? tot[q] : x;
-                atomicAdd(a.g8 + (rP + sid[j]) * 8 + lane, x);
-            }
+            const float tot = wave_sum8(v, lane);
+            if ((lane & 7) == 0) atomicAdd(a.g8 + (rP + sid[j]) * 8 + (lane >> 3), tot);
         }
         __syncthreads();
     }
