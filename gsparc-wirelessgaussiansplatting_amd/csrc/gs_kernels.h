@@ -76,9 +76,12 @@ hipError_t launch_chunk_count(const Dims& d, const BinArgs& a, hipStream_t s);
 hipError_t launch_chunk_scan(const Dims& d, const BinArgs& a, hipStream_t s);
 hipError_t launch_segscan(const Dims& d, const BinArgs& a, hipStream_t s);
 hipError_t launch_chunk_scatter(const Dims& d, const BinArgs& a, hipStream_t s);
+// order[R*T]: the (receiver, tile) pairs by descending list length (bucketed)
+hipError_t launch_tile_order(const Dims& d, const uint2* ranges, uint32_t* order, hipStream_t s);
 
 struct RenderFwdArgs {
     const uint2* ranges;     // [R*T]
+    const uint32_t* order;   // [R*T] (receiver*T + tile), longest lists first
     const uint32_t* plist;
     const float4* splat;     // [R*P*2]
     const float2* sig;       // [R*P]
@@ -101,6 +104,7 @@ hipError_t launch_render_fwd(const Dims& d, const RenderFwdArgs& a, int ppl, hip
 
 struct RenderBwdArgs {
     const uint2* ranges;
+    const uint32_t* order;
     const uint32_t* plist;
     const float4* splat;
     const float2* sig;

@@ -60,6 +60,7 @@ void make_layout(int P, int R, int W, int H, uint64_t K, gsparc_layout& L)
     L.ranges = o; o = al(o + RT * sizeof(uint2));
     L.segcnt = o; o = al(o + RT * sizeof(uint32_t));
     L.segbase = o; o = al(o + RT * sizeof(uint32_t));
+    L.tile_order = o; o = al(o + RT * sizeof(uint32_t));
     L.chunk_cnt = o; o = al(o + RT * gs::chunk_count_for(d) * sizeof(uint32_t));
     L.total = o; o = al(o + 4 * sizeof(uint32_t));
     L.image_bytes = o + ALIGN;
@@ -267,6 +268,7 @@ int gsparc_rasterize_forward_geom(int P, int R, int W, int H, const float* means
             GS_LAUNCH("clear_counts", gs::launch_fill_u32(ba.segcnt, 0u, (size_t)R * d.T, s), s);
         }
         GS_LAUNCH("bin_segscan", gs::launch_segscan(d, ba, s), s);
+        GS_LAUNCH("bin_order", gs::launch_tile_order(d, ba.ranges, (uint32_t*)(ib + L.tile_order), s), s);
         uint32_t tot = 0;
         GS_HIP(hipMemcpyAsync(&tot, ba.total, sizeof(tot), hipMemcpyDeviceToHost, s));
         GS_HIP(hipStreamSynchronize(s));
@@ -299,6 +301,7 @@ int gsparc_rasterize_forward_render(int P, int R, int W, int H, const float* emi
     if (P > 0) GS_LAUNCH("bin_scatter", gs::launch_chunk_scatter(d, ba, s), s);
     gs::RenderFwdArgs ra;
     ra.ranges = ba.ranges;
+    ra.order = (const uint32_t*)(ib + L.tile_order);
     ra.plist = ba.plist;
     ra.splat = ba.splat;
     ra.sig = emission ? (const float2*)emission : (const float2*)(gb + L.sig);
@@ -339,6 +342,7 @@ int gsparc_rasterize_backward(int grad_mode, int P, int R, int W, int H, const f
     GS_LAUNCH("grad_clear", gs::launch_fill_u32((uint32_t*)g8, 0u, (size_t)R * P * 8, s), s);
     gs::RenderBwdArgs ra;
     ra.ranges = (const uint2*)(ib + L.ranges);
+    ra.order = (const uint32_t*)(ib + L.tile_order);
     ra.plist = (const uint32_t*)(bb + L.plist);
     ra.splat = (const float4*)(gb + L.splat);
     ra.sig = emission ? (const float2*)emission : (const float2*)(gb + L.sig);

@@ -397,6 +397,41 @@ __global__ void __launch_bounds__(GS_WAVE) k_chunk_scatter(Dims d, BinArgs a)
     }
 }
 
+// ---------------------------------------------------------------- tile order
+// 128 buckets of list length: 4 per octave (leading bit + the next two).
+__device__ __forceinline__ int length_bucket(uint32_t L)
+{
+    if (L < 4) return (int)L;
+    const int e = 31 - __clz(L);
+    return 4 * e + (int)((L >> (e - 2)) & 3u);
+}
+
+// One workgroup: the (receiver, tile) pairs in descending bucket order.
+__global__ void __launch_bounds__(1024) k_tile_order(int n, const uint2* ranges, uint32_t* order)
+{
+    __shared__ uint32_t cnt[128];
+    for (int i = threadIdx.x; i < 128; i += 1024) cnt[i] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 1024) {
+        const uint2 rg = ranges[i];
+        atomicAdd(&cnt[length_bucket(rg.y - rg.x)], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int b = 127; b >= 0; --b) {
+            const uint32_t c = cnt[b];
+            cnt[b] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 1024) {
+        const uint2 rg = ranges[i];
+        order[atomicAdd(&cnt[length_bucket(rg.y - rg.x)], 1u)] = (uint32_t)i;
+    }
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -464,6 +499,14 @@ hipError_t launch_chunk_scatter(const Dims& d, const BinArgs& a, hipStream_t s)
 {
     if (d.R == 0 || d.P == 0) return hipSuccess;
     hipLaunchKernelGGL(k_chunk_scatter, dim3(a.nchunks, d.R), dim3(GS_WAVE), chunk_scatter_lds(d), s, d, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_order(const Dims& d, const uint2* ranges, uint32_t* order, hipStream_t s)
+{
+    const int n = d.R * d.T;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, s, n, ranges, order);
     return hipGetLastError();
 }
 

@@ -25,23 +25,30 @@ namespace gs {
 
 template <int PPL>
 struct Pixels {
-    int tile;
+    int r, tile;
     bool valid;
     int px[PPL], py[PPL];
     bool inside[PPL];
     float x0, x1, y0, y1;   // pixel-centre bounds of the wave's block (inclusive)
 };
 
+// Work item i = the i-th (receiver, tile) pair in order[] (tiles with the
+// longest lists first, so that they do not start last and set the kernel's
+// end); its WPT waves get block ids of the same XCD (b & 7), which shares
+// one L2.
 template <int PPL>
-__device__ __forceinline__ Pixels<PPL> map_pixels(const Dims& d)
+__device__ __forceinline__ Pixels<PPL> map_pixels(const Dims& d, const uint32_t* order)
 {
     constexpr int WPT = 4 / PPL;   // waves per tile
     Pixels<PPL> m;
     const int b = blockIdx.x;
     const int xcd = b & 7, s = b >> 3;
     const int sub = s % WPT;
-    m.tile = (s / WPT) * 8 + xcd;
-    m.valid = m.tile < d.T;
+    const int item = (s / WPT) * 8 + xcd;
+    m.valid = item < d.R * d.T;
+    const uint32_t rt = m.valid ? order[item] : 0u;
+    m.r = (int)(rt / (uint32_t)d.T);
+    m.tile = (int)(rt % (uint32_t)d.T);
     const int lane = threadIdx.x;
     const int tx = m.valid ? m.tile % d.gx : 0, ty = m.valid ? m.tile / d.gx : 0;
     int qx0 = 1, qx1 = 0, qy0 = 1, qy1 = 0;
@@ -67,14 +74,14 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
     __shared__ float4 s0[GS_WAVE];
     __shared__ float4 s1[GS_WAVE];
     __shared__ float2 ss[GS_WAVE];
-    const Pixels<PPL> pm = map_pixels<PPL>(d);
+    const Pixels<PPL> pm = map_pixels<PPL>(d, a.order);
     if (!pm.valid) return;
     if ((uint64_t)a.total[0] > a.Kcap) return;   // binning overflow, reported by the host
     bool any_inside = false;
 #pragma unroll
     for (int k = 0; k < PPL; ++k) any_inside |= pm.inside[k];
     if (ballot(any_inside) == 0) return;
-    const int r = blockIdx.y;
+    const int r = pm.r;
     const int lane = threadIdx.x;
     const uint2 range = a.ranges[(size_t)r * d.T + pm.tile];
     const uint32_t rb = __builtin_amdgcn_readfirstlane(range.x);
@@ -203,10 +210,10 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
     __shared__ float4 s1[GS_WAVE];
     __shared__ float2 ss[GS_WAVE];
     __shared__ uint32_t sid[GS_WAVE];
-    const Pixels<PPL> pm = map_pixels<PPL>(d);
+    const Pixels<PPL> pm = map_pixels<PPL>(d, a.order);
     if (!pm.valid) return;
     if ((uint64_t)a.total[0] > a.Kcap) return;
-    const int r = blockIdx.y;
+    const int r = pm.r;
     const int lane = threadIdx.x;
     float T[PPL], gre[PPL], gim[PPL], pxf[PPL], pyf[PPL];
     int last[PPL];
@@ -327,8 +334,8 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
 
 static inline dim3 render_grid(const Dims& d, int ppl)
 {
-    const int tiles8 = (d.T + 7) / 8 * 8;
-    return dim3(tiles8 * (4 / ppl), d.R);
+    const long long items8 = ((long long)d.R * d.T + 7) / 8 * 8;
+    return dim3((unsigned)(items8 * (4 / ppl)));
 }
 
 int choose_ppl(const Dims& d, int requested)
