@@ -88,14 +88,17 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
     const uint32_t re = __builtin_amdgcn_readfirstlane(range.y);
     const size_t rP = (size_t)r * d.P;
 
+    // T carries the termination flag in its sign: a pixel is done once
+    // T < 0, with |T| its final transmittance.  A done pixel can still pass
+    // the power and alpha tests, but then T (1 - alpha) < 0 < 1e-4 sends it
+    // down the termination path, which leaves T negative: no state changes.
     float T[PPL], are[PPL], aim[PPL], pxf[PPL], pyf[PPL];
     uint32_t last[PPL], cnt[PPL];
-    bool done[PPL];
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
-        T[k] = 1.0f; are[k] = 0.0f; aim[k] = 0.0f;
+        T[k] = pm.inside[k] ? 1.0f : -1.0f;
+        are[k] = 0.0f; aim[k] = 0.0f;
         last[k] = 0; cnt[k] = 0;
-        done[k] = !pm.inside[k];
         pxf[k] = (float)pm.px[k];
         pyf[k] = (float)pm.py[k];
     }
@@ -103,7 +106,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
     for (uint32_t start = rb; start < re; start += GS_WAVE) {
         bool live = false;
 #pragma unroll
-        for (int k = 0; k < PPL; ++k) live |= !done[k];
+        for (int k = 0; k < PPL; ++k) live |= T[k] > 0.0f;
         if (ballot(live) == 0) break;
         const uint32_t e = start + lane;
         bool keep = false;
@@ -128,14 +131,14 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
             const float4 B = s1[j];
 #pragma unroll
             for (int k = 0; k < PPL; ++k) {
-                if (COUNT) cnt[k] += done[k] ? 0u : 1u;
+                if (COUNT) cnt[k] += T[k] > 0.0f ? 1u : 0u;
                 const float dx = A.x - pxf[k], dy = A.y - pyf[k];
                 const float power = pair_power(A.z, A.w, B.x, dx, dy);
-                if (done[k] || power > 0.0f || power < B.z) continue;
+                if (power > 0.0f || power < B.z) continue;
                 const float alpha = fminf(0.99f, B.y * gs_expf_nonpos(power));
                 if (alpha < 1.0f / 255.0f) continue;
                 const float test_T = T[k] * (1.0f - alpha);
-                if (test_T < 0.0001f) { done[k] = true; continue; }
+                if (test_T < 0.0001f) { T[k] = -fabsf(T[k]); continue; }
                 const float2 S = ss[j];
                 const float w = alpha * T[k];
                 are[k] = fmaf(S.x, w, are[k]);
@@ -163,7 +166,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
         const size_t p = (size_t)r * d.W * d.H + (size_t)pm.py[k] * d.W + pm.px[k];
         a.out_re[p] = are[k];
         a.out_im[p] = aim[k];
-        a.finalT[p] = T[k];
+        a.finalT[p] = fabsf(T[k]);
         a.ncontrib[p] = last[k];
     }
 }
