@@ -45,6 +45,7 @@ def parse():
                     help="also run a plain device-to-device copy of this size (counter calibration)")
     ap.add_argument("--fwd-ppl", type=int, default=0, help="forward blend pixels per lane (0: library default)")
     ap.add_argument("--bwd-ppl", type=int, default=0, help="backward blend pixels per lane (0: automatic)")
+    ap.add_argument("--order", type=int, default=0, help="blend work order (0: automatic)")
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
 
@@ -74,6 +75,7 @@ def main():
     gh.lib()
     gh.set_option("render_fwd_ppl", args.fwd_ppl)
     gh.set_option("render_bwd_ppl", args.bwd_ppl)
+    gh.set_option("render_order", args.order)
     P, R = args.gaussians, args.receivers
     sc = S.config2(P=P, R=R)
     tgt_sc = S.config2(seed=3, P=P, R=R)

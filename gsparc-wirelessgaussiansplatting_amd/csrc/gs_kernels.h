@@ -76,12 +76,23 @@ hipError_t launch_chunk_count(const Dims& d, const BinArgs& a, hipStream_t s);
 hipError_t launch_chunk_scan(const Dims& d, const BinArgs& a, hipStream_t s);
 hipError_t launch_segscan(const Dims& d, const BinArgs& a, hipStream_t s);
 hipError_t launch_chunk_scatter(const Dims& d, const BinArgs& a, hipStream_t s);
-// order[R*T]: the (receiver, tile) pairs by descending list length (bucketed)
-hipError_t launch_tile_order(const Dims& d, const uint2* ranges, uint32_t* order, hipStream_t s);
+// Work order of the blend kernels over the (receiver, tile) pairs
+// (item = receiver * T + tile; 0xffffffff = none), longest lists first:
+//   ORDER_GLOBAL        one list; consecutive items go to consecutive XCDs
+//   ORDER_XCD_MIXED     XCD x takes the receivers r = x (mod 8), all their
+//                       tiles longest first (its L2 serves 1/8 of the data)
+//   ORDER_XCD_RECEIVER  as ORDER_XCD_MIXED, one receiver after the other
+//                       (the L2 holds about one receiver's records at a time)
+// The XCD orders are 8 lists of n8 = ceil(R/8)*T slots.  The mode used is
+// stored in total[1] for the blend kernels (forward and backward).
+enum TileOrder { ORDER_GLOBAL = 1, ORDER_XCD_MIXED = 2, ORDER_XCD_RECEIVER = 3 };
+size_t order_length(const Dims& d);
+hipError_t launch_tile_order(const Dims& d, int mode, const uint2* ranges, uint32_t* order, uint32_t* total,
+                             hipStream_t s);
 
 struct RenderFwdArgs {
     const uint2* ranges;     // [R*T]
-    const uint32_t* order;   // [R*T] (receiver*T + tile), longest lists first
+    const uint32_t* order;   // work order (see TileOrder; its mode is total[1])
     const uint32_t* plist;
     const float4* splat;     // [R*P*2]
     const float2* sig;       // [R*P]

@@ -22,6 +22,14 @@ thread_local std::string g_err;
 // enough tiles to fill the GPU with one wave per tile)
 int g_opt_fwd_ppl = 0;
 int g_opt_bwd_ppl = 0;
+int g_opt_order = 0;
+
+int choose_order(const gs::Dims& d)
+{
+    if (g_opt_order >= 1 && g_opt_order <= 3) return g_opt_order;
+    // per-XCD receiver sets once every XCD gets at least two receivers
+    return d.R >= 16 ? gs::ORDER_XCD_MIXED : gs::ORDER_GLOBAL;
+}
 
 int fail(int code, const std::string& msg)
 {
@@ -60,7 +68,7 @@ void make_layout(int P, int R, int W, int H, uint64_t K, gsparc_layout& L)
     L.ranges = o; o = al(o + RT * sizeof(uint2));
     L.segcnt = o; o = al(o + RT * sizeof(uint32_t));
     L.segbase = o; o = al(o + RT * sizeof(uint32_t));
-    L.tile_order = o; o = al(o + RT * sizeof(uint32_t));
+    L.tile_order = o; o = al(o + gs::order_length(d) * sizeof(uint32_t));
     L.chunk_cnt = o; o = al(o + RT * gs::chunk_count_for(d) * sizeof(uint32_t));
     L.total = o; o = al(o + 4 * sizeof(uint32_t));
     L.image_bytes = o + ALIGN;
@@ -169,6 +177,7 @@ static int* option_slot(const char* name)
 {
     if (std::strcmp(name, "render_fwd_ppl") == 0) return &g_opt_fwd_ppl;
     if (std::strcmp(name, "render_bwd_ppl") == 0) return &g_opt_bwd_ppl;
+    if (std::strcmp(name, "render_order") == 0) return &g_opt_order;
     return nullptr;
 }
 
@@ -177,8 +186,11 @@ int gsparc_set_option(const char* name, int value)
     if (!name) return fail(GSPARC_ERR_ARG, "null option name");
     int* slot = option_slot(name);
     if (!slot) return fail(GSPARC_ERR_ARG, std::string("unknown option ") + name);
-    if (value != 0 && value != 1 && value != 2 && value != 4)
+    if (slot == &g_opt_order) {
+        if (value < 0 || value > 3) return fail(GSPARC_ERR_ARG, "render_order must be 0, 1, 2 or 3");
+    } else if (value != 0 && value != 1 && value != 2 && value != 4) {
         return fail(GSPARC_ERR_ARG, std::string(name) + " must be 0, 1, 2 or 4");
+    }
     *slot = value;
     return GSPARC_OK;
 }
@@ -268,7 +280,8 @@ int gsparc_rasterize_forward_geom(int P, int R, int W, int H, const float* means
             GS_LAUNCH("clear_counts", gs::launch_fill_u32(ba.segcnt, 0u, (size_t)R * d.T, s), s);
         }
         GS_LAUNCH("bin_segscan", gs::launch_segscan(d, ba, s), s);
-        GS_LAUNCH("bin_order", gs::launch_tile_order(d, ba.ranges, (uint32_t*)(ib + L.tile_order), s), s);
+        GS_LAUNCH("bin_order", gs::launch_tile_order(d, choose_order(d), ba.ranges, (uint32_t*)(ib + L.tile_order),
+                                                     ba.total, s), s);
         uint32_t tot = 0;
         GS_HIP(hipMemcpyAsync(&tot, ba.total, sizeof(tot), hipMemcpyDeviceToHost, s));
         GS_HIP(hipStreamSynchronize(s));

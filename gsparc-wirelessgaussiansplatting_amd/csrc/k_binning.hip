@@ -406,14 +406,18 @@ __device__ __forceinline__ int length_bucket(uint32_t L)
     return 4 * e + (int)((L >> (e - 2)) & 3u);
 }
 
-// One workgroup: the (receiver, tile) pairs in descending bucket order.
-__global__ void __launch_bounds__(1024) k_tile_order(int n, const uint2* ranges, uint32_t* order)
+// One workgroup per list l: the tiles of receivers l, l + L, l + 2L, ...
+// (L = number of lists) by descending length bucket, into order + l * stride.
+__global__ void __launch_bounds__(1024) k_tile_order(const uint2* ranges, int R, int T, int stride, uint32_t* order)
 {
     __shared__ uint32_t cnt[128];
+    const int l = blockIdx.x, L = gridDim.x;
+    const int n = (R - l + L - 1) / L * T;
+    uint32_t* out = order + (size_t)l * stride;
     for (int i = threadIdx.x; i < 128; i += 1024) cnt[i] = 0;
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += 1024) {
-        const uint2 rg = ranges[i];
+        const uint2 rg = ranges[(size_t)(l + (i / T) * L) * T + i % T];
         atomicAdd(&cnt[length_bucket(rg.y - rg.x)], 1u);
     }
     __syncthreads();
@@ -427,8 +431,38 @@ __global__ void __launch_bounds__(1024) k_tile_order(int n, const uint2* ranges,
     }
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += 1024) {
-        const uint2 rg = ranges[i];
-        order[atomicAdd(&cnt[length_bucket(rg.y - rg.x)], 1u)] = (uint32_t)i;
+        const uint32_t rt = (uint32_t)((l + (i / T) * L) * T + i % T);
+        const uint2 rg = ranges[rt];
+        out[atomicAdd(&cnt[length_bucket(rg.y - rg.x)], 1u)] = rt;
+    }
+}
+
+// Mode 3: one workgroup per receiver r sorts its own tiles into slot r / 8
+// of XCD r % 8.
+__global__ void __launch_bounds__(256) k_tile_order_rx(const uint2* ranges, int T, int n8, uint32_t* order)
+{
+    __shared__ uint32_t cnt[128];
+    const int r = blockIdx.x;
+    uint32_t* out = order + (size_t)(r % 8) * n8 + (size_t)(r / 8) * T;
+    for (int i = threadIdx.x; i < 128; i += 256) cnt[i] = 0;
+    __syncthreads();
+    for (int t = threadIdx.x; t < T; t += 256) {
+        const uint2 rg = ranges[(size_t)r * T + t];
+        atomicAdd(&cnt[length_bucket(rg.y - rg.x)], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int b = 127; b >= 0; --b) {
+            const uint32_t c = cnt[b];
+            cnt[b] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < T; t += 256) {
+        const uint2 rg = ranges[(size_t)r * T + t];
+        out[atomicAdd(&cnt[length_bucket(rg.y - rg.x)], 1u)] = (uint32_t)(r * T + t);
     }
 }
 
@@ -502,11 +536,26 @@ hipError_t launch_chunk_scatter(const Dims& d, const BinArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_tile_order(const Dims& d, const uint2* ranges, uint32_t* order, hipStream_t s)
+size_t order_length(const Dims& d)
 {
-    const int n = d.R * d.T;
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, s, n, ranges, order);
+    // room for every mode: 8 XCD lists of ceil(R/8) receivers' tiles
+    return (size_t)8 * ((d.R + 7) / 8) * d.T;
+}
+
+hipError_t launch_tile_order(const Dims& d, int mode, const uint2* ranges, uint32_t* order, uint32_t* total,
+                             hipStream_t s)
+{
+    if (d.R == 0 || d.T == 0) return hipSuccess;
+    hipError_t e = launch_fill_u32(total + 1, (uint32_t)mode, 1, s);
+    if (e == hipSuccess) e = launch_fill_u32(order, 0xffffffffu, order_length(d), s);
+    if (e != hipSuccess) return e;
+    const int n8 = (d.R + 7) / 8 * d.T;
+    if (mode == ORDER_XCD_RECEIVER)
+        hipLaunchKernelGGL(k_tile_order_rx, dim3(d.R), dim3(256), 0, s, ranges, d.T, n8, order);
+    else if (mode == ORDER_XCD_MIXED)
+        hipLaunchKernelGGL(k_tile_order, dim3(8), dim3(1024), 0, s, ranges, d.R, d.T, n8, order);
+    else
+        hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, s, ranges, d.R, d.T, 0, order);
     return hipGetLastError();
 }
 

@@ -32,21 +32,24 @@ struct Pixels {
     float x0, x1, y0, y1;   // pixel-centre bounds of the wave's block (inclusive)
 };
 
-// Work item i = the i-th (receiver, tile) pair in order[] (tiles with the
-// longest lists first, so that they do not start last and set the kernel's
-// end); its WPT waves get block ids of the same XCD (b & 7), which shares
-// one L2.
+// Workgroup b runs on XCD b & 7 (workgroups are dealt to the XCDs round
+// robin).  Its wave takes slot s = b >> 3 of that XCD: the work item is
+// entry s / WPT of the XCD's order list (ORDER_XCD_*) or of the global list
+// dealt round robin (ORDER_GLOBAL); the WPT waves of an item share the XCD
+// and its L2.  Items come longest list first, so that they do not start last
+// and set the kernel's end.
 template <int PPL>
-__device__ __forceinline__ Pixels<PPL> map_pixels(const Dims& d, const uint32_t* order)
+__device__ __forceinline__ Pixels<PPL> map_pixels(const Dims& d, const uint32_t* order, int order_mode)
 {
     constexpr int WPT = 4 / PPL;   // waves per tile
     Pixels<PPL> m;
     const int b = blockIdx.x;
     const int xcd = b & 7, s = b >> 3;
     const int sub = s % WPT;
-    const int item = (s / WPT) * 8 + xcd;
-    m.valid = item < d.R * d.T;
-    const uint32_t rt = m.valid ? order[item] : 0u;
+    const size_t n8 = (size_t)(d.R + 7) / 8 * d.T;
+    const size_t idx = order_mode == ORDER_GLOBAL ? (size_t)(s / WPT) * 8 + xcd : xcd * n8 + s / WPT;
+    const uint32_t rt = idx < 8 * n8 ? order[idx] : 0xffffffffu;
+    m.valid = rt != 0xffffffffu;
     m.r = (int)(rt / (uint32_t)d.T);
     m.tile = (int)(rt % (uint32_t)d.T);
     const int lane = threadIdx.x;
@@ -74,7 +77,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
     __shared__ float4 s0[GS_WAVE];
     __shared__ float4 s1[GS_WAVE];
     __shared__ float2 ss[GS_WAVE];
-    const Pixels<PPL> pm = map_pixels<PPL>(d, a.order);
+    const Pixels<PPL> pm = map_pixels<PPL>(d, a.order, (int)a.total[1]);
     if (!pm.valid) return;
     if ((uint64_t)a.total[0] > a.Kcap) return;   // binning overflow, reported by the host
     bool any_inside = false;
@@ -213,7 +216,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
     __shared__ float4 s1[GS_WAVE];
     __shared__ float2 ss[GS_WAVE];
     __shared__ uint32_t sid[GS_WAVE];
-    const Pixels<PPL> pm = map_pixels<PPL>(d, a.order);
+    const Pixels<PPL> pm = map_pixels<PPL>(d, a.order, (int)a.total[1]);
     if (!pm.valid) return;
     if ((uint64_t)a.total[0] > a.Kcap) return;
     const int r = pm.r;
@@ -337,8 +340,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
 
 static inline dim3 render_grid(const Dims& d, int ppl)
 {
-    const long long items8 = ((long long)d.R * d.T + 7) / 8 * 8;
-    return dim3((unsigned)(items8 * (4 / ppl)));
+    return dim3((unsigned)(order_length(d) * (4 / ppl)));
 }
 
 int choose_ppl(const Dims& d, int requested)

@@ -59,7 +59,12 @@ const char* gsparc_last_error(void);
  *       backward blend kernels: 0 = automatic (1 for few tiles, else 2),
  *       1 = one 8x8 quarter-tile per wave, 2 = half a tile per wave, 4 = one
  *       16x16 tile per wave.  Speed only: the forward result is bit-identical
- *       either way, the backward differs by summation order. */
+ *       either way, the backward differs by summation order.
+ *   "render_order"  order in which the blend kernels take the (receiver,
+ *       tile) pairs, longest lists first: 0 = automatic (2 from 16
+ *       receivers on, else 1), 1 = one list dealt
+ *       round robin to the 8 XCDs, 2 = XCD x takes the receivers x mod 8,
+ *       3 = as 2, one receiver after the other.  Speed only. */
 int gsparc_set_option(const char* name, int value);
 int gsparc_get_option(const char* name, int* value);
 

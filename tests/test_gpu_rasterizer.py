@@ -105,7 +105,7 @@ def test_forward_batched_equals_single(ppl):
 
 
 def test_forward_long_segments():
-    """Segments longer than the LDS sort chunk take the global-memory path."""
+    """Tile lists of more than 8192 instances (the 50k-Gaussian wall scene)."""
     sc = S.config2(R=3)
     _, out = gpu_forward(sc)
     st = read_state(sc, out)
@@ -183,6 +183,34 @@ def test_backward_parity(mode, seed, P, R, ppl):
     assert _rel(dcov, acc.cov3) < GRAD_TOL
     assert _rel(dsc, acc.scales) < GRAD_TOL
     assert _rel(drot, acc.rotations) < GRAD_TOL
+
+
+@pytest.mark.parametrize("order", [1, 2, 3], ids=["global", "xcd_mixed", "xcd_receiver"])
+def test_work_orders(order):
+    """Every blend work order covers every (receiver, tile) exactly once,
+    forward and backward, also when R is not a multiple of the 8 XCDs."""
+    import gsparc_hip as gh
+    gh.set_option("render_order", order)
+    try:
+        R = 10
+        sc = S.small_scene(41, 1500, R=R, walls=True)
+        d, out = gpu_forward(sc)
+        st = read_state(sc, out)
+        for r in range(R):
+            _check_forward(sc, st, r)
+        rng = np.random.default_rng(41)
+        dre = rng.normal(size=(R, sc.H, sc.W)).astype(np.float32)
+        dim = rng.normal(size=(R, sc.H, sc.W)).astype(np.float32)
+        g = _gpu_backward(sc, d, out, dre, dim, "reference")
+        acc = O.Grads(sc.P)
+        for r in range(R):
+            pre, keys, vals, ranges, fwd = O.render(sc, r)
+            O.backward(sc, pre, vals, ranges, fwd, dre[r], dim[r], O.GRAD_REFERENCE, r=r, grads=acc)
+        assert _rel(g[1], acc.means) < GRAD_TOL
+        assert _rel(g[2], acc.mlp) < GRAD_TOL
+        assert _rel(g[3], acc.opacities) < GRAD_TOL
+    finally:
+        gh.set_option("render_order", 0)
 
 
 def test_autograd_drop_in():
