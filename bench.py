@@ -50,6 +50,24 @@ def parse():
     return ap.parse_args()
 
 
+TRAFFIC_FILE = "profiles/traffic_50k_64rx.txt"
+
+
+def measured_traffic(args, P, R):
+    """HBM bytes per launch of each kernel, from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs summarised in TRAFFIC_FILE (calibrated against a copy of
+    known size there).  Only for the configuration those runs measured."""
+    path = os.path.join(ROOT, TRAFFIC_FILE)
+    if (P, R, args.grad_mode) != (50000, 64, "reference") or not os.path.exists(path):
+        return {}
+    out = {}
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 4 and parts[3].startswith("traffic_bytes="):
+            out[parts[0]] = int(parts[3].split("=")[1])
+    return out
+
+
 def log(args, *a):
     if not args.quiet:
         print(*a, file=sys.stderr, flush=True)
@@ -173,40 +191,53 @@ def main():
     with torch.no_grad():
         K = wr.forward_batched(means, mlp, opac, scales, rots, sc.H, sc.W, 1.0, views, txp, rxp)[0]
     N = sc.W * sc.H
+    # algorithmic HBM bytes per launch (each datum read or written once)
     work = {
-        "render_fwd": ("valu", cnt["fwd_pairs"] * FWD_FLOP_PER_PAIR + cnt["fwd_entries"] * CULL_FLOP, "TFLOP/s"),
-        "render_bwd": ("valu", cnt["bwd_pairs"] * BWD_FLOP_PER_PAIR + cnt["bwd_entries"] * CULL_FLOP, "TFLOP/s"),
-        "preprocess_fwd": ("hbm", P * 820 + 44 * P * R, "GB/s"),
-        "preprocess_bwd": ("hbm", P * (820 + 32 * 2 + 776) + 32 * P * R, "GB/s"),
+        # 44 B per tile-list instance (index 4, splat record 32, emission 8)
+        # + 16 B per pixel (out re/im, final T, contributor count)
+        "render_fwd": ("hbm", 44.0 * K + 16.0 * N * R, "GB/s"),
+        # + 16 B per pixel read (final T, count, dL/dout re/im) + 32 B per
+        # (receiver, Gaussian) accumulated gradient record
+        "render_bwd": ("hbm", 44.0 * K + 16.0 * N * R + 32.0 * P * R, "GB/s"),
+        "preprocess_fwd": ("hbm", P * (820 + 24) + 48 * P * R, "GB/s"),
+        "preprocess_bwd": ("hbm", P * 1690 + 72 * P * R, "GB/s"),
         "bin_depth_sort": ("hbm", 84.0 * P * R, "GB/s"),
         "bin_chunk_count": ("hbm", 16.0 * P * R, "GB/s"),
         "bin_scatter": ("hbm", 4.0 * K + 20.0 * P * R, "GB/s"),
         "ssim_fwd": ("hbm", 8 * N * R + 16 * N * R, "GB/s"),
         "ssim_bwd": ("hbm", 28 * N * R, "GB/s"),
     }
+    # FP32 work of the blend kernels (vector ALU): per evaluated (pixel,
+    # Gaussian) pair and per list entry tested against a wave's pixel block
+    flops = {"render_fwd": cnt["fwd_pairs"] * FWD_FLOP_PER_PAIR + cnt["fwd_entries"] * CULL_FLOP,
+             "render_bwd": cnt["bwd_pairs"] * BWD_FLOP_PER_PAIR + cnt["bwd_entries"] * CULL_FLOP}
+    traffic = measured_traffic(args, P, R)
     kernels = {}
     for name, (ms, n) in kt.items():
         avg = ms / max(n, 1)
         entry = {"ms_per_launch": round(avg, 4), "launches_per_step": n / nprof}
         if name in work:
             bound, amount, unit = work[name]
-            if unit == "GB/s":
-                ach = amount / (avg * 1e-3) / 1e9
-                entry.update(bound=bound, achieved=round(ach, 1), unit=unit, frac=round(ach / PEAK_HBM_GBS, 4))
-            else:
-                ach = amount / (avg * 1e-3) / 1e12
-                entry.update(bound=bound, achieved=round(ach, 3), unit=unit, frac=round(ach / PEAK_FP32_TFLOPS, 4))
+            ach = amount / (avg * 1e-3) / 1e9
+            entry.update(bound=bound, achieved=round(ach, 1), unit=unit, frac=round(ach / PEAK_HBM_GBS, 4))
+        if name in flops:
+            entry["fp32_tflops"] = round(flops[name] / (avg * 1e-3) / 1e12, 3)
+        if name in traffic:
+            entry["traffic"] = traffic[name]
         kernels[name] = entry
     dom = max(kt.items(), key=lambda kv: kv[1][0])[0]
     d = kernels.get(dom, {})
-    roof = {"kernel": dom, "bound": d.get("bound"), "achieved": d.get("achieved"),
-            "peak": PEAK_HBM_GBS if d.get("unit") == "GB/s" else PEAK_FP32_TFLOPS, "unit": d.get("unit"),
-            "frac": d.get("frac"), "traffic": None,
+    roof = {"kernel": dom, "bound": d.get("bound"), "achieved": d.get("achieved"), "peak": PEAK_HBM_GBS,
+            "unit": d.get("unit"), "frac": d.get("frac"), "traffic": traffic.get(dom),
             "ms_per_launch": d.get("ms_per_launch")}
-    if dom in ("render_fwd", "render_bwd"):
+    if dom in flops:
         side = dom[-3:]
+        roof["fp32_tflops"] = d.get("fp32_tflops")
+        roof["fp32_peak_tflops"] = PEAK_FP32_TFLOPS
         roof["pairs_per_launch"] = cnt[f"{side}_pairs"]
         roof["entry_tests_per_launch"] = cnt[f"{side}_entries"]
+    if roof["traffic"] is not None:
+        roof["traffic_source"] = TRAFFIC_FILE
 
     cpu = None
     if args.cpu_receivers > 0:

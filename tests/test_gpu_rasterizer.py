@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(params=[1, 2, 4], ids=["ppl1", "ppl2", "ppl4"])
 def ppl(request):
-    """Run under both blend work mappings (8x8 quarter-tile or 16x16 tile per wave)."""
+    """Run under every blend pixel mapping (quarter, half or whole tile per wave)."""
     import gsparc_hip as gh
     gh.set_option("render_fwd_ppl", request.param)
     gh.set_option("render_bwd_ppl", request.param)

@@ -98,3 +98,13 @@ def test_tuning_options():
         gh.set_option("render_fwd_ppl", 3)
     with pytest.raises(RuntimeError, match="unknown option"):
         gh.set_option("no_such_option", 1)
+
+
+def test_reference_import_paths():
+    """GSpaRC's own import lines resolve to this build (renderer.py:5, train.py:40)."""
+    from cuda_kernel_rasterize.cuda_rasterizer import GaussianRasterizer, GaussianRasterizerSetting
+    import cuda_rasterizer
+    from fused_ssim import fused_ssim
+    assert GaussianRasterizer is cuda_rasterizer.GaussianRasterizer
+    assert GaussianRasterizerSetting is cuda_rasterizer.GaussianRasterizerSetting
+    assert callable(fused_ssim)
