@@ -247,18 +247,17 @@ int gsparc_layout_query(int P, int R, int W, int H, uint64_t K, gsparc_layout* o
     return GSPARC_OK;
 }
 
-int gsparc_rasterize_forward_geom(int P, int R, int W, int H, const float* means, const float* scales,
-                                  const float* rotations, const float* opacities, const float* emission_mlps,
-                                  const float* viewmatrices, const float* tx_pos, const float* rx_pos,
-                                  float scale_modifier, void* geom, void* image, int* radii_out,
-                                  uint64_t* K_out, void* stream)
+// Forward part 1 on the stream, without reading K back.
+static int forward_geom_async(int P, int R, int W, int H, const float* means, const float* scales,
+                              const float* rotations, const float* opacities, const float* emission_mlps,
+                              const float* viewmatrices, const float* tx_pos, const float* rx_pos,
+                              float scale_modifier, void* geom, void* image, int* radii_out, hipStream_t s)
 {
     if (int e = check_dims(P, R, W, H)) return e;
-    if (!geom || !image || !K_out || (P > 0 && R > 0 && (!means || !scales || !rotations || !opacities ||
-                                                        !viewmatrices || !tx_pos || !rx_pos || !radii_out)))
+    if (!geom || !image || (P > 0 && R > 0 && (!means || !scales || !rotations || !opacities || !viewmatrices ||
+                                               !tx_pos || !rx_pos || !radii_out)))
         return fail(GSPARC_ERR_ARG, "null pointer");
     if (((uintptr_t)emission_mlps & 15) != 0) return fail(GSPARC_ERR_ARG, "emission_mlps must be 16-byte aligned");
-    hipStream_t s = (hipStream_t)stream;
     const gs::Dims d = gs::make_dims(P, R, W, H);
     gsparc_layout L;
     make_layout(P, R, W, H, 0, L);
@@ -270,26 +269,55 @@ int gsparc_rasterize_forward_geom(int P, int R, int W, int H, const float* means
     pa.splat = (float4*)(gb + L.splat); pa.sig = (float2*)(gb + L.sig); pa.depth = (float*)(gb + L.depth);
     pa.radii = radii_out; pa.cov3 = (float*)(gb + L.cov3);
     if (P > 0 && R > 0) GS_LAUNCH("preprocess_fwd", gs::launch_preprocess_fwd(d, pa, s), s);
+    if (R == 0) return GSPARC_OK;
     gs::BinArgs ba = bin_args(d, L, gb, ib, radii_out);
-    if (R > 0) {
-        if (P > 0) {
-            GS_LAUNCH("bin_depth_sort", gs::launch_depth_sort(d, ba, s), s);
-            GS_LAUNCH("bin_chunk_count", gs::launch_chunk_count(d, ba, s), s);
-            GS_LAUNCH("bin_chunk_scan", gs::launch_chunk_scan(d, ba, s), s);
-        } else {
-            GS_LAUNCH("clear_counts", gs::launch_fill_u32(ba.segcnt, 0u, (size_t)R * d.T, s), s);
-        }
-        GS_LAUNCH("bin_segscan", gs::launch_segscan(d, ba, s), s);
-        GS_LAUNCH("bin_order", gs::launch_tile_order(d, choose_order(d), ba.ranges, (uint32_t*)(ib + L.tile_order),
-                                                     ba.total, s), s);
-        uint32_t tot = 0;
-        GS_HIP(hipMemcpyAsync(&tot, ba.total, sizeof(tot), hipMemcpyDeviceToHost, s));
-        GS_HIP(hipStreamSynchronize(s));
-        *K_out = tot;
+    if (P > 0) {
+        GS_LAUNCH("bin_depth_sort", gs::launch_depth_sort(d, ba, s), s);
+        GS_LAUNCH("bin_chunk_count", gs::launch_chunk_count(d, ba, s), s);
+        GS_LAUNCH("bin_chunk_scan", gs::launch_chunk_scan(d, ba, s), s);
     } else {
-        *K_out = 0;
+        GS_LAUNCH("clear_counts", gs::launch_fill_u32(ba.segcnt, 0u, (size_t)R * d.T, s), s);
     }
+    GS_LAUNCH("bin_segscan", gs::launch_segscan(d, ba, s), s);
+    GS_LAUNCH("bin_order", gs::launch_tile_order(d, choose_order(d), ba.ranges, (uint32_t*)(ib + L.tile_order),
+                                                 ba.total, s), s);
     return GSPARC_OK;
+}
+
+static int read_count(int P, int R, int W, int H, const void* image, uint64_t* K, hipStream_t s)
+{
+    gsparc_layout L;
+    make_layout(P, R, W, H, 0, L);
+    uint32_t tot = 0;
+    GS_HIP(hipMemcpyAsync(&tot, base_of((void*)image) + L.total, sizeof(tot), hipMemcpyDeviceToHost, s));
+    GS_HIP(hipStreamSynchronize(s));
+    *K = tot;
+    return GSPARC_OK;
+}
+
+int gsparc_rasterize_forward_geom(int P, int R, int W, int H, const float* means, const float* scales,
+                                  const float* rotations, const float* opacities, const float* emission_mlps,
+                                  const float* viewmatrices, const float* tx_pos, const float* rx_pos,
+                                  float scale_modifier, void* geom, void* image, int* radii_out,
+                                  uint64_t* K_out, void* stream)
+{
+    if (!K_out) return fail(GSPARC_ERR_ARG, "null pointer");
+    hipStream_t s = (hipStream_t)stream;
+    if (int e = forward_geom_async(P, R, W, H, means, scales, rotations, opacities, emission_mlps, viewmatrices,
+                                   tx_pos, rx_pos, scale_modifier, geom, image, radii_out, s))
+        return e;
+    *K_out = 0;
+    if (R == 0) return GSPARC_OK;
+    return read_count(P, R, W, H, image, K_out, s);
+}
+
+int gsparc_instance_count(int P, int R, int W, int H, const void* image, uint64_t* K, void* stream)
+{
+    if (int e = check_dims(P, R, W, H)) return e;
+    if (!image || !K) return fail(GSPARC_ERR_ARG, "null pointer");
+    *K = 0;
+    if (R == 0) return GSPARC_OK;
+    return read_count(P, R, W, H, image, K, (hipStream_t)stream);
 }
 
 int gsparc_rasterize_forward_render(int P, int R, int W, int H, const float* emission, void* geom, void* image,
@@ -327,6 +355,21 @@ int gsparc_rasterize_forward_render(int P, int R, int W, int H, const float* emi
     ra.counts = g_prof_on ? g_counts_dev : nullptr;
     GS_LAUNCH("render_fwd", gs::launch_render_fwd(d, ra, gs::choose_ppl(d, g_opt_fwd_ppl), s), s);
     return GSPARC_OK;
+}
+
+int gsparc_rasterize_forward(int P, int R, int W, int H, const float* means, const float* scales,
+                             const float* rotations, const float* opacities, const float* emission_mlps,
+                             const float* emission, const float* viewmatrices, const float* tx_pos,
+                             const float* rx_pos, float scale_modifier, void* geom, void* image, void* bin,
+                             uint64_t K_capacity, int* radii_out, float* out_real, float* out_imag, int flags,
+                             void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (int e = forward_geom_async(P, R, W, H, means, scales, rotations, opacities, emission_mlps, viewmatrices,
+                                   tx_pos, rx_pos, scale_modifier, geom, image, radii_out, s))
+        return e;
+    return gsparc_rasterize_forward_render(P, R, W, H, emission, geom, image, bin, K_capacity, radii_out, out_real,
+                                           out_imag, flags, stream);
 }
 
 int gsparc_rasterize_backward(int grad_mode, int P, int R, int W, int H, const float* means, const float* scales,

@@ -98,6 +98,11 @@ class BatchedSetting(NamedTuple):
     tx_pos: torch.Tensor            # [3]
     rx_pos: torch.Tensor            # [R,3]
     grad_mode: str = "reference"
+    # 0: size the tile lists exactly (one host synchronisation per forward);
+    # > 0: a fixed instance capacity and no host synchronisation, so that a
+    # whole training step can be captured in a HIP graph.  Check the capacity
+    # afterwards with wireless_rasterizer.instance_count().
+    capacity: int = 0
 
 
 class _RasterizeBatched(torch.autograd.Function):
@@ -106,7 +111,7 @@ class _RasterizeBatched(torch.autograd.Function):
         s = settings
         K, ore, oim, radii, geom, binbuf, img = wr.forward_batched(
             means3D, emission_mlps, opacities, scales, rotations, s.image_height, s.image_width, s.scale_modifier,
-            s.viewmatrices, s.tx_pos, s.rx_pos, emission=emission)
+            s.viewmatrices, s.tx_pos, s.rx_pos, emission=emission, capacity=s.capacity or None)
         ctx.s = s
         ctx.K = K
         ctx.has_emission = emission is not None

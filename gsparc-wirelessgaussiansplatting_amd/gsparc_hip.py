@@ -50,6 +50,9 @@ _SIGS = {
     "gsparc_rasterize_backward": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f, _vp, _vp,
                                        _vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                        _vp]),
+    "gsparc_rasterize_forward": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f, _vp, _vp,
+                                      _vp, _u64, _vp, _vp, _vp, _i, _vp]),
+    "gsparc_instance_count": (_i, [_i, _i, _i, _i, _vp, C.POINTER(_u64), _vp]),
     "gsparc_mark_visible": (_i, [_i, _vp, _vp, _vp, _vp]),
     "gsparc_ssim_forward": (_i, [_i, _i, _i, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gsparc_ssim_backward": (_i, [_i, _i, _i, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -51,10 +51,14 @@ def _check_means(means3D):
 
 
 def forward_batched(means3D, emission_mlps, opacities, scales, rotations, image_height, image_width,
-                    scale_modifier, viewmatrices, tx_pos, rx_pos, emission=None, keep_keys=False):
+                    scale_modifier, viewmatrices, tx_pos, rx_pos, emission=None, keep_keys=False, capacity=None):
     """Forward for R receivers.  viewmatrices [R,4,4] (column-major storage as
     the reference's viewmatrix), rx_pos [R,3], emission None or [R,P,2].
-    Returns (K, out_real [R,H,W], out_imag [R,H,W], radii [R,P], geom, bin, img)."""
+    capacity None: the tile lists are sized by K read back from the device;
+    an int: they hold up to `capacity` instances and nothing synchronises
+    (K stays on the device, see instance_count).
+    Returns (K or capacity, out_real [R,H,W], out_imag [R,H,W], radii [R,P],
+    geom, bin, img)."""
     _check_means(means3D)
     P = int(means3D.size(0))
     H, W = int(image_height), int(image_width)
@@ -87,6 +91,18 @@ def forward_batched(means3D, emission_mlps, opacities, scales, rotations, image_
     img = _empty_bytes(dev, L.gsparc_image_buffer_size(P, R, W, H))
     radii = torch.empty((R, P), dtype=torch.int32, device=dev)
     stream = gh.stream_handle(dev)
+    flags = gh.FLAG_KEEP_KEYS if keep_keys else 0
+    if capacity is not None:
+        Kc = int(capacity)
+        binbuf = _empty_bytes(dev, L.gsparc_binning_buffer_size(Kc))
+        out_re = torch.empty((R, H, W), dtype=torch.float32, device=dev)
+        out_im = torch.empty((R, H, W), dtype=torch.float32, device=dev)
+        gh.check(L.gsparc_rasterize_forward(P, R, W, H, gh.ptr(means), gh.ptr(sc), gh.ptr(rot), gh.ptr(op),
+                                           gh.ptr(mlp), gh.ptr(em), gh.ptr(views), gh.ptr(tx), gh.ptr(rxs),
+                                           float(scale_modifier), gh.ptr(geom), gh.ptr(img), gh.ptr(binbuf), Kc,
+                                           gh.ptr(radii), gh.ptr(out_re), gh.ptr(out_im), flags, stream),
+                 "rasterize forward")
+        return Kc, out_re, out_im, radii, geom, binbuf, img
     K = C.c_uint64(0)
     gh.check(L.gsparc_rasterize_forward_geom(P, R, W, H, gh.ptr(means), gh.ptr(sc), gh.ptr(rot), gh.ptr(op),
                                             gh.ptr(mlp), gh.ptr(views), gh.ptr(tx), gh.ptr(rxs),
@@ -97,10 +113,20 @@ def forward_batched(means3D, emission_mlps, opacities, scales, rotations, image_
     out_re = torch.empty((R, H, W), dtype=torch.float32, device=dev)
     out_im = torch.empty((R, H, W), dtype=torch.float32, device=dev)
     gh.check(L.gsparc_rasterize_forward_render(P, R, W, H, gh.ptr(em), gh.ptr(geom), gh.ptr(img), gh.ptr(binbuf),
-                                              K, gh.ptr(radii), gh.ptr(out_re), gh.ptr(out_im),
-                                              gh.FLAG_KEEP_KEYS if keep_keys else 0, stream),
+                                              K, gh.ptr(radii), gh.ptr(out_re), gh.ptr(out_im), flags, stream),
              "rasterize forward (render)")
     return K, out_re, out_im, radii, geom, binbuf, img
+
+
+def instance_count(P, R, image_height, image_width, imageBuffer):
+    """K (tile-list instances) of the forward that filled imageBuffer;
+    synchronises the current stream.  With a capacity forward, K > capacity
+    means the outputs of that step were not computed."""
+    K = C.c_uint64(0)
+    dev = imageBuffer.device
+    gh.check(gh.lib().gsparc_instance_count(int(P), int(R), int(image_width), int(image_height), gh.ptr(imageBuffer),
+                                            C.byref(K), gh.stream_handle(dev)), "instance count")
+    return int(K.value)
 
 
 def backward_batched(means3D, emission_mlps, radii, opacities, scales, rotations, scale_modifier,

@@ -103,6 +103,24 @@ int gsparc_rasterize_forward_render(int P, int R, int W, int H, const float* emi
                                     void* bin, uint64_t K_capacity, const int* radii, float* out_real,
                                     float* out_imag, int flags, void* stream);
 
+/* The whole forward in one call with no host synchronisation: parts 1 and 2
+ * back to back, the binning sized by K_capacity instead of by K.  K stays on
+ * the device; if it exceeds K_capacity the tile lists are not written, the
+ * blend leaves out_real / out_imag unwritten and the backward does nothing,
+ * which gsparc_instance_count() reveals.  Everything it launches can be
+ * captured in a HIP graph.  No counterpart in the reference (which reads K
+ * back before sizing its binning buffer, rasterizer_impl.cu:270-278); for
+ * callers that cannot synchronise: graph-captured training steps. */
+int gsparc_rasterize_forward(int P, int R, int W, int H, const float* means, const float* scales,
+                             const float* rotations, const float* opacities, const float* emission_mlps,
+                             const float* emission, const float* viewmatrices, const float* tx_pos,
+                             const float* rx_pos, float scale_modifier, void* geom, void* image, void* bin,
+                             uint64_t K_capacity, int* radii_out, float* out_real, float* out_imag, int flags,
+                             void* stream);
+
+/* K of the last forward into `image` (synchronises the stream). */
+int gsparc_instance_count(int P, int R, int W, int H, const void* image, uint64_t* K, void* stream);
+
 /* Backward.  grad_mode: GSPARC_GRAD_EXACT or GSPARC_GRAD_REFERENCE.
  * Inputs as in the forward plus dL/dout_real, dL/dout_imag [R,H,W];
  * K_capacity is the instance capacity `bin` was laid out with.
