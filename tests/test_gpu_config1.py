@@ -1,0 +1,89 @@
+"""Config 1 on the GPU: the rasterizer's emission input and the full
+single-receiver training step (20k Gaussians, shared 68->128->128->1 MLP on
+hipBLASLt FP32 GEMMs, 0.8*L1 + 0.2*(1-SSIM)) against the CPU chain of the
+oracle's FP32 mirror (same discrete decisions as the GPU) and a float64 MLP."""
+import numpy as np
+import pytest
+import torch
+
+import cuda_rasterizer as cr
+import gsparc_emission as GE
+import gsparc_scenes as S
+import oracle as O
+import wireless_rasterizer as wr
+from fused_ssim import fused_ssim
+from helpers import to_dev
+from test_config1 import _gradients, _mlp64
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+@pytest.mark.parametrize("mode", ["exact", "reference"])
+def test_emission_input_parity(mode):
+    sc = S.small_scene(71, 3000, R=3, walls=True)
+    d = to_dev(sc)
+    rng = np.random.default_rng(71)
+    E = rng.uniform(0.0, 1.0, (sc.R, sc.P, 2)).astype(np.float32)
+    Et = torch.from_numpy(E).cuda()
+    K, ore, oim, radii, geom, binbuf, img = wr.forward_batched(
+        d["means"], d["mlp"], d["opac"], d["scales"], d["rots"], sc.H, sc.W, 1.0, d["views"], d["tx"], d["rx"],
+        emission=Et)
+    dre = rng.normal(size=(sc.R, sc.H, sc.W)).astype(np.float32)
+    dim = rng.normal(size=(sc.R, sc.H, sc.W)).astype(np.float32)
+    g = wr.backward_batched(d["means"], None, radii, d["opac"], d["scales"], d["rots"], 1.0, d["views"],
+                            torch.from_numpy(dre).cuda(), torch.from_numpy(dim).cuda(), d["tx"], d["rx"], geom, K,
+                            binbuf, img, grad_mode=mode, emission=Et)
+    torch.cuda.synchronize()
+    m = O.GRAD_EXACT if mode == "exact" else O.GRAD_REFERENCE
+    for r in range(sc.R):
+        pre = O.preprocess(sc, r)
+        pre.sig_re[:] = E[r, :, 0]
+        pre.sig_im[:] = E[r, :, 1]
+        keys, vals, ranges = O.bin_gaussians(pre, sc.W, sc.H)
+        f = O.render_fwd(pre, vals, ranges, sc.W, sc.H)
+        peak = np.abs(f["out_re"]).max()
+        assert np.abs(ore[r].cpu().numpy() - f["out_re"]).max() <= 1e-5 * peak
+        assert np.abs(oim[r].cpu().numpy() - f["out_im"]).max() <= 1e-5 * np.abs(f["out_im"]).max()
+        g8 = O.render_bwd(pre, vals, ranges, f, dre[r], dim[r], sc.W, sc.H, m)
+        dem = g[7][r].cpu().numpy()
+        vis = pre.radii > 0
+        assert _rel(dem[vis, 0], g8[vis, 6]) < 1e-4
+        assert _rel(dem[vis, 1], g8[vis, 7]) < 1e-4
+
+
+def test_config1_training_step():
+    sc = S.config1()
+    tgt = S.config1(seed=2)
+    d = to_dev(sc)
+    dt = to_dev(tgt)
+    mlp64 = _mlp64(3)
+    mlp = GE.SharedEmissionMLP().cuda()
+    with torch.no_grad():
+        for p32, p64 in zip(mlp.parameters(), mlp64.parameters()):
+            p32.copy_(p64.float())
+        Et = mlp(dt["means"], dt["rx"], dt["tx"])
+        target = wr.forward_batched(dt["means"], None, dt["opac"], dt["scales"], dt["rots"], tgt.H, tgt.W, 1.0,
+                                    dt["views"], dt["tx"], dt["rx"], emission=Et)[1].clone()
+    means = d["means"].clone().requires_grad_(True)
+    opac = d["opac"].clone().requires_grad_(True)
+    settings = cr.BatchedSetting(sc.H, sc.W, 1.0, d["views"], d["tx"], d["rx"], "exact")
+    E = mlp(means, d["rx"], d["tx"])
+    ore, oim, radii = cr.rasterize_gaussians_batched(means, None, opac, d["scales"], d["rots"], settings, emission=E)
+    loss = 0.8 * (ore - target).abs().mean() + 0.2 * (1.0 - fused_ssim(ore.unsqueeze(1), target.unsqueeze(1)))
+    loss.backward()
+    torch.cuda.synchronize()
+    ref = _gradients(sc, mlp64, target.cpu().numpy()[None].astype(np.float64), dtype=np.float32)
+    # 1e-4 relative to the largest magnitude of each gradient (the MLP weight
+    # gradients are FP32 GEMM reductions over the 20k rows).
+    errs = {name: _rel(p.grad.cpu().numpy(), ref[name]) for name, p in mlp.named_parameters()}
+    errs["means"] = _rel(means.grad.cpu().numpy(), ref["means"])
+    errs["opacities"] = _rel(opac.grad.cpu().numpy(), ref["opacities"])
+    print("relative errors", errs)
+    for name, e in errs.items():
+        assert e < 1e-4, (name, errs)
