@@ -7,10 +7,33 @@
 
 namespace gs {
 
+// A batch of S scenes (independent Gaussian sets) with R receivers in all.
+// Scene s owns Gaussians [g0_s, g0_s + P_s) of the packed parameter arrays
+// and R_s consecutive receivers; every (receiver, Gaussian-of-its-scene)
+// pair owns one record (splat, emission, depth, radius, sort keys,
+// gradients), receiver r's records starting at roff_r.  Receivers are
+// processed by the preprocess kernels in groups of up to GS_RC of one scene.
+//
+// tab == nullptr is the uniform batch (one scene, every receiver sees all P
+// Gaussians: roff_r = r*P); otherwise tab is the device copy of the scene
+// table (gsparc_scene_table_build):
+//   [0..7]                 S, R, G, Pmax, Ptot, Nrec, 0, 0
+//   receiver r, 4 words    roff, g0, P, scene          at GS_TAB_HDR + 4r
+//   group y, 4 words       r0, nr, scene, part offset  after the receivers
+//   scene s, 8 words       g0, P, r0, R, group0, ngroups, 0, 0  after the groups
+#define GS_RC 8            // receivers per preprocess group
+#define GS_TAB_HDR 8
+
 struct Dims {
-    int P, R, W, H;
-    int gx, gy, T;   // tile grid
-    int NB;          // blocks of 256 Gaussians
+    int P, R, W, H;   // P: Gaussians per receiver (the largest scene's count)
+    int gx, gy, T;    // tile grid
+    int NB;           // blocks of 256 Gaussians
+    int Ptot;         // Gaussians in the packed parameter arrays
+    int G;            // preprocess receiver groups
+    int S;            // scenes
+    size_t Nrec;      // (receiver, Gaussian) records
+    size_t Npart;     // preprocess-backward partial-sum slots (host side)
+    const uint32_t* tab;
 };
 
 // depth sort: 256-thread blocks of 8 elements per thread
@@ -29,23 +52,74 @@ inline Dims make_dims(int P, int R, int W, int H)
     d.P = P; d.R = R; d.W = W; d.H = H;
     d.gx = (W + 15) / 16; d.gy = (H + 15) / 16; d.T = d.gx * d.gy;
     d.NB = (P + 255) / 256;
+    d.Ptot = P;
+    d.G = (R + GS_RC - 1) / GS_RC;
+    d.S = 1;
+    d.Nrec = (size_t)R * P;
+    d.Npart = (size_t)d.G * P;
+    d.tab = nullptr;
     return d;
 }
 
+struct RxMap {
+    uint32_t roff, g0, P, scene;
+};
+struct GroupMap {
+    int r0, nr, scene;
+    uint32_t part;    // offset of the group's per-Gaussian partial sums
+};
+struct SceneMap {
+    uint32_t g0, P, r0, R, group0, ngroups;
+};
+
+__device__ __forceinline__ RxMap rx_map(const Dims& d, int r)
+{
+    if (!d.tab) return {(uint32_t)r * (uint32_t)d.P, 0u, (uint32_t)d.P, 0u};
+    const uint32_t* e = d.tab + GS_TAB_HDR + 4 * r;
+    return {e[0], e[1], e[2], e[3]};
+}
+
+__device__ __forceinline__ GroupMap group_map(const Dims& d, int y)
+{
+    if (!d.tab) return {y * GS_RC, min(GS_RC, d.R - y * GS_RC), 0, (uint32_t)y * (uint32_t)d.P};
+    const uint32_t* e = d.tab + GS_TAB_HDR + 4 * d.R + 4 * y;
+    return {(int)e[0], (int)e[1], (int)e[2], e[3]};
+}
+
+__device__ __forceinline__ SceneMap scene_map(const Dims& d, int s)
+{
+    if (!d.tab) return {0u, (uint32_t)d.P, 0u, (uint32_t)d.R, 0u, (uint32_t)d.G};
+    const uint32_t* e = d.tab + GS_TAB_HDR + 4 * d.R + 4 * d.G + 8 * s;
+    return {e[0], e[1], e[2], e[3], e[4], e[5]};
+}
+
+// scene of packed Gaussian gg (binary search over the scene starts)
+__device__ __forceinline__ int scene_of(const Dims& d, int gg)
+{
+    if (!d.tab) return 0;
+    int lo = 0, hi = d.S - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((int)scene_map(d, mid).g0 <= gg) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 struct PreFwdArgs {
-    const float* means;      // [P,3]
-    const float* scales;     // [P,3]
-    const float* rots;       // [P,4]
-    const float* opac;       // [P]
-    const float* mlp;        // [P*194] or nullptr (emission supplied elsewhere)
+    const float* means;      // [Ptot,3]
+    const float* scales;     // [Ptot,3]
+    const float* rots;       // [Ptot,4]
+    const float* opac;       // [Ptot]
+    const float* mlp;        // [Ptot*194] or nullptr (emission supplied elsewhere)
     const float* views;      // [R,16] column-major
-    const float* tx;         // [3]
+    const float* tx;         // [S,3]
     const float* rx;         // [R,3]
     float mod;
-    float4* splat;           // [R*P*2]
-    float2* sig;             // [R*P] (written only when mlp != nullptr)
-    float* depth;            // [R*P]
-    int* radii;              // [R*P]
+    float4* splat;           // [Nrec*2]
+    float2* sig;             // [Nrec] (written only when mlp != nullptr)
+    float* depth;            // [Nrec]
+    int* radii;              // [Nrec]
     float* cov3;             // [P*6]
 };
 hipError_t launch_preprocess_fwd(const Dims& d, const PreFwdArgs& a, hipStream_t s);
@@ -59,8 +133,8 @@ struct BinArgs {
     uint2* ranges;           // [R*T]
     uint32_t* total;         // [1]: K
     uint32_t* nvis;          // [R] visible Gaussians per receiver
-    uint32_t* skey[2];       // [R*P] depth-sort ping-pong: depth bits
-    uint32_t* sval[2];       // [R*P]                       Gaussian index
+    uint32_t* skey[2];       // [Nrec] depth-sort ping-pong: depth bits
+    uint32_t* sval[2];       // [Nrec]                       Gaussian index
     uint32_t* shist;         // [R*257*rs_blocks(P)] digit histograms
     uint32_t* ccnt;          // [R*T*nchunks] per-chunk tile counts / bases
     int nchunks, chunk_len;
@@ -94,8 +168,8 @@ struct RenderFwdArgs {
     const uint2* ranges;     // [R*T]
     const uint32_t* order;   // work order (see TileOrder; its mode is total[1])
     const uint32_t* plist;
-    const float4* splat;     // [R*P*2]
-    const float2* sig;       // [R*P]
+    const float4* splat;     // [Nrec*2]
+    const float2* sig;       // [Nrec]
     float* out_re;           // [R*N]
     float* out_im;           // [R*N]
     float* finalT;           // [R*N]
@@ -123,7 +197,7 @@ struct RenderBwdArgs {
     const uint32_t* ncontrib;
     const float* dout_re;    // [R*N]
     const float* dout_im;
-    float* g8;               // [R*P*8] accumulated: dpx dpy dqa dqb dqc dop dre dim
+    float* g8;               // [Nrec*8] accumulated: dpx dpy dqa dqb dqc dop dre dim
     int mode;
     const uint32_t* total;
     uint64_t Kcap;
@@ -140,9 +214,9 @@ struct PreBwdArgs {
     const float* tx;
     const float* rx;
     float mod;
-    const int* radii;        // [R*P]
-    const float* g8;         // [R*P*8]
-    const float2* sig;       // [R*P] MLP outputs of the forward (used when mlp != nullptr)
+    const int* radii;        // [Nrec]
+    const float* g8;         // [Nrec*8]
+    const float2* sig;       // [Nrec] MLP outputs of the forward (used when mlp != nullptr)
     const float* cov3;       // [P*6] cov3D of the forward
     float* dmeans;           // [P,3]
     float* dscales;          // [P,3]
@@ -151,9 +225,9 @@ struct PreBwdArgs {
     float* dmlp;             // [P*194] or nullptr
     float* dcov3;            // [P,6] or nullptr
     float* dmeans2d;         // [R*P*3] or nullptr (reference screen-space layout)
-    float* dsig;             // [R*P*2] or nullptr (emission gradient)
+    float* dsig;             // [Nrec*2] or nullptr (emission gradient)
     int mode;
-    float2* dz;              // workspace [R*P]: dL/d(MLP output) per receiver
+    float2* dz;              // workspace [Nrec]: dL/d(MLP output) per receiver
     float* part;             // workspace [ceil(R/8)*P*12]: per-chunk partial sums
 };
 hipError_t launch_preprocess_bwd(const Dims& d, const PreBwdArgs& a, hipStream_t s);

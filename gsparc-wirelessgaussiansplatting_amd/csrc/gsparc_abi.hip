@@ -2,6 +2,7 @@
 // argument checks and the launch sequence of the forward and backward.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -46,20 +47,24 @@ int fail(int code, const std::string& msg)
 constexpr size_t ALIGN = 256;
 inline size_t al(size_t x) { return (x + ALIGN - 1) / ALIGN * ALIGN; }
 
-void make_layout(int P, int R, int W, int H, uint64_t K, gsparc_layout& L)
+// partial-sum slots of the preprocess backward: one per (receiver group,
+// Gaussian of the group's scene)
+size_t part_slots(const gs::Dims& d) { return d.Npart; }
+
+void make_layout(const gs::Dims& d, uint64_t K, gsparc_layout& L)
 {
-    const gs::Dims d = gs::make_dims(P, R, W, H);
-    const size_t RP = (size_t)R * P, N = (size_t)W * H, RT = (size_t)R * d.T;
+    const size_t NR = d.Nrec, N = (size_t)d.W * d.H, RT = (size_t)d.R * d.T;
+    const int R = d.R;
     size_t o = 0;
-    L.splat = o; o = al(o + RP * 2 * sizeof(float4));
-    L.sig = o; o = al(o + RP * sizeof(float2));
-    L.depth = o; o = al(o + RP * sizeof(float));
-    L.cov3 = o; o = al(o + (size_t)P * 6 * sizeof(float));
-    L.sort_key0 = o; o = al(o + RP * sizeof(uint32_t));
-    L.sort_val0 = o; o = al(o + RP * sizeof(uint32_t));
-    L.sort_key1 = o; o = al(o + RP * sizeof(uint32_t));
-    L.sort_val1 = o; o = al(o + RP * sizeof(uint32_t));
-    L.sort_hist = o; o = al(o + (size_t)R * 257 * gs::rs_blocks(P) * sizeof(uint32_t));
+    L.splat = o; o = al(o + NR * 2 * sizeof(float4));
+    L.sig = o; o = al(o + NR * sizeof(float2));
+    L.depth = o; o = al(o + NR * sizeof(float));
+    L.cov3 = o; o = al(o + (size_t)d.Ptot * 6 * sizeof(float));
+    L.sort_key0 = o; o = al(o + NR * sizeof(uint32_t));
+    L.sort_val0 = o; o = al(o + NR * sizeof(uint32_t));
+    L.sort_key1 = o; o = al(o + NR * sizeof(uint32_t));
+    L.sort_val1 = o; o = al(o + NR * sizeof(uint32_t));
+    L.sort_hist = o; o = al(o + (size_t)R * 257 * gs::rs_blocks(d.P) * sizeof(uint32_t));
     L.nvis = o; o = al(o + (size_t)R * sizeof(uint32_t));
     L.geom_bytes = o + ALIGN;
     o = 0;
@@ -77,6 +82,19 @@ void make_layout(int P, int R, int W, int H, uint64_t K, gsparc_layout& L)
     L.plist = o; o = al(o + Kn * 4);
     L.keys_sorted = o; o = al(o + Kn * 4);
     L.bin_bytes = o + ALIGN;
+}
+
+void make_layout(int P, int R, int W, int H, uint64_t K, gsparc_layout& L)
+{
+    make_layout(gs::make_dims(P, R, W, H), K, L);
+}
+
+// grad workspace: g8 [Nrec*8] | dz [Nrec*2] | partials [slots*12]
+void grad_layout(const gs::Dims& d, size_t& dz_off, size_t& part_off, size_t& bytes)
+{
+    dz_off = al(d.Nrec * 8 * sizeof(float));
+    part_off = al(dz_off + d.Nrec * 2 * sizeof(float));
+    bytes = part_off + part_slots(d) * 12 * sizeof(float) + ALIGN;
 }
 
 // binning state shared by the two forward calls
@@ -104,6 +122,14 @@ static gs::BinArgs bin_args(const gs::Dims& d, const gsparc_layout& L, char* gb,
 
 // Buffers from torch are 256-byte aligned already; be safe for others.
 inline char* base_of(void* p) { return (char*)(((uintptr_t)p + ALIGN - 1) / ALIGN * ALIGN); }
+
+int check_map(int R, int W, int H)
+{
+    if (R < 0 || W <= 0 || H <= 0) return fail(GSPARC_ERR_ARG, "negative or zero size");
+    if ((int64_t)R * W * H > (int64_t)1 << 31) return fail(GSPARC_ERR_ARG, "R*H*W too large");
+    if ((W + 15) / 16 + (H + 15) / 16 > 8192) return fail(GSPARC_ERR_ARG, "W + H too large (at most 131072 pixels)");
+    return GSPARC_OK;
+}
 
 int check_dims(int P, int R, int W, int H)
 {
@@ -225,18 +251,10 @@ size_t gsparc_binning_buffer_size(uint64_t K)
     return L.bin_bytes;
 }
 
-// grad workspace: g8 [R*P*8] | dz [R*P*2] | partials [ceil(R/8)*P*12]
-static void grad_layout(int P, int R, size_t& dz_off, size_t& part_off, size_t& bytes)
-{
-    dz_off = al((size_t)P * R * 8 * sizeof(float));
-    part_off = al(dz_off + (size_t)P * R * 2 * sizeof(float));
-    bytes = part_off + (size_t)((R + 7) / 8) * P * 12 * sizeof(float) + ALIGN;
-}
-
 size_t gsparc_grad_buffer_size(int P, int R)
 {
     size_t a, b, c;
-    grad_layout(P, R, a, b, c);
+    grad_layout(gs::make_dims(P, R, 1, 1), a, b, c);
     return c;
 }
 
@@ -247,36 +265,43 @@ int gsparc_layout_query(int P, int R, int W, int H, uint64_t K, gsparc_layout* o
     return GSPARC_OK;
 }
 
-// Forward part 1 on the stream, without reading K back.
-static int forward_geom_async(int P, int R, int W, int H, const float* means, const float* scales,
-                              const float* rotations, const float* opacities, const float* emission_mlps,
-                              const float* viewmatrices, const float* tx_pos, const float* rx_pos,
-                              float scale_modifier, void* geom, void* image, int* radii_out, hipStream_t s)
+}  // extern "C"
+
+// ---------------------------------------------------------------- core
+// The launch sequences, for any batch shape (uniform or scene table).
+namespace {
+
+struct FwdInputs {
+    const float *means, *scales, *rotations, *opacities, *mlp, *views, *tx, *rx;
+    float mod;
+};
+
+int forward_geom_impl(const gs::Dims& d, const FwdInputs& in, void* geom, void* image, int* radii_out,
+                      hipStream_t s)
 {
-    if (int e = check_dims(P, R, W, H)) return e;
-    if (!geom || !image || (P > 0 && R > 0 && (!means || !scales || !rotations || !opacities || !viewmatrices ||
-                                               !tx_pos || !rx_pos || !radii_out)))
+    if (!geom || !image || (d.Ptot > 0 && d.R > 0 && (!in.means || !in.scales || !in.rotations ||
+                                                      !in.opacities || !in.views || !in.tx || !in.rx ||
+                                                      !radii_out)))
         return fail(GSPARC_ERR_ARG, "null pointer");
-    if (((uintptr_t)emission_mlps & 15) != 0) return fail(GSPARC_ERR_ARG, "emission_mlps must be 16-byte aligned");
-    const gs::Dims d = gs::make_dims(P, R, W, H);
+    if (((uintptr_t)in.mlp & 15) != 0) return fail(GSPARC_ERR_ARG, "emission_mlps must be 16-byte aligned");
     gsparc_layout L;
-    make_layout(P, R, W, H, 0, L);
+    make_layout(d, 0, L);
     char* gb = base_of(geom);
     char* ib = base_of(image);
     gs::PreFwdArgs pa;
-    pa.means = means; pa.scales = scales; pa.rots = rotations; pa.opac = opacities; pa.mlp = emission_mlps;
-    pa.views = viewmatrices; pa.tx = tx_pos; pa.rx = rx_pos; pa.mod = scale_modifier;
+    pa.means = in.means; pa.scales = in.scales; pa.rots = in.rotations; pa.opac = in.opacities; pa.mlp = in.mlp;
+    pa.views = in.views; pa.tx = in.tx; pa.rx = in.rx; pa.mod = in.mod;
     pa.splat = (float4*)(gb + L.splat); pa.sig = (float2*)(gb + L.sig); pa.depth = (float*)(gb + L.depth);
     pa.radii = radii_out; pa.cov3 = (float*)(gb + L.cov3);
-    if (P > 0 && R > 0) GS_LAUNCH("preprocess_fwd", gs::launch_preprocess_fwd(d, pa, s), s);
-    if (R == 0) return GSPARC_OK;
+    if (d.P > 0 && d.R > 0) GS_LAUNCH("preprocess_fwd", gs::launch_preprocess_fwd(d, pa, s), s);
+    if (d.R == 0) return GSPARC_OK;
     gs::BinArgs ba = bin_args(d, L, gb, ib, radii_out);
-    if (P > 0) {
+    if (d.P > 0) {
         GS_LAUNCH("bin_depth_sort", gs::launch_depth_sort(d, ba, s), s);
         GS_LAUNCH("bin_chunk_count", gs::launch_chunk_count(d, ba, s), s);
         GS_LAUNCH("bin_chunk_scan", gs::launch_chunk_scan(d, ba, s), s);
     } else {
-        GS_LAUNCH("clear_counts", gs::launch_fill_u32(ba.segcnt, 0u, (size_t)R * d.T, s), s);
+        GS_LAUNCH("clear_counts", gs::launch_fill_u32(ba.segcnt, 0u, (size_t)d.R * d.T, s), s);
     }
     GS_LAUNCH("bin_segscan", gs::launch_segscan(d, ba, s), s);
     GS_LAUNCH("bin_order", gs::launch_tile_order(d, choose_order(d), ba.ranges, (uint32_t*)(ib + L.tile_order),
@@ -284,10 +309,12 @@ static int forward_geom_async(int P, int R, int W, int H, const float* means, co
     return GSPARC_OK;
 }
 
-static int read_count(int P, int R, int W, int H, const void* image, uint64_t* K, hipStream_t s)
+int read_count_impl(const gs::Dims& d, const void* image, uint64_t* K, hipStream_t s)
 {
+    *K = 0;
+    if (d.R == 0) return GSPARC_OK;
     gsparc_layout L;
-    make_layout(P, R, W, H, 0, L);
+    make_layout(d, 0, L);
     uint32_t tot = 0;
     GS_HIP(hipMemcpyAsync(&tot, base_of((void*)image) + L.total, sizeof(tot), hipMemcpyDeviceToHost, s));
     GS_HIP(hipStreamSynchronize(s));
@@ -295,43 +322,15 @@ static int read_count(int P, int R, int W, int H, const void* image, uint64_t* K
     return GSPARC_OK;
 }
 
-int gsparc_rasterize_forward_geom(int P, int R, int W, int H, const float* means, const float* scales,
-                                  const float* rotations, const float* opacities, const float* emission_mlps,
-                                  const float* viewmatrices, const float* tx_pos, const float* rx_pos,
-                                  float scale_modifier, void* geom, void* image, int* radii_out,
-                                  uint64_t* K_out, void* stream)
+int forward_render_impl(const gs::Dims& d, const float* emission, void* geom, void* image, void* bin,
+                        uint64_t K_capacity, const int* radii, float* out_real, float* out_imag, int flags,
+                        hipStream_t s)
 {
-    if (!K_out) return fail(GSPARC_ERR_ARG, "null pointer");
-    hipStream_t s = (hipStream_t)stream;
-    if (int e = forward_geom_async(P, R, W, H, means, scales, rotations, opacities, emission_mlps, viewmatrices,
-                                   tx_pos, rx_pos, scale_modifier, geom, image, radii_out, s))
-        return e;
-    *K_out = 0;
-    if (R == 0) return GSPARC_OK;
-    return read_count(P, R, W, H, image, K_out, s);
-}
-
-int gsparc_instance_count(int P, int R, int W, int H, const void* image, uint64_t* K, void* stream)
-{
-    if (int e = check_dims(P, R, W, H)) return e;
-    if (!image || !K) return fail(GSPARC_ERR_ARG, "null pointer");
-    *K = 0;
-    if (R == 0) return GSPARC_OK;
-    return read_count(P, R, W, H, image, K, (hipStream_t)stream);
-}
-
-int gsparc_rasterize_forward_render(int P, int R, int W, int H, const float* emission, void* geom, void* image,
-                                    void* bin, uint64_t K_capacity, const int* radii, float* out_real,
-                                    float* out_imag, int flags, void* stream)
-{
-    if (int e = check_dims(P, R, W, H)) return e;
-    if (!geom || !image || !bin || (R > 0 && (!out_real || !out_imag || (P > 0 && !radii))))
+    if (!geom || !image || !bin || (d.R > 0 && (!out_real || !out_imag || (d.P > 0 && !radii))))
         return fail(GSPARC_ERR_ARG, "null pointer");
-    if (R == 0) return GSPARC_OK;
-    hipStream_t s = (hipStream_t)stream;
-    const gs::Dims d = gs::make_dims(P, R, W, H);
+    if (d.R == 0) return GSPARC_OK;
     gsparc_layout L;
-    make_layout(P, R, W, H, K_capacity, L);
+    make_layout(d, K_capacity, L);
     char* gb = base_of(geom);
     char* ib = base_of(image);
     char* bb = base_of(bin);
@@ -339,7 +338,7 @@ int gsparc_rasterize_forward_render(int P, int R, int W, int H, const float* emi
     ba.plist = (uint32_t*)(bb + L.plist);
     ba.keys_sorted = (flags & GSPARC_FLAG_KEEP_KEYS) ? (uint32_t*)(bb + L.keys_sorted) : nullptr;
     ba.Kcap = K_capacity;
-    if (P > 0) GS_LAUNCH("bin_scatter", gs::launch_chunk_scatter(d, ba, s), s);
+    if (d.P > 0) GS_LAUNCH("bin_scatter", gs::launch_chunk_scatter(d, ba, s), s);
     gs::RenderFwdArgs ra;
     ra.ranges = ba.ranges;
     ra.order = (const uint32_t*)(ib + L.tile_order);
@@ -357,45 +356,28 @@ int gsparc_rasterize_forward_render(int P, int R, int W, int H, const float* emi
     return GSPARC_OK;
 }
 
-int gsparc_rasterize_forward(int P, int R, int W, int H, const float* means, const float* scales,
-                             const float* rotations, const float* opacities, const float* emission_mlps,
-                             const float* emission, const float* viewmatrices, const float* tx_pos,
-                             const float* rx_pos, float scale_modifier, void* geom, void* image, void* bin,
-                             uint64_t K_capacity, int* radii_out, float* out_real, float* out_imag, int flags,
-                             void* stream)
-{
-    hipStream_t s = (hipStream_t)stream;
-    if (int e = forward_geom_async(P, R, W, H, means, scales, rotations, opacities, emission_mlps, viewmatrices,
-                                   tx_pos, rx_pos, scale_modifier, geom, image, radii_out, s))
-        return e;
-    return gsparc_rasterize_forward_render(P, R, W, H, emission, geom, image, bin, K_capacity, radii_out, out_real,
-                                           out_imag, flags, stream);
-}
+struct BwdOutputs {
+    float *dmeans2D, *dmeans3D, *dmlp, *dopacity, *dcov3D, *dscales, *drotations, *demission;
+};
 
-int gsparc_rasterize_backward(int grad_mode, int P, int R, int W, int H, const float* means, const float* scales,
-                              const float* rotations, const float* emission_mlps, const float* emission,
-                              const float* viewmatrices, const float* tx_pos, const float* rx_pos,
-                              float scale_modifier, const int* radii, void* geom, void* image, void* bin,
-                              uint64_t K_capacity, void* grad, const float* dout_real, const float* dout_imag, float* dmeans2D,
-                              float* dmeans3D, float* demission_mlps, float* dopacity, float* dcov3D,
-                              float* dscales, float* drotations, float* demission, void* stream)
+int backward_impl(int grad_mode, const gs::Dims& d, const FwdInputs& in, const float* emission, const int* radii,
+                  void* geom, void* image, void* bin, uint64_t K_capacity, void* grad, const float* dout_real,
+                  const float* dout_imag, const BwdOutputs& out, hipStream_t s)
 {
-    if (int e = check_dims(P, R, W, H)) return e;
     if (grad_mode != GSPARC_GRAD_EXACT && grad_mode != GSPARC_GRAD_REFERENCE)
         return fail(GSPARC_ERR_ARG, "unknown grad_mode");
-    if (P == 0) return GSPARC_OK;
-    if (!geom || !image || !bin || !grad || !means || !scales || !rotations || !viewmatrices || !tx_pos ||
-        !rx_pos || !radii || !dout_real || !dout_imag || !dmeans3D || !dopacity || !dscales || !drotations)
+    if (d.P == 0 || d.R == 0) return GSPARC_OK;
+    if (!geom || !image || !bin || !grad || !in.means || !in.scales || !in.rotations || !in.views || !in.tx ||
+        !in.rx || !radii || !dout_real || !dout_imag || !out.dmeans3D || !out.dopacity || !out.dscales ||
+        !out.drotations)
         return fail(GSPARC_ERR_ARG, "null pointer");
-    hipStream_t s = (hipStream_t)stream;
-    const gs::Dims d = gs::make_dims(P, R, W, H);
     gsparc_layout L;
-    make_layout(P, R, W, H, K_capacity, L);
+    make_layout(d, K_capacity, L);
     char* gb = base_of(geom);
     char* ib = base_of(image);
     char* bb = base_of(bin);
     float* g8 = (float*)base_of(grad);
-    GS_LAUNCH("grad_clear", gs::launch_fill_u32((uint32_t*)g8, 0u, (size_t)R * P * 8, s), s);
+    GS_LAUNCH("grad_clear", gs::launch_fill_u32((uint32_t*)g8, 0u, d.Nrec * 8, s), s);
     gs::RenderBwdArgs ra;
     ra.ranges = (const uint2*)(ib + L.ranges);
     ra.order = (const uint32_t*)(ib + L.tile_order);
@@ -413,19 +395,248 @@ int gsparc_rasterize_backward(int grad_mode, int P, int R, int W, int H, const f
     ra.counts = g_prof_on ? g_counts_dev : nullptr;
     GS_LAUNCH("render_bwd", gs::launch_render_bwd(d, ra, gs::choose_ppl(d, g_opt_bwd_ppl), s), s);
     gs::PreBwdArgs pa;
-    pa.means = means; pa.scales = scales; pa.rots = rotations; pa.mlp = emission_mlps; pa.views = viewmatrices;
+    pa.means = in.means; pa.scales = in.scales; pa.rots = in.rotations; pa.mlp = in.mlp; pa.views = in.views;
     pa.sig = (const float2*)(gb + L.sig); pa.cov3 = (const float*)(gb + L.cov3);
-    pa.tx = tx_pos; pa.rx = rx_pos; pa.mod = scale_modifier; pa.radii = radii; pa.g8 = g8;
-    pa.dmeans = dmeans3D; pa.dscales = dscales; pa.drots = drotations; pa.dopac = dopacity;
-    pa.dmlp = demission_mlps; pa.dcov3 = dcov3D; pa.dmeans2d = dmeans2D; pa.dsig = demission; pa.mode = grad_mode;
+    pa.tx = in.tx; pa.rx = in.rx; pa.mod = in.mod; pa.radii = radii; pa.g8 = g8;
+    pa.dmeans = out.dmeans3D; pa.dscales = out.dscales; pa.drots = out.drotations; pa.dopac = out.dopacity;
+    pa.dmlp = out.dmlp; pa.dcov3 = out.dcov3D; pa.dmeans2d = out.dmeans2D; pa.dsig = out.demission;
+    pa.mode = grad_mode;
     {
         size_t dz_off, part_off, bytes;
-        grad_layout(P, R, dz_off, part_off, bytes);
+        grad_layout(d, dz_off, part_off, bytes);
         pa.dz = (float2*)((char*)g8 + dz_off);
         pa.part = (float*)((char*)g8 + part_off);
     }
     GS_LAUNCH("preprocess_bwd", gs::launch_preprocess_bwd(d, pa, s), s);
     return GSPARC_OK;
+}
+
+// Scene-table batch: the Dims of a batch built by gsparc_scene_table_build.
+int scene_dims(const gsparc_batch* b, const uint32_t* table, int W, int H, gs::Dims& d)
+{
+    if (!b || !table) return fail(GSPARC_ERR_ARG, "null batch or scene table");
+    if (b->S < 1 || b->R < 0 || b->G < 0 || b->Pmax < 0 || b->Ptot < 0) return fail(GSPARC_ERR_ARG, "bad batch");
+    if (b->Nrec >= ((uint64_t)1 << 31)) return fail(GSPARC_ERR_ARG, "too many (receiver, Gaussian) records");
+    if (int e = check_map(b->R, W, H)) return e;
+    d = gs::make_dims(b->Pmax, b->R, W, H);
+    d.Ptot = b->Ptot;
+    d.G = b->G;
+    d.S = b->S;
+    d.Nrec = (size_t)b->Nrec;
+    d.Npart = (size_t)b->Npart;
+    d.tab = table;
+    return GSPARC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gsparc_rasterize_forward_geom(int P, int R, int W, int H, const float* means, const float* scales,
+                                  const float* rotations, const float* opacities, const float* emission_mlps,
+                                  const float* viewmatrices, const float* tx_pos, const float* rx_pos,
+                                  float scale_modifier, void* geom, void* image, int* radii_out,
+                                  uint64_t* K_out, void* stream)
+{
+    if (int e = check_dims(P, R, W, H)) return e;
+    if (!K_out) return fail(GSPARC_ERR_ARG, "null pointer");
+    const gs::Dims d = gs::make_dims(P, R, W, H);
+    const FwdInputs in{means, scales, rotations, opacities, emission_mlps, viewmatrices, tx_pos, rx_pos,
+                       scale_modifier};
+    hipStream_t s = (hipStream_t)stream;
+    if (int e = forward_geom_impl(d, in, geom, image, radii_out, s)) return e;
+    return read_count_impl(d, image, K_out, s);
+}
+
+int gsparc_instance_count(int P, int R, int W, int H, const void* image, uint64_t* K, void* stream)
+{
+    if (int e = check_dims(P, R, W, H)) return e;
+    if (!image || !K) return fail(GSPARC_ERR_ARG, "null pointer");
+    return read_count_impl(gs::make_dims(P, R, W, H), image, K, (hipStream_t)stream);
+}
+
+int gsparc_rasterize_forward_render(int P, int R, int W, int H, const float* emission, void* geom, void* image,
+                                    void* bin, uint64_t K_capacity, const int* radii, float* out_real,
+                                    float* out_imag, int flags, void* stream)
+{
+    if (int e = check_dims(P, R, W, H)) return e;
+    return forward_render_impl(gs::make_dims(P, R, W, H), emission, geom, image, bin, K_capacity, radii, out_real,
+                               out_imag, flags, (hipStream_t)stream);
+}
+
+int gsparc_rasterize_forward(int P, int R, int W, int H, const float* means, const float* scales,
+                             const float* rotations, const float* opacities, const float* emission_mlps,
+                             const float* emission, const float* viewmatrices, const float* tx_pos,
+                             const float* rx_pos, float scale_modifier, void* geom, void* image, void* bin,
+                             uint64_t K_capacity, int* radii_out, float* out_real, float* out_imag, int flags,
+                             void* stream)
+{
+    if (int e = check_dims(P, R, W, H)) return e;
+    const gs::Dims d = gs::make_dims(P, R, W, H);
+    const FwdInputs in{means, scales, rotations, opacities, emission_mlps, viewmatrices, tx_pos, rx_pos,
+                       scale_modifier};
+    hipStream_t s = (hipStream_t)stream;
+    if (int e = forward_geom_impl(d, in, geom, image, radii_out, s)) return e;
+    return forward_render_impl(d, emission, geom, image, bin, K_capacity, radii_out, out_real, out_imag, flags, s);
+}
+
+int gsparc_rasterize_backward(int grad_mode, int P, int R, int W, int H, const float* means, const float* scales,
+                              const float* rotations, const float* emission_mlps, const float* emission,
+                              const float* viewmatrices, const float* tx_pos, const float* rx_pos,
+                              float scale_modifier, const int* radii, void* geom, void* image, void* bin,
+                              uint64_t K_capacity, void* grad, const float* dout_real, const float* dout_imag, float* dmeans2D,
+                              float* dmeans3D, float* demission_mlps, float* dopacity, float* dcov3D,
+                              float* dscales, float* drotations, float* demission, void* stream)
+{
+    if (int e = check_dims(P, R, W, H)) return e;
+    const FwdInputs in{means, scales, rotations, nullptr, emission_mlps, viewmatrices, tx_pos, rx_pos,
+                       scale_modifier};
+    const BwdOutputs out{dmeans2D, dmeans3D, demission_mlps, dopacity, dcov3D, dscales, drotations, demission};
+    return backward_impl(grad_mode, gs::make_dims(P, R, W, H), in, emission, radii, geom, image, bin, K_capacity,
+                         grad, dout_real, dout_imag, out, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------- scene batches
+size_t gsparc_scene_table_words(int S, int R)
+{
+    return (size_t)GS_TAB_HDR + 8 * (size_t)(R > 0 ? R : 0) + 8 * (size_t)(S > 0 ? S : 0);
+}
+
+int gsparc_scene_table_build(int S, const int* scene_P, const int* scene_R, uint32_t* table, size_t table_words,
+                             gsparc_batch* batch)
+{
+    if (S < 1 || !scene_P || !scene_R || !table || !batch) return fail(GSPARC_ERR_ARG, "bad scene description");
+    int64_t R = 0, Ptot = 0, G = 0, Pmax = 0, Nrec = 0, Npart = 0;
+    for (int s = 0; s < S; ++s) {
+        if (scene_P[s] < 0 || scene_R[s] < 0) return fail(GSPARC_ERR_ARG, "negative scene size");
+        R += scene_R[s];
+        Ptot += scene_P[s];
+        G += (scene_R[s] + GS_RC - 1) / GS_RC;
+        Pmax = std::max<int64_t>(Pmax, scene_P[s]);
+        Nrec += (int64_t)scene_R[s] * scene_P[s];
+        Npart += (int64_t)((scene_R[s] + GS_RC - 1) / GS_RC) * scene_P[s];
+    }
+    if (Nrec >= ((int64_t)1 << 31) || Ptot >= ((int64_t)1 << 31) || Npart >= ((int64_t)1 << 32))
+        return fail(GSPARC_ERR_ARG, "batch too large");
+    if (table_words < gsparc_scene_table_words(S, (int)R)) return fail(GSPARC_ERR_ARG, "scene table too small");
+    uint32_t* hdr = table;
+    hdr[0] = (uint32_t)S; hdr[1] = (uint32_t)R; hdr[2] = (uint32_t)G; hdr[3] = (uint32_t)Pmax;
+    hdr[4] = (uint32_t)Ptot; hdr[5] = (uint32_t)Nrec; hdr[6] = (uint32_t)Npart; hdr[7] = 0;
+    uint32_t* rx = table + GS_TAB_HDR;
+    uint32_t* gr = rx + 4 * R;
+    uint32_t* sc = gr + 4 * G;
+    uint32_t roff = 0, g0 = 0, r0 = 0, y = 0, part = 0;
+    for (int s = 0; s < S; ++s) {
+        const uint32_t P = (uint32_t)scene_P[s], Rs = (uint32_t)scene_R[s];
+        const uint32_t ng = (Rs + GS_RC - 1) / GS_RC;
+        uint32_t* e = sc + 8 * s;
+        e[0] = g0; e[1] = P; e[2] = r0; e[3] = Rs; e[4] = y; e[5] = ng; e[6] = 0; e[7] = 0;
+        for (uint32_t i = 0; i < Rs; ++i) {
+            uint32_t* f = rx + 4 * (r0 + i);
+            f[0] = roff; f[1] = g0; f[2] = P; f[3] = (uint32_t)s;
+            roff += P;
+        }
+        for (uint32_t k = 0; k < ng; ++k) {
+            uint32_t* f = gr + 4 * (y + k);
+            f[0] = r0 + k * GS_RC; f[1] = std::min<uint32_t>(GS_RC, Rs - k * GS_RC); f[2] = (uint32_t)s; f[3] = part;
+            part += P;
+        }
+        g0 += P;
+        r0 += Rs;
+        y += ng;
+    }
+    batch->S = S; batch->R = (int)R; batch->G = (int)G; batch->Pmax = (int)Pmax; batch->Ptot = (int)Ptot;
+    batch->Nrec = (uint64_t)Nrec;
+    batch->Npart = (uint64_t)Npart;
+    return GSPARC_OK;
+}
+
+int gsparc_scene_buffer_sizes(const gsparc_batch* batch, int W, int H, uint64_t K, size_t* sizes)
+{
+    if (!batch || !sizes) return fail(GSPARC_ERR_ARG, "null pointer");
+    gs::Dims d = gs::make_dims(batch->Pmax, batch->R, W, H);
+    d.Ptot = batch->Ptot; d.G = batch->G; d.S = batch->S; d.Nrec = (size_t)batch->Nrec;
+    d.Npart = (size_t)batch->Npart;
+    gsparc_layout L;
+    make_layout(d, K, L);
+    size_t a, b, c;
+    grad_layout(d, a, b, c);
+    sizes[0] = L.geom_bytes;
+    sizes[1] = L.image_bytes;
+    sizes[2] = L.bin_bytes;
+    sizes[3] = c;
+    return GSPARC_OK;
+}
+
+int gsparc_rasterize_forward_scenes(const gsparc_batch* batch, const uint32_t* scene_table, int W, int H,
+                                    const float* means, const float* scales, const float* rotations,
+                                    const float* opacities, const float* emission_mlps, const float* emission,
+                                    const float* viewmatrices, const float* tx_pos, const float* rx_pos,
+                                    float scale_modifier, void* geom, void* image, void* bin, uint64_t K_capacity,
+                                    int* radii_out, float* out_real, float* out_imag, int flags, void* stream)
+{
+    gs::Dims d;
+    if (int e = scene_dims(batch, scene_table, W, H, d)) return e;
+    const FwdInputs in{means, scales, rotations, opacities, emission_mlps, viewmatrices, tx_pos, rx_pos,
+                       scale_modifier};
+    hipStream_t s = (hipStream_t)stream;
+    if (int e = forward_geom_impl(d, in, geom, image, radii_out, s)) return e;
+    return forward_render_impl(d, emission, geom, image, bin, K_capacity, radii_out, out_real, out_imag, flags, s);
+}
+
+int gsparc_rasterize_forward_geom_scenes(const gsparc_batch* batch, const uint32_t* scene_table, int W, int H,
+                                         const float* means, const float* scales, const float* rotations,
+                                         const float* opacities, const float* emission_mlps,
+                                         const float* viewmatrices, const float* tx_pos, const float* rx_pos,
+                                         float scale_modifier, void* geom, void* image, int* radii_out,
+                                         uint64_t* K_out, void* stream)
+{
+    gs::Dims d;
+    if (int e = scene_dims(batch, scene_table, W, H, d)) return e;
+    if (!K_out) return fail(GSPARC_ERR_ARG, "null pointer");
+    const FwdInputs in{means, scales, rotations, opacities, emission_mlps, viewmatrices, tx_pos, rx_pos,
+                       scale_modifier};
+    hipStream_t s = (hipStream_t)stream;
+    if (int e = forward_geom_impl(d, in, geom, image, radii_out, s)) return e;
+    return read_count_impl(d, image, K_out, s);
+}
+
+int gsparc_rasterize_forward_render_scenes(const gsparc_batch* batch, const uint32_t* scene_table, int W, int H,
+                                           const float* emission, void* geom, void* image, void* bin,
+                                           uint64_t K_capacity, const int* radii, float* out_real,
+                                           float* out_imag, int flags, void* stream)
+{
+    gs::Dims d;
+    if (int e = scene_dims(batch, scene_table, W, H, d)) return e;
+    return forward_render_impl(d, emission, geom, image, bin, K_capacity, radii, out_real, out_imag, flags,
+                               (hipStream_t)stream);
+}
+
+int gsparc_instance_count_scenes(const gsparc_batch* batch, const uint32_t* scene_table, int W, int H,
+                                 const void* image, uint64_t* K, void* stream)
+{
+    gs::Dims d;
+    if (int e = scene_dims(batch, scene_table, W, H, d)) return e;
+    if (!image || !K) return fail(GSPARC_ERR_ARG, "null pointer");
+    return read_count_impl(d, image, K, (hipStream_t)stream);
+}
+
+int gsparc_rasterize_backward_scenes(int grad_mode, const gsparc_batch* batch, const uint32_t* scene_table, int W,
+                                     int H, const float* means, const float* scales, const float* rotations,
+                                     const float* emission_mlps, const float* emission, const float* viewmatrices,
+                                     const float* tx_pos, const float* rx_pos, float scale_modifier,
+                                     const int* radii, void* geom, void* image, void* bin, uint64_t K_capacity,
+                                     void* grad, const float* dout_real, const float* dout_imag, float* dmeans2D,
+                                     float* dmeans3D, float* demission_mlps, float* dopacity, float* dcov3D,
+                                     float* dscales, float* drotations, float* demission, void* stream)
+{
+    gs::Dims d;
+    if (int e = scene_dims(batch, scene_table, W, H, d)) return e;
+    const FwdInputs in{means, scales, rotations, nullptr, emission_mlps, viewmatrices, tx_pos, rx_pos,
+                       scale_modifier};
+    const BwdOutputs out{dmeans2D, dmeans3D, demission_mlps, dopacity, dcov3D, dscales, drotations, demission};
+    return backward_impl(grad_mode, d, in, emission, radii, geom, image, bin, K_capacity, grad, dout_real,
+                         dout_imag, out, (hipStream_t)stream);
 }
 
 int gsparc_mark_visible(int P, const float* means, const float* viewmatrix, uint8_t* present, void* stream)

@@ -115,8 +115,9 @@ __global__ void __launch_bounds__(GS_RS_THREADS) k_rs_hist(Dims d, BinArgs a, in
     constexpr int NB = FIRST ? 257 : 256;
     __shared__ uint32_t h[NB];
     const int r = blockIdx.y, blk = blockIdx.x, nblk = gridDim.x;
-    const int n = FIRST ? d.P : (int)a.nvis[r];
-    const size_t rP = (size_t)r * d.P;
+    const RxMap rm = rx_map(d, r);
+    const int n = FIRST ? (int)rm.P : (int)a.nvis[r];
+    const size_t rP = rm.roff;
     for (int i = threadIdx.x; i < NB; i += GS_RS_THREADS) h[i] = 0;
     __syncthreads();
     if (blk * GS_RS_TILE < n) {
@@ -175,9 +176,10 @@ __global__ void __launch_bounds__(GS_RS_THREADS) k_rs_scatter(Dims d, BinArgs a,
     __shared__ uint32_t run[NB];
     __shared__ uint32_t wcnt[NW][NB];
     const int r = blockIdx.y, blk = blockIdx.x, nblk = gridDim.x;
-    const int n = FIRST ? d.P : (int)a.nvis[r];
+    const RxMap rm = rx_map(d, r);
+    const int n = FIRST ? (int)rm.P : (int)a.nvis[r];
     if (blk * GS_RS_TILE >= n) return;
-    const size_t rP = (size_t)r * d.P;
+    const size_t rP = rm.roff;
     const int wave = threadIdx.x / GS_WAVE;
     for (int i = threadIdx.x; i < NB; i += GS_RS_THREADS) {
         run[i] = a.shist[((size_t)r * 257 + i) * nblk + blk];
@@ -242,7 +244,7 @@ __global__ void __launch_bounds__(GS_RS_THREADS) k_chunk_count(Dims d, BinArgs a
     const bool lds = chunk_lds(d);
     const int n = (int)a.nvis[r];
     const int i0 = c * a.chunk_len, i1 = min(n, i0 + a.chunk_len);
-    const size_t rP = (size_t)r * d.P;
+    const size_t rP = rx_map(d, r).roff;
     uint32_t* slot = a.ccnt + (size_t)r * d.T * a.nchunks + c;   // tile t at slot[t * nchunks]
     for (int t = threadIdx.x; t < d.T; t += GS_RS_THREADS) {
         if (lds) s_cnt[t] = 0;
@@ -314,7 +316,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_chunk_scatter(Dims d, BinArgs a)
     const int i1 = min(n, i0 + a.chunk_len);
     const bool lds = chunk_lds(d);
     const int lane = threadIdx.x;
-    const size_t rP = (size_t)r * d.P;
+    const size_t rP = rx_map(d, r).roff;
     const size_t rT = (size_t)r * d.T;
     uint32_t* slot = a.ccnt + rT * a.nchunks + c;
     for (int t = lane; t < d.T; t += GS_WAVE) {

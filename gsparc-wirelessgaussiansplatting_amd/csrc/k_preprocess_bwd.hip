@@ -18,7 +18,6 @@
 namespace gs {
 
 #define GS_BWD_THREADS 64
-#define GS_BRC 8
 #define GS_PART 12   // per (receiver chunk, Gaussian): dmean[3], dcov3[6], dopacity, db2[2]
 
 template <int MODE>
@@ -158,36 +157,37 @@ __device__ __forceinline__ void geo_bwd(const float* mean, const float* c3, cons
     for (int k = 0; k < 3; ++k) dmean[k] += view[k * 4 + 0] * dt[0] + view[k * 4 + 1] * dt[1] + view[k * 4 + 2] * dt[2];
 }
 
-// Kernel 1: one thread per (Gaussian, chunk of GS_BRC receivers).  Recomputes
-// the MLP outputs of the chunk (weights streamed once per chunk), runs the
-// geometry adjoint per receiver and writes the chunk's partial sums
-// (dmean[3], dcov3[6], dopacity, db2[2]) plus dz[r] = dL/d(MLP output) for the
-// weight-gradient kernel.
+// Kernel 1: one thread per (Gaussian, receiver group of its scene).  Runs the
+// geometry adjoint per receiver (with the MLP outputs and cov3D the forward
+// stored) and writes the group's partial sums (dmean[3], dcov3[6], dopacity,
+// db2[2]) plus dz[r] = dL/d(MLP output) for the weight-gradient kernel.
 template <int MODE>
 __global__ void __launch_bounds__(GS_BWD_THREADS) k_pre_bwd_geo(Dims d, PreBwdArgs a)
 {
-    const int g = blockIdx.x * GS_BWD_THREADS + threadIdx.x;
-    if (g >= d.P) return;   // no block-level synchronisation below
-    const int P = d.P;
-    const int chunk = blockIdx.y;
-    const int r0 = chunk * GS_BRC;
-    const int nr = min(GS_BRC, d.R - r0);
-    const float mean[3] = {a.means[3 * g], a.means[3 * g + 1], a.means[3 * g + 2]};
+    const GroupMap grp = group_map(d, blockIdx.y);
+    const SceneMap sc = scene_map(d, grp.scene);
+    const int g = blockIdx.x * GS_BWD_THREADS + threadIdx.x;   // within the scene
+    if (g >= (int)sc.P) return;   // no block-level synchronisation below
+    const int gg = (int)sc.g0 + g;
+    const int r0 = grp.r0;
+    const int nr = grp.nr;
+    const float mean[3] = {a.means[3 * gg], a.means[3 * gg + 1], a.means[3 * gg + 2]};
     // cov3D as the forward stored it
     float c3[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) c3[k] = a.cov3[6 * g + k];
-    const float diff[3] = {a.tx[0] - mean[0], a.tx[1] - mean[1], a.tx[2] - mean[2]};
+    for (int k = 0; k < 6; ++k) c3[k] = a.cov3[6 * gg + k];
+    const float* tx = a.tx + 3 * grp.scene;
+    const float diff[3] = {tx[0] - mean[0], tx[1] - mean[1], tx[2] - mean[2]};
     const float dist = sqrtf(diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2]);
     const float de = dist + 1e-6f;
     const float den = (MODE == GRAD_REFERENCE) ? de * de : dist * de;
 
     float dmean[3] = {0, 0, 0}, dc[6] = {0, 0, 0, 0, 0, 0}, dop = 0.0f, db2_0 = 0.0f, db2_1 = 0.0f;
 #pragma unroll
-    for (int c = 0; c < GS_BRC; ++c) {
+    for (int c = 0; c < GS_RC; ++c) {
         if (c >= nr) break;
         const int r = r0 + c;
-        const size_t ri = (size_t)r * P + g;
+        const size_t ri = (size_t)rx_map(d, r).roff + g;
         float2 dz = make_float2(0.0f, 0.0f);
         if (a.radii[ri] > 0) {
             const float4 ga = reinterpret_cast<const float4*>(a.g8)[2 * ri];
@@ -228,23 +228,26 @@ __global__ void __launch_bounds__(GS_BWD_THREADS) k_pre_bwd_geo(Dims d, PreBwdAr
         }
         if (a.mlp) a.dz[ri] = dz;
     }
-    float* pt = a.part + ((size_t)chunk * P + g) * GS_PART;
+    float* pt = a.part + ((size_t)grp.part + g) * GS_PART;
     pt[0] = dmean[0]; pt[1] = dmean[1]; pt[2] = dmean[2];
 #pragma unroll
     for (int k = 0; k < 6; ++k) pt[3 + k] = dc[k];
     pt[9] = dop; pt[10] = db2_0; pt[11] = db2_1;
 }
 
-// Kernel 2: 32 threads per Gaussian, one per hidden unit j of its MLP: the six
-// weight gradients of unit j summed over all receivers (backward.cu:95-115).
-// Thread j == 0 also sums the chunk partials and runs the cov3D adjoint
-// (backward.cu:308-371), which is linear in dL/dcov3D and so done once.
-__global__ void __launch_bounds__(256) k_pre_bwd_param(Dims d, PreBwdArgs a, int nchunks)
+// Kernel 2: 32 threads per (packed) Gaussian, one per hidden unit j of its
+// MLP: the six weight gradients of unit j summed over the receivers of its
+// scene (backward.cu:95-115).  Thread j == 0 also sums the group partials and
+// runs the cov3D adjoint (backward.cu:308-371), which is linear in dL/dcov3D
+// and so done once.
+__global__ void __launch_bounds__(256) k_pre_bwd_param(Dims d, PreBwdArgs a)
 {
     const int t = blockIdx.x * 256 + threadIdx.x;
-    const int g = t >> 5, j = t & 31;
-    if (g >= d.P) return;
-    const int P = d.P;
+    const int g = t >> 5, j = t & 31;   // g: packed index
+    if (g >= d.Ptot) return;
+    const int P = d.Ptot;               // MLP section stride
+    const SceneMap sc = scene_map(d, scene_of(d, g));
+    const int gl = g - (int)sc.g0;      // index within the scene
     if (a.mlp && a.dmlp) {
         const float* W1 = a.mlp + (size_t)g * 96;
         const float* b1 = a.mlp + (size_t)P * 96 + (size_t)g * 32;
@@ -252,8 +255,8 @@ __global__ void __launch_bounds__(256) k_pre_bwd_param(Dims d, PreBwdArgs a, int
         const float w0 = W1[j * 3], w1 = W1[j * 3 + 1], w2 = W1[j * 3 + 2], bb = b1[j];
         const float v0 = W2[j], v1 = W2[32 + j];
         float gw0 = 0.0f, gw1 = 0.0f, gw2 = 0.0f, gb = 0.0f, gv0 = 0.0f, gv1 = 0.0f;
-        for (int r = 0; r < d.R; ++r) {
-            const float2 dz = a.dz[(size_t)r * P + g];
+        for (int r = (int)sc.r0; r < (int)(sc.r0 + sc.R); ++r) {
+            const float2 dz = a.dz[(size_t)rx_map(d, r).roff + gl];
             const float* rx = a.rx + 3 * r;
             float hs = bb;
             hs = hs + w0 * rx[0];
@@ -280,8 +283,8 @@ __global__ void __launch_bounds__(256) k_pre_bwd_param(Dims d, PreBwdArgs a, int
     float acc[GS_PART];
 #pragma unroll
     for (int k = 0; k < GS_PART; ++k) acc[k] = 0.0f;
-    for (int c = 0; c < nchunks; ++c) {
-        const float* pt = a.part + ((size_t)c * P + g) * GS_PART;
+    for (int y = (int)sc.group0; y < (int)(sc.group0 + sc.ngroups); ++y) {
+        const float* pt = a.part + ((size_t)group_map(d, y).part + gl) * GS_PART;
 #pragma unroll
         for (int k = 0; k < GS_PART; ++k) acc[k] += pt[k];
     }
@@ -348,16 +351,15 @@ __global__ void __launch_bounds__(256) k_pre_bwd_param(Dims d, PreBwdArgs a, int
 hipError_t launch_preprocess_bwd(const Dims& d, const PreBwdArgs& a, hipStream_t s)
 {
     if (d.P == 0 || d.R == 0) return hipSuccess;
-    const int nchunks = (d.R + GS_BRC - 1) / GS_BRC;
-    const dim3 g1((d.P + GS_BWD_THREADS - 1) / GS_BWD_THREADS, nchunks);
+    const dim3 g1((d.P + GS_BWD_THREADS - 1) / GS_BWD_THREADS, d.G);
     if (a.mode == GRAD_REFERENCE)
         hipLaunchKernelGGL(k_pre_bwd_geo<GRAD_REFERENCE>, g1, dim3(GS_BWD_THREADS), 0, s, d, a);
     else
         hipLaunchKernelGGL(k_pre_bwd_geo<GRAD_EXACT>, g1, dim3(GS_BWD_THREADS), 0, s, d, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const int threads = d.P * 32;
-    hipLaunchKernelGGL(k_pre_bwd_param, dim3((threads + 255) / 256), dim3(256), 0, s, d, a, nchunks);
+    const long long threads = (long long)d.Ptot * 32;
+    hipLaunchKernelGGL(k_pre_bwd_param, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, d, a);
     return hipGetLastError();
 }
 

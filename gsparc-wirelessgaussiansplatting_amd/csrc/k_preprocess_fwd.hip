@@ -2,52 +2,56 @@
 //
 // One thread per Gaussian.  Receiver-independent work (cov3D, the log of the
 // transmitter distance, opacity threshold) is done once; the emission MLP is
-// evaluated for chunks of GS_RC receivers (one chunk per blockIdx.y) so that
-// each Gaussian's 194 weights are streamed once per chunk, not per receiver.
+// evaluated for groups of up to GS_RC receivers of one scene (one group per
+// blockIdx.y) so that each Gaussian's 194 weights are streamed once per group,
+// not per receiver.  blockIdx.x * 256 + threadIdx.x is the Gaussian's index
+// within the group's scene.
 #include "gs_device.h"
 #include "gs_kernels.h"
 
 namespace gs {
 
-#define GS_RC 8
-
 __global__ void __launch_bounds__(GS_PRE_THREADS)
 k_preprocess_fwd(Dims d, PreFwdArgs a)
 {
-    const int g = blockIdx.x * GS_PRE_THREADS + threadIdx.x;
-    const bool valid = g < d.P;
-    const int P = d.P;
+    const GroupMap grp = group_map(d, blockIdx.y);
+    const SceneMap sc = scene_map(d, grp.scene);
+    const int g = blockIdx.x * GS_PRE_THREADS + threadIdx.x;   // within the scene
+    if (blockIdx.x * GS_PRE_THREADS >= (int)sc.P) return;
+    const bool valid = g < (int)sc.P;
+    const int P = d.Ptot;                                      // MLP section stride
+    const int gg = (int)sc.g0 + g;                             // packed index
+    const float* tx = a.tx + 3 * grp.scene;
 
     float mean[3] = {0, 0, 0}, c3[6] = {0, 0, 0, 0, 0, 0}, o = 0.0f, ld = 0.0f;
     if (valid) {
-        mean[0] = a.means[3 * g]; mean[1] = a.means[3 * g + 1]; mean[2] = a.means[3 * g + 2];
-        const float s[3] = {a.scales[3 * g], a.scales[3 * g + 1], a.scales[3 * g + 2]};
-        const float q[4] = {a.rots[4 * g], a.rots[4 * g + 1], a.rots[4 * g + 2], a.rots[4 * g + 3]};
+        mean[0] = a.means[3 * gg]; mean[1] = a.means[3 * gg + 1]; mean[2] = a.means[3 * gg + 2];
+        const float s[3] = {a.scales[3 * gg], a.scales[3 * gg + 1], a.scales[3 * gg + 2]};
+        const float q[4] = {a.rots[4 * gg], a.rots[4 * gg + 1], a.rots[4 * gg + 2], a.rots[4 * gg + 3]};
         cov3d(s, a.mod, q, c3);
-        if (blockIdx.y == 0)
+        if (grp.r0 == (int)sc.r0)   // the scene's first group
 #pragma unroll
-            for (int k = 0; k < 6; ++k) a.cov3[6 * g + k] = c3[k];
-        o = a.opac[g];
-        const float df[3] = {a.tx[0] - mean[0], a.tx[1] - mean[1], a.tx[2] - mean[2]};
+            for (int k = 0; k < 6; ++k) a.cov3[6 * gg + k] = c3[k];
+        o = a.opac[gg];
+        const float df[3] = {tx[0] - mean[0], tx[1] - mean[1], tx[2] - mean[2]};
         const float dist = sqrtf(df[0] * df[0] + df[1] * df[1] + df[2] * df[2]);
         ld = gs_logf(dist + 1e-6f);
     }
     // alpha = o*G < 1/255 for every power < thr (see gs_device.h)
     const float thr = -gs_logf(255.0f * o) - 1e-3f;
 
-    // blockIdx.y selects a chunk of GS_RC receivers
     {
-        const int r0 = blockIdx.y * GS_RC;
-        const int nr = min(GS_RC, d.R - r0);
+        const int r0 = grp.r0;
+        const int nr = grp.nr;
         // ---- emission MLP 3->32->2 for receivers r0..r0+nr (forward.cu:17-75)
         float z0[GS_RC], z1[GS_RC];
         if (a.mlp && valid) {
             // the Gaussian's weights as 16-byte loads (the ABI checks the
             // alignment), four hidden units per step, summed in unit order
-            const float4* W1v = reinterpret_cast<const float4*>(a.mlp + (size_t)g * 96);
-            const float4* b1v = reinterpret_cast<const float4*>(a.mlp + (size_t)P * 96 + (size_t)g * 32);
-            const float4* W2v = reinterpret_cast<const float4*>(a.mlp + (size_t)P * 128 + (size_t)g * 64);
-            const float2 b2 = *reinterpret_cast<const float2*>(a.mlp + (size_t)P * 192 + (size_t)g * 2);
+            const float4* W1v = reinterpret_cast<const float4*>(a.mlp + (size_t)gg * 96);
+            const float4* b1v = reinterpret_cast<const float4*>(a.mlp + (size_t)P * 96 + (size_t)gg * 32);
+            const float4* W2v = reinterpret_cast<const float4*>(a.mlp + (size_t)P * 128 + (size_t)gg * 64);
+            const float2 b2 = *reinterpret_cast<const float2*>(a.mlp + (size_t)P * 192 + (size_t)gg * 2);
 #pragma unroll
             for (int c = 0; c < GS_RC; ++c) { z0[c] = b2.x; z1[c] = b2.y; }
             for (int j4 = 0; j4 < 8; ++j4) {
@@ -79,7 +83,7 @@ k_preprocess_fwd(Dims d, PreFwdArgs a)
         for (int c = 0; c < nr; ++c) {
             const int r = r0 + c;
             const float* view = a.views + 16 * r;
-            const size_t ri = (size_t)r * P + g;
+            const size_t ri = (size_t)rx_map(d, r).roff + g;
             int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
             if (valid) {
                 int radius = 0;
@@ -125,10 +129,8 @@ k_preprocess_fwd(Dims d, PreFwdArgs a)
 hipError_t launch_preprocess_fwd(const Dims& d, const PreFwdArgs& a, hipStream_t s)
 {
     const int NB = (d.P + GS_PRE_THREADS - 1) / GS_PRE_THREADS;
-    if (NB == 0) return hipSuccess;
-    const int nchunks = (d.R + GS_RC - 1) / GS_RC;
-    if (nchunks == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_preprocess_fwd, dim3(NB, nchunks), dim3(GS_PRE_THREADS), 0, s, d, a);
+    if (NB == 0 || d.G == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_preprocess_fwd, dim3(NB, d.G), dim3(GS_PRE_THREADS), 0, s, d, a);
     return hipGetLastError();
 }
 

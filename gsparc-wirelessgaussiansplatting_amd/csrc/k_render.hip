@@ -89,7 +89,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
     const uint2 range = a.ranges[(size_t)r * d.T + pm.tile];
     const uint32_t rb = __builtin_amdgcn_readfirstlane(range.x);
     const uint32_t re = __builtin_amdgcn_readfirstlane(range.y);
-    const size_t rP = (size_t)r * d.P;
+    const size_t rP = rx_map(d, r).roff;
 
     // T carries the termination flag in its sign: a pixel is done once
     // T < 0, with |T| its final transmittance.  A done pixel can still pass
@@ -242,7 +242,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
     if (maxlast == 0) return;
     const uint2 range = a.ranges[(size_t)r * d.T + pm.tile];
     const uint32_t rb = __builtin_amdgcn_readfirstlane(range.x);
-    const size_t rP = (size_t)r * d.P;
+    const size_t rP = rx_map(d, r).roff;
 
     float acc_re[PPL], acc_im[PPL], last_alpha[PPL], last_re[PPL], last_im[PPL];
 #pragma unroll

@@ -138,3 +138,55 @@ def rasterize_gaussians_batched(means3D, emission_mlps, opacities, scales, rotat
     """Render R receivers: returns (out_real [R,H,W], out_imag [R,H,W], radii [R,P]).
     Pass emission_mlps (per-Gaussian 3->32->2 MLPs) or emission [R,P,2]."""
     return _RasterizeBatched.apply(means3D, emission_mlps, emission, opacities, scales, rotations, settings)
+
+
+# ------------------------------------------------------------------ uneven scenes
+class SceneSetting(NamedTuple):
+    """A batch of uneven scenes in one launch sequence (no reference
+    counterpart).  scenes: wireless_rasterizer.SceneBatch; viewmatrices
+    [R,4,4] and rx_pos [R,3] packed by scene; tx_pos [S,3]."""
+    image_height: int
+    image_width: int
+    scale_modifier: float
+    scenes: object
+    viewmatrices: torch.Tensor
+    tx_pos: torch.Tensor
+    rx_pos: torch.Tensor
+    grad_mode: str = "reference"
+    capacity: int = 0
+
+
+class _RasterizeScenes(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, means3D, emission_mlps, emission, opacities, scales, rotations, settings):
+        s = settings
+        K, ore, oim, radii, geom, binbuf, img = wr.forward_scenes(
+            means3D, emission_mlps, opacities, scales, rotations, s.image_height, s.image_width, s.scale_modifier,
+            s.viewmatrices, s.tx_pos, s.rx_pos, s.scenes, emission=emission, capacity=s.capacity or None)
+        ctx.s = s
+        ctx.K = K
+        ctx.has_emission = emission is not None
+        ctx.save_for_backward(means3D, emission_mlps if emission is None else emission, opacities, scales, rotations,
+                              radii, geom, binbuf, img)
+        ctx.mark_non_differentiable(radii)
+        return ore, oim, radii
+
+    @staticmethod
+    def backward(ctx, gre, gim, _gr):
+        s = ctx.s
+        means3D, em_or_mlp, opacities, scales, rotations, radii, geom, binbuf, img = ctx.saved_tensors
+        mlp = None if ctx.has_emission else em_or_mlp
+        em = em_or_mlp if ctx.has_emission else None
+        d2, d3, dmlp, dop, _dcov, dsc, drot, dem = wr.backward_scenes(
+            means3D, mlp, radii, opacities, scales, rotations, s.scale_modifier, s.viewmatrices, gre, gim, s.tx_pos,
+            s.rx_pos, s.scenes, geom, ctx.K, binbuf, img, grad_mode=s.grad_mode, emission=em)
+        return (d3, None if dmlp is None else dmlp.reshape(mlp.shape), None if dem is None else dem.reshape(em.shape),
+                dop.reshape(opacities.shape), dsc, drot, None)
+
+
+def rasterize_scenes(means3D, emission_mlps, opacities, scales, rotations, settings: SceneSetting,
+                     emission: Optional[torch.Tensor] = None):
+    """Render a batch of uneven scenes: (out_real [R,H,W], out_imag [R,H,W],
+    radii [Nrec]).  Pass emission_mlps (packed per-Gaussian MLPs, see
+    wireless_rasterizer.pack_mlps) or emission [Nrec,2]."""
+    return _RasterizeScenes.apply(means3D, emission_mlps, emission, opacities, scales, rotations, settings)

@@ -34,6 +34,14 @@ class Layout(C.Structure):
         "plist", "keys_sorted", "bin_bytes")]
 
 
+class Batch(C.Structure):
+    """gsparc_batch: totals of a batch of uneven scenes."""
+    _fields_ = [("S", C.c_int), ("R", C.c_int), ("G", C.c_int), ("Pmax", C.c_int), ("Ptot", C.c_int),
+                ("Nrec", C.c_uint64), ("Npart", C.c_uint64)]
+
+
+_BP = C.POINTER(Batch)
+
 _SIGS = {
     "gsparc_version": (C.c_char_p, []),
     "gsparc_last_error": (C.c_char_p, []),
@@ -53,6 +61,19 @@ _SIGS = {
     "gsparc_rasterize_forward": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f, _vp, _vp,
                                       _vp, _u64, _vp, _vp, _vp, _i, _vp]),
     "gsparc_instance_count": (_i, [_i, _i, _i, _i, _vp, C.POINTER(_u64), _vp]),
+    "gsparc_scene_table_words": (_sz, [_i, _i]),
+    "gsparc_scene_table_build": (_i, [_i, C.POINTER(_i), C.POINTER(_i), C.POINTER(C.c_uint32), _sz, _BP]),
+    "gsparc_scene_buffer_sizes": (_i, [_BP, _i, _i, _u64, C.POINTER(_sz)]),
+    "gsparc_rasterize_forward_scenes": (_i, [_BP, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f,
+                                             _vp, _vp, _vp, _u64, _vp, _vp, _vp, _i, _vp]),
+    "gsparc_rasterize_forward_geom_scenes": (_i, [_BP, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f,
+                                                  _vp, _vp, _vp, C.POINTER(_u64), _vp]),
+    "gsparc_rasterize_forward_render_scenes": (_i, [_BP, _vp, _i, _i, _vp, _vp, _vp, _vp, _u64, _vp, _vp, _vp,
+                                                    _i, _vp]),
+    "gsparc_instance_count_scenes": (_i, [_BP, _vp, _i, _i, _vp, C.POINTER(_u64), _vp]),
+    "gsparc_rasterize_backward_scenes": (_i, [_i, _BP, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f,
+                                              _vp, _vp, _vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                              _vp, _vp, _vp, _vp]),
     "gsparc_mark_visible": (_i, [_i, _vp, _vp, _vp, _vp]),
     "gsparc_ssim_forward": (_i, [_i, _i, _i, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gsparc_ssim_backward": (_i, [_i, _i, _i, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -224,3 +224,149 @@ def mark_visible(means3D, viewmatrix, projmatrix):
 def mark_visisble(means3D, viewmatrix, projmatrix):
     """The name the reference's Python wrapper calls (cuda_rasterizer/__init__.py:115)."""
     return mark_visible(means3D, viewmatrix, projmatrix)
+
+
+# ---------------------------------------------------------------- uneven scenes
+class SceneBatch:
+    """S independent scenes in one launch sequence (no reference counterpart).
+    Scene s has scene_P[s] Gaussians and scene_R[s] receivers, packed one
+    scene after the other; holds the device scene table and the batch totals."""
+
+    def __init__(self, scene_P, scene_R, device):
+        scene_P = [int(p) for p in scene_P]
+        scene_R = [int(r) for r in scene_R]
+        if len(scene_P) != len(scene_R) or not scene_P:
+            raise RuntimeError("one Gaussian count and one receiver count per scene")
+        S = len(scene_P)
+        L = gh.lib()
+        words = L.gsparc_scene_table_words(S, sum(scene_R))
+        tab = (C.c_uint32 * words)()
+        self.batch = gh.Batch()
+        gh.check(L.gsparc_scene_table_build(S, (C.c_int * S)(*scene_P), (C.c_int * S)(*scene_R), tab, words,
+                                            C.byref(self.batch)), "scene table")
+        import numpy as np
+        self.table = torch.from_numpy(np.frombuffer(tab, dtype=np.int32).copy()).to(device)
+        self.scene_P, self.scene_R = scene_P, scene_R
+        self.S, self.R, self.Ptot, self.Nrec = S, self.batch.R, self.batch.Ptot, int(self.batch.Nrec)
+
+    def sizes(self, H, W, K=0):
+        out = (C.c_size_t * 4)()
+        gh.check(gh.lib().gsparc_scene_buffer_sizes(C.byref(self.batch), int(W), int(H), int(K), out), "sizes")
+        return list(out)
+
+
+def pack_mlps(mlps):
+    """Per-scene flat [P_s*194] MLP arrays -> the packed [Ptot*194] layout
+    (each of the W1 | b1 | W2 | b2 sections holds all scenes in order)."""
+    cuts = ((0, 96), (96, 128), (128, 192), (192, 194))
+    parts = []
+    for a, b in cuts:
+        for m in mlps:
+            P = m.numel() // 194
+            parts.append(m.reshape(-1)[P * a:P * b])
+    return torch.cat(parts)
+
+
+def forward_scenes(means3D, emission_mlps, opacities, scales, rotations, image_height, image_width, scale_modifier,
+                   viewmatrices, tx_pos, rx_pos, scenes: SceneBatch, emission=None, capacity=None):
+    """Forward of a batch of uneven scenes.  means etc. [Ptot,...] packed by
+    scene; viewmatrices [R,4,4], rx_pos [R,3] packed by scene; tx_pos [S,3];
+    emission None or [Nrec,2].  Returns (K or capacity, out_real [R,H,W],
+    out_imag [R,H,W], radii [Nrec], geom, bin, img)."""
+    _check_means(means3D)
+    H, W = int(image_height), int(image_width)
+    dev = means3D.device
+    if int(means3D.size(0)) != scenes.Ptot:
+        raise RuntimeError("means3D must hold every scene's Gaussians")
+    views = gh.f32(viewmatrices).reshape(-1, 16)
+    rxs = gh.f32(rx_pos).reshape(-1, 3)
+    tx = gh.f32(tx_pos).reshape(-1, 3)
+    if views.size(0) != scenes.R or rxs.size(0) != scenes.R or tx.size(0) != scenes.S:
+        raise RuntimeError("one view matrix and position per receiver, one TX position per scene")
+    means, sc, rot = gh.f32(means3D), gh.f32(scales), gh.f32(rotations)
+    op = gh.f32(opacities).reshape(-1)
+    mlp = gh.f32_aligned(emission_mlps).reshape(-1) if emission is None else None
+    em = gh.f32(emission).reshape(-1, 2) if emission is not None else None
+    if em is None and mlp.numel() != scenes.Ptot * 194:
+        raise RuntimeError("emission_mlps must hold 194 floats per Gaussian")
+    gh.require_cuda(means, sc, rot, op, tx, views, rxs, mlp, em)
+    L = gh.lib()
+    geom_b, img_b, _, _ = scenes.sizes(H, W)
+    geom = _empty_bytes(dev, geom_b)
+    img = _empty_bytes(dev, img_b)
+    radii = torch.empty(scenes.Nrec, dtype=torch.int32, device=dev)
+    out_re = torch.empty((scenes.R, H, W), dtype=torch.float32, device=dev)
+    out_im = torch.empty((scenes.R, H, W), dtype=torch.float32, device=dev)
+    stream = gh.stream_handle(dev)
+    bp = C.byref(scenes.batch)
+    if capacity is not None:
+        Kc = int(capacity)
+        binbuf = _empty_bytes(dev, scenes.sizes(H, W, Kc)[2])
+        gh.check(L.gsparc_rasterize_forward_scenes(bp, gh.ptr(scenes.table), W, H, gh.ptr(means), gh.ptr(sc),
+                                                  gh.ptr(rot), gh.ptr(op), gh.ptr(mlp), gh.ptr(em), gh.ptr(views),
+                                                  gh.ptr(tx), gh.ptr(rxs), float(scale_modifier), gh.ptr(geom),
+                                                  gh.ptr(img), gh.ptr(binbuf), Kc, gh.ptr(radii), gh.ptr(out_re),
+                                                  gh.ptr(out_im), 0, stream), "rasterize forward (scenes)")
+        return Kc, out_re, out_im, radii, geom, binbuf, img
+    K = C.c_uint64(0)
+    gh.check(L.gsparc_rasterize_forward_geom_scenes(bp, gh.ptr(scenes.table), W, H, gh.ptr(means), gh.ptr(sc),
+                                                   gh.ptr(rot), gh.ptr(op), gh.ptr(mlp), gh.ptr(views), gh.ptr(tx),
+                                                   gh.ptr(rxs), float(scale_modifier), gh.ptr(geom), gh.ptr(img),
+                                                   gh.ptr(radii), C.byref(K), stream), "rasterize forward (scenes)")
+    K = int(K.value)
+    binbuf = _empty_bytes(dev, scenes.sizes(H, W, K)[2])
+    gh.check(L.gsparc_rasterize_forward_render_scenes(bp, gh.ptr(scenes.table), W, H, gh.ptr(em), gh.ptr(geom),
+                                                     gh.ptr(img), gh.ptr(binbuf), K, gh.ptr(radii), gh.ptr(out_re),
+                                                     gh.ptr(out_im), 0, stream), "rasterize forward (scenes)")
+    return K, out_re, out_im, radii, geom, binbuf, img
+
+
+def backward_scenes(means3D, emission_mlps, radii, opacities, scales, rotations, scale_modifier, viewmatrices,
+                    dL_dout_real, dL_dout_imag, tx_pos, rx_pos, scenes: SceneBatch, geomBuffer, num_rendered,
+                    binningBuffer, imageBuffer, grad_mode="reference", emission=None):
+    """Backward of forward_scenes.  Returns (dmeans2D [Nrec,3], dmeans3D
+    [Ptot,3], demission_mlps [Ptot*194] or None, dopacity [Ptot,1], dcov3D
+    [Ptot,6], dscales [Ptot,3], drotations [Ptot,4], demission [Nrec,2] or None)."""
+    H, W = int(dL_dout_real.size(1)), int(dL_dout_real.size(2))
+    dev = means3D.device
+    opts = dict(dtype=torch.float32, device=dev)
+    mode = gh.GRAD_MODES[grad_mode] if isinstance(grad_mode, str) else int(grad_mode)
+    P = scenes.Ptot
+    dmeans2D = torch.zeros((scenes.Nrec, 3), **opts)
+    dmeans3D = torch.zeros((P, 3), **opts)
+    dmlp = torch.zeros(P * 194, **opts) if emission is None else None
+    dop = torch.zeros((P, 1), **opts)
+    dcov = torch.zeros((P, 6), **opts)
+    dsc = torch.zeros((P, 3), **opts)
+    drot = torch.zeros((P, 4), **opts)
+    dem = torch.zeros((scenes.Nrec, 2), **opts) if emission is not None else None
+    means, sc, rot = gh.f32(means3D), gh.f32(scales), gh.f32(rotations)
+    views = gh.f32(viewmatrices).reshape(-1, 16)
+    rxs = gh.f32(rx_pos).reshape(-1, 3)
+    tx = gh.f32(tx_pos).reshape(-1, 3)
+    mlp = gh.f32_aligned(emission_mlps).reshape(-1) if emission is None else None
+    em = gh.f32(emission).reshape(-1, 2) if emission is not None else None
+    rad = radii.reshape(-1).to(torch.int32).contiguous()
+    gre, gim = gh.f32(dL_dout_real), gh.f32(dL_dout_imag)
+    gh.require_cuda(means, sc, rot, views, rxs, tx, mlp, em, rad, gre, gim)
+    grad = _empty_bytes(dev, scenes.sizes(H, W)[3])
+    gh.check(gh.lib().gsparc_rasterize_backward_scenes(
+        mode, C.byref(scenes.batch), gh.ptr(scenes.table), W, H, gh.ptr(means), gh.ptr(sc), gh.ptr(rot),
+        gh.ptr(mlp), gh.ptr(em), gh.ptr(views), gh.ptr(tx), gh.ptr(rxs), float(scale_modifier), gh.ptr(rad),
+        gh.ptr(geomBuffer), gh.ptr(imageBuffer), gh.ptr(binningBuffer), int(num_rendered), gh.ptr(grad),
+        gh.ptr(gre), gh.ptr(gim), gh.ptr(dmeans2D), gh.ptr(dmeans3D), gh.ptr(dmlp), gh.ptr(dop), gh.ptr(dcov),
+        gh.ptr(dsc), gh.ptr(drot), gh.ptr(dem), gh.stream_handle(dev)), "rasterize backward (scenes)")
+    return dmeans2D, dmeans3D, dmlp, dop, dcov, dsc, drot, dem
+
+
+def unpack_mlps(packed, scene_P):
+    """Inverse of pack_mlps: [Ptot*194] -> one flat [P_s*194] array per scene."""
+    Ptot = sum(int(p) for p in scene_P)
+    out = [[] for _ in scene_P]
+    for a, b in ((0, 96), (96, 128), (128, 192), (192, 194)):
+        off = Ptot * a
+        for s, P in enumerate(scene_P):
+            n = int(P) * (b - a)
+            out[s].append(packed[off:off + n])
+            off += n
+    return [torch.cat(o) for o in out]

@@ -171,6 +171,59 @@ int gsparc_profile_collect(int* n_kernels);
 int gsparc_profile_query(int i, char* name, int name_len, double* total_ms, long long* launches);
 int gsparc_profile_counts(uint64_t counts[4]);
 
+/* ---- batches of uneven scenes --------------------------------------------
+ * S independent scenes in one launch sequence: scene s has P_s Gaussians
+ * (packed one scene after the other in means / scales / rotations /
+ * opacities, and in each section of emission_mlps) and R_s receivers (packed
+ * likewise in viewmatrices and rx_pos); tx_pos is [S,3].  Every receiver
+ * sees only its own scene.  Per-(receiver, Gaussian) arrays (radii,
+ * emission, dmeans2D, demission) hold Nrec = sum_s R_s P_s records, receiver
+ * after receiver; maps are [R,H,W] with R = sum_s R_s.
+ * gsparc_scene_table_build fills a host table (gsparc_scene_table_words
+ * words) and the batch totals; the caller copies the table to the device
+ * and passes that copy.  No counterpart in the reference (one receiver of
+ * one scene per call, renderer.py:39-88); for callers that train or serve
+ * many scenes of different sizes at once. */
+typedef struct gsparc_batch {
+    int S, R, G, Pmax, Ptot;   /* scenes, receivers, receiver groups, largest and total Gaussian counts */
+    uint64_t Nrec, Npart;      /* (receiver, Gaussian) records; backward partial-sum slots */
+} gsparc_batch;
+
+size_t gsparc_scene_table_words(int S, int R);
+int gsparc_scene_table_build(int S, const int* scene_P, const int* scene_R, uint32_t* table, size_t table_words,
+                             gsparc_batch* batch);
+/* sizes[0..3] = geom, image, bin (for K_capacity instances) and grad bytes */
+int gsparc_scene_buffer_sizes(const gsparc_batch* batch, int W, int H, uint64_t K_capacity, size_t* sizes);
+/* As gsparc_rasterize_forward (no host synchronisation; K stays on the device) */
+int gsparc_rasterize_forward_scenes(const gsparc_batch* batch, const uint32_t* scene_table, int W, int H,
+                                    const float* means, const float* scales, const float* rotations,
+                                    const float* opacities, const float* emission_mlps, const float* emission,
+                                    const float* viewmatrices, const float* tx_pos, const float* rx_pos,
+                                    float scale_modifier, void* geom, void* image, void* bin, uint64_t K_capacity,
+                                    int* radii_out, float* out_real, float* out_imag, int flags, void* stream);
+/* As gsparc_rasterize_forward_geom / gsparc_rasterize_forward_render (K read back) */
+int gsparc_rasterize_forward_geom_scenes(const gsparc_batch* batch, const uint32_t* scene_table, int W, int H,
+                                         const float* means, const float* scales, const float* rotations,
+                                         const float* opacities, const float* emission_mlps,
+                                         const float* viewmatrices, const float* tx_pos, const float* rx_pos,
+                                         float scale_modifier, void* geom, void* image, int* radii_out,
+                                         uint64_t* K_out, void* stream);
+int gsparc_rasterize_forward_render_scenes(const gsparc_batch* batch, const uint32_t* scene_table, int W, int H,
+                                           const float* emission, void* geom, void* image, void* bin,
+                                           uint64_t K_capacity, const int* radii, float* out_real,
+                                           float* out_imag, int flags, void* stream);
+int gsparc_instance_count_scenes(const gsparc_batch* batch, const uint32_t* scene_table, int W, int H,
+                                 const void* image, uint64_t* K, void* stream);
+/* As gsparc_rasterize_backward; dmeans2D [Nrec,3], demission [Nrec,2] */
+int gsparc_rasterize_backward_scenes(int grad_mode, const gsparc_batch* batch, const uint32_t* scene_table, int W,
+                                     int H, const float* means, const float* scales, const float* rotations,
+                                     const float* emission_mlps, const float* emission, const float* viewmatrices,
+                                     const float* tx_pos, const float* rx_pos, float scale_modifier,
+                                     const int* radii, void* geom, void* image, void* bin, uint64_t K_capacity,
+                                     void* grad, const float* dout_real, const float* dout_imag, float* dmeans2D,
+                                     float* dmeans3D, float* demission_mlps, float* dopacity, float* dcov3D,
+                                     float* dscales, float* drotations, float* demission, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
