@@ -11,6 +11,11 @@ Prints ONE JSON line.  Also reported: the same step done the plain way (one
 receiver per call, the reference's calling pattern), p50/p99 single-spectrum
 forward latency, per-kernel times from HIP events, and the CPU oracle timed
 on a bounded sample as a baseline.
+
+--workload scenes measures the uneven-scene batch instead: 16 independent
+scenes with Gaussian counts log-uniform in [5k, 200k] and 4..64 receivers
+each, forward+backward of all of them in one launch sequence; it adds p50/p99
+of the batch step and of single spectra rendered alone.
 """
 import argparse
 import json
@@ -46,6 +51,8 @@ def parse():
     ap.add_argument("--fwd-ppl", type=int, default=0, help="forward blend pixels per lane (0: library default)")
     ap.add_argument("--bwd-ppl", type=int, default=0, help="backward blend pixels per lane (0: automatic)")
     ap.add_argument("--order", type=int, default=0, help="blend work order (0: automatic)")
+    ap.add_argument("--workload", default="receivers", choices=["receivers", "scenes"],
+                    help="receivers: one scene, R receivers (default); scenes: 16 uneven scenes in one batch")
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
 
@@ -73,6 +80,199 @@ def log(args, *a):
         print(*a, file=sys.stderr, flush=True)
 
 
+class Workload:
+    """What one timed step does and how to describe it."""
+
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+
+def loss_fn(ore, tgt):
+    """0.8 * L1 + 0.2 * (1 - SSIM) on the real channel (train.py:418-427)."""
+    from fused_ssim import fused_ssim
+    l1 = (ore - tgt).abs().mean()
+    ss = fused_ssim(ore.unsqueeze(1), tgt.unsqueeze(1))
+    return 0.8 * l1 + 0.2 * (1.0 - ss)
+
+
+def plain_step(params, settings_for, targets, rx_count):
+    """One receiver per call through the reference API (renderer.py:39-88)."""
+    import torch
+    import cuda_rasterizer as cr
+    for prm, st, tgt in zip(params, settings_for, targets):
+        m, s_, q, o, w = prm
+        m2d = torch.zeros_like(m, requires_grad=True)
+        ore, oim, radii = cr.rasterize_gaussians(m, m2d, w, o, s_, q, st)
+        loss = loss_fn(ore, tgt) / rx_count
+        loss.backward()
+
+
+def cpu_oracle_sample(args, scenes_and_rx, targets, label):
+    """The CPU oracle (FP32, one core) on a bounded sample of the same
+    spectra: forward, loss gradient and backward of each sampled spectrum."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    log(args, f"CPU baseline: oracle fwd+bwd on {len(scenes_and_rx)} spectra, 1 core")
+    t0 = time.perf_counter()
+    accs = {}
+    for (sc, r), tgt in zip(scenes_and_rx, targets):
+        acc = accs.setdefault(id(sc), O.Grads(sc.P))
+        pre, keys, vals, ranges, fwd = O.render(sc, r)
+        img1 = fwd["out_re"][None, None].astype(np.float32)
+        img2 = tgt[None, None]
+        m_, d1, d2, d3 = O.ssim_fwd(img1, img2, dtype=np.float32)
+        gss = O.ssim_bwd(img1, img2, np.full_like(img1, -0.2 / img1.size), d1, d2, d3, dtype=np.float32)
+        dre = (0.8 / img1.size) * np.sign(img1 - img2)[0, 0] + gss[0, 0]
+        O.backward(sc, pre, vals, ranges, fwd, dre, np.zeros_like(dre), O.GRAD_REFERENCE, r=r, grads=acc)
+    el = time.perf_counter() - t0
+    n = len(scenes_and_rx)
+    return {"value": round(n / el, 4), "unit": "spectra/s", "cores": 1, "kind": "port",
+            "sample": f"{label}, fwd+bwd incl. SSIM, FP32 C++ oracle single-threaded ({el:.1f} s)"}
+
+
+def setup_receivers(args, dev):
+    """One scene (config2 generator), R receivers in one batch."""
+    import numpy as np
+    import torch
+    import cuda_rasterizer as cr
+    import gsparc_scenes as S
+    import wireless_rasterizer as wr
+    P, R = args.gaussians, args.receivers
+    sc = S.config2(P=P, R=R)
+    tgt_sc = S.config2(seed=3, P=P, R=R)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    means, scales, rots, opac, mlp = (t(sc.means), t(sc.scales), t(sc.rotations), t(sc.opacities), t(sc.mlp))
+    views, txp, rxp = t(sc.views), t(sc.tx_pos), t(sc.rx_pos)
+    with torch.no_grad():
+        target = wr.forward_batched(t(tgt_sc.means), t(tgt_sc.mlp), t(tgt_sc.opacities), t(tgt_sc.scales),
+                                    t(tgt_sc.rotations), sc.H, sc.W, 1.0, views, txp, rxp)[1].clone()
+    params = [x.clone().requires_grad_(True) for x in (means, scales, rots, opac, mlp)]
+    settings = cr.BatchedSetting(sc.H, sc.W, 1.0, views, txp, rxp, args.grad_mode)
+
+    def step():
+        m, s_, q, o, w = params
+        ore, oim, radii = cr.rasterize_gaussians_batched(m, w, o, s_, q, settings)
+        loss_fn(ore, target).backward()
+        for p_ in params:
+            p_.grad = None
+
+    plain_settings = [cr.GaussianRasterizerSetting(sc.H, sc.W, 1.0, views[r], torch.eye(4, device=dev), txp,
+                                                   rxp[r], args.grad_mode) for r in range(R)]
+
+    def step_plain():
+        plain_step([params] * R, plain_settings, [target[r:r + 1] for r in range(R)], R)
+        for p_ in params:
+            p_.grad = None
+
+    def single(i):
+        wr.rasterize_gaussians(means, mlp, opac, scales, rots, sc.H, sc.W, 1.0, views[i % R], None, txp, rxp[i % R])
+
+    def count():
+        return wr.forward_batched(means, mlp, opac, scales, rots, sc.H, sc.W, 1.0, views, txp, rxp)[0]
+
+    def cpu():
+        nr = min(args.cpu_receivers, R)
+        tg = target[:nr].cpu().numpy()
+        return cpu_oracle_sample(args, [(sc, r) for r in range(nr)], list(tg),
+                                 f"{nr} of the {R} receivers of the same {P // 1000}k-Gaussian scene")
+
+    return Workload(
+        step=step, step_plain=step_plain, single=single, count=count, cpu=cpu, spectra=R, Ptot=P, Nrec=P * R,
+        H=sc.H, W=sc.W, traffic_key=(P, R),
+        data="synthetic (seeded: 70% wall-clustered Gaussians, receivers uniform in a 6x6x3 m room)",
+        config={"workload": "batched receivers, fwd+bwd, 0.8*L1 + 0.2*(1-SSIM)", "gaussians": P, "receivers": R,
+                "map": f"{sc.H}x{sc.W}", "emission": "per-Gaussian MLP 3->32->2", "grad_mode": args.grad_mode})
+
+
+def setup_scenes(args, dev):
+    """16 uneven scenes (config4 generator) in one launch sequence."""
+    import numpy as np
+    import torch
+    import cuda_rasterizer as cr
+    import gsparc_scenes as S
+    import wireless_rasterizer as wr
+    scs = S.config4()
+    tgts = S.config4(variant=1)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    per = [dict(means=t(sc.means), scales=t(sc.scales), rots=t(sc.rotations), opac=t(sc.opacities),
+                mlp=t(sc.mlp), views=t(sc.views), tx=t(sc.tx_pos), rx=t(sc.rx_pos)) for sc in scs]
+    cat = lambda k: torch.cat([d[k] for d in per])
+    batch = wr.SceneBatch([sc.P for sc in scs], [sc.R for sc in scs], dev)
+    views, rxp, txp = cat("views"), cat("rx"), torch.stack([d["tx"] for d in per])
+    H, W = scs[0].H, scs[0].W
+    with torch.no_grad():
+        tm = torch.cat([t(x.means) for x in tgts])
+        tmlp = wr.pack_mlps([t(x.mlp) for x in tgts])
+        target = wr.forward_scenes(tm, tmlp, torch.cat([t(x.opacities) for x in tgts]),
+                                   torch.cat([t(x.scales) for x in tgts]), torch.cat([t(x.rotations) for x in tgts]),
+                                   H, W, 1.0, views, txp, rxp, batch)[1].clone()
+        del tm, tmlp
+    packed = [cat("means"), cat("scales"), cat("rots"), cat("opac"), wr.pack_mlps([d["mlp"] for d in per])]
+    params = [x.clone().requires_grad_(True) for x in packed]
+    settings = cr.SceneSetting(H, W, 1.0, batch, views, txp, rxp, args.grad_mode)
+
+    def step():
+        m, s_, q, o, w = params
+        ore, oim, radii = cr.rasterize_scenes(m, w, o, s_, q, settings)
+        loss_fn(ore, target).backward()
+        for p_ in params:
+            p_.grad = None
+
+    # one receiver per call, scene by scene; loss normalised per spectrum
+    pl_params, pl_settings, pl_targets = [], [], []
+    r0 = 0
+    for sc, d in zip(scs, per):
+        prm = [d[k].clone().requires_grad_(True) for k in ("means", "scales", "rots", "opac", "mlp")]
+        for r in range(sc.R):
+            pl_params.append(prm)
+            pl_settings.append(cr.GaussianRasterizerSetting(H, W, 1.0, d["views"][r], torch.eye(4, device=dev),
+                                                            d["tx"], d["rx"][r], args.grad_mode))
+            pl_targets.append(target[r0 + r:r0 + r + 1])
+        r0 += sc.R
+
+    def step_plain():
+        plain_step(pl_params, pl_settings, pl_targets, batch.R)
+        for prm in pl_params:
+            for p_ in prm:
+                p_.grad = None
+
+    pairs = [(s, r) for s, sc in enumerate(scs) for r in range(sc.R)]
+
+    def single(i):
+        s, r = pairs[(i * 7919) % len(pairs)]
+        d = per[s]
+        wr.rasterize_gaussians(d["means"], d["mlp"], d["opac"], d["scales"], d["rots"], H, W, 1.0, d["views"][r],
+                               None, d["tx"], d["rx"][r])
+
+    def count():
+        m, s_, q, o, w = packed
+        return wr.forward_scenes(m, w, o, s_, q, H, W, 1.0, views, txp, rxp, batch)[0]
+
+    def cpu():
+        # the smallest scene's receivers, in order, up to the sample size
+        order = np.argsort([sc.P for sc in scs])
+        sample, tg = [], []
+        offs = np.cumsum([0] + [sc.R for sc in scs])
+        tcpu = target.cpu().numpy()
+        for s in order:
+            for r in range(scs[s].R):
+                if len(sample) < args.cpu_receivers:
+                    sample.append((scs[s], r))
+                    tg.append(tcpu[offs[s] + r])
+        return cpu_oracle_sample(args, sample, tg, f"{len(sample)} spectra of the smallest scenes "
+                                                   f"(P = {sorted(sc.P for sc in scs)[:3]}...)")
+
+    return Workload(
+        step=step, step_plain=step_plain, single=single, count=count, cpu=cpu, spectra=batch.R, Ptot=batch.Ptot,
+        Nrec=batch.Nrec, H=H, W=W, traffic_key=None,
+        data="synthetic (seeded: 16 rooms, 70% wall-clustered Gaussians, receivers uniform in each room)",
+        config={"workload": "uneven scene batch, fwd+bwd, 0.8*L1 + 0.2*(1-SSIM)", "scenes": len(scs),
+                "gaussians": [sc.P for sc in scs], "receivers": [sc.R for sc in scs],
+                "total_gaussians": batch.Ptot, "total_receivers": batch.R, "map": f"{H}x{W}",
+                "emission": "per-Gaussian MLP 3->32->2", "grad_mode": args.grad_mode})
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -82,11 +282,7 @@ def main():
     import numpy as np
     import torch
 
-    import cuda_rasterizer as cr
     import gsparc_hip as gh
-    import gsparc_scenes as S
-    import wireless_rasterizer as wr
-    from fused_ssim import fused_ssim
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
@@ -94,45 +290,8 @@ def main():
     gh.set_option("render_fwd_ppl", args.fwd_ppl)
     gh.set_option("render_bwd_ppl", args.bwd_ppl)
     gh.set_option("render_order", args.order)
-    P, R = args.gaussians, args.receivers
-    sc = S.config2(P=P, R=R)
-    tgt_sc = S.config2(seed=3, P=P, R=R)
-    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
-    means, scales, rots, opac, mlp = (t(sc.means), t(sc.scales), t(sc.rotations), t(sc.opacities), t(sc.mlp))
-    views, txp, rxp = t(sc.views), t(sc.tx_pos), t(sc.rx_pos)
-    with torch.no_grad():
-        tg = wr.forward_batched(t(tgt_sc.means), t(tgt_sc.mlp), t(tgt_sc.opacities), t(tgt_sc.scales),
-                                t(tgt_sc.rotations), sc.H, sc.W, 1.0, views, txp, rxp)
-        target = tg[1].clone()
-    del tg
-    params = [x.clone().requires_grad_(True) for x in (means, scales, rots, opac, mlp)]
-    settings = cr.BatchedSetting(sc.H, sc.W, 1.0, views, txp, rxp, args.grad_mode)
-
-    def loss_fn(ore, tgt):
-        l1 = (ore - tgt).abs().mean()
-        ss = fused_ssim(ore.unsqueeze(1), tgt.unsqueeze(1))
-        return 0.8 * l1 + 0.2 * (1.0 - ss)
-
-    def step_batched():
-        m, s_, q, o, w = params
-        ore, oim, radii = cr.rasterize_gaussians_batched(m, w, o, s_, q, settings)
-        loss = loss_fn(ore, target)
-        loss.backward()
-        for p_ in params:
-            p_.grad = None
-        return loss
-
-    def step_plain():
-        m, s_, q, o, w = params
-        for r in range(R):
-            st = cr.GaussianRasterizerSetting(sc.H, sc.W, 1.0, views[r], torch.eye(4, device=dev), txp, rxp[r],
-                                              args.grad_mode)
-            m2d = torch.zeros_like(m, requires_grad=True)
-            ore, oim, radii = cr.rasterize_gaussians(m, m2d, w, o, s_, q, st)
-            loss = loss_fn(ore, target[r:r + 1]) / R
-            loss.backward()
-        for p_ in params:
-            p_.grad = None
+    wl = setup_receivers(args, dev) if args.workload == "receivers" else setup_scenes(args, dev)
+    R, Ptot, Nrec = wl.spectra, wl.Ptot, wl.Nrec
 
     def timed(fn, steps):
         torch.cuda.synchronize()
@@ -142,21 +301,27 @@ def main():
         torch.cuda.synchronize()
         return time.perf_counter() - t0
 
-    log(args, f"warmup {args.warmup} batched steps (P={P}, R={R})")
+    log(args, f"warmup {args.warmup} batched steps ({args.workload}: {R} spectra, {Ptot} Gaussians)")
     for _ in range(args.warmup):
-        step_batched()
+        wl.step()
     torch.cuda.synchronize()
     log(args, f"timing {args.steps} batched steps")
-    el = timed(step_batched, args.steps)
+    el = timed(wl.step, args.steps)
     ms_per_step = 1e3 * el / args.steps
     value = R * args.steps / el
     log(args, f"batched: {ms_per_step:.2f} ms/step, {value:.1f} spectra/s")
 
+    step_lat = []
+    if args.workload == "scenes":
+        # every spectrum of a batch is ready when its step ends
+        for _ in range(max(5, args.steps // 2)):
+            step_lat.append(1e3 * timed(wl.step, 1))
+
     plain_value = None
     if args.plain_steps > 0:
-        step_plain()
+        wl.step_plain()
         log(args, f"timing {args.plain_steps} plain steps (one receiver per call)")
-        el_plain = timed(step_plain, args.plain_steps)
+        el_plain = timed(wl.step_plain, args.plain_steps)
         plain_value = R * args.plain_steps / el_plain
         log(args, f"plain: {1e3 * el_plain / args.plain_steps:.2f} ms/step, {plain_value:.1f} spectra/s")
 
@@ -166,8 +331,7 @@ def main():
         for i in range(args.latency_iters + 5):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            wr.rasterize_gaussians(means, mlp, opac, scales, rots, sc.H, sc.W, 1.0, views[i % R], None, txp,
-                                   rxp[i % R])
+            wl.single(i)
             torch.cuda.synchronize()
             if i >= 5:
                 lat.append(1e3 * (time.perf_counter() - t0))
@@ -183,14 +347,14 @@ def main():
     gh.profile_enable(True)
     nprof = 3
     for _ in range(nprof):
-        step_batched()
+        wl.step()
     torch.cuda.synchronize()
     kt = gh.profile_collect()
     cnt = {k: v / nprof for k, v in gh.profile_counts().items()}
     gh.profile_enable(False)
     with torch.no_grad():
-        K = wr.forward_batched(means, mlp, opac, scales, rots, sc.H, sc.W, 1.0, views, txp, rxp)[0]
-    N = sc.W * sc.H
+        K = wl.count()
+    N = wl.W * wl.H
     # algorithmic HBM bytes per launch (each datum read or written once)
     work = {
         # 44 B per tile-list instance (index 4, splat record 32, emission 8)
@@ -198,12 +362,12 @@ def main():
         "render_fwd": ("hbm", 44.0 * K + 16.0 * N * R, "GB/s"),
         # + 16 B per pixel read (final T, count, dL/dout re/im) + 32 B per
         # (receiver, Gaussian) accumulated gradient record
-        "render_bwd": ("hbm", 44.0 * K + 16.0 * N * R + 32.0 * P * R, "GB/s"),
-        "preprocess_fwd": ("hbm", P * (820 + 24) + 48 * P * R, "GB/s"),
-        "preprocess_bwd": ("hbm", P * 1690 + 72 * P * R, "GB/s"),
-        "bin_depth_sort": ("hbm", 84.0 * P * R, "GB/s"),
-        "bin_chunk_count": ("hbm", 16.0 * P * R, "GB/s"),
-        "bin_scatter": ("hbm", 4.0 * K + 20.0 * P * R, "GB/s"),
+        "render_bwd": ("hbm", 44.0 * K + 16.0 * N * R + 32.0 * Nrec, "GB/s"),
+        "preprocess_fwd": ("hbm", Ptot * (820 + 24) + 48 * Nrec, "GB/s"),
+        "preprocess_bwd": ("hbm", Ptot * 1690 + 72 * Nrec, "GB/s"),
+        "bin_depth_sort": ("hbm", 84.0 * Nrec, "GB/s"),
+        "bin_chunk_count": ("hbm", 16.0 * Nrec, "GB/s"),
+        "bin_scatter": ("hbm", 4.0 * K + 20.0 * Nrec, "GB/s"),
         "ssim_fwd": ("hbm", 8 * N * R + 16 * N * R, "GB/s"),
         "ssim_bwd": ("hbm", 28 * N * R, "GB/s"),
     }
@@ -211,7 +375,7 @@ def main():
     # Gaussian) pair and per list entry tested against a wave's pixel block
     flops = {"render_fwd": cnt["fwd_pairs"] * FWD_FLOP_PER_PAIR + cnt["fwd_entries"] * CULL_FLOP,
              "render_bwd": cnt["bwd_pairs"] * BWD_FLOP_PER_PAIR + cnt["bwd_entries"] * CULL_FLOP}
-    traffic = measured_traffic(args, P, R)
+    traffic = measured_traffic(args, *wl.traffic_key) if wl.traffic_key else {}
     kernels = {}
     for name, (ms, n) in kt.items():
         avg = ms / max(n, 1)
@@ -239,30 +403,11 @@ def main():
     if roof["traffic"] is not None:
         roof["traffic_source"] = TRAFFIC_FILE
 
-    cpu = None
-    if args.cpu_receivers > 0:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
-        nr = min(args.cpu_receivers, R)
-        tgt_np = target[:nr].cpu().numpy()
-        log(args, f"CPU baseline: oracle fwd+bwd on {nr} receivers, 1 core")
-        t0 = time.perf_counter()
-        acc = O.Grads(P)
-        for r in range(nr):
-            pre, keys, vals, ranges, fwd = O.render(sc, r)
-            img1 = fwd["out_re"][None, None].astype(np.float32)
-            img2 = tgt_np[r][None, None]
-            m_, d1, d2, d3 = O.ssim_fwd(img1, img2, dtype=np.float32)
-            gss = O.ssim_bwd(img1, img2, np.full_like(img1, -0.2 / img1.size), d1, d2, d3, dtype=np.float32)
-            dre = (0.8 / img1.size) * np.sign(img1 - img2)[0, 0] + gss[0, 0]
-            O.backward(sc, pre, vals, ranges, fwd, dre, np.zeros_like(dre), O.GRAD_REFERENCE, r=r, grads=acc)
-        el_cpu = time.perf_counter() - t0
-        cpu = {"value": round(nr / el_cpu, 4), "unit": "spectra/s", "cores": 1, "kind": "port",
-               "sample": f"{nr} of the {R} receivers of the same 50k-Gaussian scene, fwd+bwd incl. SSIM, "
-                         f"FP32 C++ oracle single-threaded ({el_cpu:.1f} s)"}
+    cpu = wl.cpu() if args.cpu_receivers > 0 else None
 
     out = {
-        "metric": METRIC,
+        "metric": METRIC if args.workload == "receivers" else
+        "fwd+bwd spectra/sec, 16 uneven scenes (5k-200k Gaussians, 4-64 rx each), 90x360, one batch",
         "value": round(value, 2),
         "unit": "spectra/s",
         "n_gpus": 1,
@@ -273,10 +418,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
-        "data": "synthetic (seeded: 70% wall-clustered Gaussians, receivers uniform in a 6x6x3 m room)",
-        "config": {"workload": "batched receivers, fwd+bwd, 0.8*L1 + 0.2*(1-SSIM)", "gaussians": P,
-                   "receivers": R, "map": f"{sc.H}x{sc.W}", "emission": "per-Gaussian MLP 3->32->2",
-                   "grad_mode": args.grad_mode},
+        "data": wl.data,
+        "config": wl.config,
         "plain_spectra_per_s": None if plain_value is None else round(plain_value, 2),
         "batched_over_plain": None if plain_value is None else round(value / plain_value, 2),
         "single_spectrum_fwd_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
@@ -286,6 +429,9 @@ def main():
         "kernels": kernels,
         "cpu_baseline": cpu,
     }
+    if step_lat:
+        out["batch_step_ms"] = {"p50": round(float(np.percentile(step_lat, 50)), 3),
+                                "p99": round(float(np.percentile(step_lat, 99)), 3)}
     print(json.dumps(out))
 
 

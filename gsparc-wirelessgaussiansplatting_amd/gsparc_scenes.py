@@ -159,3 +159,26 @@ def target_spectra(seed: int, shape) -> np.ndarray:
     """A smooth positive synthetic target map (stand-in for a measured spectrum)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     return rng.uniform(0.0, 0.5, size=shape).astype(np.float32)
+
+
+def config4(seed: int = 4, n_scenes: int = 16, p_min: int = 5000, p_max: int = 200000, r_min: int = 4,
+            r_max: int = 64, variant: int = 0) -> list:
+    """Uneven scenes: n_scenes independent rooms with Gaussian counts
+    log-uniform in [p_min, p_max] and r_min..r_max receivers (uniform integer),
+    each drawn as in config2.  `variant` changes only the Gaussians (sizes,
+    receivers and transmitter stay), e.g. for the target spectra."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    Ps = np.rint(np.exp(rng.uniform(np.log(p_min), np.log(p_max), size=n_scenes))).astype(int)
+    Rs = rng.integers(r_min, r_max + 1, size=n_scenes)
+    out = []
+    for s in range(n_scenes):
+        rr = np.random.Generator(np.random.PCG64([seed, s]))
+        rx = (0.25 + rr.uniform(size=(int(Rs[s]), 3)) * (ROOM - 0.5)).astype(np.float32)
+        g = np.random.Generator(np.random.PCG64([seed, s, variant + 1]))
+        P = int(Ps[s])
+        means = wall_means(g, P)
+        scales, rots, opac = _common(g, P)
+        mlp = random_mlp(g, P)
+        views = np.stack([view_matrix(r) for r in rx])
+        out.append(Scene(means, scales, rots, opac, mlp, TX_POS.copy(), rx, views))
+    return out
