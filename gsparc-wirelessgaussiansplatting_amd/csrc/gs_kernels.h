@@ -239,23 +239,32 @@ struct SsimArgs {
     float C1, C2;
     const float* img1;
     const float* img2;
-    float* map;
+    float* map;              // nullptr: the map is not stored
     float* dmu1;             // nullptr when not training
     float* ds1;
     float* ds12;
+    float* part;             // nullptr, or per workgroup (sum of the map, sum of |img1 - img2|)
 };
 hipError_t launch_ssim_fwd(const SsimArgs& a, hipStream_t s);
+int ssim_blocks(int planes, int H, int W);
 
 struct SsimBwdArgs {
     int planes, H, W;
     const float* img1;
     const float* img2;
-    const float* dmap;
+    const float* dmap;       // nullptr: every pixel's dL/dmap is dmap_const
     const float* dmu1;
     const float* ds1;
     const float* ds12;
     float* dimg1;
+    float dmap_const;
+    float l1w;               // adds l1w * sign(img1 - img2) to dimg1 (0: nothing)
 };
 hipError_t launch_ssim_bwd(const SsimBwdArgs& a, hipStream_t s);
+
+// loss = l1w * sum|img1 - img2| * inv_n + sw * (1 - sum(map) * inv_n) from
+// the per-workgroup partials of k_ssim_fwd, in a fixed order.
+hipError_t launch_loss_reduce(const float* part, int nb, double inv_n, float l1w, float sw, float* loss,
+                              hipStream_t s);
 
 }  // namespace gs

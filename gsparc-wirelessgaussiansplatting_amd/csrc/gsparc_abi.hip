@@ -313,6 +313,11 @@ int read_count_impl(const gs::Dims& d, const void* image, uint64_t* K, hipStream
 {
     *K = 0;
     if (d.R == 0) return GSPARC_OK;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    GS_HIP(hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone)
+        return fail(GSPARC_ERR_ARG, "the instance count synchronises the stream: not allowed while capturing; "
+                                    "use gsparc_rasterize_forward with a capacity");
     gsparc_layout L;
     make_layout(d, 0, L);
     uint32_t tot = 0;
@@ -658,7 +663,7 @@ int gsparc_ssim_forward(int planes, int H, int W, float C1, float C2, const floa
     if ((size_t)planes * H * W == 0) return GSPARC_OK;
     if (!img1 || !img2 || !ssim_map) return fail(GSPARC_ERR_ARG, "null pointer");
     if (dm_dmu1 && (!dm_dsigma1_sq || !dm_dsigma12)) return fail(GSPARC_ERR_ARG, "partial derivative maps");
-    gs::SsimArgs a{planes, H, W, C1, C2, img1, img2, ssim_map, dm_dmu1, dm_dsigma1_sq, dm_dsigma12};
+    gs::SsimArgs a{planes, H, W, C1, C2, img1, img2, ssim_map, dm_dmu1, dm_dsigma1_sq, dm_dsigma12, nullptr};
     GS_LAUNCH("ssim_fwd", gs::launch_ssim_fwd(a, (hipStream_t)stream), (hipStream_t)stream);
     return GSPARC_OK;
 }
@@ -673,8 +678,95 @@ int gsparc_ssim_backward(int planes, int H, int W, float C1, float C2, const flo
     if ((size_t)planes * H * W == 0) return GSPARC_OK;
     if (!img1 || !img2 || !dL_dmap || !dm_dmu1 || !dm_dsigma1_sq || !dm_dsigma12 || !dL_dimg1)
         return fail(GSPARC_ERR_ARG, "null pointer");
-    gs::SsimBwdArgs a{planes, H, W, img1, img2, dL_dmap, dm_dmu1, dm_dsigma1_sq, dm_dsigma12, dL_dimg1};
+    gs::SsimBwdArgs a{planes, H, W, img1, img2, dL_dmap, dm_dmu1, dm_dsigma1_sq, dm_dsigma12, dL_dimg1, 0.0f, 0.0f};
     GS_LAUNCH("ssim_bwd", gs::launch_ssim_bwd(a, (hipStream_t)stream), (hipStream_t)stream);
+    return GSPARC_OK;
+}
+
+size_t gsparc_loss_work_size(int planes, int H, int W)
+{
+    if (planes <= 0 || H <= 0 || W <= 0) return 0;
+    const size_t n = (size_t)planes * H * W;
+    return 256 + al(3 * n * sizeof(float)) + 2 * sizeof(float) * (size_t)gs::ssim_blocks(planes, H, W);
+}
+
+int gsparc_loss_l1_ssim(int planes, int H, int W, float lambda_ssim, float C1, float C2, const float* img,
+                        const float* target, void* work, float* loss, float* dL_dimg, void* stream)
+{
+    if (planes < 0 || H < 0 || W < 0) return fail(GSPARC_ERR_ARG, "negative size");
+    if (planes > 65535) return fail(GSPARC_ERR_ARG, "at most 65535 planes per call");
+    const size_t n = (size_t)planes * H * W;
+    if (n == 0) return fail(GSPARC_ERR_ARG, "empty images");
+    if (!img || !target || !work || !loss) return fail(GSPARC_ERR_ARG, "null pointer");
+    hipStream_t s = (hipStream_t)stream;
+    char* base = base_of(work);
+    float* dmu1 = dL_dimg ? reinterpret_cast<float*>(base) : nullptr;
+    float* ds1 = dL_dimg ? dmu1 + n : nullptr;
+    float* ds12 = dL_dimg ? ds1 + n : nullptr;
+    float* part = reinterpret_cast<float*>(base + al(3 * n * sizeof(float)));
+    const int nb = gs::ssim_blocks(planes, H, W);
+    const float l1w = 1.0f - lambda_ssim;
+    gs::SsimArgs a{planes, H, W, C1, C2, img, target, nullptr, dmu1, ds1, ds12, part};
+    GS_LAUNCH("ssim_fwd", gs::launch_ssim_fwd(a, s), s);
+    GS_LAUNCH("loss_reduce", gs::launch_loss_reduce(part, nb, 1.0 / (double)n, l1w, lambda_ssim, loss, s), s);
+    if (dL_dimg) {
+        // d/dimg of l1w * mean|img - target| + lambda_ssim * (1 - mean SSIM)
+        const float inv = (float)(1.0 / (double)n);
+        gs::SsimBwdArgs b{planes, H, W, img, target, nullptr, dmu1, ds1, ds12, dL_dimg, -lambda_ssim * inv, l1w * inv};
+        GS_LAUNCH("ssim_bwd", gs::launch_ssim_bwd(b, s), s);
+    }
+    return GSPARC_OK;
+}
+
+// ---- HIP graphs of the library's own launch sequences
+struct gsparc_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int gsparc_graph_begin(void* stream)
+{
+    if (!stream) return fail(GSPARC_ERR_ARG, "capture needs a stream other than the null stream");
+    if (g_prof_on) return fail(GSPARC_ERR_ARG, "disable profiling before capturing a graph");
+    GS_HIP(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+    return GSPARC_OK;
+}
+
+int gsparc_graph_end(void* stream, gsparc_graph** out)
+{
+    if (!stream || !out) return fail(GSPARC_ERR_ARG, "null pointer");
+    *out = nullptr;
+    hipGraph_t g = nullptr;
+    GS_HIP(hipStreamEndCapture((hipStream_t)stream, &g));
+    if (!g) return fail(GSPARC_ERR_HIP, "capture produced no graph");
+    hipGraphExec_t e = nullptr;
+    const hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    if (err != hipSuccess) {
+        hipGraphDestroy(g);
+        return fail(GSPARC_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(err));
+    }
+    gsparc_graph* h = new gsparc_graph;
+    h->graph = g;
+    h->exec = e;
+    *out = h;
+    return GSPARC_OK;
+}
+
+int gsparc_graph_launch(gsparc_graph* graph, void* stream)
+{
+    if (!graph || !graph->exec) return fail(GSPARC_ERR_ARG, "null graph");
+    GS_HIP(hipGraphLaunch(graph->exec, (hipStream_t)stream));
+    return GSPARC_OK;
+}
+
+int gsparc_graph_destroy(gsparc_graph* graph)
+{
+    if (!graph) return GSPARC_OK;
+    hipError_t e1 = graph->exec ? hipGraphExecDestroy(graph->exec) : hipSuccess;
+    hipError_t e2 = graph->graph ? hipGraphDestroy(graph->graph) : hipSuccess;
+    delete graph;
+    GS_HIP(e1);
+    GS_HIP(e2);
     return GSPARC_OK;
 }
 

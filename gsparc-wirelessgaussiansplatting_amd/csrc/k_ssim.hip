@@ -56,6 +56,7 @@ __global__ void __launch_bounds__(SS_THREADS) k_ssim_fwd(SsimArgs a, Taps tp)
         h[0][ly][lx] = m1; h[1][ly][lx] = m2; h[2][ly][lx] = e11; h[3][ly][lx] = e22; h[4][ly][lx] = e12;
     }
     __syncthreads();
+    float sum_map = 0.0f, sum_abs = 0.0f;
     for (int i = tid; i < SS_TH * SS_TW; i += SS_THREADS) {
         const int ly = i / SS_TW, lx = i % SS_TW;
         const int gy = y0 + ly, gx = x0 + lx;
@@ -77,12 +78,34 @@ __global__ void __launch_bounds__(SS_THREADS) k_ssim_fwd(SsimArgs a, Taps tp)
         const float A = mu1 * mu1 + mu2 * mu2 + a.C1;
         const float B = sigma1_sq + sigma2_sq + a.C2;
         const size_t o = plane + (size_t)gy * a.W + gx;
-        a.map[o] = (C * D) / (A * B);
+        const float m = (C * D) / (A * B);
+        if (a.map) a.map[o] = m;
         if (a.dmu1) {
             a.dmu1[o] = (mu2 * 2.0f * D) / (A * B) - (mu2 * 2.0f * C) / (A * B) - (mu1 * 2.0f * C * D) / (A * A * B) +
                         (mu1 * 2.0f * C * D) / (A * B * B);
             a.ds1[o] = (-C * D) / (A * B * B);
             a.ds12[o] = (2.0f * C) / (A * B);
+        }
+        sum_map += m;
+        sum_abs += fabsf(im1[(size_t)gy * a.W + gx] - im2[(size_t)gy * a.W + gx]);
+    }
+    if (a.part) {
+        // fixed-order tree over the workgroup: the loss is the same run to run
+        __shared__ float r0[SS_THREADS], r1[SS_THREADS];
+        r0[tid] = sum_map;
+        r1[tid] = sum_abs;
+        __syncthreads();
+        for (int w = SS_THREADS / 2; w > 0; w >>= 1) {
+            if (tid < w) {
+                r0[tid] += r0[tid + w];
+                r1[tid] += r1[tid + w];
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            const size_t b = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+            a.part[2 * b] = r0[0];
+            a.part[2 * b + 1] = r1[0];
         }
     }
 }
@@ -100,7 +123,7 @@ __global__ void __launch_bounds__(SS_THREADS) k_ssim_bwd(SsimBwdArgs a, Taps tp)
         float p0 = 0.0f, p1 = 0.0f, p2 = 0.0f;
         if (gx >= 0 && gx < a.W && gy >= 0 && gy < a.H) {
             const size_t o = plane + (size_t)gy * a.W + gx;
-            const float g = a.dmap[o];
+            const float g = a.dmap ? a.dmap[o] : a.dmap_const;
             p0 = g * a.dmu1[o];
             p1 = g * a.ds1[o];
             p2 = g * a.ds12[o];
@@ -132,8 +155,38 @@ __global__ void __launch_bounds__(SS_THREADS) k_ssim_bwd(SsimBwdArgs a, Taps tp)
             v[c] = acc;
         }
         const size_t o = plane + (size_t)gy * a.W + gx;
-        a.dimg1[o] = v[0] + a.img1[o] * 2.0f * v[1] + a.img2[o] * v[2];
+        const float x = a.img1[o], y = a.img2[o];
+        float g = v[0] + x * 2.0f * v[1] + y * v[2];
+        if (a.l1w != 0.0f) {
+            // d|x - y|/dx with sign(0) = 0, as torch's abs backward
+            const float sgn = x > y ? 1.0f : (x < y ? -1.0f : 0.0f);
+            g = g + a.l1w * sgn;
+        }
+        a.dimg1[o] = g;
     }
+}
+
+__global__ void __launch_bounds__(256) k_loss_reduce(const float* part, int nb, double inv_n, float l1w, float sw,
+                                                     float* loss)
+{
+    __shared__ double r0[256], r1[256];
+    const int tid = threadIdx.x;
+    double m = 0.0, a = 0.0;
+    for (int b = tid; b < nb; b += 256) {
+        m += (double)part[2 * b];
+        a += (double)part[2 * b + 1];
+    }
+    r0[tid] = m;
+    r1[tid] = a;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+            r0[tid] += r0[tid + w];
+            r1[tid] += r1[tid + w];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) loss[0] = (float)((double)l1w * r1[0] * inv_n + (double)sw * (1.0 - r0[0] * inv_n));
 }
 
 // 11-tap Gaussian window, sigma 1.5 (utils.py:116-124): float32 samples
@@ -158,6 +211,18 @@ hipError_t launch_ssim_fwd(const SsimArgs& a, hipStream_t s)
     static const Taps tp = make_taps();
     dim3 grid((a.W + SS_TW - 1) / SS_TW, (a.H + SS_TH - 1) / SS_TH, a.planes);
     hipLaunchKernelGGL(k_ssim_fwd, grid, dim3(SS_THREADS), 0, s, a, tp);
+    return hipGetLastError();
+}
+
+int ssim_blocks(int planes, int H, int W)
+{
+    return ((W + SS_TW - 1) / SS_TW) * ((H + SS_TH - 1) / SS_TH) * planes;
+}
+
+hipError_t launch_loss_reduce(const float* part, int nb, double inv_n, float l1w, float sw, float* loss,
+                              hipStream_t s)
+{
+    hipLaunchKernelGGL(k_loss_reduce, dim3(1), dim3(256), 0, s, part, nb, inv_n, l1w, sw, loss);
     return hipGetLastError();
 }
 

@@ -77,6 +77,12 @@ _SIGS = {
     "gsparc_mark_visible": (_i, [_i, _vp, _vp, _vp, _vp]),
     "gsparc_ssim_forward": (_i, [_i, _i, _i, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gsparc_ssim_backward": (_i, [_i, _i, _i, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "gsparc_loss_work_size": (C.c_size_t, [_i, _i, _i]),
+    "gsparc_loss_l1_ssim": (_i, [_i, _i, _i, _f, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "gsparc_graph_begin": (_i, [_vp]),
+    "gsparc_graph_end": (_i, [_vp, C.POINTER(_vp)]),
+    "gsparc_graph_launch": (_i, [_vp, _vp]),
+    "gsparc_graph_destroy": (_i, [_vp]),
     "gsparc_profile_enable": (_i, [_i]),
     "gsparc_profile_collect": (_i, [C.POINTER(_i)]),
     "gsparc_profile_query": (_i, [_i, C.c_char_p, _i, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),

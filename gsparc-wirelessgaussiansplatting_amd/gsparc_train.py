@@ -1,0 +1,196 @@
+"""Training-step helpers over the C ABI: the reference's loss as one fused
+call, and a whole forward + loss + backward step captured in a HIP graph.
+
+The reference trains one receiver at a time (train.py:382-440): render, then
+loss = (1 - lambda1) * L1 + lambda1 * (1 - SSIM) with lambda1 = 0.2
+(optimizer.py:8, train.py:418-427), then loss.backward().  Here:
+
+* `l1_ssim_loss(img, target)` computes that loss and its gradient in three
+  kernels (SSIM forward with per-workgroup loss partials, a fixed-order
+  reduction, SSIM backward with the L1 term fused), for R spectra at once.
+* `GraphedTrainingStep` records the sync-free forward (tile lists sized by a
+  capacity, K kept on the device), the loss and the backward once into a HIP
+  graph owned by the library; `replay()` then re-runs the step on whatever
+  the parameter tensors hold, with no host synchronisation and no allocation.
+  The confidence weighting of train.py:429-436 stays with the caller.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+import gsparc_hip as gh
+import wireless_rasterizer as wr
+
+SSIM_C1 = 0.01 ** 2
+SSIM_C2 = 0.03 ** 2
+LAMBDA_SSIM = 0.2          # lambda1 of optimizer.py:8
+
+
+def _work(planes, H, W, dev):
+    n = gh.lib().gsparc_loss_work_size(int(planes), int(H), int(W))
+    return torch.empty(n, dtype=torch.uint8, device=dev)
+
+
+def l1_ssim_loss(img: torch.Tensor, target: torch.Tensor, lambda_ssim: float = LAMBDA_SSIM, want_grad: bool = True):
+    """img, target [R,H,W] (or [H,W]) float32 on the GPU.  Returns (loss [1],
+    dL/dimg like img or None): loss = (1 - lambda_ssim) * mean|img - target| +
+    lambda_ssim * (1 - mean SSIM), the mean over all R*H*W pixels."""
+    x = gh.f32(img)
+    y = gh.f32(target)
+    if x.shape != y.shape or x.dim() not in (2, 3):
+        raise RuntimeError("img and target must both be [H,W] or [R,H,W]")
+    gh.require_cuda(x, y)
+    R = 1 if x.dim() == 2 else x.size(0)
+    H, W = x.shape[-2], x.shape[-1]
+    loss = torch.empty(1, dtype=torch.float32, device=x.device)
+    grad = torch.empty_like(x) if want_grad else None
+    work = _work(R, H, W, x.device)
+    gh.check(gh.lib().gsparc_loss_l1_ssim(R, H, W, float(lambda_ssim), SSIM_C1, SSIM_C2, gh.ptr(x), gh.ptr(y),
+                                          gh.ptr(work), gh.ptr(loss), gh.ptr(grad), gh.stream_handle(x.device)),
+             "l1 + ssim loss")
+    return loss, grad
+
+
+class L1SSIMLoss(torch.autograd.Function):
+    """Autograd form of l1_ssim_loss (gradient to img only, as fused_ssim)."""
+
+    @staticmethod
+    def forward(ctx, img, target, lambda_ssim=LAMBDA_SSIM):
+        loss, grad = l1_ssim_loss(img, target, lambda_ssim, want_grad=img.requires_grad)
+        ctx.save_for_backward(grad if grad is not None else torch.empty(0, device=img.device))
+        return loss.reshape(())
+
+    @staticmethod
+    def backward(ctx, g):
+        (grad,) = ctx.saved_tensors
+        return grad * g, None, None
+
+
+def _require_param(t, name):
+    if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()):
+        raise RuntimeError(f"{name} must be a contiguous float32 GPU tensor (the graph reads it in place)")
+    return t
+
+
+class GraphedTrainingStep:
+    """Forward, loss and backward of one batched training step, captured once
+    and replayed.
+
+    The parameter tensors (means [P,3], scales [P,3], rotations [P,4],
+    opacities [P,1], emission_mlps [P*194]) and the target [R,H,W] are read in
+    place at every replay: update them in place (an optimizer step) between
+    replays.  After replay(): `loss` [1], `out_real` / `out_imag` [R,H,W],
+    `radii` [R,P] and the gradients `grads` (means3D, emission_mlps, opacities,
+    scales, rotations) hold the step's results.  `capacity` bounds the tile
+    lists; `overflowed()` tells whether the last replay needed more (then its
+    spectra and gradients are not valid)."""
+
+    def __init__(self, means, scales, rotations, opacities, emission_mlps, viewmatrices, tx_pos, rx_pos, target,
+                 image_height, image_width, capacity, scale_modifier=1.0, grad_mode="reference",
+                 lambda_ssim=LAMBDA_SSIM):
+        self.params = dict(means=_require_param(means, "means"), scales=_require_param(scales, "scales"),
+                           rotations=_require_param(rotations, "rotations"),
+                           opacities=_require_param(opacities, "opacities"),
+                           emission_mlps=_require_param(emission_mlps, "emission_mlps"))
+        if self.params["emission_mlps"].data_ptr() % 16:
+            raise RuntimeError("emission_mlps must be 16-byte aligned")
+        dev = means.device
+        P = int(means.size(0))
+        self.views = gh.f32(viewmatrices).reshape(-1, 16)
+        self.rx = gh.f32(rx_pos).reshape(-1, 3)
+        self.tx = gh.f32(tx_pos).reshape(3)
+        R = int(self.rx.size(0))
+        H, W = int(image_height), int(image_width)
+        self.target = _require_param(target, "target")
+        if tuple(self.target.shape) != (R, H, W):
+            raise RuntimeError("target must be [R,H,W]")
+        if self.params["emission_mlps"].numel() != P * 194:
+            raise RuntimeError("emission_mlps must hold 194 floats per Gaussian")
+        self.P, self.R, self.H, self.W = P, R, H, W
+        self.capacity = int(capacity)
+        self.mode = gh.GRAD_MODES[grad_mode] if isinstance(grad_mode, str) else int(grad_mode)
+        self.scale_modifier = float(scale_modifier)
+        self.lambda_ssim = float(lambda_ssim)
+        L = gh.lib()
+        f32 = dict(dtype=torch.float32, device=dev)
+        u8 = lambda n: torch.empty(int(n), dtype=torch.uint8, device=dev)
+        self.geom = u8(L.gsparc_geom_buffer_size(P, R, W, H))
+        self.img = u8(L.gsparc_image_buffer_size(P, R, W, H))
+        self.bin = u8(L.gsparc_binning_buffer_size(self.capacity))
+        self.gradbuf = u8(L.gsparc_grad_buffer_size(P, R))
+        self.work = _work(R, H, W, dev)
+        self.radii = torch.empty((R, P), dtype=torch.int32, device=dev)
+        self.out_real = torch.empty((R, H, W), **f32)
+        self.out_imag = torch.empty((R, H, W), **f32)
+        self.loss = torch.empty(1, **f32)
+        self.dimg = torch.empty((R, H, W), **f32)
+        self.zero_imag = torch.zeros((R, H, W), **f32)
+        self.grads = dict(means=torch.empty((P, 3), **f32), emission_mlps=torch.empty(P * 194, **f32),
+                          opacities=torch.empty((P, 1), **f32), scales=torch.empty((P, 3), **f32),
+                          rotations=torch.empty((P, 4), **f32))
+        self.graph = C.c_void_p(None)
+        self.stream = torch.cuda.Stream(device=dev)
+        cur = torch.cuda.current_stream(dev)
+        self.stream.wait_stream(cur)
+        with torch.cuda.stream(self.stream):
+            h = self.stream.cuda_stream
+            self._record(h)                       # one eager run: checks every argument
+            gh.check(L.gsparc_graph_begin(h), "graph begin")
+            try:
+                self._record(h)
+            finally:
+                gh.check(L.gsparc_graph_end(h, C.byref(self.graph)), "graph end")
+        cur.wait_stream(self.stream)
+
+    def _record(self, stream):
+        L = gh.lib()
+        p = self.params
+        P, R, W, H = self.P, self.R, self.W, self.H
+        gh.check(L.gsparc_rasterize_forward(P, R, W, H, gh.ptr(p["means"]), gh.ptr(p["scales"]),
+                                            gh.ptr(p["rotations"]), gh.ptr(p["opacities"]),
+                                            gh.ptr(p["emission_mlps"]), None, gh.ptr(self.views), gh.ptr(self.tx),
+                                            gh.ptr(self.rx), self.scale_modifier, gh.ptr(self.geom),
+                                            gh.ptr(self.img), gh.ptr(self.bin), self.capacity, gh.ptr(self.radii),
+                                            gh.ptr(self.out_real), gh.ptr(self.out_imag), 0, stream),
+                 "graphed step: forward")
+        gh.check(L.gsparc_loss_l1_ssim(R, H, W, self.lambda_ssim, SSIM_C1, SSIM_C2, gh.ptr(self.out_real),
+                                       gh.ptr(self.target), gh.ptr(self.work), gh.ptr(self.loss), gh.ptr(self.dimg),
+                                       stream), "graphed step: loss")
+        g = self.grads
+        gh.check(L.gsparc_rasterize_backward(self.mode, P, R, W, H, gh.ptr(p["means"]), gh.ptr(p["scales"]),
+                                             gh.ptr(p["rotations"]), gh.ptr(p["emission_mlps"]), None,
+                                             gh.ptr(self.views), gh.ptr(self.tx), gh.ptr(self.rx),
+                                             self.scale_modifier, gh.ptr(self.radii), gh.ptr(self.geom),
+                                             gh.ptr(self.img), gh.ptr(self.bin), self.capacity, gh.ptr(self.gradbuf),
+                                             gh.ptr(self.dimg), gh.ptr(self.zero_imag), None, gh.ptr(g["means"]),
+                                             gh.ptr(g["emission_mlps"]), gh.ptr(g["opacities"]), None,
+                                             gh.ptr(g["scales"]), gh.ptr(g["rotations"]), None, stream),
+                 "graphed step: backward")
+
+    def replay(self):
+        """Launch the captured step on the current stream."""
+        gh.check(gh.lib().gsparc_graph_launch(self.graph, gh.stream_handle(self.target.device)), "graph launch")
+
+    def run_eager(self):
+        """The same launch sequence without the graph, on the current stream."""
+        self._record(gh.stream_handle(self.target.device))
+
+    def instance_count(self) -> int:
+        """K of the last step (synchronises the current stream)."""
+        return wr.instance_count(self.P, self.R, self.H, self.W, self.img)
+
+    def overflowed(self) -> bool:
+        return self.instance_count() > self.capacity
+
+    def close(self):
+        if self.graph:
+            gh.lib().gsparc_graph_destroy(self.graph)
+            self.graph = C.c_void_p(None)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -157,6 +157,38 @@ int gsparc_ssim_backward(int planes, int H, int W, float C1, float C2, const flo
                          const float* dL_dmap, const float* dm_dmu1, const float* dm_dsigma1_sq,
                          const float* dm_dsigma12, float* dL_dimg1, void* stream);
 
+/* Training loss of the reference, value and gradient in one call:
+ *   loss = (1 - lambda_ssim) * mean|img - target| + lambda_ssim * (1 - mean SSIM(img, target))
+ * over `planes` planes of H x W (one per receiver spectrum), 11x11 Gaussian
+ * window (sigma 1.5), zero padding ("same"), constants C1 / C2; loss [1] on
+ * the device.  dL_dimg (may be NULL: loss only) receives the gradient, with
+ * sign(0) = 0 for the L1 term.  work: gsparc_loss_work_size bytes.  The mean
+ * runs over all planes, so a batch of R spectra gives the mean of their
+ * per-spectrum losses.  Replaces the loss composition of train.py:418-427
+ * (l1_loss, utils.py:40-41; fused_ssim, fused_ssim/__init__.py:34-41;
+ * lambda1 = 0.2, optimizer.py:8), which runs as separate torch kernels there.
+ * The confidence weighting of train.py:429-436 is left to the caller. */
+size_t gsparc_loss_work_size(int planes, int H, int W);
+int gsparc_loss_l1_ssim(int planes, int H, int W, float lambda_ssim, float C1, float C2, const float* img,
+                        const float* target, void* work, float* loss, float* dL_dimg, void* stream);
+
+/* HIP graphs of this library's launch sequences.  Between begin and end,
+ * every call above that does not synchronise (gsparc_rasterize_forward with a
+ * capacity, gsparc_rasterize_backward, the SSIM and loss calls, the _scenes
+ * variants without _geom) is recorded on `stream` instead of run; end
+ * instantiates the graph, launch replays it on any stream with the pointers
+ * and sizes given at capture.  The calls that read K back
+ * (gsparc_rasterize_forward_geom, gsparc_instance_count) fail while
+ * capturing.  Profiling must be off.  Capture is thread-local: the capturing
+ * thread may not allocate device memory in between.  No counterpart in the
+ * reference (one synchronising launch sequence per call,
+ * rasterizer_impl.cu:206-335); for training loops that replay one step. */
+typedef struct gsparc_graph gsparc_graph;
+int gsparc_graph_begin(void* stream);
+int gsparc_graph_end(void* stream, gsparc_graph** graph);
+int gsparc_graph_launch(gsparc_graph* graph, void* stream);
+int gsparc_graph_destroy(gsparc_graph* graph);
+
 /* Per-kernel timing.  While enabled, every kernel launched by the calls above
  * is bracketed by HIP events on its own stream; collect() waits for them,
  * adds the elapsed times to per-kernel totals and returns how many kernels

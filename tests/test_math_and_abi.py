@@ -65,7 +65,7 @@ def test_atan2f_accuracy():
 
 def test_library_exports_every_declared_symbol():
     header = open(os.path.join(ROOT, "include", "gsparc.h")).read()
-    declared = set(re.findall(r"\b(gsparc_[a-z_]+)\s*\(", header))
+    declared = set(re.findall(r"\b(gsparc_\w+)\s*\(", header))
     import gsparc_hip as gh
     lib = ctypes.CDLL(gh.LIB_PATH)
     for name in declared:
