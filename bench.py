@@ -6,6 +6,9 @@ Workload (one "step"): 50,000 Gaussians (70% within 0.3 m of the room walls),
 loss 0.8*L1 + 0.2*(1 - SSIM) (11x11 window, sigma 1.5) against target spectra
 rendered from a second seeded scene; full backward to means, scales,
 rotations, opacities and MLP weights.  All inputs resident on the GPU.
+The timed step is the library's HIP graph of forward (tile lists sized by a
+capacity), fused loss and backward, replayed; the eager torch-autograd form
+of the same step is timed and reported beside it (--eager times only that).
 
 Prints ONE JSON line.  Also reported: the same step done the plain way (one
 receiver per call, the reference's calling pattern), p50/p99 single-spectrum
@@ -53,6 +56,8 @@ def parse():
     ap.add_argument("--order", type=int, default=0, help="blend work order (0: automatic)")
     ap.add_argument("--workload", default="receivers", choices=["receivers", "scenes"],
                     help="receivers: one scene, R receivers (default); scenes: 16 uneven scenes in one batch")
+    ap.add_argument("--eager", action="store_true",
+                    help="time the eager autograd step instead of the HIP-graph-captured one")
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
 
@@ -171,6 +176,16 @@ def setup_receivers(args, dev):
     def count():
         return wr.forward_batched(means, mlp, opac, scales, rots, sc.H, sc.W, 1.0, views, txp, rxp)[0]
 
+    def graphed():
+        """The same step (forward, fused loss, backward) captured in a HIP
+        graph; tile lists sized for 1.25 K."""
+        import gsparc_train as T
+        cap = int(count() * 1.25) + 1024
+        live = [x.clone() for x in (means, scales, rots, opac, mlp)]
+        g = T.GraphedTrainingStep(*live, views, txp, rxp, target, sc.H, sc.W, capacity=cap,
+                                  grad_mode=args.grad_mode)
+        return g, live
+
     def cpu():
         nr = min(args.cpu_receivers, R)
         tg = target[:nr].cpu().numpy()
@@ -178,7 +193,8 @@ def setup_receivers(args, dev):
                                  f"{nr} of the {R} receivers of the same {P // 1000}k-Gaussian scene")
 
     return Workload(
-        step=step, step_plain=step_plain, single=single, count=count, cpu=cpu, spectra=R, Ptot=P, Nrec=P * R,
+        step=step, step_plain=step_plain, single=single, count=count, cpu=cpu, graphed=graphed, spectra=R, Ptot=P,
+        Nrec=P * R,
         H=sc.H, W=sc.W, traffic_key=(P, R),
         data="synthetic (seeded: 70% wall-clustered Gaussians, receivers uniform in a 6x6x3 m room)",
         config={"workload": "batched receivers, fwd+bwd, 0.8*L1 + 0.2*(1-SSIM)", "gaussians": P, "receivers": R,
@@ -264,7 +280,8 @@ def setup_scenes(args, dev):
                                                    f"(P = {sorted(sc.P for sc in scs)[:3]}...)")
 
     return Workload(
-        step=step, step_plain=step_plain, single=single, count=count, cpu=cpu, spectra=batch.R, Ptot=batch.Ptot,
+        step=step, step_plain=step_plain, single=single, count=count, cpu=cpu, graphed=None, spectra=batch.R,
+        Ptot=batch.Ptot,
         Nrec=batch.Nrec, H=H, W=W, traffic_key=None,
         data="synthetic (seeded: 16 rooms, 70% wall-clustered Gaussians, receivers uniform in each room)",
         config={"workload": "uneven scene batch, fwd+bwd, 0.8*L1 + 0.2*(1-SSIM)", "scenes": len(scs),
@@ -305,11 +322,27 @@ def main():
     for _ in range(args.warmup):
         wl.step()
     torch.cuda.synchronize()
-    log(args, f"timing {args.steps} batched steps")
-    el = timed(wl.step, args.steps)
+    log(args, f"timing {args.steps} eager batched steps")
+    el_eager = timed(wl.step, args.steps)
+    eager_value = R * args.steps / el_eager
+    log(args, f"eager batched: {1e3 * el_eager / args.steps:.2f} ms/step, {eager_value:.1f} spectra/s")
+    graph_value = None
+    if wl.graphed is not None and not args.eager:
+        gstep, _live = wl.graphed()
+        for _ in range(args.warmup):
+            gstep.replay()
+        torch.cuda.synchronize()
+        if gstep.overflowed():
+            raise RuntimeError("graphed step: tile lists exceeded their capacity")
+        log(args, f"timing {args.steps} graph-captured batched steps")
+        el_graph = timed(gstep.replay, args.steps)
+        graph_value = R * args.steps / el_graph
+        log(args, f"graphed batched: {1e3 * el_graph / args.steps:.2f} ms/step, {graph_value:.1f} spectra/s")
+        gstep.close()
+        del gstep, _live
+    el = el_graph if graph_value is not None else el_eager
     ms_per_step = 1e3 * el / args.steps
     value = R * args.steps / el
-    log(args, f"batched: {ms_per_step:.2f} ms/step, {value:.1f} spectra/s")
 
     step_lat = []
     if args.workload == "scenes":
@@ -420,6 +453,9 @@ def main():
         "dtype": "fp32",
         "data": wl.data,
         "config": wl.config,
+        "step": "hip graph: forward, fused loss, backward" if graph_value is not None else
+                "eager: torch autograd over the rasterizer and fused SSIM",
+        "eager_spectra_per_s": round(eager_value, 2),
         "plain_spectra_per_s": None if plain_value is None else round(plain_value, 2),
         "batched_over_plain": None if plain_value is None else round(value / plain_value, 2),
         "single_spectrum_fwd_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
