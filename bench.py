@@ -56,6 +56,7 @@ def parse():
     ap.add_argument("--order", type=int, default=0, help="blend work order (0: automatic)")
     ap.add_argument("--workload", default="receivers", choices=["receivers", "scenes"],
                     help="receivers: one scene, R receivers (default); scenes: 16 uneven scenes in one batch")
+    ap.add_argument("--no-tune", action="store_true", help="skip the blend-mapping autotuner")
     ap.add_argument("--eager", action="store_true",
                     help="time the eager autograd step instead of the HIP-graph-captured one")
     ap.add_argument("--quiet", action="store_true")
@@ -176,6 +177,11 @@ def setup_receivers(args, dev):
     def count():
         return wr.forward_batched(means, mlp, opac, scales, rots, sc.H, sc.W, 1.0, views, txp, rxp)[0]
 
+    def tune():
+        with torch.no_grad():
+            out = wr.forward_batched(means, mlp, opac, scales, rots, sc.H, sc.W, 1.0, views, txp, rxp)
+        return wr.autotune(out, grad_mode=args.grad_mode)
+
     def graphed():
         """The same step (forward, fused loss, backward) captured in a HIP
         graph; tile lists sized for 1.25 K."""
@@ -193,7 +199,8 @@ def setup_receivers(args, dev):
                                  f"{nr} of the {R} receivers of the same {P // 1000}k-Gaussian scene")
 
     return Workload(
-        step=step, step_plain=step_plain, single=single, count=count, cpu=cpu, graphed=graphed, spectra=R, Ptot=P,
+        step=step, step_plain=step_plain, single=single, count=count, cpu=cpu, graphed=graphed, tune=tune,
+        spectra=R, Ptot=P,
         Nrec=P * R,
         H=sc.H, W=sc.W, traffic_key=(P, R),
         data="synthetic (seeded: 70% wall-clustered Gaussians, receivers uniform in a 6x6x3 m room)",
@@ -265,6 +272,12 @@ def setup_scenes(args, dev):
         m, s_, q, o, w = packed
         return wr.forward_scenes(m, w, o, s_, q, H, W, 1.0, views, txp, rxp, batch)[0]
 
+    def tune():
+        m, s_, q, o, w = packed
+        with torch.no_grad():
+            out = wr.forward_scenes(m, w, o, s_, q, H, W, 1.0, views, txp, rxp, batch)
+        return wr.autotune(out, grad_mode=args.grad_mode, scenes=batch)
+
     def cpu():
         # the smallest scene's receivers, in order, up to the sample size
         order = np.argsort([sc.P for sc in scs])
@@ -280,7 +293,8 @@ def setup_scenes(args, dev):
                                                    f"(P = {sorted(sc.P for sc in scs)[:3]}...)")
 
     return Workload(
-        step=step, step_plain=step_plain, single=single, count=count, cpu=cpu, graphed=None, spectra=batch.R,
+        step=step, step_plain=step_plain, single=single, count=count, cpu=cpu, graphed=None, tune=tune,
+        spectra=batch.R,
         Ptot=batch.Ptot,
         Nrec=batch.Nrec, H=H, W=W, traffic_key=None,
         data="synthetic (seeded: 16 rooms, 70% wall-clustered Gaussians, receivers uniform in each room)",
@@ -309,6 +323,10 @@ def main():
     gh.set_option("render_order", args.order)
     wl = setup_receivers(args, dev) if args.workload == "receivers" else setup_scenes(args, dev)
     R, Ptot, Nrec = wl.spectra, wl.Ptot, wl.Nrec
+    tuned = None
+    if not args.no_tune and not (args.fwd_ppl or args.bwd_ppl or args.order):
+        tuned = wl.tune()
+        log(args, f"autotuned blend mapping: {tuned}")
 
     def timed(fn, steps):
         torch.cuda.synchronize()
@@ -376,13 +394,18 @@ def main():
         torch.cuda.synchronize()
         log(args, f"calibration copy of {args.copy_mib} MiB done")
 
-    # per-kernel times with HIP events on the launch stream
-    gh.profile_enable(True)
+    # per-kernel times with HIP events on the launch stream, then (separately:
+    # the counting variants are slower) the blend kernels' pair counts
     nprof = 3
+    gh.profile_enable(True, timing=True, counting=False)
     for _ in range(nprof):
         wl.step()
     torch.cuda.synchronize()
     kt = gh.profile_collect()
+    gh.profile_enable(True, timing=False, counting=True)
+    for _ in range(nprof):
+        wl.step()
+    torch.cuda.synchronize()
     cnt = {k: v / nprof for k, v in gh.profile_counts().items()}
     gh.profile_enable(False)
     with torch.no_grad():
@@ -461,6 +484,7 @@ def main():
         "single_spectrum_fwd_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                                    "p99": round(float(np.percentile(lat, 99)), 3)},
         "K_instances": int(K),
+        "autotune": tuned,
         "roofline": roof,
         "kernels": kernels,
         "cpu_baseline": cpu,

@@ -8,6 +8,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/gsparc.h"
@@ -19,17 +20,56 @@ namespace {
 thread_local std::string g_err;
 
 // tuning options (gsparc_set_option): pixels per lane of the forward and
-// backward blend; 0 = automatic (forward 1; backward 4 when the batch has
-// enough tiles to fill the GPU with one wave per tile)
+// backward blend and the blend work order; 0 = automatic (what the
+// autotuner chose for this batch shape, else the built-in rule)
 int g_opt_fwd_ppl = 0;
 int g_opt_bwd_ppl = 0;
 int g_opt_order = 0;
 
+// Autotuned blend mappings, per batch shape (gsparc_autotune).
+struct TuneKey {
+    int P, R, W, H, S;
+    bool operator<(const TuneKey& o) const
+    {
+        return std::tie(P, R, W, H, S) < std::tie(o.P, o.R, o.W, o.H, o.S);
+    }
+};
+struct TuneVal { int fwd_ppl, bwd_ppl, order; };
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneVal> g_tuned;
+
+TuneKey tune_key(const gs::Dims& d) { return {d.Ptot, d.R, d.W, d.H, d.S}; }
+
+bool tuned_for(const gs::Dims& d, TuneVal& v)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(tune_key(d));
+    if (it == g_tuned.end()) return false;
+    v = it->second;
+    return true;
+}
+
 int choose_order(const gs::Dims& d)
 {
     if (g_opt_order >= 1 && g_opt_order <= 3) return g_opt_order;
+    TuneVal v;
+    if (tuned_for(d, v)) return v.order;
     // per-XCD receiver sets once every XCD gets at least two receivers
     return d.R >= 16 ? gs::ORDER_XCD_MIXED : gs::ORDER_GLOBAL;
+}
+
+int fwd_ppl(const gs::Dims& d)
+{
+    TuneVal v;
+    if (g_opt_fwd_ppl == 0 && tuned_for(d, v)) return v.fwd_ppl;
+    return gs::choose_ppl(d, g_opt_fwd_ppl);
+}
+
+int bwd_ppl(const gs::Dims& d)
+{
+    TuneVal v;
+    if (g_opt_bwd_ppl == 0 && tuned_for(d, v)) return v.bwd_ppl;
+    return gs::choose_ppl(d, g_opt_bwd_ppl);
 }
 
 int fail(int code, const std::string& msg)
@@ -147,7 +187,8 @@ struct ProfPending {
     hipEvent_t a, b;
 };
 std::mutex g_prof_mu;
-bool g_prof_on = false;
+bool g_prof_on = false;      // per-kernel HIP-event timing
+bool g_prof_count = false;   // blend kernels' pair / entry counters (their counting variants)
 std::vector<ProfPending> g_prof_pending;
 std::vector<hipEvent_t> g_prof_free;
 std::map<std::string, std::pair<double, long long>> g_prof_acc;
@@ -356,8 +397,8 @@ int forward_render_impl(const gs::Dims& d, const float* emission, void* geom, vo
     ra.ncontrib = (uint32_t*)(ib + L.n_contrib);
     ra.total = ba.total;
     ra.Kcap = K_capacity;
-    ra.counts = g_prof_on ? g_counts_dev : nullptr;
-    GS_LAUNCH("render_fwd", gs::launch_render_fwd(d, ra, gs::choose_ppl(d, g_opt_fwd_ppl), s), s);
+    ra.counts = g_prof_count ? g_counts_dev : nullptr;
+    GS_LAUNCH("render_fwd", gs::launch_render_fwd(d, ra, fwd_ppl(d), s), s);
     return GSPARC_OK;
 }
 
@@ -397,8 +438,8 @@ int backward_impl(int grad_mode, const gs::Dims& d, const FwdInputs& in, const f
     ra.mode = grad_mode;
     ra.total = (const uint32_t*)(ib + L.total);
     ra.Kcap = K_capacity;
-    ra.counts = g_prof_on ? g_counts_dev : nullptr;
-    GS_LAUNCH("render_bwd", gs::launch_render_bwd(d, ra, gs::choose_ppl(d, g_opt_bwd_ppl), s), s);
+    ra.counts = g_prof_count ? g_counts_dev : nullptr;
+    GS_LAUNCH("render_bwd", gs::launch_render_bwd(d, ra, bwd_ppl(d), s), s);
     gs::PreBwdArgs pa;
     pa.means = in.means; pa.scales = in.scales; pa.rots = in.rotations; pa.mlp = in.mlp; pa.views = in.views;
     pa.sig = (const float2*)(gb + L.sig); pa.cov3 = (const float*)(gb + L.cov3);
@@ -413,6 +454,129 @@ int backward_impl(int grad_mode, const gs::Dims& d, const FwdInputs& in, const f
         pa.part = (float*)((char*)g8 + part_off);
     }
     GS_LAUNCH("preprocess_bwd", gs::launch_preprocess_bwd(d, pa, s), s);
+    return GSPARC_OK;
+}
+
+// Times every blend mapping on the buffers of a finished forward and keeps
+// the fastest per batch shape.  Each timed launch runs where it runs in a
+// training step: the forward blend right after the ordered scatter that
+// writes its tile lists, the backward blend right after a forward blend and
+// the gradient clear, so caches hold what they hold in the step (back-to-back
+// launches of one kernel would keep its lists hot and favour the mappings
+// that re-read them).  The forward result does not depend on the mapping
+// (bit-identical outputs); the backward only by summation order.
+int autotune_impl(int grad_mode, const gs::Dims& d, const float* emission, const int* radii, void* geom,
+                  void* image, void* bin, uint64_t K_capacity, void* grad, const float* dout_real,
+                  const float* dout_imag, float* out_real, float* out_imag, int iters, int* choice, float* ms,
+                  float* table, hipStream_t s)
+{
+    if (grad_mode != GSPARC_GRAD_EXACT && grad_mode != GSPARC_GRAD_REFERENCE)
+        return fail(GSPARC_ERR_ARG, "unknown grad_mode");
+    if (!geom || !image || !bin || !grad || !radii || !dout_real || !dout_imag || !out_real || !out_imag ||
+        !choice)
+        return fail(GSPARC_ERR_ARG, "null pointer");
+    if (d.R == 0 || d.Ptot == 0) return fail(GSPARC_ERR_ARG, "nothing to tune");
+    if (g_prof_on || g_prof_count) return fail(GSPARC_ERR_ARG, "disable profiling before tuning");
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    GS_HIP(hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone) return fail(GSPARC_ERR_ARG, "tuning times kernels: not while capturing");
+    iters = std::max(1, std::min(iters, 100));
+    gsparc_layout L;
+    make_layout(d, K_capacity, L);
+    char* gb = base_of(geom);
+    char* ib = base_of(image);
+    char* bb = base_of(bin);
+    float* g8 = (float*)base_of(grad);
+    gs::BinArgs sa = bin_args(d, L, gb, ib, radii);
+    sa.plist = (uint32_t*)(bb + L.plist);
+    sa.keys_sorted = nullptr;
+    sa.Kcap = K_capacity;
+    uint32_t* order = (uint32_t*)(ib + L.tile_order);
+    gs::RenderFwdArgs fa;
+    fa.ranges = sa.ranges; fa.order = order; fa.plist = sa.plist; fa.splat = sa.splat;
+    fa.sig = emission ? (const float2*)emission : (const float2*)(gb + L.sig);
+    fa.out_re = out_real; fa.out_im = out_imag;
+    fa.finalT = (float*)(ib + L.final_t); fa.ncontrib = (uint32_t*)(ib + L.n_contrib);
+    fa.total = sa.total; fa.Kcap = K_capacity; fa.counts = nullptr;
+    gs::RenderBwdArgs ba;
+    ba.ranges = sa.ranges; ba.order = order; ba.plist = sa.plist; ba.splat = sa.splat; ba.sig = fa.sig;
+    ba.finalT = fa.finalT; ba.ncontrib = fa.ncontrib; ba.dout_re = dout_real; ba.dout_im = dout_imag; ba.g8 = g8;
+    ba.mode = grad_mode; ba.total = sa.total; ba.Kcap = K_capacity; ba.counts = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    GS_HIP(hipEventCreate(&ev[0]));
+    if (hipEventCreate(&ev[1]) != hipSuccess) {
+        hipEventDestroy(ev[0]);
+        return fail(GSPARC_ERR_HIP, "hipEventCreate");
+    }
+    // median of `iters` single timed launches, each after its predecessors
+    auto time_it = [&](auto&& before, auto&& launch, float& out_ms) -> hipError_t {
+        std::vector<float> t(iters + 1);
+        hipError_t e = hipSuccess;
+        for (int i = 0; i <= iters && e == hipSuccess; ++i) {   // launch 0 warms up
+            e = before();
+            if (e == hipSuccess) e = hipEventRecord(ev[0], s);
+            if (e == hipSuccess) e = launch();
+            if (e == hipSuccess) e = hipEventRecord(ev[1], s);
+            if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+            if (e == hipSuccess) e = hipEventElapsedTime(&t[i], ev[0], ev[1]);
+        }
+        std::sort(t.begin() + 1, t.end());
+        out_ms = t[1 + iters / 2];
+        return e;
+    };
+    const int ppls[3] = {1, 2, 4};
+    TuneVal best{1, 1, 1};
+    float best_f = 0.0f, best_b = 0.0f, best_total = 1e30f;
+    hipError_t err = hipSuccess;
+    auto scatter = [&] { return gs::launch_chunk_scatter(d, sa, s); };
+    for (int mode = 1; mode <= 3 && err == hipSuccess; ++mode) {
+        err = gs::launch_tile_order(d, mode, sa.ranges, order, sa.total, s);
+        float bf = 1e30f, bbk = 1e30f;
+        int pf = 1, pb = 1;
+        for (int k = 0; k < 3 && err == hipSuccess; ++k) {
+            float t = 0.0f;
+            err = time_it(scatter, [&] { return gs::launch_render_fwd(d, fa, ppls[k], s); }, t);
+            if (table) table[(mode - 1) * 6 + k] = t;
+            if (t < bf) { bf = t; pf = ppls[k]; }
+        }
+        for (int k = 0; k < 3 && err == hipSuccess; ++k) {
+            float t = 0.0f;
+            auto before = [&] {
+                hipError_t e = scatter();
+                if (e == hipSuccess) e = gs::launch_render_fwd(d, fa, pf, s);
+                if (e == hipSuccess) e = gs::launch_fill_u32((uint32_t*)g8, 0u, d.Nrec * 8, s);
+                return e;
+            };
+            err = time_it(before, [&] { return gs::launch_render_bwd(d, ba, ppls[k], s); }, t);
+            if (table) table[(mode - 1) * 6 + 3 + k] = t;
+            if (t < bbk) { bbk = t; pb = ppls[k]; }
+        }
+        if (err == hipSuccess && bf + bbk < best_total) {
+            best_total = bf + bbk;
+            best = {pf, pb, mode};
+            best_f = bf;
+            best_b = bbk;
+        }
+    }
+    // leave the buffers as the chosen mapping's forward would
+    if (err == hipSuccess) err = gs::launch_tile_order(d, best.order, sa.ranges, order, sa.total, s);
+    if (err == hipSuccess) err = scatter();
+    if (err == hipSuccess) err = gs::launch_render_fwd(d, fa, best.fwd_ppl, s);
+    if (err == hipSuccess) err = hipStreamSynchronize(s);
+    hipEventDestroy(ev[0]);
+    hipEventDestroy(ev[1]);
+    if (err != hipSuccess) return fail(GSPARC_ERR_HIP, std::string("autotune: ") + hipGetErrorString(err));
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[tune_key(d)] = best;
+    }
+    choice[0] = best.fwd_ppl;
+    choice[1] = best.bwd_ppl;
+    choice[2] = best.order;
+    if (ms) {
+        ms[0] = best_f;
+        ms[1] = best_b;
+    }
     return GSPARC_OK;
 }
 
@@ -683,6 +847,35 @@ int gsparc_ssim_backward(int planes, int H, int W, float C1, float C2, const flo
     return GSPARC_OK;
 }
 
+int gsparc_autotune(int grad_mode, int P, int R, int W, int H, const float* emission, const int* radii,
+                    void* geom, void* image, void* bin, uint64_t K_capacity, void* grad, const float* dout_real,
+                    const float* dout_imag, float* out_real, float* out_imag, int iters, int* choice, float* ms,
+                    float* table, void* stream)
+{
+    if (int e = check_dims(P, R, W, H)) return e;
+    return autotune_impl(grad_mode, gs::make_dims(P, R, W, H), emission, radii, geom, image, bin, K_capacity, grad,
+                         dout_real, dout_imag, out_real, out_imag, iters, choice, ms, table, (hipStream_t)stream);
+}
+
+int gsparc_autotune_scenes(int grad_mode, const gsparc_batch* batch, const uint32_t* scene_table, int W, int H,
+                           const float* emission, const int* radii, void* geom, void* image, void* bin,
+                           uint64_t K_capacity, void* grad, const float* dout_real, const float* dout_imag,
+                           float* out_real, float* out_imag, int iters, int* choice, float* ms, float* table,
+                           void* stream)
+{
+    gs::Dims d;
+    if (int e = scene_dims(batch, scene_table, W, H, d)) return e;
+    return autotune_impl(grad_mode, d, emission, radii, geom, image, bin, K_capacity, grad, dout_real, dout_imag,
+                         out_real, out_imag, iters, choice, ms, table, (hipStream_t)stream);
+}
+
+int gsparc_autotune_clear(void)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned.clear();
+    return GSPARC_OK;
+}
+
 size_t gsparc_loss_work_size(int planes, int H, int W)
 {
     if (planes <= 0 || H <= 0 || W <= 0) return 0;
@@ -727,7 +920,7 @@ struct gsparc_graph {
 int gsparc_graph_begin(void* stream)
 {
     if (!stream) return fail(GSPARC_ERR_ARG, "capture needs a stream other than the null stream");
-    if (g_prof_on) return fail(GSPARC_ERR_ARG, "disable profiling before capturing a graph");
+    if (g_prof_on || g_prof_count) return fail(GSPARC_ERR_ARG, "disable profiling before capturing a graph");
     GS_HIP(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
     return GSPARC_OK;
 }
@@ -779,9 +972,12 @@ int gsparc_profile_enable(int on)
     }
     g_prof_pending.clear();
     g_prof_acc.clear();
-    if (on && !g_counts_dev) GS_HIP(hipMalloc((void**)&g_counts_dev, 4 * sizeof(unsigned long long)));
-    if (on) GS_HIP(hipMemset(g_counts_dev, 0, 4 * sizeof(unsigned long long)));
-    g_prof_on = on != 0;
+    if (on < 0 || on > 3) return fail(GSPARC_ERR_ARG, "profile mode must be 0..3");
+    const bool count = on == 1 || on == 3;
+    if (count && !g_counts_dev) GS_HIP(hipMalloc((void**)&g_counts_dev, 4 * sizeof(unsigned long long)));
+    if (count) GS_HIP(hipMemset(g_counts_dev, 0, 4 * sizeof(unsigned long long)));
+    g_prof_on = on == 1 || on == 2;
+    g_prof_count = count;
     return GSPARC_OK;
 }
 

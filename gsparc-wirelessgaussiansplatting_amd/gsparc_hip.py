@@ -77,6 +77,11 @@ _SIGS = {
     "gsparc_mark_visible": (_i, [_i, _vp, _vp, _vp, _vp]),
     "gsparc_ssim_forward": (_i, [_i, _i, _i, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gsparc_ssim_backward": (_i, [_i, _i, _i, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "gsparc_autotune": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, C.c_uint64, _vp, _vp, _vp, _vp, _vp, _i,
+                             C.POINTER(_i), C.POINTER(_f), C.POINTER(_f), _vp]),
+    "gsparc_autotune_scenes": (_i, [_i, _BP, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp, C.c_uint64, _vp, _vp, _vp, _vp,
+                                    _vp, _i, C.POINTER(_i), C.POINTER(_f), C.POINTER(_f), _vp]),
+    "gsparc_autotune_clear": (_i, []),
     "gsparc_loss_work_size": (C.c_size_t, [_i, _i, _i]),
     "gsparc_loss_l1_ssim": (_i, [_i, _i, _i, _f, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gsparc_graph_begin": (_i, [_vp]),
@@ -136,8 +141,11 @@ def get_option(name: str) -> int:
     return v.value
 
 
-def profile_enable(on: bool = True):
-    check(lib().gsparc_profile_enable(1 if on else 0), "gsparc_profile_enable")
+def profile_enable(on: bool = True, timing: bool = True, counting: bool = True):
+    """Per-kernel HIP-event timing and/or the blend kernels' counters (the
+    counting variants are slower, so time and count in separate passes)."""
+    mode = 0 if not on else (1 if timing and counting else (2 if timing else (3 if counting else 0)))
+    check(lib().gsparc_profile_enable(mode), "gsparc_profile_enable")
 
 
 def profile_collect() -> dict:

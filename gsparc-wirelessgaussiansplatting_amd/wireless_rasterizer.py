@@ -370,3 +370,50 @@ def unpack_mlps(packed, scene_P):
             out[s].append(packed[off:off + n])
             off += n
     return [torch.cat(o) for o in out]
+
+
+def autotune(forward_out, grad_mode="reference", iters=5, emission=None, scenes: SceneBatch | None = None):
+    """Tune the blend kernels' mapping (pixels per lane of the forward and
+    backward blend, work order) on the buffers of a finished forward, the
+    tuple returned by forward_batched or forward_scenes.  Later calls with a
+    batch of the same shape use the choice.  Returns {"fwd_ppl", "bwd_ppl",
+    "order", "fwd_ms", "bwd_ms", "times_ms"} (every candidate's median time)."""
+    K, ore, oim, radii, geom, binbuf, img = forward_out
+    R, H, W = int(ore.size(0)), int(ore.size(1)), int(ore.size(2))
+    dev = ore.device
+    L = gh.lib()
+    mode = gh.GRAD_MODES[grad_mode] if isinstance(grad_mode, str) else int(grad_mode)
+    g = torch.Generator(device=dev).manual_seed(0)
+    dre = torch.randn((R, H, W), generator=g, device=dev)
+    dim = torch.randn((R, H, W), generator=g, device=dev)
+    out_re, out_im = torch.empty_like(ore), torch.empty_like(oim)
+    em = gh.f32(emission).reshape(-1, 2) if emission is not None else None
+    choice = (C.c_int * 3)()
+    ms = (C.c_float * 2)()
+    table = (C.c_float * 18)()
+    rad = radii.reshape(-1).to(torch.int32).contiguous()
+    stream = gh.stream_handle(dev)
+    if scenes is None:
+        P = int(radii.shape[-1]) if radii.dim() == 2 else int(radii.numel()) // max(R, 1)
+        grad = _empty_bytes(dev, L.gsparc_grad_buffer_size(P, R))
+        gh.check(L.gsparc_autotune(mode, P, R, W, H, gh.ptr(em), gh.ptr(rad), gh.ptr(geom), gh.ptr(img),
+                                   gh.ptr(binbuf), int(K), gh.ptr(grad), gh.ptr(dre), gh.ptr(dim), gh.ptr(out_re),
+                                   gh.ptr(out_im), int(iters), choice, ms, table, stream), "autotune")
+    else:
+        grad = _empty_bytes(dev, scenes.sizes(H, W)[3])
+        gh.check(L.gsparc_autotune_scenes(mode, C.byref(scenes.batch), gh.ptr(scenes.table), W, H, gh.ptr(em),
+                                          gh.ptr(rad), gh.ptr(geom), gh.ptr(img), gh.ptr(binbuf), int(K),
+                                          gh.ptr(grad), gh.ptr(dre), gh.ptr(dim), gh.ptr(out_re), gh.ptr(out_im),
+                                          int(iters), choice, ms, table, stream), "autotune")
+    times = {}
+    for o in range(3):
+        for k, ppl in enumerate((1, 2, 4)):
+            times[f"order{o + 1}_fwd_ppl{ppl}"] = round(table[o * 6 + k], 4)
+            times[f"order{o + 1}_bwd_ppl{ppl}"] = round(table[o * 6 + 3 + k], 4)
+    return {"fwd_ppl": choice[0], "bwd_ppl": choice[1], "order": choice[2], "fwd_ms": round(ms[0], 4),
+            "bwd_ms": round(ms[1], 4), "times_ms": times}
+
+
+def autotune_clear():
+    """Forget every tuned mapping (the built-in rule applies again)."""
+    gh.check(gh.lib().gsparc_autotune_clear(), "autotune clear")

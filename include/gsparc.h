@@ -64,9 +64,33 @@ const char* gsparc_last_error(void);
  *       tile) pairs, longest lists first: 0 = automatic (2 from 16
  *       receivers on, else 1), 1 = one list dealt
  *       round robin to the 8 XCDs, 2 = XCD x takes the receivers x mod 8,
- *       3 = as 2, one receiver after the other.  Speed only. */
+ *       3 = as 2, one receiver after the other.  Speed only.
+ * "Automatic" means: what gsparc_autotune chose for a batch of this shape
+ * (P, R, W, H, scenes), else the built-in rule above. */
 int gsparc_set_option(const char* name, int value);
 int gsparc_get_option(const char* name, int* value);
+
+/* Autotuner of the blend kernels' compute mapping: on the buffers of a
+ * finished forward (any form, K_capacity and radii as there), times the
+ * forward and backward blend under every pixels-per-lane mapping (1, 2, 4)
+ * and work order (1, 2, 3) and keeps the fastest combination for batches of
+ * this shape; later calls with the options at 0 use it.  Each candidate is
+ * timed `iters` times (median) where it runs in a step: the forward blend
+ * after the ordered scatter that writes its lists, the backward blend after a
+ * forward blend and the gradient clear.  choice[3] = {forward ppl, backward
+ * ppl, order}; ms[2] (may be NULL) = their per-launch times; table[18] (may be
+ * NULL) = per order, forward then backward times for ppl 1, 2, 4.
+ * dout_real / dout_imag [R,H,W] drive the timed backward; out_real / out_imag
+ * [R,H,W] receive the (unchanged) spectra; `grad` is overwritten, so run the
+ * backward after tuning.  Synchronises the stream; not while capturing.  The
+ * 16x16 tile binning itself stays fixed: it is part of the numerics
+ * (config.h:16-17, auxiliary.h:14-24).  No counterpart in the reference
+ * (fixed BLOCK_X x BLOCK_Y thread blocks, forward.cu:151-249). */
+int gsparc_autotune(int grad_mode, int P, int R, int W, int H, const float* emission, const int* radii,
+                    void* geom, void* image, void* bin, uint64_t K_capacity, void* grad, const float* dout_real,
+                    const float* dout_imag, float* out_real, float* out_imag, int iters, int* choice, float* ms,
+                    float* table, void* stream);
+int gsparc_autotune_clear(void);
 
 size_t gsparc_geom_buffer_size(int P, int R, int W, int H);
 size_t gsparc_image_buffer_size(int P, int R, int W, int H);
@@ -189,15 +213,17 @@ int gsparc_graph_end(void* stream, gsparc_graph** graph);
 int gsparc_graph_launch(gsparc_graph* graph, void* stream);
 int gsparc_graph_destroy(gsparc_graph* graph);
 
-/* Per-kernel timing.  While enabled, every kernel launched by the calls above
- * is bracketed by HIP events on its own stream; collect() waits for them,
- * adds the elapsed times to per-kernel totals and returns how many kernels
- * have totals; query() reads entry i (sorted by kernel name).  The blend
- * kernels also count, since enable: counts[0] forward (pixel, Gaussian) pairs
- * evaluated, counts[1] backward pairs evaluated, counts[2] / counts[3] list
- * entries tested against a wave's pixel block in the forward / backward
- * (profile_counts).  No counterpart in the reference (which only times
- * render() on the host, train.py:381-384). */
+/* Per-kernel timing and counters.  gsparc_profile_enable(mode): 0 off;
+ * 2 timing: every kernel launched by the calls above is bracketed by HIP
+ * events on its own stream; collect() waits for them, adds the elapsed times
+ * to per-kernel totals and returns how many kernels have totals; query()
+ * reads entry i (sorted by kernel name); 3 counting: the blend kernels run
+ * their counting variants (slower: extra atomics) and count counts[0] forward
+ * (pixel, Gaussian) pairs evaluated, counts[1] backward pairs evaluated,
+ * counts[2] / counts[3] list entries tested against a wave's pixel block in
+ * the forward / backward (profile_counts); 1 both (the timings then include
+ * the counting).  No counterpart in the reference (which only times render()
+ * on the host, train.py:381-384). */
 int gsparc_profile_enable(int on);
 int gsparc_profile_collect(int* n_kernels);
 int gsparc_profile_query(int i, char* name, int name_len, double* total_ms, long long* launches);
@@ -247,6 +273,11 @@ int gsparc_rasterize_forward_render_scenes(const gsparc_batch* batch, const uint
 int gsparc_instance_count_scenes(const gsparc_batch* batch, const uint32_t* scene_table, int W, int H,
                                  const void* image, uint64_t* K, void* stream);
 /* As gsparc_rasterize_backward; dmeans2D [Nrec,3], demission [Nrec,2] */
+int gsparc_autotune_scenes(int grad_mode, const gsparc_batch* batch, const uint32_t* scene_table, int W, int H,
+                           const float* emission, const int* radii, void* geom, void* image, void* bin,
+                           uint64_t K_capacity, void* grad, const float* dout_real, const float* dout_imag,
+                           float* out_real, float* out_imag, int iters, int* choice, float* ms, float* table,
+                           void* stream);
 int gsparc_rasterize_backward_scenes(int grad_mode, const gsparc_batch* batch, const uint32_t* scene_table, int W,
                                      int H, const float* means, const float* scales, const float* rotations,
                                      const float* emission_mlps, const float* emission, const float* viewmatrices,
