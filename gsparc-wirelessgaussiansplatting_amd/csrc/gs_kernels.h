@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "gs_csi.h"
+
 namespace gs {
 
 // A batch of S scenes (independent Gaussian sets) with R receivers in all.
@@ -121,6 +123,7 @@ struct PreFwdArgs {
     float* depth;            // [Nrec]
     int* radii;              // [Nrec]
     float* cov3;             // [P*6]
+    float* plen;             // [Nrec] transmitter -> Gaussian -> receiver path length, or nullptr
 };
 hipError_t launch_preprocess_fwd(const Dims& d, const PreFwdArgs& a, hipStream_t s);
 
@@ -226,11 +229,37 @@ struct PreBwdArgs {
     float* dcov3;            // [P,6] or nullptr
     float* dmeans2d;         // [R*P*3] or nullptr (reference screen-space layout)
     float* dsig;             // [Nrec*2] or nullptr (emission gradient)
+    const float* gL;         // [Nrec] dL/d(path length) of the CSI blend, or nullptr
     int mode;
     float2* dz;              // workspace [Nrec]: dL/d(MLP output) per receiver
     float* part;             // workspace [ceil(R/8)*P*12]: per-chunk partial sums
 };
 hipError_t launch_preprocess_bwd(const Dims& d, const PreBwdArgs& a, hipStream_t s);
+
+// Multi-frequency (CSI) blend: GS_CSI_NC complex channels per pixel, stored
+// interleaved [R*N][NC][2]; emission per record s * exp(-j 2 pi f_k L / c)
+// (gs_csi.h).  Forward writes out, finalT, ncontrib; backward (exact adjoint)
+// accumulates g8 as the single-channel backward does, with dsig the gradient
+// of the complex amplitude, plus gL = dL/dL per record.
+struct RenderCsiArgs {
+    const uint2* ranges;
+    const uint32_t* order;
+    const uint32_t* plist;
+    const float4* splat;
+    const float2* sig;       // [Nrec] complex amplitude
+    const float* plen;       // [Nrec] path length
+    float* out;              // forward: [R*N*NC*2]
+    float* finalT;           // forward writes, backward reads
+    uint32_t* ncontrib;
+    const float* dout;       // backward: [R*N*NC*2]
+    float* g8;               // backward: [Nrec*8]
+    float* gL;               // backward: [Nrec]
+    const uint32_t* total;
+    uint64_t Kcap;
+    GsCsi csi;
+};
+hipError_t launch_render_fwd_csi(const Dims& d, const RenderCsiArgs& a, hipStream_t s);
+hipError_t launch_render_bwd_csi(const Dims& d, const RenderCsiArgs& a, hipStream_t s);
 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 
@@ -261,6 +290,16 @@ struct SsimBwdArgs {
     float l1w;               // adds l1w * sign(img1 - img2) to dimg1 (0: nothing)
 };
 hipError_t launch_ssim_bwd(const SsimBwdArgs& a, hipStream_t s);
+
+// CSI loss pieces (k_loss_csi.hip): magnitude planes + L1 partials, the
+// gradient back to the interleaved map, and the two-part loss reduction.
+int csi_loss_blocks(size_t npix);
+hipError_t launch_csi_mag(size_t npix, int N, const float* out, const float* tgt, float* magA, float* magB,
+                          float* part, hipStream_t s);
+hipError_t launch_csi_grad(size_t npix, int N, const float* out, const float* tgt, const float* magA,
+                           const float* dmag, float l1w, float* dout, hipStream_t s);
+hipError_t launch_loss_reduce2(const float* ps, int nbs, double inv_s, const float* pl, int nbl, double inv_l,
+                               float l1w, float sw, float* loss, hipStream_t s);
 
 // loss = l1w * sum|img1 - img2| * inv_n + sw * (1 - sum(map) * inv_n) from
 // the per-workgroup partials of k_ssim_fwd, in a fixed order.

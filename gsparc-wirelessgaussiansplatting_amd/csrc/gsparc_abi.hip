@@ -106,6 +106,7 @@ void make_layout(const gs::Dims& d, uint64_t K, gsparc_layout& L)
     L.sort_val1 = o; o = al(o + NR * sizeof(uint32_t));
     L.sort_hist = o; o = al(o + (size_t)R * 257 * gs::rs_blocks(d.P) * sizeof(uint32_t));
     L.nvis = o; o = al(o + (size_t)R * sizeof(uint32_t));
+    L.plen = o; o = al(o + NR * sizeof(float));
     L.geom_bytes = o + ALIGN;
     o = 0;
     L.final_t = o; o = al(o + (size_t)R * N * sizeof(float));
@@ -129,12 +130,13 @@ void make_layout(int P, int R, int W, int H, uint64_t K, gsparc_layout& L)
     make_layout(gs::make_dims(P, R, W, H), K, L);
 }
 
-// grad workspace: g8 [Nrec*8] | dz [Nrec*2] | partials [slots*12]
-void grad_layout(const gs::Dims& d, size_t& dz_off, size_t& part_off, size_t& bytes)
+// grad workspace: g8 [Nrec*8] | dz [Nrec*2] | partials [slots*12] | gL [Nrec]
+void grad_layout(const gs::Dims& d, size_t& dz_off, size_t& part_off, size_t& gl_off, size_t& bytes)
 {
     dz_off = al(d.Nrec * 8 * sizeof(float));
     part_off = al(dz_off + d.Nrec * 2 * sizeof(float));
-    bytes = part_off + part_slots(d) * 12 * sizeof(float) + ALIGN;
+    gl_off = al(part_off + part_slots(d) * 12 * sizeof(float));
+    bytes = gl_off + d.Nrec * sizeof(float) + ALIGN;
 }
 
 // binning state shared by the two forward calls
@@ -294,9 +296,9 @@ size_t gsparc_binning_buffer_size(uint64_t K)
 
 size_t gsparc_grad_buffer_size(int P, int R)
 {
-    size_t a, b, c;
-    grad_layout(gs::make_dims(P, R, 1, 1), a, b, c);
-    return c;
+    size_t a, b, c, e;
+    grad_layout(gs::make_dims(P, R, 1, 1), a, b, c, e);
+    return e;
 }
 
 int gsparc_layout_query(int P, int R, int W, int H, uint64_t K, gsparc_layout* out)
@@ -333,7 +335,7 @@ int forward_geom_impl(const gs::Dims& d, const FwdInputs& in, void* geom, void* 
     pa.means = in.means; pa.scales = in.scales; pa.rots = in.rotations; pa.opac = in.opacities; pa.mlp = in.mlp;
     pa.views = in.views; pa.tx = in.tx; pa.rx = in.rx; pa.mod = in.mod;
     pa.splat = (float4*)(gb + L.splat); pa.sig = (float2*)(gb + L.sig); pa.depth = (float*)(gb + L.depth);
-    pa.radii = radii_out; pa.cov3 = (float*)(gb + L.cov3);
+    pa.radii = radii_out; pa.cov3 = (float*)(gb + L.cov3); pa.plen = (float*)(gb + L.plen);
     if (d.P > 0 && d.R > 0) GS_LAUNCH("preprocess_fwd", gs::launch_preprocess_fwd(d, pa, s), s);
     if (d.R == 0) return GSPARC_OK;
     gs::BinArgs ba = bin_args(d, L, gb, ib, radii_out);
@@ -406,16 +408,18 @@ struct BwdOutputs {
     float *dmeans2D, *dmeans3D, *dmlp, *dopacity, *dcov3D, *dscales, *drotations, *demission;
 };
 
+// csi != nullptr: the multi-frequency blend's adjoint (exact mode), with
+// dout_real the [R*N*NC*2] gradient of its output (dout_imag unused).
 int backward_impl(int grad_mode, const gs::Dims& d, const FwdInputs& in, const float* emission, const int* radii,
                   void* geom, void* image, void* bin, uint64_t K_capacity, void* grad, const float* dout_real,
-                  const float* dout_imag, const BwdOutputs& out, hipStream_t s)
+                  const float* dout_imag, const BwdOutputs& out, hipStream_t s, const GsCsi* csi = nullptr)
 {
     if (grad_mode != GSPARC_GRAD_EXACT && grad_mode != GSPARC_GRAD_REFERENCE)
         return fail(GSPARC_ERR_ARG, "unknown grad_mode");
     if (d.P == 0 || d.R == 0) return GSPARC_OK;
     if (!geom || !image || !bin || !grad || !in.means || !in.scales || !in.rotations || !in.views || !in.tx ||
-        !in.rx || !radii || !dout_real || !dout_imag || !out.dmeans3D || !out.dopacity || !out.dscales ||
-        !out.drotations)
+        !in.rx || !radii || !dout_real || (!csi && !dout_imag) || !out.dmeans3D || !out.dopacity ||
+        !out.dscales || !out.drotations)
         return fail(GSPARC_ERR_ARG, "null pointer");
     gsparc_layout L;
     make_layout(d, K_capacity, L);
@@ -423,37 +427,103 @@ int backward_impl(int grad_mode, const gs::Dims& d, const FwdInputs& in, const f
     char* ib = base_of(image);
     char* bb = base_of(bin);
     float* g8 = (float*)base_of(grad);
+    size_t dz_off, part_off, gl_off, bytes;
+    grad_layout(d, dz_off, part_off, gl_off, bytes);
+    float* gL = (float*)((char*)g8 + gl_off);
     GS_LAUNCH("grad_clear", gs::launch_fill_u32((uint32_t*)g8, 0u, d.Nrec * 8, s), s);
-    gs::RenderBwdArgs ra;
-    ra.ranges = (const uint2*)(ib + L.ranges);
-    ra.order = (const uint32_t*)(ib + L.tile_order);
-    ra.plist = (const uint32_t*)(bb + L.plist);
-    ra.splat = (const float4*)(gb + L.splat);
-    ra.sig = emission ? (const float2*)emission : (const float2*)(gb + L.sig);
-    ra.finalT = (const float*)(ib + L.final_t);
-    ra.ncontrib = (const uint32_t*)(ib + L.n_contrib);
-    ra.dout_re = dout_real;
-    ra.dout_im = dout_imag;
-    ra.g8 = g8;
-    ra.mode = grad_mode;
-    ra.total = (const uint32_t*)(ib + L.total);
-    ra.Kcap = K_capacity;
-    ra.counts = g_prof_count ? g_counts_dev : nullptr;
-    GS_LAUNCH("render_bwd", gs::launch_render_bwd(d, ra, bwd_ppl(d), s), s);
+    if (csi) {
+        GS_LAUNCH("grad_clear", gs::launch_fill_u32((uint32_t*)gL, 0u, d.Nrec, s), s);
+        gs::RenderCsiArgs ca;
+        ca.ranges = (const uint2*)(ib + L.ranges);
+        ca.order = (const uint32_t*)(ib + L.tile_order);
+        ca.plist = (const uint32_t*)(bb + L.plist);
+        ca.splat = (const float4*)(gb + L.splat);
+        ca.sig = emission ? (const float2*)emission : (const float2*)(gb + L.sig);
+        ca.plen = (const float*)(gb + L.plen);
+        ca.out = nullptr;
+        ca.finalT = (float*)(ib + L.final_t);
+        ca.ncontrib = (uint32_t*)(ib + L.n_contrib);
+        ca.dout = dout_real;
+        ca.g8 = g8;
+        ca.gL = gL;
+        ca.total = (const uint32_t*)(ib + L.total);
+        ca.Kcap = K_capacity;
+        ca.csi = *csi;
+        GS_LAUNCH("render_bwd_csi", gs::launch_render_bwd_csi(d, ca, s), s);
+    } else {
+        gs::RenderBwdArgs ra;
+        ra.ranges = (const uint2*)(ib + L.ranges);
+        ra.order = (const uint32_t*)(ib + L.tile_order);
+        ra.plist = (const uint32_t*)(bb + L.plist);
+        ra.splat = (const float4*)(gb + L.splat);
+        ra.sig = emission ? (const float2*)emission : (const float2*)(gb + L.sig);
+        ra.finalT = (const float*)(ib + L.final_t);
+        ra.ncontrib = (const uint32_t*)(ib + L.n_contrib);
+        ra.dout_re = dout_real;
+        ra.dout_im = dout_imag;
+        ra.g8 = g8;
+        ra.mode = grad_mode;
+        ra.total = (const uint32_t*)(ib + L.total);
+        ra.Kcap = K_capacity;
+        ra.counts = g_prof_count ? g_counts_dev : nullptr;
+        GS_LAUNCH("render_bwd", gs::launch_render_bwd(d, ra, bwd_ppl(d), s), s);
+    }
     gs::PreBwdArgs pa;
     pa.means = in.means; pa.scales = in.scales; pa.rots = in.rotations; pa.mlp = in.mlp; pa.views = in.views;
     pa.sig = (const float2*)(gb + L.sig); pa.cov3 = (const float*)(gb + L.cov3);
     pa.tx = in.tx; pa.rx = in.rx; pa.mod = in.mod; pa.radii = radii; pa.g8 = g8;
     pa.dmeans = out.dmeans3D; pa.dscales = out.dscales; pa.drots = out.drotations; pa.dopac = out.dopacity;
     pa.dmlp = out.dmlp; pa.dcov3 = out.dcov3D; pa.dmeans2d = out.dmeans2D; pa.dsig = out.demission;
+    pa.gL = csi ? gL : nullptr;
     pa.mode = grad_mode;
-    {
-        size_t dz_off, part_off, bytes;
-        grad_layout(d, dz_off, part_off, bytes);
-        pa.dz = (float2*)((char*)g8 + dz_off);
-        pa.part = (float*)((char*)g8 + part_off);
-    }
+    pa.dz = (float2*)((char*)g8 + dz_off);
+    pa.part = (float*)((char*)g8 + part_off);
     GS_LAUNCH("preprocess_bwd", gs::launch_preprocess_bwd(d, pa, s), s);
+    return GSPARC_OK;
+}
+
+// Multi-frequency blend forward on the geometry of a finished forward_geom:
+// ordered scatter into the tile lists, then the CSI blend.
+int forward_render_csi_impl(const gs::Dims& d, const GsCsi& csi, const float* emission, void* geom,
+                            void* image, void* bin, uint64_t K_capacity, const int* radii, float* out,
+                            hipStream_t s)
+{
+    if (!geom || !image || !bin || (d.R > 0 && (!out || (d.P > 0 && !radii))))
+        return fail(GSPARC_ERR_ARG, "null pointer");
+    if (d.R == 0) return GSPARC_OK;
+    gsparc_layout L;
+    make_layout(d, K_capacity, L);
+    char* gb = base_of(geom);
+    char* ib = base_of(image);
+    char* bb = base_of(bin);
+    gs::BinArgs ba = bin_args(d, L, gb, ib, radii);
+    ba.plist = (uint32_t*)(bb + L.plist);
+    ba.keys_sorted = nullptr;
+    ba.Kcap = K_capacity;
+    if (d.P > 0) GS_LAUNCH("bin_scatter", gs::launch_chunk_scatter(d, ba, s), s);
+    gs::RenderCsiArgs ca;
+    ca.ranges = ba.ranges;
+    ca.order = (const uint32_t*)(ib + L.tile_order);
+    ca.plist = ba.plist;
+    ca.splat = ba.splat;
+    ca.sig = emission ? (const float2*)emission : (const float2*)(gb + L.sig);
+    ca.plen = (const float*)(gb + L.plen);
+    ca.out = out;
+    ca.finalT = (float*)(ib + L.final_t);
+    ca.ncontrib = (uint32_t*)(ib + L.n_contrib);
+    ca.dout = nullptr; ca.g8 = nullptr; ca.gL = nullptr;
+    ca.total = ba.total;
+    ca.Kcap = K_capacity;
+    ca.csi = csi;
+    GS_LAUNCH("render_fwd_csi", gs::launch_render_fwd_csi(d, ca, s), s);
+    return GSPARC_OK;
+}
+
+int check_csi(double f0_hz, double df_hz, GsCsi& t)
+{
+    if (!(f0_hz >= 0.0) || !(df_hz >= 0.0) || !(f0_hz + GS_CSI_NC * df_hz < 1e12))
+        return fail(GSPARC_ERR_ARG, "subcarrier frequencies must be finite, non-negative and below 1 THz");
+    t = gs_csi_make(f0_hz, df_hz, GS_LIGHT_SPEED);
     return GSPARC_OK;
 }
 
@@ -728,12 +798,12 @@ int gsparc_scene_buffer_sizes(const gsparc_batch* batch, int W, int H, uint64_t 
     d.Npart = (size_t)batch->Npart;
     gsparc_layout L;
     make_layout(d, K, L);
-    size_t a, b, c;
-    grad_layout(d, a, b, c);
+    size_t a, b, c, e;
+    grad_layout(d, a, b, c, e);
     sizes[0] = L.geom_bytes;
     sizes[1] = L.image_bytes;
     sizes[2] = L.bin_bytes;
-    sizes[3] = c;
+    sizes[3] = e;
     return GSPARC_OK;
 }
 
@@ -847,6 +917,53 @@ int gsparc_ssim_backward(int planes, int H, int W, float C1, float C2, const flo
     return GSPARC_OK;
 }
 
+int gsparc_rasterize_forward_render_csi(int P, int R, int W, int H, double f0_hz, double df_hz,
+                                        const float* emission, void* geom, void* image, void* bin,
+                                        uint64_t K_capacity, const int* radii, float* out, void* stream)
+{
+    if (int e = check_dims(P, R, W, H)) return e;
+    GsCsi t;
+    if (int e = check_csi(f0_hz, df_hz, t)) return e;
+    return forward_render_csi_impl(gs::make_dims(P, R, W, H), t, emission, geom, image, bin, K_capacity, radii, out,
+                                   (hipStream_t)stream);
+}
+
+int gsparc_rasterize_forward_csi(int P, int R, int W, int H, double f0_hz, double df_hz, const float* means,
+                                 const float* scales, const float* rotations, const float* opacities,
+                                 const float* emission_mlps, const float* emission, const float* viewmatrices,
+                                 const float* tx_pos, const float* rx_pos, float scale_modifier, void* geom,
+                                 void* image, void* bin, uint64_t K_capacity, int* radii_out, float* out,
+                                 void* stream)
+{
+    if (int e = check_dims(P, R, W, H)) return e;
+    GsCsi t;
+    if (int e = check_csi(f0_hz, df_hz, t)) return e;
+    const gs::Dims d = gs::make_dims(P, R, W, H);
+    const FwdInputs in{means, scales, rotations, opacities, emission_mlps, viewmatrices, tx_pos, rx_pos,
+                       scale_modifier};
+    hipStream_t s = (hipStream_t)stream;
+    if (int e = forward_geom_impl(d, in, geom, image, radii_out, s)) return e;
+    return forward_render_csi_impl(d, t, emission, geom, image, bin, K_capacity, radii_out, out, s);
+}
+
+int gsparc_rasterize_backward_csi(int P, int R, int W, int H, double f0_hz, double df_hz, const float* means,
+                                  const float* scales, const float* rotations, const float* emission_mlps,
+                                  const float* emission, const float* viewmatrices, const float* tx_pos,
+                                  const float* rx_pos, float scale_modifier, const int* radii, void* geom,
+                                  void* image, void* bin, uint64_t K_capacity, void* grad, const float* dout,
+                                  float* dmeans2D, float* dmeans3D, float* demission_mlps, float* dopacity,
+                                  float* dcov3D, float* dscales, float* drotations, float* demission, void* stream)
+{
+    if (int e = check_dims(P, R, W, H)) return e;
+    GsCsi t;
+    if (int e = check_csi(f0_hz, df_hz, t)) return e;
+    const FwdInputs in{means, scales, rotations, nullptr, emission_mlps, viewmatrices, tx_pos, rx_pos,
+                       scale_modifier};
+    const BwdOutputs out{dmeans2D, dmeans3D, demission_mlps, dopacity, dcov3D, dscales, drotations, demission};
+    return backward_impl(GSPARC_GRAD_EXACT, gs::make_dims(P, R, W, H), in, emission, radii, geom, image, bin,
+                         K_capacity, grad, dout, nullptr, out, (hipStream_t)stream, &t);
+}
+
 int gsparc_autotune(int grad_mode, int P, int R, int W, int H, const float* emission, const int* radii,
                     void* geom, void* image, void* bin, uint64_t K_capacity, void* grad, const float* dout_real,
                     const float* dout_imag, float* out_real, float* out_imag, int iters, int* choice, float* ms,
@@ -907,6 +1024,55 @@ int gsparc_loss_l1_ssim(int planes, int H, int W, float lambda_ssim, float C1, f
         const float inv = (float)(1.0 / (double)n);
         gs::SsimBwdArgs b{planes, H, W, img, target, nullptr, dmu1, ds1, ds12, dL_dimg, -lambda_ssim * inv, l1w * inv};
         GS_LAUNCH("ssim_bwd", gs::launch_ssim_bwd(b, s), s);
+    }
+    return GSPARC_OK;
+}
+
+size_t gsparc_csi_loss_work_size(int R, int H, int W)
+{
+    if (R <= 0 || H <= 0 || W <= 0) return 0;
+    const size_t n = (size_t)R * GS_CSI_NC * H * W;
+    const size_t npix = (size_t)R * H * W;
+    return 256 + 6 * al(n * sizeof(float)) + al(2 * sizeof(float) * (size_t)gs::ssim_blocks(R * GS_CSI_NC, H, W)) +
+           sizeof(float) * (size_t)gs::csi_loss_blocks(npix);
+}
+
+int gsparc_csi_loss(int R, int H, int W, float lambda_ssim, float C1, float C2, const float* out, const float* target,
+                    void* work, float* loss, float* dL_dout, void* stream)
+{
+    if (R < 0 || H < 0 || W < 0) return fail(GSPARC_ERR_ARG, "negative size");
+    if ((size_t)R * GS_CSI_NC > 65535) return fail(GSPARC_ERR_ARG, "at most 1260 receivers per call");
+    const size_t npix = (size_t)R * H * W;
+    if (npix == 0) return fail(GSPARC_ERR_ARG, "empty map");
+    if (!out || !target || !work || !loss) return fail(GSPARC_ERR_ARG, "null pointer");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t n = (size_t)R * GS_CSI_NC * H * W;
+    char* base = base_of(work);
+    float* magA = (float*)base;
+    float* magB = (float*)(base + al(n * sizeof(float)));
+    float* dmu1 = (float*)(base + 2 * al(n * sizeof(float)));
+    float* ds1 = (float*)(base + 3 * al(n * sizeof(float)));
+    float* ds12 = (float*)(base + 4 * al(n * sizeof(float)));
+    float* dmag = (float*)(base + 5 * al(n * sizeof(float)));
+    const int planes = R * GS_CSI_NC;
+    const int nbs = gs::ssim_blocks(planes, H, W);
+    float* ps = (float*)(base + 6 * al(n * sizeof(float)));
+    float* pl = (float*)((char*)ps + al(2 * sizeof(float) * (size_t)nbs));
+    const int nbl = gs::csi_loss_blocks(npix);
+    const float l1w = 1.0f - lambda_ssim;
+    const double n1 = 2.0 * (double)n;   // real + imaginary parts
+    GS_LAUNCH("csi_mag", gs::launch_csi_mag(npix, H * W, out, target, magA, magB, pl, s), s);
+    gs::SsimArgs a{planes, H, W, C1, C2, magA, magB, nullptr, dL_dout ? dmu1 : nullptr, dL_dout ? ds1 : nullptr,
+                   dL_dout ? ds12 : nullptr, ps};
+    GS_LAUNCH("ssim_fwd", gs::launch_ssim_fwd(a, s), s);
+    GS_LAUNCH("loss_reduce", gs::launch_loss_reduce2(ps, nbs, 1.0 / (double)n, pl, nbl, 1.0 / n1, l1w, lambda_ssim,
+                                                     loss, s), s);
+    if (dL_dout) {
+        gs::SsimBwdArgs b{planes, H, W, magA, magB, nullptr, dmu1, ds1, ds12, dmag,
+                          (float)(-(double)lambda_ssim / (double)n), 0.0f};
+        GS_LAUNCH("ssim_bwd", gs::launch_ssim_bwd(b, s), s);
+        GS_LAUNCH("csi_grad", gs::launch_csi_grad(npix, H * W, out, target, magA, dmag, (float)((double)l1w / n1),
+                                                  dL_dout, s), s);
     }
     return GSPARC_OK;
 }

@@ -1,0 +1,121 @@
+// Training loss of the multi-frequency (CSI) map, value and gradient.  No
+// counterpart in the reference; it carries the reference's loss composition
+// (train.py:418-427: (1 - lambda) * L1 + lambda * (1 - SSIM), lambda = 0.2 in
+// optimizer.py:8) over to a complex map of GS_CSI_NC subcarriers:
+//   L1    mean |out - target| over the real and imaginary parts of every
+//         subcarrier of every pixel (R*N*NC*2 values);
+//   SSIM  mean SSIM of the magnitude maps |C_k|, one 11x11-window SSIM plane
+//         per (receiver, subcarrier) (R*NC planes of H x W).
+// k_csi_mag turns the interleaved [R*N][NC][2] maps into magnitude planes
+// [R*NC][N] (coalesced per subcarrier) and sums |out - target| per
+// workgroup; the SSIM kernels run on the planes; k_csi_grad adds the L1 sign
+// term and carries dL/d|C_k| back to (re, im) through C_k / |C_k| (0 at 0).
+#include "gs_device.h"
+#include "gs_kernels.h"
+
+namespace gs {
+
+#define CL_THREADS 256
+
+__global__ void __launch_bounds__(CL_THREADS) k_csi_mag(size_t npix, int N, const float* out, const float* tgt,
+                                                        float* magA, float* magB, float* part)
+{
+    __shared__ float red[CL_THREADS];
+    const size_t p = (size_t)blockIdx.x * CL_THREADS + threadIdx.x;
+    float l1 = 0.0f;
+    if (p < npix) {
+        const size_t r = p / (size_t)N, q = p % (size_t)N;
+        const float4* a = reinterpret_cast<const float4*>(out + p * (2 * GS_CSI_NC));
+        const float4* b = reinterpret_cast<const float4*>(tgt + p * (2 * GS_CSI_NC));
+        for (int k2 = 0; k2 < GS_CSI_NC / 2; ++k2) {
+            const float4 x = a[k2], y = b[k2];
+            const size_t o0 = (r * GS_CSI_NC + 2 * k2) * (size_t)N + q;
+            magA[o0] = sqrtf(x.x * x.x + x.y * x.y);
+            magA[o0 + N] = sqrtf(x.z * x.z + x.w * x.w);
+            magB[o0] = sqrtf(y.x * y.x + y.y * y.y);
+            magB[o0 + N] = sqrtf(y.z * y.z + y.w * y.w);
+            l1 += fabsf(x.x - y.x) + fabsf(x.y - y.y) + fabsf(x.z - y.z) + fabsf(x.w - y.w);
+        }
+    }
+    red[threadIdx.x] = l1;
+    __syncthreads();
+    for (int w = CL_THREADS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__device__ __forceinline__ float sgnf(float v) { return v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f); }
+
+__global__ void __launch_bounds__(CL_THREADS) k_csi_grad(size_t npix, int N, const float* out, const float* tgt,
+                                                         const float* magA, const float* dmag, float l1w,
+                                                         float* dout)
+{
+    const size_t p = (size_t)blockIdx.x * CL_THREADS + threadIdx.x;
+    if (p >= npix) return;
+    const size_t r = p / (size_t)N, q = p % (size_t)N;
+    const float4* a = reinterpret_cast<const float4*>(out + p * (2 * GS_CSI_NC));
+    const float4* b = reinterpret_cast<const float4*>(tgt + p * (2 * GS_CSI_NC));
+    float4* g = reinterpret_cast<float4*>(dout + p * (2 * GS_CSI_NC));
+    for (int k2 = 0; k2 < GS_CSI_NC / 2; ++k2) {
+        const float4 x = a[k2], y = b[k2];
+        const size_t o0 = (r * GS_CSI_NC + 2 * k2) * (size_t)N + q;
+        const float m0 = magA[o0], m1 = magA[o0 + N];
+        const float f0 = m0 > 0.0f ? dmag[o0] / m0 : 0.0f;
+        const float f1 = m1 > 0.0f ? dmag[o0 + N] / m1 : 0.0f;
+        g[k2] = make_float4(fmaf(l1w, sgnf(x.x - y.x), f0 * x.x), fmaf(l1w, sgnf(x.y - y.y), f0 * x.y),
+                            fmaf(l1w, sgnf(x.z - y.z), f1 * x.z), fmaf(l1w, sgnf(x.w - y.w), f1 * x.w));
+    }
+}
+
+// loss = l1w * sum(l1 partials) * inv_l1 + sw * (1 - sum(map partials) * inv_map), fixed order, FP64
+__global__ void __launch_bounds__(256) k_loss_reduce2(const float* ps, int nbs, double inv_s, const float* pl,
+                                                      int nbl, double inv_l, float l1w, float sw, float* loss)
+{
+    __shared__ double r0[256], r1[256];
+    const int tid = threadIdx.x;
+    double m = 0.0, a = 0.0;
+    for (int b = tid; b < nbs; b += 256) m += (double)ps[2 * b];
+    for (int b = tid; b < nbl; b += 256) a += (double)pl[b];
+    r0[tid] = m;
+    r1[tid] = a;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+            r0[tid] += r0[tid + w];
+            r1[tid] += r1[tid + w];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) loss[0] = (float)((double)l1w * r1[0] * inv_l + (double)sw * (1.0 - r0[0] * inv_s));
+}
+
+int csi_loss_blocks(size_t npix) { return (int)((npix + CL_THREADS - 1) / CL_THREADS); }
+
+hipError_t launch_csi_mag(size_t npix, int N, const float* out, const float* tgt, float* magA, float* magB,
+                          float* part, hipStream_t s)
+{
+    if (npix == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_csi_mag, dim3(csi_loss_blocks(npix)), dim3(CL_THREADS), 0, s, npix, N, out, tgt, magA, magB,
+                       part);
+    return hipGetLastError();
+}
+
+hipError_t launch_csi_grad(size_t npix, int N, const float* out, const float* tgt, const float* magA,
+                           const float* dmag, float l1w, float* dout, hipStream_t s)
+{
+    if (npix == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_csi_grad, dim3(csi_loss_blocks(npix)), dim3(CL_THREADS), 0, s, npix, N, out, tgt, magA,
+                       dmag, l1w, dout);
+    return hipGetLastError();
+}
+
+hipError_t launch_loss_reduce2(const float* ps, int nbs, double inv_s, const float* pl, int nbl, double inv_l,
+                               float l1w, float sw, float* loss, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_loss_reduce2, dim3(1), dim3(256), 0, s, ps, nbs, inv_s, pl, nbl, inv_l, l1w, sw, loss);
+    return hipGetLastError();
+}
+
+}  // namespace gs

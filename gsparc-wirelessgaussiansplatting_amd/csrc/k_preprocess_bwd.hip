@@ -195,6 +195,15 @@ __global__ void __launch_bounds__(GS_BWD_THREADS) k_pre_bwd_geo(Dims d, PreBwdAr
             const float gv[8] = {ga.x, ga.y, ga.z, ga.w, gb.x, gb.y, gb.z, gb.w};
             geo_bwd<MODE>(mean, c3, a.views + 16 * r, d.W, d.H, gv, dmean, dc);
             dop += gv[5];
+            if (a.gL) {
+                // CSI phase: L = |mu - tx| + |mu - rx|
+                const float gl = a.gL[ri];
+                const float* rxp = a.rx + 3 * r;
+                const float b[3] = {mean[0] - rxp[0], mean[1] - rxp[1], mean[2] - rxp[2]};
+                const float db = sqrtf(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) dmean[k] += gl * (b[k] / db - diff[k] / dist);
+            }
             if (a.dmeans2d) {
                 a.dmeans2d[3 * ri + 0] = gv[0] * (0.5f * (float)d.W);
                 a.dmeans2d[3 * ri + 1] = gv[1] * (0.5f * (float)d.H);

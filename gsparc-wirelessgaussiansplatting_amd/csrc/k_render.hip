@@ -17,59 +17,12 @@
 // Per pair the exponential is skipped when power < thr: then
 // opacity * exp(power) < 1/255 is guaranteed (thr carries a 1e-3 margin), so
 // the decision is the same as evaluating it.
+#include "gs_blend.h"
 #include "gs_cull.h"
 #include "gs_device.h"
 #include "gs_kernels.h"
 
 namespace gs {
-
-template <int PPL>
-struct Pixels {
-    int r, tile;
-    bool valid;
-    int px[PPL], py[PPL];
-    bool inside[PPL];
-    float x0, x1, y0, y1;   // pixel-centre bounds of the wave's block (inclusive)
-};
-
-// Workgroup b runs on XCD b & 7 (workgroups are dealt to the XCDs round
-// robin).  Its wave takes slot s = b >> 3 of that XCD: the work item is
-// entry s / WPT of the XCD's order list (ORDER_XCD_*) or of the global list
-// dealt round robin (ORDER_GLOBAL); the WPT waves of an item share the XCD
-// and its L2.  Items come longest list first, so that they do not start last
-// and set the kernel's end.
-template <int PPL>
-__device__ __forceinline__ Pixels<PPL> map_pixels(const Dims& d, const uint32_t* order, int order_mode)
-{
-    constexpr int WPT = 4 / PPL;   // waves per tile
-    Pixels<PPL> m;
-    const int b = blockIdx.x;
-    const int xcd = b & 7, s = b >> 3;
-    const int sub = s % WPT;
-    const size_t n8 = (size_t)(d.R + 7) / 8 * d.T;
-    const size_t idx = order_mode == ORDER_GLOBAL ? (size_t)(s / WPT) * 8 + xcd : xcd * n8 + s / WPT;
-    const uint32_t rt = idx < 8 * n8 ? order[idx] : 0xffffffffu;
-    m.valid = rt != 0xffffffffu;
-    m.r = (int)(rt / (uint32_t)d.T);
-    m.tile = (int)(rt % (uint32_t)d.T);
-    const int lane = threadIdx.x;
-    const int tx = m.valid ? m.tile % d.gx : 0, ty = m.valid ? m.tile / d.gx : 0;
-    int qx0 = 1, qx1 = 0, qy0 = 1, qy1 = 0;
-#pragma unroll
-    for (int k = 0; k < PPL; ++k) {
-        const int q = sub * PPL + k;   // quarter of the tile
-        m.px[k] = tx * GS_BLOCK + (q & 1) * 8 + (lane & 7);
-        m.py[k] = ty * GS_BLOCK + (q >> 1) * 8 + (lane >> 3);
-        m.inside[k] = m.px[k] < d.W && m.py[k] < d.H;
-        qx0 = min(qx0, q & 1); qx1 = max(qx1, q & 1);
-        qy0 = min(qy0, q >> 1); qy1 = max(qy1, q >> 1);
-    }
-    m.x0 = (float)(tx * GS_BLOCK + qx0 * 8);
-    m.x1 = (float)(tx * GS_BLOCK + qx1 * 8 + 7);
-    m.y0 = (float)(ty * GS_BLOCK + qy0 * 8);
-    m.y1 = (float)(ty * GS_BLOCK + qy1 * 8 + 7);
-    return m;
-}
 
 template <int PPL, bool COUNT>
 __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
@@ -172,41 +125,6 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
         a.finalT[p] = fabsf(T[k]);
         a.ncontrib[p] = last[k];
     }
-}
-
-template <int CTRL>
-__device__ __forceinline__ float dpp_move(float v)
-{
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
-}
-
-// Sums each of v[0..7] over the 64 lanes; lane 8*i returns the total of v[i]
-// (every lane of the 8-lane group i holds it).  Transpose reduction: the
-// exchanges across 32, 16 and 8 lanes each halve the number of values a lane
-// carries (4, 2, 1), then an 8-lane sum: quad_perm [1,0,3,2], quad_perm
-// [2,3,0,1], row_half_mirror.  ~30 instructions instead of 8 full reductions.
-__device__ __forceinline__ float wave_sum8(const float (&v)[8], int lane)
-{
-    const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8;
-    float w[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const float keep = b5 ? v[i + 4] : v[i];
-        const float send = b5 ? v[i] : v[i + 4];
-        w[i] = keep + __shfl_xor(send, 32);
-    }
-    float u[2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        const float keep = b4 ? w[i + 2] : w[i];
-        const float send = b4 ? w[i] : w[i + 2];
-        u[i] = keep + __shfl_xor(send, 16);
-    }
-    float x = (b3 ? u[1] : u[0]) + dpp_move<0x128>(b3 ? u[0] : u[1]);   // row_ror:8 = lane ^ 8
-    x = x + dpp_move<0xb1>(x);
-    x = x + dpp_move<0x4e>(x);
-    x = x + dpp_move<0x141>(x);
-    return x;
 }
 
 template <int MODE, int PPL, bool COUNT>

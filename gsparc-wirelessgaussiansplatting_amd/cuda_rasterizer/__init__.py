@@ -190,3 +190,53 @@ def rasterize_scenes(means3D, emission_mlps, opacities, scales, rotations, setti
     radii [Nrec]).  Pass emission_mlps (packed per-Gaussian MLPs, see
     wireless_rasterizer.pack_mlps) or emission [Nrec,2]."""
     return _RasterizeScenes.apply(means3D, emission_mlps, emission, opacities, scales, rotations, settings)
+
+
+# ----------------------------------------------------- multi-frequency CSI
+class CsiSetting(NamedTuple):
+    """Multi-frequency render of R receivers (no reference counterpart):
+    viewmatrices [R,4,4], tx_pos [3], rx_pos [R,3]; 52 subcarriers from f0
+    spaced df (Hz).  The backward is the exact adjoint."""
+    image_height: int
+    image_width: int
+    scale_modifier: float
+    viewmatrices: torch.Tensor
+    tx_pos: torch.Tensor
+    rx_pos: torch.Tensor
+    f0: float = wr.CSI_F0
+    df: float = wr.CSI_DF
+    capacity: int = 0
+
+
+class _RasterizeCsi(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, means3D, emission_mlps, emission, opacities, scales, rotations, settings):
+        s = settings
+        K, out, radii, geom, binbuf, img = wr.forward_csi(
+            means3D, emission_mlps, opacities, scales, rotations, s.image_height, s.image_width, s.scale_modifier,
+            s.viewmatrices, s.tx_pos, s.rx_pos, emission=emission, capacity=s.capacity or None, f0=s.f0, df=s.df)
+        ctx.s = s
+        ctx.K = K
+        ctx.has_emission = emission is not None
+        ctx.save_for_backward(means3D, emission_mlps if emission is None else emission, opacities, scales, rotations,
+                              radii, geom, binbuf, img)
+        ctx.mark_non_differentiable(radii)
+        return out, radii
+
+    @staticmethod
+    def backward(ctx, gout, _gr):
+        s = ctx.s
+        means3D, em_or_mlp, opacities, scales, rotations, radii, geom, binbuf, img = ctx.saved_tensors
+        mlp = None if ctx.has_emission else em_or_mlp
+        em = em_or_mlp if ctx.has_emission else None
+        d2, d3, dmlp, dop, _dcov, dsc, drot, dem = wr.backward_csi(
+            means3D, mlp, radii, opacities, scales, rotations, s.scale_modifier, s.viewmatrices, gout, s.tx_pos,
+            s.rx_pos, geom, ctx.K, binbuf, img, emission=em, f0=s.f0, df=s.df)
+        return (d3, None if dmlp is None else dmlp.reshape(mlp.shape), None if dem is None else dem.reshape(em.shape),
+                dop.reshape(opacities.shape), dsc, drot, None)
+
+
+def rasterize_csi(means3D, emission_mlps, opacities, scales, rotations, settings: CsiSetting,
+                  emission: Optional[torch.Tensor] = None):
+    """CSI maps (out [R,H,W,52,2], radii [R,P]) with autograd."""
+    return _RasterizeCsi.apply(means3D, emission_mlps, emission, opacities, scales, rotations, settings)

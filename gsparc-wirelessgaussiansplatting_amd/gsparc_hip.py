@@ -29,7 +29,7 @@ _sz = C.c_size_t
 class Layout(C.Structure):
     _fields_ = [(n, C.c_size_t) for n in (
         "splat", "sig", "depth", "cov3", "sort_key0", "sort_val0", "sort_key1", "sort_val1", "sort_hist", "nvis",
-        "geom_bytes",
+        "plen", "geom_bytes",
         "final_t", "n_contrib", "ranges", "segcnt", "segbase", "tile_order", "chunk_cnt", "total", "image_bytes",
         "plist", "keys_sorted", "bin_bytes")]
 
@@ -82,6 +82,14 @@ _SIGS = {
     "gsparc_autotune_scenes": (_i, [_i, _BP, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp, C.c_uint64, _vp, _vp, _vp, _vp,
                                     _vp, _i, C.POINTER(_i), C.POINTER(_f), C.POINTER(_f), _vp]),
     "gsparc_autotune_clear": (_i, []),
+    "gsparc_rasterize_forward_render_csi": (_i, [_i, _i, _i, _i, C.c_double, C.c_double, _vp, _vp, _vp, _vp,
+                                                 C.c_uint64, _vp, _vp, _vp]),
+    "gsparc_rasterize_forward_csi": (_i, [_i, _i, _i, _i, C.c_double, C.c_double] + [_vp] * 9 + [_f, _vp, _vp, _vp,
+                                          C.c_uint64, _vp, _vp, _vp]),
+    "gsparc_rasterize_backward_csi": (_i, [_i, _i, _i, _i, C.c_double, C.c_double] + [_vp] * 8 + [_f, _vp, _vp, _vp,
+                                           _vp, C.c_uint64, _vp, _vp] + [_vp] * 8 + [_vp]),
+    "gsparc_csi_loss_work_size": (C.c_size_t, [_i, _i, _i]),
+    "gsparc_csi_loss": (_i, [_i, _i, _i, _f, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gsparc_loss_work_size": (C.c_size_t, [_i, _i, _i]),
     "gsparc_loss_l1_ssim": (_i, [_i, _i, _i, _f, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gsparc_graph_begin": (_i, [_vp]),

@@ -68,6 +68,40 @@ class L1SSIMLoss(torch.autograd.Function):
         return grad * g, None, None
 
 
+def csi_loss(out: torch.Tensor, target: torch.Tensor, lambda_ssim: float = LAMBDA_SSIM, want_grad: bool = True):
+    """Loss of CSI maps out, target [R,H,W,52,2]: (1 - lambda) * mean |out -
+    target| over all real and imaginary parts + lambda * (1 - mean SSIM of the
+    52 magnitude maps per receiver).  Returns (loss [1], dL/dout or None)."""
+    x = gh.f32(out)
+    y = gh.f32(target)
+    if x.shape != y.shape or x.dim() != 5 or x.shape[-2:] != (52, 2):
+        raise RuntimeError("out and target must both be [R,H,W,52,2]")
+    gh.require_cuda(x, y)
+    R, H, W = x.shape[:3]
+    L = gh.lib()
+    loss = torch.empty(1, dtype=torch.float32, device=x.device)
+    grad = torch.empty_like(x) if want_grad else None
+    work = torch.empty(L.gsparc_csi_loss_work_size(R, H, W), dtype=torch.uint8, device=x.device)
+    gh.check(L.gsparc_csi_loss(R, H, W, float(lambda_ssim), SSIM_C1, SSIM_C2, gh.ptr(x), gh.ptr(y), gh.ptr(work),
+                               gh.ptr(loss), gh.ptr(grad), gh.stream_handle(x.device)), "csi loss")
+    return loss, grad
+
+
+class CSILoss(torch.autograd.Function):
+    """Autograd form of csi_loss (gradient to out only)."""
+
+    @staticmethod
+    def forward(ctx, out, target, lambda_ssim=LAMBDA_SSIM):
+        loss, grad = csi_loss(out, target, lambda_ssim, want_grad=out.requires_grad)
+        ctx.save_for_backward(grad if grad is not None else torch.empty(0, device=out.device))
+        return loss.reshape(())
+
+    @staticmethod
+    def backward(ctx, g):
+        (grad,) = ctx.saved_tensors
+        return grad * g, None, None
+
+
 def _require_param(t, name):
     if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()):
         raise RuntimeError(f"{name} must be a contiguous float32 GPU tensor (the graph reads it in place)")

@@ -417,3 +417,104 @@ def autotune(forward_out, grad_mode="reference", iters=5, emission=None, scenes:
 def autotune_clear():
     """Forget every tuned mapping (the built-in rule applies again)."""
     gh.check(gh.lib().gsparc_autotune_clear(), "autotune clear")
+
+
+# ------------------------------------------------------- multi-frequency CSI
+CSI_NC = 52
+CSI_F0 = 3.5e9 - 25.5 * 312.5e3      # Hz, subcarrier 0
+CSI_DF = 312.5e3                     # Hz, spacing
+
+
+def forward_csi(means3D, emission_mlps, opacities, scales, rotations, image_height, image_width, scale_modifier,
+                viewmatrices, tx_pos, rx_pos, emission=None, capacity=None, f0=CSI_F0, df=CSI_DF):
+    """Multi-frequency forward for R receivers: 52 subcarriers, record (r, g)
+    emitting s * exp(-j 2 pi f_k L / c) with s its MLP output (or `emission`
+    [R,P,2]) and L = |tx - mu| + |mu - rx_r|.  Returns (K or capacity, out
+    [R,H,W,52,2], radii [R,P], geom, bin, img)."""
+    _check_means(means3D)
+    P = int(means3D.size(0))
+    H, W = int(image_height), int(image_width)
+    dev = means3D.device
+    views = gh.f32(viewmatrices).reshape(-1, 16)
+    rxs = gh.f32(rx_pos).reshape(-1, 3)
+    R = int(rxs.size(0))
+    if views.size(0) != R:
+        raise RuntimeError("one view matrix per receiver position is required")
+    out = torch.zeros((R, H, W, CSI_NC, 2), dtype=torch.float32, device=dev)
+    if P == 0 or R == 0:
+        return (0, out, torch.zeros((R, P), dtype=torch.int32, device=dev), _empty_bytes(dev), _empty_bytes(dev),
+                _empty_bytes(dev))
+    means, sc, rot = gh.f32(means3D), gh.f32(scales), gh.f32(rotations)
+    op = gh.f32(opacities).reshape(-1)
+    tx = gh.f32(tx_pos).reshape(-1)
+    mlp = gh.f32_aligned(emission_mlps).reshape(-1) if emission is None else None
+    em = gh.f32(emission).reshape(R, P, 2) if emission is not None else None
+    if em is None and mlp.numel() != P * 194:
+        raise RuntimeError("emission_mlps must hold 194 floats per Gaussian")
+    gh.require_cuda(means, sc, rot, op, tx, views, rxs, mlp, em)
+    L = gh.lib()
+    geom = _empty_bytes(dev, L.gsparc_geom_buffer_size(P, R, W, H))
+    img = _empty_bytes(dev, L.gsparc_image_buffer_size(P, R, W, H))
+    radii = torch.empty((R, P), dtype=torch.int32, device=dev)
+    stream = gh.stream_handle(dev)
+    if capacity is not None:
+        Kc = int(capacity)
+        binbuf = _empty_bytes(dev, L.gsparc_binning_buffer_size(Kc))
+        gh.check(L.gsparc_rasterize_forward_csi(P, R, W, H, float(f0), float(df), gh.ptr(means), gh.ptr(sc),
+                                               gh.ptr(rot), gh.ptr(op), gh.ptr(mlp), gh.ptr(em), gh.ptr(views),
+                                               gh.ptr(tx), gh.ptr(rxs), float(scale_modifier), gh.ptr(geom),
+                                               gh.ptr(img), gh.ptr(binbuf), Kc, gh.ptr(radii), gh.ptr(out), stream),
+                 "rasterize forward (csi)")
+        return Kc, out, radii, geom, binbuf, img
+    K = C.c_uint64(0)
+    gh.check(L.gsparc_rasterize_forward_geom(P, R, W, H, gh.ptr(means), gh.ptr(sc), gh.ptr(rot), gh.ptr(op),
+                                            gh.ptr(mlp), gh.ptr(views), gh.ptr(tx), gh.ptr(rxs),
+                                            float(scale_modifier), gh.ptr(geom), gh.ptr(img), gh.ptr(radii),
+                                            C.byref(K), stream), "rasterize forward (csi preprocess)")
+    K = int(K.value)
+    binbuf = _empty_bytes(dev, L.gsparc_binning_buffer_size(K))
+    gh.check(L.gsparc_rasterize_forward_render_csi(P, R, W, H, float(f0), float(df), gh.ptr(em), gh.ptr(geom),
+                                                  gh.ptr(img), gh.ptr(binbuf), K, gh.ptr(radii), gh.ptr(out), stream),
+             "rasterize forward (csi render)")
+    return K, out, radii, geom, binbuf, img
+
+
+def backward_csi(means3D, emission_mlps, radii, opacities, scales, rotations, scale_modifier, viewmatrices,
+                 dL_dout, tx_pos, rx_pos, geomBuffer, num_rendered, binningBuffer, imageBuffer, emission=None,
+                 f0=CSI_F0, df=CSI_DF):
+    """Exact adjoint of forward_csi.  dL_dout [R,H,W,52,2].  Returns (dmeans2D
+    [R,P,3], dmeans3D [P,3], demission_mlps [M] or None, dopacity [P,1],
+    dcov3D [P,6], dscales [P,3], drotations [P,4], demission [R,P,2] or None)."""
+    P = int(means3D.size(0))
+    R = int(dL_dout.size(0))
+    dev = means3D.device
+    opts = dict(dtype=torch.float32, device=dev)
+    dmeans2D = torch.zeros((R, P, 3), **opts)
+    dmeans3D = torch.zeros((P, 3), **opts)
+    dmlp = torch.zeros(P * 194, **opts) if emission is None else None
+    dop = torch.zeros((P, 1), **opts)
+    dcov = torch.zeros((P, 6), **opts)
+    dsc = torch.zeros((P, 3), **opts)
+    drot = torch.zeros((P, 4), **opts)
+    dem = torch.zeros((R, P, 2), **opts) if emission is not None else None
+    if P == 0 or R == 0:
+        return dmeans2D, dmeans3D, dmlp, dop, dcov, dsc, drot, dem
+    H, W = int(dL_dout.size(1)), int(dL_dout.size(2))
+    means, sc, rot = gh.f32(means3D), gh.f32(scales), gh.f32(rotations)
+    views = gh.f32(viewmatrices).reshape(-1, 16)
+    rxs = gh.f32(rx_pos).reshape(-1, 3)
+    tx = gh.f32(tx_pos).reshape(-1)
+    mlp = gh.f32_aligned(emission_mlps).reshape(-1) if emission is None else None
+    em = gh.f32(emission).reshape(R, P, 2) if emission is not None else None
+    rad = radii.reshape(R, P).to(torch.int32).contiguous()
+    g = gh.f32(dL_dout)
+    gh.require_cuda(means, sc, rot, views, rxs, tx, mlp, em, rad, g)
+    L = gh.lib()
+    grad = _empty_bytes(dev, L.gsparc_grad_buffer_size(P, R))
+    gh.check(L.gsparc_rasterize_backward_csi(
+        P, R, W, H, float(f0), float(df), gh.ptr(means), gh.ptr(sc), gh.ptr(rot), gh.ptr(mlp), gh.ptr(em),
+        gh.ptr(views), gh.ptr(tx), gh.ptr(rxs), float(scale_modifier), gh.ptr(rad), gh.ptr(geomBuffer),
+        gh.ptr(imageBuffer), gh.ptr(binningBuffer), int(num_rendered), gh.ptr(grad), gh.ptr(g), gh.ptr(dmeans2D),
+        gh.ptr(dmeans3D), gh.ptr(dmlp), gh.ptr(dop), gh.ptr(dcov), gh.ptr(dsc), gh.ptr(drot), gh.ptr(dem),
+        gh.stream_handle(dev)), "rasterize backward (csi)")
+    return dmeans2D, dmeans3D, dmlp, dop, dcov, dsc, drot, dem

@@ -44,7 +44,7 @@ extern "C" {
 /* Byte offsets of the arrays inside the caller's buffers (for inspection). */
 typedef struct gsparc_layout {
     /* geom: per (receiver, Gaussian) records and the per-receiver depth sort */
-    size_t splat, sig, depth, cov3, sort_key0, sort_val0, sort_key1, sort_val1, sort_hist, nvis, geom_bytes;
+    size_t splat, sig, depth, cov3, sort_key0, sort_val0, sort_key1, sort_val1, sort_hist, nvis, plen, geom_bytes;
     /* image: per pixel and per tile state, per (chunk, tile) counts */
     size_t final_t, n_contrib, ranges, segcnt, segbase, tile_order, chunk_cnt, total, image_bytes;
     /* bin: the tile lists */
@@ -180,6 +180,46 @@ int gsparc_ssim_forward(int planes, int H, int W, float C1, float C2, const floa
 int gsparc_ssim_backward(int planes, int H, int W, float C1, float C2, const float* img1, const float* img2,
                          const float* dL_dmap, const float* dm_dmu1, const float* dm_dsigma1_sq,
                          const float* dm_dsigma12, float* dL_dimg1, void* stream);
+
+/* ---- multi-frequency (CSI) maps -------------------------------------------
+ * 52 complex channels per pixel instead of one.  A (receiver, Gaussian)
+ * record with complex amplitude s (the MLP output, or `emission`) and path
+ * length L = |tx - mu| + |mu - rx| emits s * exp(-j 2 pi f_k L / c) on
+ * subcarrier k = 0..51, f_k = f0_hz + k * df_hz (3.5 GHz centre, 312.5 kHz
+ * spacing: f0 = 3.5e9 - 25.5 * 312.5e3, df = 312.5e3); c = 299792458 m/s.
+ * out / dout: [R, H, W, 52, 2] float (real, imaginary interleaved).  The
+ * geometry, binning and buffers are those of the single-channel path:
+ * gsparc_rasterize_forward_geom then gsparc_rasterize_forward_render_csi, or
+ * gsparc_rasterize_forward_csi with a capacity.  The backward is the exact
+ * adjoint (grad_mode EXACT), including dL/dL to the means.  No counterpart in
+ * the reference, which renders one channel (config.h:15) one receiver at a
+ * time (renderer.py:39-88). */
+int gsparc_rasterize_forward_render_csi(int P, int R, int W, int H, double f0_hz, double df_hz,
+                                        const float* emission, void* geom, void* image, void* bin,
+                                        uint64_t K_capacity, const int* radii, float* out, void* stream);
+int gsparc_rasterize_forward_csi(int P, int R, int W, int H, double f0_hz, double df_hz, const float* means,
+                                 const float* scales, const float* rotations, const float* opacities,
+                                 const float* emission_mlps, const float* emission, const float* viewmatrices,
+                                 const float* tx_pos, const float* rx_pos, float scale_modifier, void* geom,
+                                 void* image, void* bin, uint64_t K_capacity, int* radii_out, float* out,
+                                 void* stream);
+int gsparc_rasterize_backward_csi(int P, int R, int W, int H, double f0_hz, double df_hz, const float* means,
+                                  const float* scales, const float* rotations, const float* emission_mlps,
+                                  const float* emission, const float* viewmatrices, const float* tx_pos,
+                                  const float* rx_pos, float scale_modifier, const int* radii, void* geom,
+                                  void* image, void* bin, uint64_t K_capacity, void* grad, const float* dout,
+                                  float* dmeans2D, float* dmeans3D, float* demission_mlps, float* dopacity,
+                                  float* dcov3D, float* dscales, float* drotations, float* demission, void* stream);
+
+/* Loss of a CSI map against a target map (both [R, H, W, 52, 2]):
+ *   (1 - lambda_ssim) * mean |out - target| over all real and imaginary parts
+ *   + lambda_ssim * (1 - mean SSIM of the magnitude maps |C_k|),
+ * one SSIM plane per (receiver, subcarrier).  loss [1]; dL_dout (may be NULL)
+ * like out; work: gsparc_csi_loss_work_size bytes.  Carries the loss
+ * composition of train.py:418-427 over to the complex map. */
+size_t gsparc_csi_loss_work_size(int R, int H, int W);
+int gsparc_csi_loss(int R, int H, int W, float lambda_ssim, float C1, float C2, const float* out, const float* target,
+                    void* work, float* loss, float* dL_dout, void* stream);
 
 /* Training loss of the reference, value and gradient in one call:
  *   loss = (1 - lambda_ssim) * mean|img - target| + lambda_ssim * (1 - mean SSIM(img, target))

@@ -42,7 +42,7 @@ def lib():
 
 def math_vec(which, x, y=None):
     """gs_math.h functions over arrays: which in {"exp", "log", "asin", "atan2"}."""
-    code = {"exp": 0, "log": 1, "asin": 2, "atan2": 3, "exp_nonpos": 4}[which]
+    code = {"exp": 0, "log": 1, "asin": 2, "atan2": 3, "exp_nonpos": 4, "cos2pi": 5, "sin2pi": 6}[which]
     x = np.ascontiguousarray(x, np.float32)
     y = np.ascontiguousarray(x if y is None else y, np.float32)
     out = np.empty_like(x)
@@ -215,3 +215,83 @@ def ssim_bwd(img1, img2, dmap, d1, d2, d3, dtype=np.float64):
     f(C.c_int(B * CH), C.c_int(H), C.c_int(W), _p(_c(img1, dtype)), _p(_c(img2, dtype)), _p(_c(dmap, dtype)),
       _p(_c(d1, dtype)), _p(_c(d2, dtype)), _p(_c(d3, dtype)), _p(out))
     return out
+
+
+# ---------------------------------------------------------------- CSI
+CSI_F0 = 3.5e9 - 25.5 * 312.5e3
+CSI_DF = 312.5e3
+CSI_NC = 52
+LIGHT_SPEED = 299792458.0
+
+
+def sincos2pi(x):
+    """gs_sincos2pi over an array: (cos 2 pi x, sin 2 pi x) in float32."""
+    return math_vec("cos2pi", x), math_vec("sin2pi", x)
+
+
+def path_length(scene, r=0, dtype=np.float32):
+    """|mu - tx| + |mu - rx_r| per Gaussian (mirror: the kernels' op order)."""
+    P = scene.P
+    out = np.zeros(P, dtype)
+    f = lib().orc_path_length_f32 if dtype == np.float32 else lib().orc_path_length_f64
+    f(C.c_int(P), _p(_c(scene.means, dtype)), _p(_c(scene.tx_pos, dtype)), _p(_c(scene.rx_pos[r], dtype)), _p(out))
+    return out
+
+
+def render_fwd_csi(pre, plen, vals, ranges, W, H, dtype=None, f0=CSI_F0, df=CSI_DF, sig=None):
+    """CSI blend of one receiver: out [H,W,NC,2].  sig (optional) replaces
+    the preprocess emission: (sig_re, sig_im) arrays of length P."""
+    dtype = dtype or pre.xy.dtype
+    N = W * H
+    out = np.zeros(N * CSI_NC * 2, dtype)
+    T = np.zeros(N, dtype)
+    nc = np.zeros(N, np.uint32)
+    sre, sim = (pre.sig_re, pre.sig_im) if sig is None else sig
+    vals = np.ascontiguousarray(vals if len(vals) else np.zeros(1, np.uint32), dtype=np.uint32)
+    f = lib().orc_render_fwd_csi_f32 if dtype == np.float32 else lib().orc_render_fwd_csi_f64
+    f(C.c_int(W), C.c_int(H), C.c_double(f0), C.c_double(df), _p(np.ascontiguousarray(ranges)), _p(vals),
+      _p(_c(pre.xy, dtype)), _p(_c(pre.conic_o, dtype)), _p(_c(sre, dtype)), _p(_c(sim, dtype)),
+      _p(_c(plen, dtype)), _p(out), _p(T), _p(nc))
+    return dict(out=out.reshape(H, W, CSI_NC, 2), final_T=T.reshape(H, W), n_contrib=nc.reshape(H, W))
+
+
+def render_bwd_csi(pre, plen, vals, ranges, fwd, dout, W, H, dtype=None, f0=CSI_F0, df=CSI_DF, sig=None):
+    """Exact adjoint of render_fwd_csi: [P,9] float64 = dpx, dpy, dqa, dqb,
+    dqc, dopacity, dsig_re, dsig_im, dL/d(path length)."""
+    dtype = dtype or pre.xy.dtype
+    P = pre.radii.shape[0]
+    g9 = np.zeros((P, 9), np.float64)
+    sre, sim = (pre.sig_re, pre.sig_im) if sig is None else sig
+    vals = np.ascontiguousarray(vals if len(vals) else np.zeros(1, np.uint32), dtype=np.uint32)
+    f = lib().orc_render_bwd_csi_f32 if dtype == np.float32 else lib().orc_render_bwd_csi_f64
+    f(C.c_int(W), C.c_int(H), C.c_double(f0), C.c_double(df), _p(np.ascontiguousarray(ranges)), _p(vals),
+      _p(_c(pre.xy, dtype)), _p(_c(pre.conic_o, dtype)), _p(_c(sre, dtype)), _p(_c(sim, dtype)),
+      _p(_c(plen, dtype)), _p(_c(fwd["final_T"].reshape(-1), dtype)),
+      _p(np.ascontiguousarray(fwd["n_contrib"].reshape(-1))), _p(_c(np.asarray(dout).reshape(-1), dtype)), _p(g9))
+    return g9
+
+
+def path_length_grad(scene, r, gL, radii):
+    """d/dmu of L = |mu - tx| + |mu - rx| times gL (FP64), visible Gaussians only."""
+    m = np.asarray(scene.means, np.float64)
+    a = m - np.asarray(scene.tx_pos, np.float64)
+    b = m - np.asarray(scene.rx_pos[r], np.float64)
+    u = a / np.linalg.norm(a, axis=1, keepdims=True) + b / np.linalg.norm(b, axis=1, keepdims=True)
+    return np.where((radii > 0)[:, None], gL[:, None] * u, 0.0)
+
+
+def render_csi(scene, r=0, dtype=np.float32, f0=CSI_F0, df=CSI_DF):
+    pre = preprocess(scene, r, dtype)
+    plen = path_length(scene, r, dtype)
+    keys, vals, ranges = bin_gaussians(pre, scene.W, scene.H)
+    fwd = render_fwd_csi(pre, plen, vals, ranges, scene.W, scene.H, dtype, f0, df)
+    return pre, plen, vals, ranges, fwd
+
+
+def backward_csi(scene, pre, plen, vals, ranges, fwd, dout, r=0, dtype=np.float32, f0=CSI_F0, df=CSI_DF,
+                 grads=None):
+    """Exact gradients of the CSI render of receiver r (accumulated into grads)."""
+    g9 = render_bwd_csi(pre, plen, vals, ranges, fwd, dout, scene.W, scene.H, dtype, f0, df)
+    g = preprocess_bwd(scene, pre, g9[:, :8], GRAD_EXACT, r, dtype, grads=grads)
+    g.means += path_length_grad(scene, r, g9[:, 8], pre.radii)
+    return g, g9
