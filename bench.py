@@ -19,6 +19,10 @@ on a bounded sample as a baseline.
 scenes with Gaussian counts log-uniform in [5k, 200k] and 4..64 receivers
 each, forward+backward of all of them in one launch sequence; it adds p50/p99
 of the batch step and of single spectra rendered alone.
+
+--workload csi measures multi-frequency maps: 50k Gaussians, 32 receivers,
+52 subcarriers (complex, 90x360x104 floats per receiver), L1 on the complex
+map + SSIM on the magnitudes, forward+backward.
 """
 import argparse
 import json
@@ -54,8 +58,10 @@ def parse():
     ap.add_argument("--fwd-ppl", type=int, default=0, help="forward blend pixels per lane (0: library default)")
     ap.add_argument("--bwd-ppl", type=int, default=0, help="backward blend pixels per lane (0: automatic)")
     ap.add_argument("--order", type=int, default=0, help="blend work order (0: automatic)")
-    ap.add_argument("--workload", default="receivers", choices=["receivers", "scenes"],
-                    help="receivers: one scene, R receivers (default); scenes: 16 uneven scenes in one batch")
+    ap.add_argument("--workload", default="receivers", choices=["receivers", "scenes", "csi"],
+                    help="receivers: one scene, R receivers (default); scenes: 16 uneven scenes in one batch; "
+                         "csi: 52-subcarrier complex maps")
+    ap.add_argument("--csi-receivers", type=int, default=32)
     ap.add_argument("--no-tune", action="store_true", help="skip the blend-mapping autotuner")
     ap.add_argument("--eager", action="store_true",
                     help="time the eager autograd step instead of the HIP-graph-captured one")
@@ -304,6 +310,85 @@ def setup_scenes(args, dev):
                 "emission": "per-Gaussian MLP 3->32->2", "grad_mode": args.grad_mode})
 
 
+def setup_csi(args, dev):
+    """50k Gaussians (config2 generator), 32 receivers, 52-subcarrier maps."""
+    import numpy as np
+    import torch
+    import cuda_rasterizer as cr
+    import gsparc_scenes as S
+    import gsparc_train as T
+    import wireless_rasterizer as wr
+    P, R = args.gaussians, args.csi_receivers
+    sc = S.config2(P=P, R=R)
+    tgt_sc = S.config2(seed=3, P=P, R=R)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    means, scales, rots, opac, mlp = (t(sc.means), t(sc.scales), t(sc.rotations), t(sc.opacities), t(sc.mlp))
+    views, txp, rxp = t(sc.views), t(sc.tx_pos), t(sc.rx_pos)
+    with torch.no_grad():
+        target = wr.forward_csi(t(tgt_sc.means), t(tgt_sc.mlp), t(tgt_sc.opacities), t(tgt_sc.scales),
+                                t(tgt_sc.rotations), sc.H, sc.W, 1.0, views, txp, rxp)[1].clone()
+    params = [x.clone().requires_grad_(True) for x in (means, scales, rots, opac, mlp)]
+    settings = cr.CsiSetting(sc.H, sc.W, 1.0, views, txp, rxp)
+
+    def step():
+        m, s_, q, o, w = params
+        out, radii = cr.rasterize_csi(m, w, o, s_, q, settings)
+        T.CSILoss.apply(out, target).backward()
+        for p_ in params:
+            p_.grad = None
+
+    plain_settings = [cr.CsiSetting(sc.H, sc.W, 1.0, views[r:r + 1], txp, rxp[r:r + 1]) for r in range(R)]
+
+    def step_plain():
+        m, s_, q, o, w = params
+        for r in range(R):
+            out, radii = cr.rasterize_csi(m, w, o, s_, q, plain_settings[r])
+            (T.CSILoss.apply(out, target[r:r + 1]) / R).backward()
+        for p_ in params:
+            p_.grad = None
+
+    def single(i):
+        r = i % R
+        wr.forward_csi(means, mlp, opac, scales, rots, sc.H, sc.W, 1.0, views[r:r + 1], txp, rxp[r:r + 1])
+
+    def count():
+        return wr.forward_csi(means, mlp, opac, scales, rots, sc.H, sc.W, 1.0, views, txp, rxp)[0]
+
+    def cpu():
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        nr = min(args.cpu_receivers, R)
+        tg = target[:nr].cpu().numpy()
+        log(args, f"CPU baseline: CSI oracle fwd+bwd on {nr} receivers, 1 core")
+        t0 = time.perf_counter()
+        acc = O.Grads(P)
+        for r in range(nr):
+            pre, plen, vals, ranges, fwd = O.render_csi(sc, r)
+            x = fwd["out"][None]
+            y = tg[r][None]
+            ma = np.sqrt((x.astype(np.float32) ** 2).sum(-1)).transpose(0, 3, 1, 2).copy()
+            mb = np.sqrt((y ** 2).sum(-1)).transpose(0, 3, 1, 2).copy()
+            m_, d1, d2, d3 = O.ssim_fwd(ma, mb, dtype=np.float32)
+            dmag = O.ssim_bwd(ma, mb, np.full_like(ma, -0.2 / ma.size), d1, d2, d3, dtype=np.float32)
+            dmag = dmag.transpose(0, 2, 3, 1)[..., None]
+            mag = ma.transpose(0, 2, 3, 1)[..., None]
+            dout = (0.8 / x.size) * np.sign(x - y) + np.where(mag > 0, dmag * x / np.where(mag > 0, mag, 1), 0)
+            O.backward_csi(sc, pre, plen, vals, ranges, fwd, dout[0].astype(np.float32), r, grads=acc)
+        el = time.perf_counter() - t0
+        return {"value": round(nr / el, 4), "unit": "spectra/s", "cores": 1, "kind": "port",
+                "sample": f"{nr} of the {R} receivers, 52-subcarrier fwd+bwd incl. loss, FP32 C++ oracle "
+                          f"single-threaded ({el:.1f} s)"}
+
+    return Workload(
+        step=step, step_plain=step_plain, single=single, count=count, cpu=cpu, graphed=None, tune=None,
+        spectra=R, Ptot=P, Nrec=P * R, H=sc.H, W=sc.W, traffic_key=None, csi=True,
+        data="synthetic (seeded: 70% wall-clustered Gaussians, receivers uniform in a 6x6x3 m room)",
+        config={"workload": "multi-frequency CSI, fwd+bwd, 0.8*L1(complex) + 0.2*(1-SSIM(magnitude))",
+                "gaussians": P, "receivers": R, "map": f"{sc.H}x{sc.W}x104", "subcarriers": 52,
+                "f0_hz": wr.CSI_F0, "df_hz": wr.CSI_DF, "emission": "per-Gaussian MLP 3->32->2 x phase",
+                "grad_mode": "exact"})
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -321,10 +406,10 @@ def main():
     gh.set_option("render_fwd_ppl", args.fwd_ppl)
     gh.set_option("render_bwd_ppl", args.bwd_ppl)
     gh.set_option("render_order", args.order)
-    wl = setup_receivers(args, dev) if args.workload == "receivers" else setup_scenes(args, dev)
+    wl = {"receivers": setup_receivers, "scenes": setup_scenes, "csi": setup_csi}[args.workload](args, dev)
     R, Ptot, Nrec = wl.spectra, wl.Ptot, wl.Nrec
     tuned = None
-    if not args.no_tune and not (args.fwd_ppl or args.bwd_ppl or args.order):
+    if wl.tune is not None and not args.no_tune and not (args.fwd_ppl or args.bwd_ppl or args.order):
         tuned = wl.tune()
         log(args, f"autotuned blend mapping: {tuned}")
 
@@ -427,6 +512,18 @@ def main():
         "ssim_fwd": ("hbm", 8 * N * R + 16 * N * R, "GB/s"),
         "ssim_bwd": ("hbm", 28 * N * R, "GB/s"),
     }
+    if getattr(wl, "csi", False):
+        # 104 floats per pixel out (forward) and dL/dout in (backward); the
+        # SSIM runs on 52 magnitude planes per receiver
+        NC = 52
+        work.update({
+            "render_fwd_csi": ("hbm", 44.0 * K + (8 * NC + 8) * N * R, "GB/s"),
+            "render_bwd_csi": ("hbm", 44.0 * K + (8 * NC + 8) * N * R + 36.0 * Nrec, "GB/s"),
+            "csi_mag": ("hbm", 24.0 * NC * N * R, "GB/s"),
+            "csi_grad": ("hbm", 32.0 * NC * N * R, "GB/s"),
+            "ssim_fwd": ("hbm", 24.0 * NC * N * R, "GB/s"),
+            "ssim_bwd": ("hbm", 28.0 * NC * N * R, "GB/s"),
+        })
     # FP32 work of the blend kernels (vector ALU): per evaluated (pixel,
     # Gaussian) pair and per list entry tested against a wave's pixel block
     flops = {"render_fwd": cnt["fwd_pairs"] * FWD_FLOP_PER_PAIR + cnt["fwd_entries"] * CULL_FLOP,
@@ -461,9 +558,11 @@ def main():
 
     cpu = wl.cpu() if args.cpu_receivers > 0 else None
 
+    metric = {"receivers": METRIC,
+              "scenes": "fwd+bwd spectra/sec, 16 uneven scenes (5k-200k Gaussians, 4-64 rx each), 90x360, one batch",
+              "csi": "fwd+bwd CSI maps/sec, 50k Gaussians, 90x360x52 complex subcarriers, 32 rx batched"}
     out = {
-        "metric": METRIC if args.workload == "receivers" else
-        "fwd+bwd spectra/sec, 16 uneven scenes (5k-200k Gaussians, 4-64 rx each), 90x360, one batch",
+        "metric": metric[args.workload],
         "value": round(value, 2),
         "unit": "spectra/s",
         "n_gpus": 1,
