@@ -258,7 +258,7 @@ struct RenderCsiArgs {
     uint64_t Kcap;
     GsCsi csi;
 };
-hipError_t launch_render_fwd_csi(const Dims& d, const RenderCsiArgs& a, hipStream_t s);
+hipError_t launch_render_fwd_csi(const Dims& d, const RenderCsiArgs& a, int ppl, hipStream_t s);
 hipError_t launch_render_bwd_csi(const Dims& d, const RenderCsiArgs& a, hipStream_t s);
 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
@@ -298,8 +298,9 @@ hipError_t launch_csi_mag(size_t npix, int N, const float* out, const float* tgt
                           float* part, hipStream_t s);
 hipError_t launch_csi_grad(size_t npix, int N, const float* out, const float* tgt, const float* magA,
                            const float* dmag, float l1w, float* dout, hipStream_t s);
+size_t loss_reduce2_scratch();
 hipError_t launch_loss_reduce2(const float* ps, int nbs, double inv_s, const float* pl, int nbl, double inv_l,
-                               float l1w, float sw, float* loss, hipStream_t s);
+                               float l1w, float sw, double* stage, float* loss, hipStream_t s);
 
 // loss = l1w * sum|img1 - img2| * inv_n + sw * (1 - sum(map) * inv_n) from
 // the per-workgroup partials of k_ssim_fwd, in a fixed order.

@@ -25,6 +25,7 @@ thread_local std::string g_err;
 int g_opt_fwd_ppl = 0;
 int g_opt_bwd_ppl = 0;
 int g_opt_order = 0;
+int g_opt_csi_fwd_ppl = 0;   // CSI forward blend: 1 or 2 pixels per lane, 0 = automatic
 
 // Autotuned blend mappings, per batch shape (gsparc_autotune).
 struct TuneKey {
@@ -247,6 +248,7 @@ static int* option_slot(const char* name)
     if (std::strcmp(name, "render_fwd_ppl") == 0) return &g_opt_fwd_ppl;
     if (std::strcmp(name, "render_bwd_ppl") == 0) return &g_opt_bwd_ppl;
     if (std::strcmp(name, "render_order") == 0) return &g_opt_order;
+    if (std::strcmp(name, "csi_fwd_ppl") == 0) return &g_opt_csi_fwd_ppl;
     return nullptr;
 }
 
@@ -257,6 +259,8 @@ int gsparc_set_option(const char* name, int value)
     if (!slot) return fail(GSPARC_ERR_ARG, std::string("unknown option ") + name);
     if (slot == &g_opt_order) {
         if (value < 0 || value > 3) return fail(GSPARC_ERR_ARG, "render_order must be 0, 1, 2 or 3");
+    } else if (slot == &g_opt_csi_fwd_ppl) {
+        if (value < 0 || value > 2) return fail(GSPARC_ERR_ARG, "csi_fwd_ppl must be 0, 1 or 2");
     } else if (value != 0 && value != 1 && value != 2 && value != 4) {
         return fail(GSPARC_ERR_ARG, std::string(name) + " must be 0, 1, 2 or 4");
     }
@@ -515,7 +519,9 @@ int forward_render_csi_impl(const gs::Dims& d, const GsCsi& csi, const float* em
     ca.total = ba.total;
     ca.Kcap = K_capacity;
     ca.csi = csi;
-    GS_LAUNCH("render_fwd_csi", gs::launch_render_fwd_csi(d, ca, s), s);
+    // half tiles per wave once the batch has enough tiles to fill the chip
+    const int ppl = g_opt_csi_fwd_ppl ? g_opt_csi_fwd_ppl : ((long long)d.R * d.T >= 1024 ? 2 : 1);
+    GS_LAUNCH("render_fwd_csi", gs::launch_render_fwd_csi(d, ca, ppl, s), s);
     return GSPARC_OK;
 }
 
@@ -1034,7 +1040,7 @@ size_t gsparc_csi_loss_work_size(int R, int H, int W)
     const size_t n = (size_t)R * GS_CSI_NC * H * W;
     const size_t npix = (size_t)R * H * W;
     return 256 + 6 * al(n * sizeof(float)) + al(2 * sizeof(float) * (size_t)gs::ssim_blocks(R * GS_CSI_NC, H, W)) +
-           sizeof(float) * (size_t)gs::csi_loss_blocks(npix);
+           al(sizeof(float) * (size_t)gs::csi_loss_blocks(npix)) + gs::loss_reduce2_scratch();
 }
 
 int gsparc_csi_loss(int R, int H, int W, float lambda_ssim, float C1, float C2, const float* out, const float* target,
@@ -1059,6 +1065,7 @@ int gsparc_csi_loss(int R, int H, int W, float lambda_ssim, float C1, float C2, 
     float* ps = (float*)(base + 6 * al(n * sizeof(float)));
     float* pl = (float*)((char*)ps + al(2 * sizeof(float) * (size_t)nbs));
     const int nbl = gs::csi_loss_blocks(npix);
+    double* stage = (double*)((char*)pl + al(sizeof(float) * (size_t)nbl));
     const float l1w = 1.0f - lambda_ssim;
     const double n1 = 2.0 * (double)n;   // real + imaginary parts
     GS_LAUNCH("csi_mag", gs::launch_csi_mag(npix, H * W, out, target, magA, magB, pl, s), s);
@@ -1066,7 +1073,7 @@ int gsparc_csi_loss(int R, int H, int W, float lambda_ssim, float C1, float C2, 
                    dL_dout ? ds12 : nullptr, ps};
     GS_LAUNCH("ssim_fwd", gs::launch_ssim_fwd(a, s), s);
     GS_LAUNCH("loss_reduce", gs::launch_loss_reduce2(ps, nbs, 1.0 / (double)n, pl, nbl, 1.0 / n1, l1w, lambda_ssim,
-                                                     loss, s), s);
+                                                     stage, loss, s), s);
     if (dL_dout) {
         gs::SsimBwdArgs b{planes, H, W, magA, magB, nullptr, dmu1, ds1, ds12, dmag,
                           (float)(-(double)lambda_ssim / (double)n), 0.0f};

@@ -69,17 +69,19 @@ __global__ void __launch_bounds__(CL_THREADS) k_csi_grad(size_t npix, int N, con
     }
 }
 
-// loss = l1w * sum(l1 partials) * inv_l1 + sw * (1 - sum(map partials) * inv_map), fixed order, FP64
-__global__ void __launch_bounds__(256) k_loss_reduce2(const float* ps, int nbs, double inv_s, const float* pl,
-                                                      int nbl, double inv_l, float l1w, float sw, float* loss)
+// Two-stage fixed-order reduction of the loss partials.  Stage 1: workgroup
+// b of RD_BLOCKS sums a contiguous slice of the SSIM-map partials (ps, .x of
+// each pair) and of the L1 partials (pl) in FP64; stage 2: one workgroup sums
+// the RD_BLOCKS pairs and forms
+//   loss = l1w * sum(l1) * inv_l + sw * (1 - sum(map) * inv_s).
+#define RD_BLOCKS 256
+
+__device__ __forceinline__ void block_sum2(double& a, double& b)
 {
     __shared__ double r0[256], r1[256];
     const int tid = threadIdx.x;
-    double m = 0.0, a = 0.0;
-    for (int b = tid; b < nbs; b += 256) m += (double)ps[2 * b];
-    for (int b = tid; b < nbl; b += 256) a += (double)pl[b];
-    r0[tid] = m;
-    r1[tid] = a;
+    r0[tid] = a;
+    r1[tid] = b;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if (tid < w) {
@@ -88,7 +90,32 @@ __global__ void __launch_bounds__(256) k_loss_reduce2(const float* ps, int nbs, 
         }
         __syncthreads();
     }
-    if (tid == 0) loss[0] = (float)((double)l1w * r1[0] * inv_l + (double)sw * (1.0 - r0[0] * inv_s));
+    a = r0[0];
+    b = r1[0];
+}
+
+__global__ void __launch_bounds__(256) k_loss_stage1(const float* ps, int nbs, const float* pl, int nbl,
+                                                     double* stage)
+{
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const int s0 = (int)((long long)nbs * b / RD_BLOCKS), s1 = (int)((long long)nbs * (b + 1) / RD_BLOCKS);
+    const int l0 = (int)((long long)nbl * b / RD_BLOCKS), l1 = (int)((long long)nbl * (b + 1) / RD_BLOCKS);
+    double m = 0.0, a = 0.0;
+    for (int i = s0 + tid; i < s1; i += 256) m += (double)ps[2 * i];
+    for (int i = l0 + tid; i < l1; i += 256) a += (double)pl[i];
+    block_sum2(m, a);
+    if (tid == 0) {
+        stage[2 * b] = m;
+        stage[2 * b + 1] = a;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_loss_stage2(const double* stage, double inv_s, double inv_l, float l1w,
+                                                     float sw, float* loss)
+{
+    double m = stage[2 * threadIdx.x], a = stage[2 * threadIdx.x + 1];
+    block_sum2(m, a);
+    if (threadIdx.x == 0) loss[0] = (float)((double)l1w * a * inv_l + (double)sw * (1.0 - m * inv_s));
 }
 
 int csi_loss_blocks(size_t npix) { return (int)((npix + CL_THREADS - 1) / CL_THREADS); }
@@ -111,10 +138,15 @@ hipError_t launch_csi_grad(size_t npix, int N, const float* out, const float* tg
     return hipGetLastError();
 }
 
+size_t loss_reduce2_scratch() { return 2 * RD_BLOCKS * sizeof(double); }
+
 hipError_t launch_loss_reduce2(const float* ps, int nbs, double inv_s, const float* pl, int nbl, double inv_l,
-                               float l1w, float sw, float* loss, hipStream_t s)
+                               float l1w, float sw, double* stage, float* loss, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_loss_reduce2, dim3(1), dim3(256), 0, s, ps, nbs, inv_s, pl, nbl, inv_l, l1w, sw, loss);
+    hipLaunchKernelGGL(k_loss_stage1, dim3(RD_BLOCKS), dim3(256), 0, s, ps, nbs, pl, nbl, stage);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_loss_stage2, dim3(1), dim3(RD_BLOCKS), 0, s, stage, inv_s, inv_l, l1w, sw, loss);
     return hipGetLastError();
 }
 

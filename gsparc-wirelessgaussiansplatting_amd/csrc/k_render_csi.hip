@@ -6,14 +6,17 @@
 // alpha and T do not depend on the subcarrier, so the walk, the skips and the
 // termination are those of the single-channel blend; only the accumulation
 // widens to 104 floats per pixel.  Mapping: one wave per 8x8 quarter tile
-// (one pixel per lane, whose 104 accumulators live in VGPRs), lists staged 32
-// entries at a time.  While staging, each lane evaluates the 52 phasors of
+// (one pixel per lane, whose 104 values live in VGPRs) or, for the forward,
+// one wave per 16x8 half tile (two pixels per lane: each LDS read of an
+// entry's emissions then serves two pixels; the forward is bound by the LDS
+// return path, not by the FMAs); lists staged 32 entries at a time.  While staging, each lane evaluates the 52 phasors of
 // its entry (gs_cycle_frac + gs_sincos2pi, once per entry and wave, not per
 // pixel) into LDS: the forward stores the emission e_k = s u_k, the backward
 // the phasors u_k.
 //
 // Backward.  With G_k = dL/dC_k (complex) of a pixel and p_k = conj(G_k) u_k,
-// Z = sum_k p_k and Z1 = sum_k k p_k:
+// Z = sum_k p_k and Z1 = sum_k k p_k (formed from the top subcarrier down as
+// running suffix sums, 6 operations per subcarrier):
 //   h = Re(s Z) = sum_k Re(conj(G_k) e_k) plays the colour of the single-
 //     channel adjoint (dL/dalpha from h and the running accumulation of h);
 //   dL/ds_re = w Re(Z), dL/ds_im = -w Im(Z);
@@ -31,32 +34,45 @@ namespace gs {
 #define CSI_BATCH 32
 #define CSI_NC2 (GS_CSI_NC / 2)       // float4 = two subcarriers
 
+template <int PPL>
 __global__ void __launch_bounds__(GS_WAVE) k_render_fwd_csi(Dims d, RenderCsiArgs a)
 {
     __shared__ float4 s0[CSI_BATCH];
     __shared__ float4 s1[CSI_BATCH];
     __shared__ float4 se[CSI_BATCH][CSI_NC2];
-    const Pixels<1> pm = map_pixels<1>(d, a.order, (int)a.total[1]);
+    const Pixels<PPL> pm = map_pixels<PPL>(d, a.order, (int)a.total[1]);
     if (!pm.valid) return;
     if ((uint64_t)a.total[0] > a.Kcap) return;
-    if (ballot(pm.inside[0]) == 0) return;
+    bool any_inside = false;
+#pragma unroll
+    for (int q = 0; q < PPL; ++q) any_inside |= pm.inside[q];
+    if (ballot(any_inside) == 0) return;
     const int r = pm.r;
     const int lane = threadIdx.x;
     const uint2 range = a.ranges[(size_t)r * d.T + pm.tile];
     const uint32_t rb = __builtin_amdgcn_readfirstlane(range.x);
     const uint32_t re = __builtin_amdgcn_readfirstlane(range.y);
     const size_t rP = rx_map(d, r).roff;
-    const float pxf = (float)pm.px[0], pyf = (float)pm.py[0];
 
     // T < 0 marks a finished pixel (as in the single-channel forward)
-    float T = pm.inside[0] ? 1.0f : -1.0f;
-    uint32_t last = 0;
-    float acc[2 * GS_CSI_NC];
+    float T[PPL], pxf[PPL], pyf[PPL];
+    uint32_t last[PPL];
+    float acc[PPL][2 * GS_CSI_NC];
 #pragma unroll
-    for (int k = 0; k < 2 * GS_CSI_NC; ++k) acc[k] = 0.0f;
+    for (int q = 0; q < PPL; ++q) {
+        T[q] = pm.inside[q] ? 1.0f : -1.0f;
+        last[q] = 0;
+        pxf[q] = (float)pm.px[q];
+        pyf[q] = (float)pm.py[q];
+#pragma unroll
+        for (int k = 0; k < 2 * GS_CSI_NC; ++k) acc[q][k] = 0.0f;
+    }
 
     for (uint32_t start = rb; start < re; start += CSI_BATCH) {
-        if (ballot(T > 0.0f) == 0) break;
+        bool live = false;
+#pragma unroll
+        for (int q = 0; q < PPL; ++q) live |= T[q] > 0.0f;
+        if (ballot(live) == 0) break;
         const uint32_t e = start + lane;
         bool keep = false;
         if (lane < CSI_BATCH && e < re) {
@@ -88,35 +104,52 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd_csi(Dims d, RenderCsiArg
             mask &= mask - 1;
             const float4 A = s0[j];
             const float4 B = s1[j];
-            const float dx = A.x - pxf, dy = A.y - pyf;
-            const float power = pair_power(A.z, A.w, B.x, dx, dy);
-            if (power > 0.0f || power < B.z) continue;
-            const float alpha = fminf(0.99f, B.y * gs_expf_nonpos(power));
-            if (alpha < 1.0f / 255.0f) continue;
-            const float test_T = T * (1.0f - alpha);
-            if (test_T < 0.0001f) { T = -fabsf(T); continue; }
-            const float w = alpha * T;
+            float w[PPL];
+            bool any = false;
+#pragma unroll
+            for (int q = 0; q < PPL; ++q) {
+                w[q] = 0.0f;
+                const float dx = A.x - pxf[q], dy = A.y - pyf[q];
+                const float power = pair_power(A.z, A.w, B.x, dx, dy);
+                if (power > 0.0f || power < B.z) continue;
+                const float alpha = fminf(0.99f, B.y * gs_expf_nonpos(power));
+                if (alpha < 1.0f / 255.0f) continue;
+                const float test_T = T[q] * (1.0f - alpha);
+                if (test_T < 0.0001f) { T[q] = -fabsf(T[q]); continue; }
+                w[q] = alpha * T[q];
+                T[q] = test_T;
+                last[q] = base + j;
+                any = true;
+            }
+            if (!any) continue;
+            // one LDS read of the entry's 52 emissions serves all PPL pixels;
+            // a pixel that skipped adds e * 0 = 0 (emissions are finite), which
+            // leaves its sums unchanged except for the sign of a zero
 #pragma unroll
             for (int k2 = 0; k2 < CSI_NC2; ++k2) {
                 const float4 E = se[j][k2];
-                acc[4 * k2 + 0] = fmaf(E.x, w, acc[4 * k2 + 0]);
-                acc[4 * k2 + 1] = fmaf(E.y, w, acc[4 * k2 + 1]);
-                acc[4 * k2 + 2] = fmaf(E.z, w, acc[4 * k2 + 2]);
-                acc[4 * k2 + 3] = fmaf(E.w, w, acc[4 * k2 + 3]);
+#pragma unroll
+                for (int q = 0; q < PPL; ++q) {
+                    acc[q][4 * k2 + 0] = fmaf(E.x, w[q], acc[q][4 * k2 + 0]);
+                    acc[q][4 * k2 + 1] = fmaf(E.y, w[q], acc[q][4 * k2 + 1]);
+                    acc[q][4 * k2 + 2] = fmaf(E.z, w[q], acc[q][4 * k2 + 2]);
+                    acc[q][4 * k2 + 3] = fmaf(E.w, w[q], acc[q][4 * k2 + 3]);
+                }
             }
-            T = test_T;
-            last = base + j;
         }
         __syncthreads();
     }
-    if (!pm.inside[0]) return;
-    const size_t p = (size_t)r * d.W * d.H + (size_t)pm.py[0] * d.W + pm.px[0];
-    float4* o = reinterpret_cast<float4*>(a.out + p * (2 * GS_CSI_NC));
 #pragma unroll
-    for (int k2 = 0; k2 < CSI_NC2; ++k2)
-        o[k2] = make_float4(acc[4 * k2], acc[4 * k2 + 1], acc[4 * k2 + 2], acc[4 * k2 + 3]);
-    a.finalT[p] = fabsf(T);
-    a.ncontrib[p] = last;
+    for (int q = 0; q < PPL; ++q) {
+        if (!pm.inside[q]) continue;
+        const size_t p = (size_t)r * d.W * d.H + (size_t)pm.py[q] * d.W + pm.px[q];
+        float4* o = reinterpret_cast<float4*>(a.out + p * (2 * GS_CSI_NC));
+#pragma unroll
+        for (int k2 = 0; k2 < CSI_NC2; ++k2)
+            o[k2] = make_float4(acc[q][4 * k2], acc[q][4 * k2 + 1], acc[q][4 * k2 + 2], acc[q][4 * k2 + 3]);
+        a.finalT[p] = fabsf(T[q]);
+        a.ncontrib[p] = last[q];
+    }
 }
 
 __global__ void __launch_bounds__(GS_WAVE) k_render_bwd_csi(Dims d, RenderCsiArgs a)
@@ -209,18 +242,19 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd_csi(Dims d, RenderCsiArg
             if (c) {
                 T = T * __builtin_amdgcn_rcpf(1.0f - alpha);
                 const float w = alpha * T;
+                // subcarriers from the top: S_m = sum_{k >= m} p_k accumulates
+                // by FMAs, Z1 = sum_k k p_k = sum_{m >= 1} S_m by adds, Z = S_0
                 float zr = 0.0f, zi = 0.0f, z1r = 0.0f, z1i = 0.0f;
 #pragma unroll
-                for (int k2 = 0; k2 < CSI_NC2; ++k2) {
+                for (int k2 = CSI_NC2 - 1; k2 >= 0; --k2) {
                     const float4 U = su[j][k2];
                     const float gr0 = g[4 * k2], gi0 = g[4 * k2 + 1], gr1 = g[4 * k2 + 2], gi1 = g[4 * k2 + 3];
-                    const float pr0 = fmaf(gr0, U.x, gi0 * U.y), pi0 = fmaf(gr0, U.y, -(gi0 * U.x));
-                    const float pr1 = fmaf(gr1, U.z, gi1 * U.w), pi1 = fmaf(gr1, U.w, -(gi1 * U.z));
-                    const float k0 = (float)(2 * k2), k1 = (float)(2 * k2 + 1);
-                    zr = zr + pr0; zi = zi + pi0;
-                    z1r = fmaf(k0, pr0, z1r); z1i = fmaf(k0, pi0, z1i);
-                    zr = zr + pr1; zi = zi + pi1;
-                    z1r = fmaf(k1, pr1, z1r); z1i = fmaf(k1, pi1, z1i);
+                    zr = fmaf(gr1, U.z, zr); zr = fmaf(gi1, U.w, zr);
+                    zi = fmaf(gr1, U.w, zi); zi = fmaf(-gi1, U.z, zi);
+                    z1r = z1r + zr; z1i = z1i + zi;
+                    zr = fmaf(gr0, U.x, zr); zr = fmaf(gi0, U.y, zr);
+                    zi = fmaf(gr0, U.y, zi); zi = fmaf(-gi0, U.x, zi);
+                    if (k2 > 0) { z1r = z1r + zr; z1i = z1i + zi; }
                 }
                 const float h = fmaf(S.x, zr, -(S.y * zi));
                 acc = last_alpha * last_h + (1.0f - last_alpha) * acc;
@@ -254,10 +288,13 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd_csi(Dims d, RenderCsiArg
     }
 }
 
-hipError_t launch_render_fwd_csi(const Dims& d, const RenderCsiArgs& a, hipStream_t s)
+hipError_t launch_render_fwd_csi(const Dims& d, const RenderCsiArgs& a, int ppl, hipStream_t s)
 {
     if (d.R == 0 || d.T == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_render_fwd_csi, dim3((unsigned)(order_length(d) * 4)), dim3(GS_WAVE), 0, s, d, a);
+    if (ppl == 2)
+        hipLaunchKernelGGL(k_render_fwd_csi<2>, dim3((unsigned)(order_length(d) * 2)), dim3(GS_WAVE), 0, s, d, a);
+    else
+        hipLaunchKernelGGL(k_render_fwd_csi<1>, dim3((unsigned)(order_length(d) * 4)), dim3(GS_WAVE), 0, s, d, a);
     return hipGetLastError();
 }
 

@@ -547,7 +547,8 @@ extern "C" void orc_render_fwd_csi_f64(int W, int H, double f0, double df, const
 
 // Exact adjoint.  g9 [P*9] (double, accumulated): the eight single-channel
 // values (dpx, dpy, dqa, dqb, dqc, dopacity, dsig_re, dsig_im) and dL/dL.
-// Mirror: the kernels' Z / Z1 sums; twin: the direct sums over subcarriers.
+// Mirror: the kernels' Z / Z1 suffix sums; twin: the direct sums over
+// subcarriers.
 template <typename R>
 static void render_bwd_csi(int W, int H, double f0, double df, const uint32_t* ranges, const uint32_t* plist,
                            const R* xy, const R* conic_o, const R* sre, const R* sim, const R* plen,
@@ -583,16 +584,20 @@ static void render_bwd_csi(int W, int H, double f0, double df, const uint32_t* r
                 const R sr = sre[id], si = sim[id];
                 R h, dsr, dsi, dL;
                 if (mirror) {
+                    // suffix sums from the top subcarrier, as the kernel forms them
                     R zr = 0, zi = 0, z1r = 0, z1i = 0;
-                    for (int c = 0; c < NC; ++c) {
+                    for (int c = NC - 1; c >= 0; --c) {
                         R ur, ui;
                         ph.u(plen[id], c, ur, ui);
                         const R gr = gp[2 * c], gi = gp[2 * c + 1];
-                        const R pr = Fn<R>::fma(gr, ur, gi * ui), pi = Fn<R>::fma(gr, ui, -(gi * ur));
-                        zr = zr + pr;
-                        zi = zi + pi;
-                        z1r = Fn<R>::fma((R)c, pr, z1r);
-                        z1i = Fn<R>::fma((R)c, pi, z1i);
+                        zr = Fn<R>::fma(gr, ur, zr);
+                        zr = Fn<R>::fma(gi, ui, zr);
+                        zi = Fn<R>::fma(gr, ui, zi);
+                        zi = Fn<R>::fma(-gi, ur, zi);
+                        if (c > 0) {
+                            z1r = z1r + zr;
+                            z1i = z1i + zi;
+                        }
                     }
                     h = Fn<R>::fma(sr, zr, -(si * zi));
                     dsr = w * zr;

@@ -31,13 +31,19 @@ def _fwd(sc, d, **kw):
     return out
 
 
+@pytest.mark.parametrize("ppl", [1, 2], ids=["quarter_tile", "half_tile"])
 @pytest.mark.parametrize("seed,P,R", [(101, 300, 1), (102, 1500, 3)])
-def test_csi_forward_matches_oracle(seed, P, R):
+def test_csi_forward_matches_oracle(seed, P, R, ppl):
+    import gsparc_hip as gh
     sc = S.small_scene(seed, P, R=R, walls=True)
     sc.mlp *= 3.0
     d = to_dev(sc)
-    print("csi forward", seed, P, R, flush=True)
-    K, out, radii, geom, binbuf, img = _fwd(sc, d)
+    print("csi forward", seed, P, R, ppl, flush=True)
+    gh.set_option("csi_fwd_ppl", ppl)
+    try:
+        K, out, radii, geom, binbuf, img = _fwd(sc, d)
+    finally:
+        gh.set_option("csi_fwd_ppl", 0)
     out = out.cpu().numpy()
     for r in range(R):
         pre, plen, vals, ranges, fwd = O.render_csi(sc, r)
