@@ -291,6 +291,24 @@ struct SsimBwdArgs {
 };
 hipError_t launch_ssim_bwd(const SsimBwdArgs& a, hipStream_t s);
 
+// Shared emission MLP on MFMA (k_mlp_mfma.hip); rows = R*P, row = r*P + g.
+struct Mlp1Args {
+    int P, R;
+    const float* pe1;        // [P,128]  PE(mu) W1pe^T + b1
+    const float* rxf;        // [R*P,5]  receiver features per row
+    const float* w1rx;       // [128,5]
+    const float* w2;         // [128,128] (out, in)
+    const float* b2;         // [128]
+    const float* w3;         // [128]
+    const float* b3;         // [1]
+    float* out;              // [R*P,2] emission (a, 0)
+};
+int mlp1_bwd_groups(long long n_rows);
+size_t mlp1_part_floats(long long n_rows);
+hipError_t launch_mlp1_fwd(const Mlp1Args& a, hipStream_t s);
+// grads: dW2 [128*128] | dw3 [128] | db2 [128] | db3 [1]; dh1 [R*P,128]
+hipError_t launch_mlp1_bwd(const Mlp1Args& a, const float* da, float* dh1, float* part, float* grads, hipStream_t s);
+
 // CSI loss pieces (k_loss_csi.hip): magnitude planes + L1 partials, the
 // gradient back to the interleaved map, and the two-part loss reduction.
 int csi_loss_blocks(size_t npix);

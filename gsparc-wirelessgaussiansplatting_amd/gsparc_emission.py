@@ -16,8 +16,16 @@ pair.  It has no counterpart in the reference; its definition is this:
 
 Layer 1 is evaluated split: W1 = [W1_pe | W1_rx], with PE(mu) W1_pe^T formed
 once per Gaussian and shared by all receivers, so only the 5 receiver
-features are multiplied per pair.  The contractions are plain FP32 GEMMs run
-by hipBLASLt (matrix cores, FP32 accumulate); the backward is autograd's.
+features are multiplied per pair.
+
+Two backends compute the same function:
+  "mfma"   (default on the GPU) the per-pair part -- layer 1's receiver term,
+           the 128x128 layer, the ReLUs and the output dot -- runs in this
+           build's fused kernels, the 128x128 layer as FP32 MFMA tiles
+           (gsparc_mlp1_forward / _backward); torch keeps the plain GEMM
+           PE(mu) W1_pe^T, the feature functions and their chain rule.
+  "torch"  every contraction as a torch FP32 GEMM (hipBLASLt on the GPU),
+           autograd for the backward; also the CPU / FP64 reference.
 """
 from __future__ import annotations
 
@@ -61,11 +69,56 @@ def receiver_features(means: torch.Tensor, rx_pos: torch.Tensor, tx_pos: torch.T
     return torch.cat([d, inv, tdir[None].expand(R, -1, -1)], dim=-1)
 
 
+class _MlpCoreMfma(torch.autograd.Function):
+    """a [R*P] from pe1 [P,128] (= PE W1_pe^T + b1) and rxf [R*P,5] through the
+    fused MFMA kernels; backward returns the gradients of all seven inputs."""
+
+    @staticmethod
+    def forward(ctx, pe1, rxf, w1rx, w2, b2, w3, b3):
+        import gsparc_hip as gh
+        P = pe1.shape[0]
+        R = rxf.shape[0] // max(P, 1)
+        ins = [gh.f32(x) for x in (pe1, rxf, w1rx, w2, b2, w3, b3)]
+        gh.require_cuda(*ins)
+        em = torch.empty((R * P, 2), dtype=torch.float32, device=pe1.device)
+        gh.check(gh.lib().gsparc_mlp1_forward(P, R, *[gh.ptr(x) for x in ins], gh.ptr(em),
+                                              gh.stream_handle(pe1.device)), "mlp1 forward")
+        ctx.save_for_backward(*ins)
+        ctx.PR = (P, R)
+        return em[:, 0]
+
+    @staticmethod
+    def backward(ctx, da):
+        import gsparc_hip as gh
+        pe1, rxf, w1rx, w2, b2, w3, b3 = ctx.saved_tensors
+        P, R = ctx.PR
+        dev = pe1.device
+        dem = torch.zeros((R * P, 2), dtype=torch.float32, device=dev)
+        dem[:, 0] = da
+        dh1 = torch.empty((R * P, HIDDEN), dtype=torch.float32, device=dev)
+        L = gh.lib()
+        work = torch.empty(L.gsparc_mlp1_work_size(P, R), dtype=torch.uint8, device=dev)
+        grads = torch.empty(HIDDEN * HIDDEN + 2 * HIDDEN + 1, dtype=torch.float32, device=dev)
+        gh.check(L.gsparc_mlp1_backward(P, R, *[gh.ptr(x) for x in (pe1, rxf, w1rx, w2, b2, w3, b3)], gh.ptr(dem),
+                                        gh.ptr(dh1), gh.ptr(work), gh.ptr(grads), gh.stream_handle(dev)),
+                 "mlp1 backward")
+        dw2 = grads[:HIDDEN * HIDDEN].view(HIDDEN, HIDDEN)
+        dw3 = grads[HIDDEN * HIDDEN:HIDDEN * HIDDEN + HIDDEN].view(1, HIDDEN)
+        db2 = grads[HIDDEN * HIDDEN + HIDDEN:HIDDEN * HIDDEN + 2 * HIDDEN]
+        db3 = grads[-1:]
+        dpe1 = dh1.view(R, P, HIDDEN).sum(0)
+        dw1rx = dh1.t() @ rxf
+        drxf = dh1 @ w1rx
+        return dpe1, drxf, dw1rx, dw2, db2, dw3, db3
+
+
 class SharedEmissionMLP(torch.nn.Module):
     """68 -> 128 -> 128 -> 1 (ReLU) amplitude MLP shared by all Gaussians."""
 
-    def __init__(self, generator: torch.Generator | None = None, dtype=torch.float32, device=None):
+    def __init__(self, generator: torch.Generator | None = None, dtype=torch.float32, device=None,
+                 backend: str | None = None):
         super().__init__()
+        self.backend = backend
 
         def lin(n_out, n_in):
             bound = 1.0 / math.sqrt(n_in)
@@ -77,9 +130,20 @@ class SharedEmissionMLP(torch.nn.Module):
         self.W2, self.b2 = lin(HIDDEN, HIDDEN)
         self.w3, self.b3 = lin(1, HIDDEN)
 
+    def _use_mfma(self, means):
+        b = self.backend or ("mfma" if means.is_cuda and means.dtype == torch.float32 else "torch")
+        if b == "mfma" and not (means.is_cuda and means.dtype == torch.float32):
+            raise RuntimeError("the mfma backend needs float32 GPU tensors")
+        return b == "mfma"
+
     def amplitude(self, means: torch.Tensor, rx_pos: torch.Tensor, tx_pos: torch.Tensor) -> torch.Tensor:
         """[R,P] amplitude for every (receiver, Gaussian) pair."""
         W1_pe, W1_rx = self.W1[:, :N_PE], self.W1[:, N_PE:]
+        if self._use_mfma(means):
+            pe1 = positional_encoding(means) @ W1_pe.t() + self.b1                # [P,128]
+            rxf = receiver_features(means, rx_pos, tx_pos).reshape(-1, N_RX)      # [R*P,5]
+            a = _MlpCoreMfma.apply(pe1, rxf, W1_rx, self.W2, self.b2, self.w3, self.b3)
+            return a.view(rx_pos.shape[0], means.shape[0])
         pe = positional_encoding(means) @ W1_pe.t()                          # [P,128], once per Gaussian
         rxf = receiver_features(means, rx_pos, tx_pos)                     # [R,P,5]
         h1 = torch.relu(pe[None] + rxf @ W1_rx.t() + self.b1)             # [R,P,128]

@@ -1,7 +1,9 @@
 """Config 1 on the GPU: the rasterizer's emission input and the full
-single-receiver training step (20k Gaussians, shared 68->128->128->1 MLP on
-hipBLASLt FP32 GEMMs, 0.8*L1 + 0.2*(1-SSIM)) against the CPU chain of the
-oracle's FP32 mirror (same discrete decisions as the GPU) and a float64 MLP."""
+single-receiver training step (20k Gaussians, shared 68->128->128->1 MLP,
+0.8*L1 + 0.2*(1-SSIM)) against the CPU chain of the oracle's FP32 mirror (same
+discrete decisions as the GPU) and a float64 MLP, with the MLP on the fused
+MFMA kernels and on torch FP32 GEMMs; and the MFMA kernels against the torch
+FP32 form of the same MLP on ragged row counts."""
 import numpy as np
 import pytest
 import torch
@@ -57,13 +59,14 @@ def test_emission_input_parity(mode):
         assert _rel(dem[vis, 1], g8[vis, 7]) < 1e-4
 
 
-def test_config1_training_step():
+@pytest.mark.parametrize("backend", ["mfma", "torch"])
+def test_config1_training_step(backend):
     sc = S.config1()
     tgt = S.config1(seed=2)
     d = to_dev(sc)
     dt = to_dev(tgt)
     mlp64 = _mlp64(3)
-    mlp = GE.SharedEmissionMLP().cuda()
+    mlp = GE.SharedEmissionMLP(backend=backend).cuda()
     with torch.no_grad():
         for p32, p64 in zip(mlp.parameters(), mlp64.parameters()):
             p32.copy_(p64.float())
@@ -87,3 +90,31 @@ def test_config1_training_step():
     print("relative errors", errs)
     for name, e in errs.items():
         assert e < 1e-4, (name, errs)
+
+
+@pytest.mark.parametrize("P,R", [(5000, 3), (37, 1), (300, 7)])
+def test_mfma_mlp_matches_torch(P, R):
+    """Amplitudes within 1e-5 and every gradient (weights, biases, means)
+    within 1e-4 of the torch FP32 form; row counts not multiples of the
+    kernels' 128 / 64-row tiles."""
+    print("mfma mlp", P, R, flush=True)
+    g = torch.Generator().manual_seed(P + R)
+    means = (torch.rand((P, 3), generator=g) * torch.tensor([6.0, 6.0, 3.0])).cuda()
+    rx = (0.5 + torch.rand((R, 3), generator=g) * 2.0).cuda()
+    tx = torch.tensor([0.0, 0.5, 3.0]).cuda()
+    ref = GE.SharedEmissionMLP(generator=torch.Generator().manual_seed(5), backend="torch").cuda()
+    fast = GE.SharedEmissionMLP(generator=torch.Generator().manual_seed(5), backend="mfma").cuda()
+    w = torch.randn((R, P), generator=g).cuda()
+    out = {}
+    for name, m in (("ref", ref), ("fast", fast)):
+        mu = means.clone().requires_grad_(True)
+        a = m.amplitude(mu, rx, tx)
+        (a * w).sum().backward()
+        out[name] = (a.detach(), mu.grad, {n: p.grad for n, p in m.named_parameters()})
+    torch.cuda.synchronize()
+    a_ref, gm_ref, gp_ref = out["ref"]
+    a_fast, gm_fast, gp_fast = out["fast"]
+    assert _rel(a_fast.cpu(), a_ref.cpu()) < 1e-5
+    assert _rel(gm_fast.cpu(), gm_ref.cpu()) < 1e-4
+    for n in gp_ref:
+        assert _rel(gp_fast[n].cpu(), gp_ref[n].cpu()) < 1e-4, n
