@@ -20,6 +20,10 @@ scenes with Gaussian counts log-uniform in [5k, 200k] and 4..64 receivers
 each, forward+backward of all of them in one launch sequence; it adds p50/p99
 of the batch step and of single spectra rendered alone.
 
+--workload config1 measures the single-receiver training step with the shared
+emission MLP (20k Gaussians) and, beside it, the MLP's MFMA kernels at 3.2M
+(receiver, Gaussian) rows against the FP32 matrix peak.
+
 --workload csi measures multi-frequency maps: 50k Gaussians, 32 receivers,
 52 subcarriers (complex, 90x360x104 floats per receiver), L1 on the complex
 map + SSIM on the magnitudes, forward+backward.
@@ -58,9 +62,10 @@ def parse():
     ap.add_argument("--fwd-ppl", type=int, default=0, help="forward blend pixels per lane (0: library default)")
     ap.add_argument("--bwd-ppl", type=int, default=0, help="backward blend pixels per lane (0: automatic)")
     ap.add_argument("--order", type=int, default=0, help="blend work order (0: automatic)")
-    ap.add_argument("--workload", default="receivers", choices=["receivers", "scenes", "csi"],
+    ap.add_argument("--workload", default="receivers", choices=["receivers", "scenes", "csi", "config1"],
                     help="receivers: one scene, R receivers (default); scenes: 16 uneven scenes in one batch; "
-                         "csi: 52-subcarrier complex maps")
+                         "csi: 52-subcarrier complex maps; config1: single-receiver training step with the "
+                         "shared 68->128->128->1 emission MLP on MFMA")
     ap.add_argument("--csi-receivers", type=int, default=32)
     ap.add_argument("--no-tune", action="store_true", help="skip the blend-mapping autotuner")
     ap.add_argument("--eager", action="store_true",
@@ -389,6 +394,89 @@ def setup_csi(args, dev):
                 "grad_mode": "exact"})
 
 
+MLP_FLOPS_FWD = 2 * (128 * 128 + 128 * 5 + 128)    # per row, in the fused forward kernel
+MLP_FLOPS_BWD = 3 * 2 * 128 * 128                  # per row: recompute, dW2, dh1 (MFMA)
+PEAK_FP32_MFMA_TFLOPS = 157.3
+
+
+def setup_config1(args, dev):
+    """Single receiver, 20k Gaussians, shared MLP emission (MFMA kernels)."""
+    import numpy as np
+    import torch
+    import cuda_rasterizer as cr
+    import gsparc_emission as GE
+    import gsparc_scenes as S
+    import gsparc_train as T
+    import wireless_rasterizer as wr
+    sc = S.config1()
+    tgt = S.config1(seed=2)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    means, scales, rots, opac = t(sc.means), t(sc.scales), t(sc.rotations), t(sc.opacities)
+    views, txp, rxp = t(sc.views), t(sc.tx_pos), t(sc.rx_pos)
+    mlp = GE.SharedEmissionMLP(generator=torch.Generator().manual_seed(1), backend="mfma").to(dev)
+    with torch.no_grad():
+        Et = mlp(t(tgt.means), rxp, txp)
+        target = wr.forward_batched(t(tgt.means), None, t(tgt.opacities), t(tgt.scales), t(tgt.rotations), sc.H,
+                                    sc.W, 1.0, views, txp, rxp, emission=Et)[1].clone()
+    params = [x.clone().requires_grad_(True) for x in (means, opac)]
+    settings = cr.BatchedSetting(sc.H, sc.W, 1.0, views, txp, rxp, "exact")
+
+    def step():
+        m, o = params
+        E = mlp(m, rxp, txp)
+        ore, oim, radii = cr.rasterize_gaussians_batched(m, None, o, scales, rots, settings, emission=E)
+        T.L1SSIMLoss.apply(ore, target).backward()
+        for p_ in list(params) + list(mlp.parameters()):
+            p_.grad = None
+
+    def single(i):
+        with torch.no_grad():
+            E = mlp(means, rxp, txp)
+            wr.forward_batched(means, None, opac, scales, rots, sc.H, sc.W, 1.0, views, txp, rxp, emission=E)
+
+    def count():
+        with torch.no_grad():
+            E = mlp(means, rxp, txp)
+        return wr.forward_batched(means, None, opac, scales, rots, sc.H, sc.W, 1.0, views, txp, rxp, emission=E)[0]
+
+    def mlp_rows():
+        """The MLP alone at 50k Gaussians x 64 receivers (3.2M rows), fwd+bwd."""
+        import gsparc_hip as gh
+        s2 = S.config2()
+        mu = t(s2.means).requires_grad_(True)
+        rx2, tx2 = t(s2.rx_pos), t(s2.tx_pos)
+        w = torch.randn((s2.R, s2.P), device=dev)
+        for _ in range(2):
+            (mlp.amplitude(mu, rx2, tx2) * w).sum().backward()
+        torch.cuda.synchronize()
+        gh.profile_enable(True, timing=True, counting=False)
+        n = 5
+        for _ in range(n):
+            (mlp.amplitude(mu, rx2, tx2) * w).sum().backward()
+        torch.cuda.synchronize()
+        kt = gh.profile_collect()
+        gh.profile_enable(False)
+        rows = s2.R * s2.P
+        f_ms = kt["mlp1_fwd"][0] / kt["mlp1_fwd"][1]
+        b_ms = kt["mlp1_bwd"][0] / kt["mlp1_bwd"][1]
+        ft = MLP_FLOPS_FWD * rows / (f_ms * 1e-3) / 1e12
+        bt = MLP_FLOPS_BWD * rows / (b_ms * 1e-3) / 1e12
+        for p_ in mlp.parameters():
+            p_.grad = None
+        return {"rows": rows, "fwd_ms": round(f_ms, 4), "bwd_ms": round(b_ms, 4), "fwd_tflops": round(ft, 2),
+                "bwd_tflops": round(bt, 2), "peak_tflops": PEAK_FP32_MFMA_TFLOPS,
+                "fwd_frac": round(ft / PEAK_FP32_MFMA_TFLOPS, 4), "bwd_frac": round(bt / PEAK_FP32_MFMA_TFLOPS, 4),
+                "flops_per_row": {"fwd": MLP_FLOPS_FWD, "bwd": MLP_FLOPS_BWD}}
+
+    return Workload(
+        step=step, step_plain=None, single=single, count=count, cpu=lambda: None, graphed=None, tune=None,
+        spectra=1, Ptot=sc.P, Nrec=sc.P, H=sc.H, W=sc.W, traffic_key=None, mlp_rows=mlp_rows,
+        data="synthetic (seeded: 20k Gaussians uniform in a 6x6x3 m room, receiver at the centre)",
+        config={"workload": "single-receiver training step, shared 68->128->128->1 emission MLP (FP32 MFMA), "
+                            "0.8*L1 + 0.2*(1-SSIM)", "gaussians": sc.P, "receivers": 1, "map": f"{sc.H}x{sc.W}",
+                "grad_mode": "exact"})
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -406,7 +494,8 @@ def main():
     gh.set_option("render_fwd_ppl", args.fwd_ppl)
     gh.set_option("render_bwd_ppl", args.bwd_ppl)
     gh.set_option("render_order", args.order)
-    wl = {"receivers": setup_receivers, "scenes": setup_scenes, "csi": setup_csi}[args.workload](args, dev)
+    wl = {"receivers": setup_receivers, "scenes": setup_scenes, "csi": setup_csi,
+          "config1": setup_config1}[args.workload](args, dev)
     R, Ptot, Nrec = wl.spectra, wl.Ptot, wl.Nrec
     tuned = None
     if wl.tune is not None and not args.no_tune and not (args.fwd_ppl or args.bwd_ppl or args.order):
@@ -454,7 +543,7 @@ def main():
             step_lat.append(1e3 * timed(wl.step, 1))
 
     plain_value = None
-    if args.plain_steps > 0:
+    if args.plain_steps > 0 and wl.step_plain is not None:
         wl.step_plain()
         log(args, f"timing {args.plain_steps} plain steps (one receiver per call)")
         el_plain = timed(wl.step_plain, args.plain_steps)
@@ -512,6 +601,9 @@ def main():
         "ssim_fwd": ("hbm", 8 * N * R + 16 * N * R, "GB/s"),
         "ssim_bwd": ("hbm", 28 * N * R, "GB/s"),
     }
+    if getattr(wl, "mlp_rows", None):
+        work["mlp1_fwd"] = ("mfma", MLP_FLOPS_FWD * Nrec, "TFLOP/s")
+        work["mlp1_bwd"] = ("mfma", MLP_FLOPS_BWD * Nrec, "TFLOP/s")
     if getattr(wl, "csi", False):
         # 104 floats per pixel out (forward) and dL/dout in (backward); the
         # SSIM runs on 52 magnitude planes per receiver
@@ -535,8 +627,14 @@ def main():
         entry = {"ms_per_launch": round(avg, 4), "launches_per_step": n / nprof}
         if name in work:
             bound, amount, unit = work[name]
-            ach = amount / (avg * 1e-3) / 1e9
-            entry.update(bound=bound, achieved=round(ach, 1), unit=unit, frac=round(ach / PEAK_HBM_GBS, 4))
+            if unit == "TFLOP/s":
+                ach = amount / (avg * 1e-3) / 1e12
+                frac = ach / PEAK_FP32_MFMA_TFLOPS
+            else:
+                ach = amount / (avg * 1e-3) / 1e9
+                frac = ach / PEAK_HBM_GBS
+            entry.update(bound=bound, achieved=round(ach, 3 if unit == "TFLOP/s" else 1), unit=unit,
+                         frac=round(frac, 4))
         if name in flops:
             entry["fp32_tflops"] = round(flops[name] / (avg * 1e-3) / 1e12, 3)
         if name in traffic:
@@ -544,7 +642,8 @@ def main():
         kernels[name] = entry
     dom = max(kt.items(), key=lambda kv: kv[1][0])[0]
     d = kernels.get(dom, {})
-    roof = {"kernel": dom, "bound": d.get("bound"), "achieved": d.get("achieved"), "peak": PEAK_HBM_GBS,
+    roof = {"kernel": dom, "bound": d.get("bound"), "achieved": d.get("achieved"),
+            "peak": PEAK_FP32_MFMA_TFLOPS if d.get("unit") == "TFLOP/s" else PEAK_HBM_GBS,
             "unit": d.get("unit"), "frac": d.get("frac"), "traffic": traffic.get(dom),
             "ms_per_launch": d.get("ms_per_launch")}
     if dom in flops:
@@ -560,7 +659,8 @@ def main():
 
     metric = {"receivers": METRIC,
               "scenes": "fwd+bwd spectra/sec, 16 uneven scenes (5k-200k Gaussians, 4-64 rx each), 90x360, one batch",
-              "csi": "fwd+bwd CSI maps/sec, 50k Gaussians, 90x360x52 complex subcarriers, 32 rx batched"}
+              "csi": "fwd+bwd CSI maps/sec, 50k Gaussians, 90x360x52 complex subcarriers, 32 rx batched",
+              "config1": "training steps/sec, 20k Gaussians, 1 rx, shared emission MLP on MFMA"}
     out = {
         "metric": metric[args.workload],
         "value": round(value, 2),
@@ -588,6 +688,8 @@ def main():
         "kernels": kernels,
         "cpu_baseline": cpu,
     }
+    if getattr(wl, "mlp_rows", None):
+        out["mlp_mfma"] = wl.mlp_rows()
     if step_lat:
         out["batch_step_ms"] = {"p50": round(float(np.percentile(step_lat, 50)), 3),
                                 "p99": round(float(np.percentile(step_lat, 99)), 3)}

@@ -6,8 +6,9 @@
 //   h1 = relu(pe1[g] + W1rx rxf[row])      pe1 = PE(mu) W1pe^T + b1   [P,128]
 //   h2 = relu(W2 h1 + b2)                  W2 [128,128] (out, in)
 //   a  = w3 . h2 + b3
-// the forward kernel computes a for 128 rows per workgroup (4 waves x 32
-// rows): h1 on the vector ALU into LDS, the 128x128 layer as 32x32x2 MFMA
+// the forward kernel computes a for 128-row tiles (4 waves x 32 rows; the
+// grid strides over the tiles, so W2 is staged once per workgroup): h1 on
+// the vector ALU into LDS, the 128x128 layer as 32x32x2 MFMA
 // tiles (A = h1 rows from LDS, B = W2^T from LDS), relu, bias and the w3 dot
 // in the epilogue with a 32-lane sum.  The backward kernel recomputes h1 and
 // h2 for 64-row tiles and forms
@@ -85,42 +86,45 @@ __global__ void __launch_bounds__(256) k_mlp1_fwd(Mlp1Args a)
     __shared__ float h1s[128 * M1_S];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const long long n_rows = (long long)a.P * a.R;
-    const long long row0 = (long long)blockIdx.x * 128;
-    load_w2t(a.w2, w2t);
-    layer1(a, row0, 128, h1s);
-    __syncthreads();
-    floatx16 acc[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = floatx16{};
+    load_w2t(a.w2, w2t);   // once per workgroup: the grid strides over 128-row tiles
+    const float b3 = a.b3[0];
     const float* hrow = h1s + (32 * w + (lane & 31)) * M1_S + (lane >> 5);
     const float* bcol = w2t + (lane >> 5) * M1_S + (lane & 31);
-    for (int ks = 0; ks < M1_H / 2; ++ks) {
-        const float av = hrow[2 * ks];
+    for (long long tile = blockIdx.x; tile * 128 < n_rows; tile += gridDim.x) {
+        const long long row0 = tile * 128;
+        __syncthreads();   // w2t loaded / the previous tile's h1s reads done
+        layer1(a, row0, 128, h1s);
+        __syncthreads();
+        floatx16 acc[4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) acc[t] = mfma32x32x2(av, bcol[2 * ks * M1_S + 32 * t], acc[t]);
-    }
-    float part[16];
+        for (int t = 0; t < 4; ++t) acc[t] = floatx16{};
+        for (int ks = 0; ks < M1_H / 2; ++ks) {
+            const float av = hrow[2 * ks];
 #pragma unroll
-    for (int v = 0; v < 16; ++v) part[v] = 0.0f;
+            for (int t = 0; t < 4; ++t) acc[t] = mfma32x32x2(av, bcol[2 * ks * M1_S + 32 * t], acc[t]);
+        }
+        float part[16];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        const int nn = 32 * t + (lane & 31);
-        const float bb = a.b2[nn], ww = a.w3[nn];
+        for (int v = 0; v < 16; ++v) part[v] = 0.0f;
 #pragma unroll
-        for (int v = 0; v < 16; ++v) part[v] = fmaf(ww, fmaxf(acc[t][v] + bb, 0.0f), part[v]);
-    }
+        for (int t = 0; t < 4; ++t) {
+            const int nn = 32 * t + (lane & 31);
+            const float bb = a.b2[nn], ww = a.w3[nn];
 #pragma unroll
-    for (int v = 0; v < 16; ++v)
+            for (int v = 0; v < 16; ++v) part[v] = fmaf(ww, fmaxf(acc[t][v] + bb, 0.0f), part[v]);
+        }
 #pragma unroll
-        for (int off = 1; off < 32; off <<= 1) part[v] += __shfl_xor(part[v], off);
-    if ((lane & 31) == 0) {
-        const float b3 = a.b3[0];
+        for (int v = 0; v < 16; ++v)
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
-            const long long row = row0 + 32 * w + acc_row(v, lane);
-            if (row < n_rows) {
-                a.out[2 * row] = part[v] + b3;
-                a.out[2 * row + 1] = 0.0f;
+            for (int off = 1; off < 32; off <<= 1) part[v] += __shfl_xor(part[v], off);
+        if ((lane & 31) == 0) {
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const long long row = row0 + 32 * w + acc_row(v, lane);
+                if (row < n_rows) {
+                    a.out[2 * row] = part[v] + b3;
+                    a.out[2 * row + 1] = 0.0f;
+                }
             }
         }
     }
@@ -263,7 +267,8 @@ hipError_t launch_mlp1_fwd(const Mlp1Args& a, hipStream_t s)
 {
     const long long n_rows = (long long)a.P * a.R;
     if (n_rows == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_mlp1_fwd, dim3((unsigned)((n_rows + 127) / 128)), dim3(256), 0, s, a);
+    const long long tiles = (n_rows + 127) / 128;
+    hipLaunchKernelGGL(k_mlp1_fwd, dim3((unsigned)(tiles < 1024 ? tiles : 1024)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
