@@ -62,28 +62,37 @@ __device__ __forceinline__ float dpp_move(float v)
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
 }
 
+// Lane-pair sums through the gfx950 lane-swap instructions (VALU, no LDS
+// round trip).  v_permlane32_swap exchanges lanes 32..63 of its first operand
+// with lanes 0..31 of its second, so after swap(a, b) the two results add up
+// to a[l] + a[l + 32] in lanes 0..31 and b[l - 32] + b[l] in lanes 32..63;
+// v_permlane16_swap does the same across 16 lanes within each half.
+__device__ __forceinline__ float swap_sum32(float a, float b)
+{
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+__device__ __forceinline__ float swap_sum16(float a, float b)
+{
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
 // Sums each of v[0..7] over the 64 lanes; lane 8*i returns the total of v[i]
 // (every lane of the 8-lane group i holds it).  Transpose reduction: the
-// exchanges across 32, 16 and 8 lanes each halve the number of values a lane
-// carries (4, 2, 1), then an 8-lane sum: quad_perm [1,0,3,2], quad_perm
-// [2,3,0,1], row_half_mirror.  ~30 instructions instead of 8 full reductions.
+// exchanges across 32 and 16 lanes (lane swaps) and 8 lanes (DPP row_ror:8)
+// each halve the number of values a lane carries (4, 2, 1), then an 8-lane
+// sum: quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror.
 __device__ __forceinline__ float wave_sum8(const float (&v)[8], int lane)
 {
-    const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8;
+    const bool b3 = lane & 8;
     float w[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const float keep = b5 ? v[i + 4] : v[i];
-        const float send = b5 ? v[i] : v[i + 4];
-        w[i] = keep + __shfl_xor(send, 32);
-    }
+    for (int i = 0; i < 4; ++i) w[i] = swap_sum32(v[i], v[i + 4]);   // lanes 0..31: v[i]; 32..63: v[i+4]
     float u[2];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        const float keep = b4 ? w[i + 2] : w[i];
-        const float send = b4 ? w[i] : w[i + 2];
-        u[i] = keep + __shfl_xor(send, 16);
-    }
+    for (int i = 0; i < 2; ++i) u[i] = swap_sum16(w[i], w[i + 2]);   // lanes with bit 4 clear: w[i]
     float x = (b3 ? u[1] : u[0]) + dpp_move<0x128>(b3 ? u[0] : u[1]);   // row_ror:8 = lane ^ 8
     x = x + dpp_move<0xb1>(x);
     x = x + dpp_move<0x4e>(x);
@@ -94,8 +103,12 @@ __device__ __forceinline__ float wave_sum8(const float (&v)[8], int lane)
 // Sum of x over the 64 lanes (every lane gets it).
 __device__ __forceinline__ float wave_sum(float x)
 {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off);
+    x = swap_sum32(x, x);
+    x = swap_sum16(x, x);
+    x = x + dpp_move<0x128>(x);
+    x = x + dpp_move<0xb1>(x);
+    x = x + dpp_move<0x4e>(x);
+    x = x + dpp_move<0x141>(x);
     return x;
 }
 
