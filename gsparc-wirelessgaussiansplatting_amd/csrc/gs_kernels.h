@@ -124,7 +124,11 @@ struct PreFwdArgs {
     int* radii;              // [Nrec]
     float* cov3;             // [P*6]
     float* plen;             // [Nrec] transmitter -> Gaussian -> receiver path length, or nullptr
+    uint32_t* rect;          // [Nrec] packed tile rect (rect_packed) or nullptr
 };
+// Tile rects packed in 32 bits: x0 | y0 << 8 | x1 << 16 | y1 << 24 (tile grids
+// up to 255 x 255).
+__host__ __device__ inline bool rect_packed(const Dims& d) { return d.gx <= 255 && d.gy <= 255; }
 hipError_t launch_preprocess_fwd(const Dims& d, const PreFwdArgs& a, hipStream_t s);
 
 struct BinArgs {
@@ -144,6 +148,8 @@ struct BinArgs {
     uint32_t* plist;         // [Kcap] Gaussian indices in tile-list order
     uint32_t* keys_sorted;   // optional [Kcap] their depth bits
     uint64_t Kcap;
+    const uint32_t* rect;    // [Nrec] packed tile rect per record (nullptr: rects from radii and splat)
+    uint32_t* srect;         // [Nrec] the same in depth order (written by the last sort pass)
 };
 int rs_blocks(int P);
 int chunk_len_for(const Dims& d);

@@ -108,6 +108,8 @@ void make_layout(const gs::Dims& d, uint64_t K, gsparc_layout& L)
     L.sort_hist = o; o = al(o + (size_t)R * 257 * gs::rs_blocks(d.P) * sizeof(uint32_t));
     L.nvis = o; o = al(o + (size_t)R * sizeof(uint32_t));
     L.plen = o; o = al(o + NR * sizeof(float));
+    L.rect = o; o = al(o + NR * sizeof(uint32_t));
+    L.srect = o; o = al(o + NR * sizeof(uint32_t));
     L.geom_bytes = o + ALIGN;
     o = 0;
     L.final_t = o; o = al(o + (size_t)R * N * sizeof(float));
@@ -160,6 +162,10 @@ static gs::BinArgs bin_args(const gs::Dims& d, const gsparc_layout& L, char* gb,
     ba.ccnt = (uint32_t*)(ib + L.chunk_cnt);
     ba.chunk_len = gs::chunk_len_for(d);
     ba.nchunks = gs::chunk_count_for(d);
+    if (gs::rect_packed(d)) {
+        ba.rect = (const uint32_t*)(gb + L.rect);
+        ba.srect = (uint32_t*)(gb + L.srect);
+    }
     return ba;
 }
 
@@ -340,6 +346,7 @@ int forward_geom_impl(const gs::Dims& d, const FwdInputs& in, void* geom, void* 
     pa.views = in.views; pa.tx = in.tx; pa.rx = in.rx; pa.mod = in.mod;
     pa.splat = (float4*)(gb + L.splat); pa.sig = (float2*)(gb + L.sig); pa.depth = (float*)(gb + L.depth);
     pa.radii = radii_out; pa.cov3 = (float*)(gb + L.cov3); pa.plen = (float*)(gb + L.plen);
+    pa.rect = gs::rect_packed(d) ? (uint32_t*)(gb + L.rect) : nullptr;
     if (d.P > 0 && d.R > 0) GS_LAUNCH("preprocess_fwd", gs::launch_preprocess_fwd(d, pa, s), s);
     if (d.R == 0) return GSPARC_OK;
     gs::BinArgs ba = bin_args(d, L, gb, ib, radii_out);

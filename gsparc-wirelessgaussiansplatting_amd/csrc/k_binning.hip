@@ -189,6 +189,9 @@ __global__ void __launch_bounds__(GS_RS_THREADS) k_rs_scatter(Dims d, BinArgs a,
     __syncthreads();
     uint32_t* kout = a.skey[pass & 1] + rP;
     uint32_t* vout = a.sval[pass & 1] + rP;
+    // the last pass also puts the packed tile rects in depth order, so that
+    // the chunk kernels read them in sequence instead of gathering
+    const bool rects = !FIRST && pass == 3 && a.rect;
     for (int k = 0; k < GS_RS_ITEMS; ++k) {
         const int i = blk * GS_RS_TILE + k * GS_RS_THREADS + threadIdx.x;
         const bool valid = i < n;
@@ -203,6 +206,7 @@ __global__ void __launch_bounds__(GS_RS_THREADS) k_rs_scatter(Dims d, BinArgs a,
             for (int w = 0; w < wave; ++w) pos += wcnt[w][dg];
             kout[pos] = key;
             vout[pos] = val;
+            if (rects) a.srect[rP + pos] = a.rect[rP + val];
         }
         __syncthreads();
         for (int j = threadIdx.x; j < NB; j += GS_RS_THREADS) {
@@ -219,9 +223,18 @@ __global__ void __launch_bounds__(GS_RS_THREADS) k_rs_scatter(Dims d, BinArgs a,
 }
 
 // ---------------------------------------------------------------- chunks
-__device__ __forceinline__ void chunk_rect(const Dims& d, const BinArgs& a, size_t rP, uint32_t g, int& x0, int& y0,
-                                           int& x1, int& y1)
+// Tile rect of sorted position i (Gaussian g) of the receiver at rP.
+__device__ __forceinline__ void chunk_rect(const Dims& d, const BinArgs& a, size_t rP, int i, uint32_t g, int& x0,
+                                           int& y0, int& x1, int& y1)
 {
+    if (a.rect) {
+        const uint32_t p = a.srect[rP + i];
+        x0 = (int)(p & 0xffu);
+        y0 = (int)((p >> 8) & 0xffu);
+        x1 = (int)((p >> 16) & 0xffu);
+        y1 = (int)(p >> 24);
+        return;
+    }
     const int radius = a.radii[rP + g];
     x0 = y0 = x1 = y1 = 0;
     if (radius > 0) {
@@ -253,7 +266,7 @@ __global__ void __launch_bounds__(GS_RS_THREADS) k_chunk_count(Dims d, BinArgs a
     __syncthreads();
     for (int i = i0 + threadIdx.x; i < i1; i += GS_RS_THREADS) {
         int x0, y0, x1, y1;
-        chunk_rect(d, a, rP, a.sval[1][rP + i], x0, y0, x1, y1);
+        chunk_rect(d, a, rP, i, a.rect ? 0u : a.sval[1][rP + i], x0, y0, x1, y1);
         for (int ty = y0; ty < y1; ++ty)
             for (int tx = x0; tx < x1; ++tx) {
                 const int t = ty * d.gx + tx;
@@ -331,7 +344,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_chunk_scatter(Dims d, BinArgs a)
         if (i < i1) {
             g = a.sval[1][rP + i];
             dbits = a.skey[1][rP + i];
-            chunk_rect(d, a, rP, g, x0, y0, x1, y1);
+            chunk_rect(d, a, rP, i, g, x0, y0, x1, y1);
         }
         const bool ne = x0 < x1 && y0 < y1;
         const uint32_t area = ne ? (uint32_t)((x1 - x0) * (y1 - y0)) : 0u;

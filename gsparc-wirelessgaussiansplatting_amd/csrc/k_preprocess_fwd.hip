@@ -112,6 +112,9 @@ k_preprocess_fwd(Dims d, PreFwdArgs a)
                             a.splat[2 * ri] = make_float4(go.px, go.py, -0.5f * qa, -qb);
                             a.splat[2 * ri + 1] = make_float4(-0.5f * qc, o, thr, 0.0f);
                             a.depth[ri] = rd;
+                            if (a.rect)
+                                a.rect[ri] = (uint32_t)x0 | ((uint32_t)y0 << 8) | ((uint32_t)x1 << 16) |
+                                             ((uint32_t)y1 << 24);
                             if (a.plen) a.plen[ri] = gs_path_length(mean[0], mean[1], mean[2], tx, a.rx + 3 * r);
                             if (a.mlp) {
                                 const float s0 = 1.0f / (1.0f + gs_expf(-(z0[c] - ld)));

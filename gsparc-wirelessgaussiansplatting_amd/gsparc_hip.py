@@ -29,7 +29,7 @@ _sz = C.c_size_t
 class Layout(C.Structure):
     _fields_ = [(n, C.c_size_t) for n in (
         "splat", "sig", "depth", "cov3", "sort_key0", "sort_val0", "sort_key1", "sort_val1", "sort_hist", "nvis",
-        "plen", "geom_bytes",
+        "plen", "rect", "srect", "geom_bytes",
         "final_t", "n_contrib", "ranges", "segcnt", "segbase", "tile_order", "chunk_cnt", "total", "image_bytes",
         "plist", "keys_sorted", "bin_bytes")]
 

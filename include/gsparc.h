@@ -44,7 +44,8 @@ extern "C" {
 /* Byte offsets of the arrays inside the caller's buffers (for inspection). */
 typedef struct gsparc_layout {
     /* geom: per (receiver, Gaussian) records and the per-receiver depth sort */
-    size_t splat, sig, depth, cov3, sort_key0, sort_val0, sort_key1, sort_val1, sort_hist, nvis, plen, geom_bytes;
+    size_t splat, sig, depth, cov3, sort_key0, sort_val0, sort_key1, sort_val1, sort_hist, nvis, plen, rect, srect,
+        geom_bytes;
     /* image: per pixel and per tile state, per (chunk, tile) counts */
     size_t final_t, n_contrib, ranges, segcnt, segbase, tile_order, chunk_cnt, total, image_bytes;
     /* bin: the tile lists */
