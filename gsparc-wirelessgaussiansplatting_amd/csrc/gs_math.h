@@ -117,6 +117,34 @@ GS_HD float gs_expf_nonpos(float x)
     return (p * gs_pow2i(n1)) * gs_pow2i(n2);
 }
 
+// gs_expf on [-86, 0] with fewer operations, for the blend kernels: the
+// rounded n = rint(x log2 e) comes from adding 1.5 * 2^23 to the product (the
+// float sum rounds to an integer, round-to-nearest-even, exactly as rintf),
+// and 2^n is built in one step from that sum's low bits (n >= -124, so p * 2^n
+// is a normal float and the two-step scaling of gs_expf gives the same bits).
+// Bit-identical to gs_expf for every x in [-86, 0] (checked by the test
+// suite).  Below -86 it returns gs_expf(-86) ~ 4e-38: the blend only uses
+// exp(power) where power >= thr = -log(255 o) - 1e-3, i.e. >= -86 for any
+// opacity below 1e35.
+#define GS_EXP_MAGIC 12582912.0f   // 1.5 * 2^23
+GS_HD float gs_expf_blend(float x)
+{
+    x = x > -86.0f ? x : -86.0f;
+    const float t = x * GS_LOG2E + GS_EXP_MAGIC;
+    const float n = t - GS_EXP_MAGIC;
+    float r = __builtin_fmaf(-n, GS_LN2_HI, x);
+    r = __builtin_fmaf(-n, GS_LN2_LO, r);
+    float q = GS_EXP_Q_4;
+    q = __builtin_fmaf(q, r, GS_EXP_Q_3);
+    q = __builtin_fmaf(q, r, GS_EXP_Q_2);
+    q = __builtin_fmaf(q, r, GS_EXP_Q_1);
+    q = __builtin_fmaf(q, r, GS_EXP_Q_0);
+    const float p = __builtin_fmaf(r * r, q, r) + 1.0f;
+    // t's bit pattern is 0x4B400000 + n; shifted left by 23 the constant part
+    // drops out, leaving the exponent field of 2^n
+    return p * gs_as_float((gs_as_uint(t) << 23) + 0x3F800000u);
+}
+
 GS_HD float gs_logf(float x)
 {
     if (x != x) return x;

@@ -112,7 +112,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd_csi(Dims d, RenderCsiArg
                 const float dx = A.x - pxf[q], dy = A.y - pyf[q];
                 const float power = pair_power(A.z, A.w, B.x, dx, dy);
                 if (power > 0.0f || power < B.z) continue;
-                const float alpha = fminf(0.99f, B.y * gs_expf_nonpos(power));
+                const float alpha = fminf(0.99f, B.y * gs_expf_blend(power));
                 if (alpha < 1.0f / 255.0f) continue;
                 const float test_T = T[q] * (1.0f - alpha);
                 if (test_T < 0.0001f) { T[q] = -fabsf(T[q]); continue; }
@@ -234,7 +234,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd_csi(Dims d, RenderCsiArg
             bool c = active && !(power > 0.0f) && !(power < B.z);
             float G = 0.0f, oG = 0.0f, alpha = 0.0f;
             if (c) {
-                G = gs_expf_nonpos(power);
+                G = gs_expf_blend(power);
                 oG = B.y * G;
                 alpha = fminf(0.99f, oG);
                 c = !(alpha < 1.0f / 255.0f);

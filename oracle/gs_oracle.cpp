@@ -1052,6 +1052,7 @@ extern "C" void orc_gs_math_vec(int which, const float* x, const float* y, float
         case 1: out[i] = gs_logf(x[i]); break;
         case 2: out[i] = gs_asinf(x[i]); break;
         case 4: out[i] = gs_expf_nonpos(x[i]); break;
+        case 7: out[i] = gs_expf_blend(x[i]); break;
         case 5: { float c, sn; gs_sincos2pi(x[i], &c, &sn); out[i] = c; break; }
         case 6: { float c, sn; gs_sincos2pi(x[i], &c, &sn); out[i] = sn; break; }
         default: out[i] = gs_atan2f(y[i], x[i]); break;

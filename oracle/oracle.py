@@ -42,7 +42,7 @@ def lib():
 
 def math_vec(which, x, y=None):
     """gs_math.h functions over arrays: which in {"exp", "log", "asin", "atan2"}."""
-    code = {"exp": 0, "log": 1, "asin": 2, "atan2": 3, "exp_nonpos": 4, "cos2pi": 5, "sin2pi": 6}[which]
+    code = {"exp": 0, "log": 1, "asin": 2, "atan2": 3, "exp_nonpos": 4, "cos2pi": 5, "sin2pi": 6, "exp_blend": 7}[which]
     x = np.ascontiguousarray(x, np.float32)
     y = np.ascontiguousarray(x if y is None else y, np.float32)
     out = np.empty_like(x)
