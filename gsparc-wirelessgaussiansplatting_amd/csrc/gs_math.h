@@ -123,13 +123,13 @@ GS_HD float gs_expf_nonpos(float x)
 // and 2^n is built in one step from that sum's low bits (n >= -124, so p * 2^n
 // is a normal float and the two-step scaling of gs_expf gives the same bits).
 // Bit-identical to gs_expf for every x in [-86, 0] (checked by the test
-// suite).  Below -86 it returns gs_expf(-86) ~ 4e-38: the blend only uses
-// exp(power) where power >= thr = -log(255 o) - 1e-3, i.e. >= -86 for any
-// opacity below 1e35.
+// suite); outside that domain the result is unspecified (finite or not, no
+// trap).  The blend kernels use exp(power) only where power >= thr =
+// -log(255 o) - 1e-3, which is >= -86 for every opacity below 1e34; lanes
+// below thr discard the value.
 #define GS_EXP_MAGIC 12582912.0f   // 1.5 * 2^23
 GS_HD float gs_expf_blend(float x)
 {
-    x = x > -86.0f ? x : -86.0f;
     const float t = x * GS_LOG2E + GS_EXP_MAGIC;
     const float n = t - GS_EXP_MAGIC;
     float r = __builtin_fmaf(-n, GS_LN2_HI, x);

@@ -65,7 +65,7 @@ def test_atan2f_accuracy():
 
 def test_expf_blend_is_bit_identical_on_its_domain():
     """gs_expf_blend (the blend kernels' exp) equals gs_expf bit for bit on
-    [-86, 0], including the ties of rint(x log2 e) and the edges."""
+    its domain [-86, 0], including the ties of rint(x log2 e) and the edges."""
     rng = np.random.default_rng(11)
     x = np.concatenate([-rng.uniform(0, 86, 4_000_000), -np.exp(rng.uniform(-30, np.log(86), 1_000_000)),
                         np.array([0.0, -0.0, -86.0, -85.99999, -5.545177, -1e-30])]).astype(np.float32)
@@ -75,8 +75,6 @@ def test_expf_blend_is_bit_identical_on_its_domain():
     x = np.concatenate([x, ties, np.nextafter(ties, 0), np.nextafter(ties, -1)]).astype(np.float32)
     x = x[x >= -86.0]
     np.testing.assert_array_equal(O.math_vec("exp_blend", x).view(np.uint32), O.math_vec("exp", x).view(np.uint32))
-    below = np.array([-86.5, -100.0, -1e6], np.float32)
-    assert (O.math_vec("exp_blend", below) == O.math_vec("exp", np.array([-86.0], np.float32))[0]).all()
 
 
 def test_library_exports_every_declared_symbol():
