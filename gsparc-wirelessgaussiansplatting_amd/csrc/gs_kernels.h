@@ -23,7 +23,7 @@ namespace gs {
 //   receiver r, 4 words    roff, g0, P, scene          at GS_TAB_HDR + 4r
 //   group y, 4 words       r0, nr, scene, part offset  after the receivers
 //   scene s, 8 words       g0, P, r0, R, group0, ngroups, 0, 0  after the groups
-#define GS_RC 8            // receivers per preprocess group
+#define GS_RC 16           // receivers per preprocess group
 #define GS_TAB_HDR 8
 
 struct Dims {

@@ -126,31 +126,32 @@ def test_reference_import_paths():
 
 def test_scene_table_build():
     """Host-side scene table of an uneven batch: per-receiver record offsets,
-    receiver groups of at most 8 and per-scene spans."""
+    receiver groups of at most 16 (the preprocess kernels' group) and
+    per-scene spans."""
     import ctypes as C
     import gsparc_hip as gh
     L = gh.lib()
-    Ps, Rs = [5, 0, 7, 3], [10, 2, 0, 1]
+    Ps, Rs = [5, 0, 7, 3], [20, 2, 0, 1]
     S, R = len(Ps), sum(Rs)
     words = L.gsparc_scene_table_words(S, R)
     tab = (C.c_uint32 * words)()
     b = gh.Batch()
     assert L.gsparc_scene_table_build(S, (C.c_int * S)(*Ps), (C.c_int * S)(*Rs), tab, words, C.byref(b)) == 0
     G = 2 + 1 + 0 + 1
-    assert (b.S, b.R, b.G, b.Pmax, b.Ptot, b.Nrec, b.Npart) == (4, 13, G, 7, 15, 10 * 5 + 1 * 3, 2 * 5 + 3)
+    assert (b.S, b.R, b.G, b.Pmax, b.Ptot, b.Nrec, b.Npart) == (4, 23, G, 7, 15, 20 * 5 + 1 * 3, 2 * 5 + 3)
     t = np.frombuffer(tab, dtype=np.uint32)
-    assert list(t[:7]) == [4, 13, G, 7, 15, 53, 13]
+    assert list(t[:7]) == [4, 23, G, 7, 15, 103, 13]
     rx = t[8:8 + 4 * R].reshape(R, 4)
-    # receivers 0..9 -> scene 0 (5 records each), 10..11 -> scene 1 (no Gaussians), 12 -> scene 3
-    np.testing.assert_array_equal(rx[:, 0], [0, 5, 10, 15, 20, 25, 30, 35, 40, 45, 50, 50, 50])
-    np.testing.assert_array_equal(rx[:, 1], [0] * 10 + [5, 5, 12])
-    np.testing.assert_array_equal(rx[:, 2], [5] * 10 + [0, 0, 3])
-    np.testing.assert_array_equal(rx[:, 3], [0] * 10 + [1, 1, 3])
+    # receivers 0..19 -> scene 0 (5 records each), 20..21 -> scene 1 (no Gaussians), 22 -> scene 3
+    np.testing.assert_array_equal(rx[:, 0], [5 * i for i in range(20)] + [100, 100, 100])
+    np.testing.assert_array_equal(rx[:, 1], [0] * 20 + [5, 5, 12])
+    np.testing.assert_array_equal(rx[:, 2], [5] * 20 + [0, 0, 3])
+    np.testing.assert_array_equal(rx[:, 3], [0] * 20 + [1, 1, 3])
     gr = t[8 + 4 * R:8 + 4 * R + 4 * G].reshape(G, 4)
-    np.testing.assert_array_equal(gr, [[0, 8, 0, 0], [8, 2, 0, 5], [10, 2, 1, 10], [12, 1, 3, 10]])
+    np.testing.assert_array_equal(gr, [[0, 16, 0, 0], [16, 4, 0, 5], [20, 2, 1, 10], [22, 1, 3, 10]])
     sc = t[8 + 4 * R + 4 * G:8 + 4 * R + 4 * G + 8 * S].reshape(S, 8)
-    np.testing.assert_array_equal(sc[:, :6], [[0, 5, 0, 10, 0, 2], [5, 0, 10, 2, 2, 1], [5, 7, 12, 0, 3, 0],
-                                              [12, 3, 12, 1, 3, 1]])
+    np.testing.assert_array_equal(sc[:, :6], [[0, 5, 0, 20, 0, 2], [5, 0, 20, 2, 2, 1], [5, 7, 22, 0, 3, 0],
+                                              [12, 3, 22, 1, 3, 1]])
     sizes = (C.c_size_t * 4)()
     assert L.gsparc_scene_buffer_sizes(C.byref(b), 360, 90, 100, sizes) == 0
     assert all(x > 0 for x in sizes)
