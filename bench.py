@@ -585,6 +585,7 @@ def main():
     with torch.no_grad():
         K = wl.count()
     N = wl.W * wl.H
+    T = ((wl.W + 15) // 16) * ((wl.H + 15) // 16)
     # algorithmic HBM bytes per launch (each datum read or written once)
     work = {
         # 44 B per tile-list instance (index 4, splat record 32, emission 8)
@@ -593,11 +594,17 @@ def main():
         # + 16 B per pixel read (final T, count, dL/dout re/im) + 32 B per
         # (receiver, Gaussian) accumulated gradient record
         "render_bwd": ("hbm", 44.0 * K + 16.0 * N * R + 32.0 * Nrec, "GB/s"),
-        "preprocess_fwd": ("hbm", Ptot * (820 + 24) + 48 * Nrec, "GB/s"),
+        # + 56 B per (Gaussian, receiver): splat record 32, emission 8,
+        # radius 4, depth 4, path length 4, packed tile rect 4
+        "preprocess_fwd": ("hbm", Ptot * (820 + 24) + 56 * Nrec, "GB/s"),
         "preprocess_bwd": ("hbm", Ptot * 1690 + 72 * Nrec, "GB/s"),
-        "bin_depth_sort": ("hbm", 84.0 * Nrec, "GB/s"),
-        "bin_chunk_count": ("hbm", 16.0 * Nrec, "GB/s"),
-        "bin_scatter": ("hbm", 4.0 * K + 20.0 * Nrec, "GB/s"),
+        # 4 passes of (key, index) pairs, plus the packed rects carried into
+        # depth order by the last pass
+        "bin_depth_sort": ("hbm", 92.0 * Nrec, "GB/s"),
+        # depth-ordered rects in, one count per (256-Gaussian chunk, tile) out
+        "bin_chunk_count": ("hbm", 4.0 * Nrec + 4.0 * T * (Nrec / 256.0), "GB/s"),
+        # sorted index and rect per (Gaussian, receiver) in, one index per instance out
+        "bin_scatter": ("hbm", 4.0 * K + 8.0 * Nrec, "GB/s"),
         "ssim_fwd": ("hbm", 8 * N * R + 16 * N * R, "GB/s"),
         "ssim_bwd": ("hbm", 28 * N * R, "GB/s"),
     }
