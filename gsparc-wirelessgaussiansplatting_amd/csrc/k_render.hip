@@ -167,6 +167,11 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
     for (int k = 0; k < PPL; ++k) {
         acc_re[k] = 0.0f; acc_im[k] = 0.0f; last_alpha[k] = 0.0f; last_re[k] = 0.0f; last_im[k] = 0.0f;
     }
+    // lanes 16, 24, 32 sum the conic terms -1/2 D dx^2, -D dx dy, -1/2 D dy^2
+    // (backward.cu:608-610; the off-diagonal's 1/2 is applied by the
+    // preprocess backward)
+    const int vi = lane >> 3;
+    const float post = (vi == 2 || vi == 4) ? -0.5f : (vi == 3 ? -1.0f : 1.0f);
     uint32_t tested = 0;
     for (int top = maxlast - 1; top >= 0; top -= GS_WAVE) {
         // entry j of this batch sits at list position top - j (walked back to front)
@@ -192,9 +197,13 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
             const float4 A = s0[j];
             const float4 B = s1[j];
             const float2 S = ss[j];
-            const float qa = -2.0f * A.z, qb = -A.w, qc = -2.0f * B.x;
-            // -0 is the identity of float addition, so the first += folds away
-            float v[8] = {-0.0f, -0.0f, -0.0f, -0.0f, -0.0f, -0.0f, -0.0f, -0.0f};
+            // Per-lane sums over the lane's pixels, with D = o G dL/dalpha:
+            //   sh = sum D dx, sk = sum D dy, sxx = sum D dx^2, sxy = sum D dx dy,
+            //   syy = sum D dy^2, sg = sum G dL/dalpha, swr / swi = sum w dL/dpix.
+            // The 2D-mean and conic gradients are linear in them; -0 is the
+            // identity of float addition, so the first pixel's sums fold away.
+            float sh = -0.0f, sk = -0.0f, sxx = -0.0f, sxy = -0.0f, syy = -0.0f, sg = -0.0f, swr = -0.0f,
+                  swi = -0.0f;
             bool anyc = false;
 #pragma unroll
             for (int k = 0; k < PPL; ++k) {
@@ -202,45 +211,59 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
                 if (COUNT) cnt += active ? 1u : 0u;
                 const float dx = A.x - pxf[k], dy = A.y - pyf[k];
                 const float power = pair_power(A.z, A.w, B.x, dx, dy);
-                bool c = active && !(power > 0.0f) && !(power < B.z);
-                float G = 0.0f, oG = 0.0f, alpha = 0.0f;
-                if (c) {
-                    G = gs_expf_blend(power);
-                    oG = B.y * G;
-                    alpha = fminf(0.99f, oG);
-                    c = !(alpha < 1.0f / 255.0f);
-                }
-                if (c) {
-                    // T before this Gaussian: T_after / (1 - alpha) (backward.cu:565)
-                    T[k] = T[k] * __builtin_amdgcn_rcpf(1.0f - alpha);
-                    const float w = alpha * T[k];
-                    const float sr = S.x;
-                    const float si = (MODE == GRAD_REFERENCE) ? S.x : S.y;
-                    acc_re[k] = last_alpha[k] * last_re[k] + (1.0f - last_alpha[k]) * acc_re[k];
-                    acc_im[k] = last_alpha[k] * last_im[k] + (1.0f - last_alpha[k]) * acc_im[k];
-                    last_re[k] = sr;
-                    last_im[k] = si;
-                    float dalpha = (sr - acc_re[k]) * gre[k];
-                    if (MODE == GRAD_EXACT) dalpha += (si - acc_im[k]) * gim[k];
-                    dalpha *= T[k];
-                    last_alpha[k] = alpha;
-                    v[6] += w * gre[k];
-                    v[7] += w * gim[k];
-                    if (!(MODE == GRAD_EXACT && oG > 0.99f)) {
-                        const float dG = B.y * dalpha;
-                        const float gdx = G * dx, gdy = G * dy;
-                        v[0] += dG * (-gdx * qa - gdy * qb);
-                        v[1] += dG * (-gdy * qc - gdx * qb);
-                        v[2] += -0.5f * gdx * dx * dG;
-                        v[3] += -gdx * dy * dG;
-                        v[4] += -0.5f * gdy * dy * dG;
-                        v[5] += G * dalpha;
-                    }
-                    anyc = true;
-                }
+                const bool in = active && !(power > 0.0f) && !(power < B.z);
+                // a uniform branch: the body runs for the whole wave, selects
+                // keep the state of the lanes whose pair does not contribute
+                if (ballot(in) == 0) continue;
+                // the exponential is only valid on [thr, 0]: elsewhere G = 0,
+                // so no out-of-domain value reaches the sums
+                const float G = in ? gs_expf_blend(power) : 0.0f;
+                const float oG = B.y * G;
+                const float alpha = fminf(0.99f, oG);
+                const bool c = in && !(alpha < 1.0f / 255.0f);
+                // T before this Gaussian: T_after / (1 - alpha) (backward.cu:565)
+                T[k] = c ? T[k] * __builtin_amdgcn_rcpf(1.0f - alpha) : T[k];
+                const float w = (c ? alpha : 0.0f) * T[k];
+                const float sr = S.x;
+                const float si = (MODE == GRAD_REFERENCE) ? S.x : S.y;
+                // accum = last_alpha last_c + (1 - last_alpha) accum (backward.cu:579)
+                acc_re[k] = c ? fmaf(last_alpha[k], last_re[k] - acc_re[k], acc_re[k]) : acc_re[k];
+                if (MODE == GRAD_EXACT)
+                    acc_im[k] = c ? fmaf(last_alpha[k], last_im[k] - acc_im[k], acc_im[k]) : acc_im[k];
+                last_re[k] = c ? sr : last_re[k];
+                if (MODE == GRAD_EXACT) last_im[k] = c ? si : last_im[k];
+                last_alpha[k] = c ? alpha : last_alpha[k];
+                float dalpha = (sr - acc_re[k]) * gre[k];
+                if (MODE == GRAD_EXACT) dalpha = fmaf(si - acc_im[k], gim[k], dalpha);
+                dalpha *= T[k];
+                swr = fmaf(w, gre[k], swr);
+                swi = fmaf(w, gim[k], swi);
+                // the exact adjoint has no gradient through the clamped alpha
+                const bool geo = (MODE == GRAD_EXACT) ? (c && !(oG > 0.99f)) : c;
+                const float da = geo ? dalpha : 0.0f;
+                const float D = oG * da;
+                const float h = D * dx, q = D * dy;
+                sh += h;
+                sk += q;
+                sxx = fmaf(h, dx, sxx);
+                sxy = fmaf(h, dy, sxy);
+                syy = fmaf(q, dy, syy);
+                sg = fmaf(G, da, sg);
+                anyc |= c;
             }
             if (ballot(anyc) == 0) continue;
-            const float tot = wave_sum8(v, lane);
+            // dL/dmean2D = D (conic . d) with the conic (qa, qb; qb, qc) and
+            // qa = -2 A.z, qb = -A.w, qc = -2 B.x (backward.cu:600-605)
+            float v[8];
+            v[0] = fmaf(A.w, sk, (2.0f * A.z) * sh);
+            v[1] = fmaf(A.w, sh, (2.0f * B.x) * sk);
+            v[2] = sxx;
+            v[3] = sxy;
+            v[4] = syy;
+            v[5] = sg;
+            v[6] = swr;
+            v[7] = swi;
+            const float tot = wave_sum8(v, lane) * post;
             if ((lane & 7) == 0) atomicAdd(a.g8 + (rP + sid[j]) * 8 + (lane >> 3), tot);
         }
         __syncthreads();
