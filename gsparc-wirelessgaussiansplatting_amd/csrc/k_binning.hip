@@ -166,25 +166,90 @@ __global__ void __launch_bounds__(1024) k_rs_scan(Dims d, BinArgs a, int nblk, i
     if (first && threadIdx.x == 0) a.nvis[r] = h[(size_t)256 * nblk];
 }
 
-// Stable scatter of one pass: element order inside the block is
-// (round, thread), the order of the input positions.
+// Stable scatter of one pass.  Wave w of the block owns the block's
+// elements w*WT .. w*WT+WT-1 (WT = GS_RS_TILE / waves), in GS_RS_ITEMS rounds
+// of 64: round ranks come from the digit-match masks, a wave-private running
+// count per digit carries them across rounds, and the per-digit prefix over
+// the waves joins the waves in input order.  The block then sorts its
+// elements into LDS by (digit, input position) and writes each digit's run to
+// consecutive global positions, so stores are contiguous per digit rather than
+// one element per bucket.  The first pass drops culled Gaussians (digit 256):
+// every later pass and the chunk kernels read only the nvis visible ones.
 template <bool FIRST>
 __global__ void __launch_bounds__(GS_RS_THREADS) k_rs_scatter(Dims d, BinArgs a, int pass)
 {
     constexpr int NB = FIRST ? 257 : 256;
     constexpr int NW = GS_RS_THREADS / GS_WAVE;
-    __shared__ uint32_t run[NB];
-    __shared__ uint32_t wcnt[NW][NB];
+    constexpr int WT = GS_RS_TILE / NW;
+    static_assert(WT == GS_RS_ITEMS * GS_WAVE, "a wave owns GS_RS_ITEMS rounds of 64");
+    __shared__ uint32_t s_key[GS_RS_TILE];
+    __shared__ uint32_t s_val[GS_RS_TILE];
+    __shared__ uint16_t s_dg[GS_RS_TILE];
+    __shared__ uint32_t whist[NW][NB];
+    __shared__ uint32_t lbase[NB];
+    __shared__ uint32_t gbase[NB];
+    __shared__ uint32_t s_wave[GS_RS_THREADS / GS_WAVE];
     const int r = blockIdx.y, blk = blockIdx.x, nblk = gridDim.x;
     const RxMap rm = rx_map(d, r);
     const int n = FIRST ? (int)rm.P : (int)a.nvis[r];
     if (blk * GS_RS_TILE >= n) return;
     const size_t rP = rm.roff;
-    const int wave = threadIdx.x / GS_WAVE;
-    for (int i = threadIdx.x; i < NB; i += GS_RS_THREADS) {
-        run[i] = a.shist[((size_t)r * 257 + i) * nblk + blk];
+    const int tid = threadIdx.x, wave = tid / GS_WAVE, lane = tid % GS_WAVE;
+    for (int i = tid; i < NB; i += GS_RS_THREADS) {
+        gbase[i] = a.shist[((size_t)r * 257 + i) * nblk + blk];
 #pragma unroll
-        for (int w = 0; w < NW; ++w) wcnt[w][i] = 0;
+        for (int w = 0; w < NW; ++w) whist[w][i] = 0;
+    }
+    __syncthreads();
+    uint32_t key[GS_RS_ITEMS], val[GS_RS_ITEMS], dg[GS_RS_ITEMS], rnk[GS_RS_ITEMS];
+    const int e0 = wave * WT;
+#pragma unroll
+    for (int k = 0; k < GS_RS_ITEMS; ++k) {
+        const int i = blk * GS_RS_TILE + e0 + k * GS_WAVE + lane;
+        const bool valid = i < n;
+        key[k] = 0; val[k] = 0; dg[k] = 0;
+        if (valid) rs_load<FIRST>(d, a, rP, i, pass, key[k], val[k], dg[k]);
+        const uint64_t m = match_digit<FIRST ? 9 : 8>(dg[k], valid);
+        const uint32_t prev = valid ? whist[wave][dg[k]] : 0u;
+        rnk[k] = prev + (uint32_t)__popcll(m & lanemask_lt());
+        if (valid && (m & lanemask_lt()) == 0) whist[wave][dg[k]] = prev + (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    // per digit: exclusive prefix over the waves, then over the digits
+    uint32_t tot = 0;
+    {
+        uint32_t run = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            const uint32_t c = whist[w][tid];
+            whist[w][tid] = run;
+            run += c;
+        }
+        tot = run;
+    }
+    uint32_t all;
+    const uint32_t lb = block_exclusive_scan<GS_RS_THREADS>(tot, s_wave, all);
+    lbase[tid] = lb;
+    if (FIRST && tid == 0) {
+        uint32_t run = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            const uint32_t c = whist[w][256];
+            whist[w][256] = run;
+            run += c;
+        }
+        lbase[256] = all;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < GS_RS_ITEMS; ++k) {
+        const int i = blk * GS_RS_TILE + e0 + k * GS_WAVE + lane;
+        if (i < n) {
+            const uint32_t lp = lbase[dg[k]] + whist[wave][dg[k]] + rnk[k];
+            s_key[lp] = key[k];
+            s_val[lp] = val[k];
+            s_dg[lp] = (uint16_t)dg[k];
+        }
     }
     __syncthreads();
     uint32_t* kout = a.skey[pass & 1] + rP;
@@ -192,33 +257,15 @@ __global__ void __launch_bounds__(GS_RS_THREADS) k_rs_scatter(Dims d, BinArgs a,
     // the last pass also puts the packed tile rects in depth order, so that
     // the chunk kernels read them in sequence instead of gathering
     const bool rects = !FIRST && pass == 3 && a.rect;
-    for (int k = 0; k < GS_RS_ITEMS; ++k) {
-        const int i = blk * GS_RS_TILE + k * GS_RS_THREADS + threadIdx.x;
-        const bool valid = i < n;
-        uint32_t key = 0, val = 0, dg = 0;
-        if (valid) rs_load<FIRST>(d, a, rP, i, pass, key, val, dg);
-        const uint64_t m = match_digit<FIRST ? 9 : 8>(dg, valid);
-        const uint32_t rank = (uint32_t)__popcll(m & lanemask_lt());
-        if (valid && rank == 0) wcnt[wave][dg] = (uint32_t)__popcll(m);
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = run[dg] + rank;
-            for (int w = 0; w < wave; ++w) pos += wcnt[w][dg];
-            kout[pos] = key;
-            vout[pos] = val;
-            if (rects) a.srect[rP + pos] = a.rect[rP + val];
-        }
-        __syncthreads();
-        for (int j = threadIdx.x; j < NB; j += GS_RS_THREADS) {
-            uint32_t s = 0;
-#pragma unroll
-            for (int w = 0; w < NW; ++w) {
-                s += wcnt[w][j];
-                wcnt[w][j] = 0;
-            }
-            run[j] += s;
-        }
-        __syncthreads();
+    const int nb = min(GS_RS_TILE, n - blk * GS_RS_TILE);
+    for (int e = tid; e < nb; e += GS_RS_THREADS) {
+        const uint32_t g = s_dg[e];
+        if (FIRST && g == 256) continue;     // culled: never read again
+        const uint32_t pos = gbase[g] + (uint32_t)e - lbase[g];
+        const uint32_t v = s_val[e];
+        kout[pos] = s_key[e];
+        vout[pos] = v;
+        if (rects) a.srect[rP + pos] = a.rect[rP + v];
     }
 }
 
