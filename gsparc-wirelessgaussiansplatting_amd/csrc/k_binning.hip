@@ -428,34 +428,39 @@ __global__ void __launch_bounds__(GS_WAVE) k_chunk_scatter(Dims d, BinArgs a)
         s_own[lane] = make_int4(x0, y0, x1 - x0, (int)g);
         s_key[lane] = dbits;
         __syncthreads();
-        // pass 0 places the instances (reading the bases), pass 1 advances
-        // the bases (the highest lane covering a tile does it)
-        for (int pass = 0; pass < 2; ++pass) {
-            for (uint32_t p = lane; p < npairs; p += GS_WAVE) {
-                // owner: the first lane whose inclusive prefix exceeds p
-                int lo = 0;
+        // place the instances: pair p of the batch belongs to the first lane
+        // whose inclusive prefix exceeds p; its rank inside the tile is the
+        // number of lower lanes of the batch covering the same tile
+        for (uint32_t p = lane; p < npairs; p += GS_WAVE) {
+            int lo = 0;
 #pragma unroll
-                for (int step = GS_WAVE / 2; step >= 1; step >>= 1)
-                    if (s_end[lo + step - 1] <= p) lo += step;
-                const int4 ow = s_own[lo];
-                const uint32_t q = p - (lo ? s_end[lo - 1] : 0u);   // pair index within the owner's rect
-                const int ty = ow.y + (int)(q / (uint32_t)ow.z);
-                const int tx = ow.x + (int)(q % (uint32_t)ow.z);
-                const int t = ty * d.gx + tx;
-                const uint64_t m = s_mx[tx] & s_my[ty];
-                if (pass == 0) {
-                    const uint32_t b = lds ? s_base[t] : slot[(size_t)t * a.nchunks];
-                    const uint32_t pos = b + (uint32_t)__popcll(m & ((1ull << lo) - 1ull));
-                    a.plist[pos] = (uint32_t)ow.w;
-                    if (a.keys_sorted) a.keys_sorted[pos] = s_key[lo];
-                } else if ((m >> lo) == 1ull) {
-                    const uint32_t cnt = (uint32_t)__popcll(m);
-                    if (lds) s_base[t] += cnt;
-                    else slot[(size_t)t * a.nchunks] += cnt;
-                }
-            }
-            __syncthreads();
+            for (int step = GS_WAVE / 2; step >= 1; step >>= 1)
+                if (s_end[lo + step - 1] <= p) lo += step;
+            const int4 ow = s_own[lo];
+            const uint32_t q = p - (lo ? s_end[lo - 1] : 0u);   // pair index within the owner's rect
+            const int ty = ow.y + (int)(q / (uint32_t)ow.z);
+            const int tx = ow.x + (int)(q % (uint32_t)ow.z);
+            const int t = ty * d.gx + tx;
+            const uint64_t m = s_mx[tx] & s_my[ty];
+            const uint32_t b = lds ? s_base[t] : slot[(size_t)t * a.nchunks];
+            const uint32_t pos = b + (uint32_t)__popcll(m & ((1ull << lo) - 1ull));
+            a.plist[pos] = (uint32_t)ow.w;
+            if (a.keys_sorted) a.keys_sorted[pos] = s_key[lo];
         }
+        __syncthreads();
+        // advance the base of every tile of the batch's union rect by the
+        // number of the batch's Gaussians covering it (one lane per tile)
+        // (an empty batch has ux0 = uy0 = INT_MAX: no tiles)
+        const int uw = npairs ? ux1 - ux0 : 0, uh = npairs ? uy1 - uy0 : 0;
+        for (int q = lane; q < uw * uh; q += GS_WAVE) {
+            const int ty = uy0 + q / uw, tx = ux0 + q % uw;
+            const uint32_t cnt = (uint32_t)__popcll(s_mx[tx] & s_my[ty]);
+            if (cnt == 0) continue;
+            const int t = ty * d.gx + tx;
+            if (lds) s_base[t] += cnt;
+            else slot[(size_t)t * a.nchunks] += cnt;
+        }
+        __syncthreads();
     }
 }
 
