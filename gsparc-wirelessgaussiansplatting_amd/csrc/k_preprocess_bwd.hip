@@ -268,9 +268,9 @@ __global__ void __launch_bounds__(256) k_pre_bwd_param(Dims d, PreBwdArgs a)
             const float2 dz = a.dz[(size_t)rx_map(d, r).roff + gl];
             const float* rx = a.rx + 3 * r;
             float hs = bb;
-            hs = hs + w0 * rx[0];
-            hs = hs + w1 * rx[1];
-            hs = hs + w2 * rx[2];
+            hs = fmaf(w0, rx[0], hs);
+            hs = fmaf(w1, rx[1], hs);
+            hs = fmaf(w2, rx[2], hs);
             const float h = (hs > 0.0f) ? hs : 0.01f * hs;
             gv0 += dz.x * h;
             gv1 += dz.y * h;

@@ -173,6 +173,8 @@ static void get_rect(R px, R py, int radius, int gx, int gy, int* rect)
 
 // Per-Gaussian emission MLP 3->32->2 (forward.cu:17-75, packing
 // gaussian.py:554-572): params = [W1 P*32*3 | b1 P*32 | W2 P*2*32 | b2 P*2].
+// The dot products accumulate with fused multiply-adds in index order, the
+// order the kernel uses.
 template <typename R>
 static void mlp_forward(int idx, int P, const R* mlp, const R* mean, const R* rx, const R* tx,
                         R* hidden_pre, R* out_pre, R* sig, R* logd, R* dist)
@@ -187,13 +189,13 @@ static void mlp_forward(int idx, int P, const R* mlp, const R* mean, const R* rx
     R h[32];
     for (int i = 0; i < 32; ++i) {
         R s = b1[i];
-        for (int j = 0; j < 3; ++j) s += W1[i * 3 + j] * rx[j];
+        for (int j = 0; j < 3; ++j) s = Fn<R>::fma(W1[i * 3 + j], rx[j], s);
         hidden_pre[i] = s;
         h[i] = (s > R(0)) ? s : K<R>(0.01f) * s;
     }
     for (int k = 0; k < 2; ++k) {
         R s = b2[k];
-        for (int j = 0; j < 32; ++j) s += W2[k * 32 + j] * h[j];
+        for (int j = 0; j < 32; ++j) s = Fn<R>::fma(W2[k * 32 + j], h[j], s);
         out_pre[k] = s;
         sig[k] = R(1) / (R(1) + Fn<R>::exp(-(s - ld)));
     }
