@@ -432,6 +432,8 @@ int backward_impl(int grad_mode, const gs::Dims& d, const FwdInputs& in, const f
         !in.rx || !radii || !dout_real || (!csi && !dout_imag) || !out.dmeans3D || !out.dopacity ||
         !out.dscales || !out.drotations)
         return fail(GSPARC_ERR_ARG, "null pointer");
+    if (((uintptr_t)in.mlp & 15) != 0 || ((uintptr_t)out.dmlp & 15) != 0)
+        return fail(GSPARC_ERR_ARG, "emission_mlps and its gradient must be 16-byte aligned");
     gsparc_layout L;
     make_layout(d, K_capacity, L);
     char* gb = base_of(geom);

@@ -244,51 +244,12 @@ __global__ void __launch_bounds__(GS_BWD_THREADS) k_pre_bwd_geo(Dims d, PreBwdAr
     pt[9] = dop; pt[10] = db2_0; pt[11] = db2_1;
 }
 
-// Kernel 2: 32 threads per (packed) Gaussian, one per hidden unit j of its
-// MLP: the six weight gradients of unit j summed over the receivers of its
-// scene (backward.cu:95-115).  Thread j == 0 also sums the group partials and
-// runs the cov3D adjoint (backward.cu:308-371), which is linear in dL/dcov3D
-// and so done once.
-__global__ void __launch_bounds__(256) k_pre_bwd_param(Dims d, PreBwdArgs a)
+// Receiver-summed partials of Gaussian gl of scene sc (packed index gg):
+// dmean, dopacity and db2 go out as they are; the cov3D adjoint
+// (backward.cu:308-371) is linear in dL/dcov3D and so runs once, on the sum.
+__device__ __forceinline__ void pre_bwd_finish(const Dims& d, const PreBwdArgs& a, const SceneMap& sc, int gl, int gg)
 {
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    const int g = t >> 5, j = t & 31;   // g: packed index
-    if (g >= d.Ptot) return;
-    const int P = d.Ptot;               // MLP section stride
-    const SceneMap sc = scene_map(d, scene_of(d, g));
-    const int gl = g - (int)sc.g0;      // index within the scene
-    if (a.mlp && a.dmlp) {
-        const float* W1 = a.mlp + (size_t)g * 96;
-        const float* b1 = a.mlp + (size_t)P * 96 + (size_t)g * 32;
-        const float* W2 = a.mlp + (size_t)P * 128 + (size_t)g * 64;
-        const float w0 = W1[j * 3], w1 = W1[j * 3 + 1], w2 = W1[j * 3 + 2], bb = b1[j];
-        const float v0 = W2[j], v1 = W2[32 + j];
-        float gw0 = 0.0f, gw1 = 0.0f, gw2 = 0.0f, gb = 0.0f, gv0 = 0.0f, gv1 = 0.0f;
-        for (int r = (int)sc.r0; r < (int)(sc.r0 + sc.R); ++r) {
-            const float2 dz = a.dz[(size_t)rx_map(d, r).roff + gl];
-            const float* rx = a.rx + 3 * r;
-            float hs = bb;
-            hs = fmaf(w0, rx[0], hs);
-            hs = fmaf(w1, rx[1], hs);
-            hs = fmaf(w2, rx[2], hs);
-            const float h = (hs > 0.0f) ? hs : 0.01f * hs;
-            gv0 += dz.x * h;
-            gv1 += dz.y * h;
-            const float dh = dz.x * v0 + dz.y * v1;
-            const float dp = (hs > 0.0f) ? dh : 0.01f * dh;
-            gw0 += dp * rx[0];
-            gw1 += dp * rx[1];
-            gw2 += dp * rx[2];
-            gb += dp;
-        }
-        a.dmlp[(size_t)g * 96 + j * 3 + 0] = gw0;
-        a.dmlp[(size_t)g * 96 + j * 3 + 1] = gw1;
-        a.dmlp[(size_t)g * 96 + j * 3 + 2] = gw2;
-        a.dmlp[(size_t)P * 96 + (size_t)g * 32 + j] = gb;
-        a.dmlp[(size_t)P * 128 + (size_t)g * 64 + j] = gv0;
-        a.dmlp[(size_t)P * 128 + (size_t)g * 64 + 32 + j] = gv1;
-    }
-    if (j != 0) return;
+    const int P = d.Ptot;   // MLP section stride
     float acc[GS_PART];
 #pragma unroll
     for (int k = 0; k < GS_PART; ++k) acc[k] = 0.0f;
@@ -298,19 +259,19 @@ __global__ void __launch_bounds__(256) k_pre_bwd_param(Dims d, PreBwdArgs a)
         for (int k = 0; k < GS_PART; ++k) acc[k] += pt[k];
     }
     if (a.mlp && a.dmlp) {
-        a.dmlp[(size_t)P * 192 + (size_t)g * 2] = acc[10];
-        a.dmlp[(size_t)P * 192 + (size_t)g * 2 + 1] = acc[11];
+        a.dmlp[(size_t)P * 192 + (size_t)gg * 2] = acc[10];
+        a.dmlp[(size_t)P * 192 + (size_t)gg * 2 + 1] = acc[11];
     }
     const float* dc = acc + 3;
-    const float s[3] = {a.scales[3 * g], a.scales[3 * g + 1], a.scales[3 * g + 2]};
-    const float q[4] = {a.rots[4 * g], a.rots[4 * g + 1], a.rots[4 * g + 2], a.rots[4 * g + 3]};
+    const float sv[3] = {a.scales[3 * gg], a.scales[3 * gg + 1], a.scales[3 * gg + 2]};
+    const float q[4] = {a.rots[4 * gg], a.rots[4 * gg + 1], a.rots[4 * gg + 2], a.rots[4 * gg + 3]};
     {
         const float r = q[0], x = q[1], y = q[2], z = q[3];
         const float Rq[3][3] = {
             {1.0f - 2.0f * (y * y + z * z), 2.0f * (x * y - r * z), 2.0f * (x * z + r * y)},
             {2.0f * (x * y + r * z), 1.0f - 2.0f * (x * x + z * z), 2.0f * (y * z - r * x)},
             {2.0f * (x * z - r * y), 2.0f * (y * z + r * x), 1.0f - 2.0f * (x * x + y * y)}};
-        const float se[3] = {a.mod * s[0], a.mod * s[1], a.mod * s[2]};
+        const float se[3] = {a.mod * sv[0], a.mod * sv[1], a.mod * sv[2]};
         const float Dg[3] = {se[0] * se[0], se[1] * se[1], se[2] * se[2]};
         const float G[3][3] = {{dc[0], 0.5f * dc[1], 0.5f * dc[2]},
                                {0.5f * dc[1], dc[3], 0.5f * dc[4]},
@@ -330,7 +291,7 @@ __global__ void __launch_bounds__(256) k_pre_bwd_param(Dims d, PreBwdArgs a)
                 for (int jj = 0; jj < 3; ++jj) dD += Rq[i][k] * G[i][jj] * Rq[jj][k];
             // dD/ds = 2 mod^2 s; the reference drops one factor mod (backward.cu:353-355)
             const float f = (a.mode == GRAD_REFERENCE) ? 2.0f * se[k] : 2.0f * se[k] * a.mod;
-            a.dscales[3 * g + k] = dD * f;
+            a.dscales[3 * gg + k] = dD * f;
         }
         const float dRr[3][3] = {{0, -2 * z, 2 * y}, {2 * z, 0, -2 * x}, {-2 * y, 2 * x, 0}};
         const float dRx[3][3] = {{0, 2 * y, 2 * z}, {2 * y, -4 * x, -2 * r}, {2 * z, 2 * r, -4 * x}};
@@ -347,14 +308,109 @@ __global__ void __launch_bounds__(256) k_pre_bwd_param(Dims d, PreBwdArgs a)
                 dq[3] += GR[i][jj] * dRz[i][jj];
             }
 #pragma unroll
-        for (int k = 0; k < 4; ++k) a.drots[4 * g + k] = dq[k];
+        for (int k = 0; k < 4; ++k) a.drots[4 * gg + k] = dq[k];
     }
 #pragma unroll
-    for (int k = 0; k < 3; ++k) a.dmeans[3 * g + k] = acc[k];
-    a.dopac[g] = acc[9];
+    for (int k = 0; k < 3; ++k) a.dmeans[3 * gg + k] = acc[k];
+    a.dopac[gg] = acc[9];
     if (a.dcov3)
 #pragma unroll
-        for (int k = 0; k < 6; ++k) a.dcov3[6 * g + k] = dc[k];
+        for (int k = 0; k < 6; ++k) a.dcov3[6 * gg + k] = dc[k];
+}
+
+// Kernel 2: the 194 MLP weight gradients, summed over the receivers of the
+// scene (backward.cu:95-115).  A workgroup owns GS_PB_G consecutive Gaussians
+// of one scene (blockIdx.y), one per lane; wave w takes hidden units
+// 4w .. 4w+3, so the per-receiver dL/d(MLP output) loads are coalesced over
+// the lanes and shared by the waves.  Receiver positions and record offsets
+// are staged in LDS 64 receivers at a time.  Wave 0 then sums the group
+// partials of its lane's Gaussian and runs the cov3D adjoint.
+#define GS_PB_G 64
+#define GS_PB_THREADS 512    // 8 waves x 4 hidden units
+__global__ void __launch_bounds__(GS_PB_THREADS) k_pre_bwd_param(Dims d, PreBwdArgs a)
+{
+    const SceneMap sc = scene_map(d, blockIdx.y);
+    const int gb = blockIdx.x * GS_PB_G;
+    if (gb >= (int)sc.P) return;
+    const int tid = threadIdx.x, wave = tid / GS_WAVE, lane = tid % GS_WAVE;
+    const int gl = gb + lane;                 // within the scene
+    const bool valid = gl < (int)sc.P;
+    const int gg = (int)sc.g0 + gl;           // packed
+    const int P = d.Ptot;                     // MLP section stride
+    if (a.mlp && a.dmlp) {
+        __shared__ float s_rx[GS_WAVE * 3];
+        __shared__ uint32_t s_roff[GS_WAVE];
+        // units j = 4 wave + u: W1 rows, b1, W2 of both outputs as 16-byte loads
+        float w[4][3], bb[4], v0[4], v1[4];
+        if (valid) {
+            const float4* W1v = reinterpret_cast<const float4*>(a.mlp + (size_t)gg * 96 + 12 * wave);
+            const float4 wa = W1v[0], wb = W1v[1], wc = W1v[2];
+            const float wl[12] = {wa.x, wa.y, wa.z, wa.w, wb.x, wb.y, wb.z, wb.w, wc.x, wc.y, wc.z, wc.w};
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) w[u][i] = wl[3 * u + i];
+            const float4 bq = *reinterpret_cast<const float4*>(a.mlp + (size_t)P * 96 + (size_t)gg * 32 + 4 * wave);
+            const float4 va = *reinterpret_cast<const float4*>(a.mlp + (size_t)P * 128 + (size_t)gg * 64 + 4 * wave);
+            const float4 vb =
+                *reinterpret_cast<const float4*>(a.mlp + (size_t)P * 128 + (size_t)gg * 64 + 32 + 4 * wave);
+            bb[0] = bq.x; bb[1] = bq.y; bb[2] = bq.z; bb[3] = bq.w;
+            v0[0] = va.x; v0[1] = va.y; v0[2] = va.z; v0[3] = va.w;
+            v1[0] = vb.x; v1[1] = vb.y; v1[2] = vb.z; v1[3] = vb.w;
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) { w[u][0] = w[u][1] = w[u][2] = bb[u] = v0[u] = v1[u] = 0.0f; }
+        }
+        float gw[4][3], gbias[4], gv0[4], gv1[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) { gw[u][0] = gw[u][1] = gw[u][2] = gbias[u] = gv0[u] = gv1[u] = 0.0f; }
+        for (int rb = 0; rb < (int)sc.R; rb += GS_WAVE) {
+            const int nrb = min(GS_WAVE, (int)sc.R - rb);
+            __syncthreads();   // the previous chunk's receivers are consumed
+            if (tid < nrb) {
+                const int r = (int)sc.r0 + rb + tid;
+                s_roff[tid] = rx_map(d, r).roff;
+                s_rx[3 * tid] = a.rx[3 * r];
+                s_rx[3 * tid + 1] = a.rx[3 * r + 1];
+                s_rx[3 * tid + 2] = a.rx[3 * r + 2];
+            }
+            __syncthreads();
+            for (int k = 0; k < nrb; ++k) {
+                const float2 dz = valid ? a.dz[(size_t)s_roff[k] + gl] : make_float2(0.0f, 0.0f);
+                const float rx0 = s_rx[3 * k], rx1 = s_rx[3 * k + 1], rx2 = s_rx[3 * k + 2];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    float hs = bb[u];
+                    hs = fmaf(w[u][0], rx0, hs);
+                    hs = fmaf(w[u][1], rx1, hs);
+                    hs = fmaf(w[u][2], rx2, hs);
+                    const bool pos = hs > 0.0f;
+                    const float h = pos ? hs : 0.01f * hs;
+                    gv0[u] = fmaf(dz.x, h, gv0[u]);
+                    gv1[u] = fmaf(dz.y, h, gv1[u]);
+                    const float dh = fmaf(dz.y, v1[u], dz.x * v0[u]);
+                    const float dp = pos ? dh : 0.01f * dh;
+                    gw[u][0] = fmaf(dp, rx0, gw[u][0]);
+                    gw[u][1] = fmaf(dp, rx1, gw[u][1]);
+                    gw[u][2] = fmaf(dp, rx2, gw[u][2]);
+                    gbias[u] += dp;
+                }
+            }
+        }
+        if (valid) {
+            float4* gW1 = reinterpret_cast<float4*>(a.dmlp + (size_t)gg * 96 + 12 * wave);
+            gW1[0] = make_float4(gw[0][0], gw[0][1], gw[0][2], gw[1][0]);
+            gW1[1] = make_float4(gw[1][1], gw[1][2], gw[2][0], gw[2][1]);
+            gW1[2] = make_float4(gw[2][2], gw[3][0], gw[3][1], gw[3][2]);
+            *reinterpret_cast<float4*>(a.dmlp + (size_t)P * 96 + (size_t)gg * 32 + 4 * wave) =
+                make_float4(gbias[0], gbias[1], gbias[2], gbias[3]);
+            *reinterpret_cast<float4*>(a.dmlp + (size_t)P * 128 + (size_t)gg * 64 + 4 * wave) =
+                make_float4(gv0[0], gv0[1], gv0[2], gv0[3]);
+            *reinterpret_cast<float4*>(a.dmlp + (size_t)P * 128 + (size_t)gg * 64 + 32 + 4 * wave) =
+                make_float4(gv1[0], gv1[1], gv1[2], gv1[3]);
+        }
+    }
+    if (wave == 0 && valid) pre_bwd_finish(d, a, sc, gl, gg);
 }
 
 hipError_t launch_preprocess_bwd(const Dims& d, const PreBwdArgs& a, hipStream_t s)
@@ -367,8 +423,7 @@ hipError_t launch_preprocess_bwd(const Dims& d, const PreBwdArgs& a, hipStream_t
         hipLaunchKernelGGL(k_pre_bwd_geo<GRAD_EXACT>, g1, dim3(GS_BWD_THREADS), 0, s, d, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const long long threads = (long long)d.Ptot * 32;
-    hipLaunchKernelGGL(k_pre_bwd_param, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, d, a);
+    hipLaunchKernelGGL(k_pre_bwd_param, dim3((d.P + GS_PB_G - 1) / GS_PB_G, d.S), dim3(GS_PB_THREADS), 0, s, d, a);
     return hipGetLastError();
 }
 

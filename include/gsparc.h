@@ -151,7 +151,7 @@ int gsparc_instance_count(int P, int R, int W, int H, const void* image, uint64_
  * K_capacity is the instance capacity `bin` was laid out with.
  * Outputs (all written, none accumulated):
  *   dmeans2D [R,P,3] (screen-space, reference layout; may be NULL),
- *   dmeans3D [P,3], demission_mlps [P*194] (NULL if no MLP),
+ *   dmeans3D [P,3], demission_mlps [P*194] (NULL if no MLP; 16-byte aligned),
  *   dopacity [P], dcov3D [P,6] (may be NULL), dscales [P,3], drotations [P,4],
  *   demission [R,P,2] (emission supplied to the forward; may be NULL).
  * Gradients are summed over the R receivers.
