@@ -137,4 +137,38 @@ __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
 
+// Inclusive prefix over the wave's 64 lanes by DPP, no LDS round trip:
+// row_shr 1, 2, 4, 8 scan each row of 16 lanes (lanes without a source keep
+// the identity), then row_bcast 15 carries rows 0 and 2 into rows 1 and 3 and
+// row_bcast 31 carries lane 31 into rows 2 and 3.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_or_identity(int identity, int v)
+{
+    return __builtin_amdgcn_update_dpp(identity, v, CTRL, ROWS, 0xf, false);
+}
+
+__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t x)
+{
+    int v = (int)x;
+    v += dpp_or_identity<0x111, 0xf>(0, v);
+    v += dpp_or_identity<0x112, 0xf>(0, v);
+    v += dpp_or_identity<0x114, 0xf>(0, v);
+    v += dpp_or_identity<0x118, 0xf>(0, v);
+    v += dpp_or_identity<0x142, 0xa>(0, v);
+    v += dpp_or_identity<0x143, 0xc>(0, v);
+    return (uint32_t)v;
+}
+
+// Inclusive prefix maximum of values >= -1 (identity -1).
+__device__ __forceinline__ int wave_incl_max(int v)
+{
+    v = max(v, dpp_or_identity<0x111, 0xf>(-1, v));
+    v = max(v, dpp_or_identity<0x112, 0xf>(-1, v));
+    v = max(v, dpp_or_identity<0x114, 0xf>(-1, v));
+    v = max(v, dpp_or_identity<0x118, 0xf>(-1, v));
+    v = max(v, dpp_or_identity<0x142, 0xa>(-1, v));
+    v = max(v, dpp_or_identity<0x143, 0xc>(-1, v));
+    return v;
+}
+
 }  // namespace gs

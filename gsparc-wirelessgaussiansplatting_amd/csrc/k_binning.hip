@@ -365,6 +365,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_chunk_scatter(Dims d, BinArgs a)
     __shared__ uint32_t s_end[GS_WAVE];    // inclusive prefix of the pair counts
     __shared__ int4 s_own[GS_WAVE];        // x0, y0, width, Gaussian index
     __shared__ uint32_t s_key[GS_WAVE];
+    __shared__ int s_mark[GS_WAVE];        // lane whose pairs start at this window position
     uint64_t* s_mx = s_mask;
     uint64_t* s_my = s_mask + d.gx;
     uint32_t* s_base = (uint32_t*)(s_mask + d.gx + d.gy);
@@ -395,12 +396,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_chunk_scatter(Dims d, BinArgs a)
         }
         const bool ne = x0 < x1 && y0 < y1;
         const uint32_t area = ne ? (uint32_t)((x1 - x0) * (y1 - y0)) : 0u;
-        uint32_t incl = area;
-#pragma unroll
-        for (int off = 1; off < GS_WAVE; off <<= 1) {
-            const uint32_t y = __shfl_up(incl, off);
-            if (lane >= off) incl += y;
-        }
+        const uint32_t incl = wave_incl_sum(area);
         const uint32_t npairs = __builtin_amdgcn_readlane(incl, GS_WAVE - 1);
         int ux0 = ne ? x0 : INT_MAX, uy0 = ne ? y0 : INT_MAX, ux1 = ne ? x1 : 0, uy1 = ne ? y1 : 0;
 #pragma unroll
@@ -428,14 +424,23 @@ __global__ void __launch_bounds__(GS_WAVE) k_chunk_scatter(Dims d, BinArgs a)
         s_own[lane] = make_int4(x0, y0, x1 - x0, (int)g);
         s_key[lane] = dbits;
         __syncthreads();
-        // place the instances: pair p of the batch belongs to the first lane
-        // whose inclusive prefix exceeds p; its rank inside the tile is the
-        // number of lower lanes of the batch covering the same tile
-        for (uint32_t p = lane; p < npairs; p += GS_WAVE) {
-            int lo = 0;
-#pragma unroll
-            for (int step = GS_WAVE / 2; step >= 1; step >>= 1)
-                if (s_end[lo + step - 1] <= p) lo += step;
+        // place the instances, 64 pairs per window: pair p of the batch
+        // belongs to the last lane with a nonempty rect starting at or before
+        // p (start markers in LDS, a wave max-scan, the owner carried across
+        // windows); its rank inside the tile is the number of lower lanes of
+        // the batch covering the same tile
+        const uint32_t first = incl - area;
+        int carry = 0;
+        for (uint32_t base = 0; base < npairs; base += GS_WAVE) {
+            s_mark[lane] = -1;
+            __builtin_amdgcn_wave_barrier();
+            if (area > 0 && first >= base && first < base + GS_WAVE) s_mark[first - base] = lane;
+            __builtin_amdgcn_wave_barrier();
+            const int lo = max(carry, wave_incl_max(s_mark[lane]));
+            carry = __builtin_amdgcn_readlane(lo, GS_WAVE - 1);
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t p = base + lane;
+            if (p >= npairs) continue;
             const int4 ow = s_own[lo];
             const uint32_t q = p - (lo ? s_end[lo - 1] : 0u);   // pair index within the owner's rect
             const int ty = ow.y + (int)(q / (uint32_t)ow.z);
