@@ -177,6 +177,9 @@ int check_map(int R, int W, int H)
     if (R < 0 || W <= 0 || H <= 0) return fail(GSPARC_ERR_ARG, "negative or zero size");
     if ((int64_t)R * W * H > (int64_t)1 << 31) return fail(GSPARC_ERR_ARG, "R*H*W too large");
     if ((W + 15) / 16 + (H + 15) / 16 > 8192) return fail(GSPARC_ERR_ARG, "W + H too large (at most 131072 pixels)");
+    // the ordered scatter divides tile indices in float: exact below 2^20 tiles
+    if ((int64_t)((W + 15) / 16) * ((H + 15) / 16) >= (int64_t)1 << 20)
+        return fail(GSPARC_ERR_ARG, "W * H too large (at most 2^20 tiles of 16x16)");
     return GSPARC_OK;
 }
 
@@ -187,6 +190,9 @@ int check_dims(int P, int R, int W, int H)
     if ((int64_t)P * R >= (int64_t)1 << 31) return fail(GSPARC_ERR_ARG, "P*R too large");
     // the binning keeps one 64-bit mask per tile column and row in LDS
     if ((W + 15) / 16 + (H + 15) / 16 > 8192) return fail(GSPARC_ERR_ARG, "W + H too large (at most 131072 pixels)");
+    // the ordered scatter divides tile indices in float: exact below 2^20 tiles
+    if ((int64_t)((W + 15) / 16) * ((H + 15) / 16) >= (int64_t)1 << 20)
+        return fail(GSPARC_ERR_ARG, "W * H too large (at most 2^20 tiles of 16x16)");
     return GSPARC_OK;
 }
 

@@ -366,6 +366,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_chunk_scatter(Dims d, BinArgs a)
     __shared__ int4 s_own[GS_WAVE];        // x0, y0, width, Gaussian index
     __shared__ uint32_t s_key[GS_WAVE];
     __shared__ int s_mark[GS_WAVE];        // lane whose pairs start at this window position
+    __shared__ float s_inv[GS_WAVE];       // 1 / rect width
     uint64_t* s_mx = s_mask;
     uint64_t* s_my = s_mask + d.gx;
     uint32_t* s_base = (uint32_t*)(s_mask + d.gx + d.gy);
@@ -422,6 +423,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_chunk_scatter(Dims d, BinArgs a)
         }
         s_end[lane] = incl;
         s_own[lane] = make_int4(x0, y0, x1 - x0, (int)g);
+        s_inv[lane] = ne ? 1.0f / (float)(x1 - x0) : 0.0f;
         s_key[lane] = dbits;
         __syncthreads();
         // place the instances, 64 pairs per window: pair p of the batch
@@ -443,8 +445,12 @@ __global__ void __launch_bounds__(GS_WAVE) k_chunk_scatter(Dims d, BinArgs a)
             if (p >= npairs) continue;
             const int4 ow = s_own[lo];
             const uint32_t q = p - (lo ? s_end[lo - 1] : 0u);   // pair index within the owner's rect
-            const int ty = ow.y + (int)(q / (uint32_t)ow.z);
-            const int tx = ow.x + (int)(q % (uint32_t)ow.z);
+            // row = floor(q / w) as floor((q + 1/2) / w) in float: the half
+            // keeps the quotient 1/(2w) away from every integer, more than the
+            // rounding error while q + w < 2^20 (the host checks T < 2^20)
+            const uint32_t qy = (uint32_t)fmaf((float)q, s_inv[lo], 0.5f * s_inv[lo]);
+            const int ty = ow.y + (int)qy;
+            const int tx = ow.x + (int)(q - qy * (uint32_t)ow.z);
             const int t = ty * d.gx + tx;
             const uint64_t m = s_mx[tx] & s_my[ty];
             const uint32_t b = lds ? s_base[t] : slot[(size_t)t * a.nchunks];

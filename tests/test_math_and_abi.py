@@ -158,3 +158,20 @@ def test_scene_table_build():
     # too small a table and negative sizes are refused
     assert L.gsparc_scene_table_build(S, (C.c_int * S)(*Ps), (C.c_int * S)(*Rs), tab, words - 1, C.byref(b)) != 0
     assert L.gsparc_scene_table_build(1, (C.c_int * 1)(-1), (C.c_int * 1)(1), tab, words, C.byref(b)) != 0
+
+
+def test_argument_checks_before_any_device_work():
+    """Size checks run on the host before any HIP call: a map of 2^20 or more
+    16x16 tiles (the float tile-row division of the ordered scatter is exact
+    below that) and a column-plus-row count over 8192 are refused."""
+    import ctypes as C
+    import gsparc_hip as gh
+    L = gh.lib()
+    K = C.c_uint64(0)
+    img = C.c_void_p(1)
+    assert L.gsparc_instance_count(10, 1, 16 * 1024, 16 * 1024, img, C.byref(K), None) != 0
+    assert b"2^20 tiles" in L.gsparc_last_error()
+    assert L.gsparc_instance_count(10, 1, 16 * 8000, 16 * 200, img, C.byref(K), None) != 0
+    assert b"W + H too large" in L.gsparc_last_error()
+    assert L.gsparc_instance_count(10, 1, 360, 90, None, C.byref(K), None) != 0
+    assert b"null pointer" in L.gsparc_last_error()
