@@ -85,6 +85,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
             mask &= mask - 1;
             const float4 A = s0[j];
             const float4 B = s1[j];
+            const float2 S = ss[j];
 #pragma unroll
             for (int k = 0; k < PPL; ++k) {
                 if (COUNT) cnt[k] += T[k] > 0.0f ? 1u : 0u;
@@ -95,7 +96,6 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
                 if (alpha < 1.0f / 255.0f) continue;
                 const float test_T = T[k] * (1.0f - alpha);
                 if (test_T < 0.0001f) { T[k] = -fabsf(T[k]); continue; }
-                const float2 S = ss[j];
                 const float w = alpha * T[k];
                 are[k] = fmaf(S.x, w, are[k]);
                 aim[k] = fmaf(S.y, w, aim[k]);
