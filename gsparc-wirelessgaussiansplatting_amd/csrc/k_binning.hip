@@ -24,6 +24,7 @@
 #include "gs_device.h"
 #include "gs_kernels.h"
 
+#include <algorithm>
 #include <climits>
 
 namespace gs {
@@ -544,10 +545,19 @@ __global__ void __launch_bounds__(256) k_tile_order_rx(const uint2* ranges, int 
     }
 }
 
-__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n)
+// p[0 .. n) = v: the words up to 16-byte alignment one by one, the bulk as
+// 16-byte stores over a grid-stride loop, then the tail.
+__global__ void __launch_bounds__(256) k_fill_u32(uint32_t* p, uint32_t v, size_t n)
 {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
+    const size_t head = min(n, (size_t)((16u - ((uintptr_t)p & 15u)) & 15u) / 4);
+    const size_t n4 = (n - head) / 4;
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x, nth = (size_t)gridDim.x * blockDim.x;
+    if (tid < head) p[tid] = v;
+    uint4* q = reinterpret_cast<uint4*>(p + head);
+    const uint4 vv = make_uint4(v, v, v, v);
+    for (size_t i = tid; i < n4; i += nth) q[i] = vv;
+    const size_t t0 = head + 4 * n4;
+    if (tid < n - t0) p[t0 + tid] = v;
 }
 
 // ---------------------------------------------------------------- launchers
@@ -640,7 +650,8 @@ hipError_t launch_tile_order(const Dims& d, int mode, const uint2* ranges, uint3
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, v, n);
+    const size_t blocks = std::min<size_t>((n / 4 + 255) / 256 + 1, 8192);
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, s, p, v, n);
     return hipGetLastError();
 }
 
