@@ -99,8 +99,11 @@ __device__ __forceinline__ void rs_load(const Dims& d, const BinArgs& a, size_t 
                                         uint32_t& val, uint32_t& dg)
 {
     if (FIRST) {
+        // both loads unconditional (a culled Gaussian's depth is read and
+        // discarded), so that a thread's loads can all be in flight at once
         const bool vis = a.radii[rP + i] > 0;
-        key = vis ? gs_as_uint(a.depth[rP + i]) : 0xffffffffu;
+        const uint32_t bits = gs_as_uint(a.depth[rP + i]);
+        key = vis ? bits : 0xffffffffu;
         val = (uint32_t)i;
         dg = vis ? (key & 0xffu) : 256u;
     } else {
@@ -122,14 +125,22 @@ __global__ void __launch_bounds__(GS_RS_THREADS) k_rs_hist(Dims d, BinArgs a, in
     for (int i = threadIdx.x; i < NB; i += GS_RS_THREADS) h[i] = 0;
     __syncthreads();
     if (blk * GS_RS_TILE < n) {
+        // all of the thread's loads first and unconditional (positions past
+        // the end read the last element and are masked below), so that their
+        // latencies overlap
+        uint32_t dg[GS_RS_ITEMS];
+#pragma unroll
         for (int k = 0; k < GS_RS_ITEMS; ++k) {
-            const int i = blk * GS_RS_TILE + k * GS_RS_THREADS + threadIdx.x;
-            const bool valid = i < n;
-            uint32_t key = 0, val = 0, dg = 0;
-            if (valid) rs_load<FIRST>(d, a, rP, i, pass, key, val, dg);
-            const uint64_t m = match_digit<FIRST ? 9 : 8>(dg, valid);
+            const int i = min(n - 1, blk * GS_RS_TILE + k * GS_RS_THREADS + (int)threadIdx.x);
+            uint32_t key, val;
+            rs_load<FIRST>(d, a, rP, i, pass, key, val, dg[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < GS_RS_ITEMS; ++k) {
+            const bool valid = blk * GS_RS_TILE + k * GS_RS_THREADS + (int)threadIdx.x < n;
+            const uint64_t m = match_digit<FIRST ? 9 : 8>(dg[k], valid);
             // one LDS atomic per distinct digit of the wave
-            if (valid && (m & lanemask_lt()) == 0) atomicAdd(&h[dg], (uint32_t)__popcll(m));
+            if (valid && (m & lanemask_lt()) == 0) atomicAdd(&h[dg[k]], (uint32_t)__popcll(m));
         }
     }
     __syncthreads();
@@ -204,12 +215,17 @@ __global__ void __launch_bounds__(GS_RS_THREADS) k_rs_scatter(Dims d, BinArgs a,
     __syncthreads();
     uint32_t key[GS_RS_ITEMS], val[GS_RS_ITEMS], dg[GS_RS_ITEMS], rnk[GS_RS_ITEMS];
     const int e0 = wave * WT;
+    // all of the thread's loads first and unconditional (positions past the
+    // end read the last element and are masked below), so that their
+    // latencies overlap
 #pragma unroll
     for (int k = 0; k < GS_RS_ITEMS; ++k) {
-        const int i = blk * GS_RS_TILE + e0 + k * GS_WAVE + lane;
-        const bool valid = i < n;
-        key[k] = 0; val[k] = 0; dg[k] = 0;
-        if (valid) rs_load<FIRST>(d, a, rP, i, pass, key[k], val[k], dg[k]);
+        const int i = min(n - 1, blk * GS_RS_TILE + e0 + k * GS_WAVE + lane);
+        rs_load<FIRST>(d, a, rP, i, pass, key[k], val[k], dg[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < GS_RS_ITEMS; ++k) {
+        const bool valid = blk * GS_RS_TILE + e0 + k * GS_WAVE + lane < n;
         const uint64_t m = match_digit<FIRST ? 9 : 8>(dg[k], valid);
         const uint32_t prev = valid ? whist[wave][dg[k]] : 0u;
         rnk[k] = prev + (uint32_t)__popcll(m & lanemask_lt());
