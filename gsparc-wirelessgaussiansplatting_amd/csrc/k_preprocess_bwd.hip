@@ -167,10 +167,23 @@ __global__ void __launch_bounds__(GS_BWD_THREADS) k_pre_bwd_geo(Dims d, PreBwdAr
     const GroupMap grp = group_map(d, blockIdx.y);
     const SceneMap sc = scene_map(d, grp.scene);
     const int g = blockIdx.x * GS_BWD_THREADS + threadIdx.x;   // within the scene
-    if (g >= (int)sc.P) return;   // no block-level synchronisation below
-    const int gg = (int)sc.g0 + g;
     const int r0 = grp.r0;
     const int nr = grp.nr;
+    // the group's view matrices, receiver positions and record offsets, read
+    // once per block: inside the receiver loop the only vector-memory loads
+    // are then the (prefetched) per-pair records
+    __shared__ float s_view[GS_RC * 16];
+    __shared__ float s_rx[GS_RC * 3];
+    __shared__ uint32_t s_roff[GS_RC];
+    if (blockIdx.x * GS_BWD_THREADS >= (int)sc.P) return;
+    for (int i = threadIdx.x; i < GS_RC * 16; i += GS_BWD_THREADS) {
+        s_view[i] = i < 16 * nr ? a.views[16 * r0 + i] : 0.0f;
+        if (i < GS_RC * 3) s_rx[i] = i < 3 * nr ? a.rx[3 * r0 + i] : 0.0f;
+        if (i < GS_RC) s_roff[i] = i < nr ? rx_map(d, r0 + i).roff : 0u;
+    }
+    __syncthreads();
+    if (g >= (int)sc.P) return;   // no block-level synchronisation below
+    const int gg = (int)sc.g0 + g;
     const float mean[3] = {a.means[3 * gg], a.means[3 * gg + 1], a.means[3 * gg + 2]};
     // cov3D as the forward stored it
     float c3[6];
@@ -183,22 +196,37 @@ __global__ void __launch_bounds__(GS_BWD_THREADS) k_pre_bwd_geo(Dims d, PreBwdAr
     const float den = (MODE == GRAD_REFERENCE) ? de * de : dist * de;
 
     float dmean[3] = {0, 0, 0}, dc[6] = {0, 0, 0, 0, 0, 0}, dop = 0.0f, db2_0 = 0.0f, db2_1 = 0.0f;
+    // The records of receiver c + 1 are loaded (unconditionally: a culled
+    // pair's record is read and discarded) while receiver c is computed.
+    const float4* g8v = reinterpret_cast<const float4*>(a.g8);
+    size_t ri_n = (size_t)s_roff[0] + g;
+    int rad_n = a.radii[ri_n];
+    float4 ga_n = g8v[2 * ri_n], gb_n = g8v[2 * ri_n + 1];
+    float2 sg_n = a.mlp ? a.sig[ri_n] : make_float2(0.0f, 0.0f);
 #pragma unroll
     for (int c = 0; c < GS_RC; ++c) {
         if (c >= nr) break;
         const int r = r0 + c;
-        const size_t ri = (size_t)rx_map(d, r).roff + g;
+        const size_t ri = ri_n;
+        const int rad = rad_n;
+        const float4 ga = ga_n, gb = gb_n;
+        const float2 sg = sg_n;
+        if (c + 1 < nr) {
+            ri_n = (size_t)s_roff[c + 1] + g;
+            rad_n = a.radii[ri_n];
+            ga_n = g8v[2 * ri_n];
+            gb_n = g8v[2 * ri_n + 1];
+            if (a.mlp) sg_n = a.sig[ri_n];
+        }
         float2 dz = make_float2(0.0f, 0.0f);
-        if (a.radii[ri] > 0) {
-            const float4 ga = reinterpret_cast<const float4*>(a.g8)[2 * ri];
-            const float4 gb = reinterpret_cast<const float4*>(a.g8)[2 * ri + 1];
+        if (rad > 0) {
             const float gv[8] = {ga.x, ga.y, ga.z, ga.w, gb.x, gb.y, gb.z, gb.w};
-            geo_bwd<MODE>(mean, c3, a.views + 16 * r, d.W, d.H, gv, dmean, dc);
+            geo_bwd<MODE>(mean, c3, s_view + 16 * c, d.W, d.H, gv, dmean, dc);
             dop += gv[5];
             if (a.gL) {
                 // CSI phase: L = |mu - tx| + |mu - rx|
                 const float gl = a.gL[ri];
-                const float* rxp = a.rx + 3 * r;
+                const float* rxp = s_rx + 3 * c;
                 const float b[3] = {mean[0] - rxp[0], mean[1] - rxp[1], mean[2] - rxp[2]};
                 const float db = sqrtf(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
 #pragma unroll
@@ -211,7 +239,6 @@ __global__ void __launch_bounds__(GS_BWD_THREADS) k_pre_bwd_geo(Dims d, PreBwdAr
             }
             if (a.mlp) {
                 // MLP outputs as the forward computed them
-                const float2 sg = a.sig[ri];
                 const float s0 = sg.x, s1 = sg.y;
                 dz.x = gv[6] * s0 * (1.0f - s0);
                 dz.y = gv[7] * s1 * (1.0f - s1);
