@@ -402,8 +402,13 @@ __global__ void __launch_bounds__(GS_PB_THREADS) k_pre_bwd_param(Dims d, PreBwdA
                 s_rx[3 * tid + 2] = a.rx[3 * r + 2];
             }
             __syncthreads();
+            // receiver k + 1's dL/d(MLP output) is loaded while k is computed
+            const int gc = valid ? gl : 0;
+            float2 dz_n = a.dz[(size_t)s_roff[0] + gc];
             for (int k = 0; k < nrb; ++k) {
-                const float2 dz = valid ? a.dz[(size_t)s_roff[k] + gl] : make_float2(0.0f, 0.0f);
+                const float2 dzl = dz_n;
+                if (k + 1 < nrb) dz_n = a.dz[(size_t)s_roff[k + 1] + gc];
+                const float2 dz = valid ? dzl : make_float2(0.0f, 0.0f);
                 const float rx0 = s_rx[3 * k], rx1 = s_rx[3 * k + 1], rx2 = s_rx[3 * k + 2];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
