@@ -162,10 +162,12 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
     const uint32_t rb = __builtin_amdgcn_readfirstlane(range.x);
     const size_t rP = rx_map(d, r).roff;
 
-    float acc_re[PPL], acc_im[PPL], last_alpha[PPL], last_re[PPL], last_im[PPL];
+    // accum of backward.cu:579-580, updated right after each contributor
+    // instead of before the next one (the same values, no last_* state)
+    float acc_re[PPL], acc_im[PPL];
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
-        acc_re[k] = 0.0f; acc_im[k] = 0.0f; last_alpha[k] = 0.0f; last_re[k] = 0.0f; last_im[k] = 0.0f;
+        acc_re[k] = 0.0f; acc_im[k] = 0.0f;
     }
     // lanes 16, 24, 32 sum the conic terms -1/2 D dx^2, -D dx dy, -1/2 D dy^2
     // (backward.cu:608-610; the off-diagonal's 1/2 is applied by the
@@ -226,15 +228,16 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
                 const float w = (c ? alpha : 0.0f) * T[k];
                 const float sr = S.x;
                 const float si = (MODE == GRAD_REFERENCE) ? S.x : S.y;
-                // accum = last_alpha last_c + (1 - last_alpha) accum (backward.cu:579)
-                acc_re[k] = c ? fmaf(last_alpha[k], last_re[k] - acc_re[k], acc_re[k]) : acc_re[k];
-                if (MODE == GRAD_EXACT)
-                    acc_im[k] = c ? fmaf(last_alpha[k], last_im[k] - acc_im[k], acc_im[k]) : acc_im[k];
-                last_re[k] = c ? sr : last_re[k];
-                if (MODE == GRAD_EXACT) last_im[k] = c ? si : last_im[k];
-                last_alpha[k] = c ? alpha : last_alpha[k];
-                float dalpha = (sr - acc_re[k]) * gre[k];
-                if (MODE == GRAD_EXACT) dalpha = fmaf(si - acc_im[k], gim[k], dalpha);
+                // dL/dalpha = (c - accum) dL/dpixel T (backward.cu:586, 592), then
+                // accum = alpha c + (1 - alpha) accum for the next contributor
+                const float er = sr - acc_re[k];
+                float dalpha = er * gre[k];
+                acc_re[k] = c ? fmaf(alpha, er, acc_re[k]) : acc_re[k];
+                if (MODE == GRAD_EXACT) {
+                    const float ei = si - acc_im[k];
+                    dalpha = fmaf(ei, gim[k], dalpha);
+                    acc_im[k] = c ? fmaf(alpha, ei, acc_im[k]) : acc_im[k];
+                }
                 dalpha *= T[k];
                 swr = fmaf(w, gre[k], swr);
                 swi = fmaf(w, gim[k], swi);
