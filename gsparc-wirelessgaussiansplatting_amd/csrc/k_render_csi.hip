@@ -191,7 +191,9 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd_csi(Dims d, RenderCsiArg
     const float pxf = (float)pm.px[0], pyf = (float)pm.py[0];
     const float two_pi = 6.283185307179586f;
 
-    float acc = 0.0f, last_alpha = 0.0f, last_h = 0.0f;
+    // accum of backward.cu:579, updated right after each contributor instead
+    // of before the next one (the same values, no last-alpha state)
+    float acc = 0.0f;
     for (int top = maxlast - 1; top >= 0; top -= CSI_BATCH) {
         // entry j of this batch sits at list position top - j
         const int rel = top - lane;
@@ -257,10 +259,9 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd_csi(Dims d, RenderCsiArg
                     if (k2 > 0) { z1r = z1r + zr; z1i = z1i + zi; }
                 }
                 const float h = fmaf(S.x, zr, -(S.y * zi));
-                acc = last_alpha * last_h + (1.0f - last_alpha) * acc;
-                last_h = h;
-                const float dalpha = (h - acc) * T;
-                last_alpha = alpha;
+                const float eh = h - acc;
+                const float dalpha = eh * T;
+                acc = fmaf(alpha, eh, acc);
                 v[6] = w * zr;
                 v[7] = -(w * zi);
                 const float yr = fmaf(a.csi.dk, z1r, a.csi.k0 * zr);
