@@ -7,38 +7,53 @@
 //   SSIM  mean SSIM of the magnitude maps |C_k|, one 11x11-window SSIM plane
 //         per (receiver, subcarrier) (R*NC planes of H x W).
 // k_csi_mag turns the interleaved [R*N][NC][2] maps into magnitude planes
-// [R*NC][N] (coalesced per subcarrier) and sums |out - target| per
-// workgroup; the SSIM kernels run on the planes; k_csi_grad adds the L1 sign
-// term and carries dL/d|C_k| back to (re, im) through C_k / |C_k| (0 at 0).
+// [R*NC][N] and sums |out - target| per workgroup; the SSIM kernels run on the
+// planes; k_csi_grad adds the L1 sign term and carries dL/d|C_k| back to
+// (re, im) through C_k / |C_k| (0 at 0).  Both kernels take CL_PIX
+// consecutive pixels per workgroup: the interleaved maps are read (and the
+// gradient written) as consecutive 16-byte words of the block's contiguous
+// span, and the magnitude planes go through an LDS transpose, so every global
+// access is coalesced.
 #include "gs_device.h"
 #include "gs_kernels.h"
 
 namespace gs {
 
 #define CL_THREADS 256
+#define CL_PIX 64                         // pixels per workgroup
+#define CL_Q (GS_CSI_NC / 2)              // float4 per pixel (two subcarriers)
 
 __global__ void __launch_bounds__(CL_THREADS) k_csi_mag(size_t npix, int N, const float* out, const float* tgt,
                                                         float* magA, float* magB, float* part)
 {
+    __shared__ float sA[GS_CSI_NC][CL_PIX + 1];
+    __shared__ float sB[GS_CSI_NC][CL_PIX + 1];
     __shared__ float red[CL_THREADS];
-    const size_t p = (size_t)blockIdx.x * CL_THREADS + threadIdx.x;
+    const size_t p0 = (size_t)blockIdx.x * CL_PIX;
+    const int np = (int)min((size_t)CL_PIX, npix - p0);
+    const float4* a = reinterpret_cast<const float4*>(out + p0 * (2 * GS_CSI_NC));
+    const float4* b = reinterpret_cast<const float4*>(tgt + p0 * (2 * GS_CSI_NC));
     float l1 = 0.0f;
-    if (p < npix) {
-        const size_t r = p / (size_t)N, q = p % (size_t)N;
-        const float4* a = reinterpret_cast<const float4*>(out + p * (2 * GS_CSI_NC));
-        const float4* b = reinterpret_cast<const float4*>(tgt + p * (2 * GS_CSI_NC));
-        for (int k2 = 0; k2 < GS_CSI_NC / 2; ++k2) {
-            const float4 x = a[k2], y = b[k2];
-            const size_t o0 = (r * GS_CSI_NC + 2 * k2) * (size_t)N + q;
-            magA[o0] = sqrtf(x.x * x.x + x.y * x.y);
-            magA[o0 + N] = sqrtf(x.z * x.z + x.w * x.w);
-            magB[o0] = sqrtf(y.x * y.x + y.y * y.y);
-            magB[o0 + N] = sqrtf(y.z * y.z + y.w * y.w);
-            l1 += fabsf(x.x - y.x) + fabsf(x.y - y.y) + fabsf(x.z - y.z) + fabsf(x.w - y.w);
-        }
+    for (int i = threadIdx.x; i < np * CL_Q; i += CL_THREADS) {
+        const int pl = i / CL_Q, k2 = i % CL_Q;
+        const float4 x = a[i], y = b[i];
+        sA[2 * k2][pl] = sqrtf(x.x * x.x + x.y * x.y);
+        sA[2 * k2 + 1][pl] = sqrtf(x.z * x.z + x.w * x.w);
+        sB[2 * k2][pl] = sqrtf(y.x * y.x + y.y * y.y);
+        sB[2 * k2 + 1][pl] = sqrtf(y.z * y.z + y.w * y.w);
+        l1 += fabsf(x.x - y.x) + fabsf(x.y - y.y) + fabsf(x.z - y.z) + fabsf(x.w - y.w);
     }
     red[threadIdx.x] = l1;
     __syncthreads();
+    for (int i = threadIdx.x; i < GS_CSI_NC * CL_PIX; i += CL_THREADS) {
+        const int k = i / CL_PIX, pl = i % CL_PIX;
+        if (pl >= np) continue;
+        const size_t p = p0 + pl, r = p / (size_t)N, q = p % (size_t)N;
+        const size_t o = (r * GS_CSI_NC + k) * (size_t)N + q;
+        magA[o] = sA[k][pl];
+        magB[o] = sB[k][pl];
+    }
+    // fixed-order tree over the workgroup
     for (int w = CL_THREADS / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
@@ -52,20 +67,30 @@ __global__ void __launch_bounds__(CL_THREADS) k_csi_grad(size_t npix, int N, con
                                                          const float* magA, const float* dmag, float l1w,
                                                          float* dout)
 {
-    const size_t p = (size_t)blockIdx.x * CL_THREADS + threadIdx.x;
-    if (p >= npix) return;
-    const size_t r = p / (size_t)N, q = p % (size_t)N;
-    const float4* a = reinterpret_cast<const float4*>(out + p * (2 * GS_CSI_NC));
-    const float4* b = reinterpret_cast<const float4*>(tgt + p * (2 * GS_CSI_NC));
-    float4* g = reinterpret_cast<float4*>(dout + p * (2 * GS_CSI_NC));
-    for (int k2 = 0; k2 < GS_CSI_NC / 2; ++k2) {
-        const float4 x = a[k2], y = b[k2];
-        const size_t o0 = (r * GS_CSI_NC + 2 * k2) * (size_t)N + q;
-        const float m0 = magA[o0], m1 = magA[o0 + N];
-        const float f0 = m0 > 0.0f ? dmag[o0] / m0 : 0.0f;
-        const float f1 = m1 > 0.0f ? dmag[o0 + N] / m1 : 0.0f;
-        g[k2] = make_float4(fmaf(l1w, sgnf(x.x - y.x), f0 * x.x), fmaf(l1w, sgnf(x.y - y.y), f0 * x.y),
-                            fmaf(l1w, sgnf(x.z - y.z), f1 * x.z), fmaf(l1w, sgnf(x.w - y.w), f1 * x.w));
+    __shared__ float sM[GS_CSI_NC][CL_PIX + 1];
+    __shared__ float sD[GS_CSI_NC][CL_PIX + 1];
+    const size_t p0 = (size_t)blockIdx.x * CL_PIX;
+    const int np = (int)min((size_t)CL_PIX, npix - p0);
+    for (int i = threadIdx.x; i < GS_CSI_NC * CL_PIX; i += CL_THREADS) {
+        const int k = i / CL_PIX, pl = i % CL_PIX;
+        if (pl >= np) continue;
+        const size_t p = p0 + pl, r = p / (size_t)N, q = p % (size_t)N;
+        const size_t o = (r * GS_CSI_NC + k) * (size_t)N + q;
+        sM[k][pl] = magA[o];
+        sD[k][pl] = dmag[o];
+    }
+    __syncthreads();
+    const float4* a = reinterpret_cast<const float4*>(out + p0 * (2 * GS_CSI_NC));
+    const float4* b = reinterpret_cast<const float4*>(tgt + p0 * (2 * GS_CSI_NC));
+    float4* g = reinterpret_cast<float4*>(dout + p0 * (2 * GS_CSI_NC));
+    for (int i = threadIdx.x; i < np * CL_Q; i += CL_THREADS) {
+        const int pl = i / CL_Q, k2 = i % CL_Q;
+        const float4 x = a[i], y = b[i];
+        const float m0 = sM[2 * k2][pl], m1 = sM[2 * k2 + 1][pl];
+        const float f0 = m0 > 0.0f ? sD[2 * k2][pl] / m0 : 0.0f;
+        const float f1 = m1 > 0.0f ? sD[2 * k2 + 1][pl] / m1 : 0.0f;
+        g[i] = make_float4(fmaf(l1w, sgnf(x.x - y.x), f0 * x.x), fmaf(l1w, sgnf(x.y - y.y), f0 * x.y),
+                           fmaf(l1w, sgnf(x.z - y.z), f1 * x.z), fmaf(l1w, sgnf(x.w - y.w), f1 * x.w));
     }
 }
 
@@ -118,7 +143,7 @@ __global__ void __launch_bounds__(256) k_loss_stage2(const double* stage, double
     if (threadIdx.x == 0) loss[0] = (float)((double)l1w * a * inv_l + (double)sw * (1.0 - m * inv_s));
 }
 
-int csi_loss_blocks(size_t npix) { return (int)((npix + CL_THREADS - 1) / CL_THREADS); }
+int csi_loss_blocks(size_t npix) { return (int)((npix + CL_PIX - 1) / CL_PIX); }
 
 hipError_t launch_csi_mag(size_t npix, int N, const float* out, const float* tgt, float* magA, float* magB,
                           float* part, hipStream_t s)
