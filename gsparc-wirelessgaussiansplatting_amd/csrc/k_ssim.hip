@@ -6,8 +6,11 @@
 // (42 x 26) of the inputs is staged in LDS once; the horizontal 11-tap pass
 // produces all five (forward) or three (backward) filtered quantities in one
 // sweep, then the vertical pass finishes them per output pixel.  Summation
-// order matches the reference (taps 0..10, horizontal first), so the result
-// agrees with the CPU restatement to the last bit for the same taps.
+// order matches the reference (taps 0..10, horizontal first); the taps are
+// applied with fused multiply-adds (as CUDA compiles the reference's
+// `acc += w * x`), and the per-pixel partial derivatives share the two
+// reciprocals 1/A and 1/B, so the values agree with the CPU restatement to
+// rounding (a few ulp).
 #include <cmath>
 
 #include "gs_device.h"
@@ -47,11 +50,11 @@ __global__ void __launch_bounds__(SS_THREADS) k_ssim_fwd(SsimArgs a, Taps tp)
 #pragma unroll
         for (int k = 0; k < 11; ++k) {
             const float p = s1[ly][lx + k], q = s2[ly][lx + k];
-            m1 = m1 + tp.w[k] * p;
-            m2 = m2 + tp.w[k] * q;
-            e11 = e11 + tp.w[k] * (p * p);
-            e22 = e22 + tp.w[k] * (q * q);
-            e12 = e12 + tp.w[k] * (p * q);
+            m1 = fmaf(tp.w[k], p, m1);
+            m2 = fmaf(tp.w[k], q, m2);
+            e11 = fmaf(tp.w[k], p * p, e11);
+            e22 = fmaf(tp.w[k], q * q, e22);
+            e12 = fmaf(tp.w[k], p * q, e12);
         }
         h[0][ly][lx] = m1; h[1][ly][lx] = m2; h[2][ly][lx] = e11; h[3][ly][lx] = e22; h[4][ly][lx] = e12;
     }
@@ -66,7 +69,7 @@ __global__ void __launch_bounds__(SS_THREADS) k_ssim_fwd(SsimArgs a, Taps tp)
         for (int c = 0; c < 5; ++c) {
             float acc = 0.0f;
 #pragma unroll
-            for (int k = 0; k < 11; ++k) acc = acc + tp.w[k] * h[c][ly + k][lx];
+            for (int k = 0; k < 11; ++k) acc = fmaf(tp.w[k], h[c][ly + k][lx], acc);
             v[c] = acc;
         }
         const float mu1 = v[0], mu2 = v[1];
@@ -78,13 +81,16 @@ __global__ void __launch_bounds__(SS_THREADS) k_ssim_fwd(SsimArgs a, Taps tp)
         const float A = mu1 * mu1 + mu2 * mu2 + a.C1;
         const float B = sigma1_sq + sigma2_sq + a.C2;
         const size_t o = plane + (size_t)gy * a.W + gx;
-        const float m = (C * D) / (A * B);
+        // ssim.cu:269-283 with the divisions shared: iA = 1/A, iB = 1/B
+        const float iA = 1.0f / A, iB = 1.0f / B, iAB = iA * iB;
+        const float CD = C * D;
+        const float m = CD * iAB;
         if (a.map) a.map[o] = m;
         if (a.dmu1) {
-            a.dmu1[o] = (mu2 * 2.0f * D) / (A * B) - (mu2 * 2.0f * C) / (A * B) - (mu1 * 2.0f * C * D) / (A * A * B) +
-                        (mu1 * 2.0f * C * D) / (A * B * B);
-            a.ds1[o] = (-C * D) / (A * B * B);
-            a.ds12[o] = (2.0f * C) / (A * B);
+            // (2 mu2 D - 2 mu2 C) / (AB) - 2 mu1 CD / (A^2 B) + 2 mu1 CD / (A B^2)
+            a.dmu1[o] = 2.0f * iAB * fmaf(mu1 * CD, iB - iA, mu2 * (D - C));
+            a.ds1[o] = -CD * iAB * iB;
+            a.ds12[o] = 2.0f * C * iAB;
         }
         sum_map += m;
         sum_abs += fabsf(im1[(size_t)gy * a.W + gx] - im2[(size_t)gy * a.W + gx]);
@@ -137,7 +143,7 @@ __global__ void __launch_bounds__(SS_THREADS) k_ssim_bwd(SsimBwdArgs a, Taps tp)
         for (int c = 0; c < 3; ++c) {
             float acc = 0.0f;
 #pragma unroll
-            for (int k = 0; k < 11; ++k) acc = acc + tp.w[k] * s[c][ly][lx + k];
+            for (int k = 0; k < 11; ++k) acc = fmaf(tp.w[k], s[c][ly][lx + k], acc);
             h[c][ly][lx] = acc;
         }
     }
@@ -151,7 +157,7 @@ __global__ void __launch_bounds__(SS_THREADS) k_ssim_bwd(SsimBwdArgs a, Taps tp)
         for (int c = 0; c < 3; ++c) {
             float acc = 0.0f;
 #pragma unroll
-            for (int k = 0; k < 11; ++k) acc = acc + tp.w[k] * h[c][ly + k][lx];
+            for (int k = 0; k < 11; ++k) acc = fmaf(tp.w[k], h[c][ly + k][lx], acc);
             v[c] = acc;
         }
         const size_t o = plane + (size_t)gy * a.W + gx;

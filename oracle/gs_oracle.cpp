@@ -935,7 +935,10 @@ extern "C" void orc_preprocess_bwd_f64(BWD_ARGS(double)) { pre_bwd_all<double>(m
 // Fused SSIM (fused-ssim/ssim.cu:187-366, fused_ssim/__init__.py:8-41):
 // 11x11 separable Gaussian window (sigma 1.5) with zero padding.  Forward
 // returns the map and the three partial-derivative maps; backward returns
-// dL/dimg1.  Images are [B*CH] planes of H x W.
+// dL/dimg1.  Images are [B*CH] planes of H x W.  The taps accumulate as
+// fused multiply-adds in tap order (as CUDA compiles the reference's
+// `acc += w * x`) and the partial derivatives share 1/A and 1/B: the
+// kernels' operation order.
 // ---------------------------------------------------------------------------
 template <typename R>
 static void gauss11(R* w)
@@ -960,7 +963,7 @@ static void conv_same(int H, int W, const R* in, R* out)   // separable, x then 
             R v = R(0);
             for (int k = 0; k < 11; ++k) {
                 const int xx = x + k - 5;
-                v += w[k] * ((xx >= 0 && xx < W) ? in[(size_t)y * W + xx] : R(0));
+                v = Fn<R>::fma(w[k], (xx >= 0 && xx < W) ? in[(size_t)y * W + xx] : R(0), v);
             }
             tmp[(size_t)y * W + x] = v;
         }
@@ -969,7 +972,7 @@ static void conv_same(int H, int W, const R* in, R* out)   // separable, x then 
             R v = R(0);
             for (int k = 0; k < 11; ++k) {
                 const int yy = y + k - 5;
-                v += w[k] * ((yy >= 0 && yy < H) ? tmp[(size_t)yy * W + x] : R(0));
+                v = Fn<R>::fma(w[k], (yy >= 0 && yy < H) ? tmp[(size_t)yy * W + x] : R(0), v);
             }
             out[(size_t)y * W + x] = v;
         }
@@ -994,12 +997,16 @@ static void ssim_fwd(int planes, int H, int W, R C1, R C2, const R* a, const R* 
             const R s1 = e11[i] - m1 * m1, s2 = e22[i] - m2 * m2, s12 = e12[i] - m1 * m2;
             const R Cc = R(2) * m1 * m2 + C1, D = R(2) * s12 + C2;
             const R A = m1 * m1 + m2 * m2 + C1, B = s1 + s2 + C2;
-            map[p * N + i] = (Cc * D) / (A * B);
+            // ssim.cu:269-283: CD/(AB), and the partial derivatives
+            // (2 mu2 D - 2 mu2 C)/(AB) - 2 mu1 CD/(A^2 B) + 2 mu1 CD/(A B^2),
+            // -CD/(A B^2), 2C/(AB), with the divisions shared
+            const R iA = R(1) / A, iB = R(1) / B, iAB = iA * iB;
+            const R CD = Cc * D;
+            map[p * N + i] = CD * iAB;
             if (dmu1) {
-                dmu1[p * N + i] = (m2 * R(2) * D) / (A * B) - (m2 * R(2) * Cc) / (A * B) -
-                                  (m1 * R(2) * Cc * D) / (A * A * B) + (m1 * R(2) * Cc * D) / (A * B * B);
-                ds1[p * N + i] = (-Cc * D) / (A * B * B);
-                ds12[p * N + i] = (R(2) * Cc) / (A * B);
+                dmu1[p * N + i] = R(2) * iAB * Fn<R>::fma(m1 * CD, iB - iA, m2 * (D - Cc));
+                ds1[p * N + i] = -CD * iAB * iB;
+                ds12[p * N + i] = R(2) * Cc * iAB;
             }
         }
     }
