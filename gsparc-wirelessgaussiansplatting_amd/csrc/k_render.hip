@@ -267,7 +267,10 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
             v[6] = swr;
             v[7] = swi;
             const float tot = wave_sum8(v, lane) * post;
-            if ((lane & 7) == 0) atomicAdd(a.g8 + (rP + sid[j]) * 8 + (lane >> 3), tot);
+            // the record index is wave-uniform: its address arithmetic stays scalar
+            const uint32_t gid = __builtin_amdgcn_readfirstlane(sid[j]);
+            float* rec = a.g8 + (rP + gid) * 8;
+            if ((lane & 7) == 0) atomicAdd(rec + (lane >> 3), tot);
         }
         __syncthreads();
     }

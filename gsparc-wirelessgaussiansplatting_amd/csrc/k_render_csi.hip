@@ -281,7 +281,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd_csi(Dims d, RenderCsiArg
             if (ballot(c) == 0) continue;
             const float tot = wave_sum8(v, lane);
             const float totL = wave_sum(vL);
-            const size_t gi = rP + sid[j];
+            const size_t gi = rP + __builtin_amdgcn_readfirstlane(sid[j]);   // wave-uniform
             if ((lane & 7) == 0) atomicAdd(a.g8 + gi * 8 + (lane >> 3), tot);
             if (lane == 0) atomicAdd(a.gL + gi, totL);
         }
