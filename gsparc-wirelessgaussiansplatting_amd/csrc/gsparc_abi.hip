@@ -25,7 +25,7 @@ thread_local std::string g_err;
 int g_opt_fwd_ppl = 0;
 int g_opt_bwd_ppl = 0;
 int g_opt_order = 0;
-int g_opt_csi_fwd_ppl = 0;   // CSI forward blend: 1 or 2 pixels per lane, 0 = automatic
+int g_opt_csi_fwd_ppl = 0;   // CSI forward blend: 1 or 2 pixels per lane on the VALU, 0 = the MFMA kernel
 
 // Autotuned blend mappings, per batch shape (gsparc_autotune).
 struct TuneKey {
@@ -534,9 +534,8 @@ int forward_render_csi_impl(const gs::Dims& d, const GsCsi& csi, const float* em
     ca.total = ba.total;
     ca.Kcap = K_capacity;
     ca.csi = csi;
-    // half tiles per wave once the batch has enough tiles to fill the chip
-    const int ppl = g_opt_csi_fwd_ppl ? g_opt_csi_fwd_ppl : ((long long)d.R * d.T >= 1024 ? 2 : 1);
-    GS_LAUNCH("render_fwd_csi", gs::launch_render_fwd_csi(d, ca, ppl, s), s);
+    // 0: the matrix-core forward; 1, 2: the VALU forward with that many pixels per lane
+    GS_LAUNCH("render_fwd_csi", gs::launch_render_fwd_csi(d, ca, g_opt_csi_fwd_ppl, s), s);
     return GSPARC_OK;
 }
 

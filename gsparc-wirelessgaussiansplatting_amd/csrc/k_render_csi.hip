@@ -152,6 +152,159 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd_csi(Dims d, RenderCsiArg
     }
 }
 
+// Forward on the matrix cores.  Per batch of CSI_BATCH staged entries the
+// kept ones (block test passed) are compacted into slots 0..nk-1 in list
+// order; all 64 lanes then evaluate the kept entries' emissions
+// (slot, subcarrier pair) into LDS, and each lane walks the slots for its
+// pixel exactly as the VALU forward does (same alpha, T, termination and
+// contributor count), writing its blend weight w = alpha T (0 when the pair
+// does not contribute) to W[pixel][slot].  The batch's contribution to the
+// 64 x 104 map block is then the product W[64 x nk] E[nk x 104], done as
+// v_mfma_f32_32x32x2f32 steps (2 pixel blocks x 4 column blocks of 32, the
+// last padded from 104 to 128 with zeros) over ceil(nk / 2) entry pairs.  Only the order
+// of the sums over entries differs from the VALU forward.
+//
+// MFMA 32x32x2 f32 operand layout (wave64): lane l supplies A[l%32][l/32] and
+// B[l/32][l%32]; accumulator element v of lane l is C[8(v/4) + 4(l/32) +
+// v%4][l%32].
+#define CSI_EW 108            // LDS row of an entry's 104 emissions (+ 4)
+#define CSI_WW (CSI_BATCH + 1)
+
+typedef float floatx16_csi __attribute__((ext_vector_type(16)));
+
+__global__ void __launch_bounds__(GS_WAVE) __attribute__((amdgpu_waves_per_eu(2)))
+k_render_fwd_csi_mfma(Dims d, RenderCsiArgs a)
+{
+    __shared__ float4 sk0[CSI_BATCH];            // kept entries' splat records, by slot
+    __shared__ float4 sk1[CSI_BATCH];
+    __shared__ float2 ssig[CSI_BATCH];
+    __shared__ float splen[CSI_BATCH];
+    __shared__ int sjj[CSI_BATCH];               // staged position of slot k
+    __shared__ __attribute__((aligned(16))) float sE[CSI_BATCH][CSI_EW];
+    __shared__ float sW[GS_WAVE][CSI_WW];
+    const Pixels<1> pm = map_pixels<1>(d, a.order, (int)a.total[1]);
+    if (!pm.valid) return;
+    if ((uint64_t)a.total[0] > a.Kcap) return;
+    if (ballot(pm.inside[0]) == 0) return;
+    const int r = pm.r;
+    const int lane = threadIdx.x;
+    const uint2 range = a.ranges[(size_t)r * d.T + pm.tile];
+    const uint32_t rb = __builtin_amdgcn_readfirstlane(range.x);
+    const uint32_t re = __builtin_amdgcn_readfirstlane(range.y);
+    const size_t rP = rx_map(d, r).roff;
+
+    float T = pm.inside[0] ? 1.0f : -1.0f;   // T < 0: finished pixel
+    uint32_t last = 0;
+    const float pxf = (float)pm.px[0], pyf = (float)pm.py[0];
+    floatx16_csi acc[2][4];
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[mb][nb][v] = 0.0f;
+
+    for (uint32_t start = rb; start < re; start += CSI_BATCH) {
+        if (ballot(T > 0.0f) == 0) break;
+        // stage and test CSI_BATCH entries; compact the kept ones into slots
+        const uint32_t e = start + lane;
+        bool keep = false;
+        float4 A, B;
+        size_t gi = 0;
+        if (lane < CSI_BATCH && e < re) {
+            gi = rP + a.plist[e];
+            A = a.splat[2 * gi];
+            B = a.splat[2 * gi + 1];
+            keep = !block_misses(A.x, A.y, A.z, A.w, B.x, B.z, pm.x0, pm.x1, pm.y0, pm.y1);
+        }
+        const uint64_t mask = ballot(keep);
+        const int nk = __popcll(mask);
+        __syncthreads();   // the previous batch is done with the slot tables
+        if (keep) {
+            const int k = __popcll(mask & lanemask_lt());
+            sk0[k] = A;
+            sk1[k] = B;
+            ssig[k] = a.sig[gi];
+            splen[k] = a.plen[gi];
+            sjj[k] = lane;
+        }
+        if (nk == 0) continue;
+        __syncthreads();
+        // emissions of the kept entries, (slot, subcarrier pair) over all lanes;
+        // an odd last slot is paired with a zero row
+        for (int it = lane; it < nk * CSI_NC2; it += GS_WAVE) {
+            const int k = it / CSI_NC2, k2 = it % CSI_NC2;
+            const float2 S = ssig[k];
+            const float L = splen[k];
+            float c0, n0, c1, n1, e0r, e0i, e1r, e1i;
+            gs_sincos2pi(gs_cycle_frac(L, a.csi.khi[2 * k2], a.csi.klo[2 * k2]), &c0, &n0);
+            gs_sincos2pi(gs_cycle_frac(L, a.csi.khi[2 * k2 + 1], a.csi.klo[2 * k2 + 1]), &c1, &n1);
+            gs_csi_emit(S.x, S.y, c0, n0, &e0r, &e0i);
+            gs_csi_emit(S.x, S.y, c1, n1, &e1r, &e1i);
+            *reinterpret_cast<float4*>(&sE[k][4 * k2]) = make_float4(e0r, e0i, e1r, e1i);
+        }
+        if (nk & 1)
+            for (int c = lane; c < 2 * GS_CSI_NC; c += GS_WAVE) sE[nk][c] = 0.0f;
+        // the lane's pixel against the kept entries in list order
+        const uint32_t base = start - rb + 1;
+        for (int k = 0; k < nk; ++k) {
+            const float4 Ak = sk0[k], Bk = sk1[k];
+            float w = 0.0f;
+            const float dx = Ak.x - pxf, dy = Ak.y - pyf;
+            const float power = pair_power(Ak.z, Ak.w, Bk.x, dx, dy);
+            if (!(power > 0.0f || power < Bk.z)) {
+                const float alpha = fminf(0.99f, Bk.y * gs_expf_blend(power));
+                if (!(alpha < 1.0f / 255.0f)) {
+                    const float test_T = T * (1.0f - alpha);
+                    if (test_T < 0.0001f) {
+                        T = -fabsf(T);
+                    } else {
+                        w = alpha * T;
+                        T = test_T;
+                        last = base + (uint32_t)sjj[k];
+                    }
+                }
+            }
+            sW[lane][k] = w;
+        }
+        if (nk & 1) sW[lane][nk] = 0.0f;
+        __syncthreads();
+        // C[64 x 128] += W[64 x nk] E[nk x 128]
+        const int half = lane >> 5, l32 = lane & 31;
+        for (int k0 = 0; k0 < nk; k0 += 2) {
+            const float a0 = sW[l32][k0 + half];
+            const float a1 = sW[32 + l32][k0 + half];
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) {
+                // columns past the 104 emissions multiply zeros
+                const float bv = (nb < 3 || l32 < 2 * GS_CSI_NC - 96) ? sE[k0 + half][32 * nb + l32] : 0.0f;
+                acc[0][nb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, bv, acc[0][nb], 0, 0, 0);
+                acc[1][nb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, bv, acc[1][nb], 0, 0, 0);
+            }
+        }
+    }
+    // the map block: row m of the accumulators is the pixel of lane m
+    const int bx = pm.px[0] - (lane & 7), by = pm.py[0] - (lane >> 3);
+    const size_t plane = (size_t)r * d.W * d.H;
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int m = 32 * mb + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                const int c = 32 * nb + (lane & 31);
+                const int x = bx + (m & 7), y = by + (m >> 3);
+                if (c < 2 * GS_CSI_NC && x < d.W && y < d.H)
+                    a.out[(plane + (size_t)y * d.W + x) * (2 * GS_CSI_NC) + c] = acc[mb][nb][v];
+            }
+    if (pm.inside[0]) {
+        const size_t p = plane + (size_t)pm.py[0] * d.W + pm.px[0];
+        a.finalT[p] = fabsf(T);
+        a.ncontrib[p] = last;
+    }
+}
+
 __global__ void __launch_bounds__(GS_WAVE) k_render_bwd_csi(Dims d, RenderCsiArgs a)
 {
     __shared__ float4 s0[CSI_BATCH];
@@ -292,7 +445,9 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd_csi(Dims d, RenderCsiArg
 hipError_t launch_render_fwd_csi(const Dims& d, const RenderCsiArgs& a, int ppl, hipStream_t s)
 {
     if (d.R == 0 || d.T == 0) return hipSuccess;
-    if (ppl == 2)
+    if (ppl == 0)
+        hipLaunchKernelGGL(k_render_fwd_csi_mfma, dim3((unsigned)(order_length(d) * 4)), dim3(GS_WAVE), 0, s, d, a);
+    else if (ppl == 2)
         hipLaunchKernelGGL(k_render_fwd_csi<2>, dim3((unsigned)(order_length(d) * 2)), dim3(GS_WAVE), 0, s, d, a);
     else
         hipLaunchKernelGGL(k_render_fwd_csi<1>, dim3((unsigned)(order_length(d) * 4)), dim3(GS_WAVE), 0, s, d, a);

@@ -31,7 +31,7 @@ def _fwd(sc, d, **kw):
     return out
 
 
-@pytest.mark.parametrize("ppl", [1, 2], ids=["quarter_tile", "half_tile"])
+@pytest.mark.parametrize("ppl", [0, 1, 2], ids=["mfma", "quarter_tile", "half_tile"])
 @pytest.mark.parametrize("seed,P,R", [(101, 300, 1), (102, 1500, 3)])
 def test_csi_forward_matches_oracle(seed, P, R, ppl):
     import gsparc_hip as gh
