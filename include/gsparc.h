@@ -66,6 +66,11 @@ const char* gsparc_last_error(void);
  *       receivers on, else 1), 1 = one list dealt
  *       round robin to the 8 XCDs, 2 = XCD x takes the receivers x mod 8,
  *       3 = as 2, one receiver after the other.  Speed only.
+ *   "csi_fwd_ppl"  the multi-frequency forward blend: 0 = the matrix-core
+ *       kernel (the blend weights of a batch of entries times their 104
+ *       emissions by MFMA), 1 or 2 = the vector-ALU kernel with that many
+ *       pixels per lane.  The maps differ only by the order of the sums over
+ *       entries; final T and contributor counts are identical.
  * "Automatic" means: what gsparc_autotune chose for a batch of this shape
  * (P, R, W, H, scenes), else the built-in rule above. */
 int gsparc_set_option(const char* name, int value);
