@@ -36,12 +36,32 @@ __global__ void __launch_bounds__(SS_THREADS) k_ssim_fwd(SsimArgs a, Taps tp)
     const size_t plane = (size_t)blockIdx.z * a.H * a.W;
     const float* im1 = a.img1 + plane;
     const float* im2 = a.img2 + plane;
-    for (int i = tid; i < SS_HH * SS_HW; i += SS_THREADS) {
-        const int ly = i / SS_HW, lx = i % SS_HW;
-        const int gy = y0 + ly - 5, gx = x0 + lx - 5;
-        const bool in = gx >= 0 && gx < a.W && gy >= 0 && gy < a.H;
-        s1[ly][lx] = in ? im1[(size_t)gy * a.W + gx] : 0.0f;
-        s2[ly][lx] = in ? im2[(size_t)gy * a.W + gx] : 0.0f;
+    // halo staging: all of a thread's loads are issued before the first LDS
+    // store (clamped in-plane addresses, zero outside), so that their
+    // latencies overlap
+    {
+        constexpr int NS = (SS_HH * SS_HW + SS_THREADS - 1) / SS_THREADS;
+        float v1[NS], v2[NS];
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+            const int i = tid + u * SS_THREADS;
+            const int ly = i / SS_HW, lx = i % SS_HW;
+            const int gy = y0 + ly - 5, gx = x0 + lx - 5;
+            const size_t o = (size_t)min(max(gy, 0), a.H - 1) * a.W + min(max(gx, 0), a.W - 1);
+            v1[u] = im1[o];
+            v2[u] = im2[o];
+        }
+        __builtin_amdgcn_sched_barrier(0);   // keep the loads ahead of the stores
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+            const int i = tid + u * SS_THREADS;
+            if (i >= SS_HH * SS_HW) break;
+            const int ly = i / SS_HW, lx = i % SS_HW;
+            const int gy = y0 + ly - 5, gx = x0 + lx - 5;
+            const bool in = gx >= 0 && gx < a.W && gy >= 0 && gy < a.H;
+            s1[ly][lx] = in ? v1[u] : 0.0f;
+            s2[ly][lx] = in ? v2[u] : 0.0f;
+        }
     }
     __syncthreads();
     for (int i = tid; i < SS_HH * SS_TW; i += SS_THREADS) {
@@ -93,7 +113,7 @@ __global__ void __launch_bounds__(SS_THREADS) k_ssim_fwd(SsimArgs a, Taps tp)
             a.ds12[o] = 2.0f * C * iAB;
         }
         sum_map += m;
-        sum_abs += fabsf(im1[(size_t)gy * a.W + gx] - im2[(size_t)gy * a.W + gx]);
+        sum_abs += fabsf(s1[ly + 5][lx + 5] - s2[ly + 5][lx + 5]);   // the staged pixel values
     }
     if (a.part) {
         // fixed-order tree over the workgroup: the loss is the same run to run
@@ -123,18 +143,44 @@ __global__ void __launch_bounds__(SS_THREADS) k_ssim_bwd(SsimBwdArgs a, Taps tp)
     const int tid = threadIdx.x;
     const int x0 = blockIdx.x * SS_TW, y0 = blockIdx.y * SS_TH;
     const size_t plane = (size_t)blockIdx.z * a.H * a.W;
-    for (int i = tid; i < SS_HH * SS_HW; i += SS_THREADS) {
-        const int ly = i / SS_HW, lx = i % SS_HW;
-        const int gy = y0 + ly - 5, gx = x0 + lx - 5;
-        float p0 = 0.0f, p1 = 0.0f, p2 = 0.0f;
-        if (gx >= 0 && gx < a.W && gy >= 0 && gy < a.H) {
-            const size_t o = plane + (size_t)gy * a.W + gx;
-            const float g = a.dmap ? a.dmap[o] : a.dmap_const;
-            p0 = g * a.dmu1[o];
-            p1 = g * a.ds1[o];
-            p2 = g * a.ds12[o];
+    // the images at the thread's output pixels, loaded now and used at the end
+    constexpr int NO = SS_TH * SS_TW / SS_THREADS;
+    static_assert(NO * SS_THREADS == SS_TH * SS_TW, "whole outputs per thread");
+    float xi[NO], yi[NO];
+#pragma unroll
+    for (int u = 0; u < NO; ++u) {
+        const int i = tid + u * SS_THREADS;
+        const int gy = min(y0 + i / SS_TW, a.H - 1), gx = min(x0 + i % SS_TW, a.W - 1);
+        xi[u] = a.img1[plane + (size_t)gy * a.W + gx];
+        yi[u] = a.img2[plane + (size_t)gy * a.W + gx];
+    }
+    {
+        // all of a thread's loads first (clamped in-plane addresses, zero
+        // outside), so that their latencies overlap
+        constexpr int NS = (SS_HH * SS_HW + SS_THREADS - 1) / SS_THREADS;
+        float g[NS], t0[NS], t1[NS], t2[NS];
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+            const int i = tid + u * SS_THREADS;
+            const int ly = i / SS_HW, lx = i % SS_HW;
+            const int gy = y0 + ly - 5, gx = x0 + lx - 5;
+            const size_t o = plane + (size_t)min(max(gy, 0), a.H - 1) * a.W + min(max(gx, 0), a.W - 1);
+            g[u] = a.dmap ? a.dmap[o] : a.dmap_const;
+            t0[u] = a.dmu1[o];
+            t1[u] = a.ds1[o];
+            t2[u] = a.ds12[o];
         }
-        s[0][ly][lx] = p0; s[1][ly][lx] = p1; s[2][ly][lx] = p2;
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+            const int i = tid + u * SS_THREADS;
+            if (i >= SS_HH * SS_HW) break;
+            const int ly = i / SS_HW, lx = i % SS_HW;
+            const int gy = y0 + ly - 5, gx = x0 + lx - 5;
+            const bool in = gx >= 0 && gx < a.W && gy >= 0 && gy < a.H;
+            s[0][ly][lx] = in ? g[u] * t0[u] : 0.0f;
+            s[1][ly][lx] = in ? g[u] * t1[u] : 0.0f;
+            s[2][ly][lx] = in ? g[u] * t2[u] : 0.0f;
+        }
     }
     __syncthreads();
     for (int i = tid; i < SS_HH * SS_TW; i += SS_THREADS) {
@@ -148,7 +194,9 @@ __global__ void __launch_bounds__(SS_THREADS) k_ssim_bwd(SsimBwdArgs a, Taps tp)
         }
     }
     __syncthreads();
-    for (int i = tid; i < SS_TH * SS_TW; i += SS_THREADS) {
+#pragma unroll
+    for (int u = 0; u < NO; ++u) {
+        const int i = tid + u * SS_THREADS;
         const int ly = i / SS_TW, lx = i % SS_TW;
         const int gy = y0 + ly, gx = x0 + lx;
         if (gy >= a.H || gx >= a.W) continue;
@@ -161,7 +209,7 @@ __global__ void __launch_bounds__(SS_THREADS) k_ssim_bwd(SsimBwdArgs a, Taps tp)
             v[c] = acc;
         }
         const size_t o = plane + (size_t)gy * a.W + gx;
-        const float x = a.img1[o], y = a.img2[o];
+        const float x = xi[u], y = yi[u];
         float g = v[0] + x * 2.0f * v[1] + y * v[2];
         if (a.l1w != 0.0f) {
             // d|x - y|/dx with sign(0) = 0, as torch's abs backward
