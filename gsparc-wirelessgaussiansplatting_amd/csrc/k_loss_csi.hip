@@ -34,9 +34,22 @@ __global__ void __launch_bounds__(CL_THREADS) k_csi_mag(size_t npix, int N, cons
     const float4* a = reinterpret_cast<const float4*>(out + p0 * (2 * GS_CSI_NC));
     const float4* b = reinterpret_cast<const float4*>(tgt + p0 * (2 * GS_CSI_NC));
     float l1 = 0.0f;
-    for (int i = threadIdx.x; i < np * CL_Q; i += CL_THREADS) {
+    // CL_PIX * CL_Q = 1664 words over 256 threads: up to 7 per thread, all
+    // loaded before the first use (clamped to the block's span)
+    constexpr int NI = (CL_PIX * CL_Q + CL_THREADS - 1) / CL_THREADS;
+    float4 xs[NI], ys[NI];
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+        const int i = min((int)threadIdx.x + u * CL_THREADS, np * CL_Q - 1);
+        xs[u] = a[i];
+        ys[u] = b[i];
+    }
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+        const int i = (int)threadIdx.x + u * CL_THREADS;
+        if (i >= np * CL_Q) break;
         const int pl = i / CL_Q, k2 = i % CL_Q;
-        const float4 x = a[i], y = b[i];
+        const float4 x = xs[u], y = ys[u];
         sA[2 * k2][pl] = sqrtf(x.x * x.x + x.y * x.y);
         sA[2 * k2 + 1][pl] = sqrtf(x.z * x.z + x.w * x.w);
         sB[2 * k2][pl] = sqrtf(y.x * y.x + y.y * y.y);
@@ -71,21 +84,44 @@ __global__ void __launch_bounds__(CL_THREADS) k_csi_grad(size_t npix, int N, con
     __shared__ float sD[GS_CSI_NC][CL_PIX + 1];
     const size_t p0 = (size_t)blockIdx.x * CL_PIX;
     const int np = (int)min((size_t)CL_PIX, npix - p0);
-    for (int i = threadIdx.x; i < GS_CSI_NC * CL_PIX; i += CL_THREADS) {
-        const int k = i / CL_PIX, pl = i % CL_PIX;
-        if (pl >= np) continue;
-        const size_t p = p0 + pl, r = p / (size_t)N, q = p % (size_t)N;
-        const size_t o = (r * GS_CSI_NC + k) * (size_t)N + q;
-        sM[k][pl] = magA[o];
-        sD[k][pl] = dmag[o];
+    {
+        constexpr int NP = GS_CSI_NC * CL_PIX / CL_THREADS;
+        static_assert(NP * CL_THREADS == GS_CSI_NC * CL_PIX, "whole planes per thread");
+        float mv[NP], dv[NP];
+#pragma unroll
+        for (int u = 0; u < NP; ++u) {
+            const int i = (int)threadIdx.x + u * CL_THREADS;
+            const int k = i / CL_PIX, pl = min(i % CL_PIX, np - 1);
+            const size_t p = p0 + pl, r = p / (size_t)N, q = p % (size_t)N;
+            const size_t o = (r * GS_CSI_NC + k) * (size_t)N + q;
+            mv[u] = magA[o];
+            dv[u] = dmag[o];
+        }
+#pragma unroll
+        for (int u = 0; u < NP; ++u) {
+            const int i = (int)threadIdx.x + u * CL_THREADS;
+            sM[i / CL_PIX][i % CL_PIX] = mv[u];
+            sD[i / CL_PIX][i % CL_PIX] = dv[u];
+        }
     }
     __syncthreads();
     const float4* a = reinterpret_cast<const float4*>(out + p0 * (2 * GS_CSI_NC));
     const float4* b = reinterpret_cast<const float4*>(tgt + p0 * (2 * GS_CSI_NC));
     float4* g = reinterpret_cast<float4*>(dout + p0 * (2 * GS_CSI_NC));
-    for (int i = threadIdx.x; i < np * CL_Q; i += CL_THREADS) {
+    constexpr int NI = (CL_PIX * CL_Q + CL_THREADS - 1) / CL_THREADS;
+    float4 xs[NI], ys[NI];
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+        const int i = min((int)threadIdx.x + u * CL_THREADS, np * CL_Q - 1);
+        xs[u] = a[i];
+        ys[u] = b[i];
+    }
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+        const int i = (int)threadIdx.x + u * CL_THREADS;
+        if (i >= np * CL_Q) break;
         const int pl = i / CL_Q, k2 = i % CL_Q;
-        const float4 x = a[i], y = b[i];
+        const float4 x = xs[u], y = ys[u];
         const float m0 = sM[2 * k2][pl], m1 = sM[2 * k2 + 1][pl];
         const float f0 = m0 > 0.0f ? sD[2 * k2][pl] / m0 : 0.0f;
         const float f1 = m1 > 0.0f ? sD[2 * k2 + 1][pl] / m1 : 0.0f;

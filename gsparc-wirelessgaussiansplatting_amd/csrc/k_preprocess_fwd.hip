@@ -35,6 +35,12 @@ __global__ void __launch_bounds__(GS_PF_THREADS) k_preprocess_fwd(Dims d, PreFwd
     const int gg0 = (int)sc.g0 + gb;                           // packed index of the first
     const int gg = gg0 + lane;
     const float* tx = a.tx + 3 * grp.scene;
+    // the lane's Gaussian, loaded before the staging so that the loads overlap
+    const int gc = gg0 + min(lane, nG - 1);
+    const float mean[3] = {a.means[3 * gc], a.means[3 * gc + 1], a.means[3 * gc + 2]};
+    const float sv[3] = {a.scales[3 * gc], a.scales[3 * gc + 1], a.scales[3 * gc + 2]};
+    const float q[4] = {a.rots[4 * gc], a.rots[4 * gc + 1], a.rots[4 * gc + 2], a.rots[4 * gc + 3]};
+    const float o = valid ? a.opac[gc] : 0.0f;
 
     // the group's receivers (zero beyond its nr: those slots are never used)
     __shared__ float s_rx[GS_RC * 3];
@@ -51,29 +57,53 @@ __global__ void __launch_bounds__(GS_PF_THREADS) k_preprocess_fwd(Dims d, PreFwd
         const float4* b1v = reinterpret_cast<const float4*>(a.mlp + (size_t)P * 96) + (size_t)gg0 * 8;
         const float4* W2v = reinterpret_cast<const float4*>(a.mlp + (size_t)P * 128) + (size_t)gg0 * 16;
         const float2* b2v = reinterpret_cast<const float2*>(a.mlp + (size_t)P * 192) + gg0;
-        for (int i = tid; i < nG * 24; i += GS_PF_THREADS) s_w[(i / 24) * GS_PF_ROW + i % 24] = W1v[i];
-        for (int i = tid; i < nG * 8; i += GS_PF_THREADS) s_w[(i / 8) * GS_PF_ROW + 24 + i % 8] = b1v[i];
-        for (int i = tid; i < nG * 16; i += GS_PF_THREADS) s_w[(i / 16) * GS_PF_ROW + 32 + i % 16] = W2v[i];
-        for (int i = tid; i < nG; i += GS_PF_THREADS) {
-            const float2 b2 = b2v[i];
-            s_w[i * GS_PF_ROW + 48] = make_float4(b2.x, b2.y, 0.0f, 0.0f);
+        // a thread's loads (3 + 1 + 2 + 1) are all issued before its LDS
+        // stores, so that their latencies overlap.  The stores cover the
+        // whole 64-row table unconditionally: rows past the block's nG
+        // Gaussians get clamped duplicates, which no valid lane reads.
+        constexpr int N1 = GS_PF_G * 24 / GS_PF_THREADS;
+        constexpr int NB = GS_PF_G * 8 / GS_PF_THREADS;
+        constexpr int N2 = GS_PF_G * 16 / GS_PF_THREADS;
+        static_assert(N1 * GS_PF_THREADS == GS_PF_G * 24 && NB * GS_PF_THREADS == GS_PF_G * 8 &&
+                          N2 * GS_PF_THREADS == GS_PF_G * 16 && GS_PF_G <= GS_PF_THREADS,
+                      "the staging covers the table exactly");
+        float4 w1[N1], bb[NB], w2[N2];
+        float2 b2 = make_float2(0.0f, 0.0f);
+#pragma unroll
+        for (int u = 0; u < N1; ++u) w1[u] = W1v[min(tid + u * GS_PF_THREADS, nG * 24 - 1)];
+#pragma unroll
+        for (int u = 0; u < NB; ++u) bb[u] = b1v[min(tid + u * GS_PF_THREADS, nG * 8 - 1)];
+#pragma unroll
+        for (int u = 0; u < N2; ++u) w2[u] = W2v[min(tid + u * GS_PF_THREADS, nG * 16 - 1)];
+        if (tid < GS_PF_G) b2 = b2v[min(tid, nG - 1)];
+#pragma unroll
+        for (int u = 0; u < N1; ++u) {
+            const int i = tid + u * GS_PF_THREADS;
+            s_w[(i / 24) * GS_PF_ROW + i % 24] = w1[u];
         }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const int i = tid + u * GS_PF_THREADS;
+            s_w[(i / 8) * GS_PF_ROW + 24 + i % 8] = bb[u];
+        }
+#pragma unroll
+        for (int u = 0; u < N2; ++u) {
+            const int i = tid + u * GS_PF_THREADS;
+            s_w[(i / 16) * GS_PF_ROW + 32 + i % 16] = w2[u];
+        }
+        if (tid < GS_PF_G) s_w[tid * GS_PF_ROW + 48] = make_float4(b2.x, b2.y, 0.0f, 0.0f);
     }
     __syncthreads();
     const int c0 = wave * GS_PF_RW;
     if (c0 >= grp.nr) return;                                  // no barrier below
     const int c1 = min(grp.nr, c0 + GS_PF_RW);
 
-    float mean[3] = {0, 0, 0}, c3[6] = {0, 0, 0, 0, 0, 0}, o = 0.0f, ld = 0.0f;
+    float c3[6] = {0, 0, 0, 0, 0, 0}, ld = 0.0f;
     if (valid) {
-        mean[0] = a.means[3 * gg]; mean[1] = a.means[3 * gg + 1]; mean[2] = a.means[3 * gg + 2];
-        const float sv[3] = {a.scales[3 * gg], a.scales[3 * gg + 1], a.scales[3 * gg + 2]};
-        const float q[4] = {a.rots[4 * gg], a.rots[4 * gg + 1], a.rots[4 * gg + 2], a.rots[4 * gg + 3]};
         cov3d(sv, a.mod, q, c3);
         if (wave == 0 && grp.r0 == (int)sc.r0)   // the scene's first group, once
 #pragma unroll
             for (int k = 0; k < 6; ++k) a.cov3[6 * gg + k] = c3[k];
-        o = a.opac[gg];
         const float df[3] = {tx[0] - mean[0], tx[1] - mean[1], tx[2] - mean[2]};
         const float dist = sqrtf(df[0] * df[0] + df[1] * df[1] + df[2] * df[2]);
         ld = gs_logf(dist + 1e-6f);
