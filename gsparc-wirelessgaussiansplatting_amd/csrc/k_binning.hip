@@ -61,17 +61,32 @@ __global__ void __launch_bounds__(1024) k_segscan(int n, const uint32_t* segcnt,
     __shared__ uint32_t s_wave[1024 / GS_WAVE];
     const int per = (n + 1023) / 1024;
     const int i0 = min(n, (int)threadIdx.x * per), i1 = min(n, i0 + per);
+    // eight loads at a time (clamped to the thread's range), so that their
+    // latencies overlap: one workgroup does the whole scan
     uint32_t sum = 0;
-    for (int i = i0; i < i1; ++i) sum += segcnt[i];
+    for (int i = i0; i < i1; i += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = segcnt[min(i + u, i1 - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) sum += i + u < i1 ? v[u] : 0u;
+    }
     uint32_t tot;
     uint32_t run = block_exclusive_scan<1024>(sum, s_wave, tot);
-    for (int i = i0; i < i1; ++i) {
-        const uint32_t c = segcnt[i];
-        segbase[i] = run;
-        // empty tiles keep the range (0, 0) as after the reference's memset
-        // (rasterizer_impl.cu:303)
-        ranges[i] = c ? make_uint2(run, run + c) : make_uint2(0u, 0u);
-        run += c;
+    for (int i = i0; i < i1; i += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = segcnt[min(i + u, i1 - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (i + u >= i1) break;
+            const uint32_t c = v[u];
+            segbase[i + u] = run;
+            // empty tiles keep the range (0, 0) as after the reference's memset
+            // (rasterizer_impl.cu:303)
+            ranges[i + u] = c ? make_uint2(run, run + c) : make_uint2(0u, 0u);
+            run += c;
+        }
     }
     if (threadIdx.x == 0) total[0] = tot;
 }
@@ -353,17 +368,20 @@ __global__ void __launch_bounds__(256) k_chunk_scan(Dims d, BinArgs a)
     if (row >= d.R * d.T) return;
     uint32_t* cnt = a.ccnt + (size_t)row * a.nchunks;
     uint32_t carry = 0;
-    for (int c0 = 0; c0 < a.nchunks; c0 += GS_WAVE) {
-        const int c = c0 + lane;
-        const uint32_t v = c < a.nchunks ? cnt[c] : 0u;
-        uint32_t x = v;
+    // four 64-chunk steps loaded at a time, so that their latencies overlap
+    for (int cb = 0; cb < a.nchunks; cb += 4 * GS_WAVE) {
+        uint32_t v[4];
 #pragma unroll
-        for (int off = 1; off < GS_WAVE; off <<= 1) {
-            const uint32_t y = __shfl_up(x, off);
-            if (lane >= off) x += y;
+        for (int u = 0; u < 4; ++u) v[u] = cnt[min(cb + u * GS_WAVE + lane, a.nchunks - 1)];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int c = cb + u * GS_WAVE + lane;
+            if (cb + u * GS_WAVE >= a.nchunks) break;
+            const uint32_t vc = c < a.nchunks ? v[u] : 0u;
+            const uint32_t x = wave_incl_sum(vc);
+            if (c < a.nchunks) cnt[c] = carry + x - vc;
+            carry += __builtin_amdgcn_readlane(x, GS_WAVE - 1);
         }
-        if (c < a.nchunks) cnt[c] = carry + x - v;
-        carry += __shfl(x, GS_WAVE - 1);
     }
     if (lane == 0) a.segcnt[row] = carry;
 }
