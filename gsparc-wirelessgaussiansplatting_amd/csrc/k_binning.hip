@@ -520,13 +520,20 @@ __device__ __forceinline__ int length_bucket(uint32_t L)
 }
 
 // One workgroup per list l: the tiles of receivers l, l + L, l + 2L, ...
-// (L = number of lists) by descending length bucket, into order + l * stride.
-__global__ void __launch_bounds__(1024) k_tile_order(const uint2* ranges, int R, int T, int stride, uint32_t* order)
+// (L = number of lists) by descending length bucket, into order + l * stride;
+// the rest of the list's slots (up to stride, or up to len for a single list)
+// get the empty marker 0xffffffff.  Workgroup 0 also records the mode in
+// total[1].
+__global__ void __launch_bounds__(1024) k_tile_order(const uint2* ranges, int R, int T, int stride, uint32_t* order,
+                                                     uint32_t* total, int mode, int len)
 {
     __shared__ uint32_t cnt[128];
     const int l = blockIdx.x, L = gridDim.x;
     const int n = (R - l + L - 1) / L * T;
     uint32_t* out = order + (size_t)l * stride;
+    if (l == 0 && threadIdx.x == 0) total[1] = (uint32_t)mode;
+    const int pad_end = stride ? stride : len;
+    for (int i = n + (int)threadIdx.x; i < pad_end; i += 1024) out[i] = 0xffffffffu;
     for (int i = threadIdx.x; i < 128; i += 1024) cnt[i] = 0;
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += 1024) {
@@ -551,12 +558,19 @@ __global__ void __launch_bounds__(1024) k_tile_order(const uint2* ranges, int R,
 }
 
 // Mode 3: one workgroup per receiver r sorts its own tiles into slot r / 8
-// of XCD r % 8.
-__global__ void __launch_bounds__(256) k_tile_order_rx(const uint2* ranges, int T, int n8, uint32_t* order)
+// of XCD r % 8; workgroups r >= R (up to a multiple of 8) mark their slot
+// empty.  Workgroup 0 also records the mode in total[1].
+__global__ void __launch_bounds__(256) k_tile_order_rx(const uint2* ranges, int R, int T, int n8, uint32_t* order,
+                                                       uint32_t* total, int mode)
 {
     __shared__ uint32_t cnt[128];
     const int r = blockIdx.x;
     uint32_t* out = order + (size_t)(r % 8) * n8 + (size_t)(r / 8) * T;
+    if (r == 0 && threadIdx.x == 0) total[1] = (uint32_t)mode;
+    if (r >= R) {
+        for (int t = threadIdx.x; t < T; t += 256) out[t] = 0xffffffffu;
+        return;
+    }
     for (int i = threadIdx.x; i < 128; i += 256) cnt[i] = 0;
     __syncthreads();
     for (int t = threadIdx.x; t < T; t += 256) {
@@ -668,16 +682,17 @@ hipError_t launch_tile_order(const Dims& d, int mode, const uint2* ranges, uint3
                              hipStream_t s)
 {
     if (d.R == 0 || d.T == 0) return hipSuccess;
-    hipError_t e = launch_fill_u32(total + 1, (uint32_t)mode, 1, s);
-    if (e == hipSuccess) e = launch_fill_u32(order, 0xffffffffu, order_length(d), s);
-    if (e != hipSuccess) return e;
+    // every slot of order_length(d) is written: an entry or the empty marker
     const int n8 = (d.R + 7) / 8 * d.T;
     if (mode == ORDER_XCD_RECEIVER)
-        hipLaunchKernelGGL(k_tile_order_rx, dim3(d.R), dim3(256), 0, s, ranges, d.T, n8, order);
+        hipLaunchKernelGGL(k_tile_order_rx, dim3((unsigned)((d.R + 7) / 8 * 8)), dim3(256), 0, s, ranges, d.R, d.T, n8,
+                           order, total, mode);
     else if (mode == ORDER_XCD_MIXED)
-        hipLaunchKernelGGL(k_tile_order, dim3(8), dim3(1024), 0, s, ranges, d.R, d.T, n8, order);
+        hipLaunchKernelGGL(k_tile_order, dim3(8), dim3(1024), 0, s, ranges, d.R, d.T, n8, order, total, mode,
+                           (int)order_length(d));
     else
-        hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, s, ranges, d.R, d.T, 0, order);
+        hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, s, ranges, d.R, d.T, 0, order, total, mode,
+                           (int)order_length(d));
     return hipGetLastError();
 }
 
