@@ -289,6 +289,16 @@ def setup_scenes(args, dev):
             out = wr.forward_scenes(m, w, o, s_, q, H, W, 1.0, views, txp, rxp, batch)
         return wr.autotune(out, grad_mode=args.grad_mode, scenes=batch)
 
+    def graphed():
+        """The same scene step captured in a HIP graph; tile lists sized for
+        1.25 K."""
+        import gsparc_train as T
+        cap = int(count() * 1.25) + 1024
+        live = [x.clone() for x in packed]
+        g = T.GraphedSceneStep(*live, views, txp, rxp, target, H, W, batch, capacity=cap,
+                               grad_mode=args.grad_mode)
+        return g, live
+
     def cpu():
         # the smallest scene's receivers, in order, up to the sample size
         order = np.argsort([sc.P for sc in scs])
@@ -304,7 +314,7 @@ def setup_scenes(args, dev):
                                                    f"(P = {sorted(sc.P for sc in scs)[:3]}...)")
 
     return Workload(
-        step=step, step_plain=step_plain, single=single, count=count, cpu=cpu, graphed=None, tune=tune,
+        step=step, step_plain=step_plain, single=single, count=count, cpu=cpu, graphed=graphed, tune=tune,
         spectra=batch.R,
         Ptot=batch.Ptot,
         Nrec=batch.Nrec, H=H, W=W, traffic_key=None,

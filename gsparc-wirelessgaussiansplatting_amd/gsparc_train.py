@@ -108,7 +108,51 @@ def _require_param(t, name):
     return t
 
 
-class GraphedTrainingStep:
+class _GraphedStep:
+    """Capture of a step's launch sequence into a library-owned HIP graph
+    (gsparc_graph_begin / _end) and its replay.  Subclasses provide
+    _record(stream) and instance_count()."""
+
+    def _capture(self, dev):
+        L = gh.lib()
+        self.graph = C.c_void_p(None)
+        self.stream = torch.cuda.Stream(device=dev)
+        cur = torch.cuda.current_stream(dev)
+        self.stream.wait_stream(cur)
+        with torch.cuda.stream(self.stream):
+            h = self.stream.cuda_stream
+            self._record(h)                       # one eager run: checks every argument
+            gh.check(L.gsparc_graph_begin(h), "graph begin")
+            try:
+                self._record(h)
+            finally:
+                gh.check(L.gsparc_graph_end(h, C.byref(self.graph)), "graph end")
+        cur.wait_stream(self.stream)
+
+    def replay(self):
+        """Launch the captured step on the current stream."""
+        gh.check(gh.lib().gsparc_graph_launch(self.graph, gh.stream_handle(self.target.device)), "graph launch")
+
+    def run_eager(self):
+        """The same launch sequence without the graph, on the current stream."""
+        self._record(gh.stream_handle(self.target.device))
+
+    def overflowed(self) -> bool:
+        return self.instance_count() > self.capacity
+
+    def close(self):
+        if getattr(self, "graph", None):
+            gh.lib().gsparc_graph_destroy(self.graph)
+            self.graph = C.c_void_p(None)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GraphedTrainingStep(_GraphedStep):
     """Forward, loss and backward of one batched training step, captured once
     and replayed.
 
@@ -164,19 +208,7 @@ class GraphedTrainingStep:
         self.grads = dict(means=torch.empty((P, 3), **f32), emission_mlps=torch.empty(P * 194, **f32),
                           opacities=torch.empty((P, 1), **f32), scales=torch.empty((P, 3), **f32),
                           rotations=torch.empty((P, 4), **f32))
-        self.graph = C.c_void_p(None)
-        self.stream = torch.cuda.Stream(device=dev)
-        cur = torch.cuda.current_stream(dev)
-        self.stream.wait_stream(cur)
-        with torch.cuda.stream(self.stream):
-            h = self.stream.cuda_stream
-            self._record(h)                       # one eager run: checks every argument
-            gh.check(L.gsparc_graph_begin(h), "graph begin")
-            try:
-                self._record(h)
-            finally:
-                gh.check(L.gsparc_graph_end(h, C.byref(self.graph)), "graph end")
-        cur.wait_stream(self.stream)
+        self._capture(dev)
 
     def _record(self, stream):
         L = gh.lib()
@@ -203,28 +235,91 @@ class GraphedTrainingStep:
                                              gh.ptr(g["scales"]), gh.ptr(g["rotations"]), None, stream),
                  "graphed step: backward")
 
-    def replay(self):
-        """Launch the captured step on the current stream."""
-        gh.check(gh.lib().gsparc_graph_launch(self.graph, gh.stream_handle(self.target.device)), "graph launch")
-
-    def run_eager(self):
-        """The same launch sequence without the graph, on the current stream."""
-        self._record(gh.stream_handle(self.target.device))
-
     def instance_count(self) -> int:
         """K of the last step (synchronises the current stream)."""
         return wr.instance_count(self.P, self.R, self.H, self.W, self.img)
 
-    def overflowed(self) -> bool:
-        return self.instance_count() > self.capacity
 
-    def close(self):
-        if self.graph:
-            gh.lib().gsparc_graph_destroy(self.graph)
-            self.graph = C.c_void_p(None)
+class GraphedSceneStep(_GraphedStep):
+    """GraphedTrainingStep for a batch of uneven scenes (wireless_rasterizer.
+    SceneBatch): forward of every scene's receivers, the fused loss over all
+    R maps, backward, captured once and replayed.  Parameters are the packed
+    [Ptot,...] tensors (emission_mlps packed by wireless_rasterizer.pack_mlps),
+    viewmatrices [R,4,4] and rx_pos [R,3] packed by scene, tx_pos [S,3],
+    target [R,H,W]; they are read in place at every replay.  No counterpart in
+    the reference."""
 
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+    def __init__(self, means, scales, rotations, opacities, emission_mlps, viewmatrices, tx_pos, rx_pos, target,
+                 image_height, image_width, scenes, capacity, scale_modifier=1.0, grad_mode="reference",
+                 lambda_ssim=LAMBDA_SSIM):
+        self.params = dict(means=_require_param(means, "means"), scales=_require_param(scales, "scales"),
+                           rotations=_require_param(rotations, "rotations"),
+                           opacities=_require_param(opacities, "opacities"),
+                           emission_mlps=_require_param(emission_mlps, "emission_mlps"))
+        if self.params["emission_mlps"].data_ptr() % 16:
+            raise RuntimeError("emission_mlps must be 16-byte aligned")
+        dev = means.device
+        self.scenes = scenes
+        P, R = scenes.Ptot, scenes.R
+        if int(means.size(0)) != P or self.params["emission_mlps"].numel() != P * 194:
+            raise RuntimeError("means and emission_mlps must hold every scene's Gaussians")
+        self.views = gh.f32(viewmatrices).reshape(-1, 16)
+        self.rx = gh.f32(rx_pos).reshape(-1, 3)
+        self.tx = gh.f32(tx_pos).reshape(-1, 3)
+        if self.views.size(0) != R or self.rx.size(0) != R or self.tx.size(0) != scenes.S:
+            raise RuntimeError("one view matrix and position per receiver, one TX position per scene")
+        H, W = int(image_height), int(image_width)
+        self.target = _require_param(target, "target")
+        if tuple(self.target.shape) != (R, H, W):
+            raise RuntimeError("target must be [R,H,W]")
+        self.P, self.R, self.H, self.W = P, R, H, W
+        self.capacity = int(capacity)
+        self.mode = gh.GRAD_MODES[grad_mode] if isinstance(grad_mode, str) else int(grad_mode)
+        self.scale_modifier = float(scale_modifier)
+        self.lambda_ssim = float(lambda_ssim)
+        f32 = dict(dtype=torch.float32, device=dev)
+        u8 = lambda n: torch.empty(int(n), dtype=torch.uint8, device=dev)
+        geom_b, img_b, bin_b, grad_b = scenes.sizes(H, W, self.capacity)
+        self.geom, self.img, self.bin, self.gradbuf = u8(geom_b), u8(img_b), u8(bin_b), u8(grad_b)
+        self.work = _work(R, H, W, dev)
+        self.radii = torch.empty(scenes.Nrec, dtype=torch.int32, device=dev)
+        self.out_real = torch.empty((R, H, W), **f32)
+        self.out_imag = torch.empty((R, H, W), **f32)
+        self.loss = torch.empty(1, **f32)
+        self.dimg = torch.empty((R, H, W), **f32)
+        self.zero_imag = torch.zeros((R, H, W), **f32)
+        self.grads = dict(means=torch.empty((P, 3), **f32), emission_mlps=torch.empty(P * 194, **f32),
+                          opacities=torch.empty((P, 1), **f32), scales=torch.empty((P, 3), **f32),
+                          rotations=torch.empty((P, 4), **f32))
+        self._capture(dev)
+
+    def _record(self, stream):
+        L = gh.lib()
+        p, g = self.params, self.grads
+        bp, tab = C.byref(self.scenes.batch), gh.ptr(self.scenes.table)
+        R, W, H = self.R, self.W, self.H
+        gh.check(L.gsparc_rasterize_forward_scenes(bp, tab, W, H, gh.ptr(p["means"]), gh.ptr(p["scales"]),
+                                                  gh.ptr(p["rotations"]), gh.ptr(p["opacities"]),
+                                                  gh.ptr(p["emission_mlps"]), None, gh.ptr(self.views),
+                                                  gh.ptr(self.tx), gh.ptr(self.rx), self.scale_modifier,
+                                                  gh.ptr(self.geom), gh.ptr(self.img), gh.ptr(self.bin),
+                                                  self.capacity, gh.ptr(self.radii), gh.ptr(self.out_real),
+                                                  gh.ptr(self.out_imag), 0, stream), "graphed scene step: forward")
+        gh.check(L.gsparc_loss_l1_ssim(R, H, W, self.lambda_ssim, SSIM_C1, SSIM_C2, gh.ptr(self.out_real),
+                                       gh.ptr(self.target), gh.ptr(self.work), gh.ptr(self.loss), gh.ptr(self.dimg),
+                                       stream), "graphed scene step: loss")
+        gh.check(L.gsparc_rasterize_backward_scenes(
+            self.mode, bp, tab, W, H, gh.ptr(p["means"]), gh.ptr(p["scales"]), gh.ptr(p["rotations"]),
+            gh.ptr(p["emission_mlps"]), None, gh.ptr(self.views), gh.ptr(self.tx), gh.ptr(self.rx),
+            self.scale_modifier, gh.ptr(self.radii), gh.ptr(self.geom), gh.ptr(self.img), gh.ptr(self.bin),
+            self.capacity, gh.ptr(self.gradbuf), gh.ptr(self.dimg), gh.ptr(self.zero_imag), None,
+            gh.ptr(g["means"]), gh.ptr(g["emission_mlps"]), gh.ptr(g["opacities"]), None, gh.ptr(g["scales"]),
+            gh.ptr(g["rotations"]), None, stream), "graphed scene step: backward")
+
+    def instance_count(self) -> int:
+        """K of the last step (synchronises the current stream)."""
+        K = C.c_uint64(0)
+        gh.check(gh.lib().gsparc_instance_count_scenes(C.byref(self.scenes.batch), gh.ptr(self.scenes.table), self.W,
+                                                       self.H, gh.ptr(self.img), C.byref(K),
+                                                       gh.stream_handle(self.target.device)), "instance count")
+        return int(K.value)

@@ -190,3 +190,61 @@ def test_scene_table_rejects_bad_input():
         wr.SceneBatch([10, -1], [1, 1], "cuda")
     with pytest.raises(RuntimeError):
         wr.SceneBatch([], [], "cuda")
+
+
+def test_graphed_scene_step_matches_autograd():
+    """The captured scene step (forward, fused L1+SSIM loss, backward) equals
+    the torch-autograd scene step on the same parameters, replay after replay,
+    with parameters changed in place and eager work between replays."""
+    import cuda_rasterizer as cr
+    import gsparc_train as T
+    from fused_ssim import fused_ssim
+    scs = _scenes()
+    p, _ = _packed(scs)
+    batch = wr.SceneBatch([sc.P for sc in scs], [sc.R for sc in scs], "cuda")
+    H, W = scs[0].H, scs[0].W
+    g = torch.Generator().manual_seed(17)
+    target = (torch.rand((batch.R, H, W), generator=g) * 0.3).cuda()
+    K = _forward_scenes(scs, p, batch)[0]
+    names = ("means", "scales", "rots", "opac", "mlp")
+    live = {k: p[k].clone() for k in names}
+    print("capturing graphed scene step, K", K, flush=True)
+    step = T.GraphedSceneStep(live["means"], live["scales"], live["rots"], live["opac"], live["mlp"], p["views"],
+                              p["tx"], p["rx"], target, H, W, batch, capacity=2 * K)
+    st = cr.SceneSetting(H, W, 1.0, batch, p["views"], p["tx"], p["rx"], "reference")
+
+    def reference(vals):
+        prm = [v.clone().requires_grad_(True) for v in vals]
+        m, s_, q, o, w = prm
+        ore, _, _ = cr.rasterize_scenes(m, w, o, s_, q, st)
+        loss = 0.8 * (ore - target).abs().mean() + 0.2 * (1.0 - fused_ssim(ore.unsqueeze(1), target.unsqueeze(1)))
+        loss.backward()
+        torch.cuda.synchronize()
+        return float(loss), ore.detach(), [x.grad for x in prm]
+
+    rng = np.random.default_rng(3)
+    sets = []
+    for i in range(3):
+        noise = torch.from_numpy(rng.normal(scale=2e-3, size=(batch.Ptot, 3))).float().cuda()
+        sets.append([p["means"] + noise, p["scales"], p["rots"], p["opac"] * (1.0 - 0.05 * i), p["mlp"]])
+    refs = [reference(v) for v in sets]
+    gname = dict(means="means", scales="scales", rots="rotations", opac="opacities", mlp="emission_mlps")
+    for rep in range(2):
+        for i, vals in enumerate(sets):
+            with torch.no_grad():
+                for k, v in zip(names, vals):
+                    live[k].copy_(v)
+            print("replay", rep, i, flush=True)
+            step.replay()
+            torch.cuda.synchronize()
+            assert not step.overflowed()
+            ref_loss, ref_out, ref_grads = refs[i]
+            assert torch.equal(step.out_real, ref_out), (rep, i)
+            assert abs(float(step.loss) - ref_loss) <= 1e-5 * abs(ref_loss), (rep, i)
+            for k, gref in zip(names, ref_grads):
+                assert _rel(step.grads[gname[k]].reshape(gref.shape), gref) < GRAD_TOL, (rep, i, k)
+            _forward_scenes(scs, p, batch)            # unrelated eager work between replays
+    step.run_eager()
+    torch.cuda.synchronize()
+    assert torch.equal(step.out_real, refs[2][1])
+    step.close()
