@@ -47,6 +47,9 @@ struct Dims {
 // 1080x1920 has 8160), in the global count matrix beyond
 #define GS_CHUNK_MIN 256
 #define GS_CHUNK_LDS_TILES 8192
+#define GS_SCATTER_WAVES 8        // independent waves (chunks) per ordered-scatter workgroup
+#define GS_ROW_PLACE_MAX_T 1024   // counts and placement per (chunk, tile row) up to this many tiles
+#define GS_RP_WAVES 4              // waves per k_row_place workgroup
 
 inline Dims make_dims(int P, int R, int W, int H)
 {
@@ -159,6 +162,11 @@ hipError_t launch_chunk_count(const Dims& d, const BinArgs& a, hipStream_t s);
 hipError_t launch_chunk_scan(const Dims& d, const BinArgs& a, hipStream_t s);
 hipError_t launch_segscan(const Dims& d, const BinArgs& a, hipStream_t s);
 hipError_t launch_chunk_scatter(const Dims& d, const BinArgs& a, hipStream_t s);
+// Placement of the instances after launch_chunk_count and launch_chunk_scan:
+// per (chunk, tile row) waves when row_place_ok(d) (packed rects and at most
+// GS_ROW_PLACE_MAX_T tiles), else launch_chunk_scatter.
+bool row_place_ok(const Dims& d);
+hipError_t launch_bin_place(const Dims& d, const BinArgs& a, hipStream_t s);
 // Work order of the blend kernels over the (receiver, tile) pairs
 // (item = receiver * T + tile; 0xffffffff = none), longest lists first:
 //   ORDER_GLOBAL        one list; consecutive items go to consecutive XCDs

@@ -403,7 +403,7 @@ int forward_render_impl(const gs::Dims& d, const float* emission, void* geom, vo
     ba.plist = (uint32_t*)(bb + L.plist);
     ba.keys_sorted = (flags & GSPARC_FLAG_KEEP_KEYS) ? (uint32_t*)(bb + L.keys_sorted) : nullptr;
     ba.Kcap = K_capacity;
-    if (d.P > 0) GS_LAUNCH("bin_scatter", gs::launch_chunk_scatter(d, ba, s), s);
+    if (d.P > 0) GS_LAUNCH("bin_scatter", gs::launch_bin_place(d, ba, s), s);
     gs::RenderFwdArgs ra;
     ra.ranges = ba.ranges;
     ra.order = (const uint32_t*)(ib + L.tile_order);
@@ -519,7 +519,7 @@ int forward_render_csi_impl(const gs::Dims& d, const GsCsi& csi, const float* em
     ba.plist = (uint32_t*)(bb + L.plist);
     ba.keys_sorted = nullptr;
     ba.Kcap = K_capacity;
-    if (d.P > 0) GS_LAUNCH("bin_scatter", gs::launch_chunk_scatter(d, ba, s), s);
+    if (d.P > 0) GS_LAUNCH("bin_scatter", gs::launch_bin_place(d, ba, s), s);
     gs::RenderCsiArgs ca;
     ca.ranges = ba.ranges;
     ca.order = (const uint32_t*)(ib + L.tile_order);
@@ -618,7 +618,7 @@ int autotune_impl(int grad_mode, const gs::Dims& d, const float* emission, const
     TuneVal best{1, 1, 1};
     float best_f = 0.0f, best_b = 0.0f, best_total = 1e30f;
     hipError_t err = hipSuccess;
-    auto scatter = [&] { return gs::launch_chunk_scatter(d, sa, s); };
+    auto scatter = [&] { return gs::launch_bin_place(d, sa, s); };
     for (int mode = 1; mode <= 3 && err == hipSuccess; ++mode) {
         err = gs::launch_tile_order(d, mode, sa.ranges, order, sa.total, s);
         float bf = 1e30f, bbk = 1e30f;

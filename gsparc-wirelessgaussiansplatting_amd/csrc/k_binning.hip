@@ -322,10 +322,26 @@ __device__ __forceinline__ void chunk_rect(const Dims& d, const BinArgs& a, size
     }
 }
 
-// per-tile counters of a chunk in LDS (else in the global count matrix)
+// per-tile counters of a chunk in LDS (else in the global count matrix); the
+// bound keeps the GS_SCATTER_WAVES slices of a scatter workgroup within LDS
 __host__ __device__ inline bool chunk_lds(const Dims& d)
 {
-    return d.T <= GS_CHUNK_LDS_TILES && (size_t)(d.gx + d.gy) * 8 + (size_t)d.T * 4 <= 65536;
+    return d.T <= GS_CHUNK_LDS_TILES && (size_t)(d.gx + d.gy) * 8 + (size_t)d.T * 4 <= 122880 / GS_SCATTER_WAVES;
+}
+
+// u64 words of one wave's dynamic LDS in k_chunk_scatter
+__host__ __device__ inline size_t chunk_scatter_slice(const Dims& d)
+{
+    return (size_t)(d.gx + d.gy) + (chunk_lds(d) ? ((size_t)d.T + 1) / 2 : 0);
+}
+
+// Orders the LDS accesses of one wave (its lanes exchange data through LDS
+// and no other wave touches its slice).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // instances per (chunk, tile) -> ccnt[(r*T + t) * nchunks + c]
@@ -392,27 +408,38 @@ __global__ void __launch_bounds__(256) k_chunk_scan(Dims d, BinArgs a)
 // Gaussian does not serialise its batch.  An instance of lane i in tile t goes
 // to base(t) + (lanes below i covering t); the lanes covering t are the AND of
 // the column and row ballot masks.
-// Dynamic LDS: column masks [gx], row masks [gy] (u64), then (when the tiles
-// fit) the running base of every tile [T] (u32).
-__global__ void __launch_bounds__(GS_WAVE) k_chunk_scatter(Dims d, BinArgs a)
+// A workgroup holds GS_SCATTER_WAVES independent waves (chunks
+// blockIdx.x * GS_SCATTER_WAVES + wave), each with its own LDS slice and no
+// workgroup barrier, so that more waves are resident per CU.
+// Dynamic LDS per wave: column masks [gx], row masks [gy] (u64), then (when
+// the tiles fit) the running base of every tile [T] (u32).
+__global__ void __launch_bounds__(GS_WAVE * GS_SCATTER_WAVES) k_chunk_scatter(Dims d, BinArgs a)
 {
-    extern __shared__ uint64_t s_mask[];
-    __shared__ uint32_t s_end[GS_WAVE];    // inclusive prefix of the pair counts
-    __shared__ int4 s_own[GS_WAVE];        // x0, y0, width, Gaussian index
-    __shared__ uint32_t s_key[GS_WAVE];
-    __shared__ int s_mark[GS_WAVE];        // lane whose pairs start at this window position
-    __shared__ float s_inv[GS_WAVE];       // 1 / rect width
+    extern __shared__ uint64_t s_dyn[];
+    __shared__ uint32_t s_end_w[GS_SCATTER_WAVES][GS_WAVE];   // inclusive prefix of the pair counts
+    __shared__ int4 s_own_w[GS_SCATTER_WAVES][GS_WAVE];       // x0, y0, width, Gaussian index
+    __shared__ uint32_t s_key_w[GS_SCATTER_WAVES][GS_WAVE];
+    __shared__ int s_mark_w[GS_SCATTER_WAVES][GS_WAVE];       // lane whose pairs start at this window position
+    __shared__ float s_inv_w[GS_SCATTER_WAVES][GS_WAVE];      // 1 / rect width
+    const int wave = (int)threadIdx.x / GS_WAVE;
+    uint32_t* s_end = s_end_w[wave];
+    int4* s_own = s_own_w[wave];
+    uint32_t* s_key = s_key_w[wave];
+    int* s_mark = s_mark_w[wave];
+    float* s_inv = s_inv_w[wave];
+    uint64_t* s_mask = s_dyn + (size_t)wave * chunk_scatter_slice(d);
     uint64_t* s_mx = s_mask;
     uint64_t* s_my = s_mask + d.gx;
     uint32_t* s_base = (uint32_t*)(s_mask + d.gx + d.gy);
     if ((uint64_t)a.total[0] > a.Kcap) return;   // capacity overflow: reported by the host
-    const int c = blockIdx.x, r = blockIdx.y;
+    const int c = (int)blockIdx.x * GS_SCATTER_WAVES + wave, r = blockIdx.y;
+    if (c >= a.nchunks) return;
     const int n = (int)a.nvis[r];
     const int i0 = c * a.chunk_len;
     if (i0 >= n) return;
     const int i1 = min(n, i0 + a.chunk_len);
     const bool lds = chunk_lds(d);
-    const int lane = threadIdx.x;
+    const int lane = (int)threadIdx.x % GS_WAVE;
     const size_t rP = rx_map(d, r).roff;
     const size_t rT = (size_t)r * d.T;
     uint32_t* slot = a.ccnt + rT * a.nchunks + c;
@@ -446,7 +473,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_chunk_scatter(Dims d, BinArgs a)
         uy0 = __builtin_amdgcn_readfirstlane(uy0);
         ux1 = __builtin_amdgcn_readfirstlane(ux1);
         uy1 = __builtin_amdgcn_readfirstlane(uy1);
-        __syncthreads();   // the previous batch is done with the LDS tables
+        wave_sync();   // the previous batch is done with the LDS tables
         // lanes (Gaussians of this batch) covering each column / row
         for (int tx = ux0; tx < ux1; ++tx) {
             const uint64_t m = ballot(ne && x0 <= tx && tx < x1);
@@ -460,7 +487,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_chunk_scatter(Dims d, BinArgs a)
         s_own[lane] = make_int4(x0, y0, x1 - x0, (int)g);
         s_inv[lane] = ne ? 1.0f / (float)(x1 - x0) : 0.0f;
         s_key[lane] = dbits;
-        __syncthreads();
+        wave_sync();
         // place the instances, 64 pairs per window: pair p of the batch
         // belongs to the last lane with a nonempty rect starting at or before
         // p (start markers in LDS, a wave max-scan, the owner carried across
@@ -493,7 +520,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_chunk_scatter(Dims d, BinArgs a)
             a.plist[pos] = (uint32_t)ow.w;
             if (a.keys_sorted) a.keys_sorted[pos] = s_key[lo];
         }
-        __syncthreads();
+        wave_sync();
         // advance the base of every tile of the batch's union rect by the
         // number of the batch's Gaussians covering it (one lane per tile)
         // (an empty batch has ux0 = uy0 = INT_MAX: no tiles)
@@ -506,7 +533,78 @@ __global__ void __launch_bounds__(GS_WAVE) k_chunk_scatter(Dims d, BinArgs a)
             if (lds) s_base[t] += cnt;
             else slot[(size_t)t * a.nchunks] += cnt;
         }
-        __syncthreads();
+        wave_sync();
+    }
+}
+
+// ---------------------------------------------------------------- row placement
+// For maps of up to GS_ROW_PLACE_MAX_T tiles the placement runs one wave per
+// (chunk, tile row): the wave holds the chunk's depth-ordered packed rects in
+// registers (NI per lane) and walks the row's tiles; the lanes covering tile
+// (tx, ty) (x0 <= tx < x1, y0 <= ty < y1) are one ballot per 64-Gaussian
+// group, their ranks popcounts of it, and each covering Gaussian goes to the
+// tile's base for the chunk + its rank: the lists of k_chunk_scatter, with
+// each store writing consecutive positions.  Tiles outside the union of the
+// row's rects in the chunk are skipped.
+template <int NI, bool KEYS>
+__global__ void __launch_bounds__(GS_WAVE * GS_RP_WAVES) k_row_place(Dims d, BinArgs a)
+{
+    if ((uint64_t)a.total[0] > a.Kcap) return;   // capacity overflow: reported by the host
+    const int r = blockIdx.y;
+    const int idx = (int)blockIdx.x * GS_RP_WAVES + (int)threadIdx.x / GS_WAVE;
+    const int c = idx / d.gy, ty = idx % d.gy;
+    if (c >= a.nchunks) return;
+    const int lane = (int)threadIdx.x % GS_WAVE;
+    const int n = (int)a.nvis[r];
+    const int i0 = c * a.chunk_len;
+    const int m = min(n - i0, NI * GS_WAVE);   // Gaussians of this chunk
+    if (m <= 0) return;
+    const size_t rP = rx_map(d, r).roff;
+    const size_t rowT = (size_t)r * d.T + (size_t)ty * d.gx;
+    // every load first (positions past the end read the last Gaussian and
+    // are masked below), so that their latencies overlap
+    uint32_t rc[NI], gv[NI], kv[NI];
+#pragma unroll
+    for (int k = 0; k < NI; ++k) {
+        const size_t i = rP + (size_t)(i0 + min(k * GS_WAVE + lane, m - 1));
+        rc[k] = a.srect[i];
+        gv[k] = a.sval[1][i];
+        kv[k] = KEYS ? a.skey[1][i] : 0u;
+    }
+    int x0[NI], x1[NI];
+    int ux0 = INT_MAX, ux1 = 0;
+#pragma unroll
+    for (int k = 0; k < NI; ++k) {
+        const uint32_t p = rc[k];
+        const int y0 = (int)((p >> 8) & 0xffu), y1 = (int)(p >> 24);
+        const bool in = k * GS_WAVE + lane < m && y0 <= ty && ty < y1;
+        x0[k] = in ? (int)(p & 0xffu) : 0;
+        x1[k] = in ? (int)((p >> 16) & 0xffu) : 0;   // an empty column range when not in the row
+        if (in && x0[k] < x1[k]) {
+            ux0 = min(ux0, x0[k]);
+            ux1 = max(ux1, x1[k]);
+        }
+    }
+    ux0 = -__builtin_amdgcn_readlane(wave_incl_max(-ux0), GS_WAVE - 1);
+    ux1 = __builtin_amdgcn_readlane(wave_incl_max(ux1), GS_WAVE - 1);
+    for (int tb = ux0; tb < ux1; tb += GS_WAVE) {
+        const int tl = tb + lane;
+        const uint32_t bl = tl < ux1 ? a.segbase[rowT + tl] + a.ccnt[(rowT + tl) * a.nchunks + c] : 0u;
+        const int te = min(tb + GS_WAVE, ux1);
+        for (int tx = tb; tx < te; ++tx) {
+            uint32_t base = (uint32_t)__builtin_amdgcn_readlane((int)bl, tx - tb);
+#pragma unroll
+            for (int k = 0; k < NI; ++k) {
+                const bool cv = x0[k] <= tx && tx < x1[k];
+                const uint64_t mk = ballot(cv);
+                if (cv) {
+                    const uint32_t pos = base + (uint32_t)__popcll(mk & lanemask_lt());
+                    a.plist[pos] = gv[k];
+                    if (KEYS) a.keys_sorted[pos] = kv[k];
+                }
+                base += (uint32_t)__popcll(mk);
+            }
+        }
     }
 }
 
@@ -619,10 +717,7 @@ int chunk_len_for(const Dims& d)
 
 int chunk_count_for(const Dims& d) { return max(1, (d.P + chunk_len_for(d) - 1) / chunk_len_for(d)); }
 
-size_t chunk_scatter_lds(const Dims& d)
-{
-    return (size_t)(d.gx + d.gy) * sizeof(uint64_t) + (chunk_lds(d) ? (size_t)d.T * 4 : 0);
-}
+size_t chunk_scatter_lds(const Dims& d) { return GS_SCATTER_WAVES * chunk_scatter_slice(d) * sizeof(uint64_t); }
 
 hipError_t launch_depth_sort(const Dims& d, const BinArgs& a, hipStream_t s)
 {
@@ -665,10 +760,38 @@ hipError_t launch_segscan(const Dims& d, const BinArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
+static_assert(GS_CHUNK_MIN == 256 && GS_ROW_PLACE_MAX_T <= 1024, "row placement holds 4 to 16 groups of 64");
+bool row_place_ok(const Dims& d) { return rect_packed(d) && d.T <= GS_ROW_PLACE_MAX_T; }
+
+template <int NI>
+static void launch_row_place_ni(const Dims& d, const BinArgs& a, hipStream_t s)
+{
+    const dim3 grid((unsigned)((a.nchunks * d.gy + GS_RP_WAVES - 1) / GS_RP_WAVES), (unsigned)d.R);
+    const dim3 block(GS_WAVE * GS_RP_WAVES);
+    if (a.keys_sorted) hipLaunchKernelGGL((k_row_place<NI, true>), grid, block, 0, s, d, a);
+    else hipLaunchKernelGGL((k_row_place<NI, false>), grid, block, 0, s, d, a);
+}
+
+hipError_t launch_bin_place(const Dims& d, const BinArgs& a, hipStream_t s)
+{
+    if (d.R == 0 || d.P == 0) return hipSuccess;
+    if (!row_place_ok(d)) return launch_chunk_scatter(d, a, s);
+    // chunk_len = 256 * ceil(T / 256) <= GS_ROW_PLACE_MAX_T: 4, 8, 12 or 16 groups of 64
+    switch (a.chunk_len / GS_WAVE) {
+    case 4: launch_row_place_ni<4>(d, a, s); break;
+    case 8: launch_row_place_ni<8>(d, a, s); break;
+    case 12: launch_row_place_ni<12>(d, a, s); break;
+    default: launch_row_place_ni<16>(d, a, s); break;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_chunk_scatter(const Dims& d, const BinArgs& a, hipStream_t s)
 {
     if (d.R == 0 || d.P == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_chunk_scatter, dim3(a.nchunks, d.R), dim3(GS_WAVE), chunk_scatter_lds(d), s, d, a);
+    const unsigned nb = (unsigned)((a.nchunks + GS_SCATTER_WAVES - 1) / GS_SCATTER_WAVES);
+    hipLaunchKernelGGL(k_chunk_scatter, dim3(nb, d.R), dim3(GS_WAVE * GS_SCATTER_WAVES), chunk_scatter_lds(d), s,
+                       d, a);
     return hipGetLastError();
 }
 

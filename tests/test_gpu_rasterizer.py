@@ -115,6 +115,18 @@ def test_forward_long_segments():
         _check_forward(sc, st, r)
 
 
+@pytest.mark.parametrize("H,W", [(512, 512), (528, 600)], ids=["1024_tiles_tile_scan", "1254_tiles_chunked"])
+def test_forward_placement_paths(H, W):
+    """Maps of up to 1024 tiles place the instances by per-tile scans, larger
+    ones by the chunked count and scatter: both give the reference's lists."""
+    sc = S.small_scene(31, 1500, R=2, H=H, W=W, walls=True)
+    print("placement", H, W, flush=True)
+    _, out = gpu_forward(sc)
+    st = read_state(sc, out)
+    for r in range(sc.R):
+        _check_forward(sc, st, r)
+
+
 def test_equal_depth_ties_in_index_order():
     """Gaussians sharing a centre have identical depth keys; the reference's
     stable sort keeps them in index order (rasterizer_impl.cu:295-300)."""
