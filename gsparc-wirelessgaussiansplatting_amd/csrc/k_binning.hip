@@ -539,19 +539,23 @@ __global__ void __launch_bounds__(GS_WAVE * GS_SCATTER_WAVES) k_chunk_scatter(Di
 
 // ---------------------------------------------------------------- row placement
 // For maps of up to GS_ROW_PLACE_MAX_T tiles the placement runs one wave per
-// (chunk, tile row): the wave holds the chunk's depth-ordered packed rects in
-// registers (NI per lane) and walks the row's tiles; the lanes covering tile
-// (tx, ty) (x0 <= tx < x1, y0 <= ty < y1) are one ballot per 64-Gaussian
-// group, their ranks popcounts of it, and each covering Gaussian goes to the
-// tile's base for the chunk + its rank: the lists of k_chunk_scatter, with
-// each store writing consecutive positions.  Tiles outside the union of the
-// row's rects in the chunk are skipped.
+// (chunk, tile row).  The wave loads the chunk's depth-ordered packed rects
+// (NI groups of 64) and compacts the Gaussians that reach its row, in order,
+// through LDS; then, per tile of the row, the members covering it
+// (x0 <= tx < x1) are one ballot per 64-member group, their ranks popcounts
+// of it, and each goes to the tile's base for the chunk + its rank: the lists
+// of k_chunk_scatter, with each store writing consecutive positions.  Tiles
+// outside the union of the members' rects are skipped.
 template <int NI, bool KEYS>
 __global__ void __launch_bounds__(GS_WAVE * GS_RP_WAVES) k_row_place(Dims d, BinArgs a)
 {
+    __shared__ uint32_t s_rc[GS_RP_WAVES][NI * GS_WAVE];
+    __shared__ uint32_t s_gv[GS_RP_WAVES][NI * GS_WAVE];
+    __shared__ uint32_t s_kv[GS_RP_WAVES][KEYS ? NI * GS_WAVE : 1];
     if ((uint64_t)a.total[0] > a.Kcap) return;   // capacity overflow: reported by the host
     const int r = blockIdx.y;
-    const int idx = (int)blockIdx.x * GS_RP_WAVES + (int)threadIdx.x / GS_WAVE;
+    const int w = (int)threadIdx.x / GS_WAVE;
+    const int idx = (int)blockIdx.x * GS_RP_WAVES + w;
     const int c = idx / d.gy, ty = idx % d.gy;
     if (c >= a.nchunks) return;
     const int lane = (int)threadIdx.x % GS_WAVE;
@@ -571,18 +575,39 @@ __global__ void __launch_bounds__(GS_WAVE * GS_RP_WAVES) k_row_place(Dims d, Bin
         gv[k] = a.sval[1][i];
         kv[k] = KEYS ? a.skey[1][i] : 0u;
     }
-    int x0[NI], x1[NI];
-    int ux0 = INT_MAX, ux1 = 0;
+    // members: Gaussians of the chunk whose rect reaches row ty, in order
+    int nm = 0;
 #pragma unroll
     for (int k = 0; k < NI; ++k) {
         const uint32_t p = rc[k];
         const int y0 = (int)((p >> 8) & 0xffu), y1 = (int)(p >> 24);
-        const bool in = k * GS_WAVE + lane < m && y0 <= ty && ty < y1;
-        x0[k] = in ? (int)(p & 0xffu) : 0;
-        x1[k] = in ? (int)((p >> 16) & 0xffu) : 0;   // an empty column range when not in the row
-        if (in && x0[k] < x1[k]) {
-            ux0 = min(ux0, x0[k]);
-            ux1 = max(ux1, x1[k]);
+        const bool in = k * GS_WAVE + lane < m && y0 <= ty && ty < y1 && (p & 0xffu) < ((p >> 16) & 0xffu);
+        const uint64_t mk = ballot(in);
+        if (in) {
+            const int j = nm + (int)__popcll(mk & lanemask_lt());
+            s_rc[w][j] = p;
+            s_gv[w][j] = gv[k];
+            if (KEYS) s_kv[w][j] = kv[k];
+        }
+        nm += (int)__popcll(mk);
+    }
+    if (nm == 0) return;
+    wave_sync();
+    int x0[NI], x1[NI];
+    int ux0 = INT_MAX, ux1 = 0;
+#pragma unroll
+    for (int g = 0; g < NI; ++g) {
+        x0[g] = x1[g] = 0;   // an empty column range past the members
+        if (g * GS_WAVE >= nm) continue;
+        const int j = g * GS_WAVE + lane;
+        if (j < nm) {
+            const uint32_t p = s_rc[w][j];
+            x0[g] = (int)(p & 0xffu);
+            x1[g] = (int)((p >> 16) & 0xffu);
+            gv[g] = s_gv[w][j];
+            if (KEYS) kv[g] = s_kv[w][j];
+            ux0 = min(ux0, x0[g]);
+            ux1 = max(ux1, x1[g]);
         }
     }
     ux0 = -__builtin_amdgcn_readlane(wave_incl_max(-ux0), GS_WAVE - 1);
@@ -594,13 +619,14 @@ __global__ void __launch_bounds__(GS_WAVE * GS_RP_WAVES) k_row_place(Dims d, Bin
         for (int tx = tb; tx < te; ++tx) {
             uint32_t base = (uint32_t)__builtin_amdgcn_readlane((int)bl, tx - tb);
 #pragma unroll
-            for (int k = 0; k < NI; ++k) {
-                const bool cv = x0[k] <= tx && tx < x1[k];
+            for (int g = 0; g < NI; ++g) {
+                if (g * GS_WAVE >= nm) break;
+                const bool cv = x0[g] <= tx && tx < x1[g];
                 const uint64_t mk = ballot(cv);
                 if (cv) {
                     const uint32_t pos = base + (uint32_t)__popcll(mk & lanemask_lt());
-                    a.plist[pos] = gv[k];
-                    if (KEYS) a.keys_sorted[pos] = kv[k];
+                    a.plist[pos] = gv[g];
+                    if (KEYS) a.keys_sorted[pos] = kv[g];
                 }
                 base += (uint32_t)__popcll(mk);
             }
