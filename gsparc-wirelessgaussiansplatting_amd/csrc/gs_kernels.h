@@ -50,6 +50,7 @@ struct Dims {
 #define GS_SCATTER_WAVES 8        // independent waves (chunks) per ordered-scatter workgroup
 #define GS_ROW_PLACE_MAX_T 1024   // counts and placement per (chunk, tile row) up to this many tiles
 #define GS_RP_WAVES 4              // waves per k_row_place workgroup
+#define GS_SEGSCAN_LDS 15360      // (receiver, tile) counts scanned through LDS by k_segscan_lds
 
 inline Dims make_dims(int P, int R, int W, int H)
 {

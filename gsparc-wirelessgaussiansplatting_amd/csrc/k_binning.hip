@@ -35,12 +35,7 @@ template <int NT>
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, uint32_t& total)
 {
     const int wave = threadIdx.x / GS_WAVE, lane = threadIdx.x % GS_WAVE;
-    uint32_t x = v;
-#pragma unroll
-    for (int off = 1; off < GS_WAVE; off <<= 1) {
-        const uint32_t y = __shfl_up(x, off);
-        if (lane >= off) x += y;
-    }
+    const uint32_t x = wave_incl_sum(v);
     if (lane == GS_WAVE - 1) s_wave[wave] = x;
     __syncthreads();
     uint32_t before = 0, tot = 0;
@@ -89,6 +84,47 @@ __global__ void __launch_bounds__(1024) k_segscan(int n, const uint32_t* segcnt,
         }
     }
     if (threadIdx.x == 0) total[0] = tot;
+}
+
+// The same for n <= GS_SEGSCAN_LDS counts: coalesced loads (all in flight)
+// into LDS, the per-thread runs scanned there, coalesced stores.
+__global__ void __launch_bounds__(1024) k_segscan_lds(int n, const uint32_t* segcnt, uint32_t* segbase,
+                                                       uint2* ranges, uint32_t* total)
+{
+    constexpr int PER = GS_SEGSCAN_LDS / 1024;
+    __shared__ uint32_t s_v[GS_SEGSCAN_LDS];
+    __shared__ uint32_t s_wave[1024 / GS_WAVE];
+    const int tid = (int)threadIdx.x;
+    uint32_t v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) v[u] = segcnt[min(u * 1024 + tid, n - 1)];
+#pragma unroll
+    for (int u = 0; u < PER; ++u)
+        if (u * 1024 + tid < n) s_v[u * 1024 + tid] = v[u];
+    __syncthreads();
+    const int per = (n + 1023) / 1024;
+    const int i0 = min(n, tid * per), i1 = min(n, i0 + per);
+    uint32_t sum = 0;
+    for (int i = i0; i < i1; ++i) sum += s_v[i];
+    uint32_t tot;
+    uint32_t run = block_exclusive_scan<1024>(sum, s_wave, tot);
+    for (int i = i0; i < i1; ++i) {
+        const uint32_t c = s_v[i];
+        s_v[i] = run;
+        run += c;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int i = u * 1024 + tid;
+        if (i >= n) break;
+        const uint32_t b = s_v[i], c = v[u];
+        segbase[i] = b;
+        // empty tiles keep the range (0, 0) as after the reference's memset
+        // (rasterizer_impl.cu:303)
+        ranges[i] = c ? make_uint2(b, b + c) : make_uint2(0u, 0u);
+    }
+    if (tid == 0) total[0] = tot;
 }
 
 // ---------------------------------------------------------------- depth sort
@@ -594,7 +630,9 @@ __global__ void __launch_bounds__(GS_WAVE * GS_RP_WAVES) k_row_place(Dims d, Bin
     if (nm == 0) return;
     wave_sync();
     int x0[NI], x1[NI];
-    int ux0 = INT_MAX, ux1 = 0;
+    // union of the members' columns: the largest 255 - x0 and x1 (-1 for a
+    // lane without members, the identity of wave_incl_max)
+    int lo = -1, hi = -1;
 #pragma unroll
     for (int g = 0; g < NI; ++g) {
         x0[g] = x1[g] = 0;   // an empty column range past the members
@@ -606,12 +644,12 @@ __global__ void __launch_bounds__(GS_WAVE * GS_RP_WAVES) k_row_place(Dims d, Bin
             x1[g] = (int)((p >> 16) & 0xffu);
             gv[g] = s_gv[w][j];
             if (KEYS) kv[g] = s_kv[w][j];
-            ux0 = min(ux0, x0[g]);
-            ux1 = max(ux1, x1[g]);
+            lo = max(lo, 255 - x0[g]);
+            hi = max(hi, x1[g]);
         }
     }
-    ux0 = -__builtin_amdgcn_readlane(wave_incl_max(-ux0), GS_WAVE - 1);
-    ux1 = __builtin_amdgcn_readlane(wave_incl_max(ux1), GS_WAVE - 1);
+    const int ux0 = 255 - __builtin_amdgcn_readlane(wave_incl_max(lo), GS_WAVE - 1);
+    const int ux1 = __builtin_amdgcn_readlane(wave_incl_max(hi), GS_WAVE - 1);
     for (int tb = ux0; tb < ux1; tb += GS_WAVE) {
         const int tl = tb + lane;
         const uint32_t bl = tl < ux1 ? a.segbase[rowT + tl] + a.ccnt[(rowT + tl) * a.nchunks + c] : 0u;
@@ -782,7 +820,11 @@ hipError_t launch_chunk_scan(const Dims& d, const BinArgs& a, hipStream_t s)
 
 hipError_t launch_segscan(const Dims& d, const BinArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_segscan, dim3(1), dim3(1024), 0, s, d.R * d.T, a.segcnt, a.segbase, a.ranges, a.total);
+    const int n = d.R * d.T;
+    if (n > 0 && n <= GS_SEGSCAN_LDS)
+        hipLaunchKernelGGL(k_segscan_lds, dim3(1), dim3(1024), 0, s, n, a.segcnt, a.segbase, a.ranges, a.total);
+    else
+        hipLaunchKernelGGL(k_segscan, dim3(1), dim3(1024), 0, s, n, a.segcnt, a.segbase, a.ranges, a.total);
     return hipGetLastError();
 }
 

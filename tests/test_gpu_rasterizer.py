@@ -115,15 +115,18 @@ def test_forward_long_segments():
         _check_forward(sc, st, r)
 
 
-@pytest.mark.parametrize("H,W", [(512, 512), (528, 600)], ids=["1024_tiles_tile_scan", "1254_tiles_chunked"])
-def test_forward_placement_paths(H, W):
-    """Maps of up to 1024 tiles place the instances by per-tile scans, larger
-    ones by the chunked count and scatter: both give the reference's lists."""
-    sc = S.small_scene(31, 1500, R=2, H=H, W=W, walls=True)
-    print("placement", H, W, flush=True)
+@pytest.mark.parametrize("H,W,R", [(512, 512, 2), (528, 600, 2), (528, 600, 13)],
+                         ids=["1024_tiles_row_place", "1254_tiles_chunk_scatter", "16302_tile_lists"])
+def test_forward_placement_paths(H, W, R):
+    """Maps of up to 1024 tiles place the instances per (chunk, tile row),
+    larger ones by the chunked scatter; up to 15360 (receiver, tile) lists are
+    scanned through LDS, more by the two-pass scan.  All give the reference's
+    lists (the last receiver's ranges carry the whole prefix)."""
+    sc = S.small_scene(31, 1500, R=R, H=H, W=W, walls=True)
+    print("placement", H, W, R, flush=True)
     _, out = gpu_forward(sc)
     st = read_state(sc, out)
-    for r in range(sc.R):
+    for r in sorted({0, sc.R - 1}):
         _check_forward(sc, st, r)
 
 
