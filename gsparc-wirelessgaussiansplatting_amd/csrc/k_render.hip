@@ -217,9 +217,8 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
                 // a uniform branch: the body runs for the whole wave, selects
                 // keep the state of the lanes whose pair does not contribute
                 if (ballot(in) == 0) continue;
-                // the exponential is only valid on [thr, 0]: elsewhere G = 0,
-                // so no out-of-domain value reaches the sums
-                const float G = in ? gs_expf_blend(power) : 0.0f;
+                // G = 0 off [thr, 0], so no out-of-range value reaches the sums
+                const float G = in ? gs_expf_grad(power) : 0.0f;
                 const float oG = B.y * G;
                 const float alpha = fminf(0.99f, oG);
                 const bool c = in && !(alpha < 1.0f / 255.0f);

@@ -389,7 +389,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd_csi(Dims d, RenderCsiArg
             bool c = active && !(power > 0.0f) && !(power < B.z);
             float G = 0.0f, oG = 0.0f, alpha = 0.0f;
             if (c) {
-                G = gs_expf_blend(power);
+                G = gs_expf_grad(power);
                 oG = B.y * G;
                 alpha = fminf(0.99f, oG);
                 c = !(alpha < 1.0f / 255.0f);
