@@ -17,6 +17,32 @@
 
 namespace gs {
 
+// The geometry adjoint only produces gradients (held to 1e-4 of the largest
+// magnitude), so its quotients and square roots use the hardware reciprocal
+// and square root (1 ulp) instead of the correctly rounded sequences.
+__device__ __forceinline__ float rcp_g(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float sqrt_g(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// The Jacobian of the equirectangular projection (as project() forms it)
+// with those operations.
+__device__ __forceinline__ void jacobian_g(const float* t, int W, int H, float J[2][3])
+{
+    const float rxz2 = t[0] * t[0] + t[2] * t[2];
+    const float rxz2_inv = rcp_g(rxz2 + 1e-7f);
+    const float rxz = sqrt_g(rxz2);
+    const float rxz_inv = rcp_g(rxz + 1e-7f);
+    const float r2 = rxz2 + t[1] * t[1];
+    const float r2_inv = rcp_g(r2 + 1e-7f);
+    const float Wd = (float)W * 0.5f * GS_ONE_OVER_PI_F;
+    const float Hd = (float)H * GS_ONE_OVER_PI_F;
+    J[0][0] = Wd * t[2] * rxz2_inv;
+    J[0][1] = 0.0f;
+    J[0][2] = -Wd * t[0] * rxz2_inv;
+    J[1][0] = -Hd * t[0] * t[1] * rxz_inv * r2_inv;
+    J[1][1] = Hd * rxz * r2_inv;
+    J[1][2] = -Hd * t[2] * t[1] * rxz_inv * r2_inv;
+}
+
 #define GS_BWD_THREADS 64
 #define GS_PART 12   // per (receiver chunk, Gaussian): dmean[3], dcov3[6], dopacity, db2[2]
 
@@ -27,15 +53,10 @@ __device__ __forceinline__ void geo_bwd(const float* mean, const float* c3, cons
     float t[3];
     view_xform(mean[0], mean[1], mean[2], view, t);
     const float rr = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
-    GeoOut go;
-    const float rd = sqrtf(rr);
-    project(t, rd, W, H, go);
+    const float rd = sqrt_g(rr);
     const float dpx = g[0], dpy = g[1], gqa = g[2], gqb = g[3], gqc = g[4];
     float Jc[2][3];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) Jc[a][k] = go.J[a][k];
+    jacobian_g(t, W, H, Jc);
     if (MODE == GRAD_REFERENCE) {   // backward.cu:172-173
         Jc[0][0] *= 0.5f;
         Jc[0][2] *= 0.5f;
@@ -45,13 +66,13 @@ __device__ __forceinline__ void geo_bwd(const float* mean, const float* c3, cons
     const float det = A * C - B * B;
     float dA, dB, dC;
     if (MODE == GRAD_REFERENCE) {   // backward.cu:212-224
-        const float d2i = 1.0f / (det * det + 1e-7f);
+        const float d2i = rcp_g(det * det + 1e-7f);
         const float gy = 0.5f * gqb;
         dA = d2i * (-C * C * gqa + 2.0f * B * C * gy + (det - A * C) * gqc);
         dC = d2i * (-A * A * gqc + 2.0f * A * B * gy + (det - A * C) * gqa);
         dB = d2i * 2.0f * (B * C * gqa - (det + 2.0f * B * B) * gy + A * B * gqc);
     } else {
-        const float d2i = 1.0f / (det * det);
+        const float d2i = rcp_g(det * det);
         dA = d2i * (-C * C * gqa + B * C * gqb - B * B * gqc);
         dB = d2i * (2.0f * B * C * gqa - (det + 2.0f * B * B) * gqb + 2.0f * A * B * gqc);
         dC = d2i * (-B * B * gqa + A * B * gqb - A * A * gqc);
@@ -92,12 +113,12 @@ __device__ __forceinline__ void geo_bwd(const float* mean, const float* c3, cons
         // backward.cu:152-177, 249-292
         const float txtx = tx * tx, tyty = ty * ty, tztz = tz * tz, txtytz = tx * ty * tz;
         const float rxz2 = txtx + tztz;
-        const float rxz2_inv = 1.0f / (rxz2 + 1e-7f);
+        const float rxz2_inv = rcp_g(rxz2 + 1e-7f);
         const float rxz4_inv = rxz2_inv * rxz2_inv;
-        const float rxz = sqrtf(rxz2);
-        const float rxz_inv = 1.0f / (rxz + 1e-7f);
+        const float rxz = sqrt_g(rxz2);
+        const float rxz_inv = rcp_g(rxz + 1e-7f);
         const float r2 = rxz2 + tyty;
-        const float r2_inv = 1.0f / (r2 + 1e-7f);
+        const float r2_inv = rcp_g(r2 + 1e-7f);
         const float r4_inv = r2_inv * r2_inv;
         const float rxz_r4_inv = rxz_inv * r4_inv;
         const float rxz3_r4_inv = rxz2_inv * rxz_r4_inv;
@@ -117,16 +138,17 @@ __device__ __forceinline__ void geo_bwd(const float* mean, const float* c3, cons
     } else {
         const float eps = 1e-7f;
         const float rxz2 = tx * tx + tz * tz;
-        const float ia = 1.0f / (rxz2 + eps);
-        const float rho = sqrtf(rxz2);
-        const float ib = 1.0f / (rho + eps);
+        const float ia = rcp_g(rxz2 + eps);
+        const float rho = sqrt_g(rxz2);
+        const float ib = rcp_g(rho + eps);
         const float r2 = rxz2 + ty * ty;
-        const float ic = 1.0f / (r2 + eps);
+        const float ic = rcp_g(r2 + eps);
         const float g0x = dJ[0][0], g0z = dJ[0][2], h0 = dJ[1][0], h1 = dJ[1][1], h2 = dJ[1][2];
         dt[0] += g0x * Wd * tz * (-2.0f * tx * ia * ia) + g0z * (-Wd) * (ia - 2.0f * tx * tx * ia * ia);
         dt[2] += g0x * Wd * (ia - 2.0f * tz * tz * ia * ia) + g0z * (-Wd) * tx * (-2.0f * tz * ia * ia);
         const float Kc = ib * ic;
-        const float drho_x = rho > 0.0f ? tx / rho : 0.0f, drho_z = rho > 0.0f ? tz / rho : 0.0f;
+        const float irho = rcp_g(rho);
+        const float drho_x = rho > 0.0f ? tx * irho : 0.0f, drho_z = rho > 0.0f ? tz * irho : 0.0f;
         const float dK0 = -ic * ib * ib * drho_x - 2.0f * ib * ic * ic * tx;
         const float dK1 = -2.0f * ib * ic * ic * ty;
         const float dK2 = -ic * ib * ib * drho_z - 2.0f * ib * ic * ic * tz;
@@ -142,15 +164,17 @@ __device__ __forceinline__ void geo_bwd(const float* mean, const float* c3, cons
         const float Wdp = (float)W * 0.5f * GS_ONE_OVER_PI_F;
         const float Hdp = (float)H * 0.5f * GS_TWO_OVER_PI_F;
         if (rxz2 > 0.0f) {
-            dt[0] += dpx * Wdp * tz / rxz2;
-            dt[2] += dpx * Wdp * (-tx) / rxz2;
+            const float irxz2 = rcp_g(rxz2);
+            dt[0] += dpx * Wdp * tz * irxz2;
+            dt[2] += dpx * Wdp * (-tx) * irxz2;
         }
         const float re = rd + eps;
-        const float u = ty / re;
-        const float dlat_du = 1.0f / sqrtf(1.0f - u * u);
-        const float k2 = ty / (rd * re * re);
+        const float ire = rcp_g(re);
+        const float u = ty * ire;
+        const float dlat_du = rcp_g(sqrt_g(1.0f - u * u));
+        const float k2 = ty * rcp_g(rd * re * re);
         dt[0] += dpy * Hdp * dlat_du * (-k2 * tx);
-        dt[1] += dpy * Hdp * dlat_du * (1.0f / re - k2 * ty);
+        dt[1] += dpy * Hdp * dlat_du * (ire - k2 * ty);
         dt[2] += dpy * Hdp * dlat_du * (-k2 * tz);
     }
 #pragma unroll
@@ -191,9 +215,10 @@ __global__ void __launch_bounds__(GS_BWD_THREADS) k_pre_bwd_geo(Dims d, PreBwdAr
     for (int k = 0; k < 6; ++k) c3[k] = a.cov3[6 * gg + k];
     const float* tx = a.tx + 3 * grp.scene;
     const float diff[3] = {tx[0] - mean[0], tx[1] - mean[1], tx[2] - mean[2]};
-    const float dist = sqrtf(diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2]);
+    const float dist = sqrt_g(diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2]);
     const float de = dist + 1e-6f;
-    const float den = (MODE == GRAD_REFERENCE) ? de * de : dist * de;
+    const float iden = rcp_g((MODE == GRAD_REFERENCE) ? de * de : dist * de);
+    const float idist = rcp_g(dist);
 
     float dmean[3] = {0, 0, 0}, dc[6] = {0, 0, 0, 0, 0, 0}, dop = 0.0f, db2_0 = 0.0f, db2_1 = 0.0f;
     // The records of receiver c + 1 are loaded (unconditionally: a culled
@@ -228,9 +253,9 @@ __global__ void __launch_bounds__(GS_BWD_THREADS) k_pre_bwd_geo(Dims d, PreBwdAr
                 const float gl = a.gL[ri];
                 const float* rxp = s_rx + 3 * c;
                 const float b[3] = {mean[0] - rxp[0], mean[1] - rxp[1], mean[2] - rxp[2]};
-                const float db = sqrtf(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+                const float idb = rcp_g(sqrt_g(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]));
 #pragma unroll
-                for (int k = 0; k < 3; ++k) dmean[k] += gl * (b[k] / db - diff[k] / dist);
+                for (int k = 0; k < 3; ++k) dmean[k] += gl * (b[k] * idb - diff[k] * idist);
             }
             if (a.dmeans2d) {
                 a.dmeans2d[3 * ri + 0] = gv[0] * (0.5f * (float)d.W);
@@ -244,7 +269,7 @@ __global__ void __launch_bounds__(GS_BWD_THREADS) k_pre_bwd_geo(Dims d, PreBwdAr
                 dz.y = gv[7] * s1 * (1.0f - s1);
                 const float dld = -(dz.x + dz.y);
 #pragma unroll
-                for (int k = 0; k < 3; ++k) dmean[k] += -dld * diff[k] / den;
+                for (int k = 0; k < 3; ++k) dmean[k] += -dld * diff[k] * iden;
                 db2_0 += dz.x;
                 db2_1 += dz.y;
             } else if (a.dsig) {
