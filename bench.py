@@ -579,7 +579,12 @@ def main():
         log(args, f"calibration copy of {args.copy_mib} MiB done")
 
     # per-kernel times with HIP events on the launch stream, then (separately:
-    # the counting variants are slower) the blend kernels' pair counts
+    # the counting variants are slower) the blend kernels' pair counts; after
+    # the host-bound plain and latency runs a few steps first bring the clocks
+    # back to where the timed steps ran
+    for _ in range(max(3, args.warmup)):
+        wl.step()
+    torch.cuda.synchronize()
     nprof = 3
     gh.profile_enable(True, timing=True, counting=False)
     for _ in range(nprof):
