@@ -383,31 +383,44 @@ __device__ __forceinline__ void wave_sync()
 // instances per (chunk, tile) -> ccnt[(r*T + t) * nchunks + c]
 __global__ void __launch_bounds__(GS_RS_THREADS) k_chunk_count(Dims d, BinArgs a)
 {
+    // LDS path: a difference array per tile row, (gx + 1) wide: a rect adds
+    // +1 at (y, x0) and -1 at (y, x1) for each of its rows; each tile's count
+    // is then the prefix of its row up to its column
     extern __shared__ uint32_t s_cnt[];
     const int c = blockIdx.x, r = blockIdx.y;
     const bool lds = chunk_lds(d);
     const int n = (int)a.nvis[r];
     const int i0 = c * a.chunk_len, i1 = min(n, i0 + a.chunk_len);
     const size_t rP = rx_map(d, r).roff;
+    const int rw = d.gx + 1;
     uint32_t* slot = a.ccnt + (size_t)r * d.T * a.nchunks + c;   // tile t at slot[t * nchunks]
-    for (int t = threadIdx.x; t < d.T; t += GS_RS_THREADS) {
-        if (lds) s_cnt[t] = 0;
-        else slot[(size_t)t * a.nchunks] = 0;
+    if (lds) {
+        for (int t = threadIdx.x; t < rw * d.gy; t += GS_RS_THREADS) s_cnt[t] = 0;
+    } else {
+        for (int t = threadIdx.x; t < d.T; t += GS_RS_THREADS) slot[(size_t)t * a.nchunks] = 0;
     }
     __syncthreads();
     for (int i = i0 + threadIdx.x; i < i1; i += GS_RS_THREADS) {
         int x0, y0, x1, y1;
         chunk_rect(d, a, rP, i, a.rect ? 0u : a.sval[1][rP + i], x0, y0, x1, y1);
-        for (int ty = y0; ty < y1; ++ty)
-            for (int tx = x0; tx < x1; ++tx) {
-                const int t = ty * d.gx + tx;
-                if (lds) atomicAdd(&s_cnt[t], 1u);
-                else atomicAdd(&slot[(size_t)t * a.nchunks], 1u);
+        if (x0 >= x1) continue;
+        for (int ty = y0; ty < y1; ++ty) {
+            if (lds) {
+                atomicAdd(&s_cnt[ty * rw + x0], 1u);
+                atomicAdd(&s_cnt[ty * rw + x1], 0xffffffffu);   // -1 modulo 2^32
+            } else {
+                for (int tx = x0; tx < x1; ++tx) atomicAdd(&slot[(size_t)(ty * d.gx + tx) * a.nchunks], 1u);
             }
+        }
     }
     if (lds) {
         __syncthreads();
-        for (int t = threadIdx.x; t < d.T; t += GS_RS_THREADS) slot[(size_t)t * a.nchunks] = s_cnt[t];
+        for (int t = threadIdx.x; t < d.T; t += GS_RS_THREADS) {
+            const int ty = t / d.gx, tx = t % d.gx;
+            uint32_t v = 0;
+            for (int x = 0; x <= tx; ++x) v += s_cnt[ty * rw + x];
+            slot[(size_t)t * a.nchunks] = v;
+        }
     }
 }
 
@@ -805,7 +818,7 @@ hipError_t launch_depth_sort(const Dims& d, const BinArgs& a, hipStream_t s)
 hipError_t launch_chunk_count(const Dims& d, const BinArgs& a, hipStream_t s)
 {
     if (d.R == 0) return hipSuccess;
-    const size_t lds = chunk_lds(d) ? (size_t)d.T * 4 : 0;
+    const size_t lds = chunk_lds(d) ? (size_t)(d.gx + 1) * d.gy * 4 : 0;
     hipLaunchKernelGGL(k_chunk_count, dim3(a.nchunks, d.R), dim3(GS_RS_THREADS), lds, s, d, a);
     return hipGetLastError();
 }
@@ -828,7 +841,8 @@ hipError_t launch_segscan(const Dims& d, const BinArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
-static_assert(GS_CHUNK_MIN == 256 && GS_ROW_PLACE_MAX_T <= 1024, "row placement holds 4 to 16 groups of 64");
+static_assert((GS_CHUNK_MIN == 256 || GS_CHUNK_MIN == 512) && GS_ROW_PLACE_MAX_T <= 1024,
+              "row placement holds 4 to 16 groups of 64");
 bool row_place_ok(const Dims& d) { return rect_packed(d) && d.T <= GS_ROW_PLACE_MAX_T; }
 
 template <int NI>
@@ -844,7 +858,7 @@ hipError_t launch_bin_place(const Dims& d, const BinArgs& a, hipStream_t s)
 {
     if (d.R == 0 || d.P == 0) return hipSuccess;
     if (!row_place_ok(d)) return launch_chunk_scatter(d, a, s);
-    // chunk_len = 256 * ceil(T / 256) <= GS_ROW_PLACE_MAX_T: 4, 8, 12 or 16 groups of 64
+    // chunk_len = GS_CHUNK_MIN * ceil(T / GS_CHUNK_MIN) <= 1024: 4, 8, 12 or 16 groups of 64
     switch (a.chunk_len / GS_WAVE) {
     case 4: launch_row_place_ni<4>(d, a, s); break;
     case 8: launch_row_place_ni<8>(d, a, s); break;
