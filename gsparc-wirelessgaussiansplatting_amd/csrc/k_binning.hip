@@ -189,9 +189,7 @@ __global__ void __launch_bounds__(GS_RS_THREADS) k_rs_hist(Dims d, BinArgs a, in
 #pragma unroll
         for (int k = 0; k < GS_RS_ITEMS; ++k) {
             const bool valid = blk * GS_RS_TILE + k * GS_RS_THREADS + (int)threadIdx.x < n;
-            const uint64_t m = match_digit<FIRST ? 9 : 8>(dg[k], valid);
-            // one LDS atomic per distinct digit of the wave
-            if (valid && (m & lanemask_lt()) == 0) atomicAdd(&h[dg[k]], (uint32_t)__popcll(m));
+            if (valid) atomicAdd(&h[dg[k]], 1u);
         }
     }
     __syncthreads();
