@@ -162,6 +162,9 @@ hipError_t launch_depth_sort(const Dims& d, const BinArgs& a, hipStream_t s);
 hipError_t launch_chunk_count(const Dims& d, const BinArgs& a, hipStream_t s);
 hipError_t launch_chunk_scan(const Dims& d, const BinArgs& a, hipStream_t s);
 hipError_t launch_segscan(const Dims& d, const BinArgs& a, hipStream_t s);
+// launch_segscan followed by launch_tile_order(mode) on the counts, in one
+// launch where the counts fit the LDS scan and the mode has at most 8 lists
+hipError_t launch_segscan_order(const Dims& d, const BinArgs& a, int mode, uint32_t* order, hipStream_t s);
 hipError_t launch_chunk_scatter(const Dims& d, const BinArgs& a, hipStream_t s);
 // Placement of the instances after launch_chunk_count and launch_chunk_scan:
 // per (chunk, tile row) waves when row_place_ok(d) (packed rects and at most

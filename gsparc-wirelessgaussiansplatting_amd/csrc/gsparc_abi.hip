@@ -363,9 +363,7 @@ int forward_geom_impl(const gs::Dims& d, const FwdInputs& in, void* geom, void* 
     } else {
         GS_LAUNCH("clear_counts", gs::launch_fill_u32(ba.segcnt, 0u, (size_t)d.R * d.T, s), s);
     }
-    GS_LAUNCH("bin_segscan", gs::launch_segscan(d, ba, s), s);
-    GS_LAUNCH("bin_order", gs::launch_tile_order(d, choose_order(d), ba.ranges, (uint32_t*)(ib + L.tile_order),
-                                                 ba.total, s), s);
+    GS_LAUNCH("bin_segscan", gs::launch_segscan_order(d, ba, choose_order(d), (uint32_t*)(ib + L.tile_order), s), s);
     return GSPARC_OK;
 }
 

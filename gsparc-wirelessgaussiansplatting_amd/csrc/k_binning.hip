@@ -87,13 +87,12 @@ __global__ void __launch_bounds__(1024) k_segscan(int n, const uint32_t* segcnt,
 }
 
 // The same for n <= GS_SEGSCAN_LDS counts: coalesced loads (all in flight)
-// into LDS, the per-thread runs scanned there, coalesced stores.
-__global__ void __launch_bounds__(1024) k_segscan_lds(int n, const uint32_t* segcnt, uint32_t* segbase,
-                                                       uint2* ranges, uint32_t* total)
+// into LDS, the per-thread runs scanned there, coalesced stores.  Run by a
+// 1024-thread workgroup.
+__device__ __forceinline__ void segscan_lds_body(int n, const uint32_t* segcnt, uint32_t* segbase, uint2* ranges,
+                                                 uint32_t* total, uint32_t* s_v, uint32_t* s_wave)
 {
     constexpr int PER = GS_SEGSCAN_LDS / 1024;
-    __shared__ uint32_t s_v[GS_SEGSCAN_LDS];
-    __shared__ uint32_t s_wave[1024 / GS_WAVE];
     const int tid = (int)threadIdx.x;
     uint32_t v[PER];
 #pragma unroll
@@ -125,6 +124,14 @@ __global__ void __launch_bounds__(1024) k_segscan_lds(int n, const uint32_t* seg
         ranges[i] = c ? make_uint2(b, b + c) : make_uint2(0u, 0u);
     }
     if (tid == 0) total[0] = tot;
+}
+
+__global__ void __launch_bounds__(1024) k_segscan_lds(int n, const uint32_t* segcnt, uint32_t* segbase,
+                                                       uint2* ranges, uint32_t* total)
+{
+    __shared__ uint32_t s_v[GS_SEGSCAN_LDS];
+    __shared__ uint32_t s_wave[1024 / GS_WAVE];
+    segscan_lds_body(n, segcnt, segbase, ranges, total, s_v, s_wave);
 }
 
 // ---------------------------------------------------------------- depth sort
@@ -697,11 +704,12 @@ __device__ __forceinline__ int length_bucket(uint32_t L)
 // the rest of the list's slots (up to stride, or up to len for a single list)
 // get the empty marker 0xffffffff.  Workgroup 0 also records the mode in
 // total[1].
-__global__ void __launch_bounds__(1024) k_tile_order(const uint2* ranges, int R, int T, int stride, uint32_t* order,
-                                                     uint32_t* total, int mode, int len)
+// List l of L (a 1024-thread workgroup); len_of(rt) is the list length of
+// (receiver, tile) rt.
+template <class LenOf>
+__device__ __forceinline__ void tile_order_body(LenOf len_of, int R, int T, int stride, uint32_t* order,
+                                                uint32_t* total, int mode, int len, int l, int L, uint32_t* cnt)
 {
-    __shared__ uint32_t cnt[128];
-    const int l = blockIdx.x, L = gridDim.x;
     const int n = (R - l + L - 1) / L * T;
     uint32_t* out = order + (size_t)l * stride;
     if (l == 0 && threadIdx.x == 0) total[1] = (uint32_t)mode;
@@ -709,10 +717,8 @@ __global__ void __launch_bounds__(1024) k_tile_order(const uint2* ranges, int R,
     for (int i = n + (int)threadIdx.x; i < pad_end; i += 1024) out[i] = 0xffffffffu;
     for (int i = threadIdx.x; i < 128; i += 1024) cnt[i] = 0;
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += 1024) {
-        const uint2 rg = ranges[(size_t)(l + (i / T) * L) * T + i % T];
-        atomicAdd(&cnt[length_bucket(rg.y - rg.x)], 1u);
-    }
+    for (int i = threadIdx.x; i < n; i += 1024)
+        atomicAdd(&cnt[length_bucket(len_of((size_t)(l + (i / T) * L) * T + i % T))], 1u);
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t run = 0;
@@ -725,9 +731,33 @@ __global__ void __launch_bounds__(1024) k_tile_order(const uint2* ranges, int R,
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += 1024) {
         const uint32_t rt = (uint32_t)((l + (i / T) * L) * T + i % T);
-        const uint2 rg = ranges[rt];
-        out[atomicAdd(&cnt[length_bucket(rg.y - rg.x)], 1u)] = rt;
+        out[atomicAdd(&cnt[length_bucket(len_of(rt))], 1u)] = rt;
     }
+}
+
+__global__ void __launch_bounds__(1024) k_tile_order(const uint2* ranges, int R, int T, int stride, uint32_t* order,
+                                                     uint32_t* total, int mode, int len)
+{
+    __shared__ uint32_t cnt[128];
+    tile_order_body([ranges](size_t rt) { const uint2 rg = ranges[rt]; return rg.y - rg.x; }, R, T, stride, order,
+                    total, mode, len, blockIdx.x, gridDim.x, cnt);
+}
+
+// Workgroup 0 scans the (receiver, tile) counts (k_segscan_lds); workgroups
+// 1 .. L build the L work-order lists from the same counts (a list's length
+// is its count), so both run in one launch.
+__global__ void __launch_bounds__(1024) k_segscan_order(int n, const uint32_t* segcnt, uint32_t* segbase,
+                                                         uint2* ranges, uint32_t* total, int R, int T, int stride,
+                                                         uint32_t* order, int mode, int len)
+{
+    __shared__ uint32_t s_v[GS_SEGSCAN_LDS];
+    __shared__ uint32_t s_wave[1024 / GS_WAVE];
+    if (blockIdx.x == 0) {
+        segscan_lds_body(n, segcnt, segbase, ranges, total, s_v, s_wave);
+        return;
+    }
+    tile_order_body([segcnt](size_t rt) { return segcnt[rt]; }, R, T, stride, order, total, mode, len,
+                    (int)blockIdx.x - 1, (int)gridDim.x - 1, s_v);
 }
 
 // Mode 3: one workgroup per receiver r sorts its own tiles into slot r / 8
@@ -879,6 +909,20 @@ size_t order_length(const Dims& d)
 {
     // room for every mode: 8 XCD lists of ceil(R/8) receivers' tiles
     return (size_t)8 * ((d.R + 7) / 8) * d.T;
+}
+
+hipError_t launch_segscan_order(const Dims& d, const BinArgs& a, int mode, uint32_t* order, hipStream_t s)
+{
+    const int n = d.R * d.T;
+    if (d.R == 0 || d.T == 0 || n > GS_SEGSCAN_LDS || mode == ORDER_XCD_RECEIVER) {
+        const hipError_t e = launch_segscan(d, a, s);
+        return e == hipSuccess ? launch_tile_order(d, mode, a.ranges, order, a.total, s) : e;
+    }
+    const int L = mode == ORDER_XCD_MIXED ? 8 : 1;
+    const int stride = mode == ORDER_XCD_MIXED ? (d.R + 7) / 8 * d.T : 0;
+    hipLaunchKernelGGL(k_segscan_order, dim3(1 + L), dim3(1024), 0, s, n, a.segcnt, a.segbase, a.ranges, a.total, d.R,
+                       d.T, stride, order, mode, (int)order_length(d));
+    return hipGetLastError();
 }
 
 hipError_t launch_tile_order(const Dims& d, int mode, const uint2* ranges, uint32_t* order, uint32_t* total,
