@@ -115,13 +115,15 @@ def test_forward_long_segments():
         _check_forward(sc, st, r)
 
 
-@pytest.mark.parametrize("H,W,R", [(512, 512, 2), (528, 600, 2), (528, 600, 13)],
-                         ids=["1024_tiles_row_place", "1254_tiles_chunk_scatter", "16302_tile_lists"])
+@pytest.mark.parametrize("H,W,R", [(512, 512, 2), (528, 600, 2), (528, 600, 13), (1024, 1024, 1)],
+                         ids=["1024_tiles_row_place", "1254_tiles_chunk_scatter", "16302_tile_lists",
+                              "4096_tiles_global_counts"])
 def test_forward_placement_paths(H, W, R):
     """Maps of up to 1024 tiles place the instances per (chunk, tile row),
-    larger ones by the chunked scatter; up to 15360 (receiver, tile) lists are
-    scanned through LDS, more by the two-pass scan.  All give the reference's
-    lists (the last receiver's ranges carry the whole prefix)."""
+    larger ones by the chunked scatter (its tile tables in LDS up to a few
+    thousand tiles, in global memory beyond); up to 15360 (receiver, tile)
+    lists are scanned through LDS, more by the two-pass scan.  All give the
+    reference's lists (the last receiver's ranges carry the whole prefix)."""
     sc = S.small_scene(31, 1500, R=R, H=H, W=W, walls=True)
     print("placement", H, W, R, flush=True)
     _, out = gpu_forward(sc)
