@@ -362,13 +362,16 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd_csi(Dims d, RenderCsiArg
             ss[lane] = a.sig[gi];
             keep = !block_misses(A.x, A.y, A.z, A.w, B.x, B.z, pm.x0, pm.x1, pm.y0, pm.y1);
             if (keep) {
+                // the phase factors only weight gradients here: the hardware
+                // v_sin_f32 / v_cos_f32 of the exact cycle fraction (their
+                // argument is in turns) instead of the mirrored polynomials
                 const float L = a.plen[gi];
 #pragma unroll 2
                 for (int k2 = 0; k2 < CSI_NC2; ++k2) {
-                    float c0, n0, c1, n1;
-                    gs_sincos2pi(gs_cycle_frac(L, a.csi.khi[2 * k2], a.csi.klo[2 * k2]), &c0, &n0);
-                    gs_sincos2pi(gs_cycle_frac(L, a.csi.khi[2 * k2 + 1], a.csi.klo[2 * k2 + 1]), &c1, &n1);
-                    su[lane][k2] = make_float4(c0, -n0, c1, -n1);   // u_k = exp(-j 2 pi x)
+                    const float x0 = gs_cycle_frac(L, a.csi.khi[2 * k2], a.csi.klo[2 * k2]);
+                    const float x1 = gs_cycle_frac(L, a.csi.khi[2 * k2 + 1], a.csi.klo[2 * k2 + 1]);
+                    su[lane][k2] = make_float4(__builtin_amdgcn_cosf(x0), -__builtin_amdgcn_sinf(x0),
+                                               __builtin_amdgcn_cosf(x1), -__builtin_amdgcn_sinf(x1));   // exp(-j 2 pi x)
                 }
             }
         }
