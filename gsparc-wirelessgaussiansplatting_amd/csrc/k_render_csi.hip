@@ -34,6 +34,16 @@ namespace gs {
 #define CSI_BATCH 32
 #define CSI_NC2 (GS_CSI_NC / 2)       // float4 = two subcarriers
 
+// (cos 2 pi x, sin 2 pi x) of the forward's phase factors: the hardware
+// v_cos_f32 / v_sin_f32 (argument in turns) of the exact cycle fraction.
+// The phases scale the emitted signal only; no decision of the blend depends
+// on them, and the maps stay within 1e-5 of the oracle's peak.
+__device__ __forceinline__ void csi_phase(float x, float* c, float* s)
+{
+    *c = __builtin_amdgcn_cosf(x);
+    *s = __builtin_amdgcn_sinf(x);
+}
+
 template <int PPL>
 __global__ void __launch_bounds__(GS_WAVE) k_render_fwd_csi(Dims d, RenderCsiArgs a)
 {
@@ -88,8 +98,8 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd_csi(Dims d, RenderCsiArg
 #pragma unroll 2
                 for (int k2 = 0; k2 < CSI_NC2; ++k2) {
                     float c0, n0, c1, n1, e0r, e0i, e1r, e1i;
-                    gs_sincos2pi(gs_cycle_frac(L, a.csi.khi[2 * k2], a.csi.klo[2 * k2]), &c0, &n0);
-                    gs_sincos2pi(gs_cycle_frac(L, a.csi.khi[2 * k2 + 1], a.csi.klo[2 * k2 + 1]), &c1, &n1);
+                    csi_phase(gs_cycle_frac(L, a.csi.khi[2 * k2], a.csi.klo[2 * k2]), &c0, &n0);
+                    csi_phase(gs_cycle_frac(L, a.csi.khi[2 * k2 + 1], a.csi.klo[2 * k2 + 1]), &c1, &n1);
                     gs_csi_emit(S.x, S.y, c0, n0, &e0r, &e0i);
                     gs_csi_emit(S.x, S.y, c1, n1, &e1r, &e1i);
                     se[lane][k2] = make_float4(e0r, e0i, e1r, e1i);
@@ -237,8 +247,8 @@ k_render_fwd_csi_mfma(Dims d, RenderCsiArgs a)
             const float2 S = ssig[k];
             const float L = splen[k];
             float c0, n0, c1, n1, e0r, e0i, e1r, e1i;
-            gs_sincos2pi(gs_cycle_frac(L, a.csi.khi[2 * k2], a.csi.klo[2 * k2]), &c0, &n0);
-            gs_sincos2pi(gs_cycle_frac(L, a.csi.khi[2 * k2 + 1], a.csi.klo[2 * k2 + 1]), &c1, &n1);
+            csi_phase(gs_cycle_frac(L, a.csi.khi[2 * k2], a.csi.klo[2 * k2]), &c0, &n0);
+            csi_phase(gs_cycle_frac(L, a.csi.khi[2 * k2 + 1], a.csi.klo[2 * k2 + 1]), &c1, &n1);
             gs_csi_emit(S.x, S.y, c0, n0, &e0r, &e0i);
             gs_csi_emit(S.x, S.y, c1, n1, &e1r, &e1i);
             *reinterpret_cast<float4*>(&sE[k][4 * k2]) = make_float4(e0r, e0i, e1r, e1i);
