@@ -56,6 +56,29 @@ __device__ __forceinline__ Pixels<PPL> map_pixels(const Dims& d, const uint32_t*
     return m;
 }
 
+// The tile lists overflowed their capacity (K > Kcap, so the placement
+// wrote nothing): the forward blends do not run.  They leave defined
+// outputs instead, written by the kernel itself (nothing to clear in a
+// captured graph): NaN spectra, so that a loss computed from them cannot
+// pass for a real one, transmittance 1 and no contributors, so that the
+// backward (which returns without reading them) has a consistent state.
+// out: nch floats per pixel; out2 (may be null) one more.
+template <int PPL>
+__device__ __forceinline__ void overflow_outputs(const Dims& d, const Pixels<PPL>& pm, float* out, float* out2,
+                                                 int nch, float* finalT, uint32_t* ncontrib)
+{
+    const float qnan = __builtin_nanf("");
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+        if (!pm.inside[k]) continue;
+        const size_t p = (size_t)pm.r * d.W * d.H + (size_t)pm.py[k] * d.W + pm.px[k];
+        for (int c = 0; c < nch; ++c) out[p * nch + c] = qnan;
+        if (out2) out2[p] = qnan;
+        finalT[p] = 1.0f;
+        ncontrib[p] = 0u;
+    }
+}
+
 template <int CTRL>
 __device__ __forceinline__ float dpp_move(float v)
 {

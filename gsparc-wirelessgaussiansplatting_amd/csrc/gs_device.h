@@ -5,8 +5,9 @@
 // T = tiles per map, N = H*W pixels):
 //   splat[r*P + g]  = 2 x float4 {px, py, ha, nb} {hc, opacity, thr, 0}
 //                     ha = -qa/2, nb = -qb, hc = -qc/2 (conic (qa, qb, qc)),
-//                     thr = log(1/(255 o)) - 1e-3: pairs with power < thr
-//                     provably have alpha < 1/255 and skip the exponential.
+//                     thr = gs_alpha_thr(o): a pair has alpha >= 1/255
+//                     exactly when power >= thr, so both blends decide by
+//                     this comparison and skip the exponential below it.
 //   sig[r*P + g]    = float2 {real, imag} emission
 //   depth[r*P + g]  = float distance to the receiver (sort key)
 //   radii[r*P + g]  = int, 0 = culled
@@ -148,8 +149,10 @@ __device__ __forceinline__ int dpp_or_identity(int identity, int v)
 }
 
 // exp(x) as the hardware 2^t (v_exp_f32) of t = x * log2(e), for the blend
-// backward, where G = exp(power) only feeds gradients (the forward's
-// decisions come in as the contributor count and final T).  For power in
+// backward, where G = exp(power) only feeds gradients: the forward's
+// decisions come in as the contributor count and final T, and the skip test
+// is the forward's own power >= thr comparison (gs_alpha_thr), never a test
+// of this value.  For power in
 // [thr, 0] with opacity <= 1 (|t| < 8.1) the relative error is below 4.5e-7
 // (the rounding of t, at most 8.1 * 2^-24 * ln 2, plus 1 ulp of v_exp_f32),
 // far inside the 1e-4 gradient tolerance.  Measured against the correctly

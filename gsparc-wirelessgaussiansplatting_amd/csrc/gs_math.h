@@ -125,8 +125,7 @@ GS_HD float gs_expf_nonpos(float x)
 // Bit-identical to gs_expf for every x in [-86, 0] (checked by the test
 // suite); outside that domain the result is unspecified (finite or not, no
 // trap).  The blend kernels use exp(power) only where power >= thr =
-// -log(255 o) - 1e-3, which is >= -86 for every opacity below 1e34; lanes
-// below thr discard the value.
+// gs_alpha_thr(o) >= -86; lanes below thr discard the value.
 #define GS_EXP_MAGIC 12582912.0f   // 1.5 * 2^23
 GS_HD float gs_expf_blend(float x)
 {
@@ -143,6 +142,34 @@ GS_HD float gs_expf_blend(float x)
     // t's bit pattern is 0x4B400000 + n; shifted left by 23 the constant part
     // drops out, leaving the exponent field of 2^n
     return p * gs_as_float((gs_as_uint(t) << 23) + 0x3F800000u);
+}
+
+// The blend's skip boundary for opacity o: the smallest float p in [-86, 0]
+// with o * gs_expf_blend(p) >= 1/255 in float32 (the test of forward.cu:219-221
+// as the blend kernels evaluate it).  gs_expf_blend is non-increasing as its
+// argument decreases over all of [-86, 0] (checked exhaustively by the test
+// suite) and rounding the product preserves order, so for every power <= 0
+//     power >= gs_alpha_thr(o)   <=>   o * gs_expf_blend(power) >= 1/255
+// and the forward and backward blends decide with this one comparison.
+// Bisection over the bit patterns of the magnitude (31 steps).  Returns +1
+// (above every power <= 0) when even power = 0 stays below 1/255 (o < 1/255,
+// o <= 0), -86 when every power of the domain passes (o above ~1e35: the
+// far tail below -86 is then skipped), and NaN for a NaN opacity (every
+// comparison with it is false, so nothing is skipped, as alpha =
+// min(0.99, NaN) = 0.99 would not be).
+GS_HD float gs_alpha_thr(float o)
+{
+    const float amin = 1.0f / 255.0f;
+    if (o != o) return o;
+    if (!(o * gs_expf_blend(0.0f) >= amin)) return 1.0f;
+    if (o * gs_expf_blend(-86.0f) >= amin) return -86.0f;
+    uint32_t lo = 0u, hi = gs_as_uint(86.0f);   // magnitude bits: lo passes, hi does not
+    while (hi - lo > 1u) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if (o * gs_expf_blend(-gs_as_float(mid)) >= amin) lo = mid;
+        else hi = mid;
+    }
+    return -gs_as_float(lo);
 }
 
 GS_HD float gs_logf(float x)

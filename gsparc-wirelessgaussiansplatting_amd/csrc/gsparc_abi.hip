@@ -196,6 +196,17 @@ int check_dims(int P, int R, int W, int H)
     return GSPARC_OK;
 }
 
+// The device keeps K in 32 bits, saturated at 2^32 - 1 ("that many or more",
+// k_binning.hip); a capacity must stay below it so that a saturated count
+// always reads as an overflow.
+constexpr uint64_t K_SATURATED = 0xffffffffull;
+
+int check_capacity(uint64_t K_capacity)
+{
+    if (K_capacity >= K_SATURATED) return fail(GSPARC_ERR_ARG, "K_capacity must be below 2^32 - 1");
+    return GSPARC_OK;
+}
+
 // ---- per-kernel timing (HIP events on the launch stream), off by default
 struct ProfPending {
     const char* name;
@@ -391,6 +402,7 @@ int forward_render_impl(const gs::Dims& d, const float* emission, void* geom, vo
 {
     if (!geom || !image || !bin || (d.R > 0 && (!out_real || !out_imag || (d.P > 0 && !radii))))
         return fail(GSPARC_ERR_ARG, "null pointer");
+    if (int e = check_capacity(K_capacity)) return e;
     if (d.R == 0) return GSPARC_OK;
     gsparc_layout L;
     make_layout(d, K_capacity, L);
@@ -438,6 +450,7 @@ int backward_impl(int grad_mode, const gs::Dims& d, const FwdInputs& in, const f
         return fail(GSPARC_ERR_ARG, "null pointer");
     if (((uintptr_t)in.mlp & 15) != 0 || ((uintptr_t)out.dmlp & 15) != 0)
         return fail(GSPARC_ERR_ARG, "emission_mlps and its gradient must be 16-byte aligned");
+    if (int e = check_capacity(K_capacity)) return e;
     gsparc_layout L;
     make_layout(d, K_capacity, L);
     char* gb = base_of(geom);
@@ -507,6 +520,7 @@ int forward_render_csi_impl(const gs::Dims& d, const GsCsi& csi, const float* em
 {
     if (!geom || !image || !bin || (d.R > 0 && (!out || (d.P > 0 && !radii))))
         return fail(GSPARC_ERR_ARG, "null pointer");
+    if (int e = check_capacity(K_capacity)) return e;
     if (d.R == 0) return GSPARC_OK;
     gsparc_layout L;
     make_layout(d, K_capacity, L);
@@ -564,6 +578,7 @@ int autotune_impl(int grad_mode, const gs::Dims& d, const float* emission, const
         !choice)
         return fail(GSPARC_ERR_ARG, "null pointer");
     if (d.R == 0 || d.Ptot == 0) return fail(GSPARC_ERR_ARG, "nothing to tune");
+    if (int e = check_capacity(K_capacity)) return e;
     if (g_prof_on || g_prof_count) return fail(GSPARC_ERR_ARG, "disable profiling before tuning");
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     GS_HIP(hipStreamIsCapturing(s, &cs));
@@ -702,7 +717,10 @@ int gsparc_rasterize_forward_geom(int P, int R, int W, int H, const float* means
                        scale_modifier};
     hipStream_t s = (hipStream_t)stream;
     if (int e = forward_geom_impl(d, in, geom, image, radii_out, s)) return e;
-    return read_count_impl(d, image, K_out, s);
+    if (int e = read_count_impl(d, image, K_out, s)) return e;
+    if (*K_out >= K_SATURATED)
+        return fail(GSPARC_ERR_CAPACITY, "the tile lists would hold 2^32 - 1 or more instances");
+    return GSPARC_OK;
 }
 
 int gsparc_instance_count(int P, int R, int W, int H, const void* image, uint64_t* K, void* stream)
@@ -855,7 +873,10 @@ int gsparc_rasterize_forward_geom_scenes(const gsparc_batch* batch, const uint32
                        scale_modifier};
     hipStream_t s = (hipStream_t)stream;
     if (int e = forward_geom_impl(d, in, geom, image, radii_out, s)) return e;
-    return read_count_impl(d, image, K_out, s);
+    if (int e = read_count_impl(d, image, K_out, s)) return e;
+    if (*K_out >= K_SATURATED)
+        return fail(GSPARC_ERR_CAPACITY, "the tile lists would hold 2^32 - 1 or more instances");
+    return GSPARC_OK;
 }
 
 int gsparc_rasterize_forward_render_scenes(const gsparc_batch* batch, const uint32_t* scene_table, int W, int H,

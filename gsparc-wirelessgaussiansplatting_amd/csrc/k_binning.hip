@@ -50,24 +50,47 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
     return before + x - v;
 }
 
+// K of a workgroup's per-thread partial sums, summed in 64 bits (a thread's
+// sum of up to 15 counts below 2^31 can pass 2^32 itself) and saturated:
+// 2^32 - 1 stands for "2^32 - 1 or more", which no capacity admits (the host
+// keeps capacities below it), so an oversized batch reads as an overflow
+// instead of a wrapped count.  Every thread gets the value.
+__device__ __forceinline__ uint32_t block_total_saturated(uint64_t v, uint64_t* s_wave64)
+{
+    const int wave = threadIdx.x / GS_WAVE, lane = threadIdx.x % GS_WAVE;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    if (lane == 0) s_wave64[wave] = v;
+    __syncthreads();
+    uint64_t t = 0;
+    for (int w = 0; w < (int)(blockDim.x / GS_WAVE); ++w) t += s_wave64[w];
+    return t >= 0xffffffffull ? 0xffffffffu : (uint32_t)t;
+}
+
 __global__ void __launch_bounds__(1024) k_segscan(int n, const uint32_t* segcnt, uint32_t* segbase, uint2* ranges,
                                                    uint32_t* total)
 {
     __shared__ uint32_t s_wave[1024 / GS_WAVE];
+    __shared__ uint64_t s_wave64[1024 / GS_WAVE];
     const int per = (n + 1023) / 1024;
     const int i0 = min(n, (int)threadIdx.x * per), i1 = min(n, i0 + per);
     // eight loads at a time (clamped to the thread's range), so that their
     // latencies overlap: one workgroup does the whole scan
     uint32_t sum = 0;
+    uint64_t sum64 = 0;
     for (int i = i0; i < i1; i += 8) {
         uint32_t v[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) v[u] = segcnt[min(i + u, i1 - 1)];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) sum += i + u < i1 ? v[u] : 0u;
+        for (int u = 0; u < 8; ++u) {
+            sum += i + u < i1 ? v[u] : 0u;
+            sum64 += i + u < i1 ? v[u] : 0u;
+        }
     }
     uint32_t tot;
     uint32_t run = block_exclusive_scan<1024>(sum, s_wave, tot);
+    tot = block_total_saturated(sum64, s_wave64);
     for (int i = i0; i < i1; i += 8) {
         uint32_t v[8];
 #pragma unroll
@@ -90,7 +113,8 @@ __global__ void __launch_bounds__(1024) k_segscan(int n, const uint32_t* segcnt,
 // into LDS, the per-thread runs scanned there, coalesced stores.  Run by a
 // 1024-thread workgroup.
 __device__ __forceinline__ void segscan_lds_body(int n, const uint32_t* segcnt, uint32_t* segbase, uint2* ranges,
-                                                 uint32_t* total, uint32_t* s_v, uint32_t* s_wave)
+                                                 uint32_t* total, uint32_t* s_v, uint32_t* s_wave,
+                                                 uint64_t* s_wave64)
 {
     constexpr int PER = GS_SEGSCAN_LDS / 1024;
     const int tid = (int)threadIdx.x;
@@ -104,9 +128,14 @@ __device__ __forceinline__ void segscan_lds_body(int n, const uint32_t* segcnt, 
     const int per = (n + 1023) / 1024;
     const int i0 = min(n, tid * per), i1 = min(n, i0 + per);
     uint32_t sum = 0;
-    for (int i = i0; i < i1; ++i) sum += s_v[i];
+    uint64_t sum64 = 0;
+    for (int i = i0; i < i1; ++i) {
+        sum += s_v[i];
+        sum64 += s_v[i];
+    }
     uint32_t tot;
     uint32_t run = block_exclusive_scan<1024>(sum, s_wave, tot);
+    tot = block_total_saturated(sum64, s_wave64);
     for (int i = i0; i < i1; ++i) {
         const uint32_t c = s_v[i];
         s_v[i] = run;
@@ -131,7 +160,8 @@ __global__ void __launch_bounds__(1024) k_segscan_lds(int n, const uint32_t* seg
 {
     __shared__ uint32_t s_v[GS_SEGSCAN_LDS];
     __shared__ uint32_t s_wave[1024 / GS_WAVE];
-    segscan_lds_body(n, segcnt, segbase, ranges, total, s_v, s_wave);
+    __shared__ uint64_t s_wave64[1024 / GS_WAVE];
+    segscan_lds_body(n, segcnt, segbase, ranges, total, s_v, s_wave, s_wave64);
 }
 
 // ---------------------------------------------------------------- depth sort
@@ -752,8 +782,9 @@ __global__ void __launch_bounds__(1024) k_segscan_order(int n, const uint32_t* s
 {
     __shared__ uint32_t s_v[GS_SEGSCAN_LDS];
     __shared__ uint32_t s_wave[1024 / GS_WAVE];
+    __shared__ uint64_t s_wave64[1024 / GS_WAVE];
     if (blockIdx.x == 0) {
-        segscan_lds_body(n, segcnt, segbase, ranges, total, s_v, s_wave);
+        segscan_lds_body(n, segcnt, segbase, ranges, total, s_v, s_wave, s_wave64);
         return;
     }
     tile_order_body([segcnt](size_t rt) { return segcnt[rt]; }, R, T, stride, order, total, mode, len,

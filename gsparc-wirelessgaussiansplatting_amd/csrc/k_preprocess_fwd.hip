@@ -8,8 +8,10 @@
 // group.  Wave w then takes receivers w*GS_PF_RW .. w*GS_PF_RW+GS_PF_RW-1 of
 // the group, one Gaussian per lane: the emission MLP from the LDS copy, the
 // projection, and the per-(Gaussian, receiver) records.  Receiver-independent
-// work (cov3D, the log of the transmitter distance, the opacity threshold) is
-// recomputed by each wave for its lane's Gaussian.
+// work (cov3D, the log of the transmitter distance) is recomputed by each wave
+// for its lane's Gaussian; the opacity's skip threshold (a 31-step bisection)
+// is found once per workgroup by wave 0 during the staging and shared
+// through LDS.
 #include "gs_device.h"
 #include "gs_kernels.h"
 
@@ -46,6 +48,7 @@ __global__ void __launch_bounds__(GS_PF_THREADS) k_preprocess_fwd(Dims d, PreFwd
     __shared__ float s_rx[GS_RC * 3];
     __shared__ float s_view[GS_RC * 16];
     __shared__ float4 s_w[GS_PF_G * GS_PF_ROW];
+    __shared__ float s_thr[GS_PF_G];
     for (int i = tid; i < GS_RC * 16; i += GS_PF_THREADS) {
         if (i < GS_RC * 3) s_rx[i] = i < 3 * grp.nr ? a.rx[3 * grp.r0 + i] : 0.0f;
         s_view[i] = i < 16 * grp.nr ? a.views[16 * grp.r0 + i] : 0.0f;
@@ -93,6 +96,8 @@ __global__ void __launch_bounds__(GS_PF_THREADS) k_preprocess_fwd(Dims d, PreFwd
         }
         if (tid < GS_PF_G) s_w[tid * GS_PF_ROW + 48] = make_float4(b2.x, b2.y, 0.0f, 0.0f);
     }
+    // alpha = o*G >= 1/255 exactly when power >= thr (gs_alpha_thr)
+    if (wave == 0) s_thr[lane] = gs_alpha_thr(o);
     __syncthreads();
     const int c0 = wave * GS_PF_RW;
     if (c0 >= grp.nr) return;                                  // no barrier below
@@ -108,8 +113,7 @@ __global__ void __launch_bounds__(GS_PF_THREADS) k_preprocess_fwd(Dims d, PreFwd
         const float dist = sqrtf(df[0] * df[0] + df[1] * df[1] + df[2] * df[2]);
         ld = gs_logf(dist + 1e-6f);
     }
-    // alpha = o*G < 1/255 for every power < thr (see gs_device.h)
-    const float thr = -gs_logf(255.0f * o) - 1e-3f;
+    const float thr = s_thr[lane];
 
     // ---- emission MLP 3->32->2 for this wave's receivers (forward.cu:17-75):
     // four hidden units per step, dot products as fused multiply-adds in

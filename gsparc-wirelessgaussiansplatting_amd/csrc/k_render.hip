@@ -14,9 +14,12 @@
 // all of its pixels are saturated.  Work-group ids
 // are mapped so that all waves of a tile run on the same XCD (shared L2).
 //
-// Per pair the exponential is skipped when power < thr: then
-// opacity * exp(power) < 1/255 is guaranteed (thr carries a 1e-3 margin), so
-// the decision is the same as evaluating it.
+// Per pair the decision alpha >= 1/255 (forward.cu:219-221) is the comparison
+// power >= thr, thr = gs_alpha_thr(opacity) from the preprocess: exactly the
+// powers whose o * gs_expf_blend(power) reaches 1/255.  The forward evaluates
+// the exponential only there; the backward takes the same comparison, so its
+// contributors are the forward's whatever its faster exponential rounds to
+// (backward.cu:560-563 re-tests alpha with the forward's own exp).
 #include "gs_blend.h"
 #include "gs_cull.h"
 #include "gs_device.h"
@@ -32,7 +35,10 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
     __shared__ float2 ss[GS_WAVE];
     const Pixels<PPL> pm = map_pixels<PPL>(d, a.order, (int)a.total[1]);
     if (!pm.valid) return;
-    if ((uint64_t)a.total[0] > a.Kcap) return;   // binning overflow, reported by the host
+    if ((uint64_t)a.total[0] > a.Kcap) {   // binning overflow, reported by the host
+        overflow_outputs<PPL>(d, pm, a.out_re, a.out_im, 1, a.finalT, a.ncontrib);
+        return;
+    }
     bool any_inside = false;
 #pragma unroll
     for (int k = 0; k < PPL; ++k) any_inside |= pm.inside[k];
@@ -91,9 +97,9 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
                 if (COUNT) cnt[k] += T[k] > 0.0f ? 1u : 0u;
                 const float dx = A.x - pxf[k], dy = A.y - pyf[k];
                 const float power = pair_power(A.z, A.w, B.x, dx, dy);
+                // power >= thr: alpha >= 1/255 (gs_alpha_thr)
                 if (power > 0.0f || power < B.z) continue;
                 const float alpha = fminf(0.99f, B.y * gs_expf_blend(power));
-                if (alpha < 1.0f / 255.0f) continue;
                 const float test_T = T[k] * (1.0f - alpha);
                 if (test_T < 0.0001f) { T[k] = -fabsf(T[k]); continue; }
                 const float w = alpha * T[k];
@@ -136,7 +142,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
     __shared__ uint32_t sid[GS_WAVE];
     const Pixels<PPL> pm = map_pixels<PPL>(d, a.order, (int)a.total[1]);
     if (!pm.valid) return;
-    if ((uint64_t)a.total[0] > a.Kcap) return;
+    if ((uint64_t)a.total[0] > a.Kcap) return;   // no lists: the gradient stays cleared
     const int r = pm.r;
     const int lane = threadIdx.x;
     float T[PPL], gre[PPL], gim[PPL], pxf[PPL], pyf[PPL];
@@ -213,15 +219,15 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
                 if (COUNT) cnt += active ? 1u : 0u;
                 const float dx = A.x - pxf[k], dy = A.y - pyf[k];
                 const float power = pair_power(A.z, A.w, B.x, dx, dy);
-                const bool in = active && !(power > 0.0f) && !(power < B.z);
+                // the forward's decision: power in [thr, 0] <=> alpha >= 1/255
+                const bool c = active && !(power > 0.0f) && !(power < B.z);
                 // a uniform branch: the body runs for the whole wave, selects
                 // keep the state of the lanes whose pair does not contribute
-                if (ballot(in) == 0) continue;
+                if (ballot(c) == 0) continue;
                 // G = 0 off [thr, 0], so no out-of-range value reaches the sums
-                const float G = in ? gs_expf_grad(power) : 0.0f;
+                const float G = c ? gs_expf_grad(power) : 0.0f;
                 const float oG = B.y * G;
                 const float alpha = fminf(0.99f, oG);
-                const bool c = in && !(alpha < 1.0f / 255.0f);
                 // T before this Gaussian: T_after / (1 - alpha) (backward.cu:565)
                 T[k] = c ? T[k] * __builtin_amdgcn_rcpf(1.0f - alpha) : T[k];
                 const float w = (c ? alpha : 0.0f) * T[k];

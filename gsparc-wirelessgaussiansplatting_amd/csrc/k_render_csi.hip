@@ -52,7 +52,10 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd_csi(Dims d, RenderCsiArg
     __shared__ float4 se[CSI_BATCH][CSI_NC2];
     const Pixels<PPL> pm = map_pixels<PPL>(d, a.order, (int)a.total[1]);
     if (!pm.valid) return;
-    if ((uint64_t)a.total[0] > a.Kcap) return;
+    if ((uint64_t)a.total[0] > a.Kcap) {   // binning overflow, reported by the host
+        overflow_outputs<PPL>(d, pm, a.out, nullptr, 2 * GS_CSI_NC, a.finalT, a.ncontrib);
+        return;
+    }
     bool any_inside = false;
 #pragma unroll
     for (int q = 0; q < PPL; ++q) any_inside |= pm.inside[q];
@@ -121,9 +124,9 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd_csi(Dims d, RenderCsiArg
                 w[q] = 0.0f;
                 const float dx = A.x - pxf[q], dy = A.y - pyf[q];
                 const float power = pair_power(A.z, A.w, B.x, dx, dy);
+                // power >= thr: alpha >= 1/255 (gs_alpha_thr)
                 if (power > 0.0f || power < B.z) continue;
                 const float alpha = fminf(0.99f, B.y * gs_expf_blend(power));
-                if (alpha < 1.0f / 255.0f) continue;
                 const float test_T = T[q] * (1.0f - alpha);
                 if (test_T < 0.0001f) { T[q] = -fabsf(T[q]); continue; }
                 w[q] = alpha * T[q];
@@ -194,7 +197,10 @@ k_render_fwd_csi_mfma(Dims d, RenderCsiArgs a)
     __shared__ float sW[GS_WAVE][CSI_WW];
     const Pixels<1> pm = map_pixels<1>(d, a.order, (int)a.total[1]);
     if (!pm.valid) return;
-    if ((uint64_t)a.total[0] > a.Kcap) return;
+    if ((uint64_t)a.total[0] > a.Kcap) {   // binning overflow, reported by the host
+        overflow_outputs<1>(d, pm, a.out, nullptr, 2 * GS_CSI_NC, a.finalT, a.ncontrib);
+        return;
+    }
     if (ballot(pm.inside[0]) == 0) return;
     const int r = pm.r;
     const int lane = threadIdx.x;
@@ -262,17 +268,16 @@ k_render_fwd_csi_mfma(Dims d, RenderCsiArgs a)
             float w = 0.0f;
             const float dx = Ak.x - pxf, dy = Ak.y - pyf;
             const float power = pair_power(Ak.z, Ak.w, Bk.x, dx, dy);
+            // power >= thr: alpha >= 1/255 (gs_alpha_thr)
             if (!(power > 0.0f || power < Bk.z)) {
                 const float alpha = fminf(0.99f, Bk.y * gs_expf_blend(power));
-                if (!(alpha < 1.0f / 255.0f)) {
-                    const float test_T = T * (1.0f - alpha);
-                    if (test_T < 0.0001f) {
-                        T = -fabsf(T);
-                    } else {
-                        w = alpha * T;
-                        T = test_T;
-                        last = base + (uint32_t)sjj[k];
-                    }
+                const float test_T = T * (1.0f - alpha);
+                if (test_T < 0.0001f) {
+                    T = -fabsf(T);
+                } else {
+                    w = alpha * T;
+                    T = test_T;
+                    last = base + (uint32_t)sjj[k];
                 }
             }
             sW[lane][k] = w;
@@ -399,13 +404,13 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd_csi(Dims d, RenderCsiArg
             const bool active = (top - j) < last;
             const float dx = A.x - pxf, dy = A.y - pyf;
             const float power = pair_power(A.z, A.w, B.x, dx, dy);
-            bool c = active && !(power > 0.0f) && !(power < B.z);
+            // the forward's decision: power in [thr, 0] <=> alpha >= 1/255
+            const bool c = active && !(power > 0.0f) && !(power < B.z);
             float G = 0.0f, oG = 0.0f, alpha = 0.0f;
             if (c) {
                 G = gs_expf_grad(power);
                 oG = B.y * G;
                 alpha = fminf(0.99f, oG);
-                c = !(alpha < 1.0f / 255.0f);
             }
             if (c) {
                 T = T * __builtin_amdgcn_rcpf(1.0f - alpha);

@@ -87,6 +87,19 @@ class GaussianRasterizer(nn.Module):
         return rasterize_gaussians(means3D, means2D, emission_mlps, opacities, scales, rotations, s)
 
 
+def _refuse_torch_capture(what):
+    """The autograd forms below are not for torch.cuda.graph capture: the one
+    capture of a whole autograd step through them ended in a host
+    segmentation fault inside torch's CUDAGraph.capture_end, with its cause
+    not established.  A captured training step is built by
+    gsparc_train.GraphedTrainingStep (GraphedSceneStep for uneven scenes),
+    whose library-owned HIP graph records forward, loss and backward and is
+    tested against the eager step."""
+    if torch.cuda.is_available() and torch.cuda.is_current_stream_capturing():
+        raise RuntimeError(f"{what} cannot be recorded by torch.cuda.graph; capture a training step with "
+                           "gsparc_train.GraphedTrainingStep (or GraphedSceneStep) instead")
+
+
 # ------------------------------------------------------------------ batched
 class BatchedSetting(NamedTuple):
     """R receivers rendered in one launch sequence (no reference counterpart:
@@ -99,15 +112,18 @@ class BatchedSetting(NamedTuple):
     rx_pos: torch.Tensor            # [R,3]
     grad_mode: str = "reference"
     # 0: size the tile lists exactly (one host synchronisation per forward);
-    # > 0: a fixed instance capacity and no host synchronisation, so that a
-    # whole training step can be captured in a HIP graph.  Check the capacity
-    # afterwards with wireless_rasterizer.instance_count().
+    # > 0: a fixed instance capacity and no host synchronisation.  Check the
+    # capacity afterwards with wireless_rasterizer.instance_count(); on an
+    # overflow the spectra are NaN and the gradients zero.  These autograd
+    # forms refuse torch.cuda.graph capture: a captured step is
+    # gsparc_train.GraphedTrainingStep.
     capacity: int = 0
 
 
 class _RasterizeBatched(torch.autograd.Function):
     @staticmethod
     def forward(ctx, means3D, emission_mlps, emission, opacities, scales, rotations, settings):
+        _refuse_torch_capture("rasterize_gaussians_batched")
         s = settings
         K, ore, oim, radii, geom, binbuf, img = wr.forward_batched(
             means3D, emission_mlps, opacities, scales, rotations, s.image_height, s.image_width, s.scale_modifier,
@@ -159,6 +175,7 @@ class SceneSetting(NamedTuple):
 class _RasterizeScenes(torch.autograd.Function):
     @staticmethod
     def forward(ctx, means3D, emission_mlps, emission, opacities, scales, rotations, settings):
+        _refuse_torch_capture("rasterize_scenes")
         s = settings
         K, ore, oim, radii, geom, binbuf, img = wr.forward_scenes(
             means3D, emission_mlps, opacities, scales, rotations, s.image_height, s.image_width, s.scale_modifier,
@@ -211,6 +228,7 @@ class CsiSetting(NamedTuple):
 class _RasterizeCsi(torch.autograd.Function):
     @staticmethod
     def forward(ctx, means3D, emission_mlps, emission, opacities, scales, rotations, settings):
+        _refuse_torch_capture("rasterize_csi")
         s = settings
         K, out, radii, geom, binbuf, img = wr.forward_csi(
             means3D, emission_mlps, opacities, scales, rotations, s.image_height, s.image_width, s.scale_modifier,

@@ -134,10 +134,11 @@ int gsparc_rasterize_forward_render(int P, int R, int W, int H, const float* emi
                                     float* out_imag, int flags, void* stream);
 
 /* The whole forward in one call with no host synchronisation: parts 1 and 2
- * back to back, the binning sized by K_capacity instead of by K.  K stays on
- * the device; if it exceeds K_capacity the tile lists are not written, the
- * blend leaves out_real / out_imag unwritten and the backward does nothing,
- * which gsparc_instance_count() reveals.  Everything it launches can be
+ * back to back, the binning sized by K_capacity (below 2^32 - 1) instead of
+ * by K.  K stays on the device; if it exceeds K_capacity the tile lists are
+ * not written, the blend writes NaN to out_real / out_imag (and final T = 1,
+ * no contributors) and the backward leaves every gradient at zero, which
+ * gsparc_instance_count() reveals.  Everything it launches can be
  * captured in a HIP graph.  No counterpart in the reference (which reads K
  * back before sizing its binning buffer, rasterizer_impl.cu:270-278); for
  * callers that cannot synchronise: graph-captured training steps. */
@@ -148,7 +149,11 @@ int gsparc_rasterize_forward(int P, int R, int W, int H, const float* means, con
                              uint64_t K_capacity, int* radii_out, float* out_real, float* out_imag, int flags,
                              void* stream);
 
-/* K of the last forward into `image` (synchronises the stream). */
+/* K of the last forward into `image` (synchronises the stream).  The device
+ * sums K in 64 bits and keeps it saturated at 2^32 - 1, which stands for
+ * "2^32 - 1 or more": above every admissible capacity, so such a batch reads
+ * as an overflow, and gsparc_rasterize_forward_geom fails with
+ * GSPARC_ERR_CAPACITY on it. */
 int gsparc_instance_count(int P, int R, int W, int H, const void* image, uint64_t* K, void* stream);
 
 /* Backward.  grad_mode: GSPARC_GRAD_EXACT or GSPARC_GRAD_REFERENCE.

@@ -1115,7 +1115,7 @@ extern "C" void orc_cull_check(int64_t n, uint64_t seed, float margin, int64_t* 
         const float qa = cyy / det, qb = -cxy / det, qc = cxx / det;   // conic
         const float ha = -0.5f * qa, nb = -qb, hc = -0.5f * qc;
         const float o = (float)std::min(1.0, std::pow(10.0, -2.5 + 2.5 * rng.uni()));
-        const float thr = -gs_logf(255.0f * o) - 1e-3f;
+        const float thr = gs_alpha_thr(o);
         float mx, my;
         if (i & 1) {
             // centre anywhere within 5 sigma of the block
@@ -1154,4 +1154,84 @@ extern "C" void orc_cull_check(int64_t n, uint64_t seed, float margin, int64_t* 
     stats[1] = culled;
     stats[2] = bad;
     stats[3] = allskip;
+}
+
+// ---------------------------------------------------------------------------
+// The blend's skip decision as a power threshold (gs_math.h gs_alpha_thr).
+// The kernels decide alpha >= 1/255 (forward.cu:219-221) by power >= thr in
+// both the forward and the backward; these pin that rule.
+// ---------------------------------------------------------------------------
+extern "C" void orc_alpha_thr(int64_t n, const float* o, float* thr)
+{
+    for (int64_t i = 0; i < n; ++i) thr[i] = gs_alpha_thr(o[i]);
+}
+
+// Exhaustive: gs_expf_blend(x) is non-increasing as x walks down from -0 to
+// -86 through every float.  Returns the number of increases (0 is required).
+extern "C" int64_t orc_expf_blend_monotone(void)
+{
+    const uint32_t top = gs_as_uint(-86.0f);   // sign bit set; larger bits = more negative
+    const uint32_t first = 0x80000000u;
+    const int64_t span = (int64_t)(top - first);
+    const int nchunk = 256;
+    int64_t bad = 0;
+#pragma omp parallel for reduction(+ : bad) schedule(dynamic, 4)
+    for (int c = 0; c < nchunk; ++c) {
+        const uint32_t a = first + (uint32_t)(span * c / nchunk);
+        const uint32_t b = first + (uint32_t)(span * (c + 1) / nchunk);
+        float prev = gs_expf_blend(gs_as_float(a));
+        for (uint32_t u = a + 1; u <= b; ++u) {
+            const float e = gs_expf_blend(gs_as_float(u));
+            bad += e > prev;
+            prev = e;
+        }
+    }
+    return bad;
+}
+
+// For each opacity o: every power within `span` floats of gs_alpha_thr(o)
+// (both sides), plus `nrand` random powers in [-86, 0], decided both ways:
+// power >= thr (the kernels) and !(min(0.99, o * exp(power)) < 1/255) with
+// the oracle's exponential (the reference's test).  stats[0] = pairs
+// checked, stats[1] = disagreements.
+extern "C" void orc_check_alpha_thr(int64_t n, const float* o, int span, int nrand, uint64_t seed, int64_t* stats)
+{
+    int64_t checked = 0, bad = 0;
+#pragma omp parallel for reduction(+ : checked, bad) schedule(dynamic, 16)
+    for (int64_t i = 0; i < n; ++i) {
+        SplitMix rng{seed + 0x9e3779b97f4a7c15ull * (uint64_t)(i + 1)};
+        const float oi = o[i];
+        const float thr = gs_alpha_thr(oi);
+        auto decide = [&](float p) {
+            if (p > 0.0f) return;
+            const bool kern = !(p < thr);
+            const bool ref = !(std::min(0.99f, oi * gs_expf(p)) < 1.0f / 255.0f);
+            ++checked;
+            bad += kern != ref;
+        };
+        if (thr <= 0.0f && thr == thr) {
+            const uint32_t tb = gs_as_uint(-thr);   // magnitude bits of thr
+            for (int k = -span; k <= span; ++k) {
+                const int64_t m = (int64_t)tb + k;
+                if (m < 0 || m > (int64_t)gs_as_uint(86.0f)) continue;
+                decide(-gs_as_float((uint32_t)m));
+            }
+        }
+        for (int k = 0; k < nrand; ++k) decide((float)(-86.0 * rng.uni()));
+        decide(0.0f);
+        decide(-0.0f);
+    }
+    stats[0] = checked;
+    stats[1] = bad;
+}
+
+// The blend's power of Gaussian i (conic_o, xy as from the preprocess) at
+// pixel (px[i], py[i]), evaluated as the kernels and the FP32 mirror do.
+extern "C" void orc_pair_power_f32(int64_t n, const float* conic_o, const float* xy, const int* px, const int* py,
+                                   float* out)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        const float dx = xy[2 * i] - (float)px[i], dy = xy[2 * i + 1] - (float)py[i];
+        out[i] = pair_power(conic_o[4 * i], conic_o[4 * i + 1], conic_o[4 * i + 2], dx, dy);
+    }
 }

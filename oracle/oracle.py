@@ -50,6 +50,42 @@ def math_vec(which, x, y=None):
     return out
 
 
+def alpha_thr(o):
+    """gs_alpha_thr over an array of opacities (float32)."""
+    o = np.ascontiguousarray(o, np.float32)
+    out = np.empty_like(o)
+    lib().orc_alpha_thr(C.c_int64(o.size), _p(o), _p(out))
+    return out
+
+
+def expf_blend_monotone_violations():
+    """Exhaustive: increases of gs_expf_blend as its argument walks down
+    through every float from -0 to -86 (0 required)."""
+    f = lib().orc_expf_blend_monotone
+    f.restype = C.c_int64
+    return int(f())
+
+
+def check_alpha_thr(o, span=256, nrand=256, seed=0):
+    """(pairs checked, disagreements) between power >= gs_alpha_thr(o) and
+    the alpha >= 1/255 test, on the `span` floats either side of each
+    threshold and `nrand` random powers."""
+    o = np.ascontiguousarray(o, np.float32)
+    stats = np.zeros(2, np.int64)
+    lib().orc_check_alpha_thr(C.c_int64(o.size), _p(o), C.c_int(span), C.c_int(nrand), C.c_uint64(seed), _p(stats))
+    return int(stats[0]), int(stats[1])
+
+
+def pair_power(pre, px, py):
+    """Power of each Gaussian of `pre` at pixel (px[i], py[i]), as the FP32
+    mirror and the kernels evaluate it."""
+    P = pre.radii.shape[0]
+    out = np.zeros(P, np.float32)
+    lib().orc_pair_power_f32(C.c_int64(P), _p(_c(pre.conic_o, np.float32)), _p(_c(pre.xy, np.float32)),
+                             _p(np.ascontiguousarray(px, np.int32)), _p(np.ascontiguousarray(py, np.int32)), _p(out))
+    return out
+
+
 def cull_check(n, seed=0, margin=4.0e-6):
     """Brute-force check of the blend kernels' wave-level culling test:
     [cases, culled, violations, cases where every pixel skips]."""

@@ -77,6 +77,42 @@ def test_expf_blend_is_bit_identical_on_its_domain():
     np.testing.assert_array_equal(O.math_vec("exp_blend", x).view(np.uint32), O.math_vec("exp", x).view(np.uint32))
 
 
+def test_expf_blend_is_monotone_on_its_whole_domain():
+    """Exhaustive over the 1.1e9 floats of [-86, 0]: the blend exponential
+    never increases as its argument decreases.  That is what makes the skip
+    decision o * exp(power) >= 1/255 a single threshold on the power."""
+    assert O.expf_blend_monotone_violations() == 0
+
+
+def test_alpha_threshold_decides_exactly_like_the_alpha_test():
+    """power >= gs_alpha_thr(o) (both blend kernels) against the reference's
+    !(min(0.99, o exp(power)) < 1/255) (forward.cu:219-221) with the oracle's
+    exponential: every float within 256 of each threshold and random powers,
+    for random and edge-case opacities."""
+    rng = np.random.default_rng(5)
+    o = 1.0 / (1.0 + np.exp(-rng.normal(size=4000)))
+    a = np.float32(1.0 / 255.0)
+    edge = [a, np.nextafter(a, np.float32(1)), np.nextafter(a, np.float32(0)), 1.0, 0.99, 0.5, 1e-3, 0.0, -0.5,
+            1e30, 1e36, 2.0, 254.9, 255.0, 1e-30]
+    o = np.concatenate([o, np.exp(rng.uniform(np.log(a), 0.0, 4000)), edge]).astype(np.float32)
+    checked, bad = O.check_alpha_thr(o, span=256, nrand=256, seed=7)
+    assert checked > 3_000_000 and bad == 0
+    thr = O.alpha_thr(o)
+    # the threshold is the boundary itself: thr passes, the next float below
+    # fails (a threshold one float off in either direction breaks one of the two)
+    ok = (o >= a) & (o < 1e34)
+    t = thr[ok]
+    below = np.nextafter(t, np.float32(-np.inf))
+    ob = o[ok]
+    e_at = O.math_vec("exp", t)
+    e_below = O.math_vec("exp", below)
+    assert np.all(np.minimum(np.float32(0.99), ob * e_at) >= a)
+    assert np.all(np.minimum(np.float32(0.99), ob * e_below) < a)
+    # special values
+    s = O.alpha_thr(np.array([np.nan, 0.0, -1.0, 1e-3, 1e36], np.float32))
+    assert np.isnan(s[0]) and s[1] == 1.0 and s[2] == 1.0 and s[3] == 1.0 and s[4] == -86.0
+
+
 def test_library_exports_every_declared_symbol():
     header = open(os.path.join(ROOT, "include", "gsparc.h")).read()
     declared = set(re.findall(r"\b(gsparc_\w+)\s*\(", header))
@@ -175,3 +211,21 @@ def test_argument_checks_before_any_device_work():
     assert b"W + H too large" in L.gsparc_last_error()
     assert L.gsparc_instance_count(10, 1, 360, 90, None, C.byref(K), None) != 0
     assert b"null pointer" in L.gsparc_last_error()
+    # the device keeps K saturated at 2^32 - 1: a capacity must stay below it
+    dummy = C.c_void_p(256)
+    assert L.gsparc_rasterize_forward_render(10, 1, 360, 90, None, dummy, dummy, dummy, (1 << 32) - 1, dummy, dummy,
+                                             dummy, 0, None) != 0
+    assert b"K_capacity must be below 2^32 - 1" in L.gsparc_last_error()
+
+
+def test_autograd_forms_refuse_torch_graph_capture(monkeypatch):
+    """The batched, scene and CSI autograd forwards refuse torch.cuda.graph
+    capture before any device work, pointing to the library-owned graph."""
+    import pytest
+    import torch
+    import cuda_rasterizer as CR
+    monkeypatch.setattr(torch.cuda, "is_available", lambda: True)
+    monkeypatch.setattr(torch.cuda, "is_current_stream_capturing", lambda: True)
+    for fn in (CR.rasterize_gaussians_batched, CR.rasterize_scenes, CR.rasterize_csi):
+        with pytest.raises(RuntimeError, match="GraphedTrainingStep"):
+            fn(torch.zeros(1, 3), None, torch.zeros(1), torch.ones(1, 3), torch.ones(1, 4), None)
