@@ -1039,16 +1039,27 @@ size_t gsparc_loss_work_size(int planes, int H, int W)
     return 256 + al(3 * n * sizeof(float)) + 2 * sizeof(float) * (size_t)gs::ssim_blocks(planes, H, W);
 }
 
-int gsparc_loss_l1_ssim(int planes, int H, int W, float lambda_ssim, float C1, float C2, const float* img,
-                        const float* target, void* work, float* loss, float* dL_dimg, void* stream)
+}  // extern "C"
+
+namespace {
+
+int loss_args(int planes, int H, int W, const float* img, const float* target, float* loss)
 {
     if (planes < 0 || H < 0 || W < 0) return fail(GSPARC_ERR_ARG, "negative size");
     if (planes > 65535) return fail(GSPARC_ERR_ARG, "at most 65535 planes per call");
+    if ((size_t)planes * H * W == 0) return fail(GSPARC_ERR_ARG, "empty images");
+    if (!img || !target || !loss) return fail(GSPARC_ERR_ARG, "null pointer");
+    return GSPARC_OK;
+}
+
+// The loss's launch sequence on a 256-byte aligned scratch area of at least
+// gsparc_loss_work_size - 256 bytes: the three partial-derivative maps, then
+// the per-workgroup loss partials.  Every byte it reads is written earlier in
+// the same call, so the area needs no clearing.
+int loss_l1_ssim_impl(int planes, int H, int W, float lambda_ssim, float C1, float C2, const float* img,
+                      const float* target, char* base, float* loss, float* dL_dimg, hipStream_t s)
+{
     const size_t n = (size_t)planes * H * W;
-    if (n == 0) return fail(GSPARC_ERR_ARG, "empty images");
-    if (!img || !target || !work || !loss) return fail(GSPARC_ERR_ARG, "null pointer");
-    hipStream_t s = (hipStream_t)stream;
-    char* base = base_of(work);
     float* dmu1 = dL_dimg ? reinterpret_cast<float*>(base) : nullptr;
     float* ds1 = dL_dimg ? dmu1 + n : nullptr;
     float* ds12 = dL_dimg ? ds1 + n : nullptr;
@@ -1065,6 +1076,68 @@ int gsparc_loss_l1_ssim(int planes, int H, int W, float lambda_ssim, float C1, f
         GS_LAUNCH("ssim_bwd", gs::launch_ssim_bwd(b, s), s);
     }
     return GSPARC_OK;
+}
+
+}  // namespace
+
+// Scratch of gsparc_loss_l1_ssim_noalloc: one array per GPU, created with the
+// library's code object on each device and held until the process ends.  In
+// host code `scratch` names a host-side stand-in, never the device memory:
+// its device address is looked up per device (scratch_ptr) and only that
+// address is passed to kernels.
+static __device__ __align__(256) unsigned char scratch[GSPARC_LOSS_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of the current device's `scratch`, looked up on the first
+// call on that device (hipGetSymbolAddress with the device current).
+int scratch_ptr(char** out)
+{
+    int dev = -1;
+    GS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(GSPARC_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        GS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(GSPARC_ERR_HIP, "hipGetSymbolAddress returned no address for the loss scratch");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = (char*)g_scratch_ptr[dev];
+    return GSPARC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gsparc_loss_l1_ssim(int planes, int H, int W, float lambda_ssim, float C1, float C2, const float* img,
+                        const float* target, void* work, float* loss, float* dL_dimg, void* stream)
+{
+    if (int e = loss_args(planes, H, W, img, target, loss)) return e;
+    if (!work) return fail(GSPARC_ERR_ARG, "null pointer");
+    return loss_l1_ssim_impl(planes, H, W, lambda_ssim, C1, C2, img, target, base_of(work), loss, dL_dimg,
+                             (hipStream_t)stream);
+}
+
+int gsparc_loss_l1_ssim_noalloc(int planes, int H, int W, float lambda_ssim, float C1, float C2, const float* img,
+                                const float* target, float* loss, float* dL_dimg, void* stream)
+{
+    if (int e = loss_args(planes, H, W, img, target, loss)) return e;
+    const size_t need = gsparc_loss_work_size(planes, H, W) - ALIGN;   // the scratch is aligned already
+    if (need > GSPARC_LOSS_SCRATCH_BYTES)
+        return fail(GSPARC_ERR_ARG, "planes x H x W needs " + std::to_string(need) +
+                                        " bytes of loss scratch; the no-allocation loss holds " +
+                                        std::to_string(GSPARC_LOSS_SCRATCH_BYTES) +
+                                        " (use gsparc_loss_l1_ssim with a workspace)");
+    char* base = nullptr;
+    if (int e = scratch_ptr(&base)) return e;
+    return loss_l1_ssim_impl(planes, H, W, lambda_ssim, C1, C2, img, target, base, loss, dL_dimg,
+                             (hipStream_t)stream);
 }
 
 size_t gsparc_csi_loss_work_size(int R, int H, int W)

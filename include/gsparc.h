@@ -270,6 +270,23 @@ size_t gsparc_loss_work_size(int planes, int H, int W);
 int gsparc_loss_l1_ssim(int planes, int H, int W, float lambda_ssim, float C1, float C2, const float* img,
                         const float* target, void* work, float* loss, float* dL_dimg, void* stream);
 
+/* gsparc_loss_l1_ssim with no workspace argument, calling no allocator: its
+ * scratch (the three SSIM partial-derivative maps and the per-workgroup loss
+ * partials) is a library-owned device array of GSPARC_LOSS_SCRATCH_BYTES on
+ * every GPU the library runs on, held until the process ends; its address is
+ * looked up once per device, on the first call there.  Sizes whose scratch
+ * would not fit (gsparc_loss_work_size(planes, H, W) - 256 bytes above it:
+ * about 2.79M pixels, e.g. 86 spectra of 90 x 360) fail with GSPARC_ERR_ARG.
+ * Every call on one GPU shares that GPU's array and nothing in it needs
+ * clearing, but calls that overlap on one GPU (other streams, other threads)
+ * race: serialise them.  Results are bit-identical to gsparc_loss_l1_ssim's.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code being captured into a HIP graph, serving loops, harnesses
+ * that forbid allocation). */
+#define GSPARC_LOSS_SCRATCH_BYTES ((size_t)32 << 20)
+int gsparc_loss_l1_ssim_noalloc(int planes, int H, int W, float lambda_ssim, float C1, float C2, const float* img,
+                                const float* target, float* loss, float* dL_dimg, void* stream);
+
 /* HIP graphs of this library's launch sequences.  Between begin and end,
  * every call above that does not synchronise (gsparc_rasterize_forward with a
  * capacity, gsparc_rasterize_backward, the SSIM and loss calls, the _scenes
