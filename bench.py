@@ -55,7 +55,12 @@ def parse():
     ap.add_argument("--receivers", type=int, default=64)
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--latency-iters", type=int, default=100)
-    ap.add_argument("--cpu-receivers", type=int, default=48, help="receivers in the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-receivers", type=int, default=64,
+                    help="spectra in the multi-core CPU baseline sample (0: skip the CPU baseline)")
+    ap.add_argument("--cpu-receivers-1", type=int, default=16,
+                    help="spectra in the single-core CPU baseline sample")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the multi-core CPU baseline (0: OMP_NUM_THREADS, at most 16)")
     ap.add_argument("--grad-mode", default="reference", choices=["reference", "exact"])
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run a plain device-to-device copy of this size (counter calibration)")
@@ -124,28 +129,65 @@ def plain_step(params, settings_for, targets, rx_count):
         loss.backward()
 
 
+def cpu_threads(args):
+    """Threads of the multi-core CPU baseline: --cpu-threads, else the OpenMP
+    default (OMP_NUM_THREADS), at most 16 (one GPU's share of the host)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    n = args.cpu_threads if args.cpu_threads > 0 else O.set_threads(0)
+    return max(1, min(int(n), 16))
+
+
+def cpu_baseline(args, run, n_multi, n_single, label):
+    """The CPU oracle timed on a bounded sample: run(n) processes the first n
+    spectra of the sample.  Multi-core over n_multi spectra with
+    cpu_threads(args) OpenMP threads, then one core over n_single spectra."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    nt = cpu_threads(args)
+    res = {}
+    for threads, n in ((nt, n_multi), (1, n_single)):
+        if n <= 0:
+            continue
+        O.set_threads(threads)
+        log(args, f"CPU baseline: oracle fwd+bwd on {n} spectra, {threads} thread(s)")
+        t0 = time.perf_counter()
+        run(n)
+        res[threads] = (n, time.perf_counter() - t0)
+    O.set_threads(nt)
+    n, el = res[nt]
+    out = {"value": round(n / el, 4), "unit": "spectra/s", "cores": nt, "kind": "port",
+           "sample": f"{label}: first {n} spectra, fwd+bwd incl. loss, FP32 C++ oracle, {nt} OpenMP threads "
+                     f"({el:.1f} s)"}
+    if 1 in res and nt != 1:
+        n1, el1 = res[1]
+        out["single_core"] = {"value": round(n1 / el1, 4), "cores": 1,
+                              "sample": f"first {n1} spectra of the same sample, one thread ({el1:.1f} s)"}
+    return out
+
+
 def cpu_oracle_sample(args, scenes_and_rx, targets, label):
-    """The CPU oracle (FP32, one core) on a bounded sample of the same
-    spectra: forward, loss gradient and backward of each sampled spectrum."""
+    """The CPU oracle on a bounded sample of the same spectra: forward, loss
+    gradient and backward of each sampled spectrum (multi-core, then one
+    core on a smaller part of the sample)."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    log(args, f"CPU baseline: oracle fwd+bwd on {len(scenes_and_rx)} spectra, 1 core")
-    t0 = time.perf_counter()
-    accs = {}
-    for (sc, r), tgt in zip(scenes_and_rx, targets):
-        acc = accs.setdefault(id(sc), O.Grads(sc.P))
-        pre, keys, vals, ranges, fwd = O.render(sc, r)
-        img1 = fwd["out_re"][None, None].astype(np.float32)
-        img2 = tgt[None, None]
-        m_, d1, d2, d3 = O.ssim_fwd(img1, img2, dtype=np.float32)
-        gss = O.ssim_bwd(img1, img2, np.full_like(img1, -0.2 / img1.size), d1, d2, d3, dtype=np.float32)
-        dre = (0.8 / img1.size) * np.sign(img1 - img2)[0, 0] + gss[0, 0]
-        O.backward(sc, pre, vals, ranges, fwd, dre, np.zeros_like(dre), O.GRAD_REFERENCE, r=r, grads=acc)
-    el = time.perf_counter() - t0
+
+    def run(n):
+        accs = {}
+        for (sc, r), tgt in list(zip(scenes_and_rx, targets))[:n]:
+            acc = accs.setdefault(id(sc), O.Grads(sc.P))
+            pre, keys, vals, ranges, fwd = O.render(sc, r)
+            img1 = fwd["out_re"][None, None].astype(np.float32)
+            img2 = tgt[None, None]
+            m_, d1, d2, d3 = O.ssim_fwd(img1, img2, dtype=np.float32)
+            gss = O.ssim_bwd(img1, img2, np.full_like(img1, -0.2 / img1.size), d1, d2, d3, dtype=np.float32)
+            dre = (0.8 / img1.size) * np.sign(img1 - img2)[0, 0] + gss[0, 0]
+            O.backward(sc, pre, vals, ranges, fwd, dre, np.zeros_like(dre), O.GRAD_REFERENCE, r=r, grads=acc)
+
     n = len(scenes_and_rx)
-    return {"value": round(n / el, 4), "unit": "spectra/s", "cores": 1, "kind": "port",
-            "sample": f"{label}, fwd+bwd incl. SSIM, FP32 C++ oracle single-threaded ({el:.1f} s)"}
+    return cpu_baseline(args, run, n, min(n, args.cpu_receivers_1), label)
 
 
 def setup_receivers(args, dev):
@@ -372,10 +414,13 @@ def setup_csi(args, dev):
     def cpu():
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
-        nr = min(args.cpu_receivers, R)
+        nr = min(args.cpu_receivers, R, 8)
         tg = target[:nr].cpu().numpy()
-        log(args, f"CPU baseline: CSI oracle fwd+bwd on {nr} receivers, 1 core")
-        t0 = time.perf_counter()
+        return cpu_baseline(args, lambda n: cpu_run(n, tg), nr, min(2, nr),
+                            f"{nr} of the {R} receivers, 52-subcarrier maps")
+
+    def cpu_run(nr, tg):
+        import oracle as O
         acc = O.Grads(P)
         for r in range(nr):
             pre, plen, vals, ranges, fwd = O.render_csi(sc, r)
@@ -389,10 +434,6 @@ def setup_csi(args, dev):
             mag = ma.transpose(0, 2, 3, 1)[..., None]
             dout = (0.8 / x.size) * np.sign(x - y) + np.where(mag > 0, dmag * x / np.where(mag > 0, mag, 1), 0)
             O.backward_csi(sc, pre, plen, vals, ranges, fwd, dout[0].astype(np.float32), r, grads=acc)
-        el = time.perf_counter() - t0
-        return {"value": round(nr / el, 4), "unit": "spectra/s", "cores": 1, "kind": "port",
-                "sample": f"{nr} of the {R} receivers, 52-subcarrier fwd+bwd incl. loss, FP32 C++ oracle "
-                          f"single-threaded ({el:.1f} s)"}
 
     return Workload(
         step=step, step_plain=step_plain, single=single, count=count, cpu=cpu, graphed=None, tune=None,
@@ -529,6 +570,7 @@ def main():
     eager_value = R * args.steps / el_eager
     log(args, f"eager batched: {1e3 * el_eager / args.steps:.2f} ms/step, {eager_value:.1f} spectra/s")
     graph_value = None
+    gstep = None
     if wl.graphed is not None and not args.eager:
         gstep, _live = wl.graphed()
         for _ in range(args.warmup):
@@ -540,8 +582,6 @@ def main():
         el_graph = timed(gstep.replay, args.steps)
         graph_value = R * args.steps / el_graph
         log(args, f"graphed batched: {1e3 * el_graph / args.steps:.2f} ms/step, {graph_value:.1f} spectra/s")
-        gstep.close()
-        del gstep, _live
     el = el_graph if graph_value is not None else el_eager
     ms_per_step = 1e3 * el / args.steps
     value = R * args.steps / el
@@ -578,25 +618,31 @@ def main():
         torch.cuda.synchronize()
         log(args, f"calibration copy of {args.copy_mib} MiB done")
 
-    # per-kernel times with HIP events on the launch stream, then (separately:
-    # the counting variants are slower) the blend kernels' pair counts; after
-    # the host-bound plain and latency runs a few steps first bring the clocks
-    # back to where the timed steps ran
+    # per-kernel times with HIP events on the launch stream around every
+    # kernel of the timed step: the graph's own launch sequence run without
+    # the graph (the same kernels with the same arguments), else the eager
+    # step; then (separately: the counting variants are slower) the blend
+    # kernels' pair counts.  After the host-bound plain and latency runs a few
+    # steps first bring the clocks back to where the timed steps ran.
+    prof_step = gstep.run_eager if gstep is not None else wl.step
     for _ in range(max(3, args.warmup)):
-        wl.step()
+        prof_step()
     torch.cuda.synchronize()
-    nprof = 3
+    nprof = 5
     gh.profile_enable(True, timing=True, counting=False)
     for _ in range(nprof):
-        wl.step()
+        prof_step()
     torch.cuda.synchronize()
     kt = gh.profile_collect()
     gh.profile_enable(True, timing=False, counting=True)
     for _ in range(nprof):
-        wl.step()
+        prof_step()
     torch.cuda.synchronize()
     cnt = {k: v / nprof for k, v in gh.profile_counts().items()}
     gh.profile_enable(False)
+    if gstep is not None:
+        gstep.close()
+        del gstep, _live
     with torch.no_grad():
         K = wl.count()
     N = wl.W * wl.H
@@ -638,10 +684,13 @@ def main():
             "ssim_fwd": ("hbm", 24.0 * NC * N * R, "GB/s"),
             "ssim_bwd": ("hbm", 28.0 * NC * N * R, "GB/s"),
         })
-    # FP32 work of the blend kernels (vector ALU): per evaluated (pixel,
-    # Gaussian) pair and per list entry tested against a wave's pixel block
-    flops = {"render_fwd": cnt["fwd_pairs"] * FWD_FLOP_PER_PAIR + cnt["fwd_entries"] * CULL_FLOP,
-             "render_bwd": cnt["bwd_pairs"] * BWD_FLOP_PER_PAIR + cnt["bwd_entries"] * CULL_FLOP}
+    # The blend kernels are bound by the vector ALU: their algorithmic work is
+    # FP32 operations per evaluated (pixel, Gaussian) pair (the kernels'
+    # counting variants count the pairs), against the published FP32 vector
+    # peak; their HBM rate is kept beside it.  The wave-level entry tests
+    # (culling) are this build's overhead, reported separately.
+    valu = {"render_fwd": (cnt["fwd_pairs"] * FWD_FLOP_PER_PAIR, cnt["fwd_pairs"], cnt["fwd_entries"]),
+            "render_bwd": (cnt["bwd_pairs"] * BWD_FLOP_PER_PAIR, cnt["bwd_pairs"], cnt["bwd_entries"])}
     traffic = measured_traffic(args, *wl.traffic_key) if wl.traffic_key else {}
     kernels = {}
     for name, (ms, n) in kt.items():
@@ -657,23 +706,27 @@ def main():
                 frac = ach / PEAK_HBM_GBS
             entry.update(bound=bound, achieved=round(ach, 3 if unit == "TFLOP/s" else 1), unit=unit,
                          frac=round(frac, 4))
-        if name in flops:
-            entry["fp32_tflops"] = round(flops[name] / (avg * 1e-3) / 1e12, 3)
+        if name in valu:
+            fl, pairs, tests = valu[name]
+            gbs, gfrac = entry.get("achieved"), entry.get("frac")
+            ach = fl / (avg * 1e-3) / 1e12
+            entry.update(bound="valu", achieved=round(ach, 3), unit="TFLOP/s", frac=round(ach / PEAK_FP32_TFLOPS, 4),
+                         pairs_per_launch=pairs, entry_tests_per_launch=tests, hbm_gbs=gbs, hbm_frac=gfrac,
+                         fp32_tflops_incl_entry_tests=round((fl + tests * CULL_FLOP) / (avg * 1e-3) / 1e12, 3))
         if name in traffic:
             entry["traffic"] = traffic[name]
         kernels[name] = entry
     dom = max(kt.items(), key=lambda kv: kv[1][0])[0]
     d = kernels.get(dom, {})
-    roof = {"kernel": dom, "bound": d.get("bound"), "achieved": d.get("achieved"),
-            "peak": PEAK_FP32_MFMA_TFLOPS if d.get("unit") == "TFLOP/s" else PEAK_HBM_GBS,
+    peak = {"TFLOP/s": PEAK_FP32_TFLOPS if d.get("bound") == "valu" else PEAK_FP32_MFMA_TFLOPS}.get(d.get("unit"),
+                                                                                                 PEAK_HBM_GBS)
+    roof = {"kernel": dom, "bound": d.get("bound"), "achieved": d.get("achieved"), "peak": peak,
             "unit": d.get("unit"), "frac": d.get("frac"), "traffic": traffic.get(dom),
             "ms_per_launch": d.get("ms_per_launch")}
-    if dom in flops:
-        side = dom[-3:]
-        roof["fp32_tflops"] = d.get("fp32_tflops")
-        roof["fp32_peak_tflops"] = PEAK_FP32_TFLOPS
-        roof["pairs_per_launch"] = cnt[f"{side}_pairs"]
-        roof["entry_tests_per_launch"] = cnt[f"{side}_entries"]
+    if dom in valu:
+        roof.update(flop_per_pair=BWD_FLOP_PER_PAIR if dom.endswith("bwd") else FWD_FLOP_PER_PAIR,
+                    pairs_per_launch=d["pairs_per_launch"], entry_tests_per_launch=d["entry_tests_per_launch"],
+                    hbm_gbs=d.get("hbm_gbs"), hbm_frac=d.get("hbm_frac"))
     if roof["traffic"] is not None:
         roof["traffic_source"] = TRAFFIC_FILE
 
@@ -708,6 +761,8 @@ def main():
         "autotune": tuned,
         "roofline": roof,
         "kernels": kernels,
+        "kernels_source": ("HIP events around each kernel of the graph's launch sequence, run without the graph"
+                           if graph_value is not None else "HIP events around each kernel of the eager step"),
         "cpu_baseline": cpu,
     }
     if getattr(wl, "mlp_rows", None):

@@ -15,6 +15,13 @@
 // Each function cites the reference file:line it restates (paths relative to
 // the GSpaRC repository root).
 //
+// Threads (OpenMP, orc_set_threads): preprocess and its backward run over
+// Gaussians, the blends over pixel rows dealt round robin, the backward
+// blends into per-thread gradient buffers summed in thread order, the binning
+// through a parallel stable sort, SSIM over rows.  Every output except the
+// backward blends' double-precision sums is identical for any thread count;
+// those sums are deterministic for a given count.
+//
 // Gradient modes of the backward:
 //   GRAD_EXACT (0)      the true adjoint of the forward below.
 //   GRAD_REFERENCE (1)  reproduces the reference backward literally, including
@@ -26,10 +33,13 @@
 //                       gradient (backward.cu:42,91), the opacity clamp being
 //                       ignored (backward.cu:561) and the scale modifier being
 //                       dropped from dL/dscale (backward.cu:353-355).
+#include <omp.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <parallel/algorithm>
 #include <vector>
 
 #include "../gsparc-wirelessgaussiansplatting_amd/csrc/gs_csi.h"
@@ -284,6 +294,7 @@ static int preprocess_all(PRE_ARGS(R))
 {
     PreOut<R> o{radii, xy, depth, cov3, conic_o, sig_re, sig_im, tiles, rect};
     int vis = 0;
+#pragma omp parallel for reduction(+ : vis) schedule(static, 64)
     for (int i = 0; i < P; ++i)
         vis += preprocess_one<R>(i, P, W, H, means, scales, rots, opac, mlp, view, tx, rx, mod, o, nullptr);
     return vis;
@@ -300,20 +311,25 @@ extern "C" int64_t orc_bin(int P, int W, int H, const int* rect, const float* de
                 uint32_t* ranges)
 {
     const int gx = (W + BLOCK - 1) / BLOCK, gy = (H + BLOCK - 1) / BLOCK;
-    int64_t K = 0;
-    for (int i = 0; i < P; ++i) K += (int64_t)(rect[4 * i + 2] - rect[4 * i]) * (rect[4 * i + 3] - rect[4 * i + 1]);
+    std::vector<int64_t> off((size_t)P + 1, 0);
+    for (int i = 0; i < P; ++i)
+        off[i + 1] = off[i] + (int64_t)(rect[4 * i + 2] - rect[4 * i]) * (rect[4 * i + 3] - rect[4 * i + 1]);
+    const int64_t K = off[P];
     if (!keys) return K;
-    std::vector<std::pair<uint64_t, uint32_t>> kv;
-    kv.reserve(K);
+    // duplicateWithKeys: Gaussian i's instances at its offset, tiles row-major
+    std::vector<std::pair<uint64_t, uint32_t>> kv((size_t)K);
+#pragma omp parallel for schedule(static, 256)
     for (int i = 0; i < P; ++i) {
         const int* rc = rect + 4 * i;
         uint32_t db;
         std::memcpy(&db, depth + i, 4);
+        int64_t k = off[i];
         for (int y = rc[1]; y < rc[3]; ++y)
             for (int x = rc[0]; x < rc[2]; ++x)
-                kv.push_back({((uint64_t)(uint32_t)(y * gx + x) << 32) | db, (uint32_t)i});
+                kv[(size_t)k++] = {((uint64_t)(uint32_t)(y * gx + x) << 32) | db, (uint32_t)i};
     }
-    std::stable_sort(kv.begin(), kv.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    // stable: the result is the unique order by (key, input position)
+    __gnu_parallel::stable_sort(kv.begin(), kv.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
     for (int t = 0; t < gx * gy; ++t) ranges[2 * t] = ranges[2 * t + 1] = 0;
     for (int64_t k = 0; k < K; ++k) {
         keys[k] = kv[k].first;
@@ -334,9 +350,11 @@ static void render_fwd(int W, int H, const uint32_t* ranges, const uint32_t* pli
                        int64_t* stats, uint64_t* hash)
 {
     const int gx = (W + BLOCK - 1) / BLOCK;
-    int64_t st[4] = {0, 0, 0, 0};
+    int64_t st0 = 0, st1 = 0, st2 = 0, st3 = 0;
+#pragma omp parallel for schedule(static, 1) reduction(+ : st0, st1, st2, st3)
     for (int py = 0; py < H; ++py)
         for (int px = 0; px < W; ++px) {
+            int64_t st[4] = {0, 0, 0, 0};
             const int tile = (py / BLOCK) * gx + (px / BLOCK);
             const uint32_t b = ranges[2 * tile], e = ranges[2 * tile + 1];
             R T = R(1), are = R(0), aim = R(0);
@@ -369,10 +387,38 @@ static void render_fwd(int W, int H, const uint32_t* ranges, const uint32_t* pli
             finalT[p] = T;
             ncontrib[p] = last;
             if (hash) hash[p] = h ^ last;
+            st0 += st[0];
+            st1 += st[1];
+            st2 += st[2];
+            st3 += st[3];
         }
-    if (stats)
-        for (int i = 0; i < 4; ++i) stats[i] = st[i];
+    if (stats) {
+        stats[0] = st0;
+        stats[1] = st1;
+        stats[2] = st2;
+        stats[3] = st3;
+    }
 }
+
+// Per-thread accumulation buffers of `width` doubles per Gaussian for the
+// backward blends, summed into the caller's array in thread order.
+struct ThreadAcc {
+    int nt;
+    size_t n;
+    std::vector<double> buf;
+    ThreadAcc(size_t n_) : nt(omp_get_max_threads()), n(n_), buf(nt > 1 ? (size_t)nt * n_ : 0, 0.0) {}
+    double* mine(double* direct) { return nt > 1 ? buf.data() + (size_t)omp_get_thread_num() * n : direct; }
+    void reduce_into(double* out)
+    {
+        if (nt <= 1) return;
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < (int64_t)n; ++i) {
+            double v = out[i];
+            for (int t = 0; t < nt; ++t) v += buf[(size_t)t * n + i];
+            out[i] = v;
+        }
+    }
+};
 
 extern "C" void orc_render_fwd_f32(int W, int H, const uint32_t* ranges, const uint32_t* plist, const float* xy,
                         const float* conic_o, const float* sre, const float* sim, float* out_re, float* out_im,
@@ -390,11 +436,17 @@ extern "C" void orc_render_fwd_f64(int W, int H, const uint32_t* ranges, const u
 template <typename R>
 static void render_bwd(int mode, int W, int H, const uint32_t* ranges, const uint32_t* plist, const R* xy,
                        const R* conic_o, const R* sre, const R* sim, const R* finalT, const uint32_t* ncontrib,
-                       const R* dre, const R* dim, double* g8)
+                       const R* dre, const R* dim, double* g8_out)
 {
-    const int gx = (W + BLOCK - 1) / BLOCK;
+    const int gx = (W + BLOCK - 1) / BLOCK, gy = (H + BLOCK - 1) / BLOCK;
+    uint32_t maxid = 0;
+    for (int t = 0; t < gx * gy; ++t)
+        for (uint32_t k = ranges[2 * t]; k < ranges[2 * t + 1]; ++k) maxid = std::max(maxid, plist[k]);
+    ThreadAcc ta(((size_t)maxid + 1) * 8);
+#pragma omp parallel for schedule(static, 1)
     for (int py = 0; py < H; ++py)
         for (int px = 0; px < W; ++px) {
+            double* g8 = ta.mine(g8_out);
             const int tile = (py / BLOCK) * gx + (px / BLOCK);
             const uint32_t b = ranges[2 * tile];
             const int p = py * W + px;
@@ -442,6 +494,7 @@ static void render_bwd(int mode, int W, int H, const uint32_t* ranges, const uin
                 o[5] += (double)(G * dalpha);
             }
         }
+    ta.reduce_into(g8_out);
 }
 
 extern "C" void orc_render_bwd_f32(int mode, int W, int H, const uint32_t* ranges, const uint32_t* plist, const float* xy,
@@ -496,13 +549,14 @@ static void render_fwd_csi(int W, int H, double f0, double df, const uint32_t* r
     const CsiPhase<R> ph(f0, df);
     const int gx = (W + BLOCK - 1) / BLOCK;
     const int NC = GS_CSI_NC;
-    std::vector<R> acc(2 * NC);
+#pragma omp parallel for schedule(static, 1)
     for (int py = 0; py < H; ++py)
         for (int px = 0; px < W; ++px) {
+            R acc[2 * GS_CSI_NC];
             const int tile = (py / BLOCK) * gx + (px / BLOCK);
             const uint32_t b = ranges[2 * tile], e = ranges[2 * tile + 1];
             R T = R(1);
-            std::fill(acc.begin(), acc.end(), R(0));
+            std::fill(acc, acc + 2 * NC, R(0));
             uint32_t contributor = 0, last = 0;
             const R pxf = (R)px, pyf = (R)py;
             for (uint32_t k = b; k < e; ++k) {
@@ -561,8 +615,16 @@ static void render_bwd_csi(int W, int H, double f0, double df, const uint32_t* r
     const int NC = GS_CSI_NC;
     const bool mirror = sizeof(R) == sizeof(float);
     GsCsi tab = gs_csi_make(f0, df, GS_LIGHT_SPEED);
+    const int gy = (H + BLOCK - 1) / BLOCK;
+    uint32_t maxid = 0;
+    for (int t = 0; t < gx * gy; ++t)
+        for (uint32_t k = ranges[2 * t]; k < ranges[2 * t + 1]; ++k) maxid = std::max(maxid, plist[k]);
+    double* g9_out = g9;
+    ThreadAcc ta(((size_t)maxid + 1) * 9);
+#pragma omp parallel for schedule(static, 1)
     for (int py = 0; py < H; ++py)
         for (int px = 0; px < W; ++px) {
+            double* g9 = ta.mine(g9_out);
             const int tile = (py / BLOCK) * gx + (px / BLOCK);
             const uint32_t b = ranges[2 * tile];
             const size_t p = (size_t)py * W + px;
@@ -644,6 +706,7 @@ static void render_bwd_csi(int W, int H, double f0, double df, const uint32_t* r
                 o[5] += (double)(G * dalpha);
             }
         }
+    ta.reduce_into(g9_out);
 }
 
 extern "C" void orc_render_bwd_csi_f32(int W, int H, double f0, double df, const uint32_t* ranges,
@@ -923,6 +986,7 @@ static void pre_bwd_one(int mode, int idx, int P, int W, int H, const R* means, 
 template <typename R>
 static void pre_bwd_all(BWD_ARGS(R))
 {
+#pragma omp parallel for schedule(static, 64)
     for (int i = 0; i < P; ++i)
         if (radii[i] > 0)
             pre_bwd_one<R>(mode, i, P, W, H, means, scales, rots, mlp, view, tx, rx, mod, g8, dmean, dscale, drot,
@@ -958,6 +1022,7 @@ static void conv_same(int H, int W, const R* in, R* out)   // separable, x then 
     R w[11];
     gauss11(w);
     std::vector<R> tmp((size_t)H * W);
+#pragma omp parallel for schedule(static)
     for (int y = 0; y < H; ++y)
         for (int x = 0; x < W; ++x) {
             R v = R(0);
@@ -967,6 +1032,7 @@ static void conv_same(int H, int W, const R* in, R* out)   // separable, x then 
             }
             tmp[(size_t)y * W + x] = v;
         }
+#pragma omp parallel for schedule(static)
     for (int y = 0; y < H; ++y)
         for (int x = 0; x < W; ++x) {
             R v = R(0);
@@ -1234,4 +1300,11 @@ extern "C" void orc_pair_power_f32(int64_t n, const float* conic_o, const float*
         const float dx = xy[2 * i] - (float)px[i], dy = xy[2 * i + 1] - (float)py[i];
         out[i] = pair_power(conic_o[4 * i], conic_o[4 * i + 1], conic_o[4 * i + 2], dx, dy);
     }
+}
+
+// Threads of the oracle's parallel loops (0 or less: leave as is).
+extern "C" int orc_set_threads(int n)
+{
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
 }

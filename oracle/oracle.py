@@ -50,6 +50,14 @@ def math_vec(which, x, y=None):
     return out
 
 
+def set_threads(n=0):
+    """Threads of the oracle's OpenMP loops (n <= 0: unchanged); returns the
+    count in effect."""
+    f = lib().orc_set_threads
+    f.restype = C.c_int
+    return int(f(C.c_int(int(n))))
+
+
 def alpha_thr(o):
     """gs_alpha_thr over an array of opacities (float32)."""
     o = np.ascontiguousarray(o, np.float32)
