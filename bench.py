@@ -41,9 +41,15 @@ sys.path.insert(0, PKG)
 METRIC = "fwd+bwd spectra/sec, 50k Gaussians, 90×360, 64 rx batched vs plain; HBM GB/s"
 PEAK_HBM_GBS = 8000.0
 PEAK_FP32_TFLOPS = 157.3
-FWD_FLOP_PER_PAIR = 15      # per visited (pixel, Gaussian) pair
-BWD_FLOP_PER_PAIR = 40      # per back-propagated pair
-CULL_FLOP = 40              # per list entry tested against a wave's pixel block
+# Algorithmic FP32 work of the blend, per (pixel, list entry) pair of the
+# reference's own loops (forward.cu:198-236, backward.cu:547-615): every pair
+# it visits costs the offset, power, exponential and alpha test (15 ops);
+# a contributing pair's backward adds the transmittance recovery and the
+# gradient terms (25 more, 40 in all).  The kernels' counting variants count
+# these pairs exactly, whatever this build culls.
+FWD_FLOP_PER_PAIR = 15      # per pair the reference's forward visits
+BWD_FLOP_PER_PAIR = 15      # per pair the reference's backward walks
+BWD_FLOP_PER_CONTRIB = 25   # more per contributing pair (backward)
 
 
 def parse():
@@ -685,12 +691,12 @@ def main():
             "ssim_bwd": ("hbm", 28.0 * NC * N * R, "GB/s"),
         })
     # The blend kernels are bound by the vector ALU: their algorithmic work is
-    # FP32 operations per evaluated (pixel, Gaussian) pair (the kernels'
-    # counting variants count the pairs), against the published FP32 vector
-    # peak; their HBM rate is kept beside it.  The wave-level entry tests
-    # (culling) are this build's overhead, reported separately.
-    valu = {"render_fwd": (cnt["fwd_pairs"] * FWD_FLOP_PER_PAIR, cnt["fwd_pairs"], cnt["fwd_entries"]),
-            "render_bwd": (cnt["bwd_pairs"] * BWD_FLOP_PER_PAIR, cnt["bwd_pairs"], cnt["bwd_entries"])}
+    # the reference's FP32 operations over the pairs its loops visit (counted
+    # by the kernels' counting variants), against the published FP32 vector
+    # peak; their HBM rate is kept beside it.
+    valu = {"render_fwd": (cnt["fwd_pairs"] * FWD_FLOP_PER_PAIR, cnt["fwd_pairs"], cnt["fwd_contrib"]),
+            "render_bwd": (cnt["bwd_pairs"] * BWD_FLOP_PER_PAIR + cnt["bwd_contrib"] * BWD_FLOP_PER_CONTRIB,
+                           cnt["bwd_pairs"], cnt["bwd_contrib"])}
     traffic = measured_traffic(args, *wl.traffic_key) if wl.traffic_key else {}
     kernels = {}
     for name, (ms, n) in kt.items():
@@ -707,12 +713,12 @@ def main():
             entry.update(bound=bound, achieved=round(ach, 3 if unit == "TFLOP/s" else 1), unit=unit,
                          frac=round(frac, 4))
         if name in valu:
-            fl, pairs, tests = valu[name]
+            fl, pairs, contrib = valu[name]
             gbs, gfrac = entry.get("achieved"), entry.get("frac")
             ach = fl / (avg * 1e-3) / 1e12
             entry.update(bound="valu", achieved=round(ach, 3), unit="TFLOP/s", frac=round(ach / PEAK_FP32_TFLOPS, 4),
-                         pairs_per_launch=pairs, entry_tests_per_launch=tests, hbm_gbs=gbs, hbm_frac=gfrac,
-                         fp32_tflops_incl_entry_tests=round((fl + tests * CULL_FLOP) / (avg * 1e-3) / 1e12, 3))
+                         flop_per_launch=fl, pairs_per_launch=pairs, contributing_pairs_per_launch=contrib,
+                         hbm_gbs=gbs, hbm_frac=gfrac)
         if name in traffic:
             entry["traffic"] = traffic[name]
         kernels[name] = entry
@@ -724,8 +730,10 @@ def main():
             "unit": d.get("unit"), "frac": d.get("frac"), "traffic": traffic.get(dom),
             "ms_per_launch": d.get("ms_per_launch")}
     if dom in valu:
-        roof.update(flop_per_pair=BWD_FLOP_PER_PAIR if dom.endswith("bwd") else FWD_FLOP_PER_PAIR,
-                    pairs_per_launch=d["pairs_per_launch"], entry_tests_per_launch=d["entry_tests_per_launch"],
+        roof.update(flop_per_launch=d["flop_per_launch"], pairs_per_launch=d["pairs_per_launch"],
+                    contributing_pairs_per_launch=d["contributing_pairs_per_launch"],
+                    flop_per_pair=FWD_FLOP_PER_PAIR if dom.endswith("fwd") else
+                    f"{BWD_FLOP_PER_PAIR} per walked pair + {BWD_FLOP_PER_CONTRIB} per contributing pair",
                     hbm_gbs=d.get("hbm_gbs"), hbm_frac=d.get("hbm_frac"))
     if roof["traffic"] is not None:
         roof["traffic_source"] = TRAFFIC_FILE

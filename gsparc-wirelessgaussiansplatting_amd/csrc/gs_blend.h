@@ -15,6 +15,7 @@ struct Pixels {
     int px[PPL], py[PPL];
     bool inside[PPL];
     float x0, x1, y0, y1;   // pixel-centre bounds of the wave's block (inclusive)
+    float qx[PPL], qy[PPL]; // first pixel centre of each lane pixel's 8x8 quarter
 };
 
 // Workgroup b runs on XCD b & 7 (workgroups are dealt to the XCDs round
@@ -46,6 +47,8 @@ __device__ __forceinline__ Pixels<PPL> map_pixels(const Dims& d, const uint32_t*
         m.px[k] = tx * GS_BLOCK + (q & 1) * 8 + (lane & 7);
         m.py[k] = ty * GS_BLOCK + (q >> 1) * 8 + (lane >> 3);
         m.inside[k] = m.px[k] < d.W && m.py[k] < d.H;
+        m.qx[k] = (float)(tx * GS_BLOCK + (q & 1) * 8);
+        m.qy[k] = (float)(ty * GS_BLOCK + (q >> 1) * 8);
         qx0 = min(qx0, q & 1); qx1 = max(qx1, q & 1);
         qy0 = min(qy0, q >> 1); qy1 = max(qy1, q >> 1);
     }

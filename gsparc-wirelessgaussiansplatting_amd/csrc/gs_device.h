@@ -134,7 +134,11 @@ __device__ __forceinline__ float pair_power(float ha, float nb, float hc, float 
 }
 
 // ---- wave helpers (wave64) ----
-__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+// The mask of active lanes where p holds.  The builtin takes p as a lane
+// mask, so a predicate that is already a compare or a combination of
+// compares costs no vector instruction (HIP's __ballot goes through an
+// integer compare of p).
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
 

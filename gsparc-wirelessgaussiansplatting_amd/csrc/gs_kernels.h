@@ -199,9 +199,12 @@ struct RenderFwdArgs {
     uint64_t Kcap;           // capacity of plist (nothing is read when K > Kcap)
     unsigned long long* counts;  // optional profiling counters (see below)
 };
-// Profiling counters (u64 x 4, optional): [0] forward (pixel, Gaussian) pairs
-// evaluated, [1] backward pairs evaluated, [2] forward wave-level entry
-// tests, [3] backward wave-level entry tests.
+// Profiling counters (u64 x 4, optional), in the reference's terms
+// (forward.cu:198-236, backward.cu:547-563), whatever this build culls:
+// [0] (pixel, list entry) pairs the reference's forward visits (each pixel's
+// list up to its terminating entry, else all of it), [1] pairs its backward
+// walks (the contributor counts), [2] / [3] contributing pairs (alpha >=
+// 1/255, before termination) as the forward / backward found them.
 // pixels per lane of the blend kernels: 1 (8x8 quarter-tile per wave), 2
 // (16x8 half tile) or 4 (16x16 tile per wave); requested = 0 picks by the
 // number of tiles

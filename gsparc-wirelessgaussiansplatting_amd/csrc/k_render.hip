@@ -55,16 +55,16 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
     // the power and alpha tests, but then T (1 - alpha) < 0 < 1e-4 sends it
     // down the termination path, which leaves T negative: no state changes.
     float T[PPL], are[PPL], aim[PPL], pxf[PPL], pyf[PPL];
-    uint32_t last[PPL], cnt[PPL];
+    uint32_t last[PPL], stop[PPL], cnt[PPL];
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
         T[k] = pm.inside[k] ? 1.0f : -1.0f;
         are[k] = 0.0f; aim[k] = 0.0f;
         last[k] = 0; cnt[k] = 0;
+        stop[k] = re - rb;   // entries the reference visits: up to the terminating one, else all
         pxf[k] = (float)pm.px[k];
         pyf[k] = (float)pm.py[k];
     }
-    uint32_t tested = 0;
     for (uint32_t start = rb; start < re; start += GS_WAVE) {
         bool live = false;
 #pragma unroll
@@ -82,7 +82,6 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
             keep = !block_misses(A.x, A.y, A.z, A.w, B.x, B.z, pm.x0, pm.x1, pm.y0, pm.y1);
         }
         uint64_t mask = ballot(keep);
-        if (COUNT) tested += min(re - start, (uint32_t)GS_WAVE);
         __syncthreads();
         // list position (1-based) of entry j of this batch is base + j
         const uint32_t base = start - rb + 1;
@@ -94,32 +93,42 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
             const float2 S = ss[j];
 #pragma unroll
             for (int k = 0; k < PPL; ++k) {
-                if (COUNT) cnt[k] += T[k] > 0.0f ? 1u : 0u;
                 const float dx = A.x - pxf[k], dy = A.y - pyf[k];
                 const float power = pair_power(A.z, A.w, B.x, dx, dy);
                 // power >= thr: alpha >= 1/255 (gs_alpha_thr)
                 if (power > 0.0f || power < B.z) continue;
                 const float alpha = fminf(0.99f, B.y * gs_expf_blend(power));
                 const float test_T = T[k] * (1.0f - alpha);
-                if (test_T < 0.0001f) { T[k] = -fabsf(T[k]); continue; }
+                if (test_T < 0.0001f) {
+                    if (COUNT && T[k] > 0.0f) stop[k] = base + j;
+                    T[k] = -fabsf(T[k]);
+                    continue;
+                }
                 const float w = alpha * T[k];
                 are[k] = fmaf(S.x, w, are[k]);
                 aim[k] = fmaf(S.y, w, aim[k]);
                 T[k] = test_T;
                 last[k] = base + j;
+                if (COUNT) ++cnt[k];
             }
         }
         __syncthreads();
     }
     if (COUNT) {
-        unsigned long long c = 0;
+        unsigned long long c = 0, v = 0;
 #pragma unroll
-        for (int k = 0; k < PPL; ++k) c += cnt[k];
+        for (int k = 0; k < PPL; ++k) {
+            c += cnt[k];
+            v += pm.inside[k] ? stop[k] : 0u;
+        }
 #pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off);
+        for (int off = 32; off >= 1; off >>= 1) {
+            c += __shfl_xor(c, off);
+            v += __shfl_xor(v, off);
+        }
         if (lane == 0) {
-            atomicAdd(a.counts + 0, c);
-            atomicAdd(a.counts + 2, (unsigned long long)tested);
+            atomicAdd(a.counts + 0, v);
+            atomicAdd(a.counts + 2, c);
         }
     }
 #pragma unroll
@@ -133,13 +142,20 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
     }
 }
 
+// One staged list entry of the backward: the splat record (with the
+// Gaussian index in the second record's spare slot) and the emission, 48
+// bytes, so that one address serves all three reads.
+struct __align__(16) BwdRec {
+    float4 A;    // px, py, ha, nb
+    float4 B;    // hc, opacity, thr, Gaussian index (bits)
+    float2 S;    // emission (real, imag)
+    float2 Q2;   // 2 ha, 2 hc: the conic factors of the 2D-mean gradient
+};
+
 template <int MODE, int PPL, bool COUNT>
 __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
 {
-    __shared__ float4 s0[GS_WAVE];
-    __shared__ float4 s1[GS_WAVE];
-    __shared__ float2 ss[GS_WAVE];
-    __shared__ uint32_t sid[GS_WAVE];
+    __shared__ BwdRec rec[GS_WAVE];
     const Pixels<PPL> pm = map_pixels<PPL>(d, a.order, (int)a.total[1]);
     if (!pm.valid) return;
     if ((uint64_t)a.total[0] > a.Kcap) return;   // no lists: the gradient stays cleared
@@ -147,8 +163,8 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
     const int lane = threadIdx.x;
     float T[PPL], gre[PPL], gim[PPL], pxf[PPL], pyf[PPL];
     int last[PPL];
-    uint32_t cnt = 0;
     int maxlast = 0;
+    unsigned long long walked = 0, contrib = 0;
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
         const size_t p = (size_t)r * d.W * d.H + (size_t)pm.py[k] * d.W + pm.px[k];
@@ -156,14 +172,21 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
         last[k] = pm.inside[k] ? (int)a.ncontrib[p] : 0;
         gre[k] = pm.inside[k] ? a.dout_re[p] : 0.0f;
         gim[k] = pm.inside[k] ? a.dout_im[p] : 0.0f;
-        maxlast = max(maxlast, last[k]);
         pxf[k] = (float)pm.px[k];
         pyf[k] = (float)pm.py[k];
+        if (COUNT) walked += (unsigned long long)last[k];
     }
+    // per quarter, the largest contributor count of its pixels: entries at or
+    // past it are walked by none of them
+    int qlast[PPL];
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) maxlast = max(maxlast, __shfl_xor(maxlast, off));
-    maxlast = __builtin_amdgcn_readfirstlane(maxlast);
-    if (maxlast == 0) return;
+    for (int k = 0; k < PPL; ++k) {
+        int m = last[k];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) m = max(m, __shfl_xor(m, off));
+        qlast[k] = __builtin_amdgcn_readfirstlane(m);
+        maxlast = max(maxlast, qlast[k]);
+    }
     const uint2 range = a.ranges[(size_t)r * d.T + pm.tile];
     const uint32_t rb = __builtin_amdgcn_readfirstlane(range.x);
     const size_t rP = rx_map(d, r).roff;
@@ -180,91 +203,107 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
     // preprocess backward)
     const int vi = lane >> 3;
     const float post = (vi == 2 || vi == 4) ? -0.5f : (vi == 3 ? -1.0f : 1.0f);
-    uint32_t tested = 0;
+    // Per-lane sums over the lane's pixels for the current entry, with
+    // D = o G dL/dalpha: sh = sum D dx, sk = sum D dy, sxx = sum D dx^2,
+    // sxy = sum D dx dy, syy = sum D dy^2, sg = sum G dL/dalpha, swr / swi =
+    // sum w dL/dpix.  The 2D-mean and conic gradients are linear in them.
+    // Only contributing lanes add to them; they are cleared after each
+    // entry's reduction (an entry without contributors leaves them clear).
+    float sh = 0.0f, sk = 0.0f, sxx = 0.0f, sxy = 0.0f, syy = 0.0f, sg = 0.0f, swr = 0.0f, swi = 0.0f;
     for (int top = maxlast - 1; top >= 0; top -= GS_WAVE) {
         // entry j of this batch sits at list position top - j (walked back to front)
         const int rel = top - lane;
-        bool keep = false;
+        bool hit[PPL];
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) hit[k] = false;
         if (rel >= 0) {
             const uint32_t id = a.plist[rb + rel];
             const size_t gi = rP + id;
             const float4 A = a.splat[2 * gi];
-            const float4 B = a.splat[2 * gi + 1];
-            sid[lane] = id;
-            s0[lane] = A;
-            s1[lane] = B;
-            ss[lane] = a.sig[gi];
-            keep = !block_misses(A.x, A.y, A.z, A.w, B.x, B.z, pm.x0, pm.x1, pm.y0, pm.y1);
+            float4 B = a.splat[2 * gi + 1];
+            const float2 S = a.sig[gi];
+            B.w = __uint_as_float(id);
+            rec[lane].A = A;
+            rec[lane].B = B;
+            rec[lane].S = S;
+            rec[lane].Q2 = make_float2(2.0f * A.z, 2.0f * B.x);
+            // culled per 8x8 quarter: an entry is walked only over the
+            // quarters where it can reach alpha >= 1/255 and where some
+            // pixel's contributors reach back to it
+#pragma unroll
+            for (int k = 0; k < PPL; ++k)
+                hit[k] = rel < qlast[k] && !block_misses(A.x, A.y, A.z, A.w, B.x, B.z, pm.qx[k], pm.qx[k] + 7.0f,
+                                                         pm.qy[k], pm.qy[k] + 7.0f);
         }
-        uint64_t mask = ballot(keep);
-        if (COUNT) tested += min(top + 1, GS_WAVE);
+        uint64_t qmask[PPL];
+        uint64_t mask = 0;
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) {
+            qmask[k] = ballot(hit[k]);
+            mask |= qmask[k];
+        }
         __syncthreads();
         while (mask) {
             const int j = __builtin_ctzll(mask);
             mask &= mask - 1;
-            const float4 A = s0[j];
-            const float4 B = s1[j];
-            const float2 S = ss[j];
-            // Per-lane sums over the lane's pixels, with D = o G dL/dalpha:
-            //   sh = sum D dx, sk = sum D dy, sxx = sum D dx^2, sxy = sum D dx dy,
-            //   syy = sum D dy^2, sg = sum G dL/dalpha, swr / swi = sum w dL/dpix.
-            // The 2D-mean and conic gradients are linear in them; -0 is the
-            // identity of float addition, so the first pixel's sums fold away.
-            float sh = -0.0f, sk = -0.0f, sxx = -0.0f, sxy = -0.0f, syy = -0.0f, sg = -0.0f, swr = -0.0f,
-                  swi = -0.0f;
-            bool anyc = false;
+            const float4 A = rec[j].A;
+            const float4 B = rec[j].B;
+            const float2 S = rec[j].S;
+            uint64_t anyc = 0;
+            const int pos = top - j;
 #pragma unroll
             for (int k = 0; k < PPL; ++k) {
-                const bool active = (top - j) < last[k];
-                if (COUNT) cnt += active ? 1u : 0u;
+                if (!((qmask[k] >> j) & 1ull)) continue;   // wave-uniform: the quarter is culled
                 const float dx = A.x - pxf[k], dy = A.y - pyf[k];
                 const float power = pair_power(A.z, A.w, B.x, dx, dy);
                 // the forward's decision: power in [thr, 0] <=> alpha >= 1/255
-                const bool c = active && !(power > 0.0f) && !(power < B.z);
-                // a uniform branch: the body runs for the whole wave, selects
-                // keep the state of the lanes whose pair does not contribute
-                if (ballot(c) == 0) continue;
-                // G = 0 off [thr, 0], so no out-of-range value reaches the sums
-                const float G = c ? gs_expf_grad(power) : 0.0f;
-                const float oG = B.y * G;
-                const float alpha = fminf(0.99f, oG);
-                // T before this Gaussian: T_after / (1 - alpha) (backward.cu:565)
-                T[k] = c ? T[k] * __builtin_amdgcn_rcpf(1.0f - alpha) : T[k];
-                const float w = (c ? alpha : 0.0f) * T[k];
-                const float sr = S.x;
-                const float si = (MODE == GRAD_REFERENCE) ? S.x : S.y;
-                // dL/dalpha = (c - accum) dL/dpixel T (backward.cu:586, 592), then
-                // accum = alpha c + (1 - alpha) accum for the next contributor
-                const float er = sr - acc_re[k];
-                float dalpha = er * gre[k];
-                acc_re[k] = c ? fmaf(alpha, er, acc_re[k]) : acc_re[k];
-                if (MODE == GRAD_EXACT) {
-                    const float ei = si - acc_im[k];
-                    dalpha = fmaf(ei, gim[k], dalpha);
-                    acc_im[k] = c ? fmaf(alpha, ei, acc_im[k]) : acc_im[k];
+                const bool act = pos < last[k], nonpos = !(power > 0.0f), reach = !(power < B.z);
+                const bool c = act && nonpos && reach;
+                // the masks of single compares combine on the scalar unit
+                anyc |= ballot(act) & ballot(nonpos) & ballot(reach);
+                if (c) {
+                    // only the contributing lanes run the pair's arithmetic
+                    const float G = gs_expf_grad(power);
+                    const float oG = B.y * G;
+                    const float alpha = fminf(0.99f, oG);
+                    // T before this Gaussian: T_after / (1 - alpha) (backward.cu:565)
+                    T[k] = T[k] * __builtin_amdgcn_rcpf(1.0f - alpha);
+                    const float w = alpha * T[k];
+                    const float sr = S.x;
+                    const float si = (MODE == GRAD_REFERENCE) ? S.x : S.y;
+                    // dL/dalpha = (c - accum) dL/dpixel T (backward.cu:586, 592), then
+                    // accum = alpha c + (1 - alpha) accum for the next contributor
+                    const float er = sr - acc_re[k];
+                    float dalpha = er * gre[k];
+                    acc_re[k] = fmaf(alpha, er, acc_re[k]);
+                    if (MODE == GRAD_EXACT) {
+                        const float ei = si - acc_im[k];
+                        dalpha = fmaf(ei, gim[k], dalpha);
+                        acc_im[k] = fmaf(alpha, ei, acc_im[k]);
+                    }
+                    dalpha *= T[k];
+                    swr = fmaf(w, gre[k], swr);
+                    swi = fmaf(w, gim[k], swi);
+                    // the exact adjoint has no gradient through the clamped alpha
+                    const float da = (MODE == GRAD_EXACT && oG > 0.99f) ? 0.0f : dalpha;
+                    const float D = oG * da;
+                    const float h = D * dx, q = D * dy;
+                    sh += h;
+                    sk += q;
+                    sxx = fmaf(h, dx, sxx);
+                    sxy = fmaf(h, dy, sxy);
+                    syy = fmaf(q, dy, syy);
+                    sg = fmaf(G, da, sg);
+                    if (COUNT) ++contrib;
                 }
-                dalpha *= T[k];
-                swr = fmaf(w, gre[k], swr);
-                swi = fmaf(w, gim[k], swi);
-                // the exact adjoint has no gradient through the clamped alpha
-                const bool geo = (MODE == GRAD_EXACT) ? (c && !(oG > 0.99f)) : c;
-                const float da = geo ? dalpha : 0.0f;
-                const float D = oG * da;
-                const float h = D * dx, q = D * dy;
-                sh += h;
-                sk += q;
-                sxx = fmaf(h, dx, sxx);
-                sxy = fmaf(h, dy, sxy);
-                syy = fmaf(q, dy, syy);
-                sg = fmaf(G, da, sg);
-                anyc |= c;
             }
-            if (ballot(anyc) == 0) continue;
+            if (anyc == 0) continue;
             // dL/dmean2D = D (conic . d) with the conic (qa, qb; qb, qc) and
             // qa = -2 A.z, qb = -A.w, qc = -2 B.x (backward.cu:600-605)
+            const float2 Q2 = rec[j].Q2;
             float v[8];
-            v[0] = fmaf(A.w, sk, (2.0f * A.z) * sh);
-            v[1] = fmaf(A.w, sh, (2.0f * B.x) * sk);
+            v[0] = fmaf(A.w, sk, Q2.x * sh);
+            v[1] = fmaf(A.w, sh, Q2.y * sk);
             v[2] = sxx;
             v[3] = sxy;
             v[4] = syy;
@@ -272,20 +311,24 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
             v[6] = swr;
             v[7] = swi;
             const float tot = wave_sum8(v, lane) * post;
+            sh = sk = sxx = sxy = syy = sg = swr = swi = 0.0f;
             // the record index is wave-uniform: its address arithmetic stays scalar
-            const uint32_t gid = __builtin_amdgcn_readfirstlane(sid[j]);
-            float* rec = a.g8 + (rP + gid) * 8;
-            if ((lane & 7) == 0) atomicAdd(rec + (lane >> 3), tot);
+            const uint32_t gid = __builtin_amdgcn_readfirstlane(__float_as_uint(B.w));
+            float* rp = a.g8 + (rP + gid) * 8;
+            if ((lane & 7) == 0) atomicAdd(rp + (lane >> 3), tot);
         }
         __syncthreads();
     }
     if (COUNT) {
-        unsigned long long c = cnt;
+        unsigned long long w = walked, c = contrib;
 #pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off);
+        for (int off = 32; off >= 1; off >>= 1) {
+            w += __shfl_xor(w, off);
+            c += __shfl_xor(c, off);
+        }
         if (lane == 0) {
-            atomicAdd(a.counts + 1, c);
-            atomicAdd(a.counts + 3, (unsigned long long)tested);
+            atomicAdd(a.counts + 1, w);
+            atomicAdd(a.counts + 3, c);
         }
     }
 }

@@ -176,10 +176,12 @@ def profile_collect() -> dict:
 
 
 def profile_counts() -> dict:
-    """Blend-kernel counters accumulated since profile_enable(True)."""
+    """Blend-kernel counters accumulated since profile_enable(True), in the
+    reference's terms: pairs its forward visits and its backward walks, and
+    the contributing pairs found by the forward and by the backward."""
     v = (C.c_uint64 * 4)()
     check(lib().gsparc_profile_counts(v), "gsparc_profile_counts")
-    return {"fwd_pairs": int(v[0]), "bwd_pairs": int(v[1]), "fwd_entries": int(v[2]), "bwd_entries": int(v[3])}
+    return {"fwd_pairs": int(v[0]), "bwd_pairs": int(v[1]), "fwd_contrib": int(v[2]), "bwd_contrib": int(v[3])}
 
 
 def layout(P, R, W, H, K=0) -> Layout:

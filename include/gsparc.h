@@ -309,11 +309,14 @@ int gsparc_graph_destroy(gsparc_graph* graph);
  * events on its own stream; collect() waits for them, adds the elapsed times
  * to per-kernel totals and returns how many kernels have totals; query()
  * reads entry i (sorted by kernel name); 3 counting: the blend kernels run
- * their counting variants (slower: extra atomics) and count counts[0] forward
- * (pixel, Gaussian) pairs evaluated, counts[1] backward pairs evaluated,
- * counts[2] / counts[3] list entries tested against a wave's pixel block in
- * the forward / backward (profile_counts); 1 both (the timings then include
- * the counting).  No counterpart in the reference (which only times render()
+ * their counting variants (slower: extra atomics) and count, in the
+ * reference's terms whatever this build culls (profile_counts): counts[0]
+ * the (pixel, list entry) pairs the reference's forward visits (each pixel's
+ * list up to its terminating entry, else all of it; forward.cu:198-236),
+ * counts[1] the pairs its backward walks (the sum of the contributor counts;
+ * backward.cu:547-563), counts[2] / counts[3] the contributing pairs (alpha
+ * >= 1/255, before termination) as the forward / backward found them; 1 both
+ * (the timings then include the counting).  No counterpart in the reference (which only times render()
  * on the host, train.py:381-384). */
 int gsparc_profile_enable(int on);
 int gsparc_profile_collect(int* n_kernels);

@@ -73,9 +73,23 @@ def test_near_threshold_pairs_match_oracle(mode):
     e = O.math_vec("exp", O.pair_power(pre, px, py))
     keep = np.minimum(np.float32(0.99), pre.conic_o[:, 3] * e) >= AMIN
     assert 0 < keep[plant].sum() < plant.sum()
-    d, out = gpu_forward(sc)
+    import gsparc_hip as gh
+    gh.profile_enable(True, timing=False, counting=True)
+    try:
+        d, out = gpu_forward(sc)
+        K, ore, oim, radii, geom, binbuf, img = out
+        g1 = torch.ones_like(ore)
+        wr.backward_batched(d["means"], d["mlp"], radii, d["opac"], d["scales"], d["rots"], 1.0, d["views"], g1, g1,
+                            d["tx"], d["rx"], geom, K, binbuf, img, grad_mode=mode)
+        torch.cuda.synchronize()
+        cnt = gh.profile_counts()
+    finally:
+        gh.profile_enable(False)
+    # the backward walked exactly the forward's contributors
+    assert cnt["fwd_contrib"] == cnt["bwd_contrib"], cnt
     st = read_state(sc, out)
     pre, keys, vals, ranges, fwd = O.render(sc, 0)
+    assert cnt["fwd_contrib"] == int(fwd["stats"][3])
     np.testing.assert_array_equal(st["n_contrib"][0], fwd["n_contrib"])
     np.testing.assert_array_equal(st["final_T"][0], fwd["final_T"])
     peak = max(np.abs(fwd["out_re"]).max(), 1e-30)
@@ -135,3 +149,35 @@ def test_instance_count_saturates_past_2_32():
                              img, emission=em)
     torch.cuda.synchronize()
     assert float(gr[1].abs().max()) == 0.0 and float(gr[3].abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("ppl", [1, 2, 4])
+def test_blend_counters_count_the_reference_loops(ppl):
+    """The counting variants count, in the reference's terms, the pairs its
+    forward visits, the pairs its backward walks and the contributing pairs;
+    the oracle's loops give the same numbers exactly, and the forward and the
+    backward find the same contributing pairs."""
+    import gsparc_hip as gh
+    sc = S.small_scene(141, 2000, R=3, walls=True)
+    gh.set_option("render_fwd_ppl", ppl)
+    gh.set_option("render_bwd_ppl", ppl)
+    gh.profile_enable(True, timing=False, counting=True)
+    try:
+        d, out = gpu_forward(sc)
+        K, ore, oim, radii, geom, binbuf, img = out
+        g = torch.ones_like(ore)
+        wr.backward_batched(d["means"], d["mlp"], radii, d["opac"], d["scales"], d["rots"], 1.0, d["views"], g, g,
+                            d["tx"], d["rx"], geom, K, binbuf, img)
+        torch.cuda.synchronize()
+        cnt = gh.profile_counts()
+    finally:
+        gh.profile_enable(False)
+        gh.set_option("render_fwd_ppl", 0)
+        gh.set_option("render_bwd_ppl", 0)
+    visited = blended = walked = 0
+    for r in range(sc.R):
+        pre, keys, vals, ranges, fwd = O.render(sc, r)
+        visited += int(fwd["stats"][0])
+        blended += int(fwd["stats"][3])
+        walked += int(fwd["n_contrib"].sum())
+    assert cnt == {"fwd_pairs": visited, "bwd_pairs": walked, "fwd_contrib": blended, "bwd_contrib": blended}, cnt
