@@ -309,8 +309,32 @@ struct SsimBwdArgs {
     float* dimg1;
     float dmap_const;
     float l1w;               // adds l1w * sign(img1 - img2) to dimg1 (0: nothing)
+    const float* plane_scale;  // nullptr, or [planes]: dmap_const and l1w are scaled by plane_scale[plane]
 };
 hipError_t launch_ssim_bwd(const SsimBwdArgs& a, hipStream_t s);
+
+// The confidence-weighted loss over R spectra (train.py:418-436) from the
+// per-workgroup partials of k_ssim_fwd (nbp workgroups per plane):
+//   L_r = l1w mean|img_r - tgt_r| + sw (1 - mean SSIM_r),
+//   C_r = 1 + exp(w2 . relu(W1 rx_r + b1) + b2) (or 1 without a network),
+//   loss = mean_r (C_r L_r - alpha log C_r);
+// plane_scale[r] = C_r scales the SSIM backward of plane r; dmlp (may be
+// null) = dloss / d(W1 | b1 | w2 | b2).  scratch: 4 floats per plane.
+struct ConfLossArgs {
+    const float* part;
+    int planes, nbp, hidden;
+    double inv_np;           // 1 / (H W)
+    float l1w, sw, alpha;
+    const float* mlp;        // [hidden*3 | hidden | hidden | 1] or nullptr
+    const float* rx;         // [planes, 3]
+    float* loss;             // [1]
+    float* plane_scale;      // [planes]
+    float* rec;              // [planes] or nullptr
+    float* conf;             // [planes] or nullptr
+    float* dmlp;             // [5 hidden + 1] or nullptr
+    float* scratch;          // [planes * 4]
+};
+hipError_t launch_conf_loss(const ConfLossArgs& a, hipStream_t s);
 
 // Shared emission MLP on MFMA (k_mlp_mfma.hip); rows = R*P, row = r*P + g.
 struct Mlp1Args {

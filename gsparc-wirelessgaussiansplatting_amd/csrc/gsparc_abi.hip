@@ -1140,6 +1140,65 @@ int gsparc_loss_l1_ssim_noalloc(int planes, int H, int W, float lambda_ssim, flo
                              (hipStream_t)stream);
 }
 
+size_t gsparc_conf_loss_work_size(int planes, int H, int W)
+{
+    if (planes <= 0 || H <= 0 || W <= 0) return 0;
+    return gsparc_loss_work_size(planes, H, W) + ALIGN + al((size_t)planes * 4 * sizeof(float)) +
+           al((size_t)planes * sizeof(float));
+}
+
+int gsparc_loss_l1_ssim_conf(int planes, int H, int W, float lambda_ssim, float C1, float C2, float alpha,
+                             int hidden, const float* conf_mlp, const float* rx_pos, const float* img,
+                             const float* target, void* work, float* loss, float* rec_loss, float* confidence,
+                             float* dL_dimg, float* dconf_mlp, void* stream)
+{
+    if (int e = loss_args(planes, H, W, img, target, loss)) return e;
+    if (!work) return fail(GSPARC_ERR_ARG, "null pointer");
+    if (conf_mlp && (!rx_pos || hidden < 1 || hidden > 255))
+        return fail(GSPARC_ERR_ARG, "the confidence network needs rx_pos and 1..255 hidden units");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t n = (size_t)planes * H * W;
+    char* base = base_of(work);
+    float* dmu1 = dL_dimg ? reinterpret_cast<float*>(base) : nullptr;
+    float* ds1 = dL_dimg ? dmu1 + n : nullptr;
+    float* ds12 = dL_dimg ? ds1 + n : nullptr;
+    float* part = reinterpret_cast<float*>(base + al(3 * n * sizeof(float)));
+    const int nb = gs::ssim_blocks(planes, H, W);
+    char* tail = base + al(al(3 * n * sizeof(float)) + 2 * sizeof(float) * (size_t)nb);
+    float* scratch = reinterpret_cast<float*>(tail);
+    float* pscale = reinterpret_cast<float*>(tail + al((size_t)planes * 4 * sizeof(float)));
+    const float l1w = 1.0f - lambda_ssim;
+    gs::SsimArgs a{planes, H, W, C1, C2, img, target, nullptr, dmu1, ds1, ds12, part};
+    GS_LAUNCH("ssim_fwd", gs::launch_ssim_fwd(a, s), s);
+    gs::ConfLossArgs c;
+    c.part = part;
+    c.planes = planes;
+    c.nbp = gs::ssim_blocks(1, H, W);
+    c.hidden = hidden;
+    c.inv_np = 1.0 / ((double)H * W);
+    c.l1w = l1w;
+    c.sw = lambda_ssim;
+    c.alpha = alpha;
+    c.mlp = conf_mlp;
+    c.rx = rx_pos;
+    c.loss = loss;
+    c.plane_scale = pscale;
+    c.rec = rec_loss;
+    c.conf = confidence;
+    c.dmlp = conf_mlp ? dconf_mlp : nullptr;
+    c.scratch = scratch;
+    GS_LAUNCH("conf_loss", gs::launch_conf_loss(c, s), s);
+    if (dL_dimg) {
+        // d/dimg_r of mean_r (C_r L_r - alpha log C_r) = C_r / R dL_r/dimg_r, with
+        // dL_r/dimg_r = l1w sign / (H W) - lambda dSSIM_r / (H W)
+        const float inv = (float)(1.0 / (double)n);
+        gs::SsimBwdArgs b{planes, H, W, img, target, nullptr, dmu1, ds1, ds12, dL_dimg, -lambda_ssim * inv,
+                          l1w * inv, pscale};
+        GS_LAUNCH("ssim_bwd", gs::launch_ssim_bwd(b, s), s);
+    }
+    return GSPARC_OK;
+}
+
 size_t gsparc_csi_loss_work_size(int R, int H, int W)
 {
     if (R <= 0 || H <= 0 || W <= 0) return 0;

@@ -143,6 +143,9 @@ __global__ void __launch_bounds__(SS_THREADS) k_ssim_bwd(SsimBwdArgs a, Taps tp)
     const int tid = threadIdx.x;
     const int x0 = blockIdx.x * SS_TW, y0 = blockIdx.y * SS_TH;
     const size_t plane = (size_t)blockIdx.z * a.H * a.W;
+    // the plane's loss weight (the confidence of its spectrum), wave-uniform
+    const float ps = a.plane_scale ? a.plane_scale[blockIdx.z] : 1.0f;
+    const float dmap_const = a.dmap_const * ps, l1w = a.l1w * ps;
     // the images at the thread's output pixels, loaded now and used at the end
     constexpr int NO = SS_TH * SS_TW / SS_THREADS;
     static_assert(NO * SS_THREADS == SS_TH * SS_TW, "whole outputs per thread");
@@ -165,7 +168,7 @@ __global__ void __launch_bounds__(SS_THREADS) k_ssim_bwd(SsimBwdArgs a, Taps tp)
             const int ly = i / SS_HW, lx = i % SS_HW;
             const int gy = y0 + ly - 5, gx = x0 + lx - 5;
             const size_t o = plane + (size_t)min(max(gy, 0), a.H - 1) * a.W + min(max(gx, 0), a.W - 1);
-            g[u] = a.dmap ? a.dmap[o] : a.dmap_const;
+            g[u] = a.dmap ? a.dmap[o] : dmap_const;
             t0[u] = a.dmu1[o];
             t1[u] = a.ds1[o];
             t2[u] = a.ds12[o];
@@ -211,10 +214,10 @@ __global__ void __launch_bounds__(SS_THREADS) k_ssim_bwd(SsimBwdArgs a, Taps tp)
         const size_t o = plane + (size_t)gy * a.W + gx;
         const float x = xi[u], y = yi[u];
         float g = v[0] + x * 2.0f * v[1] + y * v[2];
-        if (a.l1w != 0.0f) {
+        if (l1w != 0.0f) {
             // d|x - y|/dx with sign(0) = 0, as torch's abs backward
             const float sgn = x > y ? 1.0f : (x < y ? -1.0f : 0.0f);
-            g = g + a.l1w * sgn;
+            g = g + l1w * sgn;
         }
         a.dimg1[o] = g;
     }
@@ -241,6 +244,95 @@ __global__ void __launch_bounds__(256) k_loss_reduce(const float* part, int nb, 
         __syncthreads();
     }
     if (tid == 0) loss[0] = (float)((double)l1w * r1[0] * inv_n + (double)sw * (1.0 - r0[0] * inv_n));
+}
+
+// One workgroup.  Phase 1, per plane (threads strided over the planes): the
+// plane's L1 and SSIM sums from its workgroups' partials (in workgroup
+// order, in FP64), its reconstruction loss L_r, its confidence C_r (the
+// ConfidenceMLP of gaussian.py:575-609, FP32 dot products in index order)
+// and its loss term; the terms are summed by a fixed tree.  Phase 2, per
+// hidden unit (thread j < hidden, over the planes in order): the gradient of
+// the mean loss with respect to the network, through dloss/dC_r =
+// (L_r - alpha / C_r) / R and dC/draw = C - 1.
+__global__ void __launch_bounds__(256) k_conf_loss(ConfLossArgs a)
+{
+    __shared__ double red[256];
+    const int tid = threadIdx.x, Hn = a.hidden;
+    const float* W1 = a.mlp;
+    const float* b1 = a.mlp ? a.mlp + 3 * Hn : nullptr;
+    const float* w2 = a.mlp ? a.mlp + 4 * Hn : nullptr;
+    const float b2 = a.mlp ? a.mlp[5 * Hn] : 0.0f;
+    const double invR = 1.0 / (double)a.planes;
+    double term = 0.0;
+    for (int r = tid; r < a.planes; r += 256) {
+        double m = 0.0, l1 = 0.0;
+        const float* pp = a.part + 2 * (size_t)r * a.nbp;
+        for (int b = 0; b < a.nbp; ++b) {
+            m += (double)pp[2 * b];
+            l1 += (double)pp[2 * b + 1];
+        }
+        const double L = (double)a.l1w * l1 * a.inv_np + (double)a.sw * (1.0 - m * a.inv_np);
+        float C = 1.0f;
+        if (a.mlp) {
+            const float x = a.rx[3 * r], y = a.rx[3 * r + 1], z = a.rx[3 * r + 2];
+            float raw = 0.0f;
+            for (int j = 0; j < Hn; ++j) {
+                float pre = fmaf(W1[3 * j], x, b1[j]);
+                pre = fmaf(W1[3 * j + 1], y, pre);
+                pre = fmaf(W1[3 * j + 2], z, pre);
+                raw = fmaf(w2[j], fmaxf(pre, 0.0f), raw);
+            }
+            C = 1.0f + expf(raw + b2);
+        }
+        term += (double)C * L - (double)a.alpha * log((double)C);
+        a.plane_scale[r] = C;
+        if (a.rec) a.rec[r] = (float)L;
+        if (a.conf) a.conf[r] = C;
+        // dloss / draw_r for phase 2
+        a.scratch[4 * r] = (float)(((L - (double)a.alpha / (double)C) * invR) * ((double)C - 1.0));
+    }
+    red[tid] = term;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) red[tid] += red[tid + w];
+        __syncthreads();
+    }
+    if (tid == 0) a.loss[0] = (float)(red[0] * invR);
+    if (!a.mlp || !a.dmlp) return;
+    __threadfence();
+    __syncthreads();
+    if (tid < Hn) {
+        const int j = tid;
+        double gw[3] = {0.0, 0.0, 0.0}, gb = 0.0, g2 = 0.0;
+        for (int r = 0; r < a.planes; ++r) {
+            const float x = a.rx[3 * r], y = a.rx[3 * r + 1], z = a.rx[3 * r + 2];
+            float pre = fmaf(W1[3 * j], x, b1[j]);
+            pre = fmaf(W1[3 * j + 1], y, pre);
+            pre = fmaf(W1[3 * j + 2], z, pre);
+            const double draw = (double)a.scratch[4 * r];
+            g2 += draw * (double)fmaxf(pre, 0.0f);
+            const double dh = pre > 0.0f ? draw * (double)w2[j] : 0.0;
+            gw[0] += dh * x;
+            gw[1] += dh * y;
+            gw[2] += dh * z;
+            gb += dh;
+        }
+        a.dmlp[3 * j] = (float)gw[0];
+        a.dmlp[3 * j + 1] = (float)gw[1];
+        a.dmlp[3 * j + 2] = (float)gw[2];
+        a.dmlp[3 * Hn + j] = (float)gb;
+        a.dmlp[4 * Hn + j] = (float)g2;
+    } else if (tid == Hn) {
+        double gb2 = 0.0;
+        for (int r = 0; r < a.planes; ++r) gb2 += (double)a.scratch[4 * r];
+        a.dmlp[5 * Hn] = (float)gb2;
+    }
+}
+
+hipError_t launch_conf_loss(const ConfLossArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_conf_loss, dim3(1), dim3(256), 0, s, a);
+    return hipGetLastError();
 }
 
 // 11-tap Gaussian window, sigma 1.5 (utils.py:116-124): float32 samples

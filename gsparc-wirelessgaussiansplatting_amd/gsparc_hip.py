@@ -96,6 +96,9 @@ _SIGS = {
     "gsparc_loss_work_size": (C.c_size_t, [_i, _i, _i]),
     "gsparc_loss_l1_ssim": (_i, [_i, _i, _i, _f, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gsparc_loss_l1_ssim_noalloc": (_i, [_i, _i, _i, _f, _f, _f, _vp, _vp, _vp, _vp, _vp]),
+    "gsparc_conf_loss_work_size": (_sz, [_i, _i, _i]),
+    "gsparc_loss_l1_ssim_conf": (_i, [_i, _i, _i, _f, _f, _f, _f, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                      _vp, _vp]),
     "gsparc_graph_begin": (_i, [_vp]),
     "gsparc_graph_end": (_i, [_vp, C.POINTER(_vp)]),
     "gsparc_graph_launch": (_i, [_vp, _vp]),

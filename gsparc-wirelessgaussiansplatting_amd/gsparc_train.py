@@ -12,7 +12,9 @@ loss = (1 - lambda1) * L1 + lambda1 * (1 - SSIM) with lambda1 = 0.2
   capacity, K kept on the device), the loss and the backward once into a HIP
   graph owned by the library; `replay()` then re-runs the step on whatever
   the parameter tensors hold, with no host synchronisation and no allocation.
-  The confidence weighting of train.py:429-436 stays with the caller.
+* `confidence_l1_ssim_loss` / `confidence_loss` add the confidence weighting
+  of train.py:429-436 (C_r from the receiver-position ConfidenceMLP) inside
+  the same fused loss.
 """
 from __future__ import annotations
 
@@ -66,6 +68,110 @@ class L1SSIMLoss(torch.autograd.Function):
     def backward(ctx, g):
         (grad,) = ctx.saved_tensors
         return grad * g, None, None
+
+
+CONFIDENCE_ALPHA = 0.3     # confidence_alpha of optimizer.py
+
+
+class ConfidenceMLP(torch.nn.Module):
+    """The receiver-position confidence network of gaussian.py:575-609:
+    Linear(3 -> hidden) -> ReLU -> Linear(hidden -> 1), C = 1 + exp(raw) > 1.
+    Same parameter names, shapes and initialisation (Xavier-uniform weights,
+    zero biases), so its state dict is the reference's.  forward() is the
+    plain torch form; the training loss evaluates the network inside the
+    fused loss kernel from packed()."""
+
+    def __init__(self, hidden_size: int = 32, device=None):
+        super().__init__()
+        self.fc1 = torch.nn.Linear(3, hidden_size)
+        self.relu = torch.nn.ReLU()
+        self.fc2 = torch.nn.Linear(hidden_size, 1)
+        torch.nn.init.zeros_(self.fc2.bias)
+        torch.nn.init.xavier_uniform_(self.fc1.weight)
+        torch.nn.init.zeros_(self.fc1.bias)
+        torch.nn.init.xavier_uniform_(self.fc2.weight)
+        if device is not None:
+            self.to(device)
+
+    @property
+    def hidden(self) -> int:
+        return self.fc1.out_features
+
+    def forward(self, rx_pos):
+        if rx_pos.dim() == 1:
+            rx_pos = rx_pos.unsqueeze(0)
+        return (1.0 + torch.exp(self.fc2(self.relu(self.fc1(rx_pos))))).squeeze(-1)
+
+    def packed(self) -> torch.Tensor:
+        """[fc1.weight | fc1.bias | fc2.weight | fc2.bias], differentiable."""
+        return torch.cat([self.fc1.weight.reshape(-1), self.fc1.bias, self.fc2.weight.reshape(-1), self.fc2.bias])
+
+
+def confidence_l1_ssim_loss(img: torch.Tensor, target: torch.Tensor, rx_pos: torch.Tensor, conf_mlp=None,
+                            hidden: int = 32, alpha: float = CONFIDENCE_ALPHA, lambda_ssim: float = LAMBDA_SSIM,
+                            want_grad: bool = True):
+    """The reference's confidence-weighted loss (train.py:418-436) of R
+    spectra img, target [R,H,W] seen from rx_pos [R,3]:
+    mean_r (C_r L_r - alpha log C_r), C_r from the packed confidence network
+    conf_mlp ([5 hidden + 1] float32, or None: C_r = 1).  Returns (loss [1],
+    dL/dimg or None, dL/dconf_mlp or None, L_r [R], C_r [R])."""
+    x = gh.f32(img)
+    y = gh.f32(target)
+    if x.dim() == 2:
+        x, y = x.unsqueeze(0), y.unsqueeze(0)
+    if x.shape != y.shape or x.dim() != 3:
+        raise RuntimeError("img and target must both be [H,W] or [R,H,W]")
+    R, H, W = x.shape
+    rx = gh.f32(rx_pos).reshape(-1, 3)
+    if rx.size(0) != R:
+        raise RuntimeError("one receiver position per spectrum is required")
+    mlp = None if conf_mlp is None else gh.f32(conf_mlp).reshape(-1)
+    if mlp is not None and mlp.numel() != 5 * hidden + 1:
+        raise RuntimeError(f"the packed confidence network holds 5 * hidden + 1 = {5 * hidden + 1} floats")
+    gh.require_cuda(x, y, rx, mlp)
+    dev = x.device
+    L = gh.lib()
+    loss = torch.empty(1, dtype=torch.float32, device=dev)
+    rec = torch.empty(R, dtype=torch.float32, device=dev)
+    conf = torch.empty(R, dtype=torch.float32, device=dev)
+    dimg = torch.empty_like(x) if want_grad else None
+    dmlp = torch.empty_like(mlp) if (want_grad and mlp is not None) else None
+    work = torch.empty(L.gsparc_conf_loss_work_size(R, H, W), dtype=torch.uint8, device=dev)
+    gh.check(L.gsparc_loss_l1_ssim_conf(R, H, W, float(lambda_ssim), SSIM_C1, SSIM_C2, float(alpha), int(hidden),
+                                        gh.ptr(mlp), gh.ptr(rx), gh.ptr(x), gh.ptr(y), gh.ptr(work), gh.ptr(loss),
+                                        gh.ptr(rec), gh.ptr(conf), gh.ptr(dimg), gh.ptr(dmlp),
+                                        gh.stream_handle(dev)), "confidence-weighted loss")
+    if dimg is not None and img.dim() == 2:
+        dimg = dimg[0]
+    return loss, dimg, dmlp, rec, conf
+
+
+class ConfidenceL1SSIMLoss(torch.autograd.Function):
+    """Autograd form of confidence_l1_ssim_loss: gradients to img and to the
+    packed confidence network."""
+
+    @staticmethod
+    def forward(ctx, img, target, rx_pos, conf_packed, hidden=32, alpha=CONFIDENCE_ALPHA,
+                lambda_ssim=LAMBDA_SSIM):
+        loss, dimg, dmlp, rec, conf = confidence_l1_ssim_loss(img, target, rx_pos, conf_packed, hidden, alpha,
+                                                              lambda_ssim, want_grad=True)
+        ctx.save_for_backward(dimg, dmlp if dmlp is not None else torch.empty(0, device=img.device))
+        ctx.has_mlp = dmlp is not None
+        return loss.reshape(())
+
+    @staticmethod
+    def backward(ctx, g):
+        dimg, dmlp = ctx.saved_tensors
+        return dimg * g, None, None, (dmlp * g if ctx.has_mlp else None), None, None, None
+
+
+def confidence_loss(img, target, rx_pos, conf_mlp: ConfidenceMLP = None, alpha: float = CONFIDENCE_ALPHA,
+                    lambda_ssim: float = LAMBDA_SSIM):
+    """loss = mean_r (C_r L_r - alpha log C_r) with autograd to img and to
+    the ConfidenceMLP's parameters (train.py:418-436)."""
+    packed = None if conf_mlp is None else conf_mlp.packed()
+    hidden = 32 if conf_mlp is None else conf_mlp.hidden
+    return ConfidenceL1SSIMLoss.apply(img, target, rx_pos, packed, hidden, alpha, lambda_ssim)
 
 
 def csi_loss(out: torch.Tensor, target: torch.Tensor, lambda_ssim: float = LAMBDA_SSIM, want_grad: bool = True):
@@ -167,7 +273,15 @@ class GraphedTrainingStep(_GraphedStep):
 
     def __init__(self, means, scales, rotations, opacities, emission_mlps, viewmatrices, tx_pos, rx_pos, target,
                  image_height, image_width, capacity, scale_modifier=1.0, grad_mode="reference",
-                 lambda_ssim=LAMBDA_SSIM):
+                 lambda_ssim=LAMBDA_SSIM, confidence=None, confidence_hidden=32, confidence_alpha=CONFIDENCE_ALPHA):
+        """confidence: None (the plain loss) or the packed ConfidenceMLP
+        [5 hidden + 1] as a live tensor (ConfidenceMLP.packed() copied into
+        it; update it in place): the loss is then the confidence-weighted one
+        and grads["confidence"] its gradient, `rec_loss` / `conf` the per
+        spectrum L_r and C_r."""
+        self.conf = None if confidence is None else _require_param(confidence, "confidence")
+        self.conf_hidden = int(confidence_hidden)
+        self.conf_alpha = float(confidence_alpha)
         self.params = dict(means=_require_param(means, "means"), scales=_require_param(scales, "scales"),
                            rotations=_require_param(rotations, "rotations"),
                            opacities=_require_param(opacities, "opacities"),
@@ -208,7 +322,29 @@ class GraphedTrainingStep(_GraphedStep):
         self.grads = dict(means=torch.empty((P, 3), **f32), emission_mlps=torch.empty(P * 194, **f32),
                           opacities=torch.empty((P, 1), **f32), scales=torch.empty((P, 3), **f32),
                           rotations=torch.empty((P, 4), **f32))
+        if self.conf is not None:
+            if self.conf.numel() != 5 * self.conf_hidden + 1:
+                raise RuntimeError("confidence must hold 5 * confidence_hidden + 1 floats")
+            self.work = torch.empty(L.gsparc_conf_loss_work_size(R, H, W), dtype=torch.uint8, device=dev)
+            self.grads["confidence"] = torch.empty_like(self.conf)
+            self.rec_loss = torch.empty(R, **f32)
+            self.confidence = torch.empty(R, **f32)
         self._capture(dev)
+
+    def _loss(self, stream):
+        L = gh.lib()
+        R, W, H = self.R, self.W, self.H
+        if self.conf is None:
+            gh.check(L.gsparc_loss_l1_ssim(R, H, W, self.lambda_ssim, SSIM_C1, SSIM_C2, gh.ptr(self.out_real),
+                                           gh.ptr(self.target), gh.ptr(self.work), gh.ptr(self.loss),
+                                           gh.ptr(self.dimg), stream), "graphed step: loss")
+        else:
+            gh.check(L.gsparc_loss_l1_ssim_conf(R, H, W, self.lambda_ssim, SSIM_C1, SSIM_C2, self.conf_alpha,
+                                                self.conf_hidden, gh.ptr(self.conf), gh.ptr(self.rx),
+                                                gh.ptr(self.out_real), gh.ptr(self.target), gh.ptr(self.work),
+                                                gh.ptr(self.loss), gh.ptr(self.rec_loss), gh.ptr(self.confidence),
+                                                gh.ptr(self.dimg), gh.ptr(self.grads["confidence"]), stream),
+                     "graphed step: confidence-weighted loss")
 
     def _record(self, stream):
         L = gh.lib()
@@ -221,9 +357,7 @@ class GraphedTrainingStep(_GraphedStep):
                                             gh.ptr(self.img), gh.ptr(self.bin), self.capacity, gh.ptr(self.radii),
                                             gh.ptr(self.out_real), gh.ptr(self.out_imag), 0, stream),
                  "graphed step: forward")
-        gh.check(L.gsparc_loss_l1_ssim(R, H, W, self.lambda_ssim, SSIM_C1, SSIM_C2, gh.ptr(self.out_real),
-                                       gh.ptr(self.target), gh.ptr(self.work), gh.ptr(self.loss), gh.ptr(self.dimg),
-                                       stream), "graphed step: loss")
+        self._loss(stream)
         g = self.grads
         gh.check(L.gsparc_rasterize_backward(self.mode, P, R, W, H, gh.ptr(p["means"]), gh.ptr(p["scales"]),
                                              gh.ptr(p["rotations"]), gh.ptr(p["emission_mlps"]), None,

@@ -270,6 +270,27 @@ size_t gsparc_loss_work_size(int planes, int H, int W);
 int gsparc_loss_l1_ssim(int planes, int H, int W, float lambda_ssim, float C1, float C2, const float* img,
                         const float* target, void* work, float* loss, float* dL_dimg, void* stream);
 
+/* The reference's confidence-weighted training loss, value and gradients in
+ * one call, over `planes` spectra (one receiver each):
+ *   L_r = (1 - lambda_ssim) mean|img_r - target_r| + lambda_ssim (1 - mean SSIM(img_r, target_r))
+ *   C_r = 1 + exp(fc2 . relu(fc1 rx_r + b1) + b2)      (the confidence network)
+ *   loss = mean_r (C_r L_r - alpha log C_r)
+ * conf_mlp: the network packed as [fc1.weight hidden*3 | fc1.bias hidden |
+ * fc2.weight hidden | fc2.bias 1] (torch Linear layouts), or NULL: C_r = 1,
+ * the reference's value without a network.  rx_pos [planes, 3].  Outputs:
+ * loss [1]; rec_loss [planes] = L_r and confidence [planes] = C_r (each may
+ * be NULL); dL_dimg like img (may be NULL: no image gradient); dconf_mlp
+ * like conf_mlp (may be NULL).  Sums run in a fixed order (deterministic).
+ * work: gsparc_conf_loss_work_size bytes.  With one plane this is exactly
+ * the reference's loss of one training step.  Replaces train.py:418-436
+ * (reconstruction loss, DUSt3R-style confidence weighting) and the
+ * ConfidenceMLP of gaussian.py:575-609 with its backward. */
+size_t gsparc_conf_loss_work_size(int planes, int H, int W);
+int gsparc_loss_l1_ssim_conf(int planes, int H, int W, float lambda_ssim, float C1, float C2, float alpha,
+                             int hidden, const float* conf_mlp, const float* rx_pos, const float* img,
+                             const float* target, void* work, float* loss, float* rec_loss, float* confidence,
+                             float* dL_dimg, float* dconf_mlp, void* stream);
+
 /* gsparc_loss_l1_ssim with no workspace argument, calling no allocator: its
  * scratch (the three SSIM partial-derivative maps and the per-workgroup loss
  * partials) is a library-owned device array of GSPARC_LOSS_SCRATCH_BYTES on

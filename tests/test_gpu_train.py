@@ -143,3 +143,107 @@ def test_graphed_step_reports_overflow():
     for g in step.grads.values():
         assert float(g.abs().max()) == 0.0
     step.close()
+
+
+def _conf_inputs(R, H, W, seed):
+    g = torch.Generator().manual_seed(seed)
+    img = torch.rand((R, H, W), generator=g) * 0.4
+    tgt = torch.rand((R, H, W), generator=g) * 0.4
+    rx = torch.rand((R, 3), generator=g) * torch.tensor([6.0, 6.0, 3.0])
+    torch.manual_seed(seed)
+    net = T.ConfidenceMLP(32)
+    with torch.no_grad():
+        net.fc2.bias.fill_(0.3)          # away from C = 2 exactly
+    return img, tgt, rx, net
+
+
+@pytest.mark.parametrize("R,H,W", [(1, 90, 360), (5, 37, 50)])
+def test_confidence_loss_matches_torch_and_fp64(R, H, W):
+    """mean_r (C_r L_r - alpha log C_r) (train.py:418-436) against (a) the
+    same loss written with torch ops (L1, fused_ssim per spectrum and the
+    ConfidenceMLP in torch) through autograd, and (b) an FP64 evaluation
+    from the oracle's SSIM and the network's definition."""
+    print("confidence loss", R, H, W, flush=True)
+    img, tgt, rx, net = _conf_inputs(R, H, W, 300 + R)
+    netg = T.ConfidenceMLP(32, device="cuda")
+    netg.load_state_dict(net.state_dict())
+    ic = img.cuda().requires_grad_(True)
+    loss = T.confidence_loss(ic, tgt.cuda(), rx.cuda(), netg, alpha=0.3)
+    loss.backward()
+    torch.cuda.synchronize()
+    # (a) torch composition
+    net_t = T.ConfidenceMLP(32, device="cuda")
+    net_t.load_state_dict(net.state_dict())
+    it = img.cuda().requires_grad_(True)
+    C = net_t(rx.cuda())
+    terms = []
+    for r in range(R):
+        Lr = 0.8 * (it[r] - tgt[r].cuda()).abs().mean() + 0.2 * (1.0 - fused_ssim(it[r][None, None],
+                                                                               tgt[r].cuda()[None, None]))
+        terms.append(C[r] * Lr - 0.3 * torch.log(C[r]))
+    lt = torch.stack(terms).mean()
+    lt.backward()
+    assert abs(float(loss) - float(lt)) <= 1e-5 * abs(float(lt))
+    assert _rel(ic.grad, it.grad) < 1e-5
+    for name in ("fc1.weight", "fc1.bias", "fc2.weight", "fc2.bias"):
+        a = dict(netg.named_parameters())[name].grad
+        b = dict(net_t.named_parameters())[name].grad
+        assert _rel(a, b) < 1e-5, name
+    # (b) FP64
+    x = img.numpy().astype(np.float64)[:, None]
+    y = tgt.numpy().astype(np.float64)[:, None]
+    m, d1, d2, d3 = O.ssim_fwd(x, y, dtype=np.float64)
+    W1 = net.fc1.weight.detach().double().numpy()
+    b1 = net.fc1.bias.detach().double().numpy()
+    w2 = net.fc2.weight.detach().double().numpy()[0]
+    b2 = float(net.fc2.bias.detach()[0])
+    rxd = rx.double().numpy()
+    Cd = 1.0 + np.exp(np.maximum(rxd @ W1.T + b1, 0.0) @ w2 + b2)
+    Ld = 0.8 * np.abs(x - y).mean(axis=(1, 2, 3)) + 0.2 * (1.0 - m.mean(axis=(1, 2, 3)))
+    ref = float(np.mean(Cd * Ld - 0.3 * np.log(Cd)))
+    assert abs(float(loss) - ref) <= 1e-6 * abs(ref)
+    n = H * W
+    gd = Cd[:, None, None] / R * (0.8 * np.sign(x - y)[:, 0] / n +
+                                   O.ssim_bwd(x, y, np.full_like(x, -0.2 / n), d1, d2, d3, dtype=np.float64)[:, 0])
+    assert _rel(ic.grad, torch.from_numpy(gd)) < 1e-5
+
+
+def test_confidence_loss_without_network_is_the_plain_loss():
+    img, tgt, rx, _ = _conf_inputs(4, 90, 360, 311)
+    loss, dimg, dmlp, rec, conf = T.confidence_l1_ssim_loss(img.cuda(), tgt.cuda(), rx.cuda(), None)
+    l0, g0 = T.l1_ssim_loss(img.cuda(), tgt.cuda())
+    torch.cuda.synchronize()
+    assert dmlp is None and bool((conf == 1.0).all())
+    assert abs(float(loss) - float(l0)) <= 1e-6 * abs(float(l0))
+    assert torch.equal(dimg, g0)
+    # the per-spectrum losses average to the batch loss
+    assert abs(float(rec.double().mean()) - float(l0)) <= 1e-6 * abs(float(l0))
+
+
+def test_graphed_step_with_confidence_matches_eager():
+    sc, d, target = _step_setup(seed=65, P=1500, R=3)
+    K = wr.forward_batched(d["means"], d["mlp"], d["opac"], d["scales"], d["rots"], sc.H, sc.W, 1.0, d["views"],
+                           d["tx"], d["rx"])[0]
+    torch.manual_seed(5)
+    net = T.ConfidenceMLP(32, device="cuda")
+    packed = net.packed().detach().clone().contiguous()
+    live = dict(means=d["means"].clone(), scales=d["scales"].clone(), rots=d["rots"].clone(),
+                opac=d["opac"].clone(), mlp=d["mlp"].clone())
+    step = T.GraphedTrainingStep(live["means"], live["scales"], live["rots"], live["opac"], live["mlp"], d["views"],
+                                 d["tx"], d["rx"], target, sc.H, sc.W, capacity=2 * K, confidence=packed)
+    # eager: autograd rasterizer + confidence loss
+    params = [d[k].clone().requires_grad_(True) for k in ("means", "scales", "rots", "opac", "mlp")]
+    m, s_, q, o, w = params
+    st = cr.BatchedSetting(sc.H, sc.W, 1.0, d["views"], d["tx"], d["rx"], "reference")
+    ore, oim, radii = cr.rasterize_gaussians_batched(m, w, o, s_, q, st)
+    loss = T.confidence_loss(ore, target, d["rx"], net)
+    loss.backward()
+    for rep in range(2):
+        step.replay()
+        torch.cuda.synchronize()
+        assert abs(float(step.loss) - float(loss)) <= 1e-5 * abs(float(loss)), rep
+        for k, p_ in zip(("means", "scales", "rotations", "opacities", "emission_mlps"), params):
+            assert _rel(step.grads[k].reshape(p_.shape), p_.grad) < 1e-4, (rep, k)
+        assert _rel(step.grads["confidence"], torch.cat([net.fc1.weight.grad.reshape(-1), net.fc1.bias.grad,
+                                                          net.fc2.weight.grad.reshape(-1), net.fc2.bias.grad])) < 1e-5
+    step.close()
