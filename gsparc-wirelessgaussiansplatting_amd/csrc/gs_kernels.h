@@ -365,6 +365,59 @@ size_t loss_reduce2_scratch();
 hipError_t launch_loss_reduce2(const float* ps, int nbs, double inv_s, const float* pl, int nbl, double inv_l,
                                float l1w, float sw, double* stage, float* loss, hipStream_t s);
 
+// ---- the training step around the rasterizer (k_train.hip) ----
+// Adam parameter groups, in the bit order of GSPARC_GROUP_*.
+enum { AG_CENTRE = 0, AG_SCALING, AG_ROTATION, AG_OPACITY, AG_EMISSION, AG_CONFIDENCE, AG_N };
+
+// The gsparc_gaussians arrays (P entries each).
+struct Gaussians {
+    float *centre, *log_scale, *rotation, *logit, *mlp;
+    float *scale, *rotation_n, *opacity;
+    float *state, *mlp_state;
+    float *accum, *denom, *max_radius;
+};
+// Gradients with respect to the activations.
+struct GaussianGrads { const float *centre, *scale, *rotation, *opacity, *mlp; };
+
+struct AdamCfg {
+    double beta1, beta2, eps;
+    double lr[AG_N];                    // per group; lr[AG_CENTRE] comes from the schedule
+    double lr_init, lr_final, lr_delay_mult;
+    int lr_delay_steps, lr_max_steps;
+    int clamp;
+    float lo[3], hi[3];
+};
+// Per-step scalars (k_adam_prep), in the training state after the counters.
+struct AdamStep {
+    float step_size[AG_N], bc2_sqrt[AG_N];
+    float one_minus_b1, b2, one_minus_b2, eps;
+};
+constexpr size_t TRAIN_STEP_OFFSET = 64;   // byte offset of AdamStep in the training state
+
+struct DensifyCfg {
+    int densify;
+    float grad_threshold, clone_max_scale, min_opacity, prune_max_scale;
+    int use_bounds;
+    float lo[3], hi[3];
+};
+struct DensifyPlan { uint32_t P, n_keep_orig, n_clone, n_split, n_cand, P_new; };
+// Work arrays of gsparc_densify_and_prune for P Gaussians: flags [P], keep
+// [2P], order [2P], src_of [2P], block_counts [3 * ceil(2P / 1024)],
+// totals [4] (the three list lengths, then the kept count).
+struct DensifyWork { uint8_t *flags, *keep; uint32_t *order, *src_of, *block_counts, *totals; };
+
+hipError_t launch_activate(int P, const Gaussians& p, hipStream_t s);
+hipError_t launch_adam_step(int P, int groups, const AdamCfg& c, const Gaussians& p, const GaussianGrads& d,
+                            size_t n_conf, float* conf, const float* dconf, float* conf_state, uint32_t* counters,
+                            AdamStep* step, hipStream_t s);
+hipError_t launch_densify_stats(int P, int R, float grad_scale, const float* dmeans2d, const int* radii,
+                                const Gaussians& p, hipStream_t s);
+hipError_t launch_opacity_reset(int P, const Gaussians& p, hipStream_t s);
+hipError_t launch_densify_lists(int P, const DensifyCfg& c, const Gaussians& p, const float* noise,
+                                const DensifyWork& w, hipStream_t s);
+hipError_t launch_densify_gather(const DensifyPlan& pl, const DensifyCfg& c, const Gaussians& p, const Gaussians& q,
+                                 const float* noise, const DensifyWork& w, hipStream_t s);
+
 // loss = l1w * sum|img1 - img2| * inv_n + sw * (1 - sum(map) * inv_n) from
 // the per-workgroup partials of k_ssim_fwd, in a fixed order.
 hipError_t launch_loss_reduce(const float* part, int nb, double inv_n, float l1w, float sw, float* loss,

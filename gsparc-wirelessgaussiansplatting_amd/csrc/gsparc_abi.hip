@@ -1294,6 +1294,198 @@ int gsparc_mlp1_backward(int P, int R, const float* pe1, const float* rxf, const
     return GSPARC_OK;
 }
 
+// ---- the training step around the rasterizer
+namespace {
+bool gaussians_ok(const gsparc_gaussians* g)
+{
+    return g && g->centre && g->log_scale && g->rotation && g->logit && g->emission_mlps && g->scale &&
+           g->rotation_n && g->opacity && g->state && g->state_mlp && g->accum && g->denom && g->max_radius;
+}
+
+gs::Gaussians to_dev(const gsparc_gaussians& g)
+{
+    return gs::Gaussians{g.centre, g.log_scale, g.rotation, g.logit, g.emission_mlps, g.scale, g.rotation_n,
+                         g.opacity, g.state, g.state_mlp, g.accum, g.denom, g.max_radius};
+}
+
+int check_gaussians(int P, const gsparc_gaussians* g)
+{
+    if (P < 0) return fail(GSPARC_ERR_ARG, "negative Gaussian count");
+    if ((int64_t)P * 388 >= (int64_t)1 << 31) return fail(GSPARC_ERR_ARG, "too many Gaussians");
+    if (!gaussians_ok(g)) return fail(GSPARC_ERR_ARG, "null pointer in the Gaussian set");
+    return GSPARC_OK;
+}
+}  // namespace
+
+int gsparc_gaussians_activate(int P, const gsparc_gaussians* g, void* stream)
+{
+    if (int e = check_gaussians(P, g)) return e;
+    GS_LAUNCH("activate", gs::launch_activate(P, to_dev(*g), (hipStream_t)stream), (hipStream_t)stream);
+    return GSPARC_OK;
+}
+
+int gsparc_adam_step(int P, int groups, const gsparc_adam_config* cfg, const gsparc_gaussians* g,
+                     const float* dcentre, const float* dscale, const float* drotation, const float* dopacity,
+                     const float* demission_mlps, int n_conf, float* conf, const float* dconf, float* conf_state,
+                     void* train_state, void* stream)
+{
+    if (int e = check_gaussians(P, g)) return e;
+    if (!cfg || !train_state) return fail(GSPARC_ERR_ARG, "null pointer");
+    if (groups & ~GSPARC_GROUPS_ALL) return fail(GSPARC_ERR_ARG, "unknown parameter group");
+    if (((uintptr_t)train_state & 255) != 0) return fail(GSPARC_ERR_ARG, "train_state must be 256-byte aligned");
+    if ((groups & GSPARC_GROUP_CENTRE) && !dcentre) return fail(GSPARC_ERR_ARG, "null centre gradient");
+    if ((groups & GSPARC_GROUP_SCALING) && !dscale) return fail(GSPARC_ERR_ARG, "null scale gradient");
+    if ((groups & GSPARC_GROUP_ROTATION) && !drotation) return fail(GSPARC_ERR_ARG, "null rotation gradient");
+    if ((groups & GSPARC_GROUP_OPACITY) && !dopacity) return fail(GSPARC_ERR_ARG, "null opacity gradient");
+    if ((groups & GSPARC_GROUP_EMISSION) && !demission_mlps) return fail(GSPARC_ERR_ARG, "null MLP gradient");
+    if (n_conf < 0) return fail(GSPARC_ERR_ARG, "negative confidence size");
+    if (n_conf > 0 && (groups & GSPARC_GROUP_CONFIDENCE) && (!conf || !dconf || !conf_state))
+        return fail(GSPARC_ERR_ARG, "null confidence pointer");
+    if (!(cfg->beta1 >= 0.0 && cfg->beta1 < 1.0 && cfg->beta2 >= 0.0 && cfg->beta2 < 1.0 && cfg->eps >= 0.0))
+        return fail(GSPARC_ERR_ARG, "Adam betas must lie in [0, 1) and eps be non-negative");
+    if (cfg->lr_centre_max_steps <= 0) return fail(GSPARC_ERR_ARG, "lr_centre_max_steps must be positive");
+    if ((cfg->lr_centre_init != 0.0 || cfg->lr_centre_final != 0.0) &&
+        !(cfg->lr_centre_init > 0.0 && cfg->lr_centre_final > 0.0))
+        return fail(GSPARC_ERR_ARG, "the centre schedule needs positive rates (or both zero)");
+    gs::AdamCfg c;
+    c.beta1 = cfg->beta1;
+    c.beta2 = cfg->beta2;
+    c.eps = cfg->eps;
+    c.lr[gs::AG_CENTRE] = 0.0;
+    c.lr[gs::AG_SCALING] = cfg->lr_scaling;
+    c.lr[gs::AG_ROTATION] = cfg->lr_rotation;
+    c.lr[gs::AG_OPACITY] = cfg->lr_opacity;
+    c.lr[gs::AG_EMISSION] = cfg->lr_emission;
+    c.lr[gs::AG_CONFIDENCE] = cfg->lr_confidence;
+    c.lr_init = cfg->lr_centre_init;
+    c.lr_final = cfg->lr_centre_final;
+    c.lr_delay_mult = cfg->lr_centre_delay_mult;
+    c.lr_delay_steps = cfg->lr_centre_delay_steps;
+    c.lr_max_steps = cfg->lr_centre_max_steps;
+    c.clamp = cfg->clamp_centres ? 1 : 0;
+    for (int k = 0; k < 3; ++k) {
+        c.lo[k] = cfg->scene_min[k];
+        c.hi[k] = cfg->scene_max[k];
+    }
+    const gs::GaussianGrads d{dcentre, dscale, drotation, dopacity, demission_mlps};
+    uint32_t* counters = (uint32_t*)train_state;
+    gs::AdamStep* st = (gs::AdamStep*)((char*)train_state + gs::TRAIN_STEP_OFFSET);
+    static_assert(gs::TRAIN_STEP_OFFSET + sizeof(gs::AdamStep) <= GSPARC_TRAIN_STATE_BYTES, "training state size");
+    hipStream_t s = (hipStream_t)stream;
+    GS_LAUNCH("adam", gs::launch_adam_step(P, groups, c, to_dev(*g), d, (size_t)n_conf, conf, dconf, conf_state,
+                                           counters, st, s), s);
+    return GSPARC_OK;
+}
+
+int gsparc_densify_stats(int P, int R, float grad_scale, const float* dmeans2D, const int* radii,
+                         const gsparc_gaussians* g, void* stream)
+{
+    if (int e = check_gaussians(P, g)) return e;
+    if (R < 0 || (int64_t)P * R >= (int64_t)1 << 31) return fail(GSPARC_ERR_ARG, "bad receiver count");
+    if ((int64_t)P * R > 0 && (!dmeans2D || !radii)) return fail(GSPARC_ERR_ARG, "null pointer");
+    GS_LAUNCH("densify_stats", gs::launch_densify_stats(P, R, grad_scale, dmeans2D, radii, to_dev(*g),
+                                                        (hipStream_t)stream), (hipStream_t)stream);
+    return GSPARC_OK;
+}
+
+int gsparc_opacity_reset(int P, const gsparc_gaussians* g, void* stream)
+{
+    if (int e = check_gaussians(P, g)) return e;
+    GS_LAUNCH("opacity_reset", gs::launch_opacity_reset(P, to_dev(*g), (hipStream_t)stream), (hipStream_t)stream);
+    return GSPARC_OK;
+}
+
+namespace {
+// densify work layout for P Gaussians (gs::DensifyWork)
+struct DensifyLayout { size_t flags, keep, order, src_of, block_counts, totals, bytes; };
+DensifyLayout densify_layout(int P)
+{
+    const size_t n2 = 2 * (size_t)P;
+    const size_t nb = std::max<size_t>(1, (n2 + 1023) / 1024);
+    DensifyLayout L;
+    size_t o = 0;
+    L.totals = o; o = al(o + 4 * sizeof(uint32_t));
+    L.flags = o; o = al(o + (size_t)P);
+    L.keep = o; o = al(o + n2);
+    L.order = o; o = al(o + n2 * sizeof(uint32_t));
+    L.src_of = o; o = al(o + n2 * sizeof(uint32_t));
+    L.block_counts = o; o = al(o + 3 * nb * sizeof(uint32_t));
+    L.bytes = o + ALIGN;
+    return L;
+}
+}  // namespace
+
+size_t gsparc_densify_work_size(int P)
+{
+    if (P < 0) return 0;
+    return densify_layout(P).bytes;
+}
+
+int gsparc_densify_and_prune(int P, const gsparc_densify_config* cfg, const gsparc_gaussians* src,
+                             const float* noise, const gsparc_gaussians* dst, int dst_capacity, void* work,
+                             int* P_new, int* n_clone, int* n_split, void* stream)
+{
+    if (int e = check_gaussians(P, src)) return e;
+    if (!gaussians_ok(dst) || !cfg || !work || !P_new || !n_clone || !n_split)
+        return fail(GSPARC_ERR_ARG, "null pointer");
+    if ((int64_t)P * 2 * 388 >= (int64_t)1 << 31) return fail(GSPARC_ERR_ARG, "too many Gaussians");
+    if (cfg->densify && P > 0 && !noise) return fail(GSPARC_ERR_ARG, "null noise");
+    if (dst_capacity < 0) return fail(GSPARC_ERR_ARG, "negative capacity");
+    if (src->centre == dst->centre || src->state == dst->state || src->emission_mlps == dst->emission_mlps)
+        return fail(GSPARC_ERR_ARG, "src and dst must be distinct arrays");
+    hipStream_t s = (hipStream_t)stream;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    GS_HIP(hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone)
+        return fail(GSPARC_ERR_ARG, "densify_and_prune reads the new count back: not while capturing");
+    *P_new = 0;
+    *n_clone = 0;
+    *n_split = 0;
+    if (P == 0) return GSPARC_OK;
+    gs::DensifyCfg c;
+    c.densify = cfg->densify ? 1 : 0;
+    c.grad_threshold = cfg->grad_threshold;
+    c.clone_max_scale = cfg->clone_max_scale;
+    c.min_opacity = cfg->min_opacity;
+    c.prune_max_scale = cfg->prune_max_scale;
+    c.use_bounds = cfg->use_bounds ? 1 : 0;
+    for (int k = 0; k < 3; ++k) {
+        c.lo[k] = cfg->scene_min[k];
+        c.hi[k] = cfg->scene_max[k];
+    }
+    const DensifyLayout L = densify_layout(P);
+    char* base = base_of(work);
+    gs::DensifyWork w;
+    w.totals = (uint32_t*)(base + L.totals);
+    w.flags = (uint8_t*)(base + L.flags);
+    w.keep = (uint8_t*)(base + L.keep);
+    w.order = (uint32_t*)(base + L.order);
+    w.src_of = (uint32_t*)(base + L.src_of);
+    w.block_counts = (uint32_t*)(base + L.block_counts);
+    const gs::Gaussians p = to_dev(*src), q = to_dev(*dst);
+    GS_LAUNCH("densify_lists", gs::launch_densify_lists(P, c, p, noise, w, s), s);
+    uint32_t tot[4];
+    GS_HIP(hipMemcpyAsync(tot, w.totals, sizeof(tot), hipMemcpyDeviceToHost, s));
+    GS_HIP(hipStreamSynchronize(s));
+    gs::DensifyPlan pl;
+    pl.P = (uint32_t)P;
+    pl.n_keep_orig = tot[0];
+    pl.n_clone = tot[1];
+    pl.n_split = tot[2];
+    pl.n_cand = tot[0] + tot[1] + 2 * tot[2];
+    pl.P_new = tot[3];
+    *P_new = (int)pl.P_new;
+    *n_clone = (int)pl.n_clone;
+    *n_split = (int)pl.n_split;
+    if (pl.n_keep_orig + pl.n_split != (uint32_t)P || pl.P_new > pl.n_cand)
+        return fail(GSPARC_ERR_HIP, "densify: inconsistent list lengths");
+    if (pl.P_new > (uint32_t)dst_capacity)
+        return fail(GSPARC_ERR_CAPACITY, "densify: the new set needs " + std::to_string(pl.P_new) +
+                                             " Gaussians, dst holds " + std::to_string(dst_capacity));
+    GS_LAUNCH("densify_gather", gs::launch_densify_gather(pl, c, p, q, noise, w, s), s);
+    return GSPARC_OK;
+}
+
 // ---- HIP graphs of the library's own launch sequences
 struct gsparc_graph {
     hipGraph_t graph = nullptr;

@@ -42,6 +42,38 @@ class Batch(C.Structure):
 
 _BP = C.POINTER(Batch)
 
+
+class GaussiansC(C.Structure):
+    """gsparc_gaussians: the device arrays of a training run's Gaussian set."""
+    _fields_ = [(n, C.c_void_p) for n in (
+        "centre", "log_scale", "rotation", "logit", "emission_mlps", "scale", "rotation_n", "opacity", "state",
+        "state_mlp", "accum", "denom", "max_radius")]
+
+
+class AdamConfig(C.Structure):
+    """gsparc_adam_config."""
+    _fields_ = [("beta1", C.c_double), ("beta2", C.c_double), ("eps", C.c_double),
+                ("lr_centre_init", C.c_double), ("lr_centre_final", C.c_double),
+                ("lr_centre_delay_mult", C.c_double), ("lr_centre_delay_steps", C.c_int),
+                ("lr_centre_max_steps", C.c_int), ("lr_scaling", C.c_double), ("lr_rotation", C.c_double),
+                ("lr_opacity", C.c_double), ("lr_emission", C.c_double), ("lr_confidence", C.c_double),
+                ("clamp_centres", C.c_int), ("scene_min", C.c_float * 3), ("scene_max", C.c_float * 3)]
+
+
+class DensifyConfig(C.Structure):
+    """gsparc_densify_config."""
+    _fields_ = [("densify", C.c_int), ("grad_threshold", C.c_float), ("clone_max_scale", C.c_float),
+                ("min_opacity", C.c_float), ("prune_max_scale", C.c_float), ("use_bounds", C.c_int),
+                ("scene_min", C.c_float * 3), ("scene_max", C.c_float * 3)]
+
+
+GROUP_CENTRE, GROUP_SCALING, GROUP_ROTATION, GROUP_OPACITY, GROUP_EMISSION, GROUP_CONFIDENCE = 1, 2, 4, 8, 16, 32
+GROUPS_ALL = 63
+TRAIN_STATE_BYTES = 256
+ERR_CAPACITY = 3
+
+_GP = C.POINTER(GaussiansC)
+
 _SIGS = {
     "gsparc_version": (C.c_char_p, []),
     "gsparc_last_error": (C.c_char_p, []),
@@ -99,6 +131,13 @@ _SIGS = {
     "gsparc_conf_loss_work_size": (_sz, [_i, _i, _i]),
     "gsparc_loss_l1_ssim_conf": (_i, [_i, _i, _i, _f, _f, _f, _f, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                       _vp, _vp]),
+    "gsparc_gaussians_activate": (_i, [_i, _GP, _vp]),
+    "gsparc_adam_step": (_i, [_i, _i, C.POINTER(AdamConfig), _GP] + [_vp] * 5 + [_i, _vp, _vp, _vp, _vp, _vp]),
+    "gsparc_densify_stats": (_i, [_i, _i, _f, _vp, _vp, _GP, _vp]),
+    "gsparc_opacity_reset": (_i, [_i, _GP, _vp]),
+    "gsparc_densify_work_size": (_sz, [_i]),
+    "gsparc_densify_and_prune": (_i, [_i, C.POINTER(DensifyConfig), _GP, _vp, _GP, _i, _vp, C.POINTER(_i),
+                                      C.POINTER(_i), C.POINTER(_i), _vp]),
     "gsparc_graph_begin": (_i, [_vp]),
     "gsparc_graph_end": (_i, [_vp, C.POINTER(_vp)]),
     "gsparc_graph_launch": (_i, [_vp, _vp]),

@@ -402,6 +402,136 @@ int gsparc_rasterize_backward_scenes(int grad_mode, const gsparc_batch* batch, c
                                      float* dmeans3D, float* demission_mlps, float* dopacity, float* dcov3D,
                                      float* dscales, float* drotations, float* demission, void* stream);
 
+/* ---- the training step around the rasterizer ------------------------------
+ * The per-Gaussian parameter set of a training run and its optimizer state
+ * (gaussian.py:33-38, 122-197), device pointers, P entries each:
+ *   centre [P,3], log_scale [P,3], rotation [P,4] (raw quaternion), logit [P],
+ *   emission_mlps [P*194] (W1 [P,32,3] | b1 [P,32] | W2 [P,2,32] | b2 [P,2]);
+ *   the activations the rasterizer reads: scale [P,3] = exp(log_scale),
+ *   rotation_n [P,4] = q / max(|q|, 1e-12), opacity [P] = sigmoid(logit);
+ *   Adam moments state [2][11 P] (exp_avg then exp_avg_sq, each centre 3P |
+ *   log_scale 3P | rotation 4P | logit P) and state_mlp [2][194 P];
+ *   densification statistics accum [P] (centre_gradient_accum), denom [P],
+ *   max_radius [P] (max_radii2D).
+ * A set laid out for P entries fits in arrays allocated for any capacity
+ * >= P (every section starts at a multiple of P). */
+typedef struct gsparc_gaussians {
+    float *centre, *log_scale, *rotation, *logit, *emission_mlps;
+    float *scale, *rotation_n, *opacity;
+    float *state, *state_mlp;
+    float *accum, *denom, *max_radius;
+} gsparc_gaussians;
+
+/* Adam parameter groups (training_setup, gaussian.py:175-197); the four
+ * emission MLP groups step together, as do the confidence network's four. */
+#define GSPARC_GROUP_CENTRE 1
+#define GSPARC_GROUP_SCALING 2
+#define GSPARC_GROUP_ROTATION 4
+#define GSPARC_GROUP_OPACITY 8
+#define GSPARC_GROUP_EMISSION 16
+#define GSPARC_GROUP_CONFIDENCE 32
+#define GSPARC_GROUPS_ALL 63
+
+/* torch.optim.Adam(eps=1e-15) settings and the centre learning-rate schedule
+ * get_expon_lr_func(lr_centre_init, lr_centre_final, lr_centre_delay_steps,
+ * lr_centre_delay_mult, lr_centre_max_steps) (utils.py:46-79, gaussian.py:
+ * 197-221); clamp_centres with scene_min / scene_max: the clamp after the
+ * step (train.py:558-562). */
+typedef struct gsparc_adam_config {
+    double beta1, beta2, eps;
+    double lr_centre_init, lr_centre_final, lr_centre_delay_mult;
+    int lr_centre_delay_steps, lr_centre_max_steps;
+    double lr_scaling, lr_rotation, lr_opacity, lr_emission, lr_confidence;
+    int clamp_centres;
+    float scene_min[3], scene_max[3];
+} gsparc_adam_config;
+
+/* Device training state of GSPARC_TRAIN_STATE_BYTES: uint32 words [0] the
+ * iteration being run (the reference's 1-based `iteration`; the caller sets
+ * it, each gsparc_adam_step advances it), [1..6] the steps taken by each
+ * group (torch's state["step"]), [7] set to 1 when a step leaves a NaN centre
+ * (train.py:565-566); the rest belongs to the library.  Start from zeros. */
+#define GSPARC_TRAIN_STATE_BYTES 256
+
+/* Write the activations (scale, rotation_n, opacity) from the raw
+ * parameters.  Replaces the get_scaling / get_rotation / get_opacity
+ * properties (gaussian.py:86-96). */
+int gsparc_gaussians_activate(int P, const gsparc_gaussians* g, void* stream);
+
+/* One optimizer step of the groups in `groups` (the others are left as they
+ * are and their step counts stay): gradients with respect to the
+ * activations (the rasterizer's dmeans3D, dscales, drotations, dopacity,
+ * demission_mlps) are carried through exp / normalize / sigmoid, NaN entries
+ * of the centre gradient are zeroed (train.py:441-444), Adam updates the raw
+ * parameters, the activations are rewritten and, with clamp_centres, the
+ * centres are clamped (also when the centre group does not step).  The
+ * confidence network (n_conf floats at conf, its gradient dconf and moments
+ * conf_state [2][n_conf]; n_conf may be 0) steps with GSPARC_GROUP_CONFIDENCE.
+ * Records no synchronisation: can be captured in a graph.  Replaces
+ * update_learning_rate (gaussian.py:214-221), optimizer.step over the groups
+ * of training_setup (gaussian.py:175-197; torch.optim.Adam) and the NaN guard
+ * and clamp around it (train.py:441-444, 558-566). */
+int gsparc_adam_step(int P, int groups, const gsparc_adam_config* cfg, const gsparc_gaussians* g,
+                     const float* dcentre, const float* dscale, const float* drotation, const float* dopacity,
+                     const float* demission_mlps, int n_conf, float* conf, const float* dconf, float* conf_state,
+                     void* train_state, void* stream);
+
+/* Densification statistics of one batch: for every receiver r that sees
+ * Gaussian i (radii[r,i] > 0): accum[i] += |dmeans2D[r,i,:2]| * grad_scale,
+ * denom[i] += 1, max_radius[i] = max(max_radius[i], radii[r,i]).  dmeans2D
+ * [R,P,3] and radii [R,P] as the backward and forward write them;
+ * grad_scale turns the gradient of a batch loss into the per-view gradient
+ * (R for a loss averaged over R spectra, 1 for one spectrum).  Replaces
+ * add_densification_stats (gaussian.py:538-551) and the max-radius update
+ * (train.py:510-516). */
+int gsparc_densify_stats(int P, int R, float grad_scale, const float* dmeans2D, const int* radii,
+                         const gsparc_gaussians* g, void* stream);
+
+/* reset_opacity (gaussian.py:223-231): logit = inverse_sigmoid(min(opacity,
+ * 0.01)) (non-finite results mapped as torch.nan_to_num(nan=0, posinf=1,
+ * neginf=0)), opacity rewritten, the logit's Adam moments zeroed (its step
+ * count stays, replace_tensor_to_optimizer, gaussian.py:233-253). */
+int gsparc_opacity_reset(int P, const gsparc_gaussians* g, void* stream);
+
+/* densify_and_prune settings: densify != 0 clones (mean gradient accum/denom
+ * >= grad_threshold and max scale <= clone_max_scale, the percent_dense *
+ * extent of gaussian.py:495) and splits (same gradient test, max scale
+ * above) before pruning; densify == 0 only prunes (the post-densification
+ * pruning of train.py:539-555).  Prune: opacity < min_opacity, max scale >
+ * prune_max_scale (0.1 * extent when the reference passes max_screen_size,
+ * else +inf), with use_bounds a centre outside [scene_min, scene_max].  The
+ * thresholds are compared in float, as torch compares a float tensor with a
+ * Python number: pass them rounded from double. */
+typedef struct gsparc_densify_config {
+    int densify;
+    float grad_threshold, clone_max_scale, min_opacity, prune_max_scale;
+    int use_bounds;
+    float scene_min[3], scene_max[3];
+} gsparc_densify_config;
+
+size_t gsparc_densify_work_size(int P);
+
+/* Densify and prune src (P Gaussians) into dst (distinct arrays allocated for
+ * dst_capacity Gaussians), in the reference's order: the originals that are
+ * not split, one clone per clone-selected original, the first child of every
+ * split original, then the second, each run in original order; then the
+ * pruned ones removed.  A split child of mu, scale s and quaternion q has
+ * centre R(q) (s * z) + mu with z row `child` of noise (standard normals
+ * [2 P, 3]: the first child of the k-th split original reads row k, the
+ * second row n_split + k) and log-scale log(s / 1.2).  Kept originals carry
+ * their Adam moments, new Gaussians start with zeros; the statistics of all
+ * restart at zero.  Synchronises the stream once (the new count); when
+ * P_new exceeds dst_capacity nothing is written to dst and the call returns
+ * GSPARC_ERR_CAPACITY with *P_new, *n_clone and *n_split set: call again
+ * with room.  Not while capturing.  Replaces densify_and_prune
+ * (gaussian.py:510-535) with densify_and_clone (493-508), densify_and_split
+ * (450-491), densification_postfix (395-429), prune_points (308-337) and the
+ * optimizer-state surgery of cat_tensors_to_optimizer / _prune_optimizer
+ * (266-393); with densify == 0, the pruning of train.py:539-555. */
+int gsparc_densify_and_prune(int P, const gsparc_densify_config* cfg, const gsparc_gaussians* src,
+                             const float* noise, const gsparc_gaussians* dst, int dst_capacity, void* work,
+                             int* P_new, int* n_clone, int* n_split, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
