@@ -294,6 +294,7 @@ struct SsimArgs {
     float* ds1;
     float* ds12;
     float* part;             // nullptr, or per workgroup (sum of the map, sum of |img1 - img2|)
+    int nsum = 2;            // sums per workgroup in part: 2, or 5 (also sum (img1 - img2)^2, img1^2, img2^2)
 };
 hipError_t launch_ssim_fwd(const SsimArgs& a, hipStream_t s);
 int ssim_blocks(int planes, int H, int W);
@@ -335,6 +336,21 @@ struct ConfLossArgs {
     float* scratch;          // [planes * 4]
 };
 hipError_t launch_conf_loss(const ConfLossArgs& a, hipStream_t s);
+
+// Evaluation metrics per plane from k_ssim_fwd's five-sum partials (nbp
+// workgroups per plane): metrics [planes][EVAL_METRICS] = loss, PSNR, SSIM,
+// MAE, MSE, RSSI of img, RSSI(img) - RSSI(target), confidence.
+constexpr int EVAL_METRICS = 8;
+struct EvalArgs {
+    const float* part;
+    int planes, nbp, hidden;
+    double inv_np;
+    float lambda_ssim;
+    const float* mlp;        // packed confidence network or nullptr (confidence 1)
+    const float* rx;         // [planes, 3]
+    float* metrics;
+};
+hipError_t launch_eval_metrics(const EvalArgs& a, hipStream_t s);
 
 // Shared emission MLP on MFMA (k_mlp_mfma.hip); rows = R*P, row = r*P + g.
 struct Mlp1Args {

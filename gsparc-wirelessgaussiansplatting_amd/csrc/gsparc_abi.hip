@@ -1199,6 +1199,39 @@ int gsparc_loss_l1_ssim_conf(int planes, int H, int W, float lambda_ssim, float 
     return GSPARC_OK;
 }
 
+size_t gsparc_eval_work_size(int planes, int H, int W)
+{
+    if (planes <= 0 || H <= 0 || W <= 0) return 0;
+    return 256 + al(5 * sizeof(float) * (size_t)gs::ssim_blocks(planes, H, W));
+}
+
+int gsparc_eval_metrics(int planes, int H, int W, float lambda_ssim, float C1, float C2, int hidden,
+                        const float* conf_mlp, const float* rx_pos, const float* img, const float* target, void* work,
+                        float* metrics, void* stream)
+{
+    if (int e = loss_args(planes, H, W, img, target, metrics)) return e;
+    if (!work) return fail(GSPARC_ERR_ARG, "null pointer");
+    if (conf_mlp && (!rx_pos || hidden < 1 || hidden > 255))
+        return fail(GSPARC_ERR_ARG, "the confidence network needs rx_pos and 1..255 hidden units");
+    hipStream_t s = (hipStream_t)stream;
+    float* part = reinterpret_cast<float*>(base_of(work));
+    gs::SsimArgs a{planes, H, W, C1, C2, img, target, nullptr, nullptr, nullptr, nullptr, part};
+    a.nsum = 5;
+    GS_LAUNCH("ssim_fwd", gs::launch_ssim_fwd(a, s), s);
+    gs::EvalArgs e;
+    e.part = part;
+    e.planes = planes;
+    e.nbp = gs::ssim_blocks(1, H, W);
+    e.hidden = hidden;
+    e.inv_np = 1.0 / ((double)H * W);
+    e.lambda_ssim = lambda_ssim;
+    e.mlp = conf_mlp;
+    e.rx = rx_pos;
+    e.metrics = metrics;
+    GS_LAUNCH("eval_metrics", gs::launch_eval_metrics(e, s), s);
+    return GSPARC_OK;
+}
+
 size_t gsparc_csi_loss_work_size(int R, int H, int W)
 {
     if (R <= 0 || H <= 0 || W <= 0) return 0;

@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(SS_THREADS) k_ssim_fwd(SsimArgs a, Taps tp)
         h[0][ly][lx] = m1; h[1][ly][lx] = m2; h[2][ly][lx] = e11; h[3][ly][lx] = e22; h[4][ly][lx] = e12;
     }
     __syncthreads();
-    float sum_map = 0.0f, sum_abs = 0.0f;
+    float sum_map = 0.0f, sum_abs = 0.0f, sum_d2 = 0.0f, sum_p2 = 0.0f, sum_t2 = 0.0f;
     for (int i = tid; i < SS_TH * SS_TW; i += SS_THREADS) {
         const int ly = i / SS_TW, lx = i % SS_TW;
         const int gy = y0 + ly, gx = x0 + lx;
@@ -113,25 +113,32 @@ __global__ void __launch_bounds__(SS_THREADS) k_ssim_fwd(SsimArgs a, Taps tp)
             a.ds12[o] = 2.0f * C * iAB;
         }
         sum_map += m;
-        sum_abs += fabsf(s1[ly + 5][lx + 5] - s2[ly + 5][lx + 5]);   // the staged pixel values
+        const float p = s1[ly + 5][lx + 5], t = s2[ly + 5][lx + 5];   // the staged pixel values
+        sum_abs += fabsf(p - t);
+        if (a.nsum == 5) {
+            sum_d2 += (p - t) * (p - t);
+            sum_p2 += p * p;
+            sum_t2 += t * t;
+        }
     }
     if (a.part) {
         // fixed-order tree over the workgroup: the loss is the same run to run
-        __shared__ float r0[SS_THREADS], r1[SS_THREADS];
-        r0[tid] = sum_map;
-        r1[tid] = sum_abs;
+        __shared__ float r[5][SS_THREADS];
+        const int ns = a.nsum;
+        r[0][tid] = sum_map;
+        r[1][tid] = sum_abs;
+        r[2][tid] = sum_d2;
+        r[3][tid] = sum_p2;
+        r[4][tid] = sum_t2;
         __syncthreads();
         for (int w = SS_THREADS / 2; w > 0; w >>= 1) {
-            if (tid < w) {
-                r0[tid] += r0[tid + w];
-                r1[tid] += r1[tid + w];
-            }
+            if (tid < w)
+                for (int c = 0; c < ns; ++c) r[c][tid] += r[c][tid + w];
             __syncthreads();
         }
         if (tid == 0) {
             const size_t b = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-            a.part[2 * b] = r0[0];
-            a.part[2 * b + 1] = r1[0];
+            for (int c = 0; c < ns; ++c) a.part[(size_t)ns * b + c] = r[c][0];
         }
     }
 }
@@ -254,6 +261,62 @@ __global__ void __launch_bounds__(256) k_loss_reduce(const float* part, int nb, 
 // hidden unit (thread j < hidden, over the planes in order): the gradient of
 // the mean loss with respect to the network, through dloss/dC_r =
 // (L_r - alpha / C_r) / R and dC/draw = C - 1.
+// C = 1 + exp(w2 . relu(W1 x + b1) + b2) of the packed confidence network
+// [W1 (hidden x 3) | b1 | w2 | b2] at receiver position x (gaussian.py:598-609)
+__device__ float conf_value(const float* mlp, int Hn, const float* x3)
+{
+    const float* W1 = mlp;
+    const float* b1 = mlp + 3 * Hn;
+    const float* w2 = mlp + 4 * Hn;
+    const float x = x3[0], y = x3[1], z = x3[2];
+    float raw = 0.0f;
+    for (int j = 0; j < Hn; ++j) {
+        float pre = fmaf(W1[3 * j], x, b1[j]);
+        pre = fmaf(W1[3 * j + 1], y, pre);
+        pre = fmaf(W1[3 * j + 2], z, pre);
+        raw = fmaf(w2[j], fmaxf(pre, 0.0f), raw);
+    }
+    return 1.0f + expf(raw + mlp[5 * Hn]);
+}
+
+// The evaluation metrics of every plane (train.py:76-102, 651-693) from the
+// five per-workgroup sums of k_ssim_fwd: loss, PSNR, SSIM, MAE, MSE,
+// RSSI of the rendering, RSSI difference, confidence.  One thread per plane;
+// fixed-order sums in double.
+__global__ void __launch_bounds__(256) k_eval_metrics(EvalArgs a)
+{
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= a.planes) return;
+    double sm = 0.0, sa = 0.0, sd2 = 0.0, sp2 = 0.0, st2 = 0.0;
+    const float* pp = a.part + 5 * (size_t)r * a.nbp;
+    for (int b = 0; b < a.nbp; ++b) {
+        sm += (double)pp[5 * b];
+        sa += (double)pp[5 * b + 1];
+        sd2 += (double)pp[5 * b + 2];
+        sp2 += (double)pp[5 * b + 3];
+        st2 += (double)pp[5 * b + 4];
+    }
+    const double ssim = sm * a.inv_np, mae = sa * a.inv_np, mse = sd2 * a.inv_np;
+    const double rssi_p = 10.0 * log10(sp2 * a.inv_np) - 30.0;
+    const double rssi_t = 10.0 * log10(st2 * a.inv_np) - 30.0;
+    float* out = a.metrics + (size_t)EVAL_METRICS * r;
+    out[0] = (float)((1.0 - (double)a.lambda_ssim) * mae + (double)a.lambda_ssim * (1.0 - ssim));
+    out[1] = (float)(20.0 * log10(1.0 / sqrt(mse)));
+    out[2] = (float)ssim;
+    out[3] = (float)mae;
+    out[4] = (float)mse;
+    out[5] = (float)rssi_p;
+    out[6] = (float)(rssi_p - rssi_t);
+    out[7] = a.mlp ? conf_value(a.mlp, a.hidden, a.rx + 3 * (size_t)r) : 1.0f;
+}
+
+hipError_t launch_eval_metrics(const EvalArgs& a, hipStream_t s)
+{
+    if (a.planes == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_eval_metrics, dim3((unsigned)((a.planes + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
 __global__ void __launch_bounds__(256) k_conf_loss(ConfLossArgs a)
 {
     __shared__ double red[256];

@@ -248,13 +248,15 @@ class GaussianModel:
                  "gsparc_gaussians_activate")
 
     def training_setup(self, optim_params: OptimizationParams, spatial_lr_scale=None, confidence=None,
-                       scene_min=None, scene_max=None, first_iteration: int = 1):
+                       scene_min=None, scene_max=None, first_iteration: int | None = 1):
         """Adam settings and groups of training_setup (gaussian.py:171-212).
         confidence: the packed confidence network (gsparc_train.ConfidenceMLP.
         packed()) or None.  scene_min / scene_max: the clamp bounds applied
         after every step (train.py:502-505, 561-562: xyz_min / xyz_max with a
         margin of 4), None for no clamp.  first_iteration: the iteration the
-        first step completes (the reference's loop starts at 1)."""
+        first step completes (the reference's loop starts at 1); None keeps
+        the step counts and moments a restored checkpoint brought, and a
+        restored confidence network when `confidence` is None."""
         o = optim_params
         if spatial_lr_scale is not None:
             self.spatial_lr_scale = float(spatial_lr_scale)
@@ -277,8 +279,9 @@ class GaussianModel:
                 a.scene_min[k] = float(np.float32(scene_min[k]))
                 a.scene_max[k] = float(np.float32(scene_max[k]))
         self.adam = a
-        self.train_state = torch.zeros(gh.TRAIN_STATE_BYTES // 4, dtype=torch.int32, device=self.device)
-        self.train_state[0] = int(first_iteration)
+        if first_iteration is not None or self.train_state is None:
+            self.train_state = torch.zeros(gh.TRAIN_STATE_BYTES // 4, dtype=torch.int32, device=self.device)
+            self.train_state[0] = int(first_iteration if first_iteration is not None else 1)
         if confidence is not None:
             self.conf = confidence
             self.conf_hidden = (confidence.numel() - 1) // 5
@@ -411,13 +414,28 @@ class GaussianModel:
         return out
 
     def restore(self, st: dict):
+        """Load a capture() snapshot (or a checkpoint's, which has no
+        activations: they are recomputed)."""
         P = int(st["P"])
         self._ensure(self.cur, P)
         self.P = P
+        for t in self.bank.t.values():
+            t.zero_()
         for n, w in _FIELDS:
-            self.bank.t[n][: w * P].copy_(st[n].reshape(-1))
-        if st.get("train_state") is not None and self.train_state is not None:
-            self.train_state.copy_(st["train_state"])
-        if st.get("confidence") is not None and self.conf is not None:
-            self.conf.copy_(st["confidence"])
-            self.conf_state.copy_(st["confidence_state"])
+            if st.get(n) is not None:
+                self.bank.t[n][: w * P].copy_(st[n].reshape(-1))
+        if any(st.get(n) is None for n in ("scale", "rotation_n", "opacity")):
+            self.activate()
+        if st.get("train_state") is not None:
+            if self.train_state is None:
+                self.train_state = torch.zeros(gh.TRAIN_STATE_BYTES // 4, dtype=torch.int32, device=self.device)
+            self.train_state.copy_(st["train_state"].to(torch.int32).reshape(-1))
+        if st.get("confidence") is not None:
+            n = int(st["confidence"].numel())
+            if self.conf is None or self.conf.numel() != n:
+                self.conf = torch.zeros(n, dtype=torch.float32, device=self.device)
+                self.conf_grad = torch.zeros_like(self.conf)
+                self.conf_state = torch.zeros(2 * n, dtype=torch.float32, device=self.device)
+                self.conf_hidden = (n - 1) // 5
+            self.conf.copy_(st["confidence"].reshape(-1))
+            self.conf_state.copy_(st["confidence_state"].reshape(-1))

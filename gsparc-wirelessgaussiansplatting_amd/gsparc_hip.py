@@ -138,6 +138,8 @@ _SIGS = {
     "gsparc_densify_work_size": (_sz, [_i]),
     "gsparc_densify_and_prune": (_i, [_i, C.POINTER(DensifyConfig), _GP, _vp, _GP, _i, _vp, C.POINTER(_i),
                                       C.POINTER(_i), C.POINTER(_i), _vp]),
+    "gsparc_eval_work_size": (_sz, [_i, _i, _i]),
+    "gsparc_eval_metrics": (_i, [_i, _i, _i, _f, _f, _f, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gsparc_graph_begin": (_i, [_vp]),
     "gsparc_graph_end": (_i, [_vp, C.POINTER(_vp)]),
     "gsparc_graph_launch": (_i, [_vp, _vp]),

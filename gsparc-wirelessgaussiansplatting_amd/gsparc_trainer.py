@@ -84,7 +84,10 @@ class Trainer:
     tensors read in place at every iteration (change their contents between
     iterations to feed other receivers).  confidence: a packed confidence
     network (gsparc_train.ConfidenceMLP.packed()) for the confidence-weighted
-    loss of train.py:418-436, or None for the plain loss.  k_capacity: the
+    loss of train.py:418-436, or None for the plain loss (or, with
+    first_iteration=None, the network a restored checkpoint brought).
+    first_iteration: the iteration the first step runs; None continues from
+    the model's training state (a checkpoint: its iteration + 1).  k_capacity: the
     first instance capacity; k_headroom: the capacity as a multiple of the K
     measured whenever an iteration runs eagerly (it grows when K outgrows
     it)."""
@@ -110,7 +113,7 @@ class Trainer:
         self.lambda_ssim = float(optim.lambda1)
         model.training_setup(optim, confidence=confidence, scene_min=self.scene_min, scene_max=self.scene_max,
                              first_iteration=first_iteration)
-        self.iteration = int(first_iteration)
+        self.iteration = int(model.train_state[0].item())
         self.buf = None
         self.graph = C.c_void_p(None)
         self.graph_key = None

@@ -291,6 +291,20 @@ int gsparc_loss_l1_ssim_conf(int planes, int H, int W, float lambda_ssim, float 
                              const float* target, void* work, float* loss, float* rec_loss, float* confidence,
                              float* dL_dimg, float* dconf_mlp, void* stream);
 
+/* Evaluation metrics of `planes` rendered spectra img against target [planes,
+ * H, W], one SSIM pass: metrics [planes][8] = loss (1 - lambda_ssim) * MAE +
+ * lambda_ssim * (1 - SSIM), PSNR 20 log10(1 / sqrt(MSE)), SSIM (mean of the
+ * 11 x 11 Gaussian-window map with zero padding, utils.py:81-113), MAE, MSE,
+ * RSSI of img 10 log10(mean img^2) - 30, RSSI(img) - RSSI(target), and the
+ * confidence 1 + exp(...) of the packed network conf_mlp at rx_pos (1 when
+ * conf_mlp is NULL).  Sums in double, in a fixed order.  work:
+ * gsparc_eval_work_size bytes.  Replaces the per-sample metric calls of
+ * Trainer.evaluate (train.py:76-102, 651-693). */
+size_t gsparc_eval_work_size(int planes, int H, int W);
+int gsparc_eval_metrics(int planes, int H, int W, float lambda_ssim, float C1, float C2, int hidden,
+                        const float* conf_mlp, const float* rx_pos, const float* img, const float* target, void* work,
+                        float* metrics, void* stream);
+
 /* gsparc_loss_l1_ssim with no workspace argument, calling no allocator: its
  * scratch (the three SSIM partial-derivative maps and the per-workgroup loss
  * partials) is a library-owned device array of GSPARC_LOSS_SCRATCH_BYTES on
