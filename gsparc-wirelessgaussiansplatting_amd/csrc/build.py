@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 OUT = os.path.join(PKG, "libgsparc.so")
 SOURCES = ["k_preprocess_fwd.hip", "k_binning.hip", "k_render.hip", "k_render_csi.hip", "k_preprocess_bwd.hip",
-           "k_ssim.hip", "k_loss_csi.hip", "k_mlp_mfma.hip", "k_train.hip", "gsparc_abi.hip"]
+           "k_ssim.hip", "k_loss_csi.hip", "k_mlp_mfma.hip", "k_train.hip", "k_data.hip", "gsparc_abi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = [

@@ -434,6 +434,13 @@ hipError_t launch_densify_lists(int P, const DensifyCfg& c, const Gaussians& p, 
 hipError_t launch_densify_gather(const DensifyPlan& pl, const DensifyCfg& c, const Gaussians& p, const Gaussians& q,
                                  const float* noise, const DensifyWork& w, hipStream_t s);
 
+// ---- the spectrum dataset on the device (k_data.hip) ----
+// q: gateway quaternion (x, y, z, w); out [N,16] column-major view matrices.
+hipError_t launch_view_matrices(int N, const double* q, const float* rx, float* out, hipStream_t s);
+hipError_t launch_gather_batch(int R, int N, int npix, const int* order, int L, const uint32_t* train_state,
+                               const float* spectra, const float* rx, const float* views, float* out_spec,
+                               float* out_rx, float* out_views, hipStream_t s);
+
 // loss = l1w * sum|img1 - img2| * inv_n + sw * (1 - sum(map) * inv_n) from
 // the per-workgroup partials of k_ssim_fwd, in a fixed order.
 hipError_t launch_loss_reduce(const float* part, int nb, double inv_n, float l1w, float sw, float* loss,

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -1516,6 +1517,35 @@ int gsparc_densify_and_prune(int P, const gsparc_densify_config* cfg, const gspa
         return fail(GSPARC_ERR_CAPACITY, "densify: the new set needs " + std::to_string(pl.P_new) +
                                              " Gaussians, dst holds " + std::to_string(dst_capacity));
     GS_LAUNCH("densify_gather", gs::launch_densify_gather(pl, c, p, q, noise, w, s), s);
+    return GSPARC_OK;
+}
+
+// ---- the spectrum dataset on the device
+int gsparc_view_matrices(int N, const double* gateway_quat, const float* rx_pos, float* viewmatrices, void* stream)
+{
+    if (N < 0) return fail(GSPARC_ERR_ARG, "negative count");
+    if (!gateway_quat || (N > 0 && (!rx_pos || !viewmatrices))) return fail(GSPARC_ERR_ARG, "null pointer");
+    const double* q = gateway_quat;
+    const double n2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+    if (!(n2 > 0.0) || !std::isfinite(n2)) return fail(GSPARC_ERR_ARG, "the gateway quaternion must be finite and non-zero");
+    GS_LAUNCH("view_matrices", gs::launch_view_matrices(N, q, rx_pos, viewmatrices, (hipStream_t)stream),
+              (hipStream_t)stream);
+    return GSPARC_OK;
+}
+
+int gsparc_gather_batch(int R, int N, int npix, const int* order, int order_len, const void* train_state,
+                        const float* spectra, const float* rx_pos, const float* viewmatrices, float* batch_spectra,
+                        float* batch_rx, float* batch_views, void* stream)
+{
+    if (R < 0 || N <= 0 || npix <= 0 || order_len <= 0) return fail(GSPARC_ERR_ARG, "bad size");
+    if (R > 65535) return fail(GSPARC_ERR_ARG, "at most 65535 samples per batch");
+    if (!order || !train_state || !spectra || !rx_pos || !viewmatrices || !batch_spectra || !batch_rx || !batch_views)
+        return fail(GSPARC_ERR_ARG, "null pointer");
+    if (((uintptr_t)spectra & 15) || ((uintptr_t)batch_spectra & 15))
+        return fail(GSPARC_ERR_ARG, "spectra must be 16-byte aligned");
+    GS_LAUNCH("gather_batch", gs::launch_gather_batch(R, N, npix, order, order_len, (const uint32_t*)train_state,
+                                                      spectra, rx_pos, viewmatrices, batch_spectra, batch_rx,
+                                                      batch_views, (hipStream_t)stream), (hipStream_t)stream);
     return GSPARC_OK;
 }
 

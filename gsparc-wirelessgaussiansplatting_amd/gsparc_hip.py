@@ -140,6 +140,8 @@ _SIGS = {
                                       C.POINTER(_i), C.POINTER(_i), _vp]),
     "gsparc_eval_work_size": (_sz, [_i, _i, _i]),
     "gsparc_eval_metrics": (_i, [_i, _i, _i, _f, _f, _f, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "gsparc_view_matrices": (_i, [_i, C.POINTER(C.c_double), _vp, _vp, _vp]),
+    "gsparc_gather_batch": (_i, [_i, _i, _i, _vp, _i] + [_vp] * 8),
     "gsparc_graph_begin": (_i, [_vp]),
     "gsparc_graph_end": (_i, [_vp, C.POINTER(_vp)]),
     "gsparc_graph_launch": (_i, [_vp, _vp]),

@@ -119,9 +119,35 @@ class Trainer:
         self.graph_key = None
         self.stream = torch.cuda.Stream(device=self.dev)
         self.events = []            # (iteration, event, P before, P after)
+        self.data = None            # (SpectrumDataset, sample order on the device) when fed by a dataset
         self.k_capacity0 = int(k_capacity)
 
+    @classmethod
+    def from_dataset(cls, model: GaussianModel, optim: OptimizationParams, dataset, R: int, xyz_min, xyz_max,
+                     iterations: int | None = None, seed: int | None = None, **kw):
+        """A trainer fed by a resident SpectrumDataset: iteration i trains on
+        draws (i-1)R .. iR-1 of the reference's sample order
+        (gsparc_dataset.sample_order, numpy's global RNG or RandomState(seed)),
+        gathered on the device as part of the step."""
+        import gsparc_dataset as D
+        dev = model.device
+        H, W = dataset.n_elevation, dataset.n_azimuth
+        n = int(iterations if iterations is not None else optim.iterations) * int(R)
+        order = D.sample_order(len(dataset), n, None if seed is None else np.random.RandomState(seed))
+        views = torch.zeros((R, 16), dtype=torch.float32, device=dev)
+        rx = torch.zeros((R, 3), dtype=torch.float32, device=dev)
+        targets = torch.zeros((R, H, W), dtype=torch.float32, device=dev)
+        tr = cls(model, optim, views, dataset.tx_pos, rx, targets, H, W, xyz_min, xyz_max, **kw)
+        tr.data = (dataset, torch.from_numpy(order).to(dev))
+        return tr
+
     # ---- the launch sequence
+    def _gather(self, s):
+        if self.data is not None:
+            import gsparc_dataset as D
+            ds, order = self.data
+            D.gather_batch(self.R, ds, order, self.m.train_state, self.targets, self.rx, self.views, stream=s)
+
     def _ensure_buffers(self):
         P = self.m.P
         b = self.buf
@@ -172,6 +198,7 @@ class Trainer:
         self.m.optimizer_step(b.dcentre, b.dscale, b.drot, b.dopac, b.dmlp, groups=groups, stream=s)
 
     def _record_plain(self, s, stats):
+        self._gather(s)
         self._forward(s)
         self._loss_backward(s)
         if stats:
@@ -227,6 +254,7 @@ class Trainer:
             h = gh.stream_handle(self.dev)
             if dens or post:
                 self._drop_graph()          # the arrays the graph reads are about to change
+            self._gather(h)
             self._check_capacity(h)
             self._loss_backward(h)
             if stats:
@@ -259,6 +287,7 @@ class Trainer:
         with torch.cuda.stream(s):
             h = gh.stream_handle(self.dev)
             # this iteration, eagerly, with the instance capacity checked
+            self._gather(h)
             self._check_capacity(h)
             self._loss_backward(h)
             if stats:

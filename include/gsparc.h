@@ -507,6 +507,27 @@ int gsparc_densify_stats(int P, int R, float grad_scale, const float* dmeans2D, 
  * count stays, replace_tensor_to_optimizer, gaussian.py:233-253). */
 int gsparc_opacity_reset(int P, const gsparc_gaussians* g, void* stream);
 
+/* ---- the spectrum dataset on the device ----------------------------------
+ * View matrices of N receivers at rx_pos [N,3] under the gateway orientation
+ * gateway_quat (x, y, z, w, as in gateway_info.yml; normalised here):
+ * viewmatrices [N,16] = [R^T, -R^T C; 0 1] stored transposed (column-major),
+ * R the quaternion's rotation computed in double and rounded to float.
+ * Replaces Spectrum_dataset.build_view_matrix (preprocess_data.py:177-202),
+ * one call for the whole dataset. */
+int gsparc_view_matrices(int N, const double* gateway_quat, const float* rx_pos, float* viewmatrices, void* stream);
+
+/* The batch of the iteration stored in train_state (word 0, 1-based, as
+ * gsparc_adam_step advances it): sample r = order[((iteration - 1) R + r)
+ * mod order_len] of the resident dataset (spectra [N, npix], rx_pos [N,3],
+ * viewmatrices [N,16]) is copied to row r of batch_spectra [R, npix],
+ * batch_rx [R,3], batch_views [R,16].  Reads the iteration on the device, so
+ * a captured training step replays with a new batch each time.  Indices
+ * outside [0, N) are clamped.  Replaces the per-iteration
+ * wireless_data_train.pop() of train.py:366-370 for R samples at a time. */
+int gsparc_gather_batch(int R, int N, int npix, const int* order, int order_len, const void* train_state,
+                        const float* spectra, const float* rx_pos, const float* viewmatrices, float* batch_spectra,
+                        float* batch_rx, float* batch_views, void* stream);
+
 /* densify_and_prune settings: densify != 0 clones (mean gradient accum/denom
  * >= grad_threshold and max scale <= clone_max_scale, the percent_dense *
  * extent of gaussian.py:495) and splits (same gradient test, max scale
