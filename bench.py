@@ -257,8 +257,25 @@ def setup_receivers(args, dev):
         return cpu_oracle_sample(args, [(sc, r) for r in range(nr)], list(tg),
                                  f"{nr} of the {R} receivers of the same {P // 1000}k-Gaussian scene")
 
+    def trainer(use_graphs=True):
+        """The whole training iteration (train.py:364-566 without schedule
+        events): step, densification statistics, Adam over every group with
+        the activations, centre clamp; confidence-weighted loss."""
+        import gsparc_gaussian as G
+        import gsparc_train as T
+        import gsparc_trainer as TR
+        m = G.GaussianModel()
+        m.load_raw(means, torch.log(scales), rots, torch.logit(opac.double()).float(), mlp)
+        torch.manual_seed(0)
+        conf = T.ConfidenceMLP(32, device=dev).packed().detach().clone().contiguous()
+        o = G.OptimizationParams(densify_from_iter=10 ** 9)
+        return TR.Trainer(m, o, views, txp, rxp, target, sc.H, sc.W, xyz_min=[0.0, 0.0, 0.0],
+                          xyz_max=list(S.ROOM), confidence=conf, grad_mode=args.grad_mode,
+                          k_capacity=int(count() * 1.25) + 1024, use_graphs=use_graphs)
+
     return Workload(
         step=step, step_plain=step_plain, single=single, count=count, cpu=cpu, graphed=graphed, tune=tune,
+        trainer=trainer,
         spectra=R, Ptot=P,
         Nrec=P * R,
         H=sc.H, W=sc.W, traffic_key=(P, R),
@@ -738,6 +755,38 @@ def main():
     if roof["traffic"] is not None:
         roof["traffic_source"] = TRAFFIC_FILE
 
+    train_iter = None
+    if getattr(wl, "trainer", None) is not None and not args.eager:
+        tr = wl.trainer()
+        for _ in range(args.warmup):
+            tr.step()
+        torch.cuda.synchronize()
+        log(args, f"timing {args.steps} graph-captured training iterations")
+        el_tr = timed(tr.step, args.steps)
+        if tr.overflowed() or tr.nan_centre():
+            raise RuntimeError("training iterations: tile-list overflow or NaN centres")
+        tr.close()
+        # the surround's own kernels, timed on an eager trainer
+        tr2 = wl.trainer(use_graphs=False)
+        for _ in range(3):
+            tr2.step()
+        torch.cuda.synchronize()
+        gh.profile_enable(True, timing=True, counting=False)
+        for _ in range(nprof):
+            tr2.step()
+        torch.cuda.synchronize()
+        kt2 = gh.profile_collect()
+        gh.profile_enable(False)
+        tr2.close()
+        train_iter = {"ms_per_iteration": round(1e3 * el_tr / args.steps, 3),
+                      "spectra_per_s": round(R * args.steps / el_tr, 2),
+                      "step": "hip graph: forward, confidence-weighted loss, backward, densification statistics, "
+                              "Adam with the activations, centre clamp",
+                      "surround_ms_per_iteration": {k: round(kt2[k][0] / max(kt2[k][1], 1) *
+                                                             (kt2[k][1] / nprof), 4)
+                                                    for k in ("adam", "densify_stats", "conf_loss") if k in kt2}}
+        log(args, f"training iteration: {train_iter}")
+
     cpu = wl.cpu() if args.cpu_receivers > 0 else None
 
     metric = {"receivers": METRIC,
@@ -772,6 +821,7 @@ def main():
         "kernels_source": ("HIP events around each kernel of the graph's launch sequence, run without the graph"
                            if graph_value is not None else "HIP events around each kernel of the eager step"),
         "cpu_baseline": cpu,
+        "training_iteration": train_iter,
     }
     if getattr(wl, "mlp_rows", None):
         out["mlp_mfma"] = wl.mlp_rows()
