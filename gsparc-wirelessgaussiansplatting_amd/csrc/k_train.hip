@@ -170,6 +170,25 @@ __global__ void __launch_bounds__(256) k_adam_flat(size_t n, int grp, float* x, 
         x[i] = adam_update(x[i], g[i], m[i], v[i], s, grp);
 }
 
+// The same over float4s (16-byte aligned arrays): a quarter of the memory
+// instructions, more bytes in flight per wave.
+__global__ void __launch_bounds__(256) k_adam_flat4(size_t n4, int grp, float4* x, const float4* g, float4* m,
+                                                    float4* v, const AdamStep* stp)
+{
+    const AdamStep s = *stp;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += (size_t)gridDim.x * 256) {
+        float4 xi = x[i], mi = m[i], vi = v[i];
+        const float4 gi = g[i];
+        xi.x = adam_update(xi.x, gi.x, mi.x, vi.x, s, grp);
+        xi.y = adam_update(xi.y, gi.y, mi.y, vi.y, s, grp);
+        xi.z = adam_update(xi.z, gi.z, mi.z, vi.z, s, grp);
+        xi.w = adam_update(xi.w, gi.w, mi.w, vi.w, s, grp);
+        x[i] = xi;
+        m[i] = mi;
+        v[i] = vi;
+    }
+}
+
 // ---------------------------------------------------- densification stats
 // add_densification_stats (gaussian.py:538-551) and the max-radius update
 // (train.py:510-516) for the R receivers of a batch: for each receiver that
@@ -571,9 +590,17 @@ hipError_t launch_adam_step(int P, int groups, const AdamCfg& c, const Gaussians
         hipLaunchKernelGGL(k_adam_geom, dim3(blocks_for(P)), dim3(256), 0, s, P, groups, c, p, d,
                            (const AdamStep*)step, counters);
     const size_t n = (size_t)P * 194;
-    if (n > 0 && (groups & (1 << AG_EMISSION)))
-        hipLaunchKernelGGL(k_adam_flat, dim3((unsigned)std::min<size_t>(blocks_for(n), 4096)), dim3(256), 0, s, n,
-                           (int)AG_EMISSION, p.mlp, d.mlp, p.mlp_state, (const AdamStep*)step);
+    if (n > 0 && (groups & (1 << AG_EMISSION))) {
+        const bool vec = n % 4 == 0 && ((uintptr_t)p.mlp & 15) == 0 && ((uintptr_t)d.mlp & 15) == 0 &&
+                         ((uintptr_t)p.mlp_state & 15) == 0;
+        if (vec)
+            hipLaunchKernelGGL(k_adam_flat4, dim3((unsigned)std::min<size_t>(blocks_for(n / 4), 8192)), dim3(256), 0,
+                               s, n / 4, (int)AG_EMISSION, (float4*)p.mlp, (const float4*)d.mlp,
+                               (float4*)p.mlp_state, (float4*)(p.mlp_state + n), (const AdamStep*)step);
+        else
+            hipLaunchKernelGGL(k_adam_flat, dim3((unsigned)std::min<size_t>(blocks_for(n), 8192)), dim3(256), 0, s,
+                               n, (int)AG_EMISSION, p.mlp, d.mlp, p.mlp_state, (const AdamStep*)step);
+    }
     if (n_conf > 0 && (groups & (1 << AG_CONFIDENCE)))
         hipLaunchKernelGGL(k_adam_flat, dim3(blocks_for(n_conf)), dim3(256), 0, s, n_conf, (int)AG_CONFIDENCE, conf,
                            dconf, conf_state, (const AdamStep*)step);

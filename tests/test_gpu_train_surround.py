@@ -59,8 +59,10 @@ def _gpu_model(raw, headroom=1.5):
     return m
 
 
-def test_adam_steps_match_torch_adam():
-    P = 3000
+@pytest.mark.parametrize("P", [3000, 1001])
+def test_adam_steps_match_torch_adam(P):
+    """P = 3000: the MLP moments go through the float4 kernel; P = 1001
+    (194 P not a multiple of 4): through the scalar one."""
     rng = np.random.default_rng(11)
     raw = _raw(P, 3)
     conf0 = (rng.normal(size=161) * 0.3).astype(np.float32)
