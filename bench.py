@@ -487,7 +487,7 @@ def setup_config1(args, dev):
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
     means, scales, rots, opac = t(sc.means), t(sc.scales), t(sc.rotations), t(sc.opacities)
     views, txp, rxp = t(sc.views), t(sc.tx_pos), t(sc.rx_pos)
-    mlp = GE.SharedEmissionMLP(generator=torch.Generator().manual_seed(1), backend="mfma").to(dev)
+    mlp = GE.SharedEmissionMLP(generator=torch.Generator().manual_seed(1)).to(dev)
     with torch.no_grad():
         Et = mlp(t(tgt.means), rxp, txp)
         target = wr.forward_batched(t(tgt.means), None, t(tgt.opacities), t(tgt.scales), t(tgt.rotations), sc.H,

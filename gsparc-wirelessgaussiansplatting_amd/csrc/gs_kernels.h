@@ -352,23 +352,28 @@ struct EvalArgs {
 };
 hipError_t launch_eval_metrics(const EvalArgs& a, hipStream_t s);
 
-// Shared emission MLP on MFMA (k_mlp_mfma.hip); rows = R*P, row = r*P + g.
+// Shared emission MLP on MFMA (k_mlp_mfma.hip): 68 -> 128 -> 128 -> 1 for every
+// (receiver, Gaussian) pair; emission rows r*P + g.
 struct Mlp1Args {
     int P, R;
-    const float* pe1;        // [P,128]  PE(mu) W1pe^T + b1
-    const float* rxf;        // [R*P,5]  receiver features per row
-    const float* w1rx;       // [128,5]
+    const float* means;      // [P,3]
+    const float* rx;         // [R,3]
+    const float* tx;         // [3]
+    const float* w1;         // [128,68] (out, in): PE inputs 0..62, receiver features 63..67
+    const float* b1;         // [128]
     const float* w2;         // [128,128] (out, in)
     const float* b2;         // [128]
     const float* w3;         // [128]
     const float* b3;         // [1]
+    float* pe1;              // [P,128] PE(mu) W1pe^T + b1 (work; written by the forward)
     float* out;              // [R*P,2] emission (a, 0)
 };
-int mlp1_bwd_groups(long long n_rows);
-size_t mlp1_part_floats(long long n_rows);
+size_t mlp1_work_floats(int P, int R);
+size_t mlp1_grad_floats();
 hipError_t launch_mlp1_fwd(const Mlp1Args& a, hipStream_t s);
-// grads: dW2 [128*128] | dw3 [128] | db2 [128] | db3 [1]; dh1 [R*P,128]
-hipError_t launch_mlp1_bwd(const Mlp1Args& a, const float* da, float* dh1, float* part, float* grads, hipStream_t s);
+// grads: dW1 [128*68] | db1 [128] | dW2 [128*128] | db2 [128] | dw3 [128] | db3 [1];
+// dmu [P,3] written (the MLP's gradient of the means); work as the forward left it
+hipError_t launch_mlp1_bwd(const Mlp1Args& a, const float* dem, float* dmu, float* work, float* grads, hipStream_t s);
 
 // CSI loss pieces (k_loss_csi.hip): magnitude planes + L1 partials, the
 // gradient back to the interleaved map, and the two-part loss reduction.

@@ -1284,46 +1284,50 @@ int gsparc_csi_loss(int R, int H, int W, float lambda_ssim, float C1, float C2, 
 }
 
 namespace {
-int mlp1_args(int P, int R, const float* pe1, const float* rxf, const float* w1rx, const float* w2, const float* b2,
-              const float* w3, const float* b3, gs::Mlp1Args& a)
+int mlp1_args(int P, int R, const float* means, const float* rx_pos, const float* tx_pos, const float* W1,
+              const float* b1, const float* W2, const float* b2, const float* w3, const float* b3, void* work,
+              gs::Mlp1Args& a)
 {
     if (P < 0 || R < 0) return fail(GSPARC_ERR_ARG, "negative size");
     if ((long long)P * R >= (1LL << 31)) return fail(GSPARC_ERR_ARG, "too many (receiver, Gaussian) rows");
-    if ((long long)P * R > 0 && (!pe1 || !rxf || !w1rx || !w2 || !b2 || !w3 || !b3))
+    if (P > 0 && (!means || !tx_pos || !W1 || !b1 || !W2 || !b2 || !w3 || !b3 || !work || (R > 0 && !rx_pos)))
         return fail(GSPARC_ERR_ARG, "null pointer");
-    a.P = P; a.R = R; a.pe1 = pe1; a.rxf = rxf; a.w1rx = w1rx; a.w2 = w2; a.b2 = b2; a.w3 = w3; a.b3 = b3;
+    a.P = P; a.R = R; a.means = means; a.rx = rx_pos; a.tx = tx_pos; a.w1 = W1; a.b1 = b1; a.w2 = W2; a.b2 = b2;
+    a.w3 = w3; a.b3 = b3;
+    a.pe1 = reinterpret_cast<float*>(base_of(work));
     a.out = nullptr;
     return GSPARC_OK;
 }
 }  // namespace
 
-int gsparc_mlp1_forward(int P, int R, const float* pe1, const float* rxf, const float* w1rx, const float* w2,
-                        const float* b2, const float* w3, const float* b3, float* emission, void* stream)
+size_t gsparc_mlp1_work_size(int P, int R)
+{
+    if (P < 0 || R < 0) return 0;
+    return 256 + gs::mlp1_work_floats(P, R) * sizeof(float);
+}
+
+int gsparc_mlp1_forward(int P, int R, const float* means, const float* rx_pos, const float* tx_pos, const float* W1,
+                        const float* b1, const float* W2, const float* b2, const float* w3, const float* b3,
+                        void* work, float* emission, void* stream)
 {
     gs::Mlp1Args a;
-    if (int e = mlp1_args(P, R, pe1, rxf, w1rx, w2, b2, w3, b3, a)) return e;
-    if ((long long)P * R == 0) return GSPARC_OK;
-    if (!emission) return fail(GSPARC_ERR_ARG, "null pointer");
+    if (int e = mlp1_args(P, R, means, rx_pos, tx_pos, W1, b1, W2, b2, w3, b3, work, a)) return e;
+    if (P == 0) return GSPARC_OK;
+    if ((long long)P * R > 0 && !emission) return fail(GSPARC_ERR_ARG, "null pointer");
     a.out = emission;
     GS_LAUNCH("mlp1_fwd", gs::launch_mlp1_fwd(a, (hipStream_t)stream), (hipStream_t)stream);
     return GSPARC_OK;
 }
 
-size_t gsparc_mlp1_work_size(int P, int R)
-{
-    if (P <= 0 || R <= 0) return 0;
-    return gs::mlp1_part_floats((long long)P * R) * sizeof(float);
-}
-
-int gsparc_mlp1_backward(int P, int R, const float* pe1, const float* rxf, const float* w1rx, const float* w2,
-                         const float* b2, const float* w3, const float* b3, const float* demission, float* dh1,
-                         void* work, float* grads, void* stream)
+int gsparc_mlp1_backward(int P, int R, const float* means, const float* rx_pos, const float* tx_pos, const float* W1,
+                         const float* b1, const float* W2, const float* b2, const float* w3, const float* b3,
+                         void* work, const float* demission, float* dmeans, float* grads, void* stream)
 {
     gs::Mlp1Args a;
-    if (int e = mlp1_args(P, R, pe1, rxf, w1rx, w2, b2, w3, b3, a)) return e;
-    if ((long long)P * R == 0) return GSPARC_OK;
-    if (!demission || !dh1 || !work || !grads) return fail(GSPARC_ERR_ARG, "null pointer");
-    GS_LAUNCH("mlp1_bwd", gs::launch_mlp1_bwd(a, demission, dh1, (float*)work, grads, (hipStream_t)stream),
+    if (int e = mlp1_args(P, R, means, rx_pos, tx_pos, W1, b1, W2, b2, w3, b3, work, a)) return e;
+    if (!grads || (P > 0 && !dmeans) || ((long long)P * R > 0 && !demission))
+        return fail(GSPARC_ERR_ARG, "null pointer");
+    GS_LAUNCH("mlp1_bwd", gs::launch_mlp1_bwd(a, demission, dmeans, a.pe1, grads, (hipStream_t)stream),
               (hipStream_t)stream);
     return GSPARC_OK;
 }

@@ -2,37 +2,46 @@
 // matrix cores (FP32 MFMA, FP32 accumulate).  No counterpart in the reference
 // (whose emission MLP is per Gaussian, 3->32->2, gaussian.py:47).
 //
-// Rows are (receiver, Gaussian) pairs, row = r * P + g.  With
-//   h1 = relu(pe1[g] + W1rx rxf[row])      pe1 = PE(mu) W1pe^T + b1   [P,128]
-//   h2 = relu(W2 h1 + b2)                  W2 [128,128] (out, in)
-//   a  = w3 . h2 + b3
-// the forward kernel computes a for 128-row tiles (4 waves x 32 rows; the
-// grid strides over the tiles, so W2 is staged once per workgroup): h1 on
-// the vector ALU into LDS, the 128x128 layer as 32x32x2 MFMA
-// tiles (A = h1 rows from LDS, B = W2^T from LDS), relu, bias and the w3 dot
-// in the epilogue with a 32-lane sum.  The backward kernel recomputes h1 and
-// h2 for 64-row tiles and forms
-//   g2  = da w3 (h2 > 0)                      (per row, in LDS)
-//   dW2 += g2^T h1   (MFMA, K = rows; each wave keeps 32 rows of dW2)
-//   dh1 = (g2 W2) (h1 > 0)                    (MFMA, written per row)
-//   dw3 += da h2, db2 += g2, db3 += da
-// with per-workgroup partial sums reduced by a column-sum kernel (fixed
-// order).  What remains (the PE GEMM and its transpose, dW1rx = dh1^T rxf,
-// the receiver-feature chain rule) is plain GEMM / elementwise work left to
-// the caller.
+// For receiver r and Gaussian g with mean mu:
+//   PE(mu) [63]   mu, then sin(pi x), cos(pi x) of x = 2^k mu reduced exactly
+//                 to [-1, 1] (x - 2 rint(x / 2)), k = 0..9, each a 3-vector
+//   f [5]         d = |rx - mu|, 1 / max(d^2, 0.04), (tx - mu) / |tx - mu|
+//   h1 = relu(pe1[g] + W1rx f)      pe1 = PE(mu) W1pe^T + b1, W1 = [W1pe | W1rx]
+//   h2 = relu(W2 h1 + b2)           a = w3 . h2 + b3,  emission (a, 0)
+// Kernels:
+//   k_mlp1_pe       pe1 for 128 Gaussians per workgroup: PE into LDS, then
+//                   32x32x2 MFMA tiles against W1pe^T (K = 63, padded to 64)
+//   k_mlp1_fwd      a for 128-row tiles of (Gaussian, receiver) rows
+//   k_mlp1_bwd      64-row tiles: recompute h1, h2; g2 = da w3 (h2 > 0);
+//                   dW2 += g2^T h1, dh1 = (g2 W2)(h1 > 0) as MFMA; from the
+//                   dh1 tile in LDS: dW1rx += dh1^T f, the sums over receivers
+//                   dpe1[g] += sum_r dh1 and the receiver-feature chain rule
+//                   into dmu (atomics once per run of rows of one Gaussian)
+//   k_mlp1_pe_bwd   per 128 Gaussians: dPE = dpe1 W1pe and dW1pe += dpe1^T PE
+//                   as MFMA, db1 += dpe1, and the PE chain rule into dmu
+// Rows are Gaussian-major inside the kernels, q = g R + r, so that a
+// Gaussian's receivers sit in consecutive rows; the emission and its
+// gradient keep the rasterizer's layout [R, P, 2] (row r P + g).
+// Per-workgroup partial sums are reduced by a column-sum kernel in
+// workgroup order.
 //
 // MFMA 32x32x2 f32 operand layout (wave64): lane l supplies A[l%32][l/32] and
 // B[l/32][l%32]; accumulator element v of lane l is C[8(v/4) + 4(l/32) +
-// v%4][l%32].  LDS rows are padded to 129 floats so that the 32 lanes of a
+// v%4][l%32].  LDS rows are padded to an odd stride so that the 32 lanes of a
 // half-wave hit 32 different banks in every operand read.
+#include <cmath>
+
 #include "gs_device.h"
 #include "gs_kernels.h"
 
 namespace gs {
 
 #define M1_H 128     // hidden width
-#define M1_S 129     // LDS row stride
-#define M1_FEAT 5    // receiver features per row
+#define M1_S 129     // LDS row stride of 128-wide tiles
+#define M1_IN 68     // inputs: 63 PE + 5 receiver features
+#define M1_PE 63
+#define M1_FEAT 5
+#define M1_KS 65     // LDS row stride of 64-wide (PE) tiles
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
@@ -52,27 +61,105 @@ __device__ __forceinline__ void load_w2t(const float* w2, float* w2t)
     }
 }
 
-// h1 for `rows` rows starting at row0 into h1s[m * M1_S + u]; `threads`
-// threads share the work (thread t: row t % rows, a slice of the units).
-__device__ __forceinline__ void layer1(const Mlp1Args& a, long long row0, int rows, float* h1s)
+// component c (0..63) of PE(mu); 63 is the zero pad
+__device__ __forceinline__ float pe_component(const float* mu, int c)
 {
-    const long long n_rows = (long long)a.P * a.R;
+    if (c < 3) return mu[c];
+    if (c >= M1_PE) return 0.0f;
+    const int k = (c - 3) / 6, j = (c - 3) % 6;
+    float x = mu[j % 3] * (float)(1 << k);
+    x = x - 2.0f * rintf(0.5f * x);
+    const float a = 3.14159265358979323846f * x;
+    return j < 3 ? sinf(a) : cosf(a);
+}
+
+// Receiver features of row (g, r).
+__device__ __forceinline__ void rx_features(const Mlp1Args& a, int g, int r, float* f)
+{
+    const float mx = a.means[3 * (size_t)g], my = a.means[3 * (size_t)g + 1], mz = a.means[3 * (size_t)g + 2];
+    const float dx = a.rx[3 * r] - mx, dy = a.rx[3 * r + 1] - my, dz = a.rx[3 * r + 2] - mz;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    f[0] = sqrtf(d2);
+    f[1] = 1.0f / fmaxf(d2, 0.04f);
+    const float tx = a.tx[0] - mx, ty = a.tx[1] - my, tz = a.tx[2] - mz;
+    const float tn = sqrtf(tx * tx + ty * ty + tz * tz);
+    f[2] = tx / tn;
+    f[3] = ty / tn;
+    f[4] = tz / tn;
+}
+
+// ------------------------------------------------------------ PE layer
+// pe1[g][n] = sum_c PE(mu_g)[c] W1[n][c] + b1[n], 128 Gaussians per tile,
+// wave w the tile's Gaussians 32w..32w+31.
+__global__ void __launch_bounds__(256) k_mlp1_pe(Mlp1Args a)
+{
+    __shared__ float wpt[64 * M1_S];       // W1pe^T: wpt[c * M1_S + n] = W1[n][c], row 63 zero
+    __shared__ float pes[128 * M1_KS];     // PE rows
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < 64 * M1_H; i += 256) {
+        const int n = i >> 6, c = i & 63;
+        wpt[c * M1_S + n] = c < M1_PE ? a.w1[n * M1_IN + c] : 0.0f;
+    }
+    for (int tile = blockIdx.x; tile * 128 < a.P; tile += gridDim.x) {
+        const int g0 = tile * 128;
+        __syncthreads();
+        for (int i = threadIdx.x; i < 128 * 64; i += 256) {
+            const int m = i >> 6, c = i & 63, g = g0 + m;
+            float mu[3] = {0.0f, 0.0f, 0.0f};
+            if (g < a.P)
+                for (int j = 0; j < 3; ++j) mu[j] = a.means[3 * (size_t)g + j];
+            pes[m * M1_KS + c] = g < a.P ? pe_component(mu, c) : 0.0f;
+        }
+        __syncthreads();
+        floatx16 acc[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[t] = floatx16{};
+        const float* arow = pes + (32 * w + (lane & 31)) * M1_KS + (lane >> 5);
+        const float* bcol = wpt + (lane >> 5) * M1_S + (lane & 31);
+        for (int ks = 0; ks < 32; ++ks) {
+            const float av = arow[2 * ks];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t] = mfma32x32x2(av, bcol[2 * ks * M1_S + 32 * t], acc[t]);
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int n = 32 * t + (lane & 31);
+            const float bb = a.b1[n];
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int g = g0 + 32 * w + acc_row(v, lane);
+                if (g < a.P) a.pe1[(size_t)g * M1_H + n] = acc[t][v] + bb;
+            }
+        }
+    }
+}
+
+// h1 of `rows` rows starting at q0 into h1s[m * M1_S + u] (and, when fs is
+// given, the rows' features into fs[m * M1_FEAT + i]); the block's threads
+// share the work (thread t: row t % rows, a slice of the units).
+__device__ __forceinline__ void layer1(const Mlp1Args& a, long long q0, int rows, const float* w1rx_s, float* h1s,
+                                       float* fs)
+{
+    const int n_rows = a.P * a.R;
     const int tid = threadIdx.x;
     const int m = tid % rows, part = tid / rows, nparts = blockDim.x / rows;
     const int per = M1_H / nparts, u0 = part * per;
-    const long long row = row0 + m;
+    const int q = (int)q0 + m;
     float* dst = h1s + m * M1_S;
-    if (row >= n_rows) {
+    if (q >= n_rows) {
         for (int u = u0; u < u0 + per; ++u) dst[u] = 0.0f;
+        if (fs && part == 0)
+            for (int i = 0; i < M1_FEAT; ++i) fs[m * M1_FEAT + i] = 0.0f;
         return;
     }
-    const int g = (int)(row % a.P);
+    const int g = q / a.R, r = q - g * a.R;
     float f[M1_FEAT];
-#pragma unroll
-    for (int i = 0; i < M1_FEAT; ++i) f[i] = a.rxf[row * M1_FEAT + i];
+    rx_features(a, g, r, f);
+    if (fs && part == 0)
+        for (int i = 0; i < M1_FEAT; ++i) fs[m * M1_FEAT + i] = f[i];
     const float* pe = a.pe1 + (size_t)g * M1_H;
     for (int u = u0; u < u0 + per; ++u) {
-        const float* wr = a.w1rx + u * M1_FEAT;
+        const float* wr = w1rx_s + u * M1_FEAT;
         float v = pe[u];
 #pragma unroll
         for (int i = 0; i < M1_FEAT; ++i) v = fmaf(wr[i], f[i], v);
@@ -80,20 +167,30 @@ __device__ __forceinline__ void layer1(const Mlp1Args& a, long long row0, int ro
     }
 }
 
+__device__ __forceinline__ void load_w1rx(const float* w1, float* w1rx_s)
+{
+    for (int i = threadIdx.x; i < M1_H * M1_FEAT; i += blockDim.x) {
+        const int u = i / M1_FEAT, k = i % M1_FEAT;
+        w1rx_s[i] = w1[u * M1_IN + M1_PE + k];
+    }
+}
+
 __global__ void __launch_bounds__(256) k_mlp1_fwd(Mlp1Args a)
 {
     __shared__ float w2t[M1_H * M1_S];
     __shared__ float h1s[128 * M1_S];
+    __shared__ float w1rx_s[M1_H * M1_FEAT];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const long long n_rows = (long long)a.P * a.R;
     load_w2t(a.w2, w2t);   // once per workgroup: the grid strides over 128-row tiles
+    load_w1rx(a.w1, w1rx_s);
     const float b3 = a.b3[0];
     const float* hrow = h1s + (32 * w + (lane & 31)) * M1_S + (lane >> 5);
     const float* bcol = w2t + (lane >> 5) * M1_S + (lane & 31);
     for (long long tile = blockIdx.x; tile * 128 < n_rows; tile += gridDim.x) {
-        const long long row0 = tile * 128;
-        __syncthreads();   // w2t loaded / the previous tile's h1s reads done
-        layer1(a, row0, 128, h1s);
+        const long long q0 = tile * 128;
+        __syncthreads();   // weights loaded / the previous tile's h1s reads done
+        layer1(a, q0, 128, w1rx_s, h1s, nullptr);
         __syncthreads();
         floatx16 acc[4];
 #pragma unroll
@@ -120,39 +217,52 @@ __global__ void __launch_bounds__(256) k_mlp1_fwd(Mlp1Args a)
         if ((lane & 31) == 0) {
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                const long long row = row0 + 32 * w + acc_row(v, lane);
-                if (row < n_rows) {
-                    a.out[2 * row] = part[v] + b3;
-                    a.out[2 * row + 1] = 0.0f;
+                const int q = (int)q0 + 32 * w + acc_row(v, lane);
+                if (q < n_rows) {
+                    const int g = q / a.R, r = q - g * a.R;
+                    const size_t o = (size_t)r * a.P + g;
+                    a.out[2 * o] = part[v] + b3;
+                    a.out[2 * o + 1] = 0.0f;
                 }
             }
         }
     }
 }
 
-// partial record per workgroup: dW2 [128*128] | dw3 [128] | db2 [128] | db3
-#define M1_PART (M1_H * M1_H + 2 * M1_H + 1)
+// row-kernel partial record per workgroup:
+//   dW2 [128*128] | dw3 [128] | db2 [128] | db3 | dW1rx [128*5]
+#define M1_PART (M1_H * M1_H + 2 * M1_H + 1 + M1_H * M1_FEAT)
 
-__global__ void __launch_bounds__(256) k_mlp1_bwd(Mlp1Args a, const float* da, float* dh1, float* part)
+__global__ void __launch_bounds__(256) k_mlp1_bwd(Mlp1Args a, const float* dem, float* dpe1, float* dmu, float* part)
 {
     __shared__ float w2t[M1_H * M1_S];
     __shared__ float h1s[64 * M1_S];
     __shared__ float g2s[64 * M1_S];
+    __shared__ float w1rx_s[M1_H * M1_FEAT];
+    __shared__ float fs[64 * M1_FEAT];
     __shared__ float red[4][2][2][32];
     __shared__ float red3[256];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int mh = w & 1, nh = w >> 1;     // this wave's row half and pair of 32-column tiles
     const long long n_rows = (long long)a.P * a.R;
     load_w2t(a.w2, w2t);
+    load_w1rx(a.w1, w1rx_s);
     floatx16 dw2[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) dw2[u] = floatx16{};
     float dw3p[2] = {0.0f, 0.0f}, db2p[2] = {0.0f, 0.0f}, db3p = 0.0f;
+    float dw1rx[M1_FEAT] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    // da of row q (Gaussian-major) in the rasterizer's layout
+    auto da_of = [&](long long q) -> float {
+        if (q >= n_rows) return 0.0f;
+        const int g = (int)q / a.R, r = (int)q - g * a.R;
+        return dem[2 * ((size_t)r * a.P + g)];
+    };
     for (long long tile = blockIdx.x; tile * 64 < n_rows; tile += gridDim.x) {
-        const long long row0 = tile * 64;
-        __syncthreads();   // the previous tile's LDS reads are done (and w2t is loaded)
-        layer1(a, row0, 64, h1s);
-        if (threadIdx.x < 64 && row0 + threadIdx.x < n_rows) db3p += da[2 * (row0 + threadIdx.x)];
+        const long long q0 = tile * 64;
+        __syncthreads();   // the previous tile's LDS reads are done (and the weights are loaded)
+        layer1(a, q0, 64, w1rx_s, h1s, fs);
+        if (threadIdx.x < 64) db3p += da_of(q0 + threadIdx.x);
         __syncthreads();
         // h2 for rows 32*mh.., columns of tiles 2*nh, 2*nh + 1
         floatx16 c[2] = {floatx16{}, floatx16{}};
@@ -163,6 +273,9 @@ __global__ void __launch_bounds__(256) k_mlp1_bwd(Mlp1Args a, const float* da, f
             c[0] = mfma32x32x2(av, bcol[2 * ks * M1_S], c[0]);
             c[1] = mfma32x32x2(av, bcol[2 * ks * M1_S + 32], c[1]);
         }
+        float dav[16];
+#pragma unroll
+        for (int v = 0; v < 16; ++v) dav[v] = da_of(q0 + 32 * mh + acc_row(v, lane));
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
             const int nn = 64 * nh + 32 * t + (lane & 31);
@@ -170,12 +283,10 @@ __global__ void __launch_bounds__(256) k_mlp1_bwd(Mlp1Args a, const float* da, f
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
                 const int m = 32 * mh + acc_row(v, lane);
-                const long long row = row0 + m;
-                const float dav = row < n_rows ? da[2 * row] : 0.0f;
                 const float h2 = fmaxf(c[t][v] + bb, 0.0f);
-                const float g = h2 > 0.0f ? dav * ww : 0.0f;
+                const float g = h2 > 0.0f ? dav[v] * ww : 0.0f;
                 g2s[m * M1_S + nn] = g;
-                dw3p[t] = fmaf(dav, h2, dw3p[t]);
+                dw3p[t] = fmaf(dav[v], h2, dw3p[t]);
                 db2p[t] += g;
             }
         }
@@ -191,8 +302,8 @@ __global__ void __launch_bounds__(256) k_mlp1_bwd(Mlp1Args a, const float* da, f
             }
         }
         // dh1[m][k] = (sum_n g2[m][n] W2[n][k]) (h1 > 0): rows 32*mh.., column tiles 2*nh, 2*nh + 1
+        floatx16 e[2] = {floatx16{}, floatx16{}};
         {
-            floatx16 e[2] = {floatx16{}, floatx16{}};
             const float* grow = g2s + (32 * mh + (lane & 31)) * M1_S + (lane >> 5);
             const float* wk = w2t + (64 * nh + (lane & 31)) * M1_S + (lane >> 5);
             for (int ns = 0; ns < M1_H / 2; ++ns) {
@@ -200,16 +311,106 @@ __global__ void __launch_bounds__(256) k_mlp1_bwd(Mlp1Args a, const float* da, f
                 e[0] = mfma32x32x2(av, wk[2 * ns], e[0]);
                 e[1] = mfma32x32x2(av, wk[32 * M1_S + 2 * ns], e[1]);
             }
+        }
+        __syncthreads();   // every wave's dW2 reads of h1s are done: dh1 replaces h1 in place
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                const int k = 64 * nh + 32 * t + (lane & 31);
+        for (int t = 0; t < 2; ++t) {
+            const int k = 64 * nh + 32 * t + (lane & 31);
 #pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    const int m = 32 * mh + acc_row(v, lane);
-                    const long long row = row0 + m;
-                    if (row < n_rows) dh1[row * M1_H + k] = h1s[m * M1_S + k] > 0.0f ? e[t][v] : 0.0f;
+            for (int v = 0; v < 16; ++v) {
+                const int m = 32 * mh + acc_row(v, lane);
+                float* p = h1s + m * M1_S + k;
+                *p = *p > 0.0f ? e[t][v] : 0.0f;
+            }
+        }
+        __syncthreads();
+        // from the dh1 tile: dW1rx, the per-Gaussian sums dpe1, and the
+        // receiver-feature chain rule.  Row m of the tile is (Gaussian
+        // g0 + (r0 + m) / R, receiver (r0 + m) % R), tracked incrementally.
+        const int qt = (int)q0, g0 = qt / a.R, r0 = qt - g0 * a.R;
+        {
+            const int u = threadIdx.x & (M1_H - 1), half = threadIdx.x >> 7;
+            int g = g0, r = r0;
+            for (int k = 0; k < 32 * half; ++k)
+                if (++r == a.R) { r = 0; ++g; }
+            float run = 0.0f;
+            int gprev = g;
+            for (int mm = 0; mm < 32; ++mm) {
+                const int m = 32 * half + mm;
+                if (qt + m >= n_rows) break;
+                const float dh = h1s[m * M1_S + u];
+#pragma unroll
+                for (int i = 0; i < M1_FEAT; ++i) dw1rx[i] = fmaf(dh, fs[m * M1_FEAT + i], dw1rx[i]);
+                if (g != gprev) {
+                    atomicAdd(dpe1 + (size_t)gprev * M1_H + u, run);
+                    run = 0.0f;
+                    gprev = g;
+                }
+                run += dh;
+                if (++r == a.R) { r = 0; ++g; }
+            }
+            if (qt + 32 * half < n_rows) atomicAdd(dpe1 + (size_t)gprev * M1_H + u, run);
+        }
+        {
+            // drxf = dh1 W1rx: wave qu sums units 32 qu.. for all 64 rows (lane = row)
+            const int m = threadIdx.x & 63, qu = threadIdx.x >> 6;
+            float df[M1_FEAT] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            for (int u = 32 * qu; u < 32 * qu + 32; ++u) {
+                const float dh = h1s[m * M1_S + u];
+#pragma unroll
+                for (int i = 0; i < M1_FEAT; ++i) df[i] = fmaf(dh, w1rx_s[u * M1_FEAT + i], df[i]);
+            }
+#pragma unroll
+            for (int i = 0; i < M1_FEAT; ++i) g2s[(qu * M1_FEAT + i) * 64 + m] = df[i];
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            // wave 0, lane = row: the chain rule, then a segmented sum over
+            // the lanes of one Gaussian and one atomic per segment
+            const int m = threadIdx.x;
+            float df[M1_FEAT];
+#pragma unroll
+            for (int i = 0; i < M1_FEAT; ++i)
+                df[i] = g2s[i * 64 + m] + g2s[(M1_FEAT + i) * 64 + m] + g2s[(2 * M1_FEAT + i) * 64 + m] +
+                        g2s[(3 * M1_FEAT + i) * 64 + m];
+            const int rr = r0 + m;
+            const int g = g0 + rr / a.R, r = rr - (rr / a.R) * a.R;
+            const bool valid = qt + m < n_rows;
+            float gm[3] = {0.0f, 0.0f, 0.0f};
+            if (valid) {
+                const float mx = a.means[3 * (size_t)g], my = a.means[3 * (size_t)g + 1];
+                const float mz = a.means[3 * (size_t)g + 2];
+                const float dv[3] = {a.rx[3 * r] - mx, a.rx[3 * r + 1] - my, a.rx[3 * r + 2] - mz};
+                const float d2 = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
+                const float d = sqrtf(d2);
+                // d(d)/dmu = -diff / d; d(1/d2)/dmu = 2 diff / d2^2 where d2 > 0.04
+                const float cd = d > 0.0f ? -df[0] / d : 0.0f;
+                const float ci = d2 > 0.04f ? df[1] * 2.0f / (d2 * d2) : 0.0f;
+                const float t[3] = {a.tx[0] - mx, a.tx[1] - my, a.tx[2] - mz};
+                const float tn = sqrtf(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+                const float u3[3] = {t[0] / tn, t[1] / tn, t[2] / tn};
+                const float ug = u3[0] * df[2] + u3[1] * df[3] + u3[2] * df[4];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    // d(t/|t|)/dmu = -(I - u u^T) / |t|
+                    const float gt = -(df[2 + j] - u3[j] * ug) / tn;
+                    gm[j] = (cd + ci) * dv[j] + gt;
                 }
             }
+            const int gk = valid ? g : -1;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int gp = __shfl_up(gk, off);
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const float vp = __shfl_up(gm[j], off);
+                    if (m >= off && gp == gk) gm[j] += vp;
+                }
+            }
+            const int gn = __shfl_down(gk, 1);
+            if (valid && (m == 63 || gn != gk))
+#pragma unroll
+                for (int j = 0; j < 3; ++j) atomicAdd(dmu + 3 * (size_t)g + j, gm[j]);
         }
     }
     // this workgroup's partial sums
@@ -243,44 +444,201 @@ __global__ void __launch_bounds__(256) k_mlp1_bwd(Mlp1Args a, const float* da, f
         for (int i = 0; i < 64; ++i) s += red3[i];
         out[M1_H * M1_H + 2 * M1_H] = s;
     }
+    // dW1rx[u][i]: the two row halves of unit u
+    __syncthreads();
+    float* dws = &red3[0];   // reuse: 256 threads x 5 values do not fit; two rounds of 256
+    for (int i = 0; i < M1_FEAT; ++i) {
+        dws[threadIdx.x] = dw1rx[i];
+        __syncthreads();
+        if (threadIdx.x < M1_H)
+            out[M1_H * M1_H + 2 * M1_H + 1 + threadIdx.x * M1_FEAT + i] = dws[threadIdx.x] + dws[threadIdx.x + M1_H];
+        __syncthreads();
+    }
 }
 
-// out[c] = sum over workgroups of part[wg][c], in workgroup order
-__global__ void __launch_bounds__(256) k_mlp1_colsum(const float* part, int nwg, float* out)
+// ------------------------------------------------------- PE layer backward
+// pe partial record per workgroup: dW1pe [128 * 64] (column 63 unused) | db1 [128]
+#define M1_PEPART (M1_H * 64 + M1_H)
+
+__global__ void __launch_bounds__(256) k_mlp1_pe_bwd(Mlp1Args a, const float* dpe1, float* dmu, float* part)
+{
+    __shared__ float wpe[M1_H * M1_KS];    // W1pe: wpe[u * M1_KS + c] = W1[u][c] (c < 63), 0 for c = 63
+    __shared__ float dps[128 * M1_S];      // dpe1 rows of the tile
+    __shared__ float pes[128 * M1_KS];     // PE rows of the tile; then dPE rows
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < M1_H * 64; i += 256) {
+        const int u = i >> 6, c = i & 63;
+        wpe[u * M1_KS + c] = c < M1_PE ? a.w1[u * M1_IN + c] : 0.0f;
+    }
+    floatx16 dw[2] = {floatx16{}, floatx16{}};    // dW1pe rows u = 32w.., columns c = 0..63
+    float db1 = 0.0f;
+    for (int tile = blockIdx.x; tile * 128 < a.P; tile += gridDim.x) {
+        const int g0 = tile * 128;
+        __syncthreads();
+        for (int i = threadIdx.x; i < 128 * M1_H; i += 256) {
+            const int m = i >> 7, u = i & (M1_H - 1), g = g0 + m;
+            dps[m * M1_S + u] = g < a.P ? dpe1[(size_t)g * M1_H + u] : 0.0f;
+        }
+        for (int i = threadIdx.x; i < 128 * 64; i += 256) {
+            const int m = i >> 6, c = i & 63, g = g0 + m;
+            float mu[3] = {0.0f, 0.0f, 0.0f};
+            if (g < a.P)
+                for (int j = 0; j < 3; ++j) mu[j] = a.means[3 * (size_t)g + j];
+            pes[m * M1_KS + c] = g < a.P ? pe_component(mu, c) : 0.0f;
+        }
+        __syncthreads();
+        if (threadIdx.x < M1_H)
+            for (int m = 0; m < 128; ++m) db1 += dps[m * M1_S + threadIdx.x];
+        // dW1pe[u][c] += sum_m dpe1[m][u] PE[m][c]: wave w the rows u = 32w..; K = the 128 Gaussians
+        {
+            const float* arow = dps + (lane >> 5) * M1_S + 32 * w + (lane & 31);
+            const float* bcol = pes + (lane >> 5) * M1_KS + (lane & 31);
+            for (int ms = 0; ms < 64; ++ms) {
+                const float av = arow[2 * ms * M1_S];
+                dw[0] = mfma32x32x2(av, bcol[2 * ms * M1_KS], dw[0]);
+                dw[1] = mfma32x32x2(av, bcol[2 * ms * M1_KS + 32], dw[1]);
+            }
+        }
+        // dPE[m][c] = sum_u dpe1[m][u] W1pe[u][c]: wave w the Gaussians 32w..
+        floatx16 dp[2] = {floatx16{}, floatx16{}};
+        {
+            const float* arow = dps + (32 * w + (lane & 31)) * M1_S + (lane >> 5);
+            const float* bcol = wpe + (lane >> 5) * M1_KS + (lane & 31);
+            for (int ks = 0; ks < M1_H / 2; ++ks) {
+                const float av = arow[2 * ks];
+                dp[0] = mfma32x32x2(av, bcol[2 * ks * M1_KS], dp[0]);
+                dp[1] = mfma32x32x2(av, bcol[2 * ks * M1_KS + 32], dp[1]);
+            }
+        }
+        __syncthreads();   // every wave's PE reads are done: dPE replaces PE in place
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) pes[(32 * w + acc_row(v, lane)) * M1_KS + 32 * t + (lane & 31)] = dp[t][v];
+        __syncthreads();
+        // the PE chain rule: thread (m, j) for Gaussian m, coordinate j
+        for (int i = threadIdx.x; i < 128 * 3; i += 256) {
+            const int m = i / 3, j = i % 3, g = g0 + m;
+            if (g >= a.P) continue;
+            const float* dpr = pes + m * M1_KS;
+            const float mu = a.means[3 * (size_t)g + j];
+            float s = dpr[j];
+            for (int k = 0; k < 10; ++k) {
+                float x = mu * (float)(1 << k);
+                x = x - 2.0f * rintf(0.5f * x);
+                const float ang = 3.14159265358979323846f * x;
+                const float sc = 3.14159265358979323846f * (float)(1 << k);
+                s += sc * (dpr[3 + 6 * k + j] * cosf(ang) - dpr[6 + 6 * k + j] * sinf(ang));
+            }
+            dmu[3 * (size_t)g + j] += s;
+        }
+    }
+    float* out = part + (size_t)blockIdx.x * M1_PEPART;
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int u = 32 * w + acc_row(v, lane), c = 32 * t + (lane & 31);
+            out[u * 64 + c] = dw[t][v];
+        }
+    if (threadIdx.x < M1_H) out[M1_H * 64 + threadIdx.x] = db1;
+}
+
+// out[c] = sum over workgroups of part[wg][c] (record width `width`), in
+// workgroup order
+__global__ void __launch_bounds__(256) k_mlp1_colsum(const float* part, int nwg, int width, float* out)
 {
     const int c = blockIdx.x * 256 + threadIdx.x;
-    if (c >= M1_PART) return;
+    if (c >= width) return;
     float s = 0.0f;
-    for (int i = 0; i < nwg; ++i) s += part[(size_t)i * M1_PART + c];
+    for (int i = 0; i < nwg; ++i) s += part[(size_t)i * width + c];
     out[c] = s;
 }
 
-int mlp1_bwd_groups(long long n_rows)
+// grads = dW1 [128 x 68] | db1 | dW2 [128 x 128] | db2 | dw3 | db3 from the
+// two column sums
+__global__ void __launch_bounds__(256) k_mlp1_assemble(const float* rowsum, const float* pesum, float* grads)
 {
-    const long long tiles = (n_rows + 63) / 64;
-    return (int)(tiles < 1024 ? tiles : 1024);
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int nW1 = M1_H * M1_IN;
+    if (i < nW1) {
+        const int u = i / M1_IN, c = i % M1_IN;
+        grads[i] = c < M1_PE ? pesum[u * 64 + c] : rowsum[M1_H * M1_H + 2 * M1_H + 1 + u * M1_FEAT + (c - M1_PE)];
+    } else if (i < nW1 + M1_H) {
+        grads[i] = pesum[M1_H * 64 + (i - nW1)];
+    } else if (i < nW1 + M1_H + M1_H * M1_H) {
+        grads[i] = rowsum[i - nW1 - M1_H];                                  // dW2
+    } else if (i < nW1 + 2 * M1_H + M1_H * M1_H) {
+        grads[i] = rowsum[M1_H * M1_H + M1_H + (i - nW1 - M1_H - M1_H * M1_H)];   // db2
+    } else if (i < nW1 + 3 * M1_H + M1_H * M1_H) {
+        grads[i] = rowsum[M1_H * M1_H + (i - nW1 - 2 * M1_H - M1_H * M1_H)];      // dw3
+    } else if (i == nW1 + 3 * M1_H + M1_H * M1_H) {
+        grads[i] = rowsum[M1_H * M1_H + 2 * M1_H];                           // db3
+    }
 }
 
-size_t mlp1_part_floats(long long n_rows) { return (size_t)mlp1_bwd_groups(n_rows) * M1_PART; }
+__global__ void __launch_bounds__(256) k_mlp1_zero(float* p, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = 0.0f;
+}
+
+// ---------------------------------------------------------------- launchers
+static int grid_for(long long tiles) { return (int)(tiles < 1 ? 1 : (tiles < 1024 ? tiles : 1024)); }
+
+int mlp1_bwd_groups(long long n_rows) { return grid_for((n_rows + 63) / 64); }
+int mlp1_pe_groups(int P) { return grid_for(((long long)P + 127) / 128); }
+
+size_t mlp1_work_floats(int P, int R)
+{
+    const long long rows = (long long)P * R;
+    return 2 * (size_t)P * M1_H + (size_t)mlp1_bwd_groups(rows) * M1_PART + (size_t)mlp1_pe_groups(P) * M1_PEPART +
+           M1_PART + M1_PEPART + 64;
+}
+
+size_t mlp1_grad_floats() { return (size_t)M1_H * M1_IN + 3 * M1_H + M1_H * M1_H + 1; }
 
 hipError_t launch_mlp1_fwd(const Mlp1Args& a, hipStream_t s)
 {
+    if (a.P == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mlp1_pe, dim3(mlp1_pe_groups(a.P)), dim3(256), 0, s, a);
     const long long n_rows = (long long)a.P * a.R;
-    if (n_rows == 0) return hipSuccess;
-    const long long tiles = (n_rows + 127) / 128;
-    hipLaunchKernelGGL(k_mlp1_fwd, dim3((unsigned)(tiles < 1024 ? tiles : 1024)), dim3(256), 0, s, a);
+    if (n_rows == 0) return hipGetLastError();
+    hipLaunchKernelGGL(k_mlp1_fwd, dim3(grid_for((n_rows + 127) / 128)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_mlp1_bwd(const Mlp1Args& a, const float* da, float* dh1, float* part, float* grads, hipStream_t s)
+hipError_t launch_mlp1_bwd(const Mlp1Args& a, const float* dem, float* dmu, float* work, float* grads, hipStream_t s)
 {
     const long long n_rows = (long long)a.P * a.R;
-    if (n_rows == 0) return hipSuccess;
-    const int nwg = mlp1_bwd_groups(n_rows);
-    hipLaunchKernelGGL(k_mlp1_bwd, dim3(nwg), dim3(256), 0, s, a, da, dh1, part);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_mlp1_colsum, dim3((M1_PART + 255) / 256), dim3(256), 0, s, part, nwg, grads);
+    float* dpe1 = work + (size_t)a.P * M1_H;
+    const int nwg = mlp1_bwd_groups(n_rows), npe = mlp1_pe_groups(a.P);
+    float* rpart = work + 2 * (size_t)a.P * M1_H;
+    float* ppart = rpart + (size_t)nwg * M1_PART;
+    float* rsum = ppart + (size_t)npe * M1_PEPART;
+    float* psum = rsum + M1_PART;
+    const size_t nz = (size_t)a.P * M1_H;
+    hipLaunchKernelGGL(k_mlp1_zero, dim3((unsigned)std::max<size_t>(1, std::min<size_t>((nz + 255) / 256, 4096))),
+                       dim3(256), 0, s, dpe1, nz);
+    hipLaunchKernelGGL(k_mlp1_zero, dim3((unsigned)std::max<size_t>(1, std::min<size_t>((3 * (size_t)a.P + 255) / 256,
+                                                                                      4096))),
+                       dim3(256), 0, s, dmu, 3 * (size_t)a.P);
+    if (n_rows > 0) {
+        hipLaunchKernelGGL(k_mlp1_bwd, dim3(nwg), dim3(256), 0, s, a, dem, dpe1, dmu, rpart);
+        hipLaunchKernelGGL(k_mlp1_colsum, dim3((M1_PART + 255) / 256), dim3(256), 0, s, (const float*)rpart, nwg,
+                           (int)M1_PART, rsum);
+    } else {
+        hipLaunchKernelGGL(k_mlp1_zero, dim3(64), dim3(256), 0, s, rsum, (size_t)M1_PART);
+    }
+    if (a.P > 0) {
+        hipLaunchKernelGGL(k_mlp1_pe_bwd, dim3(npe), dim3(256), 0, s, a, (const float*)dpe1, dmu, ppart);
+        hipLaunchKernelGGL(k_mlp1_colsum, dim3((M1_PEPART + 255) / 256), dim3(256), 0, s, (const float*)ppart, npe,
+                           (int)M1_PEPART, psum);
+    } else {
+        hipLaunchKernelGGL(k_mlp1_zero, dim3(64), dim3(256), 0, s, psum, (size_t)M1_PEPART);
+    }
+    const size_t ng = mlp1_grad_floats();
+    hipLaunchKernelGGL(k_mlp1_assemble, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, s, (const float*)rsum,
+                       (const float*)psum, grads);
     return hipGetLastError();
 }
 

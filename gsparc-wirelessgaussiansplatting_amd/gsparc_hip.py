@@ -120,9 +120,9 @@ _SIGS = {
                                           C.c_uint64, _vp, _vp, _vp]),
     "gsparc_rasterize_backward_csi": (_i, [_i, _i, _i, _i, C.c_double, C.c_double] + [_vp] * 8 + [_f, _vp, _vp, _vp,
                                            _vp, C.c_uint64, _vp, _vp] + [_vp] * 8 + [_vp]),
-    "gsparc_mlp1_forward": (_i, [_i, _i] + [_vp] * 9),
+    "gsparc_mlp1_forward": (_i, [_i, _i] + [_vp] * 12),
     "gsparc_mlp1_work_size": (C.c_size_t, [_i, _i]),
-    "gsparc_mlp1_backward": (_i, [_i, _i] + [_vp] * 12),
+    "gsparc_mlp1_backward": (_i, [_i, _i] + [_vp] * 14),
     "gsparc_csi_loss_work_size": (C.c_size_t, [_i, _i, _i]),
     "gsparc_csi_loss": (_i, [_i, _i, _i, _f, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gsparc_loss_work_size": (C.c_size_t, [_i, _i, _i]),

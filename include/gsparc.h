@@ -234,26 +234,30 @@ int gsparc_csi_loss(int R, int H, int W, float lambda_ssim, float C1, float C2, 
 
 /* ---- shared emission MLP on the matrix cores --------------------------------
  * The 68 -> 128 -> 128 -> 1 ReLU MLP of the single-receiver training
- * configuration, for rows = R * P (receiver, Gaussian) pairs, row = r * P + g:
- *   h1 = relu(pe1[g] + W1rx rxf[row]),  h2 = relu(W2 h1 + b2),  a = w3 . h2 + b3
- * with pe1 [P,128] = PE(mu) W1pe^T + b1 (the receiver-independent part of the
- * first layer, formed once per Gaussian by the caller), rxf [R*P,5] the
- * receiver features, w1rx [128,5], w2 [128,128] (out, in), b2 [128], w3
- * [128], b3 [1] (device).  emission [R*P,2] receives (a, 0): the rasterizer's
- * emission input.  The 128x128 layer runs as FP32 MFMA tiles with FP32
- * accumulation.  Backward: demission [R*P,2] in (the real part is dL/da);
- * dh1 [R*P,128] = dL/d(first-layer pre-activation) and grads = dW2 [128*128]
- * | dw3 [128] | db2 [128] | db3 [1] out (fixed summation order); work:
- * gsparc_mlp1_work_size bytes.  The rest of the chain (dW1 and db1 from dh1,
- * the positional encoding and receiver features) is plain GEMM and
- * elementwise work for the caller.  No counterpart in the reference, whose
- * emission MLP is per Gaussian (gaussian.py:47, forward.cu:17-75). */
-int gsparc_mlp1_forward(int P, int R, const float* pe1, const float* rxf, const float* w1rx, const float* w2,
-                        const float* b2, const float* w3, const float* b3, float* emission, void* stream);
+ * configuration, for every (receiver r, Gaussian g) pair:
+ *   x = [PE(mu_g) (63), d, 1 / max(d^2, 0.04), (tx - mu_g) / |tx - mu_g|],
+ *       d = |rx_r - mu_g|; PE = mu, then sin(pi x), cos(pi x) of x = 2^k mu
+ *       reduced exactly to [-1, 1], k = 0..9
+ *   h1 = relu(W1 x + b1),  h2 = relu(W2 h1 + b2),  a = w3 . h2 + b3
+ * means [P,3], rx_pos [R,3], tx_pos [3], W1 [128,68], b1 [128], W2 [128,128]
+ * (out, in), b2 [128], w3 [128], b3 [1] (device).  emission [R,P,2] receives
+ * (a, 0): the rasterizer's emission input.  The first layer is split: its
+ * PE part runs once per Gaussian, the 5 receiver features per pair; the
+ * layers run as FP32 MFMA tiles with FP32 accumulation.  work:
+ * gsparc_mlp1_work_size bytes, the same buffer for the forward and the
+ * backward (the backward reads what the forward left there).  Backward:
+ * demission [R,P,2] in (the real part is dL/da); dmeans [P,3] = the MLP's
+ * gradient of the means (through PE and the receiver features) and grads =
+ * dW1 [128*68] | db1 [128] | dW2 [128*128] | db2 [128] | dw3 [128] | db3 [1]
+ * out.  No counterpart in the reference, whose emission MLP is per Gaussian
+ * (gaussian.py:47, forward.cu:17-75). */
 size_t gsparc_mlp1_work_size(int P, int R);
-int gsparc_mlp1_backward(int P, int R, const float* pe1, const float* rxf, const float* w1rx, const float* w2,
-                         const float* b2, const float* w3, const float* b3, const float* demission, float* dh1,
-                         void* work, float* grads, void* stream);
+int gsparc_mlp1_forward(int P, int R, const float* means, const float* rx_pos, const float* tx_pos, const float* W1,
+                        const float* b1, const float* W2, const float* b2, const float* w3, const float* b3,
+                        void* work, float* emission, void* stream);
+int gsparc_mlp1_backward(int P, int R, const float* means, const float* rx_pos, const float* tx_pos, const float* W1,
+                         const float* b1, const float* W2, const float* b2, const float* w3, const float* b3,
+                         void* work, const float* demission, float* dmeans, float* grads, void* stream);
 
 /* Training loss of the reference, value and gradient in one call:
  *   loss = (1 - lambda_ssim) * mean|img - target| + lambda_ssim * (1 - mean SSIM(img, target))

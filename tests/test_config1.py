@@ -11,14 +11,14 @@ rects, list order, alpha skips, termination) are skipped."""
 import numpy as np
 import torch
 
-import gsparc_emission as GE
+import mlp_reference as GE
 import gsparc_scenes as S
 import oracle as O
 
 
 def _mlp64(seed=0):
     g = torch.Generator().manual_seed(seed)
-    m = GE.SharedEmissionMLP(generator=g, dtype=torch.float64)
+    m = GE.TorchSharedMLP(generator=g, dtype=torch.float64)
     with torch.no_grad():
         m.b3 += 0.5   # positive amplitudes on average
     return m

@@ -1,15 +1,16 @@
 """Config 1 on the GPU: the rasterizer's emission input and the full
 single-receiver training step (20k Gaussians, shared 68->128->128->1 MLP,
 0.8*L1 + 0.2*(1-SSIM)) against the CPU chain of the oracle's FP32 mirror (same
-discrete decisions as the GPU) and a float64 MLP, with the MLP on the fused
-MFMA kernels and on torch FP32 GEMMs; and the MFMA kernels against the torch
-FP32 form of the same MLP on ragged row counts."""
+discrete decisions as the GPU) and a float64 MLP, with the MLP on the
+library's kernels; and those kernels against the torch FP32 form of the same
+MLP (tests/mlp_reference.py) on ragged row counts."""
 import numpy as np
 import pytest
 import torch
 
 import cuda_rasterizer as cr
 import gsparc_emission as GE
+import mlp_reference as MR
 import gsparc_scenes as S
 import oracle as O
 import wireless_rasterizer as wr
@@ -59,14 +60,13 @@ def test_emission_input_parity(mode):
         assert _rel(dem[vis, 1], g8[vis, 7]) < 1e-4
 
 
-@pytest.mark.parametrize("backend", ["mfma", "torch"])
-def test_config1_training_step(backend):
+def test_config1_training_step():
     sc = S.config1()
     tgt = S.config1(seed=2)
     d = to_dev(sc)
     dt = to_dev(tgt)
     mlp64 = _mlp64(3)
-    mlp = GE.SharedEmissionMLP(backend=backend).cuda()
+    mlp = GE.SharedEmissionMLP().cuda()
     with torch.no_grad():
         for p32, p64 in zip(mlp.parameters(), mlp64.parameters()):
             p32.copy_(p64.float())
@@ -92,7 +92,7 @@ def test_config1_training_step(backend):
         assert e < 1e-4, (name, errs)
 
 
-@pytest.mark.parametrize("P,R", [(5000, 3), (37, 1), (300, 7)])
+@pytest.mark.parametrize("P,R", [(5000, 3), (37, 1), (300, 7), (130, 64), (1, 1)])
 def test_mfma_mlp_matches_torch(P, R):
     """Amplitudes within 1e-5 and every gradient (weights, biases, means)
     within 1e-4 of the torch FP32 form; row counts not multiples of the
@@ -102,8 +102,8 @@ def test_mfma_mlp_matches_torch(P, R):
     means = (torch.rand((P, 3), generator=g) * torch.tensor([6.0, 6.0, 3.0])).cuda()
     rx = (0.5 + torch.rand((R, 3), generator=g) * 2.0).cuda()
     tx = torch.tensor([0.0, 0.5, 3.0]).cuda()
-    ref = GE.SharedEmissionMLP(generator=torch.Generator().manual_seed(5), backend="torch").cuda()
-    fast = GE.SharedEmissionMLP(generator=torch.Generator().manual_seed(5), backend="mfma").cuda()
+    ref = MR.TorchSharedMLP(generator=torch.Generator().manual_seed(5)).cuda()
+    fast = GE.SharedEmissionMLP(generator=torch.Generator().manual_seed(5)).cuda()
     w = torch.randn((R, P), generator=g).cuda()
     out = {}
     for name, m in (("ref", ref), ("fast", fast)):
