@@ -26,14 +26,35 @@ namespace gs {
 
 struct Taps { float w[11]; };
 
+// The tile of this workgroup.  Workgroups are dealt to the 8 XCDs round
+// robin by id; tile t goes to the workgroup that gives XCD x a contiguous
+// range of tiles, so that neighbouring tiles, which share halo pixels, share
+// that XCD's L2.
+struct SsTile { int x, y, z; };
+
+__device__ __forceinline__ SsTile ssim_tile()
+{
+    const int gx = gridDim.x, gy = gridDim.y;
+    const int n = gx * gy * gridDim.z;
+    const int L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const int xcd = L & 7, k = L >> 3, per = n >> 3, rem = n & 7;
+    const int t = xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
+    SsTile r;
+    r.x = t % gx;
+    r.y = (t / gx) % gy;
+    r.z = t / (gx * gy);
+    return r;
+}
+
 __global__ void __launch_bounds__(SS_THREADS) k_ssim_fwd(SsimArgs a, Taps tp)
 {
     __shared__ float s1[SS_HH][SS_HW + 1];
     __shared__ float s2[SS_HH][SS_HW + 1];
     __shared__ float h[5][SS_HH][SS_TW + 1];
     const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * SS_TW, y0 = blockIdx.y * SS_TH;
-    const size_t plane = (size_t)blockIdx.z * a.H * a.W;
+    const SsTile tl = ssim_tile();
+    const int x0 = tl.x * SS_TW, y0 = tl.y * SS_TH;
+    const size_t plane = (size_t)tl.z * a.H * a.W;
     const float* im1 = a.img1 + plane;
     const float* im2 = a.img2 + plane;
     // halo staging: all of a thread's loads are issued before the first LDS
@@ -137,7 +158,7 @@ __global__ void __launch_bounds__(SS_THREADS) k_ssim_fwd(SsimArgs a, Taps tp)
             __syncthreads();
         }
         if (tid == 0) {
-            const size_t b = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+            const size_t b = ((size_t)tl.z * gridDim.y + tl.y) * gridDim.x + tl.x;
             for (int c = 0; c < ns; ++c) a.part[(size_t)ns * b + c] = r[c][0];
         }
     }
@@ -148,10 +169,11 @@ __global__ void __launch_bounds__(SS_THREADS) k_ssim_bwd(SsimBwdArgs a, Taps tp)
     __shared__ float s[3][SS_HH][SS_HW + 1];
     __shared__ float h[3][SS_HH][SS_TW + 1];
     const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * SS_TW, y0 = blockIdx.y * SS_TH;
-    const size_t plane = (size_t)blockIdx.z * a.H * a.W;
+    const SsTile tl = ssim_tile();
+    const int x0 = tl.x * SS_TW, y0 = tl.y * SS_TH;
+    const size_t plane = (size_t)tl.z * a.H * a.W;
     // the plane's loss weight (the confidence of its spectrum), wave-uniform
-    const float ps = a.plane_scale ? a.plane_scale[blockIdx.z] : 1.0f;
+    const float ps = a.plane_scale ? a.plane_scale[tl.z] : 1.0f;
     const float dmap_const = a.dmap_const * ps, l1w = a.l1w * ps;
     // the images at the thread's output pixels, loaded now and used at the end
     constexpr int NO = SS_TH * SS_TW / SS_THREADS;
