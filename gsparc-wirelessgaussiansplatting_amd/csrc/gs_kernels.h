@@ -439,6 +439,10 @@ hipError_t launch_densify_lists(int P, const DensifyCfg& c, const Gaussians& p, 
 hipError_t launch_densify_gather(const DensifyPlan& pl, const DensifyCfg& c, const Gaussians& p, const Gaussians& q,
                                  const float* noise, const DensifyWork& w, hipStream_t s);
 
+// The multi-frequency kernels' phase factors (cos 2 pi x, sin 2 pi x) of
+// cycle fractions x, for their accuracy check (k_render_csi.hip).
+hipError_t launch_phase_factors(size_t n, const float* x, float* c, float* s, hipStream_t st);
+
 // ---- the spectrum dataset on the device (k_data.hip) ----
 // q: gateway quaternion (x, y, z, w); out [N,16] column-major view matrices.
 hipError_t launch_view_matrices(int N, const double* q, const float* rx, float* out, hipStream_t s);

@@ -1524,6 +1524,14 @@ int gsparc_densify_and_prune(int P, const gsparc_densify_config* cfg, const gspa
     return GSPARC_OK;
 }
 
+int gsparc_csi_phase_factors(size_t n, const float* turns, float* cos_out, float* sin_out, void* stream)
+{
+    if (n > 0 && (!turns || !cos_out || !sin_out)) return fail(GSPARC_ERR_ARG, "null pointer");
+    GS_LAUNCH("phase_factors", gs::launch_phase_factors(n, turns, cos_out, sin_out, (hipStream_t)stream),
+              (hipStream_t)stream);
+    return GSPARC_OK;
+}
+
 // ---- the spectrum dataset on the device
 int gsparc_view_matrices(int N, const double* gateway_quat, const float* rx_pos, float* viewmatrices, void* stream)
 {

@@ -44,6 +44,25 @@ __device__ __forceinline__ void csi_phase(float x, float* c, float* s)
     *s = __builtin_amdgcn_sinf(x);
 }
 
+// csi_phase over an array, for its accuracy check
+__global__ void __launch_bounds__(256) k_phase_factors(size_t n, const float* x, float* c, float* s)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        float cv, sv;
+        csi_phase(x[i], &cv, &sv);
+        c[i] = cv;
+        s[i] = sv;
+    }
+}
+
+hipError_t launch_phase_factors(size_t n, const float* x, float* c, float* s, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_phase_factors, dim3((unsigned)std::min<size_t>((n + 255) / 256, 65536)), dim3(256), 0, st,
+                       n, x, c, s);
+    return hipGetLastError();
+}
+
 template <int PPL>
 __global__ void __launch_bounds__(GS_WAVE) k_render_fwd_csi(Dims d, RenderCsiArgs a)
 {

@@ -232,6 +232,14 @@ size_t gsparc_csi_loss_work_size(int R, int H, int W);
 int gsparc_csi_loss(int R, int H, int W, float lambda_ssim, float C1, float C2, const float* out, const float* target,
                     void* work, float* loss, float* dL_dout, void* stream);
 
+/* The multi-frequency kernels' phase factors: cos_out[i], sin_out[i] =
+ * cos 2 pi x, sin 2 pi x of x = turns[i], computed exactly as the CSI blend
+ * kernels compute them (the hardware v_cos_f32 / v_sin_f32, which take the
+ * angle in turns), for the accuracy check of that routine over the cycle
+ * fractions the kernels pass it ([-1/2, 1/2]).  No counterpart in the
+ * reference (one channel, config.h:15). */
+int gsparc_csi_phase_factors(size_t n, const float* turns, float* cos_out, float* sin_out, void* stream);
+
 /* ---- shared emission MLP on the matrix cores --------------------------------
  * The 68 -> 128 -> 128 -> 1 ReLU MLP of the single-receiver training
  * configuration, for every (receiver r, Gaussian g) pair:
