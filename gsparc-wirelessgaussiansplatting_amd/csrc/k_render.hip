@@ -148,9 +148,76 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
 struct __align__(16) BwdRec {
     float4 A;    // px, py, ha, nb
     float4 B;    // hc, opacity, thr, Gaussian index (bits)
-    float2 S;    // emission (real, imag)
-    float2 Q2;   // 2 ha, 2 hc: the conic factors of the 2D-mean gradient
+    float4 C;    // emission (real, imag); 2 ha, 2 hc: the conic factors of the 2D-mean gradient
 };
+
+// Per-lane sums over the lane's pixels for the current entry, with
+// D = o G dL/dalpha: sh = sum D dx, sk = sum D dy, sxx = sum D dx^2,
+// sxy = sum D dx dy, syy = sum D dy^2, sg = sum G dL/dalpha, swr / swi =
+// sum w dL/dpix.  The 2D-mean and conic gradients are linear in them.
+struct BwdSums {
+    float sh, sk, sxx, sxy, syy, sg, swr, swi;
+};
+
+// One (entry, quarter) step of the reverse walk (backward.cu:555-610).
+// c marks the lanes whose pixel the entry contributes to.
+//
+// FIRST (the wave's first quarter): every lane runs the arithmetic and the
+// step writes the sums instead of adding to them, so that they need no
+// clearing between entries.  The lanes without a contribution take G = 0,
+// hence alpha = 0: their accumulators are unchanged (fma(0, e, acc) = acc),
+// their T is kept by a select, and every term they write is 0.
+// Otherwise only the contributing lanes run it (EXEC-masked) and add to the
+// sums, which hold the first step's terms or were cleared before.
+template <int MODE, bool FIRST>
+__device__ __forceinline__ void bwd_step(bool c, float power, float dx, float dy, float opac, float2 S, float& T,
+                                         float& acc_re, float& acc_im, float gre, float gim, BwdSums& s)
+{
+    if (!FIRST && !c) return;
+    const float G = (FIRST && !c) ? 0.0f : gs_expf_grad(power);
+    const float oG = opac * G;
+    const float alpha = fminf(0.99f, oG);
+    // T before this Gaussian: T_after / (1 - alpha) (backward.cu:565)
+    const float Tb = T * __builtin_amdgcn_rcpf(1.0f - alpha);
+    T = (FIRST && !c) ? T : Tb;
+    const float sr = S.x;
+    const float si = (MODE == GRAD_REFERENCE) ? S.x : S.y;
+    // dL/dalpha = (c - accum) dL/dpixel T (backward.cu:586, 592), then
+    // accum = alpha c + (1 - alpha) accum for the next contributor
+    const float er = sr - acc_re;
+    float dalpha = er * gre;
+    acc_re = fmaf(alpha, er, acc_re);
+    if (MODE == GRAD_EXACT) {
+        const float ei = si - acc_im;
+        dalpha = fmaf(ei, gim, dalpha);
+        acc_im = fmaf(alpha, ei, acc_im);
+    }
+    dalpha *= T;
+    const float w = alpha * T;
+    // the exact adjoint has no gradient through the clamped alpha
+    const float da = (MODE == GRAD_EXACT && oG > 0.99f) ? 0.0f : dalpha;
+    const float D = oG * da;
+    const float h = D * dx, q = D * dy;
+    if (FIRST) {
+        s.swr = w * gre;
+        s.swi = w * gim;
+        s.sh = h;
+        s.sk = q;
+        s.sxx = h * dx;
+        s.sxy = h * dy;
+        s.syy = q * dy;
+        s.sg = G * da;
+    } else {
+        s.swr = fmaf(w, gre, s.swr);
+        s.swi = fmaf(w, gim, s.swi);
+        s.sh += h;
+        s.sk += q;
+        s.sxx = fmaf(h, dx, s.sxx);
+        s.sxy = fmaf(h, dy, s.sxy);
+        s.syy = fmaf(q, dy, s.syy);
+        s.sg = fmaf(G, da, s.sg);
+    }
+}
 
 template <int MODE, int PPL, bool COUNT>
 __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
@@ -203,13 +270,9 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
     // preprocess backward)
     const int vi = lane >> 3;
     const float post = (vi == 2 || vi == 4) ? -0.5f : (vi == 3 ? -1.0f : 1.0f);
-    // Per-lane sums over the lane's pixels for the current entry, with
-    // D = o G dL/dalpha: sh = sum D dx, sk = sum D dy, sxx = sum D dx^2,
-    // sxy = sum D dx dy, syy = sum D dy^2, sg = sum G dL/dalpha, swr / swi =
-    // sum w dL/dpix.  The 2D-mean and conic gradients are linear in them.
-    // Only contributing lanes add to them; they are cleared after each
-    // entry's reduction (an entry without contributors leaves them clear).
-    float sh = 0.0f, sk = 0.0f, sxx = 0.0f, sxy = 0.0f, syy = 0.0f, sg = 0.0f, swr = 0.0f, swi = 0.0f;
+    float* const g8r = a.g8 + rP * 8;
+    // pixel row groups of the wave: quarters k and k ^ 1 share a row when PPL >= 2
+    constexpr int NROW = PPL == 1 ? 1 : PPL / 2;
     for (int top = maxlast - 1; top >= 0; top -= GS_WAVE) {
         // entry j of this batch sits at list position top - j (walked back to front)
         const int rel = top - lane;
@@ -225,8 +288,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
             B.w = __uint_as_float(id);
             rec[lane].A = A;
             rec[lane].B = B;
-            rec[lane].S = S;
-            rec[lane].Q2 = make_float2(2.0f * A.z, 2.0f * B.x);
+            rec[lane].C = make_float4(S.x, S.y, 2.0f * A.z, 2.0f * B.x);
             // culled per 8x8 quarter: an entry is walked only over the
             // quarters where it can reach alpha >= 1/255 and where some
             // pixel's contributors reach back to it
@@ -248,73 +310,60 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
             mask &= mask - 1;
             const float4 A = rec[j].A;
             const float4 B = rec[j].B;
-            const float2 S = rec[j].S;
-            uint64_t anyc = 0;
+            const float4 C = rec[j].C;
+            const float2 S = make_float2(C.x, C.y);
             const int pos = top - j;
+            // quarters k and k ^ 1 share their pixel rows: the row terms of
+            // pair_power are computed once per row
+            float dyr[NROW], hdd[NROW];
+#pragma unroll
+            for (int g = 0; g < NROW; ++g) {
+                dyr[g] = A.y - pyf[g * (PPL / NROW)];
+                hdd[g] = (B.x * dyr[g]) * dyr[g];
+            }
+            // the entry's sums: written by its first step with a contributor
+            // (quarter 0) or cleared before the first (a later quarter), so
+            // nothing is carried from the previous entry
+            BwdSums s;
+            bool have = false;   // wave-uniform: a quarter of this entry had a contributor
 #pragma unroll
             for (int k = 0; k < PPL; ++k) {
                 if (!((qmask[k] >> j) & 1ull)) continue;   // wave-uniform: the quarter is culled
-                const float dx = A.x - pxf[k], dy = A.y - pyf[k];
-                const float power = pair_power(A.z, A.w, B.x, dx, dy);
+                const int g = k / (PPL / NROW);
+                const float dx = A.x - pxf[k];
+                const float power = fmaf(dx, fmaf(A.w, dyr[g], A.z * dx), hdd[g]);   // pair_power
                 // the forward's decision: power in [thr, 0] <=> alpha >= 1/255
                 const bool act = pos < last[k], nonpos = !(power > 0.0f), reach = !(power < B.z);
                 const bool c = act && nonpos && reach;
                 // the masks of single compares combine on the scalar unit
-                anyc |= ballot(act) & ballot(nonpos) & ballot(reach);
-                if (c) {
-                    // only the contributing lanes run the pair's arithmetic
-                    const float G = gs_expf_grad(power);
-                    const float oG = B.y * G;
-                    const float alpha = fminf(0.99f, oG);
-                    // T before this Gaussian: T_after / (1 - alpha) (backward.cu:565)
-                    T[k] = T[k] * __builtin_amdgcn_rcpf(1.0f - alpha);
-                    const float w = alpha * T[k];
-                    const float sr = S.x;
-                    const float si = (MODE == GRAD_REFERENCE) ? S.x : S.y;
-                    // dL/dalpha = (c - accum) dL/dpixel T (backward.cu:586, 592), then
-                    // accum = alpha c + (1 - alpha) accum for the next contributor
-                    const float er = sr - acc_re[k];
-                    float dalpha = er * gre[k];
-                    acc_re[k] = fmaf(alpha, er, acc_re[k]);
-                    if (MODE == GRAD_EXACT) {
-                        const float ei = si - acc_im[k];
-                        dalpha = fmaf(ei, gim[k], dalpha);
-                        acc_im[k] = fmaf(alpha, ei, acc_im[k]);
-                    }
-                    dalpha *= T[k];
-                    swr = fmaf(w, gre[k], swr);
-                    swi = fmaf(w, gim[k], swi);
-                    // the exact adjoint has no gradient through the clamped alpha
-                    const float da = (MODE == GRAD_EXACT && oG > 0.99f) ? 0.0f : dalpha;
-                    const float D = oG * da;
-                    const float h = D * dx, q = D * dy;
-                    sh += h;
-                    sk += q;
-                    sxx = fmaf(h, dx, sxx);
-                    sxy = fmaf(h, dy, sxy);
-                    syy = fmaf(q, dy, syy);
-                    sg = fmaf(G, da, sg);
-                    if (COUNT) ++contrib;
+                if ((ballot(act) & ballot(nonpos) & ballot(reach)) == 0) continue;
+                if (k == 0) {
+                    bwd_step<MODE, true>(c, power, dx, dyr[g], B.y, S, T[k], acc_re[k], acc_im[k], gre[k], gim[k], s);
+                } else {
+                    if (!have) s = BwdSums{};
+                    bwd_step<MODE, false>(c, power, dx, dyr[g], B.y, S, T[k], acc_re[k], acc_im[k], gre[k], gim[k], s);
                 }
+                have = true;
+                if (COUNT) contrib += c ? 1u : 0u;
             }
-            if (anyc == 0) continue;
+            if (!have) continue;
             // dL/dmean2D = D (conic . d) with the conic (qa, qb; qb, qc) and
             // qa = -2 A.z, qb = -A.w, qc = -2 B.x (backward.cu:600-605)
-            const float2 Q2 = rec[j].Q2;
             float v[8];
-            v[0] = fmaf(A.w, sk, Q2.x * sh);
-            v[1] = fmaf(A.w, sh, Q2.y * sk);
-            v[2] = sxx;
-            v[3] = sxy;
-            v[4] = syy;
-            v[5] = sg;
-            v[6] = swr;
-            v[7] = swi;
-            const float tot = wave_sum8(v, lane) * post;
-            sh = sk = sxx = sxy = syy = sg = swr = swi = 0.0f;
+            v[0] = fmaf(A.w, s.sk, C.z * s.sh);
+            v[1] = fmaf(A.w, s.sh, C.w * s.sk);
+            v[2] = s.sxx;
+            v[3] = s.sxy;
+            v[4] = s.syy;
+            v[5] = s.sg;
+            v[6] = s.swr;
+            v[7] = s.swi;
+            float tot = wave_sum8(v, lane) * post;
+            // completed on all lanes, so that the last DPP step is one add
+            asm volatile("" : "+v"(tot));
             // the record index is wave-uniform: its address arithmetic stays scalar
             const uint32_t gid = __builtin_amdgcn_readfirstlane(__float_as_uint(B.w));
-            float* rp = a.g8 + (rP + gid) * 8;
+            float* rp = g8r + (size_t)gid * 8;
             if ((lane & 7) == 0) atomicAdd(rp + (lane >> 3), tot);
         }
         __syncthreads();
