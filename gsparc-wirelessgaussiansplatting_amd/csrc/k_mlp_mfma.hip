@@ -31,6 +31,7 @@
 // half-wave hit 32 different banks in every operand read.
 #include <cmath>
 
+#include "gs_blend.h"
 #include "gs_device.h"
 #include "gs_kernels.h"
 
@@ -42,6 +43,7 @@ namespace gs {
 #define M1_PE 63
 #define M1_FEAT 5
 #define M1_KS 65     // LDS row stride of 64-wide (PE) tiles
+#define M1_CH 9      // chain-rule terms per row (rx_features_ch)
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
@@ -51,6 +53,65 @@ __device__ __forceinline__ floatx16 mfma32x32x2(float a, float b, floatx16 c)
 }
 
 __device__ __forceinline__ int acc_row(int v, int lane) { return 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3); }
+
+// Sums each of p[0..15] over the 32 lanes of each half-wave; lane l returns
+// the total of p[(l >> 1) & 15] of its half (lanes l and l ^ 1 both hold it).
+// Transpose reduction: the exchanges across 16 lanes (lane swap), 8 lanes
+// (row_ror:8), the half-row mirror and quad_perm [2,3,0,1] each halve the
+// values a lane carries (8, 4, 2, 1), each pairing lanes that carry the same
+// values; quad_perm [1,0,3,2] adds the last pair.
+__device__ __forceinline__ float half_sum16(const float (&p)[16], int lane)
+{
+    const bool b3 = lane & 8, b2 = lane & 4, b1 = lane & 2;
+    float w8[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) w8[i] = swap_sum16(p[i], p[i + 8]);   // bit 4 clear: p[i]; set: p[i + 8]
+    float x4[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) x4[i] = (b3 ? w8[i + 4] : w8[i]) + dpp_move<0x128>(b3 ? w8[i] : w8[i + 4]);
+    float y2[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) y2[j] = (b2 ? x4[j + 2] : x4[j]) + dpp_move<0x141>(b2 ? x4[j] : x4[j + 2]);
+    float z = (b1 ? y2[1] : y2[0]) + dpp_move<0x4e>(b1 ? y2[0] : y2[1]);
+    return z + dpp_move<0xb1>(z);
+}
+
+// acc[t] += sum over NSTEP steps of mfma(A(step), B(step, t)), with the
+// operands read in blocks of KB steps: the next block is read (from LDS)
+// while this block's products run; the scheduling barriers keep the reads
+// ahead of the products.
+template <int NACC, int NSTEP, int KB, class FA, class FB>
+__device__ __forceinline__ void mfma_steps(floatx16 (&acc)[NACC], FA a_of, FB b_of)
+{
+    static_assert(NSTEP % KB == 0, "whole blocks");
+    constexpr int NB = NSTEP / KB;
+    float ab[2][KB], bb[2][KB][NACC];
+#pragma unroll
+    for (int i = 0; i < KB; ++i) {
+        ab[0][i] = a_of(i);
+#pragma unroll
+        for (int t = 0; t < NACC; ++t) bb[0][i][t] = b_of(i, t);
+    }
+#pragma unroll
+    for (int kb = 0; kb < NB; ++kb) {
+        const int cur = kb & 1, nxt = cur ^ 1;
+        if (kb + 1 < NB) {
+#pragma unroll
+            for (int i = 0; i < KB; ++i) {
+                const int st = (kb + 1) * KB + i;
+                ab[nxt][i] = a_of(st);
+#pragma unroll
+                for (int t = 0; t < NACC; ++t) bb[nxt][i][t] = b_of(st, t);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < KB; ++i)
+#pragma unroll
+            for (int t = 0; t < NACC; ++t) acc[t] = mfma32x32x2(ab[cur][i], bb[cur][i][t], acc[t]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
 
 // W2 [n][k] (global, row-major) -> w2t[k * M1_S + n]
 __device__ __forceinline__ void load_w2t(const float* w2, float* w2t)
@@ -74,10 +135,10 @@ __device__ __forceinline__ float pe_component(const float* mu, int c)
 }
 
 // Receiver features of row (g, r).
-__device__ __forceinline__ void rx_features(const Mlp1Args& a, int g, int r, float* f)
+__device__ __forceinline__ void rx_features(const Mlp1Args& a, const float* mu, const float* rxv, float* f)
 {
-    const float mx = a.means[3 * (size_t)g], my = a.means[3 * (size_t)g + 1], mz = a.means[3 * (size_t)g + 2];
-    const float dx = a.rx[3 * r] - mx, dy = a.rx[3 * r + 1] - my, dz = a.rx[3 * r + 2] - mz;
+    const float mx = mu[0], my = mu[1], mz = mu[2];
+    const float dx = rxv[0] - mx, dy = rxv[1] - my, dz = rxv[2] - mz;
     const float d2 = dx * dx + dy * dy + dz * dz;
     f[0] = sqrtf(d2);
     f[1] = 1.0f / fmaxf(d2, 0.04f);
@@ -86,6 +147,34 @@ __device__ __forceinline__ void rx_features(const Mlp1Args& a, int g, int r, flo
     f[2] = tx / tn;
     f[3] = ty / tn;
     f[4] = tz / tn;
+}
+
+// The receiver-feature chain rule's per-row terms, from the same
+// intermediate values as rx_features: ch = [rx - mu (3), 1/d (0 at d = 0),
+// 2/d2^2 (0 where the 1/max(d2, 0.04) feature is clamped), 1/|tx - mu|,
+// the unit TX direction (3)].
+__device__ __forceinline__ void rx_features_ch(const Mlp1Args& a, const float* mu, const float* rxv, float* f, float* ch)
+{
+    const float mx = mu[0], my = mu[1], mz = mu[2];
+    const float dx = rxv[0] - mx, dy = rxv[1] - my, dz = rxv[2] - mz;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    const float d = sqrtf(d2);
+    f[0] = d;
+    f[1] = 1.0f / fmaxf(d2, 0.04f);
+    const float tx = a.tx[0] - mx, ty = a.tx[1] - my, tz = a.tx[2] - mz;
+    const float tn = sqrtf(tx * tx + ty * ty + tz * tz);
+    f[2] = tx / tn;
+    f[3] = ty / tn;
+    f[4] = tz / tn;
+    ch[0] = dx;
+    ch[1] = dy;
+    ch[2] = dz;
+    ch[3] = d > 0.0f ? 1.0f / d : 0.0f;
+    ch[4] = d2 > 0.04f ? 2.0f / (d2 * d2) : 0.0f;
+    ch[5] = 1.0f / tn;
+    ch[6] = f[2];
+    ch[7] = f[3];
+    ch[8] = f[4];
 }
 
 // ------------------------------------------------------------ PE layer
@@ -134,36 +223,147 @@ __global__ void __launch_bounds__(256) k_mlp1_pe(Mlp1Args a)
     }
 }
 
-// h1 of `rows` rows starting at q0 into h1s[m * M1_S + u] (and, when fs is
-// given, the rows' features into fs[m * M1_FEAT + i]); the block's threads
-// share the work (thread t: row t % rows, a slice of the units).
-__device__ __forceinline__ void layer1(const Mlp1Args& a, long long q0, int rows, const float* w1rx_s, float* h1s,
-                                       float* fs)
+// Layer 1 of ROWS rows starting at q0, in two steps with a barrier between.
+// l1_rows: thread t < ROWS computes row t's receiver features into
+// fs[t * M1_FEAT + i] and, when chs is given, its chain-rule terms into
+// chs[t * M1_CH + i] (zeros past the last row).
+// l1_units: thread t takes the 8 units 8 (t >> 4) .. + 8, whose W1rx
+// weights it holds in registers for the whole kernel, and the rows
+// (t & 15) + 16 j: h1 = relu(pe1 + W1rx f) into h1s[m * M1_S + u].  The
+// rows' pe1 slices are read as float4s (pe1 rows are 512-byte aligned),
+// all before the first use.
+template <int ROWS>
+__device__ __forceinline__ void l1_rows(const Mlp1Args& a, long long q0, float* fs, float* chs)
 {
-    const int n_rows = a.P * a.R;
-    const int tid = threadIdx.x;
-    const int m = tid % rows, part = tid / rows, nparts = blockDim.x / rows;
-    const int per = M1_H / nparts, u0 = part * per;
-    const int q = (int)q0 + m;
+    const int m = threadIdx.x;
+    if (m >= ROWS) return;
+    const long long q = q0 + m;
+    float f[M1_FEAT] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    float ch[M1_CH] = {};
+    if (q < (long long)a.P * a.R) {
+        const int g = (int)q / a.R, r = (int)q - g * a.R;
+        float mu[3], rxv[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            mu[i] = a.means[3 * (size_t)g + i];
+            rxv[i] = a.rx[3 * r + i];
+        }
+        if (chs) rx_features_ch(a, mu, rxv, f, ch);
+        else rx_features(a, mu, rxv, f);
+    }
+#pragma unroll
+    for (int i = 0; i < M1_FEAT; ++i) fs[m * M1_FEAT + i] = f[i];
+    if (chs)
+#pragma unroll
+        for (int i = 0; i < M1_CH; ++i) chs[m * M1_CH + i] = ch[i];
+}
+
+// this thread's 8 units' W1rx rows
+__device__ __forceinline__ void l1_weights(const Mlp1Args& a, float (&wr)[8][M1_FEAT])
+{
+    const int u0 = 8 * (threadIdx.x >> 4);
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+#pragma unroll
+        for (int i = 0; i < M1_FEAT; ++i) wr[e][i] = a.w1[(u0 + e) * M1_IN + M1_PE + i];
+}
+
+template <int ROWS>
+__device__ __forceinline__ void l1_units(const Mlp1Args& a, long long q0, const float (&wr)[8][M1_FEAT],
+                                         const float* fs, float* h1s)
+{
+    constexpr int RPT = ROWS / 16;   // rows per thread
+    const int rb = threadIdx.x & 15, u0 = 8 * (threadIdx.x >> 4);
+    const long long n_rows = (long long)a.P * a.R;
+    float4 pv[RPT][2];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+        const long long q = q0 + rb + 16 * j;
+        const int g = q < n_rows ? (int)q / a.R : 0;
+        const float4* pe4 = reinterpret_cast<const float4*>(a.pe1 + (size_t)g * M1_H + u0);
+        pv[j][0] = pe4[0];
+        pv[j][1] = pe4[1];
+    }
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+        const int m = rb + 16 * j;
+        const bool valid = q0 + m < n_rows;
+        float f[M1_FEAT];
+#pragma unroll
+        for (int i = 0; i < M1_FEAT; ++i) f[i] = fs[m * M1_FEAT + i];
+        const float pj[8] = {pv[j][0].x, pv[j][0].y, pv[j][0].z, pv[j][0].w,
+                             pv[j][1].x, pv[j][1].y, pv[j][1].z, pv[j][1].w};
+        float* dst = h1s + m * M1_S + u0;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            float v = pj[e];
+#pragma unroll
+            for (int i = 0; i < M1_FEAT; ++i) v = fmaf(wr[e][i], f[i], v);
+            dst[e] = valid ? fmaxf(v, 0.0f) : 0.0f;
+        }
+    }
+}
+
+// The backward's layer 1 (64-row tiles): thread t takes row t % 64 and the
+// 32 units 32 (t / 64) .. + 32 (W1rx from LDS), computing the row's features
+// itself, so that no thread waits for another's.  layer1_load reads the
+// row's pe1 slice (float4s), mean and receiver, a tile ahead; layer1_store
+// writes h1 into h1s[m * M1_S + u], the features into fs[m * M1_FEAT + i]
+// and the chain-rule terms into chs[m * M1_CH + i].
+struct L1In {
+    bool valid;
+    float4 pv[8];
+    float mu[3], rxv[3];
+};
+
+__device__ __forceinline__ void layer1_load(const Mlp1Args& a, long long q0, L1In& in)
+{
+    const int m = threadIdx.x & 63, part = threadIdx.x >> 6;
+    const long long q = q0 + m;
+    in.valid = q < (long long)a.P * a.R;
+    const int qq = in.valid ? (int)q : 0;
+    const int g = qq / a.R, r = qq - g * a.R;
+    const float4* pe4 = reinterpret_cast<const float4*>(a.pe1 + (size_t)g * M1_H + 32 * part);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) in.pv[j] = pe4[j];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        in.mu[i] = a.means[3 * (size_t)g + i];
+        in.rxv[i] = a.rx[3 * r + i];
+    }
+}
+
+__device__ __forceinline__ void layer1_store(const Mlp1Args& a, const L1In& in, const float* w1rx_s, float* h1s,
+                                             float* fs, float* chs)
+{
+    const int m = threadIdx.x & 63, part = threadIdx.x >> 6;
+    const int u0 = 32 * part;
     float* dst = h1s + m * M1_S;
-    if (q >= n_rows) {
-        for (int u = u0; u < u0 + per; ++u) dst[u] = 0.0f;
-        if (fs && part == 0)
+    if (!in.valid) {
+#pragma unroll
+        for (int u = 0; u < 32; ++u) dst[u0 + u] = 0.0f;
+        if (part == 0)
             for (int i = 0; i < M1_FEAT; ++i) fs[m * M1_FEAT + i] = 0.0f;
         return;
     }
-    const int g = q / a.R, r = q - g * a.R;
-    float f[M1_FEAT];
-    rx_features(a, g, r, f);
-    if (fs && part == 0)
+    float f[M1_FEAT], ch[M1_CH];
+    rx_features_ch(a, in.mu, in.rxv, f, ch);
+    if (part == 0) {
+        for (int i = 0; i < M1_CH; ++i) chs[m * M1_CH + i] = ch[i];
         for (int i = 0; i < M1_FEAT; ++i) fs[m * M1_FEAT + i] = f[i];
-    const float* pe = a.pe1 + (size_t)g * M1_H;
-    for (int u = u0; u < u0 + per; ++u) {
-        const float* wr = w1rx_s + u * M1_FEAT;
-        float v = pe[u];
+    }
 #pragma unroll
-        for (int i = 0; i < M1_FEAT; ++i) v = fmaf(wr[i], f[i], v);
-        dst[u] = fmaxf(v, 0.0f);
+    for (int j = 0; j < 8; ++j) {
+        const float pj[4] = {in.pv[j].x, in.pv[j].y, in.pv[j].z, in.pv[j].w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int u = u0 + 4 * j + e;
+            const float* wr = w1rx_s + u * M1_FEAT;
+            float v = pj[e];
+#pragma unroll
+            for (int i = 0; i < M1_FEAT; ++i) v = fmaf(wr[i], f[i], v);
+            dst[u] = fmaxf(v, 0.0f);
+        }
     }
 }
 
@@ -179,27 +379,28 @@ __global__ void __launch_bounds__(256) k_mlp1_fwd(Mlp1Args a)
 {
     __shared__ float w2t[M1_H * M1_S];
     __shared__ float h1s[128 * M1_S];
-    __shared__ float w1rx_s[M1_H * M1_FEAT];
+    __shared__ float fs[128 * M1_FEAT];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const long long n_rows = (long long)a.P * a.R;
     load_w2t(a.w2, w2t);   // once per workgroup: the grid strides over 128-row tiles
-    load_w1rx(a.w1, w1rx_s);
+    float wr[8][M1_FEAT];
+    l1_weights(a, wr);
     const float b3 = a.b3[0];
     const float* hrow = h1s + (32 * w + (lane & 31)) * M1_S + (lane >> 5);
     const float* bcol = w2t + (lane >> 5) * M1_S + (lane & 31);
     for (long long tile = blockIdx.x; tile * 128 < n_rows; tile += gridDim.x) {
         const long long q0 = tile * 128;
-        __syncthreads();   // weights loaded / the previous tile's h1s reads done
-        layer1(a, q0, 128, w1rx_s, h1s, nullptr);
+        l1_rows<128>(a, q0, fs, nullptr);
+        __syncthreads();   // weights loaded, the features written / the previous tile's h1s reads done
+        l1_units<128>(a, q0, wr, fs, h1s);
         __syncthreads();
         floatx16 acc[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) acc[t] = floatx16{};
-        for (int ks = 0; ks < M1_H / 2; ++ks) {
-            const float av = hrow[2 * ks];
-#pragma unroll
-            for (int t = 0; t < 4; ++t) acc[t] = mfma32x32x2(av, bcol[2 * ks * M1_S + 32 * t], acc[t]);
-        }
+        mfma_steps<4, M1_H / 2, 4>(
+            acc, [&](int ks) { return hrow[2 * ks]; }, [&](int ks, int t) { return bcol[2 * ks * M1_S + 32 * t]; });
+        // a = w3 . relu(h2 + b2): each lane's part over its 4 columns, then
+        // the sum over the 32 lanes of each half-wave (column l % 32)
         float part[16];
 #pragma unroll
         for (int v = 0; v < 16; ++v) part[v] = 0.0f;
@@ -210,21 +411,14 @@ __global__ void __launch_bounds__(256) k_mlp1_fwd(Mlp1Args a)
 #pragma unroll
             for (int v = 0; v < 16; ++v) part[v] = fmaf(ww, fmaxf(acc[t][v] + bb, 0.0f), part[v]);
         }
-#pragma unroll
-        for (int v = 0; v < 16; ++v)
-#pragma unroll
-            for (int off = 1; off < 32; off <<= 1) part[v] += __shfl_xor(part[v], off);
-        if ((lane & 31) == 0) {
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const int q = (int)q0 + 32 * w + acc_row(v, lane);
-                if (q < n_rows) {
-                    const int g = q / a.R, r = q - g * a.R;
-                    const size_t o = (size_t)r * a.P + g;
-                    a.out[2 * o] = part[v] + b3;
-                    a.out[2 * o + 1] = 0.0f;
-                }
-            }
+        const float tot = half_sum16(part, lane);
+        const int v = (lane >> 1) & 15;
+        const int q = (int)q0 + 32 * w + acc_row(v, lane);
+        if ((lane & 1) == 0 && q < n_rows) {
+            const int g = q / a.R, r = q - g * a.R;
+            const size_t o = (size_t)r * a.P + g;
+            a.out[2 * o] = tot + b3;
+            a.out[2 * o + 1] = 0.0f;
         }
     }
 }
@@ -240,6 +434,7 @@ __global__ void __launch_bounds__(256) k_mlp1_bwd(Mlp1Args a, const float* dem, 
     __shared__ float g2s[64 * M1_S];
     __shared__ float w1rx_s[M1_H * M1_FEAT];
     __shared__ float fs[64 * M1_FEAT];
+    __shared__ float chs[2][64 * M1_CH];   // per tile parity: the next tile's layer 1 may run beside this tile's chain rule
     __shared__ float red[4][2][2][32];
     __shared__ float red3[256];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -258,24 +453,41 @@ __global__ void __launch_bounds__(256) k_mlp1_bwd(Mlp1Args a, const float* dem, 
         const int g = (int)q / a.R, r = (int)q - g * a.R;
         return dem[2 * ((size_t)r * a.P + g)];
     };
-    for (long long tile = blockIdx.x; tile * 64 < n_rows; tile += gridDim.x) {
+    __syncthreads();   // the weights are loaded
+    int par = 0;
+    L1In in;
+    float dan = 0.0f;   // da of row q0 + threadIdx.x (wave 0), read a tile ahead
+    if ((long long)blockIdx.x * 64 < n_rows) {
+        layer1_load(a, (long long)blockIdx.x * 64, in);
+        if (threadIdx.x < 64) dan = da_of((long long)blockIdx.x * 64 + threadIdx.x);
+    }
+    for (long long tile = blockIdx.x; tile * 64 < n_rows; tile += gridDim.x, par ^= 1) {
         const long long q0 = tile * 64;
-        __syncthreads();   // the previous tile's LDS reads are done (and the weights are loaded)
-        layer1(a, q0, 64, w1rx_s, h1s, fs);
-        if (threadIdx.x < 64) db3p += da_of(q0 + threadIdx.x);
+        // No barrier here: every read of h1s and fs of the previous tile
+        // came before its last barrier, and its chain rule (wave 0, after
+        // that barrier) reads only g2s and chs[par ^ 1], which this layer 1
+        // does not write (g2s is next written after this tile's first barrier).
+        layer1_store(a, in, w1rx_s, h1s, fs, chs[par]);
+        db3p += dan;
         __syncthreads();
+        // the rows' da for this tile's epilogue, and the next tile's global
+        // reads, issued under this tile's products
+        float dav[16];
+#pragma unroll
+        for (int v = 0; v < 16; ++v) dav[v] = da_of(q0 + 32 * mh + acc_row(v, lane));
+        {
+            const long long nt = tile + gridDim.x;
+            if (nt * 64 < n_rows) {
+                layer1_load(a, nt * 64, in);
+                if (threadIdx.x < 64) dan = da_of(nt * 64 + threadIdx.x);
+            }
+        }
         // h2 for rows 32*mh.., columns of tiles 2*nh, 2*nh + 1
         floatx16 c[2] = {floatx16{}, floatx16{}};
         const float* hrow = h1s + (32 * mh + (lane & 31)) * M1_S + (lane >> 5);
         const float* bcol = w2t + (lane >> 5) * M1_S + 64 * nh + (lane & 31);
-        for (int ks = 0; ks < M1_H / 2; ++ks) {
-            const float av = hrow[2 * ks];
-            c[0] = mfma32x32x2(av, bcol[2 * ks * M1_S], c[0]);
-            c[1] = mfma32x32x2(av, bcol[2 * ks * M1_S + 32], c[1]);
-        }
-        float dav[16];
-#pragma unroll
-        for (int v = 0; v < 16; ++v) dav[v] = da_of(q0 + 32 * mh + acc_row(v, lane));
+        mfma_steps<2, M1_H / 2, 8>(
+            c, [&](int ks) { return hrow[2 * ks]; }, [&](int ks, int t) { return bcol[2 * ks * M1_S + 32 * t]; });
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
             const int nn = 64 * nh + 32 * t + (lane & 31);
@@ -295,22 +507,16 @@ __global__ void __launch_bounds__(256) k_mlp1_bwd(Mlp1Args a, const float* dem, 
         {
             const float* ga = g2s + (lane >> 5) * M1_S + 32 * w + (lane & 31);
             const float* hb = h1s + (lane >> 5) * M1_S + (lane & 31);
-            for (int ms = 0; ms < 32; ++ms) {
-                const float av = ga[2 * ms * M1_S];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) dw2[u] = mfma32x32x2(av, hb[2 * ms * M1_S + 32 * u], dw2[u]);
-            }
+            mfma_steps<4, 32, 4>(
+                dw2, [&](int ms) { return ga[2 * ms * M1_S]; }, [&](int ms, int u) { return hb[2 * ms * M1_S + 32 * u]; });
         }
         // dh1[m][k] = (sum_n g2[m][n] W2[n][k]) (h1 > 0): rows 32*mh.., column tiles 2*nh, 2*nh + 1
         floatx16 e[2] = {floatx16{}, floatx16{}};
         {
             const float* grow = g2s + (32 * mh + (lane & 31)) * M1_S + (lane >> 5);
             const float* wk = w2t + (64 * nh + (lane & 31)) * M1_S + (lane >> 5);
-            for (int ns = 0; ns < M1_H / 2; ++ns) {
-                const float av = grow[2 * ns];
-                e[0] = mfma32x32x2(av, wk[2 * ns], e[0]);
-                e[1] = mfma32x32x2(av, wk[32 * M1_S + 2 * ns], e[1]);
-            }
+            mfma_steps<2, M1_H / 2, 8>(
+                e, [&](int ns) { return grow[2 * ns]; }, [&](int ns, int t) { return wk[32 * t * M1_S + 2 * ns]; });
         }
         __syncthreads();   // every wave's dW2 reads of h1s are done: dh1 replaces h1 in place
 #pragma unroll
@@ -330,18 +536,24 @@ __global__ void __launch_bounds__(256) k_mlp1_bwd(Mlp1Args a, const float* dem, 
         const int qt = (int)q0, g0 = qt / a.R, r0 = qt - g0 * a.R;
         {
             const int u = threadIdx.x & (M1_H - 1), half = threadIdx.x >> 7;
-            int g = g0, r = r0;
-            for (int k = 0; k < 32 * half; ++k)
-                if (++r == a.R) { r = 0; ++g; }
+            const int rs = r0 + 32 * half;
+            int g = g0 + rs / a.R, r = rs % a.R;
             float run = 0.0f;
             int gprev = g;
+            // the 32 rows' values are read before the (sequential) sums
+            float dhv[32];
+#pragma unroll
+            for (int mm = 0; mm < 32; ++mm) dhv[mm] = h1s[(32 * half + mm) * M1_S + u];
+            // the valid rows are a prefix of the tile: past it dh = 0 and no
+            // Gaussian's run is flushed (the last one is, after the loop)
+#pragma unroll
             for (int mm = 0; mm < 32; ++mm) {
                 const int m = 32 * half + mm;
-                if (qt + m >= n_rows) break;
-                const float dh = h1s[m * M1_S + u];
+                const bool valid = qt + m < n_rows;
+                const float dh = valid ? dhv[mm] : 0.0f;
 #pragma unroll
                 for (int i = 0; i < M1_FEAT; ++i) dw1rx[i] = fmaf(dh, fs[m * M1_FEAT + i], dw1rx[i]);
-                if (g != gprev) {
+                if (valid && g != gprev) {
                     atomicAdd(dpe1 + (size_t)gprev * M1_H + u, run);
                     run = 0.0f;
                     gprev = g;
@@ -355,6 +567,7 @@ __global__ void __launch_bounds__(256) k_mlp1_bwd(Mlp1Args a, const float* dem, 
             // drxf = dh1 W1rx: wave qu sums units 32 qu.. for all 64 rows (lane = row)
             const int m = threadIdx.x & 63, qu = threadIdx.x >> 6;
             float df[M1_FEAT] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 8
             for (int u = 32 * qu; u < 32 * qu + 32; ++u) {
                 const float dh = h1s[m * M1_S + u];
 #pragma unroll
@@ -378,22 +591,19 @@ __global__ void __launch_bounds__(256) k_mlp1_bwd(Mlp1Args a, const float* dem, 
             const bool valid = qt + m < n_rows;
             float gm[3] = {0.0f, 0.0f, 0.0f};
             if (valid) {
-                const float mx = a.means[3 * (size_t)g], my = a.means[3 * (size_t)g + 1];
-                const float mz = a.means[3 * (size_t)g + 2];
-                const float dv[3] = {a.rx[3 * r] - mx, a.rx[3 * r + 1] - my, a.rx[3 * r + 2] - mz};
-                const float d2 = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
-                const float d = sqrtf(d2);
+                // the row's terms as layer 1 computed them
+                const float* ch = chs[par] + m * M1_CH;
+                const float dv[3] = {ch[0], ch[1], ch[2]};
                 // d(d)/dmu = -diff / d; d(1/d2)/dmu = 2 diff / d2^2 where d2 > 0.04
-                const float cd = d > 0.0f ? -df[0] / d : 0.0f;
-                const float ci = d2 > 0.04f ? df[1] * 2.0f / (d2 * d2) : 0.0f;
-                const float t[3] = {a.tx[0] - mx, a.tx[1] - my, a.tx[2] - mz};
-                const float tn = sqrtf(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
-                const float u3[3] = {t[0] / tn, t[1] / tn, t[2] / tn};
+                const float cd = -df[0] * ch[3];
+                const float ci = df[1] * ch[4];
+                const float rtn = ch[5];
+                const float u3[3] = {ch[6], ch[7], ch[8]};
                 const float ug = u3[0] * df[2] + u3[1] * df[3] + u3[2] * df[4];
 #pragma unroll
                 for (int j = 0; j < 3; ++j) {
                     // d(t/|t|)/dmu = -(I - u u^T) / |t|
-                    const float gt = -(df[2 + j] - u3[j] * ug) / tn;
+                    const float gt = -(df[2 + j] - u3[j] * ug) * rtn;
                     gm[j] = (cd + ci) * dv[j] + gt;
                 }
             }
