@@ -314,9 +314,10 @@ struct L1In {
     bool valid;
     float4 pv[8];
     float mu[3], rxv[3];
+    float da;   // dL/da of the row (the rasterizer's emission gradient, real part)
 };
 
-__device__ __forceinline__ void layer1_load(const Mlp1Args& a, long long q0, L1In& in)
+__device__ __forceinline__ void layer1_load(const Mlp1Args& a, const float* dem, long long q0, L1In& in)
 {
     const int m = threadIdx.x & 63, part = threadIdx.x >> 6;
     const long long q = q0 + m;
@@ -331,14 +332,16 @@ __device__ __forceinline__ void layer1_load(const Mlp1Args& a, long long q0, L1I
         in.mu[i] = a.means[3 * (size_t)g + i];
         in.rxv[i] = a.rx[3 * r + i];
     }
+    in.da = in.valid ? dem[2 * ((size_t)r * a.P + g)] : 0.0f;
 }
 
 __device__ __forceinline__ void layer1_store(const Mlp1Args& a, const L1In& in, const float* w1rx_s, float* h1s,
-                                             float* fs, float* chs)
+                                             float* fs, float* chs, float* das)
 {
     const int m = threadIdx.x & 63, part = threadIdx.x >> 6;
     const int u0 = 32 * part;
     float* dst = h1s + m * M1_S;
+    if (part == 0) das[m] = in.da;
     if (!in.valid) {
 #pragma unroll
         for (int u = 0; u < 32; ++u) dst[u0 + u] = 0.0f;
@@ -434,7 +437,8 @@ __global__ void __launch_bounds__(256) k_mlp1_bwd(Mlp1Args a, const float* dem, 
     __shared__ float g2s[64 * M1_S];
     __shared__ float w1rx_s[M1_H * M1_FEAT];
     __shared__ float fs[64 * M1_FEAT];
-    __shared__ float chs[2][64 * M1_CH];   // per tile parity: the next tile's layer 1 may run beside this tile's chain rule
+    __shared__ float chs[2][64 * M1_CH];
+    __shared__ float das[64];              // dL/da of the tile's rows   // per tile parity: the next tile's layer 1 may run beside this tile's chain rule
     __shared__ float red[4][2][2][32];
     __shared__ float red3[256];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -447,41 +451,22 @@ __global__ void __launch_bounds__(256) k_mlp1_bwd(Mlp1Args a, const float* dem, 
     for (int u = 0; u < 4; ++u) dw2[u] = floatx16{};
     float dw3p[2] = {0.0f, 0.0f}, db2p[2] = {0.0f, 0.0f}, db3p = 0.0f;
     float dw1rx[M1_FEAT] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    // da of row q (Gaussian-major) in the rasterizer's layout
-    auto da_of = [&](long long q) -> float {
-        if (q >= n_rows) return 0.0f;
-        const int g = (int)q / a.R, r = (int)q - g * a.R;
-        return dem[2 * ((size_t)r * a.P + g)];
-    };
     __syncthreads();   // the weights are loaded
     int par = 0;
     L1In in;
-    float dan = 0.0f;   // da of row q0 + threadIdx.x (wave 0), read a tile ahead
-    if ((long long)blockIdx.x * 64 < n_rows) {
-        layer1_load(a, (long long)blockIdx.x * 64, in);
-        if (threadIdx.x < 64) dan = da_of((long long)blockIdx.x * 64 + threadIdx.x);
-    }
+    if ((long long)blockIdx.x * 64 < n_rows) layer1_load(a, dem, (long long)blockIdx.x * 64, in);
     for (long long tile = blockIdx.x; tile * 64 < n_rows; tile += gridDim.x, par ^= 1) {
         const long long q0 = tile * 64;
         // No barrier here: every read of h1s and fs of the previous tile
         // came before its last barrier, and its chain rule (wave 0, after
         // that barrier) reads only g2s and chs[par ^ 1], which this layer 1
         // does not write (g2s is next written after this tile's first barrier).
-        layer1_store(a, in, w1rx_s, h1s, fs, chs[par]);
-        db3p += dan;
+        layer1_store(a, in, w1rx_s, h1s, fs, chs[par], das);
+        if (threadIdx.x < 64) db3p += in.da;
         __syncthreads();
-        // the rows' da for this tile's epilogue, and the next tile's global
-        // reads, issued under this tile's products
         float dav[16];
 #pragma unroll
-        for (int v = 0; v < 16; ++v) dav[v] = da_of(q0 + 32 * mh + acc_row(v, lane));
-        {
-            const long long nt = tile + gridDim.x;
-            if (nt * 64 < n_rows) {
-                layer1_load(a, nt * 64, in);
-                if (threadIdx.x < 64) dan = da_of(nt * 64 + threadIdx.x);
-            }
-        }
+        for (int v = 0; v < 16; ++v) dav[v] = das[32 * mh + acc_row(v, lane)];
         // h2 for rows 32*mh.., columns of tiles 2*nh, 2*nh + 1
         floatx16 c[2] = {floatx16{}, floatx16{}};
         const float* hrow = h1s + (32 * mh + (lane & 31)) * M1_S + (lane >> 5);
@@ -517,6 +502,11 @@ __global__ void __launch_bounds__(256) k_mlp1_bwd(Mlp1Args a, const float* dem, 
             const float* wk = w2t + (64 * nh + (lane & 31)) * M1_S + (lane >> 5);
             mfma_steps<2, M1_H / 2, 8>(
                 e, [&](int ns) { return grow[2 * ns]; }, [&](int ns, int t) { return wk[32 * t * M1_S + 2 * ns]; });
+        }
+        // the next tile's global reads, issued under the rest of this tile
+        {
+            const long long nt = tile + gridDim.x;
+            if (nt * 64 < n_rows) layer1_load(a, dem, nt * 64, in);
         }
         __syncthreads();   // every wave's dW2 reads of h1s are done: dh1 replaces h1 in place
 #pragma unroll
