@@ -681,7 +681,9 @@ def main():
         # + 56 B per (Gaussian, receiver): splat record 32, emission 8,
         # radius 4, depth 4, path length 4, packed tile rect 4
         "preprocess_fwd": ("hbm", Ptot * (820 + 24) + 56 * Nrec, "GB/s"),
-        "preprocess_bwd": ("hbm", Ptot * 1690 + 72 * Nrec, "GB/s"),
+        # + 84 B per (Gaussian, receiver): gradient record 32, conic 12,
+        # radius 4, dL/dmean2D out 12, MLP outputs 8, dL/dz 8, partials 8
+        "preprocess_bwd": ("hbm", Ptot * 1690 + 84 * Nrec, "GB/s"),
         # 4 passes of (key, index) pairs, plus the packed rects carried into
         # depth order by the last pass
         "bin_depth_sort": ("hbm", 92.0 * Nrec, "GB/s"),

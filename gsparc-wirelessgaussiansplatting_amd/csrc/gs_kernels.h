@@ -221,7 +221,11 @@ struct RenderBwdArgs {
     const uint32_t* ncontrib;
     const float* dout_re;    // [R*N]
     const float* dout_im;
-    float* g8;               // [Nrec*8] accumulated: dpx dpy dqa dqb dqc dop dre dim
+    // [Nrec*8] accumulated raw sums over the record's pixels, with
+    // D = o G dL/dalpha: D dx, D dy, D dx^2, D dx dy, D dy^2, G dL/dalpha,
+    // w dL/dre, w dL/dim (the preprocess backward turns the first five into
+    // dL/dmean2D and dL/dconic with the record's conic)
+    float* g8;
     int mode;
     const uint32_t* total;
     uint64_t Kcap;
@@ -240,6 +244,7 @@ struct PreBwdArgs {
     float mod;
     const int* radii;        // [Nrec]
     const float* g8;         // [Nrec*8]
+    const float4* splat;     // [Nrec*2] splat records of the forward (their conics)
     const float2* sig;       // [Nrec] MLP outputs of the forward (used when mlp != nullptr)
     const float* cov3;       // [P*6] cov3D of the forward
     float* dmeans;           // [P,3]

@@ -502,6 +502,7 @@ int backward_impl(int grad_mode, const gs::Dims& d, const FwdInputs& in, const f
     gs::PreBwdArgs pa;
     pa.means = in.means; pa.scales = in.scales; pa.rots = in.rotations; pa.mlp = in.mlp; pa.views = in.views;
     pa.sig = (const float2*)(gb + L.sig); pa.cov3 = (const float*)(gb + L.cov3);
+    pa.splat = (const float4*)(gb + L.splat);
     pa.tx = in.tx; pa.rx = in.rx; pa.mod = in.mod; pa.radii = radii; pa.g8 = g8;
     pa.dmeans = out.dmeans3D; pa.dscales = out.dscales; pa.drots = out.drotations; pa.dopac = out.dopacity;
     pa.dmlp = out.dmlp; pa.dcov3 = out.dcov3D; pa.dmeans2d = out.dmeans2D; pa.dsig = out.demission;

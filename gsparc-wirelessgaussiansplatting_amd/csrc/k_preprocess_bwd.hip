@@ -224,9 +224,13 @@ __global__ void __launch_bounds__(GS_BWD_THREADS) k_pre_bwd_geo(Dims d, PreBwdAr
     // The records of receiver c + 1 are loaded (unconditionally: a culled
     // pair's record is read and discarded) while receiver c is computed.
     const float4* g8v = reinterpret_cast<const float4*>(a.g8);
+    // the conic (ha, nb, hc) of each record, from its splat record
+    const float* spf = reinterpret_cast<const float*>(a.splat);
     size_t ri_n = (size_t)s_roff[0] + g;
     int rad_n = a.radii[ri_n];
     float4 ga_n = g8v[2 * ri_n], gb_n = g8v[2 * ri_n + 1];
+    float2 cq_n = *reinterpret_cast<const float2*>(spf + 8 * ri_n + 2);
+    float ch_n = spf[8 * ri_n + 4];
     float2 sg_n = a.mlp ? a.sig[ri_n] : make_float2(0.0f, 0.0f);
 #pragma unroll
     for (int c = 0; c < GS_RC; ++c) {
@@ -235,17 +239,27 @@ __global__ void __launch_bounds__(GS_BWD_THREADS) k_pre_bwd_geo(Dims d, PreBwdAr
         const size_t ri = ri_n;
         const int rad = rad_n;
         const float4 ga = ga_n, gb = gb_n;
+        const float2 cq = cq_n;
+        const float ch = ch_n;
         const float2 sg = sg_n;
         if (c + 1 < nr) {
             ri_n = (size_t)s_roff[c + 1] + g;
             rad_n = a.radii[ri_n];
             ga_n = g8v[2 * ri_n];
             gb_n = g8v[2 * ri_n + 1];
+            cq_n = *reinterpret_cast<const float2*>(spf + 8 * ri_n + 2);
+            ch_n = spf[8 * ri_n + 4];
             if (a.mlp) sg_n = a.sig[ri_n];
         }
         float2 dz = make_float2(0.0f, 0.0f);
         if (rad > 0) {
-            const float gv[8] = {ga.x, ga.y, ga.z, ga.w, gb.x, gb.y, gb.z, gb.w};
+            // dL/dmean2D = D (conic . d) with the conic (qa, qb; qb, qc) =
+            // (-2 ha, -nb; -nb, -2 hc) (backward.cu:600-605), and dL/dconic
+            // = (-1/2 D dx^2, -D dx dy, -1/2 D dy^2) (backward.cu:608-610; the
+            // off-diagonal's 1/2 is applied below)
+            const float ha = cq.x, nb = cq.y, hc = ch;
+            const float gv[8] = {fmaf(nb, ga.y, (2.0f * ha) * ga.x), fmaf(nb, ga.x, (2.0f * hc) * ga.y),
+                                 -0.5f * ga.z, -ga.w, -0.5f * gb.x, gb.y, gb.z, gb.w};
             geo_bwd<MODE>(mean, c3, s_view + 16 * c, d.W, d.H, gv, dmean, dc);
             dop += gv[5];
             if (a.gL) {

@@ -148,7 +148,8 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
 struct __align__(16) BwdRec {
     float4 A;    // px, py, ha, nb
     float4 B;    // hc, opacity, thr, Gaussian index (bits)
-    float4 C;    // emission (real, imag); 2 ha, 2 hc: the conic factors of the 2D-mean gradient
+    float2 S;    // emission (real, imag)
+    float2 pad;
 };
 
 // Per-lane sums over the lane's pixels for the current entry, with
@@ -265,11 +266,6 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
     for (int k = 0; k < PPL; ++k) {
         acc_re[k] = 0.0f; acc_im[k] = 0.0f;
     }
-    // lanes 16, 24, 32 sum the conic terms -1/2 D dx^2, -D dx dy, -1/2 D dy^2
-    // (backward.cu:608-610; the off-diagonal's 1/2 is applied by the
-    // preprocess backward)
-    const int vi = lane >> 3;
-    const float post = (vi == 2 || vi == 4) ? -0.5f : (vi == 3 ? -1.0f : 1.0f);
     float* const g8r = a.g8 + rP * 8;
     // pixel row groups of the wave: quarters k and k ^ 1 share a row when PPL >= 2
     constexpr int NROW = PPL == 1 ? 1 : PPL / 2;
@@ -288,7 +284,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
             B.w = __uint_as_float(id);
             rec[lane].A = A;
             rec[lane].B = B;
-            rec[lane].C = make_float4(S.x, S.y, 2.0f * A.z, 2.0f * B.x);
+            rec[lane].S = S;
             // culled per 8x8 quarter: an entry is walked only over the
             // quarters where it can reach alpha >= 1/255 and where some
             // pixel's contributors reach back to it
@@ -310,8 +306,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
             mask &= mask - 1;
             const float4 A = rec[j].A;
             const float4 B = rec[j].B;
-            const float4 C = rec[j].C;
-            const float2 S = make_float2(C.x, C.y);
+            const float2 S = rec[j].S;
             const int pos = top - j;
             // quarters k and k ^ 1 share their pixel rows: the row terms of
             // pair_power are computed once per row
@@ -347,18 +342,19 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
                 if (COUNT) contrib += c ? 1u : 0u;
             }
             if (!have) continue;
-            // dL/dmean2D = D (conic . d) with the conic (qa, qb; qb, qc) and
-            // qa = -2 A.z, qb = -A.w, qc = -2 B.x (backward.cu:600-605)
+            // the entry's raw sums (g8 record layout): the preprocess backward
+            // applies the conic to the first two (dL/dmean2D, backward.cu:600-605)
+            // and the factors -1/2, -1, -1/2 to the next three (backward.cu:608-610)
             float v[8];
-            v[0] = fmaf(A.w, s.sk, C.z * s.sh);
-            v[1] = fmaf(A.w, s.sh, C.w * s.sk);
+            v[0] = s.sh;
+            v[1] = s.sk;
             v[2] = s.sxx;
             v[3] = s.sxy;
             v[4] = s.syy;
             v[5] = s.sg;
             v[6] = s.swr;
             v[7] = s.swi;
-            float tot = wave_sum8(v, lane) * post;
+            float tot = wave_sum8(v, lane);
             // completed on all lanes, so that the last DPP step is one add
             asm volatile("" : "+v"(tot));
             // the record index is wave-uniform: its address arithmetic stays scalar

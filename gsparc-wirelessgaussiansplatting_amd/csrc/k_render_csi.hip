@@ -417,7 +417,6 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd_csi(Dims d, RenderCsiArg
             const float4 A = s0[j];
             const float4 B = s1[j];
             const float2 S = ss[j];
-            const float qa = -2.0f * A.z, qb = -A.w, qc = -2.0f * B.x;
             float v[8] = {-0.0f, -0.0f, -0.0f, -0.0f, -0.0f, -0.0f, -0.0f, -0.0f};
             float vL = -0.0f;
             const bool active = (top - j) < last;
@@ -460,11 +459,13 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd_csi(Dims d, RenderCsiArg
                 if (!(oG > 0.99f)) {   // clamped alpha does not depend on G or the opacity
                     const float dG = B.y * dalpha;
                     const float gdx = G * dx, gdy = G * dy;
-                    v[0] = dG * (-gdx * qa - gdy * qb);
-                    v[1] = dG * (-gdy * qc - gdx * qb);
-                    v[2] = -0.5f * gdx * dx * dG;
-                    v[3] = -gdx * dy * dG;
-                    v[4] = -0.5f * gdy * dy * dG;
+                    // the raw sums of the g8 record (the preprocess backward
+                    // applies the conic and the factors -1/2, -1, -1/2)
+                    v[0] = dG * gdx;
+                    v[1] = dG * gdy;
+                    v[2] = gdx * dx * dG;
+                    v[3] = gdx * dy * dG;
+                    v[4] = gdy * dy * dG;
                     v[5] = G * dalpha;
                 }
             }
