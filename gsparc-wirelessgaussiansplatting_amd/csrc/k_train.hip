@@ -27,40 +27,45 @@ namespace gs {
 // gaussian.py:233-305, 339-393).
 __global__ void k_adam_prep(AdamCfg c, int groups, uint32_t* counters, AdamStep* out)
 {
-    if (threadIdx.x != 0) return;
+    // thread g < AG_N: group g (its FP64 powers and, for the centres, the
+    // schedule run in parallel); thread 0 then advances the iteration
+    const int g = threadIdx.x;
     const double it = (double)counters[0];
-    AdamStep s;
-    for (int g = 0; g < AG_N; ++g) {
-        s.step_size[g] = 0.0f;
-        s.bc2_sqrt[g] = 1.0f;
-        if (!((groups >> g) & 1)) continue;
-        const uint32_t t = counters[1 + g] + 1u;
-        counters[1 + g] = t;
-        double lr = c.lr[g];
-        if (g == AG_CENTRE) {
-            // get_expon_lr_func (utils.py:46-79) at the iteration number
-            if (it < 0.0 || (c.lr_init == 0.0 && c.lr_final == 0.0)) {
-                lr = 0.0;
-            } else {
-                double delay = 1.0;
-                if (c.lr_delay_steps > 0)
-                    delay = c.lr_delay_mult + (1.0 - c.lr_delay_mult) *
-                                                  sin(0.5 * M_PI * fmin(fmax(it / c.lr_delay_steps, 0.0), 1.0));
-                const double tau = fmin(fmax(it / c.lr_max_steps, 0.0), 1.0);
-                lr = delay * exp(log(c.lr_init) * (1.0 - tau) + log(c.lr_final) * tau);
+    if (g < AG_N) {
+        float step_size = 0.0f, bc2 = 1.0f;
+        if ((groups >> g) & 1) {
+            const uint32_t t = counters[1 + g] + 1u;
+            counters[1 + g] = t;
+            double lr = c.lr[g];
+            if (g == AG_CENTRE) {
+                // get_expon_lr_func (utils.py:46-79) at the iteration number
+                if (it < 0.0 || (c.lr_init == 0.0 && c.lr_final == 0.0)) {
+                    lr = 0.0;
+                } else {
+                    double delay = 1.0;
+                    if (c.lr_delay_steps > 0)
+                        delay = c.lr_delay_mult + (1.0 - c.lr_delay_mult) *
+                                                      sin(0.5 * M_PI * fmin(fmax(it / c.lr_delay_steps, 0.0), 1.0));
+                    const double tau = fmin(fmax(it / c.lr_max_steps, 0.0), 1.0);
+                    lr = delay * exp(log(c.lr_init) * (1.0 - tau) + log(c.lr_final) * tau);
+                }
             }
+            // torch.optim.Adam (non-capturable): step_size = lr / (1 - beta1^t),
+            // denom = sqrt(v) / sqrt(1 - beta2^t) + eps
+            step_size = (float)(lr / (1.0 - pow(c.beta1, (double)t)));
+            bc2 = (float)sqrt(1.0 - pow(c.beta2, (double)t));
         }
-        // torch.optim.Adam (non-capturable): step_size = lr / (1 - beta1^t),
-        // denom = sqrt(v) / sqrt(1 - beta2^t) + eps
-        s.step_size[g] = (float)(lr / (1.0 - pow(c.beta1, (double)t)));
-        s.bc2_sqrt[g] = (float)sqrt(1.0 - pow(c.beta2, (double)t));
+        out->step_size[g] = step_size;
+        out->bc2_sqrt[g] = bc2;
     }
-    s.one_minus_b1 = (float)(1.0 - c.beta1);
-    s.b2 = (float)c.beta2;
-    s.one_minus_b2 = (float)(1.0 - c.beta2);
-    s.eps = (float)c.eps;
-    *out = s;
-    counters[0] = counters[0] + 1u;
+    if (g == 0) {
+        out->one_minus_b1 = (float)(1.0 - c.beta1);
+        out->b2 = (float)c.beta2;
+        out->one_minus_b2 = (float)(1.0 - c.beta2);
+        out->eps = (float)c.eps;
+    }
+    __syncthreads();   // every thread has read the iteration number
+    if (g == 0) counters[0] = counters[0] + 1u;
 }
 
 // one Adam update of x with gradient g and moments m, v (torch's order:
@@ -196,26 +201,52 @@ __global__ void __launch_bounds__(256) k_adam_flat4(size_t n4, int grp, float4* 
 // denom += 1, max_radius = max(max_radius, radius).  grad_scale converts the
 // gradient of a batch loss to the per-view gradient the reference takes
 // (R for a mean over R spectra).
+// Workgroup = 64 Gaussians x 4 receiver groups: thread (g, j) sums the
+// receivers j, j + 4, ... of Gaussian g (their loads four at a time, all in
+// flight together), then the four groups' sums are added in group order, so
+// the result does not depend on timing.
 __global__ void __launch_bounds__(256) k_densify_stats(int P, int R, float grad_scale, const float* dmeans2d,
                                                        const int* radii, float* accum, float* denom,
                                                        float* max_radius)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= P) return;
-    float a = accum[i], dn = denom[i], mr = max_radius[i];
-    for (int r = 0; r < R; ++r) {
-        const size_t o = (size_t)r * P + i;
-        const int rad = radii[o];
-        if (rad > 0) {
-            const float gx = dmeans2d[3 * o], gy = dmeans2d[3 * o + 1];
-            a += sqrtf(gx * gx + gy * gy) * grad_scale;
-            dn += 1.0f;
-            mr = fmaxf(mr, (float)rad);
+    __shared__ float s_a[4][64], s_d[4][64], s_m[4][64];
+    const int gl = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + gl;
+    float a = 0.0f, dn = 0.0f, mr = 0.0f;
+    if (i < P) {
+        for (int r0 = grp; r0 < R; r0 += 16) {
+            int rad[4];
+            float gx[4], gy[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int r = r0 + 4 * u;
+                const size_t o = (size_t)min(r, R - 1) * P + i;
+                rad[u] = r < R ? radii[o] : 0;
+                gx[u] = dmeans2d[3 * o];
+                gy[u] = dmeans2d[3 * o + 1];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (rad[u] > 0) {
+                    a += sqrtf(gx[u] * gx[u] + gy[u] * gy[u]) * grad_scale;
+                    dn += 1.0f;
+                    mr = fmaxf(mr, (float)rad[u]);
+                }
+            }
         }
     }
-    accum[i] = a;
-    denom[i] = dn;
-    max_radius[i] = mr;
+    s_a[grp][gl] = a;
+    s_d[grp][gl] = dn;
+    s_m[grp][gl] = mr;
+    __syncthreads();
+    if (grp == 0 && i < P) {
+        const float sa = ((s_a[0][gl] + s_a[1][gl]) + s_a[2][gl]) + s_a[3][gl];
+        const float sd = ((s_d[0][gl] + s_d[1][gl]) + s_d[2][gl]) + s_d[3][gl];
+        const float sm = fmaxf(fmaxf(s_m[0][gl], s_m[1][gl]), fmaxf(s_m[2][gl], s_m[3][gl]));
+        accum[i] += sa;
+        denom[i] += sd;
+        max_radius[i] = fmaxf(max_radius[i], sm);
+    }
 }
 
 // reset_opacity (gaussian.py:223-231): logit = inverse_sigmoid(min(o, 0.01))
@@ -611,7 +642,7 @@ hipError_t launch_densify_stats(int P, int R, float grad_scale, const float* dme
                                 const Gaussians& p, hipStream_t s)
 {
     if (P == 0 || R == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_densify_stats, dim3(blocks_for(P)), dim3(256), 0, s, P, R, grad_scale, dmeans2d, radii,
+    hipLaunchKernelGGL(k_densify_stats, dim3((unsigned)((P + 63) / 64)), dim3(256), 0, s, P, R, grad_scale, dmeans2d, radii,
                        p.accum, p.denom, p.max_radius);
     return hipGetLastError();
 }
