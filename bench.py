@@ -73,6 +73,8 @@ def parse():
     ap.add_argument("--fwd-ppl", type=int, default=0, help="forward blend pixels per lane (0: library default)")
     ap.add_argument("--bwd-ppl", type=int, default=0, help="backward blend pixels per lane (0: automatic)")
     ap.add_argument("--order", type=int, default=0, help="blend work order (0: automatic)")
+    ap.add_argument("--fwd-depth", type=int, default=0, help="forward blend LDS batch depth (0: automatic)")
+    ap.add_argument("--bwd-depth", type=int, default=0, help="backward blend LDS batch depth (0: automatic)")
     ap.add_argument("--workload", default="receivers", choices=["receivers", "scenes", "csi", "config1"],
                     help="receivers: one scene, R receivers (default); scenes: 16 uneven scenes in one batch; "
                          "csi: 52-subcarrier complex maps; config1: single-receiver training step with the "
@@ -568,11 +570,14 @@ def main():
     gh.set_option("render_fwd_ppl", args.fwd_ppl)
     gh.set_option("render_bwd_ppl", args.bwd_ppl)
     gh.set_option("render_order", args.order)
+    gh.set_option("render_fwd_depth", args.fwd_depth)
+    gh.set_option("render_bwd_depth", args.bwd_depth)
     wl = {"receivers": setup_receivers, "scenes": setup_scenes, "csi": setup_csi,
           "config1": setup_config1}[args.workload](args, dev)
     R, Ptot, Nrec = wl.spectra, wl.Ptot, wl.Nrec
     tuned = None
-    if wl.tune is not None and not args.no_tune and not (args.fwd_ppl or args.bwd_ppl or args.order):
+    if wl.tune is not None and not args.no_tune and not (args.fwd_ppl or args.bwd_ppl or args.order or
+                                                             args.fwd_depth or args.bwd_depth):
         tuned = wl.tune()
         log(args, f"autotuned blend mapping: {tuned}")
 

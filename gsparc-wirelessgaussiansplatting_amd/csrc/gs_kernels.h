@@ -209,7 +209,10 @@ struct RenderFwdArgs {
 // (16x8 half tile) or 4 (16x16 tile per wave); requested = 0 picks by the
 // number of tiles
 int choose_ppl(const Dims& d, int requested);
-hipError_t launch_render_fwd(const Dims& d, const RenderFwdArgs& a, int ppl, hipStream_t s);
+// list entries staged in LDS per batch of the blend kernels (batch depth):
+// 32, 64 or 128; any other request gives 64
+int choose_depth(int requested);
+hipError_t launch_render_fwd(const Dims& d, const RenderFwdArgs& a, int ppl, int depth, hipStream_t s);
 
 struct RenderBwdArgs {
     const uint2* ranges;
@@ -231,7 +234,7 @@ struct RenderBwdArgs {
     uint64_t Kcap;
     unsigned long long* counts;  // optional profiling counters
 };
-hipError_t launch_render_bwd(const Dims& d, const RenderBwdArgs& a, int ppl, hipStream_t s);
+hipError_t launch_render_bwd(const Dims& d, const RenderBwdArgs& a, int ppl, int depth, hipStream_t s);
 
 struct PreBwdArgs {
     const float* means;

@@ -26,6 +26,10 @@ thread_local std::string g_err;
 int g_opt_fwd_ppl = 0;
 int g_opt_bwd_ppl = 0;
 int g_opt_order = 0;
+// list entries staged per LDS batch of the forward and backward blend (32,
+// 64, 128); 0 = automatic
+int g_opt_fwd_depth = 0;
+int g_opt_bwd_depth = 0;
 int g_opt_csi_fwd_ppl = 0;   // CSI forward blend: 1 or 2 pixels per lane on the VALU, 0 = the MFMA kernel
 
 // Autotuned blend mappings, per batch shape (gsparc_autotune).
@@ -36,7 +40,7 @@ struct TuneKey {
         return std::tie(P, R, W, H, S) < std::tie(o.P, o.R, o.W, o.H, o.S);
     }
 };
-struct TuneVal { int fwd_ppl, bwd_ppl, order; };
+struct TuneVal { int fwd_ppl, bwd_ppl, order, fwd_depth, bwd_depth; };
 std::mutex g_tune_mu;
 std::map<TuneKey, TuneVal> g_tuned;
 
@@ -72,6 +76,20 @@ int bwd_ppl(const gs::Dims& d)
     TuneVal v;
     if (g_opt_bwd_ppl == 0 && tuned_for(d, v)) return v.bwd_ppl;
     return gs::choose_ppl(d, g_opt_bwd_ppl);
+}
+
+int fwd_depth(const gs::Dims& d)
+{
+    TuneVal v;
+    if (g_opt_fwd_depth == 0 && tuned_for(d, v)) return v.fwd_depth;
+    return gs::choose_depth(g_opt_fwd_depth);
+}
+
+int bwd_depth(const gs::Dims& d)
+{
+    TuneVal v;
+    if (g_opt_bwd_depth == 0 && tuned_for(d, v)) return v.bwd_depth;
+    return gs::choose_depth(g_opt_bwd_depth);
 }
 
 int fail(int code, const std::string& msg)
@@ -273,6 +291,8 @@ static int* option_slot(const char* name)
     if (std::strcmp(name, "render_bwd_ppl") == 0) return &g_opt_bwd_ppl;
     if (std::strcmp(name, "render_order") == 0) return &g_opt_order;
     if (std::strcmp(name, "csi_fwd_ppl") == 0) return &g_opt_csi_fwd_ppl;
+    if (std::strcmp(name, "render_fwd_depth") == 0) return &g_opt_fwd_depth;
+    if (std::strcmp(name, "render_bwd_depth") == 0) return &g_opt_bwd_depth;
     return nullptr;
 }
 
@@ -285,6 +305,9 @@ int gsparc_set_option(const char* name, int value)
         if (value < 0 || value > 3) return fail(GSPARC_ERR_ARG, "render_order must be 0, 1, 2 or 3");
     } else if (slot == &g_opt_csi_fwd_ppl) {
         if (value < 0 || value > 2) return fail(GSPARC_ERR_ARG, "csi_fwd_ppl must be 0, 1 or 2");
+    } else if (slot == &g_opt_fwd_depth || slot == &g_opt_bwd_depth) {
+        if (value != 0 && value != 32 && value != 64 && value != 128)
+            return fail(GSPARC_ERR_ARG, std::string(name) + " must be 0, 32, 64 or 128");
     } else if (value != 0 && value != 1 && value != 2 && value != 4) {
         return fail(GSPARC_ERR_ARG, std::string(name) + " must be 0, 1, 2 or 4");
     }
@@ -428,7 +451,7 @@ int forward_render_impl(const gs::Dims& d, const float* emission, void* geom, vo
     ra.total = ba.total;
     ra.Kcap = K_capacity;
     ra.counts = g_prof_count ? g_counts_dev : nullptr;
-    GS_LAUNCH("render_fwd", gs::launch_render_fwd(d, ra, fwd_ppl(d), s), s);
+    GS_LAUNCH("render_fwd", gs::launch_render_fwd(d, ra, fwd_ppl(d), fwd_depth(d), s), s);
     return GSPARC_OK;
 }
 
@@ -497,7 +520,7 @@ int backward_impl(int grad_mode, const gs::Dims& d, const FwdInputs& in, const f
         ra.total = (const uint32_t*)(ib + L.total);
         ra.Kcap = K_capacity;
         ra.counts = g_prof_count ? g_counts_dev : nullptr;
-        GS_LAUNCH("render_bwd", gs::launch_render_bwd(d, ra, bwd_ppl(d), s), s);
+        GS_LAUNCH("render_bwd", gs::launch_render_bwd(d, ra, bwd_ppl(d), bwd_depth(d), s), s);
     }
     gs::PreBwdArgs pa;
     pa.means = in.means; pa.scales = in.scales; pa.rots = in.rotations; pa.mlp = in.mlp; pa.views = in.views;
@@ -629,36 +652,40 @@ int autotune_impl(int grad_mode, const gs::Dims& d, const float* emission, const
         out_ms = t[1 + iters / 2];
         return e;
     };
+    // candidates: pixels per lane x batch depth, forward and backward
     const int ppls[3] = {1, 2, 4};
-    TuneVal best{1, 1, 1};
+    const int depths[3] = {32, 64, 128};
+    TuneVal best{1, 1, 1, 64, 64};
     float best_f = 0.0f, best_b = 0.0f, best_total = 1e30f;
     hipError_t err = hipSuccess;
     auto scatter = [&] { return gs::launch_bin_place(d, sa, s); };
     for (int mode = 1; mode <= 3 && err == hipSuccess; ++mode) {
         err = gs::launch_tile_order(d, mode, sa.ranges, order, sa.total, s);
         float bf = 1e30f, bbk = 1e30f;
-        int pf = 1, pb = 1;
-        for (int k = 0; k < 3 && err == hipSuccess; ++k) {
+        int pf = 1, pb = 1, df = 64, db = 64;
+        for (int k = 0; k < 9 && err == hipSuccess; ++k) {
+            const int ppl = ppls[k / 3], dep = depths[k % 3];
             float t = 0.0f;
-            err = time_it(scatter, [&] { return gs::launch_render_fwd(d, fa, ppls[k], s); }, t);
-            if (table) table[(mode - 1) * 6 + k] = t;
-            if (t < bf) { bf = t; pf = ppls[k]; }
+            err = time_it(scatter, [&] { return gs::launch_render_fwd(d, fa, ppl, dep, s); }, t);
+            if (table) table[(mode - 1) * 18 + k] = t;
+            if (t < bf) { bf = t; pf = ppl; df = dep; }
         }
-        for (int k = 0; k < 3 && err == hipSuccess; ++k) {
+        for (int k = 0; k < 9 && err == hipSuccess; ++k) {
+            const int ppl = ppls[k / 3], dep = depths[k % 3];
             float t = 0.0f;
             auto before = [&] {
                 hipError_t e = scatter();
-                if (e == hipSuccess) e = gs::launch_render_fwd(d, fa, pf, s);
+                if (e == hipSuccess) e = gs::launch_render_fwd(d, fa, pf, df, s);
                 if (e == hipSuccess) e = gs::launch_fill_u32((uint32_t*)g8, 0u, d.Nrec * 8, s);
                 return e;
             };
-            err = time_it(before, [&] { return gs::launch_render_bwd(d, ba, ppls[k], s); }, t);
-            if (table) table[(mode - 1) * 6 + 3 + k] = t;
-            if (t < bbk) { bbk = t; pb = ppls[k]; }
+            err = time_it(before, [&] { return gs::launch_render_bwd(d, ba, ppl, dep, s); }, t);
+            if (table) table[(mode - 1) * 18 + 9 + k] = t;
+            if (t < bbk) { bbk = t; pb = ppl; db = dep; }
         }
         if (err == hipSuccess && bf + bbk < best_total) {
             best_total = bf + bbk;
-            best = {pf, pb, mode};
+            best = {pf, pb, mode, df, db};
             best_f = bf;
             best_b = bbk;
         }
@@ -666,7 +693,7 @@ int autotune_impl(int grad_mode, const gs::Dims& d, const float* emission, const
     // leave the buffers as the chosen mapping's forward would
     if (err == hipSuccess) err = gs::launch_tile_order(d, best.order, sa.ranges, order, sa.total, s);
     if (err == hipSuccess) err = scatter();
-    if (err == hipSuccess) err = gs::launch_render_fwd(d, fa, best.fwd_ppl, s);
+    if (err == hipSuccess) err = gs::launch_render_fwd(d, fa, best.fwd_ppl, best.fwd_depth, s);
     if (err == hipSuccess) err = hipStreamSynchronize(s);
     hipEventDestroy(ev[0]);
     hipEventDestroy(ev[1]);
@@ -678,6 +705,8 @@ int autotune_impl(int grad_mode, const gs::Dims& d, const float* emission, const
     choice[0] = best.fwd_ppl;
     choice[1] = best.bwd_ppl;
     choice[2] = best.order;
+    choice[3] = best.fwd_depth;
+    choice[4] = best.bwd_depth;
     if (ms) {
         ms[0] = best_f;
         ms[1] = best_b;

@@ -27,12 +27,15 @@
 
 namespace gs {
 
-template <int PPL, bool COUNT>
+template <int PPL, int BD, bool COUNT>
 __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
 {
-    __shared__ float4 s0[GS_WAVE];
-    __shared__ float4 s1[GS_WAVE];
-    __shared__ float2 ss[GS_WAVE];
+    // BD list entries are staged per batch: NS per lane (BD = 64, 128), or
+    // one on the first BD lanes (BD = 32)
+    constexpr int NS = BD > GS_WAVE ? BD / GS_WAVE : 1;
+    __shared__ float4 s0[NS * GS_WAVE];
+    __shared__ float4 s1[NS * GS_WAVE];
+    __shared__ float2 ss[NS * GS_WAVE];
     const Pixels<PPL> pm = map_pixels<PPL>(d, a.order, (int)a.total[1]);
     if (!pm.valid) return;
     if ((uint64_t)a.total[0] > a.Kcap) {   // binning overflow, reported by the host
@@ -65,51 +68,66 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd(Dims d, RenderFwdArgs a)
         pxf[k] = (float)pm.px[k];
         pyf[k] = (float)pm.py[k];
     }
-    for (uint32_t start = rb; start < re; start += GS_WAVE) {
+    for (uint32_t start = rb; start < re; start += BD) {
         bool live = false;
 #pragma unroll
         for (int k = 0; k < PPL; ++k) live |= T[k] > 0.0f;
         if (ballot(live) == 0) break;
-        const uint32_t e = start + lane;
-        bool keep = false;
-        if (e < re) {
-            const size_t gi = rP + a.plist[e];
-            const float4 A = a.splat[2 * gi];
-            const float4 B = a.splat[2 * gi + 1];
-            s0[lane] = A;
-            s1[lane] = B;
-            ss[lane] = a.sig[gi];
-            keep = !block_misses(A.x, A.y, A.z, A.w, B.x, B.z, pm.x0, pm.x1, pm.y0, pm.y1);
-        }
-        uint64_t mask = ballot(keep);
-        __syncthreads();
-        // list position (1-based) of entry j of this batch is base + j
-        const uint32_t base = start - rb + 1;
-        while (mask) {
-            const int j = __builtin_ctzll(mask);
-            mask &= mask - 1;
-            const float4 A = s0[j];
-            const float4 B = s1[j];
-            const float2 S = ss[j];
+        // every load of the batch is issued before the first block test
+        bool in[NS];
+        size_t gi[NS];
 #pragma unroll
-            for (int k = 0; k < PPL; ++k) {
-                const float dx = A.x - pxf[k], dy = A.y - pyf[k];
-                const float power = pair_power(A.z, A.w, B.x, dx, dy);
-                // power >= thr: alpha >= 1/255 (gs_alpha_thr)
-                if (power > 0.0f || power < B.z) continue;
-                const float alpha = fminf(0.99f, B.y * gs_expf_blend(power));
-                const float test_T = T[k] * (1.0f - alpha);
-                if (test_T < 0.0001f) {
-                    if (COUNT && T[k] > 0.0f) stop[k] = base + j;
-                    T[k] = -fabsf(T[k]);
-                    continue;
+        for (int b = 0; b < NS; ++b) {
+            const uint32_t e = start + b * GS_WAVE + lane;
+            in[b] = e < re && (BD >= GS_WAVE || lane < BD);
+            gi[b] = in[b] ? rP + a.plist[e] : 0;
+        }
+        uint64_t masks[NS];
+#pragma unroll
+        for (int b = 0; b < NS; ++b) {
+            bool keep = false;
+            if (in[b]) {
+                const float4 A = a.splat[2 * gi[b]];
+                const float4 B = a.splat[2 * gi[b] + 1];
+                s0[b * GS_WAVE + lane] = A;
+                s1[b * GS_WAVE + lane] = B;
+                ss[b * GS_WAVE + lane] = a.sig[gi[b]];
+                keep = !block_misses(A.x, A.y, A.z, A.w, B.x, B.z, pm.x0, pm.x1, pm.y0, pm.y1);
+            }
+            masks[b] = ballot(keep);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < NS; ++b) {
+            uint64_t mask = masks[b];
+            // list position (1-based) of entry j of this sub-batch is base + j
+            const uint32_t base = start - rb + 1 + b * GS_WAVE;
+            while (mask) {
+                const int j = __builtin_ctzll(mask);
+                mask &= mask - 1;
+                const float4 A = s0[b * GS_WAVE + j];
+                const float4 B = s1[b * GS_WAVE + j];
+                const float2 S = ss[b * GS_WAVE + j];
+#pragma unroll
+                for (int k = 0; k < PPL; ++k) {
+                    const float dx = A.x - pxf[k], dy = A.y - pyf[k];
+                    const float power = pair_power(A.z, A.w, B.x, dx, dy);
+                    // power >= thr: alpha >= 1/255 (gs_alpha_thr)
+                    if (power > 0.0f || power < B.z) continue;
+                    const float alpha = fminf(0.99f, B.y * gs_expf_blend(power));
+                    const float test_T = T[k] * (1.0f - alpha);
+                    if (test_T < 0.0001f) {
+                        if (COUNT && T[k] > 0.0f) stop[k] = base + j;
+                        T[k] = -fabsf(T[k]);
+                        continue;
+                    }
+                    const float w = alpha * T[k];
+                    are[k] = fmaf(S.x, w, are[k]);
+                    aim[k] = fmaf(S.y, w, aim[k]);
+                    T[k] = test_T;
+                    last[k] = base + j;
+                    if (COUNT) ++cnt[k];
                 }
-                const float w = alpha * T[k];
-                are[k] = fmaf(S.x, w, are[k]);
-                aim[k] = fmaf(S.y, w, aim[k]);
-                T[k] = test_T;
-                last[k] = base + j;
-                if (COUNT) ++cnt[k];
             }
         }
         __syncthreads();
@@ -220,10 +238,13 @@ __device__ __forceinline__ void bwd_step(bool c, float power, float dx, float dy
     }
 }
 
-template <int MODE, int PPL, bool COUNT>
+template <int MODE, int PPL, int BD, bool COUNT>
 __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
 {
-    __shared__ BwdRec rec[GS_WAVE];
+    // BD list entries are staged per batch: NS per lane (BD = 64, 128), or
+    // one on the first BD lanes (BD = 32)
+    constexpr int NS = BD > GS_WAVE ? BD / GS_WAVE : 1;
+    __shared__ BwdRec rec[NS * GS_WAVE];
     const Pixels<PPL> pm = map_pixels<PPL>(d, a.order, (int)a.total[1]);
     if (!pm.valid) return;
     if ((uint64_t)a.total[0] > a.Kcap) return;   // no lists: the gradient stays cleared
@@ -269,98 +290,109 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
     float* const g8r = a.g8 + rP * 8;
     // pixel row groups of the wave: quarters k and k ^ 1 share a row when PPL >= 2
     constexpr int NROW = PPL == 1 ? 1 : PPL / 2;
-    for (int top = maxlast - 1; top >= 0; top -= GS_WAVE) {
-        // entry j of this batch sits at list position top - j (walked back to front)
-        const int rel = top - lane;
-        bool hit[PPL];
+    for (int top = maxlast - 1; top >= 0; top -= BD) {
+        // entry j of sub-batch b sits at list position top - 64 b - j (walked
+        // back to front); every load of the batch is issued before the tests
+        int rel[NS];
+        uint32_t id[NS];
 #pragma unroll
-        for (int k = 0; k < PPL; ++k) hit[k] = false;
-        if (rel >= 0) {
-            const uint32_t id = a.plist[rb + rel];
-            const size_t gi = rP + id;
-            const float4 A = a.splat[2 * gi];
-            float4 B = a.splat[2 * gi + 1];
-            const float2 S = a.sig[gi];
-            B.w = __uint_as_float(id);
-            rec[lane].A = A;
-            rec[lane].B = B;
-            rec[lane].S = S;
-            // culled per 8x8 quarter: an entry is walked only over the
-            // quarters where it can reach alpha >= 1/255 and where some
-            // pixel's contributors reach back to it
-#pragma unroll
-            for (int k = 0; k < PPL; ++k)
-                hit[k] = rel < qlast[k] && !block_misses(A.x, A.y, A.z, A.w, B.x, B.z, pm.qx[k], pm.qx[k] + 7.0f,
-                                                         pm.qy[k], pm.qy[k] + 7.0f);
+        for (int b = 0; b < NS; ++b) {
+            rel[b] = (BD >= GS_WAVE || lane < BD) ? top - b * GS_WAVE - lane : -1;
+            id[b] = rel[b] >= 0 ? a.plist[rb + rel[b]] : 0u;
         }
-        uint64_t qmask[PPL];
-        uint64_t mask = 0;
+        uint64_t qmask[NS][PPL];
 #pragma unroll
-        for (int k = 0; k < PPL; ++k) {
-            qmask[k] = ballot(hit[k]);
-            mask |= qmask[k];
+        for (int b = 0; b < NS; ++b) {
+            bool hit[PPL];
+#pragma unroll
+            for (int k = 0; k < PPL; ++k) hit[k] = false;
+            if (rel[b] >= 0) {
+                const size_t gi = rP + id[b];
+                const float4 A = a.splat[2 * gi];
+                float4 B = a.splat[2 * gi + 1];
+                const float2 S = a.sig[gi];
+                B.w = __uint_as_float(id[b]);
+                rec[b * GS_WAVE + lane].A = A;
+                rec[b * GS_WAVE + lane].B = B;
+                rec[b * GS_WAVE + lane].S = S;
+                // culled per 8x8 quarter: an entry is walked only over the
+                // quarters where it can reach alpha >= 1/255 and where some
+                // pixel's contributors reach back to it
+#pragma unroll
+                for (int k = 0; k < PPL; ++k)
+                    hit[k] = rel[b] < qlast[k] && !block_misses(A.x, A.y, A.z, A.w, B.x, B.z, pm.qx[k],
+                                                                 pm.qx[k] + 7.0f, pm.qy[k], pm.qy[k] + 7.0f);
+            }
+#pragma unroll
+            for (int k = 0; k < PPL; ++k) qmask[b][k] = ballot(hit[k]);
         }
         __syncthreads();
-        while (mask) {
-            const int j = __builtin_ctzll(mask);
-            mask &= mask - 1;
-            const float4 A = rec[j].A;
-            const float4 B = rec[j].B;
-            const float2 S = rec[j].S;
-            const int pos = top - j;
-            // quarters k and k ^ 1 share their pixel rows: the row terms of
-            // pair_power are computed once per row
-            float dyr[NROW], hdd[NROW];
 #pragma unroll
-            for (int g = 0; g < NROW; ++g) {
-                dyr[g] = A.y - pyf[g * (PPL / NROW)];
-                hdd[g] = (B.x * dyr[g]) * dyr[g];
-            }
-            // the entry's sums: written by its first step with a contributor
-            // (quarter 0) or cleared before the first (a later quarter), so
-            // nothing is carried from the previous entry
-            BwdSums s;
-            bool have = false;   // wave-uniform: a quarter of this entry had a contributor
+        for (int b = 0; b < NS; ++b) {
+            uint64_t mask = 0;
 #pragma unroll
-            for (int k = 0; k < PPL; ++k) {
-                if (!((qmask[k] >> j) & 1ull)) continue;   // wave-uniform: the quarter is culled
-                const int g = k / (PPL / NROW);
-                const float dx = A.x - pxf[k];
-                const float power = fmaf(dx, fmaf(A.w, dyr[g], A.z * dx), hdd[g]);   // pair_power
-                // the forward's decision: power in [thr, 0] <=> alpha >= 1/255
-                const bool act = pos < last[k], nonpos = !(power > 0.0f), reach = !(power < B.z);
-                const bool c = act && nonpos && reach;
-                // the masks of single compares combine on the scalar unit
-                if ((ballot(act) & ballot(nonpos) & ballot(reach)) == 0) continue;
-                if (k == 0) {
-                    bwd_step<MODE, true>(c, power, dx, dyr[g], B.y, S, T[k], acc_re[k], acc_im[k], gre[k], gim[k], s);
-                } else {
-                    if (!have) s = BwdSums{};
-                    bwd_step<MODE, false>(c, power, dx, dyr[g], B.y, S, T[k], acc_re[k], acc_im[k], gre[k], gim[k], s);
+            for (int k = 0; k < PPL; ++k) mask |= qmask[b][k];
+            while (mask) {
+                const int j = __builtin_ctzll(mask);
+                mask &= mask - 1;
+                const float4 A = rec[b * GS_WAVE + j].A;
+                const float4 B = rec[b * GS_WAVE + j].B;
+                const float2 S = rec[b * GS_WAVE + j].S;
+                const int pos = top - b * GS_WAVE - j;
+                // quarters k and k ^ 1 share their pixel rows: the row terms of
+                // pair_power are computed once per row
+                float dyr[NROW], hdd[NROW];
+#pragma unroll
+                for (int g = 0; g < NROW; ++g) {
+                    dyr[g] = A.y - pyf[g * (PPL / NROW)];
+                    hdd[g] = (B.x * dyr[g]) * dyr[g];
                 }
-                have = true;
-                if (COUNT) contrib += c ? 1u : 0u;
+                // the entry's sums: written by its first step with a contributor
+                // (quarter 0) or cleared before the first (a later quarter), so
+                // nothing is carried from the previous entry
+                BwdSums s;
+                bool have = false;   // wave-uniform: a quarter of this entry had a contributor
+#pragma unroll
+                for (int k = 0; k < PPL; ++k) {
+                    if (!((qmask[b][k] >> j) & 1ull)) continue;   // wave-uniform: the quarter is culled
+                    const int g = k / (PPL / NROW);
+                    const float dx = A.x - pxf[k];
+                    const float power = fmaf(dx, fmaf(A.w, dyr[g], A.z * dx), hdd[g]);   // pair_power
+                    // the forward's decision: power in [thr, 0] <=> alpha >= 1/255
+                    const bool act = pos < last[k], nonpos = !(power > 0.0f), reach = !(power < B.z);
+                    const bool c = act && nonpos && reach;
+                    // the masks of single compares combine on the scalar unit
+                    if ((ballot(act) & ballot(nonpos) & ballot(reach)) == 0) continue;
+                    if (k == 0) {
+                        bwd_step<MODE, true>(c, power, dx, dyr[g], B.y, S, T[k], acc_re[k], acc_im[k], gre[k], gim[k], s);
+                    } else {
+                        if (!have) s = BwdSums{};
+                        bwd_step<MODE, false>(c, power, dx, dyr[g], B.y, S, T[k], acc_re[k], acc_im[k], gre[k], gim[k], s);
+                    }
+                    have = true;
+                    if (COUNT) contrib += c ? 1u : 0u;
+                }
+                if (!have) continue;
+                // the entry's raw sums (g8 record layout): the preprocess backward
+                // applies the conic to the first two (dL/dmean2D, backward.cu:600-605)
+                // and the factors -1/2, -1, -1/2 to the next three (backward.cu:608-610)
+                float v[8];
+                v[0] = s.sh;
+                v[1] = s.sk;
+                v[2] = s.sxx;
+                v[3] = s.sxy;
+                v[4] = s.syy;
+                v[5] = s.sg;
+                v[6] = s.swr;
+                v[7] = s.swi;
+                float tot = wave_sum8(v, lane);
+                // completed on all lanes, so that the last DPP step is one add
+                asm volatile("" : "+v"(tot));
+                // the record index is wave-uniform: its address arithmetic stays scalar
+                const uint32_t gid = __builtin_amdgcn_readfirstlane(__float_as_uint(B.w));
+                float* rp = g8r + (size_t)gid * 8;
+                if ((lane & 7) == 0) atomicAdd(rp + (lane >> 3), tot);
             }
-            if (!have) continue;
-            // the entry's raw sums (g8 record layout): the preprocess backward
-            // applies the conic to the first two (dL/dmean2D, backward.cu:600-605)
-            // and the factors -1/2, -1, -1/2 to the next three (backward.cu:608-610)
-            float v[8];
-            v[0] = s.sh;
-            v[1] = s.sk;
-            v[2] = s.sxx;
-            v[3] = s.sxy;
-            v[4] = s.syy;
-            v[5] = s.sg;
-            v[6] = s.swr;
-            v[7] = s.swi;
-            float tot = wave_sum8(v, lane);
-            // completed on all lanes, so that the last DPP step is one add
-            asm volatile("" : "+v"(tot));
-            // the record index is wave-uniform: its address arithmetic stays scalar
-            const uint32_t gid = __builtin_amdgcn_readfirstlane(__float_as_uint(B.w));
-            float* rp = g8r + (size_t)gid * 8;
-            if ((lane & 7) == 0) atomicAdd(rp + (lane >> 3), tot);
         }
         __syncthreads();
     }
@@ -391,42 +423,65 @@ int choose_ppl(const Dims& d, int requested)
     return ((long long)d.R * d.T >= 2048) ? 2 : 1;
 }
 
-template <bool COUNT>
-static void launch_fwd_count(const Dims& d, const RenderFwdArgs& a, int ppl, hipStream_t s)
+int choose_depth(int requested)
 {
-    const dim3 grid = render_grid(d, ppl);
-    if (ppl == 4) hipLaunchKernelGGL((k_render_fwd<4, COUNT>), grid, dim3(GS_WAVE), 0, s, d, a);
-    else if (ppl == 2) hipLaunchKernelGGL((k_render_fwd<2, COUNT>), grid, dim3(GS_WAVE), 0, s, d, a);
-    else hipLaunchKernelGGL((k_render_fwd<1, COUNT>), grid, dim3(GS_WAVE), 0, s, d, a);
+    return (requested == 32 || requested == 128) ? requested : 64;
 }
 
-hipError_t launch_render_fwd(const Dims& d, const RenderFwdArgs& a, int ppl, hipStream_t s)
+template <int PPL, bool COUNT>
+static void launch_fwd_depth(const Dims& d, const RenderFwdArgs& a, int depth, hipStream_t s)
+{
+    const dim3 grid = render_grid(d, PPL);
+    if (depth == 32) hipLaunchKernelGGL((k_render_fwd<PPL, 32, COUNT>), grid, dim3(GS_WAVE), 0, s, d, a);
+    else if (depth == 128) hipLaunchKernelGGL((k_render_fwd<PPL, 128, COUNT>), grid, dim3(GS_WAVE), 0, s, d, a);
+    else hipLaunchKernelGGL((k_render_fwd<PPL, 64, COUNT>), grid, dim3(GS_WAVE), 0, s, d, a);
+}
+
+template <bool COUNT>
+static void launch_fwd_count(const Dims& d, const RenderFwdArgs& a, int ppl, int depth, hipStream_t s)
+{
+    if (ppl == 4) launch_fwd_depth<4, COUNT>(d, a, depth, s);
+    else if (ppl == 2) launch_fwd_depth<2, COUNT>(d, a, depth, s);
+    else launch_fwd_depth<1, COUNT>(d, a, depth, s);
+}
+
+hipError_t launch_render_fwd(const Dims& d, const RenderFwdArgs& a, int ppl, int depth, hipStream_t s)
 {
     if (d.R == 0 || d.T == 0) return hipSuccess;
-    if (a.counts) launch_fwd_count<true>(d, a, ppl, s);
-    else launch_fwd_count<false>(d, a, ppl, s);
+    depth = choose_depth(depth);
+    if (a.counts) launch_fwd_count<true>(d, a, ppl, depth, s);
+    else launch_fwd_count<false>(d, a, ppl, depth, s);
     return hipGetLastError();
 }
 
-template <int MODE, bool COUNT>
-static void launch_bwd_mode(const Dims& d, const RenderBwdArgs& a, int ppl, hipStream_t s)
+template <int MODE, int PPL, bool COUNT>
+static void launch_bwd_depth(const Dims& d, const RenderBwdArgs& a, int depth, hipStream_t s)
 {
-    const dim3 grid = render_grid(d, ppl);
-    if (ppl == 4) hipLaunchKernelGGL((k_render_bwd<MODE, 4, COUNT>), grid, dim3(GS_WAVE), 0, s, d, a);
-    else if (ppl == 2) hipLaunchKernelGGL((k_render_bwd<MODE, 2, COUNT>), grid, dim3(GS_WAVE), 0, s, d, a);
-    else hipLaunchKernelGGL((k_render_bwd<MODE, 1, COUNT>), grid, dim3(GS_WAVE), 0, s, d, a);
+    const dim3 grid = render_grid(d, PPL);
+    if (depth == 32) hipLaunchKernelGGL((k_render_bwd<MODE, PPL, 32, COUNT>), grid, dim3(GS_WAVE), 0, s, d, a);
+    else if (depth == 128) hipLaunchKernelGGL((k_render_bwd<MODE, PPL, 128, COUNT>), grid, dim3(GS_WAVE), 0, s, d, a);
+    else hipLaunchKernelGGL((k_render_bwd<MODE, PPL, 64, COUNT>), grid, dim3(GS_WAVE), 0, s, d, a);
 }
 
-hipError_t launch_render_bwd(const Dims& d, const RenderBwdArgs& a, int ppl, hipStream_t s)
+template <int MODE, bool COUNT>
+static void launch_bwd_mode(const Dims& d, const RenderBwdArgs& a, int ppl, int depth, hipStream_t s)
+{
+    if (ppl == 4) launch_bwd_depth<MODE, 4, COUNT>(d, a, depth, s);
+    else if (ppl == 2) launch_bwd_depth<MODE, 2, COUNT>(d, a, depth, s);
+    else launch_bwd_depth<MODE, 1, COUNT>(d, a, depth, s);
+}
+
+hipError_t launch_render_bwd(const Dims& d, const RenderBwdArgs& a, int ppl, int depth, hipStream_t s)
 {
     if (d.R == 0 || d.T == 0) return hipSuccess;
+    depth = choose_depth(depth);
     const bool count = a.counts != nullptr;
     if (a.mode == GRAD_REFERENCE) {
-        if (count) launch_bwd_mode<GRAD_REFERENCE, true>(d, a, ppl, s);
-        else launch_bwd_mode<GRAD_REFERENCE, false>(d, a, ppl, s);
+        if (count) launch_bwd_mode<GRAD_REFERENCE, true>(d, a, ppl, depth, s);
+        else launch_bwd_mode<GRAD_REFERENCE, false>(d, a, ppl, depth, s);
     } else {
-        if (count) launch_bwd_mode<GRAD_EXACT, true>(d, a, ppl, s);
-        else launch_bwd_mode<GRAD_EXACT, false>(d, a, ppl, s);
+        if (count) launch_bwd_mode<GRAD_EXACT, true>(d, a, ppl, depth, s);
+        else launch_bwd_mode<GRAD_EXACT, false>(d, a, ppl, depth, s);
     }
     return hipGetLastError();
 }

@@ -373,11 +373,13 @@ def unpack_mlps(packed, scene_P):
 
 
 def autotune(forward_out, grad_mode="reference", iters=5, emission=None, scenes: SceneBatch | None = None):
-    """Tune the blend kernels' mapping (pixels per lane of the forward and
-    backward blend, work order) on the buffers of a finished forward, the
-    tuple returned by forward_batched or forward_scenes.  Later calls with a
-    batch of the same shape use the choice.  Returns {"fwd_ppl", "bwd_ppl",
-    "order", "fwd_ms", "bwd_ms", "times_ms"} (every candidate's median time)."""
+    """Tune the blend kernels' mapping (pixels per lane and LDS batch depth
+    of the forward and backward blend, work order) on the buffers of a
+    finished forward, the tuple returned by forward_batched or
+    forward_scenes.  Later calls with a batch of the same shape use the
+    choice.  Returns {"fwd_ppl", "bwd_ppl", "order", "fwd_depth",
+    "bwd_depth", "fwd_ms", "bwd_ms", "times_ms"} (every candidate's median
+    time)."""
     K, ore, oim, radii, geom, binbuf, img = forward_out
     R, H, W = int(ore.size(0)), int(ore.size(1)), int(ore.size(2))
     dev = ore.device
@@ -388,9 +390,9 @@ def autotune(forward_out, grad_mode="reference", iters=5, emission=None, scenes:
     dim = torch.randn((R, H, W), generator=g, device=dev)
     out_re, out_im = torch.empty_like(ore), torch.empty_like(oim)
     em = gh.f32(emission).reshape(-1, 2) if emission is not None else None
-    choice = (C.c_int * 3)()
+    choice = (C.c_int * 5)()
     ms = (C.c_float * 2)()
-    table = (C.c_float * 18)()
+    table = (C.c_float * 54)()
     rad = radii.reshape(-1).to(torch.int32).contiguous()
     stream = gh.stream_handle(dev)
     if scenes is None:
@@ -407,11 +409,12 @@ def autotune(forward_out, grad_mode="reference", iters=5, emission=None, scenes:
                                           int(iters), choice, ms, table, stream), "autotune")
     times = {}
     for o in range(3):
-        for k, ppl in enumerate((1, 2, 4)):
-            times[f"order{o + 1}_fwd_ppl{ppl}"] = round(table[o * 6 + k], 4)
-            times[f"order{o + 1}_bwd_ppl{ppl}"] = round(table[o * 6 + 3 + k], 4)
-    return {"fwd_ppl": choice[0], "bwd_ppl": choice[1], "order": choice[2], "fwd_ms": round(ms[0], 4),
-            "bwd_ms": round(ms[1], 4), "times_ms": times}
+        for k in range(9):
+            ppl, depth = (1, 2, 4)[k // 3], (32, 64, 128)[k % 3]
+            times[f"order{o + 1}_fwd_ppl{ppl}_d{depth}"] = round(table[o * 18 + k], 4)
+            times[f"order{o + 1}_bwd_ppl{ppl}_d{depth}"] = round(table[o * 18 + 9 + k], 4)
+    return {"fwd_ppl": choice[0], "bwd_ppl": choice[1], "order": choice[2], "fwd_depth": choice[3],
+            "bwd_depth": choice[4], "fwd_ms": round(ms[0], 4), "bwd_ms": round(ms[1], 4), "times_ms": times}
 
 
 def autotune_clear():

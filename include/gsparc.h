@@ -61,6 +61,10 @@ const char* gsparc_last_error(void);
  *       1 = one 8x8 quarter-tile per wave, 2 = half a tile per wave, 4 = one
  *       16x16 tile per wave.  Speed only: the forward result is bit-identical
  *       either way, the backward differs by summation order.
+ *   "render_fwd_depth", "render_bwd_depth"  batch depth of the blend
+ *       kernels: list entries staged in LDS per batch (the reference stages
+ *       BLOCK_SIZE = 256 per batch, forward.cu:193-207, backward.cu:523-541):
+ *       0 = automatic (64), 32, 64 or 128.  Speed only, as above.
  *   "render_order"  order in which the blend kernels take the (receiver,
  *       tile) pairs, longest lists first: 0 = automatic (2 from 16
  *       receivers on, else 1), 1 = one list dealt
@@ -78,14 +82,17 @@ int gsparc_get_option(const char* name, int* value);
 
 /* Autotuner of the blend kernels' compute mapping: on the buffers of a
  * finished forward (any form, K_capacity and radii as there), times the
- * forward and backward blend under every pixels-per-lane mapping (1, 2, 4)
- * and work order (1, 2, 3) and keeps the fastest combination for batches of
+ * forward and backward blend under every pixels-per-lane mapping (1, 2, 4),
+ * batch depth (32, 64, 128) and work order (1, 2, 3) and keeps the fastest
+ * combination for batches of
  * this shape; later calls with the options at 0 use it.  Each candidate is
  * timed `iters` times (median) where it runs in a step: the forward blend
  * after the ordered scatter that writes its lists, the backward blend after a
- * forward blend and the gradient clear.  choice[3] = {forward ppl, backward
- * ppl, order}; ms[2] (may be NULL) = their per-launch times; table[18] (may be
- * NULL) = per order, forward then backward times for ppl 1, 2, 4.
+ * forward blend and the gradient clear.  choice[5] = {forward ppl, backward
+ * ppl, order, forward batch depth, backward batch depth}; ms[2] (may be
+ * NULL) = their per-launch times; table[54] (may be NULL) = per order, the
+ * forward then the backward times for (ppl, depth) = (1, 32), (1, 64),
+ * (1, 128), (2, 32), ..., (4, 128).
  * dout_real / dout_imag [R,H,W] drive the timed backward; out_real / out_imag
  * [R,H,W] receive the (unchanged) spectra; `grad` is overwritten, so run the
  * backward after tuning.  Synchronises the stream; not while capturing.  The

@@ -37,8 +37,13 @@ def test_autotune_uniform_batch():
     res = wr.autotune(out)
     torch.cuda.synchronize()
     assert res["fwd_ppl"] in (1, 2, 4) and res["bwd_ppl"] in (1, 2, 4) and res["order"] in (1, 2, 3)
+    assert res["fwd_depth"] in (32, 64, 128) and res["bwd_depth"] in (32, 64, 128)
     assert res["fwd_ms"] > 0 and res["bwd_ms"] > 0
-    for name in ("render_fwd_ppl", "render_bwd_ppl", "render_order"):
+    # every candidate was timed: 3 orders x (3 pixel mappings x 3 batch depths), forward and backward
+    assert len(res["times_ms"]) == 54 and all(t > 0 for t in res["times_ms"].values())
+    assert res["fwd_ms"] == min(v for k, v in res["times_ms"].items() if "_fwd_" in k and
+                                k.startswith(f"order{res['order']}_"))
+    for name in ("render_fwd_ppl", "render_bwd_ppl", "render_order", "render_fwd_depth", "render_bwd_depth"):
         assert gh.get_option(name) == 0
     # the tuned shape renders the same spectra and gradients
     d2, out2 = gpu_forward(sc)

@@ -151,16 +151,20 @@ def test_instance_count_saturates_past_2_32():
     assert float(gr[1].abs().max()) == 0.0 and float(gr[3].abs().max()) == 0.0
 
 
+@pytest.mark.parametrize("depth", [32, 64, 128])
 @pytest.mark.parametrize("ppl", [1, 2, 4])
-def test_blend_counters_count_the_reference_loops(ppl):
+def test_blend_counters_count_the_reference_loops(ppl, depth):
     """The counting variants count, in the reference's terms, the pairs its
     forward visits, the pairs its backward walks and the contributing pairs;
     the oracle's loops give the same numbers exactly, and the forward and the
-    backward find the same contributing pairs."""
+    backward find the same contributing pairs, under every pixel mapping and
+    batch depth."""
     import gsparc_hip as gh
     sc = S.small_scene(141, 2000, R=3, walls=True)
     gh.set_option("render_fwd_ppl", ppl)
     gh.set_option("render_bwd_ppl", ppl)
+    gh.set_option("render_fwd_depth", depth)
+    gh.set_option("render_bwd_depth", depth)
     gh.profile_enable(True, timing=False, counting=True)
     try:
         d, out = gpu_forward(sc)
@@ -172,8 +176,8 @@ def test_blend_counters_count_the_reference_loops(ppl):
         cnt = gh.profile_counts()
     finally:
         gh.profile_enable(False)
-        gh.set_option("render_fwd_ppl", 0)
-        gh.set_option("render_bwd_ppl", 0)
+        for name in ("render_fwd_ppl", "render_bwd_ppl", "render_fwd_depth", "render_bwd_depth"):
+            gh.set_option(name, 0)
     visited = blended = walked = 0
     for r in range(sc.R):
         pre, keys, vals, ranges, fwd = O.render(sc, r)

@@ -230,6 +230,41 @@ def test_work_orders(order):
         gh.set_option("render_order", 0)
 
 
+@pytest.mark.parametrize("depth", [32, 128])
+@pytest.mark.parametrize("pplv", [1, 2, 4])
+def test_batch_depths(pplv, depth):
+    """Every LDS batch depth of the blend kernels gives the oracle's spectra,
+    final T and contributor counts, and its gradients, on lists that span
+    several batches and terminate inside one."""
+    import gsparc_hip as gh
+    for name, v in (("render_fwd_ppl", pplv), ("render_bwd_ppl", pplv), ("render_fwd_depth", depth),
+                    ("render_bwd_depth", depth)):
+        gh.set_option(name, v)
+    try:
+        R = 3
+        sc = S.small_scene(43, 3000, R=R, walls=True)
+        print(f"batch depth {depth} ppl {pplv}", flush=True)
+        d, out = gpu_forward(sc)
+        st = read_state(sc, out)
+        for r in range(R):
+            _check_forward(sc, st, r)
+        rng = np.random.default_rng(43)
+        dre = rng.normal(size=(R, sc.H, sc.W)).astype(np.float32)
+        dim = rng.normal(size=(R, sc.H, sc.W)).astype(np.float32)
+        g = _gpu_backward(sc, d, out, dre, dim, "reference")
+        acc = O.Grads(sc.P)
+        for r in range(R):
+            pre, keys, vals, ranges, fwd = O.render(sc, r)
+            O.backward(sc, pre, vals, ranges, fwd, dre[r], dim[r], O.GRAD_REFERENCE, r=r, grads=acc)
+        assert _rel(g[1], acc.means) < GRAD_TOL
+        assert _rel(g[2], acc.mlp) < GRAD_TOL
+        assert _rel(g[3], acc.opacities) < GRAD_TOL
+        assert _rel(g[5], acc.scales) < GRAD_TOL
+    finally:
+        for name in ("render_fwd_ppl", "render_bwd_ppl", "render_fwd_depth", "render_bwd_depth"):
+            gh.set_option(name, 0)
+
+
 def test_autograd_drop_in():
     """cuda_rasterizer API: same outputs and gradient order as the reference."""
     import cuda_rasterizer as cr

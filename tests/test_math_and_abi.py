@@ -148,6 +148,14 @@ def test_tuning_options():
         gh.set_option("render_fwd_ppl", 3)
     with pytest.raises(RuntimeError, match="unknown option"):
         gh.set_option("no_such_option", 1)
+    for name in ("render_fwd_depth", "render_bwd_depth"):
+        assert gh.get_option(name) == 0
+        for v in (32, 64, 128, 0):
+            gh.set_option(name, v)
+            assert gh.get_option(name) == v
+        for bad in (16, 48, 256, -1):
+            with pytest.raises(RuntimeError, match=name):
+                gh.set_option(name, bad)
 
 
 def test_reference_import_paths():
