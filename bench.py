@@ -243,14 +243,14 @@ def setup_receivers(args, dev):
             out = wr.forward_batched(means, mlp, opac, scales, rots, sc.H, sc.W, 1.0, views, txp, rxp)
         return wr.autotune(out, grad_mode=args.grad_mode)
 
-    def graphed():
+    def graphed(deterministic=False):
         """The same step (forward, fused loss, backward) captured in a HIP
         graph; tile lists sized for 1.25 K."""
         import gsparc_train as T
         cap = int(count() * 1.25) + 1024
         live = [x.clone() for x in (means, scales, rots, opac, mlp)]
         g = T.GraphedTrainingStep(*live, views, txp, rxp, target, sc.H, sc.W, capacity=cap,
-                                  grad_mode=args.grad_mode)
+                                  grad_mode=args.grad_mode, deterministic=deterministic)
         return g, live
 
     def cpu():
@@ -794,6 +794,35 @@ def main():
                                                     for k in ("adam", "densify_stats", "conf_loss") if k in kt2}}
         log(args, f"training iteration: {train_iter}")
 
+    det_step = None
+    if args.workload == "receivers" and not args.eager:
+        # the same graphed step with deterministic gradients (fixed-order sums)
+        dstep, _dlive = wl.graphed(deterministic=True)
+        for _ in range(args.warmup):
+            dstep.replay()
+        torch.cuda.synchronize()
+        el_det = timed(dstep.replay, args.steps)
+        if dstep.overflowed():
+            raise RuntimeError("deterministic step: tile lists exceeded their capacity")
+        for _ in range(3):
+            dstep.run_eager()
+        torch.cuda.synchronize()
+        gh.profile_enable(True, timing=True, counting=False)
+        for _ in range(nprof):
+            dstep.run_eager()
+        torch.cuda.synchronize()
+        kd = gh.profile_collect()
+        gh.profile_enable(False)
+        det_step = {"ms_per_step": round(1e3 * el_det / args.steps, 3),
+                    "spectra_per_s": round(R * args.steps / el_det, 2),
+                    "step": "hip graph: the timed step with fixed-order gradient sums (grad_mode | "
+                            "GSPARC_GRAD_DETERMINISTIC): one wave per tile storing each instance's sums, "
+                            "then a per-record gather",
+                    "kernels_ms_per_step": {k: round(kd[k][0] / nprof, 4)
+                                            for k in ("render_bwd", "det_clear", "det_offsets", "det_gather") if k in kd}}
+        log(args, f"deterministic step: {det_step}")
+        del dstep, _dlive
+
     cpu = wl.cpu() if args.cpu_receivers > 0 else None
 
     metric = {"receivers": METRIC,
@@ -829,6 +858,7 @@ def main():
                            if graph_value is not None else "HIP events around each kernel of the eager step"),
         "cpu_baseline": cpu,
         "training_iteration": train_iter,
+        "deterministic_step": det_step,
     }
     if getattr(wl, "mlp_rows", None):
         out["mlp_mfma"] = wl.mlp_rows()

@@ -233,8 +233,31 @@ struct RenderBwdArgs {
     const uint32_t* total;
     uint64_t Kcap;
     unsigned long long* counts;  // optional profiling counters
+    // deterministic form (inst8 == nullptr: the atomic one): [Kcap*8]
+    // per-instance sums, zero-filled by the caller; det_off [Nrec] the first
+    // slot of each record's run (launch_det_offsets), radii [Nrec] its radius;
+    // the kernel walks whole tiles and the record sums come from
+    // launch_det_gather
+    float* inst8;
+    const uint32_t* det_off;
+    const int* radii;
 };
 hipError_t launch_render_bwd(const Dims& d, const RenderBwdArgs& a, int ppl, int depth, hipStream_t s);
+
+// Deterministic gradient sums: per-record runs of instance slots.
+struct DetArgs {
+    const float4* splat;     // [Nrec*2]
+    const int* radii;        // [Nrec]
+    uint32_t* off;           // [Nrec] first slot of each record's run
+    uint32_t* bsum;          // [ceil(Nrec/1024)] block sums, then block bases
+    const float* inst8;      // [Kcap*8]
+    float* g8;               // [Nrec*8] (visible records written, the rest untouched)
+    const uint32_t* total;
+    uint64_t Kcap;
+};
+size_t det_scratch_words(const Dims& d);   // off + bsum
+hipError_t launch_det_offsets(const Dims& d, const DetArgs& a, hipStream_t s);
+hipError_t launch_det_gather(const Dims& d, const DetArgs& a, hipStream_t s);
 
 struct PreBwdArgs {
     const float* means;

@@ -352,6 +352,14 @@ size_t gsparc_grad_buffer_size(int P, int R)
     return e;
 }
 
+size_t gsparc_deterministic_grad_size(size_t grad_bytes, uint64_t K_capacity)
+{
+    // the per-instance slots, then one u32 per record and per 1024 records:
+    // the grad workspace holds 32 bytes per record, so an eighth of it bounds
+    // the records' words
+    return al(grad_bytes) + al((size_t)(K_capacity > 0 ? K_capacity : 1) * 32) + al(grad_bytes / 8) + 4096;
+}
+
 int gsparc_layout_query(int P, int R, int W, int H, uint64_t K, gsparc_layout* out)
 {
     if (!out) return fail(GSPARC_ERR_ARG, "null layout");
@@ -465,8 +473,11 @@ int backward_impl(int grad_mode, const gs::Dims& d, const FwdInputs& in, const f
                   void* geom, void* image, void* bin, uint64_t K_capacity, void* grad, const float* dout_real,
                   const float* dout_imag, const BwdOutputs& out, hipStream_t s, const GsCsi* csi = nullptr)
 {
+    const bool det = (grad_mode & GSPARC_GRAD_DETERMINISTIC) != 0;
+    grad_mode &= ~GSPARC_GRAD_DETERMINISTIC;
     if (grad_mode != GSPARC_GRAD_EXACT && grad_mode != GSPARC_GRAD_REFERENCE)
         return fail(GSPARC_ERR_ARG, "unknown grad_mode");
+    if (det && csi) return fail(GSPARC_ERR_ARG, "deterministic gradients: single-channel backward only");
     if (d.P == 0 || d.R == 0) return GSPARC_OK;
     if (!geom || !image || !bin || !grad || !in.means || !in.scales || !in.rotations || !in.views || !in.tx ||
         !in.rx || !radii || !dout_real || (!csi && !dout_imag) || !out.dmeans3D || !out.dopacity ||
@@ -520,7 +531,30 @@ int backward_impl(int grad_mode, const gs::Dims& d, const FwdInputs& in, const f
         ra.total = (const uint32_t*)(ib + L.total);
         ra.Kcap = K_capacity;
         ra.counts = g_prof_count ? g_counts_dev : nullptr;
+        ra.inst8 = nullptr;
+        ra.det_off = nullptr;
+        ra.radii = radii;
+        gs::DetArgs da;
+        if (det) {
+            // after the grad workspace: the per-instance slots, then each
+            // record's first slot and the scan's block sums
+            ra.inst8 = (float*)((char*)g8 + al(bytes));
+            uint32_t* off = (uint32_t*)((char*)ra.inst8 + al((size_t)std::max<uint64_t>(K_capacity, 1) * 32));
+            da.splat = ra.splat;
+            da.radii = radii;
+            da.off = off;
+            da.bsum = off + d.Nrec;
+            da.inst8 = ra.inst8;
+            da.g8 = g8;
+            da.total = ra.total;
+            da.Kcap = K_capacity;
+            ra.det_off = off;
+            ra.counts = nullptr;
+            GS_LAUNCH("det_clear", gs::launch_fill_u32((uint32_t*)ra.inst8, 0u, (size_t)K_capacity * 8, s), s);
+            GS_LAUNCH("det_offsets", gs::launch_det_offsets(d, da, s), s);
+        }
         GS_LAUNCH("render_bwd", gs::launch_render_bwd(d, ra, bwd_ppl(d), bwd_depth(d), s), s);
+        if (det) GS_LAUNCH("det_gather", gs::launch_det_gather(d, da, s), s);
     }
     gs::PreBwdArgs pa;
     pa.means = in.means; pa.scales = in.scales; pa.rots = in.rotations; pa.mlp = in.mlp; pa.views = in.views;
@@ -629,7 +663,8 @@ int autotune_impl(int grad_mode, const gs::Dims& d, const float* emission, const
     gs::RenderBwdArgs ba;
     ba.ranges = sa.ranges; ba.order = order; ba.plist = sa.plist; ba.splat = sa.splat; ba.sig = fa.sig;
     ba.finalT = fa.finalT; ba.ncontrib = fa.ncontrib; ba.dout_re = dout_real; ba.dout_im = dout_imag; ba.g8 = g8;
-    ba.mode = grad_mode; ba.total = sa.total; ba.Kcap = K_capacity; ba.counts = nullptr;
+    ba.mode = grad_mode; ba.total = sa.total; ba.Kcap = K_capacity; ba.counts = nullptr; ba.inst8 = nullptr;
+    ba.det_off = nullptr; ba.radii = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
     GS_HIP(hipEventCreate(&ev[0]));
     if (hipEventCreate(&ev[1]) != hipSuccess) {

@@ -238,9 +238,16 @@ __device__ __forceinline__ void bwd_step(bool c, float power, float dx, float dy
     }
 }
 
-template <int MODE, int PPL, int BD, bool COUNT>
+// DET: the deterministic form (PPL = 4, so that one wave walks each tile
+// list): each entry's sums are stored to the entry's instance slot instead
+// of added to the record's sums.  A record's slots are a run of its own
+// (a.det_off, the exclusive scan of the records' tile counts), its tiles in
+// row order; k_det_gather adds each run in that order.  Slots of entries
+// without a contribution keep the zeros the host filled them with.
+template <int MODE, int PPL, int BD, bool COUNT, bool DET = false>
 __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
 {
+    static_assert(!DET || PPL == 4, "the deterministic backward walks each tile with one wave");
     // BD list entries are staged per batch: NS per lane (BD = 64, 128), or
     // one on the first BD lanes (BD = 32)
     constexpr int NS = BD > GS_WAVE ? BD / GS_WAVE : 1;
@@ -315,6 +322,15 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
                 rec[b * GS_WAVE + lane].A = A;
                 rec[b * GS_WAVE + lane].B = B;
                 rec[b * GS_WAVE + lane].S = S;
+                if (DET) {
+                    // the instance's slot: the record's run start plus this
+                    // tile's row-major index inside the record's rect
+                    int x0, y0, x1, y1;
+                    get_rect(A.x, A.y, a.radii[gi], d.gx, d.gy, x0, y0, x1, y1);
+                    const int tx = pm.tile % d.gx, ty = pm.tile / d.gx;
+                    const uint32_t slot = a.det_off[gi] + (uint32_t)((ty - y0) * (x1 - x0) + (tx - x0));
+                    rec[b * GS_WAVE + lane].pad.x = __uint_as_float(slot);
+                }
                 // culled per 8x8 quarter: an entry is walked only over the
                 // quarters where it can reach alpha >= 1/255 and where some
                 // pixel's contributors reach back to it
@@ -388,10 +404,18 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
                 float tot = wave_sum8(v, lane);
                 // completed on all lanes, so that the last DPP step is one add
                 asm volatile("" : "+v"(tot));
-                // the record index is wave-uniform: its address arithmetic stays scalar
-                const uint32_t gid = __builtin_amdgcn_readfirstlane(__float_as_uint(B.w));
-                float* rp = g8r + (size_t)gid * 8;
-                if ((lane & 7) == 0) atomicAdd(rp + (lane >> 3), tot);
+                if (DET) {
+                    // the only wave walking this list: a plain store to the instance slot
+                    const uint32_t slot =
+                        __builtin_amdgcn_readfirstlane(__float_as_uint(rec[b * GS_WAVE + j].pad.x));
+                    float* ip = a.inst8 + (size_t)slot * 8;
+                    if ((lane & 7) == 0) ip[lane >> 3] = tot;
+                } else {
+                    // the record index is wave-uniform: its address arithmetic stays scalar
+                    const uint32_t gid = __builtin_amdgcn_readfirstlane(__float_as_uint(B.w));
+                    float* rp = g8r + (size_t)gid * 8;
+                    if ((lane & 7) == 0) atomicAdd(rp + (lane >> 3), tot);
+                }
             }
         }
         __syncthreads();
@@ -408,6 +432,116 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
             atomicAdd(a.counts + 3, c);
         }
     }
+}
+
+// ---------------------------------------------------------------- deterministic sums
+// The instance slots of the deterministic backward: record i (visible, with
+// the rect getRect gives its centre and radius, auxiliary.h:14-24, as the
+// binning placed it) owns the slots [off[i], off[i] + its tile count), its
+// tiles in row order; the counts of all records add up to K.  Three
+// launches: per 1024 records their counts' sum, one workgroup's exclusive
+// scan of those sums, then each block's exclusive scan from its base.
+__device__ __forceinline__ uint32_t det_tiles(const Dims& d, const DetArgs& a, size_t rec)
+{
+    const int radius = a.radii[rec];
+    if (radius <= 0) return 0u;
+    const float4 A = a.splat[2 * rec];
+    int x0, y0, x1, y1;
+    get_rect(A.x, A.y, radius, d.gx, d.gy, x0, y0, x1, y1);
+    return (uint32_t)((x1 - x0) * (y1 - y0));
+}
+
+__device__ __forceinline__ uint32_t det_block_scan(uint32_t v, uint32_t& total)
+{
+    __shared__ uint32_t s_wave[1024 / GS_WAVE];
+    const int wave = threadIdx.x / GS_WAVE, lane = threadIdx.x % GS_WAVE;
+    const uint32_t x = wave_incl_sum(v);
+    if (lane == GS_WAVE - 1) s_wave[wave] = x;
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+    for (int w = 0; w < 1024 / GS_WAVE; ++w) {
+        const uint32_t t = s_wave[w];
+        if (w < wave) before += t;
+        tot += t;
+    }
+    __syncthreads();
+    total = tot;
+    return before + x - v;
+}
+
+__global__ void __launch_bounds__(1024) k_det_count(Dims d, DetArgs a)
+{
+    const size_t rec = (size_t)blockIdx.x * 1024 + threadIdx.x;
+    const uint32_t n = rec < d.Nrec ? det_tiles(d, a, rec) : 0u;
+    uint32_t total;
+    det_block_scan(n, total);
+    if (threadIdx.x == 0) a.bsum[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(1024) k_det_scan(uint32_t nb, DetArgs a)
+{
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < nb; b0 += 1024) {
+        const uint32_t b = b0 + threadIdx.x;
+        const uint32_t v = b < nb ? a.bsum[b] : 0u;
+        uint32_t total;
+        const uint32_t ex = det_block_scan(v, total);
+        if (b < nb) a.bsum[b] = carry + ex;
+        carry += total;
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_det_offsets(Dims d, DetArgs a)
+{
+    const size_t rec = (size_t)blockIdx.x * 1024 + threadIdx.x;
+    const uint32_t n = rec < d.Nrec ? det_tiles(d, a, rec) : 0u;
+    uint32_t total;
+    const uint32_t ex = det_block_scan(n, total);
+    if (rec < d.Nrec) a.off[rec] = a.bsum[blockIdx.x] + ex;
+}
+
+// Each visible record's run of instance slots summed in order into its raw
+// sums g8 (the record layout of the atomic form); the result does not depend
+// on timing.  Neighbouring threads read neighbouring runs.
+__global__ void __launch_bounds__(256) k_det_gather(Dims d, DetArgs a)
+{
+    const size_t rec = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (rec >= d.Nrec) return;
+    if ((uint64_t)a.total[0] > a.Kcap) return;   // no lists: the gradient stays cleared
+    const uint32_t n = det_tiles(d, a, rec);
+    if (n == 0) return;
+    const float4* ip = reinterpret_cast<const float4*>(a.inst8) + (size_t)a.off[rec] * 2;
+    float4 s0 = ip[0], s1 = ip[1];
+    for (uint32_t k = 1; k < n; ++k) {
+        const float4 u = ip[2 * k], v = ip[2 * k + 1];
+        s0.x += u.x; s0.y += u.y; s0.z += u.z; s0.w += u.w;
+        s1.x += v.x; s1.y += v.y; s1.z += v.z; s1.w += v.w;
+    }
+    float4* gp = reinterpret_cast<float4*>(a.g8 + rec * 8);
+    gp[0] = s0;
+    gp[1] = s1;
+}
+
+size_t det_scratch_words(const Dims& d)
+{
+    return d.Nrec + (d.Nrec + 1023) / 1024;
+}
+
+hipError_t launch_det_offsets(const Dims& d, const DetArgs& a, hipStream_t s)
+{
+    if (d.Nrec == 0) return hipSuccess;
+    const unsigned nb = (unsigned)((d.Nrec + 1023) / 1024);
+    hipLaunchKernelGGL(k_det_count, dim3(nb), dim3(1024), 0, s, d, a);
+    hipLaunchKernelGGL(k_det_scan, dim3(1), dim3(1024), 0, s, (uint32_t)nb, a);
+    hipLaunchKernelGGL(k_det_offsets, dim3(nb), dim3(1024), 0, s, d, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_det_gather(const Dims& d, const DetArgs& a, hipStream_t s)
+{
+    if (d.Nrec == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_det_gather, dim3((unsigned)((d.Nrec + 255) / 256)), dim3(256), 0, s, d, a);
+    return hipGetLastError();
 }
 
 static inline dim3 render_grid(const Dims& d, int ppl)
@@ -474,6 +608,14 @@ static void launch_bwd_mode(const Dims& d, const RenderBwdArgs& a, int ppl, int 
 hipError_t launch_render_bwd(const Dims& d, const RenderBwdArgs& a, int ppl, int depth, hipStream_t s)
 {
     if (d.R == 0 || d.T == 0) return hipSuccess;
+    if (a.inst8) {   // deterministic: one wave per tile, batch depth 64, no counters
+        const dim3 grid = render_grid(d, 4);
+        if (a.mode == GRAD_REFERENCE)
+            hipLaunchKernelGGL((k_render_bwd<GRAD_REFERENCE, 4, 64, false, true>), grid, dim3(GS_WAVE), 0, s, d, a);
+        else
+            hipLaunchKernelGGL((k_render_bwd<GRAD_EXACT, 4, 64, false, true>), grid, dim3(GS_WAVE), 0, s, d, a);
+        return hipGetLastError();
+    }
     depth = choose_depth(depth);
     const bool count = a.counts != nullptr;
     if (a.mode == GRAD_REFERENCE) {

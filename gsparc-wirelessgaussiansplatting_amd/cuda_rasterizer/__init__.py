@@ -12,7 +12,10 @@ Same names, argument order, outputs and gradient order:
                                       opacities, scales=None, rotations=None)
 
 Additions: GaussianRasterizerSetting.grad_mode ("reference", the default,
-reproduces the reference backward; "exact" is the true adjoint), the
+reproduces the reference backward; "exact" is the true adjoint),
+GaussianRasterizerSetting.deterministic (False, the default, sums the
+gradients with float atomics as the reference does; True sums them in a
+fixed order, bit-reproducible from run to run), the
 `rasterize_gaussians_batched` function for many receivers per launch, and
 `emission=` for precomputed per-Gaussian emission.
 """
@@ -53,7 +56,8 @@ class _RasterizeGaussians(torch.autograd.Function):
         g2, g3, gmlp, gop, _gcov, gsc, grot = wr.rasterize_gaussians_backward(
             means3D, emission_mlps, radii, out_real, out_imag, opacities, scales, rotations, s.scale_modifier,
             s.viewmatrix, s.projmatrix, grad_out_real, grad_out_imag, s.tx_pos, s.rx_pos, geom, ctx.num_rendered,
-            binbuf, img, grad_mode=getattr(s, "grad_mode", "reference"))
+            binbuf, img, grad_mode=getattr(s, "grad_mode", "reference"),
+            deterministic=bool(getattr(s, "deterministic", False)))
         return g3, g2, gmlp.reshape(emission_mlps.shape), gop.reshape(opacities.shape), gsc, grot, None
 
 
@@ -66,6 +70,7 @@ class GaussianRasterizerSetting(NamedTuple):
     tx_pos: torch.Tensor
     rx_pos: torch.Tensor
     grad_mode: str = "reference"
+    deterministic: bool = False
 
 
 class GaussianRasterizer(nn.Module):
@@ -118,6 +123,8 @@ class BatchedSetting(NamedTuple):
     # forms refuse torch.cuda.graph capture: a captured step is
     # gsparc_train.GraphedTrainingStep.
     capacity: int = 0
+    # fixed-order gradient sums (bit-reproducible) instead of float atomics
+    deterministic: bool = False
 
 
 class _RasterizeBatched(torch.autograd.Function):
@@ -144,7 +151,7 @@ class _RasterizeBatched(torch.autograd.Function):
         em = em_or_mlp if ctx.has_emission else None
         d2, d3, dmlp, dop, _dcov, dsc, drot, dem = wr.backward_batched(
             means3D, mlp, radii, opacities, scales, rotations, s.scale_modifier, s.viewmatrices, gre, gim, s.tx_pos,
-            s.rx_pos, geom, ctx.K, binbuf, img, grad_mode=s.grad_mode, emission=em)
+            s.rx_pos, geom, ctx.K, binbuf, img, grad_mode=s.grad_mode, emission=em, deterministic=s.deterministic)
         return (d3, None if dmlp is None else dmlp.reshape(mlp.shape), None if dem is None else dem.reshape(em.shape),
                 dop.reshape(opacities.shape), dsc, drot, None)
 
@@ -170,6 +177,7 @@ class SceneSetting(NamedTuple):
     rx_pos: torch.Tensor
     grad_mode: str = "reference"
     capacity: int = 0
+    deterministic: bool = False
 
 
 class _RasterizeScenes(torch.autograd.Function):
@@ -196,7 +204,8 @@ class _RasterizeScenes(torch.autograd.Function):
         em = em_or_mlp if ctx.has_emission else None
         d2, d3, dmlp, dop, _dcov, dsc, drot, dem = wr.backward_scenes(
             means3D, mlp, radii, opacities, scales, rotations, s.scale_modifier, s.viewmatrices, gre, gim, s.tx_pos,
-            s.rx_pos, s.scenes, geom, ctx.K, binbuf, img, grad_mode=s.grad_mode, emission=em)
+            s.rx_pos, s.scenes, geom, ctx.K, binbuf, img, grad_mode=s.grad_mode, emission=em,
+            deterministic=s.deterministic)
         return (d3, None if dmlp is None else dmlp.reshape(mlp.shape), None if dem is None else dem.reshape(em.shape),
                 dop.reshape(opacities.shape), dsc, drot, None)
 

@@ -17,7 +17,15 @@ LIB_PATH = os.path.join(_HERE, "libgsparc.so")
 GRAD_EXACT = 0
 GRAD_REFERENCE = 1
 GRAD_MODES = {"exact": GRAD_EXACT, "reference": GRAD_REFERENCE}
+GRAD_DETERMINISTIC = 16   # OR-ed into a grad mode: fixed summation order, bit-reproducible gradients
 FLAG_KEEP_KEYS = 1
+
+
+def grad_mode_code(grad_mode, deterministic=False):
+    """The C ABI's grad_mode for "exact" / "reference" (or an int), with the
+    deterministic bit when asked."""
+    m = GRAD_MODES[grad_mode] if isinstance(grad_mode, str) else int(grad_mode)
+    return m | GRAD_DETERMINISTIC if deterministic else m
 
 _vp = C.c_void_p
 _i = C.c_int
@@ -83,6 +91,7 @@ _SIGS = {
     "gsparc_image_buffer_size": (_sz, [_i, _i, _i, _i]),
     "gsparc_binning_buffer_size": (_sz, [_u64]),
     "gsparc_grad_buffer_size": (_sz, [_i, _i]),
+    "gsparc_deterministic_grad_size": (_sz, [_sz, _u64]),
     "gsparc_layout_query": (_i, [_i, _i, _i, _i, _u64, C.POINTER(Layout)]),
     "gsparc_rasterize_forward_geom": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f, _vp, _vp,
                                            _vp, C.POINTER(_u64), _vp]),

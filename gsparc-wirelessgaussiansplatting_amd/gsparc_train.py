@@ -273,12 +273,14 @@ class GraphedTrainingStep(_GraphedStep):
 
     def __init__(self, means, scales, rotations, opacities, emission_mlps, viewmatrices, tx_pos, rx_pos, target,
                  image_height, image_width, capacity, scale_modifier=1.0, grad_mode="reference",
-                 lambda_ssim=LAMBDA_SSIM, confidence=None, confidence_hidden=32, confidence_alpha=CONFIDENCE_ALPHA):
+                 lambda_ssim=LAMBDA_SSIM, confidence=None, confidence_hidden=32, confidence_alpha=CONFIDENCE_ALPHA,
+                 deterministic=False):
         """confidence: None (the plain loss) or the packed ConfidenceMLP
         [5 hidden + 1] as a live tensor (ConfidenceMLP.packed() copied into
         it; update it in place): the loss is then the confidence-weighted one
         and grads["confidence"] its gradient, `rec_loss` / `conf` the per
-        spectrum L_r and C_r."""
+        spectrum L_r and C_r.  deterministic: fixed-order gradient sums
+        (bit-reproducible replays) instead of float atomics."""
         self.conf = None if confidence is None else _require_param(confidence, "confidence")
         self.conf_hidden = int(confidence_hidden)
         self.conf_alpha = float(confidence_alpha)
@@ -302,7 +304,7 @@ class GraphedTrainingStep(_GraphedStep):
             raise RuntimeError("emission_mlps must hold 194 floats per Gaussian")
         self.P, self.R, self.H, self.W = P, R, H, W
         self.capacity = int(capacity)
-        self.mode = gh.GRAD_MODES[grad_mode] if isinstance(grad_mode, str) else int(grad_mode)
+        self.mode = gh.grad_mode_code(grad_mode, deterministic)
         self.scale_modifier = float(scale_modifier)
         self.lambda_ssim = float(lambda_ssim)
         L = gh.lib()
@@ -311,7 +313,8 @@ class GraphedTrainingStep(_GraphedStep):
         self.geom = u8(L.gsparc_geom_buffer_size(P, R, W, H))
         self.img = u8(L.gsparc_image_buffer_size(P, R, W, H))
         self.bin = u8(L.gsparc_binning_buffer_size(self.capacity))
-        self.gradbuf = u8(L.gsparc_grad_buffer_size(P, R))
+        gsize = L.gsparc_grad_buffer_size(P, R)
+        self.gradbuf = u8(L.gsparc_deterministic_grad_size(gsize, self.capacity) if deterministic else gsize)
         self.work = _work(R, H, W, dev)
         self.radii = torch.empty((R, P), dtype=torch.int32, device=dev)
         self.out_real = torch.empty((R, H, W), **f32)

@@ -131,17 +131,19 @@ def instance_count(P, R, image_height, image_width, imageBuffer):
 
 def backward_batched(means3D, emission_mlps, radii, opacities, scales, rotations, scale_modifier,
                      viewmatrices, dL_dout_real, dL_dout_imag, tx_pos, rx_pos, geomBuffer, num_rendered,
-                     binningBuffer, imageBuffer, grad_mode="reference", emission=None):
+                     binningBuffer, imageBuffer, grad_mode="reference", emission=None, deterministic=False):
     """Backward for R receivers; gradients are summed over receivers.
     Returns (dmeans2D [R,P,3], dmeans3D [P,3], demission_mlps [M] or None,
     dopacity [P,1], dcov3D [P,6], dscales [P,3], drotations [P,4],
-    demission [R,P,2] or None)."""
+    demission [R,P,2] or None).  deterministic=True sums every gradient in a
+    fixed order (bit-reproducible; 32 extra bytes of workspace per tile-list
+    instance) instead of with the reference's float atomics."""
     P = int(means3D.size(0))
     R = int(dL_dout_real.size(0))
     H, W = int(dL_dout_real.size(1)), int(dL_dout_real.size(2))
     dev = means3D.device
     opts = dict(dtype=torch.float32, device=dev)
-    mode = gh.GRAD_MODES[grad_mode] if isinstance(grad_mode, str) else int(grad_mode)
+    mode = gh.grad_mode_code(grad_mode, deterministic)
     M = int(emission_mlps.numel()) if (emission_mlps is not None and emission is None) else 0
     dmeans2D = torch.zeros((R, P, 3), **opts)
     dmeans3D = torch.zeros((P, 3), **opts)
@@ -166,7 +168,10 @@ def backward_batched(means3D, emission_mlps, radii, opacities, scales, rotations
     gim = gh.f32(dL_dout_imag)
     gh.require_cuda(means, sc, rot, views, rxs, tx, mlp, em, rad, gre, gim)
     L = gh.lib()
-    grad = _empty_bytes(dev, L.gsparc_grad_buffer_size(P, R))
+    gsize = L.gsparc_grad_buffer_size(P, R)
+    if deterministic:
+        gsize = L.gsparc_deterministic_grad_size(gsize, int(num_rendered))
+    grad = _empty_bytes(dev, gsize)
     gh.check(L.gsparc_rasterize_backward(mode, P, R, W, H, gh.ptr(means), gh.ptr(sc), gh.ptr(rot), gh.ptr(mlp),
                                         gh.ptr(em), gh.ptr(views), gh.ptr(tx), gh.ptr(rxs), float(scale_modifier),
                                         gh.ptr(rad), gh.ptr(geomBuffer), gh.ptr(imageBuffer), gh.ptr(binningBuffer),
@@ -191,16 +196,16 @@ def rasterize_gaussians(means3D, emission_mlps, opacities, scales, rotations, im
 def rasterize_gaussians_backward(means3D, emission_mlps, radii, out_signal_real, out_signal_imag, opacities, scales,
                                  rotations, scale_modifier, viewmatrix, projmatrix, dL_dout_signal_real,
                                  dL_dout_signal_imag, tx_pos, rx_pos, geomBuffer, R, binningBuffer, imageBuffer,
-                                 *, grad_mode="reference", emission=None):
+                                 *, grad_mode="reference", emission=None, deterministic=False):
     """One receiver, reference tuple layout (rasterize_points.cu:106-199); R is
-    num_rendered from the forward."""
+    num_rendered from the forward.  deterministic: as in backward_batched."""
     P = int(means3D.size(0))
     em = None if emission is None else emission.reshape(1, -1, 2)
     d2, d3, dmlp, dop, dcov, dsc, drot, dem = backward_batched(
         means3D, emission_mlps, radii.reshape(1, -1) if P else radii, opacities, scales, rotations, scale_modifier,
         viewmatrix.reshape(1, 4, 4), dL_dout_signal_real.reshape(1, *dL_dout_signal_real.shape[-2:]),
         dL_dout_signal_imag.reshape(1, *dL_dout_signal_imag.shape[-2:]), tx_pos, rx_pos.reshape(1, 3), geomBuffer, R,
-        binningBuffer, imageBuffer, grad_mode=grad_mode, emission=em)
+        binningBuffer, imageBuffer, grad_mode=grad_mode, emission=em, deterministic=deterministic)
     if dmlp is None:
         dmlp = dem.reshape(P, 2)
     return d2.reshape(P, 3), d3, dmlp, dop, dcov, dsc, drot
@@ -323,14 +328,15 @@ def forward_scenes(means3D, emission_mlps, opacities, scales, rotations, image_h
 
 def backward_scenes(means3D, emission_mlps, radii, opacities, scales, rotations, scale_modifier, viewmatrices,
                     dL_dout_real, dL_dout_imag, tx_pos, rx_pos, scenes: SceneBatch, geomBuffer, num_rendered,
-                    binningBuffer, imageBuffer, grad_mode="reference", emission=None):
+                    binningBuffer, imageBuffer, grad_mode="reference", emission=None, deterministic=False):
     """Backward of forward_scenes.  Returns (dmeans2D [Nrec,3], dmeans3D
     [Ptot,3], demission_mlps [Ptot*194] or None, dopacity [Ptot,1], dcov3D
-    [Ptot,6], dscales [Ptot,3], drotations [Ptot,4], demission [Nrec,2] or None)."""
+    [Ptot,6], dscales [Ptot,3], drotations [Ptot,4], demission [Nrec,2] or None).
+    deterministic: as in backward_batched."""
     H, W = int(dL_dout_real.size(1)), int(dL_dout_real.size(2))
     dev = means3D.device
     opts = dict(dtype=torch.float32, device=dev)
-    mode = gh.GRAD_MODES[grad_mode] if isinstance(grad_mode, str) else int(grad_mode)
+    mode = gh.grad_mode_code(grad_mode, deterministic)
     P = scenes.Ptot
     dmeans2D = torch.zeros((scenes.Nrec, 3), **opts)
     dmeans3D = torch.zeros((P, 3), **opts)
@@ -349,7 +355,10 @@ def backward_scenes(means3D, emission_mlps, radii, opacities, scales, rotations,
     rad = radii.reshape(-1).to(torch.int32).contiguous()
     gre, gim = gh.f32(dL_dout_real), gh.f32(dL_dout_imag)
     gh.require_cuda(means, sc, rot, views, rxs, tx, mlp, em, rad, gre, gim)
-    grad = _empty_bytes(dev, scenes.sizes(H, W)[3])
+    gsize = scenes.sizes(H, W)[3]
+    if deterministic:
+        gsize = gh.lib().gsparc_deterministic_grad_size(gsize, int(num_rendered))
+    grad = _empty_bytes(dev, gsize)
     gh.check(gh.lib().gsparc_rasterize_backward_scenes(
         mode, C.byref(scenes.batch), gh.ptr(scenes.table), W, H, gh.ptr(means), gh.ptr(sc), gh.ptr(rot),
         gh.ptr(mlp), gh.ptr(em), gh.ptr(views), gh.ptr(tx), gh.ptr(rxs), float(scale_modifier), gh.ptr(rad),

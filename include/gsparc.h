@@ -38,6 +38,16 @@ extern "C" {
 
 #define GSPARC_GRAD_EXACT 0     /* true adjoint of the forward */
 #define GSPARC_GRAD_REFERENCE 1 /* the reference backward, quirks included */
+/* OR-ed into grad_mode of gsparc_rasterize_backward(_scenes): deterministic
+ * gradients.  The blend adjoint stores each tile-list instance's sums to a
+ * slot of its own (one wave per tile) instead of adding them to the record
+ * with float atomics (the reference's atomicAdd, backward.cu:588-613, whose
+ * summation order changes from run to run); each record's instances are
+ * then summed in a fixed order (its tiles row by row).  Same inputs give
+ * bit-identical gradients.  `grad` must hold
+ * gsparc_deterministic_grad_size(normal grad bytes, K_capacity) bytes.  No
+ * counterpart in the reference, which has no deterministic mode. */
+#define GSPARC_GRAD_DETERMINISTIC 16
 
 #define GSPARC_FLAG_KEEP_KEYS 1 /* also store the sorted depth keys */
 
@@ -109,6 +119,10 @@ size_t gsparc_geom_buffer_size(int P, int R, int W, int H);
 size_t gsparc_image_buffer_size(int P, int R, int W, int H);
 size_t gsparc_binning_buffer_size(uint64_t K);
 size_t gsparc_grad_buffer_size(int P, int R);
+/* grad workspace of a deterministic backward: the normal size (P, R form or
+ * gsparc_scene_buffer_sizes' sizes[3]) rounded up to 256 bytes, plus 32
+ * bytes per tile-list instance the bin buffer can hold */
+size_t gsparc_deterministic_grad_size(size_t grad_bytes, uint64_t K_capacity);
 int gsparc_layout_query(int P, int R, int W, int H, uint64_t K, gsparc_layout* out);
 
 /* Forward, part 1: per-Gaussian preprocess for R receivers (projection,

@@ -158,6 +158,22 @@ def test_tuning_options():
                 gh.set_option(name, bad)
 
 
+def test_deterministic_workspace_size():
+    """A deterministic backward's workspace: the normal one rounded up to
+    256 bytes, 8 floats per tile-list instance, and a word per record and per
+    1024 records (bounded by an eighth of the normal size)."""
+    import gsparc_hip as gh
+    L = gh.lib()
+    assert L.gsparc_deterministic_grad_size(1000, 10) == 1024 + 512 + 256 + 4096
+    assert L.gsparc_deterministic_grad_size(1024, 0) == 1024 + 256 + 256 + 4096
+    P, R, K = 50000, 64, 15_740_375
+    g = L.gsparc_grad_buffer_size(P, R)
+    assert g >= 32 * P * R
+    assert L.gsparc_deterministic_grad_size(g, K) >= g + 32 * K + 4 * (P * R + P * R // 1024 + 1)
+    assert gh.grad_mode_code("exact", True) == 16 and gh.grad_mode_code("reference", True) == 17
+    assert gh.grad_mode_code("reference") == 1
+
+
 def test_reference_import_paths():
     """GSpaRC's own import lines resolve to this build (renderer.py:5, train.py:40)."""
     from cuda_kernel_rasterize.cuda_rasterizer import GaussianRasterizer, GaussianRasterizerSetting
