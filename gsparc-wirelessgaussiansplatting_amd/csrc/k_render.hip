@@ -363,10 +363,12 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
                     dyr[g] = A.y - pyf[g * (PPL / NROW)];
                     hdd[g] = (B.x * dyr[g]) * dyr[g];
                 }
-                // the entry's sums: written by its first step with a contributor
-                // (quarter 0) or cleared before the first (a later quarter), so
-                // nothing is carried from the previous entry
-                BwdSums s;
+                // the entry's sums: written by quarter 0's step, or added to
+                // from zero by a later quarter's.  Defined on every path, so
+                // that no value merges in from the previous entry: with an
+                // undefined start the compiler copies the sums before the
+                // in-place lane swaps of every reduction (8 moves each)
+                BwdSums s{};
                 bool have = false;   // wave-uniform: a quarter of this entry had a contributor
 #pragma unroll
                 for (int k = 0; k < PPL; ++k) {
@@ -382,7 +384,6 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd(Dims d, RenderBwdArgs a)
                     if (k == 0) {
                         bwd_step<MODE, true>(c, power, dx, dyr[g], B.y, S, T[k], acc_re[k], acc_im[k], gre[k], gim[k], s);
                     } else {
-                        if (!have) s = BwdSums{};
                         bwd_step<MODE, false>(c, power, dx, dyr[g], B.y, S, T[k], acc_re[k], acc_im[k], gre[k], gim[k], s);
                     }
                     have = true;
