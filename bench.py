@@ -650,10 +650,12 @@ def main():
     # kernel of the timed step: the graph's own launch sequence run without
     # the graph (the same kernels with the same arguments), else the eager
     # step; then (separately: the counting variants are slower) the blend
-    # kernels' pair counts.  After the host-bound plain and latency runs a few
-    # steps first bring the clocks back to where the timed steps ran.
+    # kernels' pair counts.  After the host-bound plain and latency runs the
+    # clocks need about 40 steps to come back to where the timed steps ran
+    # (with 5, the blend kernels read 7-10% slower and the kernels' sum
+    # exceeded the graphed step by 7%).
     prof_step = gstep.run_eager if gstep is not None else wl.step
-    for _ in range(max(3, args.warmup)):
+    for _ in range(max(40, args.warmup)):
         prof_step()
     torch.cuda.synchronize()
     nprof = 5
