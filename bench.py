@@ -50,6 +50,14 @@ PEAK_FP32_TFLOPS = 157.3
 FWD_FLOP_PER_PAIR = 15      # per pair the reference's forward visits
 BWD_FLOP_PER_PAIR = 15      # per pair the reference's backward walks
 BWD_FLOP_PER_CONTRIB = 25   # more per contributing pair (backward)
+CSI_FWD_FLOP_PER_CONTRIB = 2 * 2 * 52          # multi-frequency forward: 104 FMAs per contributing pair
+CSI_BWD_FLOP_PER_CONTRIB = 25 + 2 * 6 * 52     # multi-frequency adjoint: 6 FMAs per subcarrier and pair
+FLOP_PER_PAIR_TEXT = {
+    "render_fwd": f"{FWD_FLOP_PER_PAIR} per visited pair",
+    "render_bwd": f"{BWD_FLOP_PER_PAIR} per walked pair + {BWD_FLOP_PER_CONTRIB} per contributing pair",
+    "render_fwd_csi": f"{FWD_FLOP_PER_PAIR} per visited pair + {CSI_FWD_FLOP_PER_CONTRIB} per contributing pair",
+    "render_bwd_csi": f"{BWD_FLOP_PER_PAIR} per walked pair + {CSI_BWD_FLOP_PER_CONTRIB} per contributing pair",
+}
 
 
 def parse():
@@ -436,6 +444,26 @@ def setup_csi(args, dev):
     def count():
         return wr.forward_csi(means, mlp, opac, scales, rots, sc.H, sc.W, 1.0, views, txp, rxp)[0]
 
+    def pair_count_step():
+        """The single-channel blend forward and backward on the same geometry:
+        they walk exactly the pairs the multi-frequency kernels walk (alpha, T
+        and termination do not depend on the subcarrier), and their counting
+        variants count them."""
+        with torch.no_grad():
+            out = wr.forward_batched(means, mlp, opac, scales, rots, sc.H, sc.W, 1.0, views, txp, rxp)
+            K_, ore, oim, radii, geom, binbuf, img = out
+            g = torch.ones_like(ore)
+            wr.backward_batched(means, mlp, radii, opac, scales, rots, 1.0, views, g, g, txp, rxp, geom, K_, binbuf,
+                                img, grad_mode="exact")
+
+    def graphed(deterministic=False):
+        """The same step (CSI forward, CSI loss, exact adjoint) captured in a
+        HIP graph; tile lists sized for 1.25 K."""
+        cap = int(count() * 1.25) + 1024
+        live = [x.clone() for x in (means, scales, rots, opac, mlp)]
+        g = T.GraphedCsiStep(*live, views, txp, rxp, target.contiguous(), sc.H, sc.W, capacity=cap)
+        return g, live
+
     def cpu():
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
@@ -461,7 +489,8 @@ def setup_csi(args, dev):
             O.backward_csi(sc, pre, plen, vals, ranges, fwd, dout[0].astype(np.float32), r, grads=acc)
 
     return Workload(
-        step=step, step_plain=step_plain, single=single, count=count, cpu=cpu, graphed=None, tune=None,
+        step=step, step_plain=step_plain, single=single, count=count, cpu=cpu, graphed=graphed, tune=None,
+        pair_count_step=pair_count_step,
         spectra=R, Ptot=P, Nrec=P * R, H=sc.H, W=sc.W, traffic_key=None, csi=True,
         data="synthetic (seeded: 70% wall-clustered Gaussians, receivers uniform in a 6x6x3 m room)",
         config={"workload": "multi-frequency CSI, fwd+bwd, 0.8*L1(complex) + 0.2*(1-SSIM(magnitude))",
@@ -665,8 +694,9 @@ def main():
     torch.cuda.synchronize()
     kt = gh.profile_collect()
     gh.profile_enable(True, timing=False, counting=True)
+    count_step = getattr(wl, "pair_count_step", None) or prof_step
     for _ in range(nprof):
-        prof_step()
+        count_step()
     torch.cuda.synchronize()
     cnt = {k: v / nprof for k, v in gh.profile_counts().items()}
     gh.profile_enable(False)
@@ -723,6 +753,15 @@ def main():
     valu = {"render_fwd": (cnt["fwd_pairs"] * FWD_FLOP_PER_PAIR, cnt["fwd_pairs"], cnt["fwd_contrib"]),
             "render_bwd": (cnt["bwd_pairs"] * BWD_FLOP_PER_PAIR + cnt["bwd_contrib"] * BWD_FLOP_PER_CONTRIB,
                            cnt["bwd_pairs"], cnt["bwd_contrib"])}
+    if getattr(wl, "csi", False):
+        # the same walk with 52 complex channels: the forward accumulates
+        # C_k += w e_k (104 FMAs per contributing pair), the adjoint forms
+        # Z = sum_k conj(G_k) u_k (4 FMAs per subcarrier) and Y = sum_k f_k p_k
+        # (2 FMAs per subcarrier) per contributing pair
+        valu = {"render_fwd_csi": (cnt["fwd_pairs"] * FWD_FLOP_PER_PAIR + cnt["fwd_contrib"] * CSI_FWD_FLOP_PER_CONTRIB,
+                                   cnt["fwd_pairs"], cnt["fwd_contrib"]),
+                "render_bwd_csi": (cnt["bwd_pairs"] * BWD_FLOP_PER_PAIR + cnt["bwd_contrib"] * CSI_BWD_FLOP_PER_CONTRIB,
+                                   cnt["bwd_pairs"], cnt["bwd_contrib"])}
     traffic = measured_traffic(args, *wl.traffic_key) if wl.traffic_key else {}
     kernels = {}
     for name, (ms, n) in kt.items():
@@ -758,8 +797,7 @@ def main():
     if dom in valu:
         roof.update(flop_per_launch=d["flop_per_launch"], pairs_per_launch=d["pairs_per_launch"],
                     contributing_pairs_per_launch=d["contributing_pairs_per_launch"],
-                    flop_per_pair=FWD_FLOP_PER_PAIR if dom.endswith("fwd") else
-                    f"{BWD_FLOP_PER_PAIR} per walked pair + {BWD_FLOP_PER_CONTRIB} per contributing pair",
+                    flop_per_pair=FLOP_PER_PAIR_TEXT[dom],
                     hbm_gbs=d.get("hbm_gbs"), hbm_frac=d.get("hbm_frac"))
     if roof["traffic"] is not None:
         roof["traffic_source"] = TRAFFIC_FILE

@@ -377,6 +377,90 @@ class GraphedTrainingStep(_GraphedStep):
         return wr.instance_count(self.P, self.R, self.H, self.W, self.img)
 
 
+class GraphedCsiStep(_GraphedStep):
+    """GraphedTrainingStep for the multi-frequency maps: the CSI forward of R
+    receivers with a tile-list capacity, the CSI loss (L1 on the complex map
+    plus SSIM on the 52 magnitude maps) and the exact adjoint, captured once
+    and replayed.  Parameters (means [P,3], scales [P,3], rotations [P,4],
+    opacities [P,1], emission_mlps [P*194]) and target [R,H,W,52,2] are read
+    in place at every replay; after replay() `loss` [1], `out` [R,H,W,52,2],
+    `radii` [R,P] and `grads` hold the step's results.  No counterpart in the
+    reference, which renders one channel (config.h:15)."""
+
+    def __init__(self, means, scales, rotations, opacities, emission_mlps, viewmatrices, tx_pos, rx_pos, target,
+                 image_height, image_width, capacity, scale_modifier=1.0, lambda_ssim=LAMBDA_SSIM,
+                 f0=None, df=None):
+        self.params = dict(means=_require_param(means, "means"), scales=_require_param(scales, "scales"),
+                           rotations=_require_param(rotations, "rotations"),
+                           opacities=_require_param(opacities, "opacities"),
+                           emission_mlps=_require_param(emission_mlps, "emission_mlps"))
+        if self.params["emission_mlps"].data_ptr() % 16:
+            raise RuntimeError("emission_mlps must be 16-byte aligned")
+        dev = means.device
+        P = int(means.size(0))
+        self.views = gh.f32(viewmatrices).reshape(-1, 16)
+        self.rx = gh.f32(rx_pos).reshape(-1, 3)
+        self.tx = gh.f32(tx_pos).reshape(3)
+        R = int(self.rx.size(0))
+        H, W = int(image_height), int(image_width)
+        NC = wr.CSI_NC
+        self.target = _require_param(target, "target")
+        if tuple(self.target.shape) != (R, H, W, NC, 2):
+            raise RuntimeError(f"target must be [R,H,W,{NC},2]")
+        if self.views.size(0) != R:
+            raise RuntimeError("one view matrix per receiver position is required")
+        if self.params["emission_mlps"].numel() != P * 194:
+            raise RuntimeError("emission_mlps must hold 194 floats per Gaussian")
+        self.P, self.R, self.H, self.W = P, R, H, W
+        self.f0 = float(wr.CSI_F0 if f0 is None else f0)
+        self.df = float(wr.CSI_DF if df is None else df)
+        self.capacity = int(capacity)
+        self.scale_modifier = float(scale_modifier)
+        self.lambda_ssim = float(lambda_ssim)
+        L = gh.lib()
+        f32 = dict(dtype=torch.float32, device=dev)
+        u8 = lambda n: torch.empty(int(n), dtype=torch.uint8, device=dev)
+        self.geom = u8(L.gsparc_geom_buffer_size(P, R, W, H))
+        self.img = u8(L.gsparc_image_buffer_size(P, R, W, H))
+        self.bin = u8(L.gsparc_binning_buffer_size(self.capacity))
+        self.gradbuf = u8(L.gsparc_grad_buffer_size(P, R))
+        self.work = u8(L.gsparc_csi_loss_work_size(R, H, W))
+        self.radii = torch.empty((R, P), dtype=torch.int32, device=dev)
+        self.out = torch.empty((R, H, W, NC, 2), **f32)
+        self.loss = torch.empty(1, **f32)
+        self.dout = torch.empty((R, H, W, NC, 2), **f32)
+        self.grads = dict(means=torch.empty((P, 3), **f32), emission_mlps=torch.empty(P * 194, **f32),
+                          opacities=torch.empty((P, 1), **f32), scales=torch.empty((P, 3), **f32),
+                          rotations=torch.empty((P, 4), **f32))
+        self._capture(dev)
+
+    def _record(self, stream):
+        L = gh.lib()
+        p, g = self.params, self.grads
+        P, R, W, H = self.P, self.R, self.W, self.H
+        gh.check(L.gsparc_rasterize_forward_csi(P, R, W, H, self.f0, self.df, gh.ptr(p["means"]),
+                                                gh.ptr(p["scales"]), gh.ptr(p["rotations"]),
+                                                gh.ptr(p["opacities"]), gh.ptr(p["emission_mlps"]), None,
+                                                gh.ptr(self.views), gh.ptr(self.tx), gh.ptr(self.rx),
+                                                self.scale_modifier, gh.ptr(self.geom), gh.ptr(self.img),
+                                                gh.ptr(self.bin), self.capacity, gh.ptr(self.radii),
+                                                gh.ptr(self.out), stream), "graphed csi step: forward")
+        gh.check(L.gsparc_csi_loss(R, H, W, self.lambda_ssim, SSIM_C1, SSIM_C2, gh.ptr(self.out),
+                                   gh.ptr(self.target), gh.ptr(self.work), gh.ptr(self.loss), gh.ptr(self.dout),
+                                   stream), "graphed csi step: loss")
+        gh.check(L.gsparc_rasterize_backward_csi(
+            P, R, W, H, self.f0, self.df, gh.ptr(p["means"]), gh.ptr(p["scales"]), gh.ptr(p["rotations"]),
+            gh.ptr(p["emission_mlps"]), None, gh.ptr(self.views), gh.ptr(self.tx), gh.ptr(self.rx),
+            self.scale_modifier, gh.ptr(self.radii), gh.ptr(self.geom), gh.ptr(self.img), gh.ptr(self.bin),
+            self.capacity, gh.ptr(self.gradbuf), gh.ptr(self.dout), None, gh.ptr(g["means"]),
+            gh.ptr(g["emission_mlps"]), gh.ptr(g["opacities"]), None, gh.ptr(g["scales"]), gh.ptr(g["rotations"]),
+            None, stream), "graphed csi step: backward")
+
+    def instance_count(self) -> int:
+        """K of the last step (synchronises the current stream)."""
+        return wr.instance_count(self.P, self.R, self.H, self.W, self.img)
+
+
 class GraphedSceneStep(_GraphedStep):
     """GraphedTrainingStep for a batch of uneven scenes (wireless_rasterizer.
     SceneBatch): forward of every scene's receivers, the fused loss over all

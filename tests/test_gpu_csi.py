@@ -145,3 +145,42 @@ def test_csi_autograd_step():
     assert _rel(means.grad.cpu(), g[1].cpu()) < 1e-4
     assert _rel(mlp.grad.cpu(), g[2].cpu()) < 1e-4
     assert _rel(opac.grad.cpu(), g[3].cpu()) < 1e-4
+
+
+def test_graphed_csi_step_matches_eager():
+    """The graph-captured CSI step (forward with a capacity, CSI loss, exact
+    adjoint) equals the eager autograd step on every replay, with the
+    parameters changed in place and eager work between replays."""
+    R = 3
+    sc = S.small_scene(131, 1200, R=R, walls=True)
+    d = to_dev(sc)
+    tg = S.small_scene(132, 1200, R=R, walls=True)
+    dt = to_dev(tg)
+    with torch.no_grad():
+        target = _fwd(tg, dt)[1].clone()
+    K = _fwd(sc, d)[0]
+    live = {k: d[k].clone() for k in ("means", "scales", "rots", "opac", "mlp")}
+    print("graphed csi step, K", K, flush=True)
+    step = T.GraphedCsiStep(live["means"], live["scales"], live["rots"], live["opac"], live["mlp"], d["views"],
+                            d["tx"], d["rx"], target, sc.H, sc.W, capacity=int(1.25 * K) + 64)
+    rng = np.random.default_rng(131)
+    for rep in range(3):
+        with torch.no_grad():
+            live["means"].copy_(d["means"] + torch.from_numpy(
+                rng.normal(scale=2e-3, size=tuple(d["means"].shape))).float().cuda())
+        step.replay()
+        torch.cuda.synchronize()
+        assert not step.overflowed()
+        params = [live[k].clone().requires_grad_(True) for k in ("means", "scales", "rots", "opac", "mlp")]
+        m, s_, q, o, w = params
+        st = cr.CsiSetting(sc.H, sc.W, 1.0, d["views"], d["tx"], d["rx"])
+        out, radii = cr.rasterize_csi(m, w, o, s_, q, st)
+        loss = T.CSILoss.apply(out, target)
+        loss.backward()
+        torch.cuda.synchronize()
+        assert torch.equal(step.out, out.detach()), rep
+        assert torch.equal(step.radii, radii), rep
+        assert abs(float(step.loss) - float(loss.detach())) <= 1e-6 * abs(float(loss.detach())), rep
+        for k, p_ in zip(("means", "scales", "rotations", "opacities", "emission_mlps"), params):
+            assert _rel(step.grads[k].reshape(p_.shape).cpu(), p_.grad.cpu()) < 1e-4, (rep, k)
+        _fwd(tg, dt)   # unrelated eager work between replays
