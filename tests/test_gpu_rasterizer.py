@@ -145,6 +145,33 @@ def test_equal_depth_ties_in_index_order():
     _check_forward(sc, st, 0)
 
 
+def far_gaussians(sc, seed, n=200):
+    """Move n Gaussians 15 km to 1000 km from receiver 0, scaled to stay
+    visible."""
+    rng = np.random.default_rng(seed)
+    far = rng.choice(sc.P, n, replace=False)
+    dirs = rng.normal(size=(n, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    dist = np.exp(rng.uniform(np.log(1.5e4), np.log(1e6), size=n))
+    sc.means[far] = (sc.rx_pos[0] + dirs * dist[:, None]).astype(np.float32)
+    sc.scales[far] = np.repeat(0.05 * dist[:, None], 3, axis=1).astype(np.float32)
+    return far
+
+
+def test_far_gaussians():
+    """Depths from 15 km to 1000 km beside the room's (their depth bits differ
+    from the room's in every digit of the sort): the oracle's keys, lists and
+    ranges."""
+    sc = S.small_scene(24, 1500, R=2, walls=True)
+    far = far_gaussians(sc, 24)
+    _, out = gpu_forward(sc)
+    st = read_state(sc, out)
+    seen = (st["radii"][0][far] > 0) & (st["depth"][0][far] > 13200.0)
+    assert seen.sum() > 20, seen.sum()
+    for r in range(sc.R):
+        _check_forward(sc, st, r)
+
+
 def test_empty_and_culled():
     sc = S.small_scene(5, 50)
     # every Gaussian within 0.2 m of the receiver: all culled (auxiliary.h:143)

@@ -82,6 +82,21 @@ def test_scene_batch_forward_equals_scenes_alone(order):
         gh.set_option("render_order", 0)
 
 
+def test_scene_batch_with_far_gaussians():
+    """One scene holds Gaussians 15 km to 1000 km away, beside room-sized
+    scenes, in one launch sequence: every scene renders exactly as alone."""
+    from test_gpu_rasterizer import far_gaussians
+    scs = _scenes()
+    far_gaussians(scs[1], 77)
+    print("scenes with far Gaussians in scene 1", flush=True)
+    alone = [gpu_forward(sc)[1] for sc in scs]
+    p, _ = _packed(scs)
+    batch = wr.SceneBatch([sc.P for sc in scs], [sc.R for sc in scs], "cuda")
+    out = _forward_scenes(scs, p, batch)
+    assert out[0] == sum(a[0] for a in alone)
+    _check_forward(scs, out, alone)
+
+
 @pytest.mark.parametrize("mode", ["reference", "exact"])
 def test_scene_batch_backward_equals_scenes_alone(mode):
     scs = _scenes()
