@@ -387,9 +387,9 @@ __device__ __forceinline__ void pre_bwd_finish(const Dims& d, const PreBwdArgs& 
 // Kernel 2: the 194 MLP weight gradients, summed over the receivers of the
 // scene (backward.cu:95-115).  A workgroup owns GS_PB_G consecutive Gaussians
 // of one scene (blockIdx.y), one per lane; wave w takes hidden units
-// 4w .. 4w+3, so the per-receiver dL/d(MLP output) loads are coalesced over
-// the lanes and shared by the waves.  Receiver positions and record offsets
-// are staged in LDS 64 receivers at a time.  Wave 0 then sums the group
+// 4w .. 4w+3.  Per chunk of 64 receivers the workgroup stages their
+// positions and the block's dL/d(MLP output) in LDS (coalesced loads, each
+// value read from memory once for all 8 waves).  Wave 0 then sums the group
 // partials of its lane's Gaussian and runs the cov3D adjoint.
 #define GS_PB_G 64
 #define GS_PB_THREADS 512    // 8 waves x 4 hidden units
@@ -406,6 +406,9 @@ __global__ void __launch_bounds__(GS_PB_THREADS) k_pre_bwd_param(Dims d, PreBwdA
     if (a.mlp && a.dmlp) {
         __shared__ float s_rx[GS_WAVE * 3];
         __shared__ uint32_t s_roff[GS_WAVE];
+        // dL/d(MLP output) of the block's Gaussians for a chunk of 64
+        // receivers, read once per workgroup (all 8 waves use every value)
+        __shared__ float2 s_dz[GS_WAVE][GS_PB_G];
         // units j = 4 wave + u: W1 rows, b1, W2 of both outputs as 16-byte loads
         float w[4][3], bb[4], v0[4], v1[4];
         if (valid) {
@@ -441,13 +444,27 @@ __global__ void __launch_bounds__(GS_PB_THREADS) k_pre_bwd_param(Dims d, PreBwdA
                 s_rx[3 * tid + 2] = a.rx[3 * r + 2];
             }
             __syncthreads();
-            // receiver k + 1's dL/d(MLP output) is loaded while k is computed
-            const int gc = valid ? gl : 0;
-            float2 dz_n = a.dz[(size_t)s_roff[0] + gc];
+            // the chunk's dL/d(MLP output) into LDS: wave w loads receivers
+            // w, w + 8, ... (64 consecutive Gaussians each: coalesced), all of
+            // a thread's loads issued before its stores
+            {
+                constexpr int NK = GS_WAVE / (GS_PB_THREADS / GS_WAVE);
+                const int gc = valid ? gl : 0;
+                float2 v[NK];
+#pragma unroll
+                for (int q = 0; q < NK; ++q) {
+                    const int k = wave + q * (GS_PB_THREADS / GS_WAVE);
+                    v[q] = k < nrb ? a.dz[(size_t)s_roff[k] + gc] : make_float2(0.0f, 0.0f);
+                }
+#pragma unroll
+                for (int q = 0; q < NK; ++q) {
+                    const int k = wave + q * (GS_PB_THREADS / GS_WAVE);
+                    s_dz[k][lane] = valid ? v[q] : make_float2(0.0f, 0.0f);
+                }
+            }
+            __syncthreads();
             for (int k = 0; k < nrb; ++k) {
-                const float2 dzl = dz_n;
-                if (k + 1 < nrb) dz_n = a.dz[(size_t)s_roff[k + 1] + gc];
-                const float2 dz = valid ? dzl : make_float2(0.0f, 0.0f);
+                const float2 dz = s_dz[k][lane];
                 const float rx0 = s_rx[3 * k], rx1 = s_rx[3 * k + 1], rx2 = s_rx[3 * k + 2];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
@@ -455,12 +472,14 @@ __global__ void __launch_bounds__(GS_PB_THREADS) k_pre_bwd_param(Dims d, PreBwdA
                     hs = fmaf(w[u][0], rx0, hs);
                     hs = fmaf(w[u][1], rx1, hs);
                     hs = fmaf(w[u][2], rx2, hs);
-                    const bool pos = hs > 0.0f;
-                    const float h = pos ? hs : 0.01f * hs;
+                    // leaky ReLU (slope 0.01) and its derivative share one select:
+                    // hs * 1 and dh * 1 are exact
+                    const float slope = hs > 0.0f ? 1.0f : 0.01f;
+                    const float h = hs * slope;
                     gv0[u] = fmaf(dz.x, h, gv0[u]);
                     gv1[u] = fmaf(dz.y, h, gv1[u]);
                     const float dh = fmaf(dz.y, v1[u], dz.x * v0[u]);
-                    const float dp = pos ? dh : 0.01f * dh;
+                    const float dp = dh * slope;
                     gw[u][0] = fmaf(dp, rx0, gw[u][0]);
                     gw[u][1] = fmaf(dp, rx1, gw[u][1]);
                     gw[u][2] = fmaf(dp, rx2, gw[u][2]);
