@@ -2,12 +2,16 @@
 // of planes (B * CH planes of H x W; for the rasterizer, one plane per
 // receiver spectrum).
 //
-// A 256-thread workgroup owns a 32 x 16 output tile.  The zero-padded halo
-// (42 x 26) of the inputs is staged in LDS once; the horizontal 11-tap pass
-// produces all five (forward) or three (backward) filtered quantities in one
-// sweep, then the vertical pass finishes them per output pixel.  Summation
-// order matches the reference (taps 0..10, horizontal first); the taps are
-// applied with fused multiply-adds (as CUDA compiles the reference's
+// A 256-thread workgroup owns a 32 x 32 output tile.  The zero-padded halo
+// (42 x 42) of the inputs is staged in LDS once; the horizontal 11-tap pass
+// produces all five (forward) or three (backward) filtered quantities for
+// every halo row, then the vertical pass finishes them per output pixel.
+// Both passes slide along registers: a thread filters a run of SS_RUN
+// consecutive outputs (along a row, then down a column) from the SS_RUN + 10
+// inputs it reads once from LDS, and the forward's squares and cross products
+// are formed once per input rather than once per tap.  Summation order
+// matches the reference (taps 0..10, horizontal first; every output's own
+// chain of fused multiply-adds, as CUDA compiles the reference's
 // `acc += w * x`), and the per-pixel partial derivatives share the two
 // reciprocals 1/A and 1/B, so the values agree with the CPU restatement to
 // rounding (a few ulp).
@@ -19,10 +23,13 @@
 namespace gs {
 
 #define SS_TW 32
-#define SS_TH 16
+#define SS_TH 32
 #define SS_HW (SS_TW + 10)
 #define SS_HH (SS_TH + 10)
 #define SS_THREADS 256
+#define SS_RUN 4                                // outputs per register run
+#define SS_HRUNS (SS_HH * (SS_TW / SS_RUN))     // horizontal runs per tile
+static_assert((SS_TH / SS_RUN) * SS_TW == SS_THREADS, "one vertical run per thread");
 
 struct Taps { float w[11]; };
 
@@ -85,53 +92,79 @@ __global__ void __launch_bounds__(SS_THREADS) k_ssim_fwd(SsimArgs a, Taps tp)
         }
     }
     __syncthreads();
-    for (int i = tid; i < SS_HH * SS_TW; i += SS_THREADS) {
-        const int ly = i / SS_TW, lx = i % SS_TW;
-        float m1 = 0.0f, m2 = 0.0f, e11 = 0.0f, e22 = 0.0f, e12 = 0.0f;
+    // horizontal: run t covers halo row ly, output columns c0 .. c0 + SS_RUN - 1
+    for (int t = tid; t < SS_HRUNS; t += SS_THREADS) {
+        const int ly = t / (SS_TW / SS_RUN), c0 = (t % (SS_TW / SS_RUN)) * SS_RUN;
+        float p[SS_RUN + 10], q[SS_RUN + 10];
 #pragma unroll
-        for (int k = 0; k < 11; ++k) {
-            const float p = s1[ly][lx + k], q = s2[ly][lx + k];
-            m1 = fmaf(tp.w[k], p, m1);
-            m2 = fmaf(tp.w[k], q, m2);
-            e11 = fmaf(tp.w[k], p * p, e11);
-            e22 = fmaf(tp.w[k], q * q, e22);
-            e12 = fmaf(tp.w[k], p * q, e12);
+        for (int i = 0; i < SS_RUN + 10; ++i) {
+            p[i] = s1[ly][c0 + i];
+            q[i] = s2[ly][c0 + i];
         }
-        h[0][ly][lx] = m1; h[1][ly][lx] = m2; h[2][ly][lx] = e11; h[3][ly][lx] = e22; h[4][ly][lx] = e12;
+        float pp[SS_RUN + 10], qq[SS_RUN + 10], pq[SS_RUN + 10];
+#pragma unroll
+        for (int i = 0; i < SS_RUN + 10; ++i) {
+            pp[i] = p[i] * p[i];
+            qq[i] = q[i] * q[i];
+            pq[i] = p[i] * q[i];
+        }
+#pragma unroll
+        for (int o = 0; o < SS_RUN; ++o) {
+            float m1 = 0.0f, m2 = 0.0f, e11 = 0.0f, e22 = 0.0f, e12 = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 11; ++k) {
+                m1 = fmaf(tp.w[k], p[o + k], m1);
+                m2 = fmaf(tp.w[k], q[o + k], m2);
+                e11 = fmaf(tp.w[k], pp[o + k], e11);
+                e22 = fmaf(tp.w[k], qq[o + k], e22);
+                e12 = fmaf(tp.w[k], pq[o + k], e12);
+            }
+            h[0][ly][c0 + o] = m1; h[1][ly][c0 + o] = m2; h[2][ly][c0 + o] = e11;
+            h[3][ly][c0 + o] = e22; h[4][ly][c0 + o] = e12;
+        }
     }
     __syncthreads();
-    float sum_map = 0.0f, sum_abs = 0.0f, sum_d2 = 0.0f, sum_p2 = 0.0f, sum_t2 = 0.0f;
-    for (int i = tid; i < SS_TH * SS_TW; i += SS_THREADS) {
-        const int ly = i / SS_TW, lx = i % SS_TW;
-        const int gy = y0 + ly, gx = x0 + lx;
-        if (gy >= a.H || gx >= a.W) continue;
-        float v[5];
+    // vertical: the thread's column lx, output rows r0 .. r0 + SS_RUN - 1
+    const int lx = tid % SS_TW, r0 = (tid / SS_TW) * SS_RUN;
+    float v[SS_RUN][5];
 #pragma unroll
-        for (int c = 0; c < 5; ++c) {
+    for (int c = 0; c < 5; ++c) {
+        float col[SS_RUN + 10];
+#pragma unroll
+        for (int i = 0; i < SS_RUN + 10; ++i) col[i] = h[c][r0 + i][lx];
+#pragma unroll
+        for (int o = 0; o < SS_RUN; ++o) {
             float acc = 0.0f;
 #pragma unroll
-            for (int k = 0; k < 11; ++k) acc = fmaf(tp.w[k], h[c][ly + k][lx], acc);
-            v[c] = acc;
+            for (int k = 0; k < 11; ++k) acc = fmaf(tp.w[k], col[o + k], acc);
+            v[o][c] = acc;
         }
-        const float mu1 = v[0], mu2 = v[1];
-        const float sigma1_sq = v[2] - mu1 * mu1;
-        const float sigma2_sq = v[3] - mu2 * mu2;
-        const float sigma12 = v[4] - mu1 * mu2;
+    }
+    float sum_map = 0.0f, sum_abs = 0.0f, sum_d2 = 0.0f, sum_p2 = 0.0f, sum_t2 = 0.0f;
+#pragma unroll
+    for (int o = 0; o < SS_RUN; ++o) {
+        const int ly = r0 + o;
+        const int gy = y0 + ly, gx = x0 + lx;
+        if (gy >= a.H || gx >= a.W) continue;
+        const float mu1 = v[o][0], mu2 = v[o][1];
+        const float sigma1_sq = v[o][2] - mu1 * mu1;
+        const float sigma2_sq = v[o][3] - mu2 * mu2;
+        const float sigma12 = v[o][4] - mu1 * mu2;
         const float C = 2.0f * mu1 * mu2 + a.C1;
         const float D = 2.0f * sigma12 + a.C2;
         const float A = mu1 * mu1 + mu2 * mu2 + a.C1;
         const float B = sigma1_sq + sigma2_sq + a.C2;
-        const size_t o = plane + (size_t)gy * a.W + gx;
+        const size_t o_ = plane + (size_t)gy * a.W + gx;
         // ssim.cu:269-283 with the divisions shared: iA = 1/A, iB = 1/B
         const float iA = 1.0f / A, iB = 1.0f / B, iAB = iA * iB;
         const float CD = C * D;
         const float m = CD * iAB;
-        if (a.map) a.map[o] = m;
+        if (a.map) a.map[o_] = m;
         if (a.dmu1) {
             // (2 mu2 D - 2 mu2 C) / (AB) - 2 mu1 CD / (A^2 B) + 2 mu1 CD / (A B^2)
-            a.dmu1[o] = 2.0f * iAB * fmaf(mu1 * CD, iB - iA, mu2 * (D - C));
-            a.ds1[o] = -CD * iAB * iB;
-            a.ds12[o] = 2.0f * C * iAB;
+            a.dmu1[o_] = 2.0f * iAB * fmaf(mu1 * CD, iB - iA, mu2 * (D - C));
+            a.ds1[o_] = -CD * iAB * iB;
+            a.ds12[o_] = 2.0f * C * iAB;
         }
         sum_map += m;
         const float p = s1[ly + 5][lx + 5], t = s2[ly + 5][lx + 5];   // the staged pixel values
@@ -175,14 +208,13 @@ __global__ void __launch_bounds__(SS_THREADS) k_ssim_bwd(SsimBwdArgs a, Taps tp)
     // the plane's loss weight (the confidence of its spectrum), wave-uniform
     const float ps = a.plane_scale ? a.plane_scale[tl.z] : 1.0f;
     const float dmap_const = a.dmap_const * ps, l1w = a.l1w * ps;
-    // the images at the thread's output pixels, loaded now and used at the end
-    constexpr int NO = SS_TH * SS_TW / SS_THREADS;
-    static_assert(NO * SS_THREADS == SS_TH * SS_TW, "whole outputs per thread");
-    float xi[NO], yi[NO];
+    // the images at the thread's output pixels (column lx, rows r0 ..
+    // r0 + SS_RUN - 1 of the tile), loaded now and used at the end
+    const int lx = tid % SS_TW, r0 = (tid / SS_TW) * SS_RUN;
+    float xi[SS_RUN], yi[SS_RUN];
 #pragma unroll
-    for (int u = 0; u < NO; ++u) {
-        const int i = tid + u * SS_THREADS;
-        const int gy = min(y0 + i / SS_TW, a.H - 1), gx = min(x0 + i % SS_TW, a.W - 1);
+    for (int u = 0; u < SS_RUN; ++u) {
+        const int gy = min(y0 + r0 + u, a.H - 1), gx = min(x0 + lx, a.W - 1);
         xi[u] = a.img1[plane + (size_t)gy * a.W + gx];
         yi[u] = a.img2[plane + (size_t)gy * a.W + gx];
     }
@@ -193,9 +225,9 @@ __global__ void __launch_bounds__(SS_THREADS) k_ssim_bwd(SsimBwdArgs a, Taps tp)
         float g[NS], t0[NS], t1[NS], t2[NS];
 #pragma unroll
         for (int u = 0; u < NS; ++u) {
-            const int i = tid + u * SS_THREADS;
-            const int ly = i / SS_HW, lx = i % SS_HW;
-            const int gy = y0 + ly - 5, gx = x0 + lx - 5;
+            const int i = min(tid + u * SS_THREADS, SS_HH * SS_HW - 1);
+            const int ly = i / SS_HW, lx_ = i % SS_HW;
+            const int gy = y0 + ly - 5, gx = x0 + lx_ - 5;
             const size_t o = plane + (size_t)min(max(gy, 0), a.H - 1) * a.W + min(max(gx, 0), a.W - 1);
             g[u] = a.dmap ? a.dmap[o] : dmap_const;
             t0[u] = a.dmu1[o];
@@ -206,43 +238,54 @@ __global__ void __launch_bounds__(SS_THREADS) k_ssim_bwd(SsimBwdArgs a, Taps tp)
         for (int u = 0; u < NS; ++u) {
             const int i = tid + u * SS_THREADS;
             if (i >= SS_HH * SS_HW) break;
-            const int ly = i / SS_HW, lx = i % SS_HW;
-            const int gy = y0 + ly - 5, gx = x0 + lx - 5;
+            const int ly = i / SS_HW, lx_ = i % SS_HW;
+            const int gy = y0 + ly - 5, gx = x0 + lx_ - 5;
             const bool in = gx >= 0 && gx < a.W && gy >= 0 && gy < a.H;
-            s[0][ly][lx] = in ? g[u] * t0[u] : 0.0f;
-            s[1][ly][lx] = in ? g[u] * t1[u] : 0.0f;
-            s[2][ly][lx] = in ? g[u] * t2[u] : 0.0f;
+            s[0][ly][lx_] = in ? g[u] * t0[u] : 0.0f;
+            s[1][ly][lx_] = in ? g[u] * t1[u] : 0.0f;
+            s[2][ly][lx_] = in ? g[u] * t2[u] : 0.0f;
         }
     }
     __syncthreads();
-    for (int i = tid; i < SS_HH * SS_TW; i += SS_THREADS) {
-        const int ly = i / SS_TW, lx = i % SS_TW;
+    // horizontal: run t covers halo row ly, output columns c0 .. c0 + SS_RUN - 1
+    for (int t = tid; t < SS_HRUNS; t += SS_THREADS) {
+        const int ly = t / (SS_TW / SS_RUN), c0 = (t % (SS_TW / SS_RUN)) * SS_RUN;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            float acc = 0.0f;
+            float row[SS_RUN + 10];
 #pragma unroll
-            for (int k = 0; k < 11; ++k) acc = fmaf(tp.w[k], s[c][ly][lx + k], acc);
-            h[c][ly][lx] = acc;
+            for (int i = 0; i < SS_RUN + 10; ++i) row[i] = s[c][ly][c0 + i];
+#pragma unroll
+            for (int o = 0; o < SS_RUN; ++o) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int k = 0; k < 11; ++k) acc = fmaf(tp.w[k], row[o + k], acc);
+                h[c][ly][c0 + o] = acc;
+            }
         }
     }
     __syncthreads();
+    float v[SS_RUN][3];
 #pragma unroll
-    for (int u = 0; u < NO; ++u) {
-        const int i = tid + u * SS_THREADS;
-        const int ly = i / SS_TW, lx = i % SS_TW;
-        const int gy = y0 + ly, gx = x0 + lx;
+    for (int c = 0; c < 3; ++c) {
+        float col[SS_RUN + 10];
+#pragma unroll
+        for (int i = 0; i < SS_RUN + 10; ++i) col[i] = h[c][r0 + i][lx];
+#pragma unroll
+        for (int o = 0; o < SS_RUN; ++o) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 11; ++k) acc = fmaf(tp.w[k], col[o + k], acc);
+            v[o][c] = acc;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < SS_RUN; ++u) {
+        const int gy = y0 + r0 + u, gx = x0 + lx;
         if (gy >= a.H || gx >= a.W) continue;
-        float v[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            float acc = 0.0f;
-#pragma unroll
-            for (int k = 0; k < 11; ++k) acc = fmaf(tp.w[k], h[c][ly + k][lx], acc);
-            v[c] = acc;
-        }
         const size_t o = plane + (size_t)gy * a.W + gx;
         const float x = xi[u], y = yi[u];
-        float g = v[0] + x * 2.0f * v[1] + y * v[2];
+        float g = v[u][0] + x * 2.0f * v[u][1] + y * v[u][2];
         if (l1w != 0.0f) {
             // d|x - y|/dx with sign(0) = 0, as torch's abs backward
             const float sgn = x > y ? 1.0f : (x < y ? -1.0f : 0.0f);
