@@ -544,6 +544,16 @@ def setup_config1(args, dev):
             E = mlp(means, rxp, txp)
         return wr.forward_batched(means, None, opac, scales, rots, sc.H, sc.W, 1.0, views, txp, rxp, emission=E)[0]
 
+    def graphed(deterministic=False):
+        """The same step (MLP forward, rasterizer forward, fused loss,
+        backward, MLP backward) captured in a HIP graph; tile lists sized for
+        1.25 K."""
+        cap = int(count() * 1.25) + 1024
+        live = [x.clone() for x in (means, scales, rots, opac)]
+        weights = [p_.detach().clone().contiguous() for p_ in mlp.parameters()]
+        g = T.GraphedSharedMlpStep(*live, weights, views, txp, rxp, target, sc.H, sc.W, capacity=cap)
+        return g, (live, weights)
+
     def mlp_rows():
         """The MLP alone at 50k Gaussians x 64 receivers (3.2M rows), fwd+bwd."""
         import gsparc_hip as gh
@@ -574,7 +584,7 @@ def setup_config1(args, dev):
                 "flops_per_row": {"fwd": MLP_FLOPS_FWD, "bwd": MLP_FLOPS_BWD}}
 
     return Workload(
-        step=step, step_plain=None, single=single, count=count, cpu=lambda: None, graphed=None, tune=None,
+        step=step, step_plain=None, single=single, count=count, cpu=lambda: None, graphed=graphed, tune=None,
         spectra=1, Ptot=sc.P, Nrec=sc.P, H=sc.H, W=sc.W, traffic_key=None, mlp_rows=mlp_rows,
         data="synthetic (seeded: 20k Gaussians uniform in a 6x6x3 m room, receiver at the centre)",
         config={"workload": "single-receiver training step, shared 68->128->128->1 emission MLP (FP32 MFMA), "

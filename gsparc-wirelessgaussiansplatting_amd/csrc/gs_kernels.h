@@ -404,7 +404,8 @@ size_t mlp1_grad_floats();
 hipError_t launch_mlp1_fwd(const Mlp1Args& a, hipStream_t s);
 // grads: dW1 [128*68] | db1 [128] | dW2 [128*128] | db2 [128] | dw3 [128] | db3 [1];
 // dmu [P,3] written (the MLP's gradient of the means); work as the forward left it
-hipError_t launch_mlp1_bwd(const Mlp1Args& a, const float* dem, float* dmu, float* work, float* grads, hipStream_t s);
+hipError_t launch_mlp1_bwd(const Mlp1Args& a, const float* dem, float* dmu, float* work, float* grads, hipStream_t s,
+                           bool accumulate);
 
 // CSI loss pieces (k_loss_csi.hip): magnitude planes + L1 partials, the
 // gradient back to the interleaved map, and the two-part loss reduction.

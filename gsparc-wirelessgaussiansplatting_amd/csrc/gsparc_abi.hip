@@ -1392,7 +1392,21 @@ int gsparc_mlp1_backward(int P, int R, const float* means, const float* rx_pos, 
     if (int e = mlp1_args(P, R, means, rx_pos, tx_pos, W1, b1, W2, b2, w3, b3, work, a)) return e;
     if (!grads || (P > 0 && !dmeans) || ((long long)P * R > 0 && !demission))
         return fail(GSPARC_ERR_ARG, "null pointer");
-    GS_LAUNCH("mlp1_bwd", gs::launch_mlp1_bwd(a, demission, dmeans, a.pe1, grads, (hipStream_t)stream),
+    GS_LAUNCH("mlp1_bwd", gs::launch_mlp1_bwd(a, demission, dmeans, a.pe1, grads, (hipStream_t)stream, false),
+              (hipStream_t)stream);
+    return GSPARC_OK;
+}
+
+int gsparc_mlp1_backward_accumulate(int P, int R, const float* means, const float* rx_pos, const float* tx_pos,
+                                    const float* W1, const float* b1, const float* W2, const float* b2,
+                                    const float* w3, const float* b3, void* work, const float* demission,
+                                    float* dmeans, float* grads, void* stream)
+{
+    gs::Mlp1Args a;
+    if (int e = mlp1_args(P, R, means, rx_pos, tx_pos, W1, b1, W2, b2, w3, b3, work, a)) return e;
+    if (!grads || (P > 0 && !dmeans) || ((long long)P * R > 0 && !demission))
+        return fail(GSPARC_ERR_ARG, "null pointer");
+    GS_LAUNCH("mlp1_bwd", gs::launch_mlp1_bwd(a, demission, dmeans, a.pe1, grads, (hipStream_t)stream, true),
               (hipStream_t)stream);
     return GSPARC_OK;
 }

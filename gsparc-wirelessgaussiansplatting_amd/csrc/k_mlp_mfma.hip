@@ -744,15 +744,32 @@ __global__ void __launch_bounds__(256) k_mlp1_pe_bwd(Mlp1Args a, const float* dp
     if (threadIdx.x < M1_H) out[M1_H * 64 + threadIdx.x] = db1;
 }
 
-// out[c] = sum over workgroups of part[wg][c] (record width `width`), in
-// workgroup order
+// out[c] = sum over workgroups of part[wg][c] (record width `width`) in a
+// fixed order: a workgroup takes 64 columns, its 4 waves the rows
+// wg = slice mod 4 (each in increasing order, 4 rows in flight), and the
+// four slice sums are added in slice order.
+#define M1_CS_COLS 64
 __global__ void __launch_bounds__(256) k_mlp1_colsum(const float* part, int nwg, int width, float* out)
 {
-    const int c = blockIdx.x * 256 + threadIdx.x;
-    if (c >= width) return;
+    __shared__ float s_sum[4][M1_CS_COLS];
+    const int l = threadIdx.x % M1_CS_COLS, sl = threadIdx.x / M1_CS_COLS;
+    const int c = blockIdx.x * M1_CS_COLS + l;
     float s = 0.0f;
-    for (int i = 0; i < nwg; ++i) s += part[(size_t)i * width + c];
-    out[c] = s;
+    if (c < width) {
+        int i = sl;
+        for (; i + 12 < nwg; i += 16) {
+            const float v0 = part[(size_t)i * width + c], v1 = part[(size_t)(i + 4) * width + c];
+            const float v2 = part[(size_t)(i + 8) * width + c], v3 = part[(size_t)(i + 12) * width + c];
+            s += v0;
+            s += v1;
+            s += v2;
+            s += v3;
+        }
+        for (; i < nwg; i += 4) s += part[(size_t)i * width + c];
+    }
+    s_sum[sl][l] = s;
+    __syncthreads();
+    if (sl == 0 && c < width) out[c] = ((s_sum[0][l] + s_sum[1][l]) + s_sum[2][l]) + s_sum[3][l];
 }
 
 // grads = dW1 [128 x 68] | db1 | dW2 [128 x 128] | db2 | dw3 | db3 from the
@@ -807,7 +824,8 @@ hipError_t launch_mlp1_fwd(const Mlp1Args& a, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_mlp1_bwd(const Mlp1Args& a, const float* dem, float* dmu, float* work, float* grads, hipStream_t s)
+hipError_t launch_mlp1_bwd(const Mlp1Args& a, const float* dem, float* dmu, float* work, float* grads, hipStream_t s,
+                           bool accumulate)
 {
     const long long n_rows = (long long)a.P * a.R;
     float* dpe1 = work + (size_t)a.P * M1_H;
@@ -819,20 +837,23 @@ hipError_t launch_mlp1_bwd(const Mlp1Args& a, const float* dem, float* dmu, floa
     const size_t nz = (size_t)a.P * M1_H;
     hipLaunchKernelGGL(k_mlp1_zero, dim3((unsigned)std::max<size_t>(1, std::min<size_t>((nz + 255) / 256, 4096))),
                        dim3(256), 0, s, dpe1, nz);
-    hipLaunchKernelGGL(k_mlp1_zero, dim3((unsigned)std::max<size_t>(1, std::min<size_t>((3 * (size_t)a.P + 255) / 256,
-                                                                                      4096))),
-                       dim3(256), 0, s, dmu, 3 * (size_t)a.P);
+    // the means gradient is added to (kernels below only add to dmu): cleared
+    // first unless the caller accumulates into a gradient it already holds
+    if (!accumulate)
+        hipLaunchKernelGGL(k_mlp1_zero,
+                           dim3((unsigned)std::max<size_t>(1, std::min<size_t>((3 * (size_t)a.P + 255) / 256, 4096))),
+                           dim3(256), 0, s, dmu, 3 * (size_t)a.P);
     if (n_rows > 0) {
         hipLaunchKernelGGL(k_mlp1_bwd, dim3(nwg), dim3(256), 0, s, a, dem, dpe1, dmu, rpart);
-        hipLaunchKernelGGL(k_mlp1_colsum, dim3((M1_PART + 255) / 256), dim3(256), 0, s, (const float*)rpart, nwg,
-                           (int)M1_PART, rsum);
+        hipLaunchKernelGGL(k_mlp1_colsum, dim3((M1_PART + M1_CS_COLS - 1) / M1_CS_COLS), dim3(256), 0, s,
+                           (const float*)rpart, nwg, (int)M1_PART, rsum);
     } else {
         hipLaunchKernelGGL(k_mlp1_zero, dim3(64), dim3(256), 0, s, rsum, (size_t)M1_PART);
     }
     if (a.P > 0) {
         hipLaunchKernelGGL(k_mlp1_pe_bwd, dim3(npe), dim3(256), 0, s, a, (const float*)dpe1, dmu, ppart);
-        hipLaunchKernelGGL(k_mlp1_colsum, dim3((M1_PEPART + 255) / 256), dim3(256), 0, s, (const float*)ppart, npe,
-                           (int)M1_PEPART, psum);
+        hipLaunchKernelGGL(k_mlp1_colsum, dim3((M1_PEPART + M1_CS_COLS - 1) / M1_CS_COLS), dim3(256), 0, s,
+                           (const float*)ppart, npe, (int)M1_PEPART, psum);
     } else {
         hipLaunchKernelGGL(k_mlp1_zero, dim3(64), dim3(256), 0, s, psum, (size_t)M1_PEPART);
     }

@@ -132,6 +132,7 @@ _SIGS = {
     "gsparc_mlp1_forward": (_i, [_i, _i] + [_vp] * 12),
     "gsparc_mlp1_work_size": (C.c_size_t, [_i, _i]),
     "gsparc_mlp1_backward": (_i, [_i, _i] + [_vp] * 14),
+    "gsparc_mlp1_backward_accumulate": (_i, [_i, _i] + [_vp] * 14),
     "gsparc_csi_loss_work_size": (C.c_size_t, [_i, _i, _i]),
     "gsparc_csi_loss": (_i, [_i, _i, _i, _f, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gsparc_loss_work_size": (C.c_size_t, [_i, _i, _i]),

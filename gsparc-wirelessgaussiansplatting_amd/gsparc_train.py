@@ -461,6 +461,101 @@ class GraphedCsiStep(_GraphedStep):
         return wr.instance_count(self.P, self.R, self.H, self.W, self.img)
 
 
+class GraphedSharedMlpStep(_GraphedStep):
+    """The single-receiver training configuration's step with the shared
+    emission MLP (gsparc_emission.SharedEmissionMLP), captured once and
+    replayed: the MLP forward for every (receiver, Gaussian) pair, the
+    rasterizer forward with the emission input and a tile-list capacity, the
+    fused loss, the exact backward, and the MLP backward, which adds its
+    means gradient to the rasterizer's.  Parameters (means [P,3], scales
+    [P,3], rotations [P,4], opacities [P,1]), the MLP weights (W1 [128,68],
+    b1, W2 [128,128], b2, w3 [1,128], b3) and target [R,H,W] are read in place
+    at every replay; after replay() `loss`, `out_real`, `radii` and `grads`
+    (means, opacities, scales, rotations, and `mlp`: dW1 | db1 | dW2 | db2 |
+    dw3 | db3 flat) hold the step's results.  No counterpart in the
+    reference, whose emission MLP is per Gaussian (gaussian.py:47)."""
+
+    def __init__(self, means, scales, rotations, opacities, mlp_weights, viewmatrices, tx_pos, rx_pos, target,
+                 image_height, image_width, capacity, scale_modifier=1.0, lambda_ssim=LAMBDA_SSIM):
+        self.params = dict(means=_require_param(means, "means"), scales=_require_param(scales, "scales"),
+                           rotations=_require_param(rotations, "rotations"),
+                           opacities=_require_param(opacities, "opacities"))
+        self.weights = [_require_param(w, "mlp weight") for w in mlp_weights]
+        if len(self.weights) != 6:
+            raise RuntimeError("mlp_weights: W1, b1, W2, b2, w3, b3")
+        dev = means.device
+        P = int(means.size(0))
+        self.views = gh.f32(viewmatrices).reshape(-1, 16)
+        self.rx = gh.f32(rx_pos).reshape(-1, 3)
+        self.tx = gh.f32(tx_pos).reshape(3)
+        R = int(self.rx.size(0))
+        H, W = int(image_height), int(image_width)
+        self.target = _require_param(target, "target")
+        if tuple(self.target.shape) != (R, H, W):
+            raise RuntimeError("target must be [R,H,W]")
+        self.P, self.R, self.H, self.W = P, R, H, W
+        self.capacity = int(capacity)
+        self.scale_modifier = float(scale_modifier)
+        self.lambda_ssim = float(lambda_ssim)
+        L = gh.lib()
+        f32 = dict(dtype=torch.float32, device=dev)
+        u8 = lambda n: torch.empty(int(n), dtype=torch.uint8, device=dev)
+        self.geom = u8(L.gsparc_geom_buffer_size(P, R, W, H))
+        self.img = u8(L.gsparc_image_buffer_size(P, R, W, H))
+        self.bin = u8(L.gsparc_binning_buffer_size(self.capacity))
+        self.gradbuf = u8(L.gsparc_grad_buffer_size(P, R))
+        self.mlp_work = u8(L.gsparc_mlp1_work_size(P, R))
+        self.work = _work(R, H, W, dev)
+        self.emission = torch.empty((R, P, 2), **f32)
+        self.demission = torch.empty((R, P, 2), **f32)
+        self.radii = torch.empty((R, P), dtype=torch.int32, device=dev)
+        self.out_real = torch.empty((R, H, W), **f32)
+        self.out_imag = torch.empty((R, H, W), **f32)
+        self.loss = torch.empty(1, **f32)
+        self.dimg = torch.empty((R, H, W), **f32)
+        self.zero_imag = torch.zeros((R, H, W), **f32)
+        n_mlp = sum(int(w.numel()) for w in self.weights)
+        self.grads = dict(means=torch.empty((P, 3), **f32), opacities=torch.empty((P, 1), **f32),
+                          scales=torch.empty((P, 3), **f32), rotations=torch.empty((P, 4), **f32),
+                          mlp=torch.empty(n_mlp, **f32))
+        self._capture(dev)
+
+    def _record(self, stream):
+        L = gh.lib()
+        p, g, w = self.params, self.grads, [gh.ptr(x) for x in self.weights]
+        P, R, W, H = self.P, self.R, self.W, self.H
+        gh.check(L.gsparc_mlp1_forward(P, R, gh.ptr(p["means"]), gh.ptr(self.rx), gh.ptr(self.tx), *w,
+                                       gh.ptr(self.mlp_work), gh.ptr(self.emission), stream),
+                 "graphed shared-MLP step: MLP forward")
+        gh.check(L.gsparc_rasterize_forward(P, R, W, H, gh.ptr(p["means"]), gh.ptr(p["scales"]),
+                                            gh.ptr(p["rotations"]), gh.ptr(p["opacities"]), None,
+                                            gh.ptr(self.emission), gh.ptr(self.views), gh.ptr(self.tx),
+                                            gh.ptr(self.rx), self.scale_modifier, gh.ptr(self.geom),
+                                            gh.ptr(self.img), gh.ptr(self.bin), self.capacity, gh.ptr(self.radii),
+                                            gh.ptr(self.out_real), gh.ptr(self.out_imag), 0, stream),
+                 "graphed shared-MLP step: forward")
+        gh.check(L.gsparc_loss_l1_ssim(R, H, W, self.lambda_ssim, SSIM_C1, SSIM_C2, gh.ptr(self.out_real),
+                                       gh.ptr(self.target), gh.ptr(self.work), gh.ptr(self.loss), gh.ptr(self.dimg),
+                                       stream), "graphed shared-MLP step: loss")
+        gh.check(L.gsparc_rasterize_backward(gh.GRAD_EXACT, P, R, W, H, gh.ptr(p["means"]), gh.ptr(p["scales"]),
+                                             gh.ptr(p["rotations"]), None, gh.ptr(self.emission),
+                                             gh.ptr(self.views), gh.ptr(self.tx), gh.ptr(self.rx),
+                                             self.scale_modifier, gh.ptr(self.radii), gh.ptr(self.geom),
+                                             gh.ptr(self.img), gh.ptr(self.bin), self.capacity, gh.ptr(self.gradbuf),
+                                             gh.ptr(self.dimg), gh.ptr(self.zero_imag), None, gh.ptr(g["means"]),
+                                             None, gh.ptr(g["opacities"]), None, gh.ptr(g["scales"]),
+                                             gh.ptr(g["rotations"]), gh.ptr(self.demission), stream),
+                 "graphed shared-MLP step: backward")
+        gh.check(L.gsparc_mlp1_backward_accumulate(P, R, gh.ptr(p["means"]), gh.ptr(self.rx), gh.ptr(self.tx), *w,
+                                                   gh.ptr(self.mlp_work), gh.ptr(self.demission), gh.ptr(g["means"]),
+                                                   gh.ptr(g["mlp"]), stream),
+                 "graphed shared-MLP step: MLP backward")
+
+    def instance_count(self) -> int:
+        """K of the last step (synchronises the current stream)."""
+        return wr.instance_count(self.P, self.R, self.H, self.W, self.img)
+
+
 class GraphedSceneStep(_GraphedStep):
     """GraphedTrainingStep for a batch of uneven scenes (wireless_rasterizer.
     SceneBatch): forward of every scene's receivers, the fused loss over all

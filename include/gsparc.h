@@ -287,6 +287,13 @@ int gsparc_mlp1_forward(int P, int R, const float* means, const float* rx_pos, c
 int gsparc_mlp1_backward(int P, int R, const float* means, const float* rx_pos, const float* tx_pos, const float* W1,
                          const float* b1, const float* W2, const float* b2, const float* w3, const float* b3,
                          void* work, const float* demission, float* dmeans, float* grads, void* stream);
+/* gsparc_mlp1_backward that adds the means gradient to dmeans (for instance
+ * the rasterizer's dL/dmeans3D, giving the total) instead of overwriting it;
+ * grads are written as there.  No counterpart in the reference. */
+int gsparc_mlp1_backward_accumulate(int P, int R, const float* means, const float* rx_pos, const float* tx_pos,
+                                    const float* W1, const float* b1, const float* W2, const float* b2,
+                                    const float* w3, const float* b3, void* work, const float* demission,
+                                    float* dmeans, float* grads, void* stream);
 
 /* Training loss of the reference, value and gradient in one call:
  *   loss = (1 - lambda_ssim) * mean|img - target| + lambda_ssim * (1 - mean SSIM(img, target))

@@ -92,6 +92,60 @@ def test_config1_training_step():
         assert e < 1e-4, (name, errs)
 
 
+@pytest.mark.parametrize("R", [1, 4])
+def test_graphed_shared_mlp_step_matches_autograd(R):
+    """The graph-captured step (MLP forward, rasterizer forward with a
+    capacity, fused loss, exact backward, MLP backward adding to the means
+    gradient) equals the autograd step on every replay, with the means and
+    the MLP weights changed in place between replays."""
+    import gsparc_train as T
+    sc = S.config1() if R == 1 else S.small_scene(72, 6000, R=R, walls=True)
+    tgt = S.config1(seed=2) if R == 1 else S.small_scene(73, 6000, R=R, walls=True)
+    d, dt = to_dev(sc), to_dev(tgt)
+    mlp = GE.SharedEmissionMLP(generator=torch.Generator().manual_seed(5)).cuda()
+    with torch.no_grad():
+        Et = mlp(dt["means"], d["rx"], d["tx"])
+        target = wr.forward_batched(dt["means"], None, dt["opac"], dt["scales"], dt["rots"], sc.H, sc.W, 1.0,
+                                    d["views"], d["tx"], d["rx"], emission=Et)[1].clone()
+        K = wr.forward_batched(d["means"], None, d["opac"], d["scales"], d["rots"], sc.H, sc.W, 1.0, d["views"],
+                               d["tx"], d["rx"], emission=mlp(d["means"], d["rx"], d["tx"]))[0]
+    live = {k: d[k].clone() for k in ("means", "scales", "rots", "opac")}
+    weights = [p.detach().clone().contiguous() for p in mlp.parameters()]
+    print("graphed shared-MLP step R", R, "K", K, flush=True)
+    step = T.GraphedSharedMlpStep(live["means"], live["scales"], live["rots"], live["opac"], weights, d["views"],
+                                  d["tx"], d["rx"], target, sc.H, sc.W, capacity=int(1.25 * K) + 64)
+    rng = np.random.default_rng(72)
+    settings = cr.BatchedSetting(sc.H, sc.W, 1.0, d["views"], d["tx"], d["rx"], "exact")
+    for rep in range(3):
+        with torch.no_grad():
+            live["means"].copy_(d["means"] + torch.from_numpy(
+                rng.normal(scale=1e-3, size=tuple(d["means"].shape))).float().cuda())
+            for w in weights:
+                w.mul_(1.0 + 1e-3 * rep)
+        step.replay()
+        torch.cuda.synchronize()
+        assert not step.overflowed()
+        ref = GE.SharedEmissionMLP().cuda()
+        with torch.no_grad():
+            for p_, w in zip(ref.parameters(), weights):
+                p_.copy_(w)
+        means = live["means"].clone().requires_grad_(True)
+        opac = live["opac"].clone().requires_grad_(True)
+        scales = live["scales"].clone().requires_grad_(True)
+        rots = live["rots"].clone().requires_grad_(True)
+        E = ref(means, d["rx"], d["tx"])
+        ore, oim, radii = cr.rasterize_gaussians_batched(means, None, opac, scales, rots, settings, emission=E)
+        loss = 0.8 * (ore - target).abs().mean() + 0.2 * (1.0 - fused_ssim(ore.unsqueeze(1), target.unsqueeze(1)))
+        loss.backward()
+        torch.cuda.synchronize()
+        assert torch.equal(step.out_real, ore.detach()), rep
+        assert abs(float(step.loss) - float(loss.detach())) <= 1e-5 * abs(float(loss.detach())), rep
+        for name, t_ in (("means", means), ("opacities", opac), ("scales", scales), ("rotations", rots)):
+            assert _rel(step.grads[name].reshape(t_.shape).cpu().numpy(), t_.grad.cpu().numpy()) < 1e-4, (rep, name)
+        gm = torch.cat([p_.grad.reshape(-1) for p_ in ref.parameters()])
+        assert _rel(step.grads["mlp"].cpu().numpy(), gm.cpu().numpy()) < 1e-4, rep
+
+
 @pytest.mark.parametrize("P,R", [(5000, 3), (37, 1), (300, 7), (130, 64), (1, 1)])
 def test_mfma_mlp_matches_torch(P, R):
     """Amplitudes within 1e-5 and every gradient (weights, biases, means)
