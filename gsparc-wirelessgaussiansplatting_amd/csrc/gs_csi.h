@@ -32,11 +32,13 @@
 #define GS_LIGHT_SPEED 299792458.0                     // m/s
 
 // f_k / c split for the float32 cycle count; k0 and dk are f0 / c and df / c
-// rounded to float32 (for the path-length gradient, which is linear in f_k).
+// rounded to float32 (for the path-length gradient, which is linear in f_k);
+// dklo = df / c - dk, for the cycle count of the subcarrier spacing.
 struct GsCsi {
     float khi[GS_CSI_NC];
     float klo[GS_CSI_NC];
     float k0, dk;
+    float dklo;
 };
 
 GS_HD GsCsi gs_csi_make(double f0_hz, double df_hz, double c_mps)
@@ -49,6 +51,7 @@ GS_HD GsCsi gs_csi_make(double f0_hz, double df_hz, double c_mps)
     }
     t.k0 = (float)(f0_hz / c_mps);
     t.dk = (float)(df_hz / c_mps);
+    t.dklo = (float)(df_hz / c_mps - (double)t.dk);
     return t;
 }
 

@@ -474,6 +474,7 @@ hipError_t launch_densify_gather(const DensifyPlan& pl, const DensifyCfg& c, con
 // The multi-frequency kernels' phase factors (cos 2 pi x, sin 2 pi x) of
 // cycle fractions x, for their accuracy check (k_render_csi.hip).
 hipError_t launch_phase_factors(size_t n, const float* x, float* c, float* s, hipStream_t st);
+hipError_t launch_csi_phasors(size_t n, const float* plen, const GsCsi& t, float* out, hipStream_t st);
 
 // ---- the spectrum dataset on the device (k_data.hip) ----
 // q: gateway quaternion (x, y, z, w); out [N,16] column-major view matrices.

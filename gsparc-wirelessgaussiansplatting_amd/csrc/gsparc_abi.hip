@@ -1611,6 +1611,15 @@ int gsparc_csi_phase_factors(size_t n, const float* turns, float* cos_out, float
     return GSPARC_OK;
 }
 
+int gsparc_csi_phasors(size_t n, const float* path_len, double f0_hz, double df_hz, float* out, void* stream)
+{
+    if (n > 0 && (!path_len || !out)) return fail(GSPARC_ERR_ARG, "null pointer");
+    GsCsi t;
+    if (int rc = check_csi(f0_hz, df_hz, t)) return rc;
+    GS_LAUNCH("csi_phasors", gs::launch_csi_phasors(n, path_len, t, out, (hipStream_t)stream), (hipStream_t)stream);
+    return GSPARC_OK;
+}
+
 // ---- the spectrum dataset on the device
 int gsparc_view_matrices(int N, const double* gateway_quat, const float* rx_pos, float* viewmatrices, void* stream)
 {

@@ -63,6 +63,56 @@ hipError_t launch_phase_factors(size_t n, const float* x, float* c, float* s, hi
     return hipGetLastError();
 }
 
+// Phasors u_k = exp(-j 2 pi L f_k / c) of one half of the subcarriers,
+// k = h*NC/2 .. h*NC/2 + NC/2 - 1, as NC/4 float4 (two subcarriers each,
+// handed to put(m, value)): the exact phase at the half's first subcarrier,
+// then the complex recurrence u_{k+1} = u_k rho with rho = exp(-j 2 pi L df / c).
+// Both start from exact cycle fractions; the first phasor takes the hardware
+// sin/cos, rho the polynomial gs_sincos2pi, because rho's error accumulates
+// over the steps and the polynomial is the more accurate of the two at rho's
+// small angles.  Four operations per subcarrier instead of a cycle fraction
+// and two transcendentals; the error grows linearly over the 25 steps (the
+// measured bounds are in the phasor accuracy test).  Used where the phasors
+// weight gradients only.
+static_assert(GS_CSI_NC % 4 == 0, "two halves of whole float4 pairs");
+template <typename Put>
+__device__ __forceinline__ void csi_phasor_half(float L, int h, const GsCsi& t, Put&& put)
+{
+    const float x0 = gs_cycle_frac(L, h ? t.khi[GS_CSI_NC / 2] : t.khi[0], h ? t.klo[GS_CSI_NC / 2] : t.klo[0]);
+    const float xs = gs_cycle_frac(L, t.dk, t.dklo);
+    float ur = __builtin_amdgcn_cosf(x0), ui = -__builtin_amdgcn_sinf(x0);
+    float pr, pi;
+    gs_sincos2pi(xs, &pr, &pi);
+    pi = -pi;
+#pragma unroll
+    for (int m = 0; m < GS_CSI_NC / 4; ++m) {
+        const float vr = fmaf(ur, pr, -(ui * pi)), vi = fmaf(ur, pi, ui * pr);
+        put(m, make_float4(ur, ui, vr, vi));
+        ur = fmaf(vr, pr, -(vi * pi));
+        ui = fmaf(vr, pi, vi * pr);
+    }
+}
+
+// csi_phasor_half for n path lengths: out[i][k] = (Re, Im) u_k, for its
+// accuracy check; one thread per (length, half)
+__global__ void __launch_bounds__(256) k_csi_phasors(size_t n, const float* plen, GsCsi t, float4* out)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < 2 * n; i += (size_t)gridDim.x * 256) {
+        const size_t e = i >> 1;
+        const int h = (int)(i & 1);
+        float4* o = out + e * (GS_CSI_NC / 2) + h * (GS_CSI_NC / 4);
+        csi_phasor_half(plen[e], h, t, [&](int m, float4 v) { o[m] = v; });
+    }
+}
+
+hipError_t launch_csi_phasors(size_t n, const float* plen, const GsCsi& t, float* out, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_csi_phasors, dim3((unsigned)std::min<size_t>((2 * n + 255) / 256, 65536)), dim3(256), 0,
+                       st, n, plen, t, reinterpret_cast<float4*>(out));
+    return hipGetLastError();
+}
+
 template <int PPL>
 __global__ void __launch_bounds__(GS_WAVE) k_render_fwd_csi(Dims d, RenderCsiArgs a)
 {
@@ -381,35 +431,34 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_bwd_csi(Dims d, RenderCsiArg
     // accum of backward.cu:579, updated right after each contributor instead
     // of before the next one (the same values, no last-alpha state)
     float acc = 0.0f;
+    static_assert(CSI_BATCH == GS_WAVE / 2, "the two halves of the wave stage the same entries");
     for (int top = maxlast - 1; top >= 0; top -= CSI_BATCH) {
-        // entry j of this batch sits at list position top - j
-        const int rel = top - lane;
+        // entry j = lane % 32 of this batch sits at list position top - j;
+        // both halves of the wave load and test it, and half h evaluates the
+        // phasors of subcarriers 26h .. 26h + 25 (they weight gradients only:
+        // csi_phasor_half's recurrence from the exact phase of the half's
+        // first subcarrier)
+        const int jl = lane & (CSI_BATCH - 1), hh = lane / CSI_BATCH;
+        const int rel = top - jl;
         bool keep = false;
-        if (lane < CSI_BATCH && rel >= 0) {
+        if (rel >= 0) {
             const uint32_t id = a.plist[rb + rel];
             const size_t gi = rP + id;
             const float4 A = a.splat[2 * gi];
             const float4 B = a.splat[2 * gi + 1];
-            sid[lane] = id;
-            s0[lane] = A;
-            s1[lane] = B;
-            ss[lane] = a.sig[gi];
+            if (hh == 0) {
+                sid[jl] = id;
+                s0[jl] = A;
+                s1[jl] = B;
+                ss[jl] = a.sig[gi];
+            }
             keep = !block_misses(A.x, A.y, A.z, A.w, B.x, B.z, pm.x0, pm.x1, pm.y0, pm.y1);
             if (keep) {
-                // the phase factors only weight gradients here: the hardware
-                // v_sin_f32 / v_cos_f32 of the exact cycle fraction (their
-                // argument is in turns) instead of the mirrored polynomials
-                const float L = a.plen[gi];
-#pragma unroll 2
-                for (int k2 = 0; k2 < CSI_NC2; ++k2) {
-                    const float x0 = gs_cycle_frac(L, a.csi.khi[2 * k2], a.csi.klo[2 * k2]);
-                    const float x1 = gs_cycle_frac(L, a.csi.khi[2 * k2 + 1], a.csi.klo[2 * k2 + 1]);
-                    su[lane][k2] = make_float4(__builtin_amdgcn_cosf(x0), -__builtin_amdgcn_sinf(x0),
-                                               __builtin_amdgcn_cosf(x1), -__builtin_amdgcn_sinf(x1));   // exp(-j 2 pi x)
-                }
+                float4* dst = &su[jl][hh * (CSI_NC2 / 2)];
+                csi_phasor_half(a.plen[gi], hh, a.csi, [&](int m, float4 v) { dst[m] = v; });
             }
         }
-        uint64_t mask = ballot(keep);
+        uint64_t mask = ballot(keep) & 0xffffffffull;   // the lower half's bits: one per entry
         __syncthreads();
         while (mask) {
             const int j = __builtin_ctzll(mask);

@@ -153,6 +153,7 @@ _SIGS = {
     "gsparc_view_matrices": (_i, [_i, C.POINTER(C.c_double), _vp, _vp, _vp]),
     "gsparc_gather_batch": (_i, [_i, _i, _i, _vp, _i] + [_vp] * 8),
     "gsparc_csi_phase_factors": (_i, [_sz, _vp, _vp, _vp, _vp]),
+    "gsparc_csi_phasors": (_i, [_sz, _vp, C.c_double, C.c_double, _vp, _vp]),
     "gsparc_graph_begin": (_i, [_vp]),
     "gsparc_graph_end": (_i, [_vp, C.POINTER(_vp)]),
     "gsparc_graph_launch": (_i, [_vp, _vp]),

@@ -261,6 +261,14 @@ int gsparc_csi_loss(int R, int H, int W, float lambda_ssim, float C1, float C2, 
  * reference (one channel, config.h:15). */
 int gsparc_csi_phase_factors(size_t n, const float* turns, float* cos_out, float* sin_out, void* stream);
 
+/* The multi-frequency backward's phasors exp(-j 2 pi L f_k / c), k = 0..51,
+ * f_k = f0_hz + k df_hz, for n path lengths L (metres): out is n x 52 x 2
+ * floats (real, imaginary).  They come from the exact phase of subcarriers 0
+ * and 26 and a complex recurrence over the subcarrier spacing, which is what
+ * this entry point exposes for its accuracy check.  No counterpart in the
+ * reference (one channel, config.h:15). */
+int gsparc_csi_phasors(size_t n, const float* path_len, double f0_hz, double df_hz, float* out, void* stream);
+
 /* ---- shared emission MLP on the matrix cores --------------------------------
  * The 68 -> 128 -> 128 -> 1 ReLU MLP of the single-receiver training
  * configuration, for every (receiver r, Gaussian g) pair:

@@ -51,3 +51,36 @@ def test_phase_factors_over_every_cycle_fraction():
     s = torch.empty_like(x)
     gh.check(L.gsparc_csi_phase_factors(1, gh.ptr(x), gh.ptr(c), gh.ptr(s), gh.stream_handle(dev)), "phase")
     assert float(c) == 1.0 and float(s) == 0.0
+
+
+# the multi-frequency backward's phasors: the exact phase at subcarriers 0 and
+# 26 (hardware sin/cos), then the complex recurrence over the subcarrier
+# spacing (25 steps, step factor from the polynomial gs_sincos2pi).  Bounds: a
+# float32 emulation of the recurrence on the host reaches 1.42e-6 below 40 m
+# and 4.24e-6 below 500 m; the anchors' hardware sin/cos add at most 1.3e-7.
+PHASOR_ERR = {40.0: 1.8e-6, 500.0: 5.0e-6}
+
+
+def test_backward_phasors_against_fp64():
+    dev = "cuda"
+    L = gh.lib()
+    f0, df, c = 3.5e9 - 25.5 * 312.5e3, 312.5e3, 299792458.0
+    g = torch.Generator().manual_seed(11)
+    for top, bound in PHASOR_ERR.items():
+        n = 1 << 19
+        plen = (torch.rand(n, generator=g, dtype=torch.float64) * top).float()
+        plen[:4] = torch.tensor([0.0, 1e-6, top / 2, top])
+        d_plen = plen.to(dev)
+        out = torch.empty((n, 52, 2), dtype=torch.float32, device=dev)
+        gh.check(L.gsparc_csi_phasors(n, gh.ptr(d_plen), f0, df, gh.ptr(out), gh.stream_handle(dev)), "phasors")
+        torch.cuda.synchronize()
+        k = torch.arange(52, dtype=torch.float64)
+        cyc = (plen.double()[:, None] * ((f0 + k * df) / c)[None, :]) % 1.0
+        ref = torch.stack([torch.cos(2 * math.pi * cyc), -torch.sin(2 * math.pi * cyc)], -1)
+        err = (out.cpu().double() - ref).abs()
+        worst = float(err.max())
+        print("phasors below", top, "m: worst |error|", worst, "at subcarrier",
+              int(torch.argmax(err.amax(dim=(0, 2)))), flush=True)
+        assert worst <= bound
+        # the anchors of the two halves are the exact-phase factors
+        assert float(err[:, 0].max()) <= 2.6e-7 and float(err[:, 26].max()) <= 2.6e-7
