@@ -731,9 +731,10 @@ def main():
         # + 84 B per (Gaussian, receiver): gradient record 32, conic 12,
         # radius 4, dL/dmean2D out 12, MLP outputs 8, dL/dz 8, partials 8
         "preprocess_bwd": ("hbm", Ptot * 1690 + 84 * Nrec, "GB/s"),
-        # 4 passes of (key, index) pairs, plus the packed rects carried into
-        # depth order by the last pass
-        "bin_depth_sort": ("hbm", 92.0 * Nrec, "GB/s"),
+        # depth and radius read twice (the digit counts, then the first
+        # pass), 3 passes reading and all 4 writing (key, index) pairs, and
+        # the packed rects carried into depth order by the last pass
+        "bin_depth_sort": ("hbm", 80.0 * Nrec, "GB/s"),
         # depth-ordered rects in, one count per (256-Gaussian chunk, tile) out
         "bin_chunk_count": ("hbm", 4.0 * Nrec + 4.0 * T * (Nrec / 256.0), "GB/s"),
         # sorted index and rect per (Gaussian, receiver) in, one index per instance out

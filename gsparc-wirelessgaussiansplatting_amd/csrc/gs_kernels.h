@@ -146,7 +146,10 @@ struct BinArgs {
     uint32_t* nvis;          // [R] visible Gaussians per receiver
     uint32_t* skey[2];       // [Nrec] depth-sort ping-pong: depth bits
     uint32_t* sval[2];       // [Nrec]                       Gaussian index
-    uint32_t* shist;         // [R*257*rs_blocks(P)] digit histograms
+    uint32_t* shist;         // [os_words] depth-sort state: look-back status words, digit counts, tickets
+    uint32_t* lbstat;        // (set by the sort) [R][rs_blocks(P)][256] status words of a pass
+    uint32_t* ghist;         // (set by the sort) [R][GS_OS_GHIST] digit counts of the 4 passes, visible count
+    uint32_t* tick;          // (set by the sort) [4][R] tile tickets
     uint32_t* ccnt;          // [R*T*nchunks] per-chunk tile counts / bases
     int nchunks, chunk_len;
     uint32_t* plist;         // [Kcap] Gaussian indices in tile-list order
@@ -156,6 +159,8 @@ struct BinArgs {
     uint32_t* srect;         // [Nrec] the same in depth order (written by the last sort pass)
 };
 int rs_blocks(int P);
+#define GS_OS_GHIST 1025
+size_t os_words(const Dims& d);   // 32-bit words of BinArgs::shist
 int chunk_len_for(const Dims& d);
 int chunk_count_for(const Dims& d);
 hipError_t launch_depth_sort(const Dims& d, const BinArgs& a, hipStream_t s);

@@ -124,7 +124,7 @@ void make_layout(const gs::Dims& d, uint64_t K, gsparc_layout& L)
     L.sort_val0 = o; o = al(o + NR * sizeof(uint32_t));
     L.sort_key1 = o; o = al(o + NR * sizeof(uint32_t));
     L.sort_val1 = o; o = al(o + NR * sizeof(uint32_t));
-    L.sort_hist = o; o = al(o + (size_t)R * 257 * gs::rs_blocks(d.P) * sizeof(uint32_t));
+    L.sort_hist = o; o = al(o + gs::os_words(d) * sizeof(uint32_t));
     L.nvis = o; o = al(o + (size_t)R * sizeof(uint32_t));
     L.plen = o; o = al(o + NR * sizeof(float));
     L.rect = o; o = al(o + NR * sizeof(uint32_t));
@@ -207,6 +207,8 @@ int check_dims(int P, int R, int W, int H)
     if (P < 0 || R < 0 || W <= 0 || H <= 0) return fail(GSPARC_ERR_ARG, "negative or zero size");
     if ((int64_t)R * W * H > (int64_t)1 << 31) return fail(GSPARC_ERR_ARG, "R*H*W too large");
     if ((int64_t)P * R >= (int64_t)1 << 31) return fail(GSPARC_ERR_ARG, "P*R too large");
+    // the depth sort's look-back words hold a receiver's counts in 28 bits
+    if (P >= 1 << 28) return fail(GSPARC_ERR_ARG, "P must be below 2^28");
     // the binning keeps one 64-bit mask per tile column and row in LDS
     if ((W + 15) / 16 + (H + 15) / 16 > 8192) return fail(GSPARC_ERR_ARG, "W + H too large (at most 131072 pixels)");
     // the ordered scatter divides tile indices in float: exact below 2^20 tiles
@@ -405,6 +407,8 @@ int forward_geom_impl(const gs::Dims& d, const FwdInputs& in, void* geom, void* 
         GS_LAUNCH("bin_chunk_scan", gs::launch_chunk_scan(d, ba, s), s);
     } else {
         GS_LAUNCH("clear_counts", gs::launch_fill_u32(ba.segcnt, 0u, (size_t)d.R * d.T, s), s);
+        // no sort: its wait-limit flag (read by the segment scan) is clear
+        GS_LAUNCH("clear_counts", gs::launch_fill_u32(ba.total + 2, 0u, 1, s), s);
     }
     GS_LAUNCH("bin_segscan", gs::launch_segscan_order(d, ba, choose_order(d), (uint32_t*)(ib + L.tile_order), s), s);
     return GSPARC_OK;
@@ -755,6 +759,8 @@ int scene_dims(const gsparc_batch* b, const uint32_t* table, int W, int H, gs::D
     if (!b || !table) return fail(GSPARC_ERR_ARG, "null batch or scene table");
     if (b->S < 1 || b->R < 0 || b->G < 0 || b->Pmax < 0 || b->Ptot < 0) return fail(GSPARC_ERR_ARG, "bad batch");
     if (b->Nrec >= ((uint64_t)1 << 31)) return fail(GSPARC_ERR_ARG, "too many (receiver, Gaussian) records");
+    // the depth sort's look-back words hold a receiver's counts in 28 bits
+    if (b->Pmax >= 1 << 28) return fail(GSPARC_ERR_ARG, "a scene's Gaussian count must be below 2^28");
     if (int e = check_map(b->R, W, H)) return e;
     d = gs::make_dims(b->Pmax, b->R, W, H);
     d.Ptot = b->Ptot;

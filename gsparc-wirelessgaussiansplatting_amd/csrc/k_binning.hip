@@ -6,11 +6,14 @@
 // instances in index order).
 //
 // Sort first, then bin, so that no per-tile sort is needed:
-//   k_rs_hist / k_rs_scan / k_rs_scatter
+//   k_os_hist / k_os_pass
 //               stable LSD radix sort of each receiver's visible Gaussians by
 //               the 32 depth bits (4 passes of 8-bit digits; input in index
-//               order, so equal depths stay in index order).  The first pass
-//               reads depth and radii directly and drops culled Gaussians.
+//               order, so equal depths stay in index order): one histogram
+//               of all four digits up front, then one launch per pass whose
+//               tiles find their offsets by decoupled look-back.  The first
+//               pass reads depth and radii directly and drops culled
+//               Gaussians.
 //   k_chunk_count  the sorted sequence of a receiver is cut into chunks; per
 //               chunk and tile, the number of instances
 //   k_chunk_scan   per (receiver, tile): exclusive prefix over the chunks,
@@ -106,7 +109,9 @@ __global__ void __launch_bounds__(1024) k_segscan(int n, const uint32_t* segcnt,
             run += c;
         }
     }
-    if (threadIdx.x == 0) total[0] = tot;
+    // a depth sort that gave up waiting (total[2], k_os_pass) leaves lists
+    // that must not be used: K reads as saturated, i.e. as an overflow
+    if (threadIdx.x == 0) total[0] = total[2] ? 0xffffffffu : tot;
 }
 
 // The same for n <= GS_SEGSCAN_LDS counts: coalesced loads (all in flight)
@@ -152,7 +157,7 @@ __device__ __forceinline__ void segscan_lds_body(int n, const uint32_t* segcnt, 
         // (rasterizer_impl.cu:303)
         ranges[i] = c ? make_uint2(b, b + c) : make_uint2(0u, 0u);
     }
-    if (tid == 0) total[0] = tot;
+    if (tid == 0) total[0] = total[2] ? 0xffffffffu : tot;   // as in segscan_body
 }
 
 __global__ void __launch_bounds__(1024) k_segscan_lds(int n, const uint32_t* segcnt, uint32_t* segbase,
@@ -201,85 +206,146 @@ __device__ __forceinline__ void rs_load(const Dims& d, const BinArgs& a, size_t 
     }
 }
 
-template <bool FIRST>
-__global__ void __launch_bounds__(GS_RS_THREADS) k_rs_hist(Dims d, BinArgs a, int pass)
+// Per receiver, the counts of each of the four 8-bit digits over its visible
+// Gaussians (ghist[r][pass][digit]) and the number visible (ghist[r][1024]),
+// from the depth and radius in index order: every pass's digit bases are
+// known before the first pass runs.  One LDS atomic per (key, digit), then
+// one global atomic per nonzero (digit, block) count.
+__global__ void __launch_bounds__(GS_RS_THREADS) k_os_hist(Dims d, BinArgs a)
 {
-    constexpr int NB = FIRST ? 257 : 256;
-    __shared__ uint32_t h[NB];
-    const int r = blockIdx.y, blk = blockIdx.x, nblk = gridDim.x;
+    __shared__ uint32_t h[4 * 256];
+    __shared__ uint32_t s_vis[GS_RS_THREADS / GS_WAVE];
+    const int r = blockIdx.y, blk = blockIdx.x;
     const RxMap rm = rx_map(d, r);
-    const int n = FIRST ? (int)rm.P : (int)a.nvis[r];
+    const int n = (int)rm.P;
     const size_t rP = rm.roff;
-    for (int i = threadIdx.x; i < NB; i += GS_RS_THREADS) h[i] = 0;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < 4 * 256; i += GS_RS_THREADS) h[i] = 0;
+    if (blk == 0 && r == 0 && tid == 0) a.total[2] = 0;   // the passes' wait-limit flag, per sort
     __syncthreads();
+    uint32_t nv = 0;
     if (blk * GS_RS_TILE < n) {
-        // all of the thread's loads first and unconditional (positions past
-        // the end read the last element and are masked below), so that their
-        // latencies overlap
-        uint32_t dg[GS_RS_ITEMS];
+        // all of the thread's loads first (clamped positions, masked below)
+        uint32_t key[GS_RS_ITEMS], val, dg[GS_RS_ITEMS];
 #pragma unroll
         for (int k = 0; k < GS_RS_ITEMS; ++k) {
-            const int i = min(n - 1, blk * GS_RS_TILE + k * GS_RS_THREADS + (int)threadIdx.x);
-            uint32_t key, val;
-            rs_load<FIRST>(d, a, rP, i, pass, key, val, dg[k]);
+            const int i = min(n - 1, blk * GS_RS_TILE + k * GS_RS_THREADS + tid);
+            rs_load<true>(d, a, rP, i, 0, key[k], val, dg[k]);
         }
 #pragma unroll
         for (int k = 0; k < GS_RS_ITEMS; ++k) {
-            const bool valid = blk * GS_RS_TILE + k * GS_RS_THREADS + (int)threadIdx.x < n;
-            if (valid) atomicAdd(&h[dg[k]], 1u);
+            const bool vis = blk * GS_RS_TILE + k * GS_RS_THREADS + tid < n && dg[k] != 256u;
+            if (vis) {
+                ++nv;
+#pragma unroll
+                for (int p = 0; p < 3; ++p) atomicAdd(&h[p * 256 + ((key[k] >> (8 * p)) & 0xffu)], 1u);
+            }
+            // the top digit (sign and high exponent bits) takes a handful of
+            // values over a scene's depths: one atomic per distinct digit of
+            // the wave instead of one per key on the same few addresses
+            const uint32_t d3 = key[k] >> 24;
+            const uint64_t m = match_digit<8>(d3, vis);
+            if (vis && (m & lanemask_lt()) == 0) atomicAdd(&h[3 * 256 + d3], (uint32_t)__popcll(m));
         }
     }
+    // visible count: wave sums, then one atomic per block
+    uint32_t wv = nv;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) wv += __shfl_xor(wv, off);
+    if (tid % GS_WAVE == 0) s_vis[tid / GS_WAVE] = wv;
     __syncthreads();
-    for (int i = threadIdx.x; i < NB; i += GS_RS_THREADS) a.shist[((size_t)r * 257 + i) * nblk + blk] = h[i];
+    uint32_t* gh = a.ghist + (size_t)r * GS_OS_GHIST;
+    for (int i = tid; i < 4 * 256; i += GS_RS_THREADS)
+        if (h[i]) atomicAdd(gh + i, h[i]);
+    if (tid == 0) {
+        uint32_t t = 0;
+#pragma unroll
+        for (int w = 0; w < GS_RS_THREADS / GS_WAVE; ++w) t += s_vis[w];
+        if (t) atomicAdd(gh + 1024, t);
+    }
 }
 
-// Per receiver: exclusive scan over (digit, block) of the block histograms.
-__global__ void __launch_bounds__(1024) k_rs_scan(Dims d, BinArgs a, int nblk, int first)
+// Status word of a (receiver, tile, digit): published (OS_PUB) by a pass
+// (bits 28-29) with the tile's count of the digit, or with OS_INCL the count
+// of all tiles up to and including it, in the low 28 bits.  The words are
+// zeroed once per sort; a word still holding an earlier pass's tag reads as
+// not yet published.
+#define OS_PUB (1u << 31)
+#define OS_INCL (1u << 30)
+#define OS_VAL ((1u << 28) - 1u)
+__device__ __forceinline__ uint32_t os_word(int pass, bool incl, uint32_t v)
 {
-    __shared__ uint32_t s_wave[1024 / GS_WAVE];
-    const int r = blockIdx.x;
-    const int n = (first ? 257 : 256) * nblk;
-    uint32_t* h = a.shist + (size_t)r * 257 * nblk;
-    uint32_t carry = 0;
-    for (int base = 0; base < n; base += 1024 * 4) {
-        uint32_t v[4], s = 0;
+    return OS_PUB | (incl ? OS_INCL : 0u) | ((uint32_t)pass << 28) | v;
+}
+// a tile waits only on tiles that drew earlier tickets, which are running
+// and publish without waiting themselves; the limit bounds the wait anyway
+#define OS_SPIN_LIMIT (1 << 20)
+
+// Exclusive prefix, over the tiles before tile blk, of one digit's count:
+// walks back through the status words of that digit (st: tile t's word at
+// st[t * 256]), OS_LB tiles per round of loads (the tiles of a pass start
+// together, so most predecessors show only their own count), adding counts
+// until an inclusive prefix.  A round that finds the next word unpublished
+// waits and reloads it; past OS_SPIN_LIMIT such rounds it sets *err and
+// returns.
+#define OS_LB 16
+__device__ __forceinline__ uint32_t os_lookback(uint32_t* st, int blk, int pass, uint32_t* err)
+{
+    uint32_t excl = 0;
+    int t = blk - 1;
+    int idle = 0;
+    const uint32_t tag = OS_PUB | ((uint32_t)pass << 28);
+    while (t >= 0) {
+        uint32_t w[OS_LB];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int idx = base + threadIdx.x * 4 + j;
-            v[j] = idx < n ? h[idx] : 0u;
-            s += v[j];
-        }
-        uint32_t tot;
-        uint32_t run = carry + block_exclusive_scan<1024>(s, s_wave, tot);
+        for (int k = 0; k < OS_LB; ++k)
+            w[k] = t - k >= 0 ? __hip_atomic_load(st + (size_t)(t - k) * 256, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT)
+                              : os_word(pass, true, 0u);   // before tile 0: nothing
+        int used = 0;
+        bool done = false;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int idx = base + threadIdx.x * 4 + j;
-            if (idx < n) h[idx] = run;
-            run += v[j];
+        for (int k = 0; k < OS_LB; ++k) {
+            if (done || used < k) break;
+            if ((w[k] & (OS_PUB | (3u << 28))) != tag) break;   // not yet published in this pass
+            excl += w[k] & OS_VAL;
+            used = k + 1;
+            done = (w[k] & OS_INCL) != 0u;
         }
-        carry += tot;
+        if (done) break;
+        t -= used;
+        if (used == 0) {
+            if (++idle > OS_SPIN_LIMIT) {
+                atomicOr(err, 1u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
     }
-    __syncthreads();
-    // visible Gaussians = everything before the culled digit
-    if (first && threadIdx.x == 0) a.nvis[r] = h[(size_t)256 * nblk];
+    return excl;
 }
 
-// Stable scatter of one pass.  Wave w of the block owns the block's
-// elements w*WT .. w*WT+WT-1 (WT = GS_RS_TILE / waves), in GS_RS_ITEMS rounds
-// of 64: round ranks come from the digit-match masks, a wave-private running
-// count per digit carries them across rounds, and the per-digit prefix over
-// the waves joins the waves in input order.  The block then sorts its
-// elements into LDS by (digit, input position) and writes each digit's run to
-// consecutive global positions, so stores are contiguous per digit rather than
-// one element per bucket.  The first pass drops culled Gaussians (digit 256):
-// every later pass and the chunk kernels read only the nvis visible ones.
+// One stable pass.  Each workgroup draws a ticket (its tile of the
+// receiver's sequence, in launch order) and ranks the tile's keys as
+// follows: wave w owns elements w*WT .. w*WT+WT-1 (WT = GS_RS_TILE / waves),
+// in GS_RS_ITEMS rounds of 64; round ranks come from the digit-match masks,
+// a wave-private running count per digit carries them across rounds, and the
+// per-digit prefix over the waves joins the waves in input order.  The
+// tile's per-digit counts are published; the counts of the earlier tiles
+// come from the look-back, the digit's base in the output from the
+// receiver's digit counts (k_os_hist).  The block then sorts its elements
+// into LDS by (digit, input position) and writes each digit's run to
+// consecutive global positions.  The first pass drops culled Gaussians
+// (digit 256): every later pass and the chunk kernels read only the nvis
+// visible ones.
 template <bool FIRST>
-__global__ void __launch_bounds__(GS_RS_THREADS) k_rs_scatter(Dims d, BinArgs a, int pass)
+__global__ void __launch_bounds__(GS_RS_THREADS) k_os_pass(Dims d, BinArgs a, int pass, int nblk)
 {
     constexpr int NB = FIRST ? 257 : 256;
     constexpr int NW = GS_RS_THREADS / GS_WAVE;
     constexpr int WT = GS_RS_TILE / NW;
     static_assert(WT == GS_RS_ITEMS * GS_WAVE, "a wave owns GS_RS_ITEMS rounds of 64");
+    static_assert(GS_RS_THREADS == 256, "one thread per digit");
     __shared__ uint32_t s_key[GS_RS_TILE];
     __shared__ uint32_t s_val[GS_RS_TILE];
     __shared__ uint16_t s_dg[GS_RS_TILE];
@@ -287,18 +353,25 @@ __global__ void __launch_bounds__(GS_RS_THREADS) k_rs_scatter(Dims d, BinArgs a,
     __shared__ uint32_t lbase[NB];
     __shared__ uint32_t gbase[NB];
     __shared__ uint32_t s_wave[GS_RS_THREADS / GS_WAVE];
-    const int r = blockIdx.y, blk = blockIdx.x, nblk = gridDim.x;
+    __shared__ int s_tile;
+    const int r = blockIdx.y;
     const RxMap rm = rx_map(d, r);
+    const uint32_t* gh = a.ghist + (size_t)r * GS_OS_GHIST;
     const int n = FIRST ? (int)rm.P : (int)a.nvis[r];
-    if (blk * GS_RS_TILE >= n) return;
-    const size_t rP = rm.roff;
     const int tid = threadIdx.x, wave = tid / GS_WAVE, lane = tid % GS_WAVE;
+    if (tid == 0) s_tile = (int)atomicAdd(a.tick + (size_t)pass * d.R + r, 1u);
     for (int i = tid; i < NB; i += GS_RS_THREADS) {
-        gbase[i] = a.shist[((size_t)r * 257 + i) * nblk + blk];
 #pragma unroll
         for (int w = 0; w < NW; ++w) whist[w][i] = 0;
     }
     __syncthreads();
+    const int blk = s_tile;
+    // every receiver's visible count is written, also where the pass has no
+    // tile (a scene without Gaussians): the later passes and the chunk
+    // kernels take it as their element count
+    if (FIRST && blk == 0 && tid == 0) a.nvis[r] = gh[1024];
+    if (blk * GS_RS_TILE >= n) return;   // uniform over the block
+    const size_t rP = rm.roff;
     uint32_t key[GS_RS_ITEMS], val[GS_RS_ITEMS], dg[GS_RS_ITEMS], rnk[GS_RS_ITEMS];
     const int e0 = wave * WT;
     // all of the thread's loads first and unconditional (positions past the
@@ -309,6 +382,8 @@ __global__ void __launch_bounds__(GS_RS_THREADS) k_rs_scatter(Dims d, BinArgs a,
         const int i = min(n - 1, blk * GS_RS_TILE + e0 + k * GS_WAVE + lane);
         rs_load<FIRST>(d, a, rP, i, pass, key[k], val[k], dg[k]);
     }
+    // the receiver's count of this thread's digit in this pass (read early)
+    const uint32_t gcount = gh[pass * 256 + tid];
 #pragma unroll
     for (int k = 0; k < GS_RS_ITEMS; ++k) {
         const bool valid = blk * GS_RS_TILE + e0 + k * GS_WAVE + lane < n;
@@ -330,6 +405,10 @@ __global__ void __launch_bounds__(GS_RS_THREADS) k_rs_scatter(Dims d, BinArgs a,
         }
         tot = run;
     }
+    // publish this tile's count of digit tid (tile 0: already inclusive)
+    uint32_t* st = a.lbstat + (size_t)r * nblk * 256 + tid;
+    __hip_atomic_store(st + (size_t)blk * 256, os_word(pass, blk == 0, tot), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
     uint32_t all;
     const uint32_t lb = block_exclusive_scan<GS_RS_THREADS>(tot, s_wave, all);
     lbase[tid] = lb;
@@ -343,6 +422,17 @@ __global__ void __launch_bounds__(GS_RS_THREADS) k_rs_scatter(Dims d, BinArgs a,
         }
         lbase[256] = all;
     }
+    // the digit's first output position: digits below it over the whole
+    // receiver, then the digit's elements in earlier tiles
+    uint32_t gall;
+    const uint32_t db = block_exclusive_scan<GS_RS_THREADS>(gcount, s_wave, gall);
+    uint32_t excl = 0;
+    if (blk > 0) {
+        excl = os_lookback(st, blk, pass, a.total + 2);
+        __hip_atomic_store(st + (size_t)blk * 256, os_word(pass, true, excl + tot), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    gbase[tid] = db + excl;
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < GS_RS_ITEMS; ++k) {
@@ -855,22 +945,34 @@ int chunk_count_for(const Dims& d) { return max(1, (d.P + chunk_len_for(d) - 1) 
 
 size_t chunk_scatter_lds(const Dims& d) { return GS_SCATTER_WAVES * chunk_scatter_slice(d) * sizeof(uint64_t); }
 
-hipError_t launch_depth_sort(const Dims& d, const BinArgs& a, hipStream_t s)
+size_t os_words(const Dims& d)
+{
+    return (size_t)d.R * ((size_t)rs_blocks(d.P) * 256 + GS_OS_GHIST + 4);
+}
+
+BinArgs os_args(const Dims& d, const BinArgs& a0)
+{
+    BinArgs a = a0;
+    a.lbstat = a.shist;                                               // [R][nblk][256]
+    a.ghist = a.lbstat + (size_t)d.R * rs_blocks(d.P) * 256;          // [R][GS_OS_GHIST]
+    a.tick = a.ghist + (size_t)d.R * GS_OS_GHIST;                     // [4][R]
+    return a;
+}
+
+hipError_t launch_depth_sort(const Dims& d, const BinArgs& a0, hipStream_t s)
 {
     if (d.R == 0 || d.P == 0) return hipSuccess;
     const int nblk = rs_blocks(d.P);
     const dim3 grid(nblk, d.R);
-    for (int pass = 0; pass < 4; ++pass) {
-        if (pass == 0) {
-            hipLaunchKernelGGL(k_rs_hist<true>, grid, dim3(GS_RS_THREADS), 0, s, d, a, pass);
-            hipLaunchKernelGGL(k_rs_scan, dim3(d.R), dim3(1024), 0, s, d, a, nblk, 1);
-            hipLaunchKernelGGL(k_rs_scatter<true>, grid, dim3(GS_RS_THREADS), 0, s, d, a, pass);
-        } else {
-            hipLaunchKernelGGL(k_rs_hist<false>, grid, dim3(GS_RS_THREADS), 0, s, d, a, pass);
-            hipLaunchKernelGGL(k_rs_scan, dim3(d.R), dim3(1024), 0, s, d, a, nblk, 0);
-            hipLaunchKernelGGL(k_rs_scatter<false>, grid, dim3(GS_RS_THREADS), 0, s, d, a, pass);
-        }
-    }
+    const BinArgs a = os_args(d, a0);
+    // status words, digit counts and tickets start at zero (a fill kernel:
+    // the sort may be captured into a graph)
+    hipError_t e = launch_fill_u32(a.shist, 0u, os_words(d), s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_os_hist, grid, dim3(GS_RS_THREADS), 0, s, d, a);
+    hipLaunchKernelGGL(k_os_pass<true>, grid, dim3(GS_RS_THREADS), 0, s, d, a, 0, nblk);
+    for (int pass = 1; pass < 4; ++pass)
+        hipLaunchKernelGGL(k_os_pass<false>, grid, dim3(GS_RS_THREADS), 0, s, d, a, pass, nblk);
     return hipGetLastError();
 }
 
