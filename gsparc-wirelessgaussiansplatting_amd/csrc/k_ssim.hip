@@ -382,9 +382,13 @@ hipError_t launch_eval_metrics(const EvalArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
+#define CONF_CHUNK 256   // planes staged in LDS at a time by the weight gradients
 __global__ void __launch_bounds__(256) k_conf_loss(ConfLossArgs a)
 {
     __shared__ double red[256];
+    __shared__ float s_draw[CONF_CHUNK];
+    __shared__ float s_rx[CONF_CHUNK * 3];
+    __shared__ double s_g[256 * 5];
     const int tid = threadIdx.x, Hn = a.hidden;
     const float* W1 = a.mlp;
     const float* b1 = a.mlp ? a.mlp + 3 * Hn : nullptr;
@@ -395,6 +399,7 @@ __global__ void __launch_bounds__(256) k_conf_loss(ConfLossArgs a)
     for (int r = tid; r < a.planes; r += 256) {
         double m = 0.0, l1 = 0.0;
         const float* pp = a.part + 2 * (size_t)r * a.nbp;
+#pragma unroll 4
         for (int b = 0; b < a.nbp; ++b) {
             m += (double)pp[2 * b];
             l1 += (double)pp[2 * b + 1];
@@ -429,32 +434,66 @@ __global__ void __launch_bounds__(256) k_conf_loss(ConfLossArgs a)
     if (!a.mlp || !a.dmlp) return;
     __threadfence();
     __syncthreads();
-    if (tid < Hn) {
-        const int j = tid;
-        double gw[3] = {0.0, 0.0, 0.0}, gb = 0.0, g2 = 0.0;
-        for (int r = 0; r < a.planes; ++r) {
-            const float x = a.rx[3 * r], y = a.rx[3 * r + 1], z = a.rx[3 * r + 2];
-            float pre = fmaf(W1[3 * j], x, b1[j]);
-            pre = fmaf(W1[3 * j + 1], y, pre);
-            pre = fmaf(W1[3 * j + 2], z, pre);
-            const double draw = (double)a.scratch[4 * r];
-            g2 += draw * (double)fmaxf(pre, 0.0f);
-            const double dh = pre > 0.0f ? draw * (double)w2[j] : 0.0;
-            gw[0] += dh * x;
-            gw[1] += dh * y;
-            gw[2] += dh * z;
-            gb += dh;
-        }
-        a.dmlp[3 * j] = (float)gw[0];
-        a.dmlp[3 * j + 1] = (float)gw[1];
-        a.dmlp[3 * j + 2] = (float)gw[2];
-        a.dmlp[3 * Hn + j] = (float)gb;
-        a.dmlp[4 * Hn + j] = (float)g2;
-    } else if (tid == Hn) {
-        double gb2 = 0.0;
-        for (int r = 0; r < a.planes; ++r) gb2 += (double)a.scratch[4 * r];
-        a.dmlp[5 * Hn] = (float)gb2;
+    // weight gradients: hidden unit j = tid % Hn over the planes of slice
+    // tid / Hn (S = 256 / Hn slices, planes i = slice, slice + S, ... of each
+    // LDS chunk), the slices then combined in slice order
+    const int S = 256 / Hn;
+    const int j = tid % Hn, sl = tid / Hn;
+    const bool act = sl < S;
+    float wx = 0.0f, wy = 0.0f, wz = 0.0f, bj = 0.0f, vj = 0.0f;
+    if (act) {
+        wx = W1[3 * j]; wy = W1[3 * j + 1]; wz = W1[3 * j + 2];
+        bj = b1[j]; vj = w2[j];
     }
+    double gw0 = 0.0, gw1 = 0.0, gw2 = 0.0, gb = 0.0, g2 = 0.0, gb2 = 0.0;
+    for (int c0 = 0; c0 < a.planes; c0 += CONF_CHUNK) {
+        const int nc = min(CONF_CHUNK, a.planes - c0);
+        __syncthreads();   // the previous chunk is consumed
+        for (int i = tid; i < nc; i += 256) {
+            s_draw[i] = a.scratch[4 * (c0 + i)];
+            s_rx[3 * i] = a.rx[3 * (c0 + i)];
+            s_rx[3 * i + 1] = a.rx[3 * (c0 + i) + 1];
+            s_rx[3 * i + 2] = a.rx[3 * (c0 + i) + 2];
+        }
+        __syncthreads();
+        if (act) {
+            for (int i = sl; i < nc; i += S) {
+                const float x = s_rx[3 * i], y = s_rx[3 * i + 1], z = s_rx[3 * i + 2];
+                float pre = fmaf(wx, x, bj);
+                pre = fmaf(wy, y, pre);
+                pre = fmaf(wz, z, pre);
+                const double draw = (double)s_draw[i];
+                g2 += draw * (double)fmaxf(pre, 0.0f);
+                const double dh = pre > 0.0f ? draw * (double)vj : 0.0;
+                gw0 += dh * x;
+                gw1 += dh * y;
+                gw2 += dh * z;
+                gb += dh;
+            }
+        }
+        if (tid == 0)
+            for (int i = 0; i < nc; ++i) gb2 += (double)s_draw[i];   // planes in order
+    }
+    if (act) {
+        s_g[5 * tid] = gw0;
+        s_g[5 * tid + 1] = gw1;
+        s_g[5 * tid + 2] = gw2;
+        s_g[5 * tid + 3] = gb;
+        s_g[5 * tid + 4] = g2;
+    }
+    __syncthreads();
+    if (tid < Hn) {
+        double t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int q = 0; q < S; ++q)
+#pragma unroll
+            for (int k = 0; k < 5; ++k) t[k] += s_g[5 * (q * Hn + tid) + k];
+        a.dmlp[3 * tid] = (float)t[0];
+        a.dmlp[3 * tid + 1] = (float)t[1];
+        a.dmlp[3 * tid + 2] = (float)t[2];
+        a.dmlp[3 * Hn + tid] = (float)t[3];
+        a.dmlp[4 * Hn + tid] = (float)t[4];
+    }
+    if (tid == 0) a.dmlp[5 * Hn] = (float)gb2;
 }
 
 hipError_t launch_conf_loss(const ConfLossArgs& a, hipStream_t s)

@@ -157,7 +157,7 @@ def _conf_inputs(R, H, W, seed):
     return img, tgt, rx, net
 
 
-@pytest.mark.parametrize("R,H,W", [(1, 90, 360), (5, 37, 50)])
+@pytest.mark.parametrize("R,H,W", [(1, 90, 360), (5, 37, 50), (300, 12, 20)])
 def test_confidence_loss_matches_torch_and_fp64(R, H, W):
     """mean_r (C_r L_r - alpha log C_r) (train.py:418-436) against (a) the
     same loss written with torch ops (L1, fused_ssim per spectrum and the
