@@ -122,16 +122,46 @@ __device__ __forceinline__ void load_w2t(const float* w2, float* w2t)
     }
 }
 
-// component c (0..63) of PE(mu); 63 is the zero pad
-__device__ __forceinline__ float pe_component(const float* mu, int c)
+// sin(pi x), cos(pi x) of x = 2^k mu_j reduced exactly to [-1, 1]: the
+// hardware v_sin_f32 / v_cos_f32 of x / 2 turns (exact halving; at most
+// 1.3e-7 absolute error over the half-turn domain, the routine the
+// multi-frequency kernels use and the test suite checks on every float).
+__device__ __forceinline__ void pe_sincos(float mu, int k, float* sn, float* cs)
 {
-    if (c < 3) return mu[c];
-    if (c >= M1_PE) return 0.0f;
-    const int k = (c - 3) / 6, j = (c - 3) % 6;
-    float x = mu[j % 3] * (float)(1 << k);
+    float x = mu * (float)(1 << k);
     x = x - 2.0f * rintf(0.5f * x);
-    const float a = 3.14159265358979323846f * x;
-    return j < 3 ? sinf(a) : cosf(a);
+    *sn = __builtin_amdgcn_sinf(0.5f * x);
+    *cs = __builtin_amdgcn_cosf(0.5f * x);
+}
+
+// PE rows of the 128 Gaussians g0 .. g0+127 into pes[m * M1_KS + c] (c < 64,
+// 63 the zero pad; zero rows past P): item (m, b) of 128 x 11 writes block b
+// of row m, b = 0 the mean itself, b = 1 + k the sines and cosines of
+// frequency k, its three coordinates together.
+__device__ __forceinline__ void pe_rows(const Mlp1Args& a, int g0, float* pes)
+{
+    for (int i = threadIdx.x; i < 128 * 11; i += blockDim.x) {
+        const int m = i / 11, b = i % 11, g = g0 + m;
+        float* row = pes + m * M1_KS;
+        const bool in = g < a.P;
+        const float mu[3] = {in ? a.means[3 * (size_t)g] : 0.0f, in ? a.means[3 * (size_t)g + 1] : 0.0f,
+                             in ? a.means[3 * (size_t)g + 2] : 0.0f};
+        if (b == 0) {
+            row[0] = mu[0];
+            row[1] = mu[1];
+            row[2] = mu[2];
+            row[M1_PE] = 0.0f;
+        } else {
+            const int k = b - 1;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                float sn, cs;
+                pe_sincos(mu[j], k, &sn, &cs);
+                row[3 + 6 * k + j] = in ? sn : 0.0f;
+                row[6 + 6 * k + j] = in ? cs : 0.0f;
+            }
+        }
+    }
 }
 
 // Receiver features of row (g, r).
@@ -192,13 +222,7 @@ __global__ void __launch_bounds__(256) k_mlp1_pe(Mlp1Args a)
     for (int tile = blockIdx.x; tile * 128 < a.P; tile += gridDim.x) {
         const int g0 = tile * 128;
         __syncthreads();
-        for (int i = threadIdx.x; i < 128 * 64; i += 256) {
-            const int m = i >> 6, c = i & 63, g = g0 + m;
-            float mu[3] = {0.0f, 0.0f, 0.0f};
-            if (g < a.P)
-                for (int j = 0; j < 3; ++j) mu[j] = a.means[3 * (size_t)g + j];
-            pes[m * M1_KS + c] = g < a.P ? pe_component(mu, c) : 0.0f;
-        }
+        pe_rows(a, g0, pes);
         __syncthreads();
         floatx16 acc[4];
 #pragma unroll
@@ -679,13 +703,7 @@ __global__ void __launch_bounds__(256) k_mlp1_pe_bwd(Mlp1Args a, const float* dp
             const int m = i >> 7, u = i & (M1_H - 1), g = g0 + m;
             dps[m * M1_S + u] = g < a.P ? dpe1[(size_t)g * M1_H + u] : 0.0f;
         }
-        for (int i = threadIdx.x; i < 128 * 64; i += 256) {
-            const int m = i >> 6, c = i & 63, g = g0 + m;
-            float mu[3] = {0.0f, 0.0f, 0.0f};
-            if (g < a.P)
-                for (int j = 0; j < 3; ++j) mu[j] = a.means[3 * (size_t)g + j];
-            pes[m * M1_KS + c] = g < a.P ? pe_component(mu, c) : 0.0f;
-        }
+        pe_rows(a, g0, pes);
         __syncthreads();
         if (threadIdx.x < M1_H)
             for (int m = 0; m < 128; ++m) db1 += dps[m * M1_S + threadIdx.x];
@@ -724,11 +742,10 @@ __global__ void __launch_bounds__(256) k_mlp1_pe_bwd(Mlp1Args a, const float* dp
             const float mu = a.means[3 * (size_t)g + j];
             float s = dpr[j];
             for (int k = 0; k < 10; ++k) {
-                float x = mu * (float)(1 << k);
-                x = x - 2.0f * rintf(0.5f * x);
-                const float ang = 3.14159265358979323846f * x;
+                float sn, cs;
+                pe_sincos(mu, k, &sn, &cs);
                 const float sc = 3.14159265358979323846f * (float)(1 << k);
-                s += sc * (dpr[3 + 6 * k + j] * cosf(ang) - dpr[6 + 6 * k + j] * sinf(ang));
+                s += sc * (dpr[3 + 6 * k + j] * cs - dpr[6 + 6 * k + j] * sn);
             }
             dmu[3 * (size_t)g + j] += s;
         }
