@@ -235,6 +235,9 @@ def test_argument_checks_before_any_device_work():
     assert b"W + H too large" in L.gsparc_last_error()
     assert L.gsparc_instance_count(10, 1, 360, 90, None, C.byref(K), None) != 0
     assert b"null pointer" in L.gsparc_last_error()
+    # the depth sort's look-back words count a receiver's Gaussians in 28 bits
+    assert L.gsparc_instance_count(1 << 28, 1, 360, 90, img, C.byref(K), None) != 0
+    assert b"below 2^28" in L.gsparc_last_error()
     # the device keeps K saturated at 2^32 - 1: a capacity must stay below it
     dummy = C.c_void_p(256)
     assert L.gsparc_rasterize_forward_render(10, 1, 360, 90, None, dummy, dummy, dummy, (1 << 32) - 1, dummy, dummy,
