@@ -147,6 +147,7 @@ struct BinArgs {
     uint32_t* skey[2];       // [Nrec] depth-sort ping-pong: depth bits
     uint32_t* sval[2];       // [Nrec]                       Gaussian index
     uint32_t* shist;         // [os_words] depth-sort state: look-back status words, digit counts, tickets
+                             //   (or [R][257][rs_blocks(P)] per-tile digit histograms past GS_OS_MAX_TILES)
     uint32_t* lbstat;        // (set by the sort) [R][rs_blocks(P)][256] status words of a pass
     uint32_t* ghist;         // (set by the sort) [R][GS_OS_GHIST] digit counts of the 4 passes, visible count
     uint32_t* tick;          // (set by the sort) [4][R] tile tickets
@@ -160,6 +161,7 @@ struct BinArgs {
 };
 int rs_blocks(int P);
 #define GS_OS_GHIST 1025
+#define GS_OS_MAX_TILES 32   // one-sweep depth sort up to this many 2048-key tiles per receiver
 size_t os_words(const Dims& d);   // 32-bit words of BinArgs::shist
 int chunk_len_for(const Dims& d);
 int chunk_count_for(const Dims& d);

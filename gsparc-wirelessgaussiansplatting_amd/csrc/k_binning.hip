@@ -6,12 +6,14 @@
 // instances in index order).
 //
 // Sort first, then bin, so that no per-tile sort is needed:
-//   k_os_hist / k_os_pass
+//   k_os_hist / k_os_pass  (or k_rs_hist / k_rs_scan / k_rs_scatter)
 //               stable LSD radix sort of each receiver's visible Gaussians by
 //               the 32 depth bits (4 passes of 8-bit digits; input in index
 //               order, so equal depths stay in index order): one histogram
 //               of all four digits up front, then one launch per pass whose
-//               tiles find their offsets by decoupled look-back.  The first
+//               tiles find their offsets by decoupled look-back; for
+//               receivers of more than GS_OS_MAX_TILES tiles of 2048, a
+//               histogram, a scan and a scatter launch per pass.  The first
 //               pass reads depth and radii directly and drops culled
 //               Gaussians.
 //   k_chunk_count  the sorted sequence of a receiver is cut into chunks; per
@@ -205,6 +207,186 @@ __device__ __forceinline__ void rs_load(const Dims& d, const BinArgs& a, size_t 
         dg = (key >> (8 * pass)) & 0xffu;
     }
 }
+
+// ---- receivers with more than GS_OS_MAX_TILES tiles: three launches per pass
+// (per-tile digit histograms, their scan, the scatter), no waits between
+// workgroups.  Look-back chains grow with the tile count, and past about 32
+// tiles per receiver these measure faster.
+
+template <bool FIRST>
+__global__ void __launch_bounds__(GS_RS_THREADS) k_rs_hist(Dims d, BinArgs a, int pass)
+{
+    constexpr int NB = FIRST ? 257 : 256;
+    __shared__ uint32_t h[NB];
+    const int r = blockIdx.y, blk = blockIdx.x, nblk = gridDim.x;
+    const RxMap rm = rx_map(d, r);
+    const int n = FIRST ? (int)rm.P : (int)a.nvis[r];
+    const size_t rP = rm.roff;
+    for (int i = threadIdx.x; i < NB; i += GS_RS_THREADS) h[i] = 0;
+    // the wait-limit flag of the look-back passes, which this path never sets
+    if (FIRST && blk == 0 && r == 0 && threadIdx.x == 0) a.total[2] = 0;
+    __syncthreads();
+    if (blk * GS_RS_TILE < n) {
+        // all of the thread's loads first and unconditional (positions past
+        // the end read the last element and are masked below), so that their
+        // latencies overlap
+        uint32_t dg[GS_RS_ITEMS];
+#pragma unroll
+        for (int k = 0; k < GS_RS_ITEMS; ++k) {
+            const int i = min(n - 1, blk * GS_RS_TILE + k * GS_RS_THREADS + (int)threadIdx.x);
+            uint32_t key, val;
+            rs_load<FIRST>(d, a, rP, i, pass, key, val, dg[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < GS_RS_ITEMS; ++k) {
+            const bool valid = blk * GS_RS_TILE + k * GS_RS_THREADS + (int)threadIdx.x < n;
+            if (valid) atomicAdd(&h[dg[k]], 1u);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < NB; i += GS_RS_THREADS) a.shist[((size_t)r * 257 + i) * nblk + blk] = h[i];
+}
+
+// Per receiver: exclusive scan over (digit, block) of the block histograms.
+__global__ void __launch_bounds__(1024) k_rs_scan(Dims d, BinArgs a, int nblk, int first)
+{
+    __shared__ uint32_t s_wave[1024 / GS_WAVE];
+    const int r = blockIdx.x;
+    const int n = (first ? 257 : 256) * nblk;
+    uint32_t* h = a.shist + (size_t)r * 257 * nblk;
+    uint32_t carry = 0;
+    for (int base = 0; base < n; base += 1024 * 4) {
+        uint32_t v[4], s = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int idx = base + threadIdx.x * 4 + j;
+            v[j] = idx < n ? h[idx] : 0u;
+            s += v[j];
+        }
+        uint32_t tot;
+        uint32_t run = carry + block_exclusive_scan<1024>(s, s_wave, tot);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int idx = base + threadIdx.x * 4 + j;
+            if (idx < n) h[idx] = run;
+            run += v[j];
+        }
+        carry += tot;
+    }
+    __syncthreads();
+    // visible Gaussians = everything before the culled digit
+    if (first && threadIdx.x == 0) a.nvis[r] = h[(size_t)256 * nblk];
+}
+
+// Stable scatter of one pass.  Wave w of the block owns the block's
+// elements w*WT .. w*WT+WT-1 (WT = GS_RS_TILE / waves), in GS_RS_ITEMS rounds
+// of 64: round ranks come from the digit-match masks, a wave-private running
+// count per digit carries them across rounds, and the per-digit prefix over
+// the waves joins the waves in input order.  The block then sorts its
+// elements into LDS by (digit, input position) and writes each digit's run to
+// consecutive global positions, so stores are contiguous per digit rather than
+// one element per bucket.  The first pass drops culled Gaussians (digit 256):
+// every later pass and the chunk kernels read only the nvis visible ones.
+template <bool FIRST>
+__global__ void __launch_bounds__(GS_RS_THREADS) k_rs_scatter(Dims d, BinArgs a, int pass)
+{
+    constexpr int NB = FIRST ? 257 : 256;
+    constexpr int NW = GS_RS_THREADS / GS_WAVE;
+    constexpr int WT = GS_RS_TILE / NW;
+    static_assert(WT == GS_RS_ITEMS * GS_WAVE, "a wave owns GS_RS_ITEMS rounds of 64");
+    __shared__ uint32_t s_key[GS_RS_TILE];
+    __shared__ uint32_t s_val[GS_RS_TILE];
+    __shared__ uint16_t s_dg[GS_RS_TILE];
+    __shared__ uint32_t whist[NW][NB];
+    __shared__ uint32_t lbase[NB];
+    __shared__ uint32_t gbase[NB];
+    __shared__ uint32_t s_wave[GS_RS_THREADS / GS_WAVE];
+    const int r = blockIdx.y, blk = blockIdx.x, nblk = gridDim.x;
+    const RxMap rm = rx_map(d, r);
+    const int n = FIRST ? (int)rm.P : (int)a.nvis[r];
+    if (blk * GS_RS_TILE >= n) return;
+    const size_t rP = rm.roff;
+    const int tid = threadIdx.x, wave = tid / GS_WAVE, lane = tid % GS_WAVE;
+    for (int i = tid; i < NB; i += GS_RS_THREADS) {
+        gbase[i] = a.shist[((size_t)r * 257 + i) * nblk + blk];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) whist[w][i] = 0;
+    }
+    __syncthreads();
+    uint32_t key[GS_RS_ITEMS], val[GS_RS_ITEMS], dg[GS_RS_ITEMS], rnk[GS_RS_ITEMS];
+    const int e0 = wave * WT;
+    // all of the thread's loads first and unconditional (positions past the
+    // end read the last element and are masked below), so that their
+    // latencies overlap
+#pragma unroll
+    for (int k = 0; k < GS_RS_ITEMS; ++k) {
+        const int i = min(n - 1, blk * GS_RS_TILE + e0 + k * GS_WAVE + lane);
+        rs_load<FIRST>(d, a, rP, i, pass, key[k], val[k], dg[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < GS_RS_ITEMS; ++k) {
+        const bool valid = blk * GS_RS_TILE + e0 + k * GS_WAVE + lane < n;
+        const uint64_t m = match_digit<FIRST ? 9 : 8>(dg[k], valid);
+        const uint32_t prev = valid ? whist[wave][dg[k]] : 0u;
+        rnk[k] = prev + (uint32_t)__popcll(m & lanemask_lt());
+        if (valid && (m & lanemask_lt()) == 0) whist[wave][dg[k]] = prev + (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    // per digit: exclusive prefix over the waves, then over the digits
+    uint32_t tot = 0;
+    {
+        uint32_t run = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            const uint32_t c = whist[w][tid];
+            whist[w][tid] = run;
+            run += c;
+        }
+        tot = run;
+    }
+    uint32_t all;
+    const uint32_t lb = block_exclusive_scan<GS_RS_THREADS>(tot, s_wave, all);
+    lbase[tid] = lb;
+    if (FIRST && tid == 0) {
+        uint32_t run = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            const uint32_t c = whist[w][256];
+            whist[w][256] = run;
+            run += c;
+        }
+        lbase[256] = all;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < GS_RS_ITEMS; ++k) {
+        const int i = blk * GS_RS_TILE + e0 + k * GS_WAVE + lane;
+        if (i < n) {
+            const uint32_t lp = lbase[dg[k]] + whist[wave][dg[k]] + rnk[k];
+            s_key[lp] = key[k];
+            s_val[lp] = val[k];
+            s_dg[lp] = (uint16_t)dg[k];
+        }
+    }
+    __syncthreads();
+    uint32_t* kout = a.skey[pass & 1] + rP;
+    uint32_t* vout = a.sval[pass & 1] + rP;
+    // the last pass also puts the packed tile rects in depth order, so that
+    // the chunk kernels read them in sequence instead of gathering
+    const bool rects = !FIRST && pass == 3 && a.rect;
+    const int nb = min(GS_RS_TILE, n - blk * GS_RS_TILE);
+    for (int e = tid; e < nb; e += GS_RS_THREADS) {
+        const uint32_t g = s_dg[e];
+        if (FIRST && g == 256) continue;     // culled: never read again
+        const uint32_t pos = gbase[g] + (uint32_t)e - lbase[g];
+        const uint32_t v = s_val[e];
+        kout[pos] = s_key[e];
+        vout[pos] = v;
+        if (rects) a.srect[rP + pos] = a.rect[rP + v];
+    }
+}
+
+// ---- receivers with at most GS_OS_MAX_TILES tiles: one launch per pass
 
 // Per receiver, the counts of each of the four 8-bit digits over its visible
 // Gaussians (ghist[r][pass][digit]) and the number visible (ghist[r][1024]),
@@ -947,7 +1129,8 @@ size_t chunk_scatter_lds(const Dims& d) { return GS_SCATTER_WAVES * chunk_scatte
 
 size_t os_words(const Dims& d)
 {
-    return (size_t)d.R * ((size_t)rs_blocks(d.P) * 256 + GS_OS_GHIST + 4);
+    const size_t nblk = (size_t)rs_blocks(d.P);
+    return (size_t)d.R * std::max(nblk * 257, nblk * 256 + GS_OS_GHIST + 4);
 }
 
 BinArgs os_args(const Dims& d, const BinArgs& a0)
@@ -964,6 +1147,21 @@ hipError_t launch_depth_sort(const Dims& d, const BinArgs& a0, hipStream_t s)
     if (d.R == 0 || d.P == 0) return hipSuccess;
     const int nblk = rs_blocks(d.P);
     const dim3 grid(nblk, d.R);
+    if (nblk > GS_OS_MAX_TILES) {
+        // shist as [R][257][nblk] per-tile digit histograms
+        for (int pass = 0; pass < 4; ++pass) {
+            if (pass == 0) {
+                hipLaunchKernelGGL(k_rs_hist<true>, grid, dim3(GS_RS_THREADS), 0, s, d, a0, pass);
+                hipLaunchKernelGGL(k_rs_scan, dim3(d.R), dim3(1024), 0, s, d, a0, nblk, 1);
+                hipLaunchKernelGGL(k_rs_scatter<true>, grid, dim3(GS_RS_THREADS), 0, s, d, a0, pass);
+            } else {
+                hipLaunchKernelGGL(k_rs_hist<false>, grid, dim3(GS_RS_THREADS), 0, s, d, a0, pass);
+                hipLaunchKernelGGL(k_rs_scan, dim3(d.R), dim3(1024), 0, s, d, a0, nblk, 0);
+                hipLaunchKernelGGL(k_rs_scatter<false>, grid, dim3(GS_RS_THREADS), 0, s, d, a0, pass);
+            }
+        }
+        return hipGetLastError();
+    }
     const BinArgs a = os_args(d, a0);
     // status words, digit counts and tickets start at zero (a fill kernel:
     // the sort may be captured into a graph)
