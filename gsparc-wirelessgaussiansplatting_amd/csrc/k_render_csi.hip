@@ -234,7 +234,7 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd_csi(Dims d, RenderCsiArg
     }
 }
 
-// Forward on the matrix cores.  Per batch of CSI_BATCH staged entries the
+// Forward on the matrix cores.  Per batch of CSI_MFB staged entries the
 // kept ones (block test passed) are compacted into slots 0..nk-1 in list
 // order; all 64 lanes then evaluate the kept entries' emissions
 // (slot, subcarrier pair) into LDS, and each lane walks the slots for its
@@ -249,20 +249,25 @@ __global__ void __launch_bounds__(GS_WAVE) k_render_fwd_csi(Dims d, RenderCsiArg
 // MFMA 32x32x2 f32 operand layout (wave64): lane l supplies A[l%32][l/32] and
 // B[l/32][l%32]; accumulator element v of lane l is C[8(v/4) + 4(l/32) +
 // v%4][l%32].
+// entries staged per batch: 16 keeps the workgroup's LDS (emissions, blend
+// weights, slot tables) near 12 KB, so that the registers and not the LDS
+// bound the waves per CU (32 entries: 23.5 KB, 1.625 against 1.57 ms at
+// 50k x 32)
+#define CSI_MFB 16
 #define CSI_EW 108            // LDS row of an entry's 104 emissions (+ 4)
-#define CSI_WW (CSI_BATCH + 1)
+#define CSI_WW (CSI_MFB + 1)
 
 typedef float floatx16_csi __attribute__((ext_vector_type(16)));
 
 __global__ void __launch_bounds__(GS_WAVE) __attribute__((amdgpu_waves_per_eu(2)))
 k_render_fwd_csi_mfma(Dims d, RenderCsiArgs a)
 {
-    __shared__ float4 sk0[CSI_BATCH];            // kept entries' splat records, by slot
-    __shared__ float4 sk1[CSI_BATCH];
-    __shared__ float2 ssig[CSI_BATCH];
-    __shared__ float splen[CSI_BATCH];
-    __shared__ int sjj[CSI_BATCH];               // staged position of slot k
-    __shared__ __attribute__((aligned(16))) float sE[CSI_BATCH][CSI_EW];
+    __shared__ float4 sk0[CSI_MFB];            // kept entries' splat records, by slot
+    __shared__ float4 sk1[CSI_MFB];
+    __shared__ float2 ssig[CSI_MFB];
+    __shared__ float splen[CSI_MFB];
+    __shared__ int sjj[CSI_MFB];               // staged position of slot k
+    __shared__ __attribute__((aligned(16))) float sE[CSI_MFB][CSI_EW];
     __shared__ float sW[GS_WAVE][CSI_WW];
     const Pixels<1> pm = map_pixels<1>(d, a.order, (int)a.total[1]);
     if (!pm.valid) return;
@@ -289,14 +294,14 @@ k_render_fwd_csi_mfma(Dims d, RenderCsiArgs a)
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[mb][nb][v] = 0.0f;
 
-    for (uint32_t start = rb; start < re; start += CSI_BATCH) {
+    for (uint32_t start = rb; start < re; start += CSI_MFB) {
         if (ballot(T > 0.0f) == 0) break;
-        // stage and test CSI_BATCH entries; compact the kept ones into slots
+        // stage and test CSI_MFB entries; compact the kept ones into slots
         const uint32_t e = start + lane;
         bool keep = false;
         float4 A, B;
         size_t gi = 0;
-        if (lane < CSI_BATCH && e < re) {
+        if (lane < CSI_MFB && e < re) {
             gi = rP + a.plist[e];
             A = a.splat[2 * gi];
             B = a.splat[2 * gi + 1];
