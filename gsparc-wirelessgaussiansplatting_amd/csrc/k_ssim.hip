@@ -17,6 +17,7 @@
 // rounding (a few ulp).
 #include <cmath>
 
+#include "gs_blend.h"
 #include "gs_device.h"
 #include "gs_kernels.h"
 
@@ -176,23 +177,28 @@ __global__ void __launch_bounds__(SS_THREADS) k_ssim_fwd(SsimArgs a, Taps tp)
         }
     }
     if (a.part) {
-        // fixed-order tree over the workgroup: the loss is the same run to run
-        __shared__ float r[5][SS_THREADS];
+        // fixed-order sums over the workgroup (the loss is the same run to
+        // run): each wave's sum by lane exchanges in a fixed pattern, then the
+        // waves' sums in wave order
+        constexpr int NWV = SS_THREADS / GS_WAVE;
+        __shared__ float r[5][NWV];
         const int ns = a.nsum;
-        r[0][tid] = sum_map;
-        r[1][tid] = sum_abs;
-        r[2][tid] = sum_d2;
-        r[3][tid] = sum_p2;
-        r[4][tid] = sum_t2;
-        __syncthreads();
-        for (int w = SS_THREADS / 2; w > 0; w >>= 1) {
-            if (tid < w)
-                for (int c = 0; c < ns; ++c) r[c][tid] += r[c][tid + w];
-            __syncthreads();
+        const float vals[5] = {sum_map, sum_abs, sum_d2, sum_p2, sum_t2};
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            if (c >= ns) break;   // uniform
+            const float w = wave_sum(vals[c]);
+            if (tid % GS_WAVE == 0) r[c][tid / GS_WAVE] = w;
         }
+        __syncthreads();
         if (tid == 0) {
             const size_t b = ((size_t)tl.z * gridDim.y + tl.y) * gridDim.x + tl.x;
-            for (int c = 0; c < ns; ++c) a.part[(size_t)ns * b + c] = r[c][0];
+            for (int c = 0; c < ns; ++c) {
+                float t = r[c][0];
+#pragma unroll
+                for (int w = 1; w < NWV; ++w) t += r[c][w];
+                a.part[(size_t)ns * b + c] = t;
+            }
         }
     }
 }
