@@ -499,7 +499,8 @@ int backward_impl(int grad_mode, const gs::Dims& d, const FwdInputs& in, const f
     size_t dz_off, part_off, gl_off, bytes;
     grad_layout(d, dz_off, part_off, gl_off, bytes);
     float* gL = (float*)((char*)g8 + gl_off);
-    GS_LAUNCH("grad_clear", gs::launch_fill_u32((uint32_t*)g8, 0u, d.Nrec * 8, s), s);
+    // the accumulators start at zero (the deterministic gather writes them all)
+    if (!det) GS_LAUNCH("grad_clear", gs::launch_fill_u32((uint32_t*)g8, 0u, d.Nrec * 8, s), s);
     if (csi) {
         GS_LAUNCH("grad_clear", gs::launch_fill_u32((uint32_t*)gL, 0u, d.Nrec, s), s);
         gs::RenderCsiArgs ca;

@@ -508,9 +508,15 @@ __global__ void __launch_bounds__(256) k_det_gather(Dims d, DetArgs a)
 {
     const size_t rec = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (rec >= d.Nrec) return;
-    if ((uint64_t)a.total[0] > a.Kcap) return;   // no lists: the gradient stays cleared
-    const uint32_t n = det_tiles(d, a, rec);
-    if (n == 0) return;
+    // every record is written (zeros without tiles or lists), so the
+    // deterministic backward needs no cleared gradient buffer
+    float4* gp = reinterpret_cast<float4*>(a.g8 + rec * 8);
+    const uint32_t n = (uint64_t)a.total[0] > a.Kcap ? 0u : det_tiles(d, a, rec);
+    if (n == 0) {
+        gp[0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        gp[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
     const float4* ip = reinterpret_cast<const float4*>(a.inst8) + (size_t)a.off[rec] * 2;
     float4 s0 = ip[0], s1 = ip[1];
     for (uint32_t k = 1; k < n; ++k) {
@@ -518,7 +524,6 @@ __global__ void __launch_bounds__(256) k_det_gather(Dims d, DetArgs a)
         s0.x += u.x; s0.y += u.y; s0.z += u.z; s0.w += u.w;
         s1.x += v.x; s1.y += v.y; s1.z += v.z; s1.w += v.w;
     }
-    float4* gp = reinterpret_cast<float4*>(a.g8 + rec * 8);
     gp[0] = s0;
     gp[1] = s1;
 }
