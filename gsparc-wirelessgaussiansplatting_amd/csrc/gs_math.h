@@ -151,19 +151,46 @@ GS_HD float gs_expf_blend(float x)
 // suite) and rounding the product preserves order, so for every power <= 0
 //     power >= gs_alpha_thr(o)   <=>   o * gs_expf_blend(power) >= 1/255
 // and the forward and backward blends decide with this one comparison.
-// Bisection over the bit patterns of the magnitude (31 steps).  Returns +1
+// The search runs over the bit patterns of the power's magnitude.  Returns +1
 // (above every power <= 0) when even power = 0 stays below 1/255 (o < 1/255,
 // o <= 0), -86 when every power of the domain passes (o above ~1e35: the
 // far tail below -86 is then skipped), and NaN for a NaN opacity (every
 // comparison with it is false, so nothing is skipped, as alpha =
 // min(0.99, NaN) = 0.99 would not be).
+//
+// The boundary lies within a few floats of -log(255 o) (the log and the
+// exponential are each within about an ulp), so the search starts there and
+// steps one float at a time; a boundary further than GS_THR_STEPS floats from
+// the start (not expected) falls back to a bisection over all of [-86, 0]
+// (31 steps).  Either way the result is the one boundary the comparison
+// defines.
+#define GS_THR_STEPS 32
+GS_HD float gs_logf(float x);
 GS_HD float gs_alpha_thr(float o)
 {
     const float amin = 1.0f / 255.0f;
     if (o != o) return o;
     if (!(o * gs_expf_blend(0.0f) >= amin)) return 1.0f;
     if (o * gs_expf_blend(-86.0f) >= amin) return -86.0f;
-    uint32_t lo = 0u, hi = gs_as_uint(86.0f);   // magnitude bits: lo passes, hi does not
+    const uint32_t top = gs_as_uint(86.0f);   // magnitude bits: 0 passes, top does not
+    // local search from the estimate (magnitude bits u pass when
+    // o * exp(-u) >= 1/255; they pass exactly up to the boundary)
+    const float est = gs_logf(255.0f * o);   // -est estimates the boundary power
+    uint32_t u0 = est > 0.0f ? gs_as_uint(est) : 0u;
+    u0 = u0 < top ? u0 : top - 1u;
+    if (o * gs_expf_blend(-gs_as_float(u0)) >= amin) {
+        for (int k = 0; k < GS_THR_STEPS; ++k) {   // u0 passes: the last passing above it
+            if (!(o * gs_expf_blend(-gs_as_float(u0 + 1u)) >= amin)) return -gs_as_float(u0);
+            ++u0;
+            if (u0 + 1u >= top) break;
+        }
+    } else {
+        for (int k = 0; k < GS_THR_STEPS && u0 > 0u; ++k) {   // u0 fails: the first passing below it
+            --u0;
+            if (o * gs_expf_blend(-gs_as_float(u0)) >= amin) return -gs_as_float(u0);
+        }
+    }
+    uint32_t lo = 0u, hi = top;   // bisection: lo passes, hi does not
     while (hi - lo > 1u) {
         const uint32_t mid = lo + ((hi - lo) >> 1);
         if (o * gs_expf_blend(-gs_as_float(mid)) >= amin) lo = mid;

@@ -9,9 +9,9 @@
 // the group, one Gaussian per lane: the emission MLP from the LDS copy, the
 // projection, and the per-(Gaussian, receiver) records.  Receiver-independent
 // work (cov3D, the log of the transmitter distance) is recomputed by each wave
-// for its lane's Gaussian; the opacity's skip threshold (a 31-step bisection)
-// is found once per workgroup by wave 0 during the staging and shared
-// through LDS.
+// for its lane's Gaussian; the opacity's skip threshold (gs_alpha_thr: a
+// search of a few floats around -log(255 o)) is found once per workgroup by
+// wave 0 during the staging and shared through LDS.
 #include "gs_device.h"
 #include "gs_kernels.h"
 
