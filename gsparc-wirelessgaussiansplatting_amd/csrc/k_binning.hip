@@ -688,45 +688,54 @@ __device__ __forceinline__ void wave_sync()
 }
 
 // instances per (chunk, tile) -> ccnt[(r*T + t) * nchunks + c]
+template <int CPB>
 __global__ void __launch_bounds__(GS_RS_THREADS) k_chunk_count(Dims d, BinArgs a)
 {
-    // LDS path: a difference array per tile row, (gx + 1) wide: a rect adds
-    // +1 at (y, x0) and -1 at (y, x1) for each of its rows; each tile's count
-    // is then the prefix of its row up to its column
+    // LDS path: a difference array per tile row, (gx + 1) wide, per chunk: a
+    // rect adds +1 at (y, x0) and -1 at (y, x1) for each of its rows; each
+    // tile's count is then the prefix of its row up to its column.  The
+    // workgroup takes CPB consecutive chunks (fewer, fuller workgroups).
     extern __shared__ uint32_t s_cnt[];
-    const int c = blockIdx.x, r = blockIdx.y;
+    const int cb = blockIdx.x * CPB, r = blockIdx.y;
     const bool lds = chunk_lds(d);
     const int n = (int)a.nvis[r];
-    const int i0 = c * a.chunk_len, i1 = min(n, i0 + a.chunk_len);
     const size_t rP = rx_map(d, r).roff;
-    const int rw = d.gx + 1;
-    uint32_t* slot = a.ccnt + (size_t)r * d.T * a.nchunks + c;   // tile t at slot[t * nchunks]
+    const int rw = d.gx + 1, ra = rw * d.gy;   // one chunk's difference arrays
+    const int nc = min(CPB, a.nchunks - cb);   // chunks of this workgroup (>= 1)
+    uint32_t* slot = a.ccnt + (size_t)r * d.T * a.nchunks + cb;   // chunk cb + q, tile t at slot[t * nchunks + q]
     if (lds) {
-        for (int t = threadIdx.x; t < rw * d.gy; t += GS_RS_THREADS) s_cnt[t] = 0;
+        for (int t = threadIdx.x; t < nc * ra; t += GS_RS_THREADS) s_cnt[t] = 0;
     } else {
-        for (int t = threadIdx.x; t < d.T; t += GS_RS_THREADS) slot[(size_t)t * a.nchunks] = 0;
+        for (int t = threadIdx.x; t < nc * d.T; t += GS_RS_THREADS)
+            slot[(size_t)(t % d.T) * a.nchunks + t / d.T] = 0;
     }
     __syncthreads();
-    for (int i = i0 + threadIdx.x; i < i1; i += GS_RS_THREADS) {
-        int x0, y0, x1, y1;
-        chunk_rect(d, a, rP, i, a.rect ? 0u : a.sval[1][rP + i], x0, y0, x1, y1);
-        if (x0 >= x1) continue;
-        for (int ty = y0; ty < y1; ++ty) {
-            if (lds) {
-                atomicAdd(&s_cnt[ty * rw + x0], 1u);
-                atomicAdd(&s_cnt[ty * rw + x1], 0xffffffffu);   // -1 modulo 2^32
-            } else {
-                for (int tx = x0; tx < x1; ++tx) atomicAdd(&slot[(size_t)(ty * d.gx + tx) * a.nchunks], 1u);
+    for (int q = 0; q < nc; ++q) {
+        const int i0 = (cb + q) * a.chunk_len, i1 = min(n, i0 + a.chunk_len);
+        uint32_t* sq = s_cnt + q * ra;
+        for (int i = i0 + threadIdx.x; i < i1; i += GS_RS_THREADS) {
+            int x0, y0, x1, y1;
+            chunk_rect(d, a, rP, i, a.rect ? 0u : a.sval[1][rP + i], x0, y0, x1, y1);
+            if (x0 >= x1) continue;
+            for (int ty = y0; ty < y1; ++ty) {
+                if (lds) {
+                    atomicAdd(&sq[ty * rw + x0], 1u);
+                    atomicAdd(&sq[ty * rw + x1], 0xffffffffu);   // -1 modulo 2^32
+                } else {
+                    for (int tx = x0; tx < x1; ++tx) atomicAdd(&slot[(size_t)(ty * d.gx + tx) * a.nchunks + q], 1u);
+                }
             }
         }
     }
     if (lds) {
         __syncthreads();
-        for (int t = threadIdx.x; t < d.T; t += GS_RS_THREADS) {
-            const int ty = t / d.gx, tx = t % d.gx;
+        for (int t = threadIdx.x; t < nc * d.T; t += GS_RS_THREADS) {
+            const int q = t / d.T, tt = t % d.T;
+            const int ty = tt / d.gx, tx = tt % d.gx;
+            const uint32_t* sq = s_cnt + q * ra + ty * rw;
             uint32_t v = 0;
-            for (int x = 0; x <= tx; ++x) v += s_cnt[ty * rw + x];
-            slot[(size_t)t * a.nchunks] = v;
+            for (int x = 0; x <= tx; ++x) v += sq[x];
+            slot[(size_t)tt * a.nchunks + q] = v;
         }
     }
 }
@@ -1174,11 +1183,17 @@ hipError_t launch_depth_sort(const Dims& d, const BinArgs& a0, hipStream_t s)
     return hipGetLastError();
 }
 
+#define GS_CC_CPB 4
 hipError_t launch_chunk_count(const Dims& d, const BinArgs& a, hipStream_t s)
 {
     if (d.R == 0) return hipSuccess;
-    const size_t lds = chunk_lds(d) ? (size_t)(d.gx + 1) * d.gy * 4 : 0;
-    hipLaunchKernelGGL(k_chunk_count, dim3(a.nchunks, d.R), dim3(GS_RS_THREADS), lds, s, d, a);
+    const size_t one = chunk_lds(d) ? (size_t)(d.gx + 1) * d.gy * 4 : 0;
+    // four chunks per workgroup where their difference arrays fit in 32 KB
+    if (one * GS_CC_CPB <= 32768)
+        hipLaunchKernelGGL(k_chunk_count<GS_CC_CPB>, dim3((a.nchunks + GS_CC_CPB - 1) / GS_CC_CPB, d.R),
+                           dim3(GS_RS_THREADS), one * GS_CC_CPB, s, d, a);
+    else
+        hipLaunchKernelGGL(k_chunk_count<1>, dim3(a.nchunks, d.R), dim3(GS_RS_THREADS), one, s, d, a);
     return hipGetLastError();
 }
 
