@@ -701,7 +701,11 @@ __global__ void __launch_bounds__(GS_RS_THREADS) k_chunk_count(Dims d, BinArgs a
     const int n = (int)a.nvis[r];
     const size_t rP = rx_map(d, r).roff;
     const int rw = d.gx + 1, ra = rw * d.gy;   // one chunk's difference arrays
-    const int nc = min(CPB, a.nchunks - cb);   // chunks of this workgroup (>= 1)
+    // the receiver's own chunks only: the counts of chunks past its visible
+    // Gaussians are never read (the scan and the placement stop there too)
+    const int nch = (n + a.chunk_len - 1) / a.chunk_len;
+    if (cb >= nch) return;   // uniform over the workgroup
+    const int nc = min(CPB, nch - cb);         // chunks of this workgroup (>= 1)
     uint32_t* slot = a.ccnt + (size_t)r * d.T * a.nchunks + cb;   // chunk cb + q, tile t at slot[t * nchunks + q]
     if (lds) {
         for (int t = threadIdx.x; t < nc * ra; t += GS_RS_THREADS) s_cnt[t] = 0;
@@ -748,19 +752,21 @@ __global__ void __launch_bounds__(256) k_chunk_scan(Dims d, BinArgs a)
     const int lane = threadIdx.x % GS_WAVE;
     if (row >= d.R * d.T) return;
     uint32_t* cnt = a.ccnt + (size_t)row * a.nchunks;
+    // the receiver's chunks (k_chunk_count writes no others)
+    const int nch = ((int)a.nvis[row / d.T] + a.chunk_len - 1) / a.chunk_len;
     uint32_t carry = 0;
     // four 64-chunk steps loaded at a time, so that their latencies overlap
-    for (int cb = 0; cb < a.nchunks; cb += 4 * GS_WAVE) {
+    for (int cb = 0; cb < nch; cb += 4 * GS_WAVE) {
         uint32_t v[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = cnt[min(cb + u * GS_WAVE + lane, a.nchunks - 1)];
+        for (int u = 0; u < 4; ++u) v[u] = cnt[min(cb + u * GS_WAVE + lane, nch - 1)];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int c = cb + u * GS_WAVE + lane;
-            if (cb + u * GS_WAVE >= a.nchunks) break;
-            const uint32_t vc = c < a.nchunks ? v[u] : 0u;
+            if (cb + u * GS_WAVE >= nch) break;
+            const uint32_t vc = c < nch ? v[u] : 0u;
             const uint32_t x = wave_incl_sum(vc);
-            if (c < a.nchunks) cnt[c] = carry + x - vc;
+            if (c < nch) cnt[c] = carry + x - vc;
             carry += __builtin_amdgcn_readlane(x, GS_WAVE - 1);
         }
     }
